@@ -1,0 +1,284 @@
+"""Checker for the GPU library: a CPU restatement of HFM's hot path, plus a runner for the
+reference's own FileHFM executables where they have been built from source.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in ``hamiltonfastmarching_amd`` imports this package.  Only
+``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` use it, and
+only as the checker or the timed CPU baseline -- never as the computation being measured.
+
+* ``libhfm_oracle.so`` (``oracle/hfm_oracle.cpp``): FP64 restatement of the Selling reduction,
+  the curvature stencils, the heap Fast-Marching loop, the geodesic flow and the backward
+  differentiation.  Built by ``make -C oracle``.
+* ``oracle/_ref/FileHFM_<Model>``: the reference front-end ``Interfaces/FileHFM/FileHFM.cpp``
+  compiled from its sources by ``make -C oracle ref`` (only where ``/root/reference`` exists).
+  It is driven through the reference's file format (``JMM_CPPLibs/Output/FileIO.hxx:7-132``:
+  ``<prefix>_Format.txt`` + ``<prefix>_Data.dat`` of float64), written by ``write_rules`` here.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+import tempfile
+from functools import reduce
+from operator import mul
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libhfm_oracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+MODELS = {"Riemann2": 0, "Riemann3": 1, "ReedsSheppForward2": 2, "Elastica2": 3}
+MODEL_DIM = {"Riemann2": 2, "Riemann3": 3, "ReedsSheppForward2": 3, "Elastica2": 3}
+
+_lib = None
+
+
+def build(ref_sources: str | None = "/root/reference") -> None:
+    """Compile the restatement, and the reference binaries when their sources are present."""
+    subprocess.check_call(["make", "-s", "-C", HERE])
+    if ref_sources and os.path.isdir(ref_sources):
+        subprocess.check_call(["make", "-s", "-C", HERE, "ref", f"REF={ref_sources}"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library not built: {LIB_PATH} (run make -C oracle)")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        d, i32, i64 = ctypes.c_double, ctypes.c_int, ctypes.c_int64
+        L.oracle_last_error.restype = ctypes.c_char_p
+        L.oracle_riemann_stencils.argtypes = [i32, P(i32), P(d), P(d), i32, i32, P(ctypes.c_int8), P(d)]
+        L.oracle_curvature_stencils.argtypes = [i32, i32, d, d, d, P(ctypes.c_int8), P(d), P(i32)]
+        L.oracle_solve.argtypes = [i32, P(i32), P(d), P(d), P(d), i32, i32, d, d, P(d), i32, i32, P(d), P(d),
+                                   P(d), P(ctypes.c_uint64), P(d), P(d)]
+        L.oracle_solve_backward.argtypes = [i32, P(i32), P(d), P(d), P(d), i32, i32, d, d, P(d), i32, i32,
+                                            P(d), P(d), i32, P(d), P(d), P(d), P(d), P(i64), P(d), i32, P(i32)]
+        _lib = L
+    return _lib
+
+
+def _ptr(a, ct):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+def _err():
+    return lib().oracle_last_error().decode()
+
+
+def riemann_stencils(dims, h, metric, dual=False):
+    """Selling stencils: (offsets int8 [N,K,d], weights float64 [N,K])."""
+    dims = [int(n) for n in dims]
+    dim = len(dims)
+    K = 3 if dim == 2 else 6
+    h = np.broadcast_to(np.asarray(h, np.float64), (dim,)).copy()
+    metric = np.ascontiguousarray(metric, np.float64)
+    N = reduce(mul, dims)
+    is_field = metric.size != K
+    off = np.zeros((N, K, dim), np.int8)
+    w = np.zeros((N, K))
+    dimsa = np.array(dims, np.int32)
+    r = lib().oracle_riemann_stencils(dim, _ptr(dimsa, ctypes.c_int), _ptr(h, ctypes.c_double),
+                                      _ptr(metric, ctypes.c_double), int(is_field), int(dual),
+                                      _ptr(off, ctypes.c_int8), _ptr(w, ctypes.c_double))
+    if r:
+        raise RuntimeError(_err())
+    return off, w
+
+
+def curvature_stencils(model, ntheta, h, xi, eps=0.1):
+    """Shared per-theta stencils: (offsets [ntheta,n,3], weights [ntheta,n], symmetric [n])."""
+    n = 4 if model == "ReedsSheppForward2" else 27
+    off = np.zeros((ntheta, n, 3), np.int8)
+    w = np.zeros((ntheta, n))
+    sym = np.zeros(n, np.int32)
+    r = lib().oracle_curvature_stencils(MODELS[model], int(ntheta), float(h), float(xi), float(eps),
+                                        _ptr(off, ctypes.c_int8), _ptr(w, ctypes.c_double),
+                                        _ptr(sym, ctypes.c_int))
+    if r:
+        raise RuntimeError(_err())
+    return off, w, sym.astype(bool)
+
+
+def _common_args(params):
+    model = params["model"]
+    dims = [int(n) for n in np.asarray(params["dims"]).ravel()]
+    dim = MODEL_DIM[model]
+    if len(dims) != dim:
+        raise ValueError("dims has the wrong length")
+    N = reduce(mul, dims)
+    if "gridScales" in params:
+        h = np.asarray(params["gridScales"], np.float64).ravel().copy()
+    else:
+        h = np.full(dim, float(params["gridScale"]))
+    origin = params.get("origin")
+    origin = None if origin is None else np.asarray(origin, np.float64).ravel().copy()
+    metric = None
+    dual = 0
+    metric_is_field = 0
+    cost = None
+    cost_is_field = 0
+    if model in ("Riemann2", "Riemann3"):
+        if "dualMetric" in params:
+            metric, dual = params["dualMetric"], 1
+        else:
+            metric = params["metric"]
+        metric = np.ascontiguousarray(metric, np.float64)
+        metric_is_field = int(metric.ndim > 1)
+    else:
+        if "speed" in params:
+            cost = 1.0 / np.ascontiguousarray(params["speed"], np.float64)
+        else:
+            cost = np.ascontiguousarray(params["cost"], np.float64)
+        cost = np.ascontiguousarray(np.broadcast_to(cost.reshape(cost.shape + (1,) * (3 - cost.ndim))
+                                                    if 0 < cost.ndim < 3 else cost, tuple(dims)))
+        cost_is_field = 1
+    seeds = np.ascontiguousarray(np.asarray(params["seeds"], np.float64).reshape(-1, dim))
+    sv = params.get("seedValues")
+    sv = None if sv is None else np.ascontiguousarray(sv, np.float64).ravel()
+    xi = float(params.get("xi", 1.0))
+    eps = float(params.get("eps", 0.1))
+    return model, dims, dim, N, h, origin, metric, metric_is_field, dual, cost, cost_is_field, seeds, sv, xi, eps
+
+
+def solve(params, flow=False, active=False):
+    """FP64 Fast Marching on an HFM-style input dict.  Returns a dict with 'values'
+    (and 'activeNeighs', 'geodesicFlow', 'StencilCPUTime', 'FMCPUTime')."""
+    (model, dims, dim, N, h, origin, metric, mif, dual, cost, cif, seeds, sv, xi, eps) = _common_args(params)
+    values = np.zeros(N)
+    act = np.zeros(N, np.uint64) if active else None
+    fl = np.zeros((N, dim)) if flow else None
+    tm = np.zeros(2)
+    dimsa = np.array(dims, np.int32)
+    r = lib().oracle_solve(MODELS[model], _ptr(dimsa, ctypes.c_int), _ptr(h, ctypes.c_double),
+                           _ptr(origin, ctypes.c_double), _ptr(metric, ctypes.c_double), mif, dual, xi, eps,
+                           _ptr(cost, ctypes.c_double), cif, len(seeds), _ptr(seeds, ctypes.c_double),
+                           _ptr(sv, ctypes.c_double), _ptr(values, ctypes.c_double),
+                           _ptr(act, ctypes.c_uint64), _ptr(fl, ctypes.c_double), _ptr(tm, ctypes.c_double))
+    if r:
+        raise RuntimeError(_err())
+    out = {"values": values.reshape(dims), "StencilCPUTime": tm[0], "FMCPUTime": tm[1]}
+    if active:
+        out["activeNeighs"] = act.reshape(dims)
+    if flow:
+        out["geodesicFlow"] = fl.reshape(tuple(dims) + (dim,))
+    return out
+
+
+def solve_backward(params, targets, weights=None):
+    """FM + BackwardVariation for one group of targets: (values, costSensitivity, seedSens)."""
+    (model, dims, dim, N, h, origin, metric, mif, dual, cost, cif, seeds, sv, xi, eps) = _common_args(params)
+    targets = np.ascontiguousarray(np.asarray(targets, np.float64).reshape(-1, dim))
+    tw = np.ones(len(targets)) if weights is None else np.ascontiguousarray(weights, np.float64)
+    values = np.zeros(N)
+    sens = np.zeros(N)
+    nmax = 1024
+    si = np.zeros(nmax, np.int64)
+    sw = np.zeros(nmax)
+    ns = ctypes.c_int(0)
+    dimsa = np.array(dims, np.int32)
+    r = lib().oracle_solve_backward(MODELS[model], _ptr(dimsa, ctypes.c_int), _ptr(h, ctypes.c_double),
+                                    _ptr(origin, ctypes.c_double), _ptr(metric, ctypes.c_double), mif, dual,
+                                    xi, eps, _ptr(cost, ctypes.c_double), cif, len(seeds),
+                                    _ptr(seeds, ctypes.c_double), _ptr(sv, ctypes.c_double), len(targets),
+                                    _ptr(targets, ctypes.c_double), _ptr(tw, ctypes.c_double),
+                                    _ptr(values, ctypes.c_double), _ptr(sens, ctypes.c_double),
+                                    _ptr(si, ctypes.c_int64), _ptr(sw, ctypes.c_double), nmax, ctypes.byref(ns))
+    if r:
+        raise RuntimeError(_err())
+    n = min(ns.value, nmax)
+    return values.reshape(dims), sens.reshape(dims), (si[:n].copy(), sw[:n].copy())
+
+
+# ------------------------------------------------------------------------------------------
+# Reference executables, driven through the reference's file format.
+# ------------------------------------------------------------------------------------------
+def ref_binary(model: str) -> str:
+    return os.path.join(REF_DIR, f"FileHFM_{model}")
+
+
+def ref_available(model: str) -> bool:
+    return os.path.exists(ref_binary(model))
+
+
+def write_rules(params: dict, prefix: str) -> None:
+    """Write an input dict in the FileHFM format (FileIO.hxx:7-95): per key, the key name, the
+    number of dims (-1 for strings, 0 for scalars), the dims, a blank line; data as float64."""
+    data = []
+    with open(prefix + "_Format.txt", "w") as f:
+        for key, val in params.items():
+            if isinstance(val, str):
+                f.write(f"{key}\n-1\n{val}\n\n")
+            elif np.isscalar(val):
+                f.write(f"{key}\n0\n\n")
+                data.append(np.array([float(val)]))
+            else:
+                a = np.asarray(val, np.float64)
+                f.write(f"{key}\n{a.ndim}\n")
+                for n in a.shape:
+                    f.write(f"{n}\n")
+                f.write("\n")
+                data.append(a.ravel())
+    np.concatenate(data).astype("<f8").tofile(prefix + "_Data.dat")
+
+
+def read_rules(prefix: str) -> dict:
+    """Read the FileHFM output pair (FileIO.hxx:97-132)."""
+    data = np.fromfile(prefix + "_Data.dat", dtype="<f8")
+    out, pos = {}, 0
+    with open(prefix + "_Format.txt") as f:
+        while True:
+            key = f.readline().strip()
+            if not key:
+                break
+            nd = int(f.readline())
+            if nd == -1:
+                out[key] = f.readline().rstrip("\n")
+            elif nd == 0:
+                out[key] = float(data[pos])
+                pos += 1
+            else:
+                dims = [int(f.readline()) for _ in range(nd)]
+                size = reduce(mul, dims, 1)
+                out[key] = data[pos:pos + size].reshape(dims).copy()
+                pos += size
+            f.readline()
+    return out
+
+
+def run_reference(params: dict, timeout: float = 600.0) -> dict:
+    """Run the reference FileHFM executable of params['model'] on params; returns its outputs."""
+    model = params["model"]
+    exe = ref_binary(model)
+    if not os.path.exists(exe):
+        raise RuntimeError(f"reference binary not built: {exe}")
+    with tempfile.TemporaryDirectory() as td:
+        ip, op = os.path.join(td, "input"), os.path.join(td, "output")
+        write_rules({k: v for k, v in params.items()}, ip)
+        proc = subprocess.run([exe, ip, op], capture_output=True, text=True, timeout=timeout)
+        out = read_rules(op) if os.path.exists(op + "_Format.txt") else {}
+        out["retcode"] = proc.returncode
+        out["log"] = proc.stdout + proc.stderr
+    return out
+
+
+def parse_stencils(s: str, dim: int):
+    """Parse the 'stencils' string printed for the getStencils key (HFMInterface.hxx:360-375,
+    EulerianStencil.hxx:76-87): returns a list of (index, [(weight, offset), ...]) in print order."""
+    import re
+    out = []
+    # each entry: {{i,j[,k]},{forward...,symmetric...}}
+    num = r"-?\d+"
+    flt = r"[-+0-9.eE]+|inf|nan"
+    pat_entry = re.compile(r"\{\{(" + num + r"(?:," + num + r")*)\},(\{.*?\}\})\}")
+    pat_diff = re.compile(r"\{(" + flt + r"),\{(" + num + r"(?:," + num + r")*)\}\}")
+    for m in pat_entry.finditer(s):
+        idx = tuple(int(t) for t in m.group(1).split(","))
+        diffs = [(float(d.group(1)), tuple(int(t) for t in d.group(2).split(",")))
+                 for d in pat_diff.finditer(m.group(2))]
+        out.append((idx, diffs))
+    return out

@@ -1,0 +1,730 @@
+// hfm_oracle.cpp -- CPU restatement of HamiltonFastMarching's hot path (test infrastructure).
+//
+// This file is the checker, never the product: it is built into oracle/_build/ and is only
+// loaded by tests/, by the smoke check and by the CPU-baseline leg of bench.py.
+// Every function names the reference lines it restates.  All arithmetic is FP64 in the
+// reference's operation order; the file is compiled with -ffp-contract=off like the reference.
+#include "hfm_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <ctime>
+#include <limits>
+#include <map>
+#include <queue>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+const double kInf = std::numeric_limits<double>::infinity();
+const double kPi = 3.141592653589793238462643383279502884L;  // CommonTraits.h:46 (long double -> double)
+const double kEps = std::numeric_limits<double>::epsilon();
+
+struct OracleError {
+    std::string msg;
+};
+
+// ---------------------------------------------------------------------------------------------
+// Small FP64 linear algebra on packed symmetric matrices.
+// Packing (SymmetricMatrixType.hxx:11-16): index(i,j) = i(i+1)/2 + j for i >= j.
+// ---------------------------------------------------------------------------------------------
+template <int D> struct Sym {
+    static const int K = D * (D + 1) / 2;
+    double m[K];
+    static int lin(int i, int j) { if (i < j) std::swap(i, j); return i * (i + 1) / 2 + j; }
+    double operator()(int i, int j) const { return m[lin(i, j)]; }
+    double &at(int i, int j) { return m[lin(i, j)]; }
+    // Trace: sequential sum of diagonal (SymmetricMatrixType.hxx:186-192).
+    double trace() const { double r = 0; for (int i = 0; i < D; ++i) r += (*this)(i, i); return r; }
+    // Determinant, same expansion as SymmetricMatrixType.hxx:199-209.
+    double det() const {
+        const Sym &c = *this;
+        if (D == 2) return c(0, 0) * c(1, 1) - c(1, 0) * c(0, 1);
+        return c(0, 0) * (c(1, 1) * c(2, 2) - c(1, 2) * c(2, 1))
+             + c(0, 1) * (2. * c(1, 2) * c(2, 0) - c(1, 0) * c(2, 2))
+             - c(0, 2) * (c(1, 1) * c(2, 0));
+    }
+    // Comatrix (SymmetricMatrixType.hxx:220-237).
+    Sym comatrix() const {
+        Sym r; const Sym &c = *this;
+        if (D == 2) { r.m[0] = c(1, 1); r.m[1] = -c(1, 0); r.m[2] = c(0, 0); return r; }
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j <= i; ++j) {
+                const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+                r.at(i, j) = c(i1, j1) * c(i2, j2) - c(i1, j2) * c(i2, j1);
+            }
+        return r;
+    }
+    // Inverse = Comatrix / Determinant, the division done as a reciprocal multiply
+    // (SymmetricMatrixType.hxx:239-243, FriendOperators.h:35, VectorType.h:37-42).
+    Sym inverse() const {
+        Sym r = comatrix();
+        const double b = 1. / det();
+        for (int k = 0; k < K; ++k) r.m[k] *= b;
+        return r;
+    }
+    // <u, M v> with the reference's summation order (SymmetricMatrixType.hxx:18-30).
+    double sp(const int *u, const int *v) const {
+        double sum = 0.;
+        for (int i = 0; i < D; ++i) {
+            double sumi = 0.;
+            for (int j = 0; j < D; ++j) sumi += (*this)(i, j) * v[j];
+            sum += u[i] * sumi;
+        }
+        return sum;
+    }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Selling obtuse superbase + tensor decomposition (BasisReduction.hpp:12-15,65-157).
+// ---------------------------------------------------------------------------------------------
+const int kMaxIt = 200;  // BasisReduction.h:72
+
+template <int D> struct Decomp {
+    static const int K = D * (D + 1) / 2;
+    int off[K][D];
+    double w[K];
+};
+
+template <int D> Decomp<D> tensor_decomposition(const Sym<D> &m) {
+    int sb[D + 1][D];
+    for (int i = 0; i <= D; ++i)
+        for (int j = 0; j < D; ++j) sb[i][j] = i < D ? (i == j) : -1;
+    if (m.det() <= 0 || m.trace() <= 0)
+        throw OracleError{"ObtuseSuperbase error : input must be positive definite"};
+    bool reduced = true;
+    for (int it = 0; reduced && it < kMaxIt; ++it) {
+        reduced = false;
+        for (int i = 0; i < D; ++i)
+            for (int j = i + 1; j <= D; ++j) {
+                if (m.sp(sb[i], sb[j]) > 0) {
+                    reduced = true;
+                    if (D == 2) {
+                        const int k = 3 - i - j;
+                        for (int c = 0; c < D; ++c) sb[k][c] = sb[i][c] - sb[j][c];
+                    } else {
+                        int kl[2], n = 0;
+                        for (int q = 0; q <= D; ++q) if (q != i && q != j) kl[n++] = q;
+                        for (int c = 0; c < D; ++c) { sb[kl[0]][c] += sb[i][c]; sb[kl[1]][c] += sb[i][c]; }
+                    }
+                    for (int c = 0; c < D; ++c) sb[i][c] *= -1;
+                }
+            }
+    }
+    if (reduced) throw OracleError{"ObtuseSuperbase error : did not terminate"};
+    Decomp<D> r;
+    if (D == 2) {
+        for (int i = 0; i < 3; ++i) {
+            const int j = (i + 1) % 3, k = (i + 2) % 3;
+            r.off[i][0] = -sb[i][1];
+            r.off[i][1] = sb[i][0];
+            r.w[i] = -m.sp(sb[j], sb[k]);
+        }
+    } else {
+        int s = 0;
+        for (int i = 0; i < 4; ++i)
+            for (int j = i + 1; j < 4; ++j) {
+                int rem[2], n = 0;
+                for (int q = 0; q < 4; ++q) if (q != i && q != j) rem[n++] = q;
+                const int *u = sb[rem[0]], *v = sb[rem[1]];
+                for (int c = 0; c < 3; ++c) {
+                    const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+                    r.off[s][c] = u[c1] * v[c2] - u[c2] * v[c1];
+                }
+                r.w[s] = -m.sp(sb[i], sb[j]);
+                ++s;
+            }
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Stencils.
+// ---------------------------------------------------------------------------------------------
+const int kMaxDiff = 27;
+
+struct Diff {
+    int8_t off[3];
+    double w;
+};
+
+// One stencil in the reference layout: forward differences first, then symmetric ones
+// (EulerianStencil.h:23-40, bit layout of ActiveNeighFlagType).
+struct Stencil {
+    int nfwd = 0, nsym = 0;
+    Diff d[kMaxDiff];
+};
+
+// Voronoi1Mat (CommonTraits.h:82-105) applied to D = Inverse(M) o [1/(h_i h_j)]
+// (Riemannian.h:41-44,50-52,68-71).
+template <int D>
+void riemann_stencil(const double *mcomp, bool dual, const double *h, Stencil &st) {
+    Sym<D> m;
+    for (int k = 0; k < Sym<D>::K; ++k) m.m[k] = mcomp[k];
+    if (!dual) m = m.inverse();
+    Sym<D> gs;
+    for (int i = 0; i < D; ++i)
+        for (int j = 0; j <= i; ++j) gs.at(i, j) = h[i] * h[j];
+    for (int k = 0; k < Sym<D>::K; ++k) m.m[k] = m.m[k] * (1. / gs.m[k]);
+    const double atol = 1000 * kEps * m.trace();
+    const Decomp<D> dec = tensor_decomposition<D>(m);
+    st.nfwd = 0;
+    st.nsym = Sym<D>::K;
+    for (int k = 0; k < Sym<D>::K; ++k) {
+        Diff &df = st.d[k];
+        df.w = dec.w[k] < atol ? 0. : dec.w[k];
+        df.off[0] = df.off[1] = df.off[2] = 0;
+        for (int c = 0; c < D; ++c) df.off[c] = (int8_t)dec.off[k][c];
+    }
+}
+
+// Voronoi1Vec (CommonTraits.h:118-161), writing Dv diffs into out[] with the reduction's
+// dimension Dv (offset components beyond Dv stay 0).
+template <int Dv>
+void voronoi1vec(const double *v, double eps, Diff *out) {
+    const double angleTol = 1. / std::sqrt(double(Dv - 1));
+    const double weightTol = 1000 * kEps;
+    Sym<Dv> relaxed;
+    double sqn = 0.;
+    for (int i = 0; i < Dv; ++i) sqn += v[i] * v[i];
+    for (int i = 0; i < Dv; ++i)
+        for (int j = 0; j <= i; ++j) {
+            const double exact = v[i] * v[j];
+            const double id = (i == j) ? 1. : 0.;
+            relaxed.at(i, j) = exact * (1. - eps * eps) + (sqn * id) * (eps * eps);
+        }
+    const Decomp<Dv> dec = tensor_decomposition<Dv>(relaxed);
+    const double vNorm2 = sqn;
+    const double wTol = vNorm2 * weightTol;
+    const double angTol = (1 + angleTol * angleTol) / vNorm2;
+    for (int k = 0; k < Sym<Dv>::K; ++k) {
+        double off[Dv];
+        for (int c = 0; c < Dv; ++c) off[c] = double(dec.off[k][c]);
+        double scal = 0., offn = 0.;
+        for (int c = 0; c < Dv; ++c) scal += v[c] * off[c];
+        for (int c = 0; c < Dv; ++c) offn += off[c] * off[c];
+        const double weight = dec.w[k];
+        out[k].w = (weight < wTol || (scal * scal) * angTol < offn) ? 0. : weight;
+        out[k].off[0] = out[k].off[1] = out[k].off[2] = 0;
+        for (int c = 0; c < Dv; ++c) out[k].off[c] = (int8_t)(scal <= 0 ? dec.off[k][c] : -dec.off[k][c]);
+    }
+}
+
+// StencilReedsSheppForward2::SetStencil (Curvature2.h:88-102): 3 forward + 1 symmetric.
+void rsf2_stencil(int itheta, int ntheta, double gridScale, double xi, double eps, Stencil &st) {
+    const double dependScale = 2 * kPi / ntheta;
+    const double theta = itheta * dependScale;
+    double v[2] = {std::cos(theta), std::sin(theta)};
+    const double b = 1. / gridScale;  // Vector /= scalar is a reciprocal multiply
+    v[0] *= b; v[1] *= b;
+    st.nfwd = 3; st.nsym = 1;
+    voronoi1vec<2>(v, eps, st.d);
+    Diff &ang = st.d[3];
+    ang.off[0] = 0; ang.off[1] = 0; ang.off[2] = -1;
+    const double t = xi * dependScale;
+    ang.w = 1. / (t * t);
+}
+
+// StencilElastica2<5>::SetStencil (Curvature2.h:178-217) with the Fejer weights of :233-234.
+void elastica2_stencil(int itheta, int ntheta, double gridScale, double xi, double eps, Stencil &st) {
+    const int nFejer = 5;
+    const double fejer[5] = {0.167781, 0.525552, 0.613333, 0.525552, 0.167781};
+    const double dependScale = 2 * kPi / ntheta;
+    const double theta = (2. * kPi * itheta) / ntheta;
+    st.nfwd = 27; st.nsym = 0;
+    for (int l = 0; l < nFejer / 2; ++l) {
+        const double phi = kPi * (l + 0.5) / nFejer;
+        const double v[3] = {std::sin(phi) * std::cos(theta) / gridScale,
+                             std::sin(phi) * std::sin(theta) / gridScale,
+                             std::cos(phi) / (xi * dependScale)};
+        const int s0 = 2 * l, s1 = 2 * l + 1;
+        voronoi1vec<3>(v, eps, &st.d[6 * s0]);
+        for (int i = 0; i < 6; ++i) {
+            st.d[6 * s0 + i].w *= fejer[l];
+            st.d[6 * s1 + i] = st.d[6 * s0 + i];
+            st.d[6 * s1 + i].off[2] = (int8_t)(-st.d[6 * s1 + i].off[2]);
+        }
+    }
+    const double v2[2] = {std::cos(theta) / gridScale, std::sin(theta) / gridScale};
+    const int s = nFejer - 1;
+    voronoi1vec<2>(v2, eps, &st.d[6 * s]);
+    for (int i = 0; i < 3; ++i) st.d[6 * s + i].w *= fejer[nFejer / 2];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Grid and problem description.
+// ---------------------------------------------------------------------------------------------
+struct Problem {
+    int model = 0;
+    int dim = 2;
+    int dims[3] = {1, 1, 1};
+    bool periodic[3] = {false, false, false};
+    long N = 0;
+    double h[3] = {1, 1, 1};
+    double origin[3] = {0, 0, 0};
+    double dependScale = 1;  // curvature models: 2 pi / ntheta
+    // Riemann: per-point stencils. Curvature: per-theta stencils + per-point speed.
+    std::vector<Stencil> stencils;
+    std::vector<double> speed;
+    bool shared = false;
+
+    const Stencil &stencil(long x) const {
+        if (!shared) return stencils[x];
+        return stencils[x % dims[2]];
+    }
+    double mult(long x) const { return shared ? speed[x] : 1.; }
+    // EulerianDifference<...,1>::Weight = baseWeight * mult^2 (EulerianStencil.h:84-97);
+    // the Riemann difference ignores the multiplier.
+    double weight(const Diff &d, double mlt) const { return shared ? d.w * (mlt * mlt) : d.w; }
+
+    void unravel(long x, int *idx) const {
+        for (int i = dim - 1; i > 0; --i) { idx[i] = int(x % dims[i]); x /= dims[i]; }
+        idx[0] = int(x);
+    }
+    long ravel(const int *idx) const {
+        long r = idx[0];
+        for (int i = 1; i < dim; ++i) r = r * dims[i] + idx[i];
+        return r;
+    }
+    // PeriodicGrid::PeriodizeNoBase for Closed / Periodic axes (PeriodicGrid.hxx:120-140).
+    bool periodize(int *idx) const {
+        for (int i = 0; i < dim; ++i) {
+            if (periodic[i]) {
+                int r = idx[i] % dims[i];
+                if (r < 0) r += dims[i];
+                idx[i] = r;
+            } else if (idx[i] < 0 || idx[i] >= dims[i]) return false;
+        }
+        return true;
+    }
+    // x + s*offset, or -1 when it leaves a closed axis.
+    long shift(long x, const int8_t *off, int s) const {
+        int idx[3] = {0, 0, 0};
+        unravel(x, idx);
+        for (int i = 0; i < dim; ++i) idx[i] += s * off[i];
+        if (!periodize(idx)) return -1;
+        return ravel(idx);
+    }
+    // ADim (HFM_ParamDefault.hxx:77-80 for Riemann; :106-119 for the R2xS1 bundle).
+    void adim(const double *p, double *q) const {
+        for (int i = 0; i < dim; ++i) {
+            if (shared && i == 2) q[i] = (p[i] - origin[i]) / dependScale;
+            else q[i] = (p[i] - origin[i]) / h[i];
+        }
+    }
+    double redim_vec(double v, int i) const { return (shared && i == 2) ? v * dependScale : v * h[i]; }
+};
+
+void setup_problem(Problem &P, int model, const int *dims, const double *gridScales, const double *origin,
+                   const double *metric, int metric_is_field, int dual, double xi, double eps,
+                   const double *cost, int cost_is_field) {
+    P.model = model;
+    P.dim = (model == ORACLE_RIEMANN2) ? 2 : 3;
+    P.N = 1;
+    for (int i = 0; i < P.dim; ++i) {
+        if (dims[i] <= 0) throw OracleError{"BaseGrid error : domain size must be positive"};
+        P.dims[i] = dims[i];
+        P.N *= dims[i];
+    }
+    if (model == ORACLE_RIEMANN2 || model == ORACLE_RIEMANN3) {
+        for (int i = 0; i < P.dim; ++i) {
+            P.h[i] = gridScales[i];
+            if (!(P.h[i] > 0)) throw OracleError{"Parametrization error : gridscales should be positive"};
+            P.origin[i] = origin ? origin[i] : 0.;
+        }
+        P.shared = false;
+        P.stencils.resize(P.N);
+        const int K = P.dim == 2 ? 3 : 6;
+        for (long x = 0; x < P.N; ++x) {
+            const double *mc = metric + (metric_is_field ? x * K : 0);
+            if (P.dim == 2) riemann_stencil<2>(mc, dual != 0, P.h, P.stencils[x]);
+            else riemann_stencil<3>(mc, dual != 0, P.h, P.stencils[x]);
+        }
+    } else {
+        const double gs = gridScales[0];
+        if (!(gs > 0)) throw OracleError{"Parametrization error : gridscale should be positive"};
+        P.h[0] = P.h[1] = gs;
+        P.periodic[2] = true;
+        P.dependScale = 2 * kPi / dims[2];
+        P.h[2] = P.dependScale;
+        P.origin[0] = origin ? origin[0] : 0.;
+        P.origin[1] = origin ? origin[1] : 0.;
+        P.origin[2] = -P.dependScale / 2.;
+        P.shared = true;
+        P.stencils.resize(dims[2]);
+        for (int t = 0; t < dims[2]; ++t) {
+            if (model == ORACLE_RSF2) rsf2_stencil(t, dims[2], gs, xi, eps, P.stencils[t]);
+            else elastica2_stencil(t, dims[2], gs, xi, eps, P.stencils[t]);
+        }
+        P.speed.resize(P.N);
+        for (long x = 0; x < P.N; ++x) P.speed[x] = 1. / cost[cost_is_field ? x : 0];  // DataSource_Inverse
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Fast marching (HamiltonFastMarching.hxx:59-169, EulerianStencil.hxx:30-153).
+// ---------------------------------------------------------------------------------------------
+struct Quad {
+    double minVal = -kInf;
+    double a = 0, b = 0, c = -1;
+    void add(double value, double weight) {  // EulerianStencil.hxx:63-72
+        const double w = weight, v = value - minVal;
+        const double wv = w * v, wv2 = (w * v) * v;
+        a += w; b += wv; c += wv2;
+    }
+    double solve() const {  // EulerianStencil.hxx:30-50, n = 1
+        if (a == 0) return minVal + kInf;
+        const double delta2 = b * b - a * c;
+        const double delta = std::sqrt(std::max(0., delta2));
+        const double res = (b + delta) / a;
+        return minVal + res;
+    }
+};
+
+struct Off3 {
+    int8_t o[3];
+    bool operator<(const Off3 &r) const {
+        for (int i = 0; i < 3; ++i) if (o[i] != r.o[i]) return o[i] < r.o[i];
+        return false;
+    }
+    bool operator==(const Off3 &r) const { return o[0] == r.o[0] && o[1] == r.o[1] && o[2] == r.o[2]; }
+};
+
+struct FMState {
+    std::vector<double> values;
+    std::vector<uint64_t> active;
+    std::vector<char> accepted;
+};
+
+// Reversed offsets (HFM_StencilRecompute.hxx:81-124; HFM_StencilShare.hxx:132-181): for each
+// accepted point y, the sorted unique offsets e such that some x = y - e has a nonzero
+// difference reaching y.  Stored as CSR.
+void reversed_offsets(const Problem &P, std::vector<long> &start, std::vector<Off3> &offs) {
+    const long nKeys = P.shared ? P.dims[2] : P.N;
+    std::vector<std::vector<Off3>> lists(nKeys);
+    auto insert = [&](long x, const int8_t *off, int s) {
+        const long y = P.shift(x, off, s);
+        if (y < 0) return;
+        Off3 e;
+        for (int c = 0; c < 3; ++c) e.o[c] = (int8_t)(s * off[c]);
+        lists[P.shared ? y % P.dims[2] : y].push_back(e);
+    };
+    for (long key = 0; key < nKeys; ++key) {
+        long x = key;
+        if (P.shared) {  // IndexFromShortIndex: mid-domain (x,y), theta = key
+            int idx[3] = {P.dims[0] / 2, P.dims[1] / 2, int(key)};
+            x = P.ravel(idx);
+        }
+        const Stencil &st = P.stencil(x);
+        for (int k = 0; k < st.nfwd; ++k) if (st.d[k].w != 0.) insert(x, st.d[k].off, 1);
+        for (int k = st.nfwd; k < st.nfwd + st.nsym; ++k)
+            if (st.d[k].w != 0.) { insert(x, st.d[k].off, 1); insert(x, st.d[k].off, -1); }
+    }
+    start.assign(nKeys + 1, 0);
+    offs.clear();
+    for (long key = 0; key < nKeys; ++key) {
+        auto &l = lists[key];
+        std::sort(l.begin(), l.end());
+        l.erase(std::unique(l.begin(), l.end()), l.end());
+        start[key] = long(offs.size());
+        offs.insert(offs.end(), l.begin(), l.end());
+        std::vector<Off3>().swap(l);
+    }
+    start[nKeys] = long(offs.size());
+}
+
+struct QueueElement {
+    long linearIndex;
+    double value;
+    bool operator<(const QueueElement &o) const { return value > o.value; }
+};
+
+void fast_marching(const Problem &P, const std::vector<std::pair<long, double>> &seeds, FMState &S,
+                   double *tStencil, double *tFM) {
+    const clock_t t0 = clock();
+    std::vector<long> start;
+    std::vector<Off3> offs;
+    reversed_offsets(P, start, offs);
+    const clock_t t1 = clock();
+    S.values.assign(P.N, kInf);
+    S.active.assign(P.N, 0);
+    S.accepted.assign(P.N, 0);
+    std::vector<Quad> quads(P.N);
+    std::priority_queue<QueueElement> queue;
+    for (const auto &sv : seeds) {
+        S.values[sv.first] = sv.second;
+        queue.push({sv.first, sv.second});
+    }
+    while (!queue.empty()) {
+        const QueueElement top = queue.top();
+        queue.pop();
+        if (S.accepted[top.linearIndex]) continue;
+        const long y = top.linearIndex;
+        S.accepted[y] = 1;
+        const long key = P.shared ? y % P.dims[2] : y;
+        for (long r = start[key]; r < start[key + 1]; ++r) {
+            const Off3 &e = offs[r];
+            // ConditionalUpdate (HamiltonFastMarching.hxx:136-154)
+            const long x = P.shift(y, e.o, -1);
+            if (x < 0) continue;
+            if (S.accepted[x]) continue;
+            if (S.active[x] == 0 && S.values[x] != kInf) continue;  // seeds are never updated
+            if (S.values[x] <= top.value) continue;
+            // HopfLaxUpdate (EulerianStencil.hxx:104-153)
+            Quad &q = quads[x];
+            if (q.minVal == -kInf) q.minVal = top.value;
+            const Stencil &st = P.stencil(x);
+            const double mlt = P.mult(x);
+            uint64_t &act = S.active[x];
+            int iNeigh = 0;
+            for (int k = 0; k < st.nfwd; ++k, ++iNeigh) {
+                Off3 d; std::memcpy(d.o, st.d[k].off, 3);
+                if (d == e) { act |= (1ull << iNeigh); q.add(top.value, P.weight(st.d[k], mlt)); }
+            }
+            for (int k = st.nfwd; k < st.nfwd + st.nsym; ++k, iNeigh += 2) {
+                Off3 d, md;
+                std::memcpy(d.o, st.d[k].off, 3);
+                for (int c = 0; c < 3; ++c) md.o[c] = (int8_t)(-d.o[c]);
+                if (d == e && !(act >> (iNeigh + 1) & 1)) {
+                    act |= (1ull << iNeigh); q.add(top.value, P.weight(st.d[k], mlt));
+                }
+                if (md == e && !(act >> iNeigh & 1)) {
+                    act |= (1ull << (iNeigh + 1)); q.add(top.value, P.weight(st.d[k], mlt));
+                }
+            }
+            const double value = q.solve();
+            S.values[x] = value;
+            queue.push({x, value});
+        }
+    }
+    const clock_t t2 = clock();
+    if (tStencil) *tStencil = double(t1 - t0) / CLOCKS_PER_SEC;
+    if (tFM) *tFM = double(t2 - t1) / CLOCKS_PER_SEC;
+}
+
+// Active differences of x as (neighbour, weight, offset, sign) in the reference's order.
+struct ActiveDiff { long neigh; double w; const int8_t *off; int s; };
+int active_diffs(const Problem &P, const FMState &S, long x, ActiveDiff *out) {
+    const Stencil &st = P.stencil(x);
+    const double mlt = P.mult(x);
+    const uint64_t act = S.active[x];
+    int n = 0, iNeigh = 0;
+    for (int k = 0; k < st.nfwd; ++k, ++iNeigh)
+        if (act >> iNeigh & 1) out[n++] = {P.shift(x, st.d[k].off, 1), P.weight(st.d[k], mlt), st.d[k].off, 1};
+    for (int k = st.nfwd; k < st.nfwd + st.nsym; ++k, iNeigh += 2) {
+        if (act >> iNeigh & 1) out[n++] = {P.shift(x, st.d[k].off, 1), P.weight(st.d[k], mlt), st.d[k].off, 1};
+        if (act >> (iNeigh + 1) & 1) out[n++] = {P.shift(x, st.d[k].off, -1), P.weight(st.d[k], mlt), st.d[k].off, -1};
+    }
+    return n;
+}
+
+// GeodesicFlow (HamiltonFastMarching.hxx:363-378) via Recompute / HopfLaxRecompute at order 1
+// (EulerianStencil.hxx:155-233), exported as -ReDim(flow) (HFMInterface.hxx:544-553).
+void geodesic_flow(const Problem &P, const FMState &S, double *flow) {
+    ActiveDiff ad[2 * kMaxDiff];
+    for (long x = 0; x < P.N; ++x) {
+        double f[3] = {0, 0, 0};
+        const int n = S.active[x] ? active_diffs(P, S, x, ad) : 0;
+        if (n > 0) {
+            double vals[2 * kMaxDiff];
+            Quad q; q.minVal = kInf;
+            for (int i = 0; i < n; ++i) { vals[i] = S.values[ad[i].neigh]; q.minVal = std::min(q.minVal, vals[i]); }
+            for (int i = 0; i < n; ++i) q.add(vals[i], 1. * ad[i].w);
+            const double value = q.solve();
+            for (int i = 0; i < n; ++i) {
+                const double wpd = ad[i].w * std::max(0., value - vals[i]);
+                for (int c = 0; c < P.dim; ++c) f[c] += wpd * double(ad[i].s * ad[i].off[c]);
+            }
+        }
+        for (int c = 0; c < P.dim; ++c) flow[x * P.dim + c] = -P.redim_vec(f[c], c);
+    }
+}
+
+// FirstVariation::ValueVariation (FirstVariation.h:198-251) and BackwardVariation (:341-385).
+double value_variation(const Problem &P, const FMState &S, long x,
+                       std::vector<std::pair<long, double>> &neighbors) {
+    ActiveDiff ad[2 * kMaxDiff];
+    const int n = active_diffs(P, S, x, ad);
+    const double value = S.values[x];
+    double var = 0, weightSum = 0;
+    neighbors.clear();
+    for (int i = 0; i < n; ++i) {
+        const double nv = S.values[ad[i].neigh];
+        const double valueDiff = std::max(0., value - nv);
+        const double w = ad[i].w * valueDiff;
+        weightSum += w;
+        neighbors.push_back({ad[i].neigh, w});
+        var += w * valueDiff;
+    }
+    if (weightSum == 0) return var;
+    const double invWeight = 1. / weightSum;
+    for (auto &nb : neighbors) nb.second *= invWeight;
+    return P.shared ? invWeight * (P.mult(x) * var) : invWeight * var;
+}
+
+void backward_variation(const Problem &P, const FMState &S, const std::vector<std::pair<long, double>> &base,
+                        double *result, std::vector<std::pair<long, double>> &seedSens) {
+    std::map<std::pair<double, long>, double> geo;
+    for (const auto &iw : base) geo.insert({{S.values[iw.first], iw.first}, iw.second});
+    std::vector<std::pair<long, double>> neighbors;
+    while (!geo.empty()) {
+        auto it = std::prev(geo.end());
+        const double value = it->first.first; (void)value;
+        const long index = it->first.second;
+        const double weight = it->second;
+        geo.erase(it);
+        if (weight == 0) continue;
+        const double var = value_variation(P, S, index, neighbors);
+        if (neighbors.empty()) seedSens.push_back({index, weight});
+        result[index] = weight * var;
+        for (const auto &nb : neighbors) {
+            const std::pair<double, long> key{S.values[nb.first], nb.first};
+            auto jt = geo.find(key);
+            if (jt == geo.end()) geo[key] = nb.second * weight;
+            else jt->second += nb.second * weight;
+        }
+    }
+}
+
+// Seeds: ADim then floor, out-of-range seeds dropped, first value kept for duplicates
+// (StaticFactoring.hxx:242-393, BaseGrid.h:61-67).
+std::vector<std::pair<long, double>> make_seeds(const Problem &P, int nseeds, const double *seeds,
+                                                const double *seedValues) {
+    std::map<long, double> m;
+    for (int s = 0; s < nseeds; ++s) {
+        double q[3];
+        P.adim(seeds + s * P.dim, q);
+        int idx[3] = {0, 0, 0};
+        for (int c = 0; c < P.dim; ++c) idx[c] = (int)std::floor(q[c]);
+        if (!P.periodize(idx)) continue;
+        m.insert({P.ravel(idx), seedValues ? seedValues[s] : 0.});
+    }
+    if (m.empty() && nseeds > 0) throw OracleError{"Error : seeds incorrectly set"};
+    return std::vector<std::pair<long, double>>(m.begin(), m.end());
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *oracle_last_error(void) { return g_err.c_str(); }
+
+int oracle_riemann_stencils(int dim, const int *dims, const double *gridScales, const double *metric,
+                            int metric_is_field, int dual, int8_t *offsets, double *weights) {
+    try {
+        g_err.clear();
+        long N = 1;
+        for (int i = 0; i < dim; ++i) N *= dims[i];
+        const int K = dim == 2 ? 3 : 6;
+        Stencil st;
+        for (long x = 0; x < N; ++x) {
+            const double *mc = metric + (metric_is_field ? x * K : 0);
+            if (dim == 2) riemann_stencil<2>(mc, dual != 0, gridScales, st);
+            else riemann_stencil<3>(mc, dual != 0, gridScales, st);
+            for (int k = 0; k < K; ++k) {
+                weights[x * K + k] = st.d[k].w;
+                for (int c = 0; c < dim; ++c) offsets[(x * K + k) * dim + c] = st.d[k].off[c];
+            }
+        }
+        return 0;
+    } catch (const OracleError &e) {
+        g_err = e.msg;
+        return -1;
+    }
+}
+
+int oracle_curvature_stencils(int model, int ntheta, double gridScale, double xi, double eps,
+                              int8_t *offsets, double *weights, int *symmetric) {
+    try {
+        g_err.clear();
+        Stencil st;
+        for (int t = 0; t < ntheta; ++t) {
+            if (model == ORACLE_RSF2) rsf2_stencil(t, ntheta, gridScale, xi, eps, st);
+            else if (model == ORACLE_ELASTICA2) elastica2_stencil(t, ntheta, gridScale, xi, eps, st);
+            else throw OracleError{"not a curvature model"};
+            const int n = st.nfwd + st.nsym;
+            for (int k = 0; k < n; ++k) {
+                weights[t * n + k] = st.d[k].w;
+                for (int c = 0; c < 3; ++c) offsets[(t * n + k) * 3 + c] = st.d[k].off[c];
+                if (t == 0) symmetric[k] = k >= st.nfwd;
+            }
+        }
+        return 0;
+    } catch (const OracleError &e) {
+        g_err = e.msg;
+        return -1;
+    }
+}
+
+int oracle_solve(int model, const int *dims, const double *gridScales, const double *origin,
+                 const double *metric, int metric_is_field, int dual, double xi, double eps,
+                 const double *cost, int cost_is_field, int nseeds, const double *seeds,
+                 const double *seedValues, double *values, uint64_t *activeNeighs, double *flow,
+                 double *timings) {
+    try {
+        g_err.clear();
+        Problem P;
+        const clock_t t0 = clock();
+        setup_problem(P, model, dims, gridScales, origin, metric, metric_is_field, dual, xi, eps, cost,
+                      cost_is_field);
+        const clock_t t1 = clock();
+        const auto sd = make_seeds(P, nseeds, seeds, seedValues);
+        FMState S;
+        double tS = 0, tF = 0;
+        fast_marching(P, sd, S, &tS, &tF);
+        if (values) std::copy(S.values.begin(), S.values.end(), values);
+        if (activeNeighs) std::copy(S.active.begin(), S.active.end(), activeNeighs);
+        if (flow) geodesic_flow(P, S, flow);
+        if (timings) { timings[0] = double(t1 - t0) / CLOCKS_PER_SEC + tS; timings[1] = tF; }
+        return 0;
+    } catch (const OracleError &e) {
+        g_err = e.msg;
+        return -1;
+    }
+}
+
+int oracle_solve_backward(int model, const int *dims, const double *gridScales, const double *origin,
+                          const double *metric, int metric_is_field, int dual, double xi, double eps,
+                          const double *cost, int cost_is_field, int nseeds, const double *seeds,
+                          const double *seedValues, int ntargets, const double *targets,
+                          const double *tweights, double *values, double *costSensitivity,
+                          int64_t *seedSensIndex, double *seedSensWeight, int nseeds_max, int *nseedSens) {
+    try {
+        g_err.clear();
+        Problem P;
+        setup_problem(P, model, dims, gridScales, origin, metric, metric_is_field, dual, xi, eps, cost,
+                      cost_is_field);
+        const auto sd = make_seeds(P, nseeds, seeds, seedValues);
+        FMState S;
+        fast_marching(P, sd, S, nullptr, nullptr);
+        if (values) std::copy(S.values.begin(), S.values.end(), values);
+        std::vector<std::pair<long, double>> base;
+        for (int t = 0; t < ntargets; ++t) {
+            double q[3];
+            P.adim(targets + t * P.dim, q);
+            int idx[3] = {0, 0, 0};
+            for (int c = 0; c < P.dim; ++c) idx[c] = (int)std::floor(q[c]);
+            if (!P.periodize(idx)) throw OracleError{"Error : inspectSensitivity data points are out of range"};
+            base.push_back({P.ravel(idx), tweights ? tweights[t] : 1.});
+        }
+        std::fill(costSensitivity, costSensitivity + P.N, 0.);
+        std::vector<std::pair<long, double>> ss;
+        backward_variation(P, S, base, costSensitivity, ss);
+        int n = 0;
+        for (const auto &p : ss) {
+            if (n < nseeds_max) { seedSensIndex[n] = p.first; seedSensWeight[n] = p.second; }
+            ++n;
+        }
+        *nseedSens = n;
+        return 0;
+    } catch (const OracleError &e) {
+        g_err = e.msg;
+        return -1;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,88 @@
+/*
+ * hfm_oracle.h -- CPU restatement of HamiltonFastMarching's hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the GPU library is compared
+ * against, and the single-core CPU baseline timed beside it.  It is never linked into, loaded
+ * by, or called from the product library (hamiltonfastmarching_amd/).  Only tests/, the smoke
+ * check in __graft_entry__.py and the cpu_baseline leg of bench.py may use it.
+ *
+ * It restates, in FP64 and in the reference's operation order:
+ *   - the Selling obtuse-superbase tensor decomposition
+ *       (JMM_CPPLibs/JMM_CPPLibs/LinearAlgebra/Implementation/BasisReduction.hpp:65-157)
+ *     and the Riemannian stencil construction
+ *       (Headers/Specializations/Riemannian.h:41-71, Headers/Specializations/CommonTraits.h:82-105);
+ *   - the relaxed rank-one reduction Voronoi1Vec (CommonTraits.h:118-161) and the shared
+ *     per-theta stencils of ReedsSheppForward2 / Elastica2<5> (Headers/Specializations/Curvature2.h);
+ *   - the serial heap Fast-Marching loop over reversed offsets
+ *       (Headers/Base/Implementation/HamiltonFastMarching.hxx:59-169,
+ *        HFM_StencilRecompute.hxx:81-124, HFM_StencilShare.hxx:132-181)
+ *     with the incremental quadratic solve (Headers/Base/Implementation/EulerianStencil.hxx:30-153);
+ *   - the per-point recompute / geodesic flow (HamiltonFastMarching.hxx:187-272,363-378,
+ *     EulerianStencil.hxx:155-233);
+ *   - the backward differentiation (Headers/ExtraAlgorithms/FirstVariation.h:198-251,341-385).
+ *
+ * Array conventions: every field is row-major over the grid (first index slowest), element
+ * components last, exactly like the reference's RowMajor IO ordering.
+ */
+#ifndef HFM_ORACLE_H
+#define HFM_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORACLE_RIEMANN2 = 0, ORACLE_RIEMANN3 = 1, ORACLE_RSF2 = 2, ORACLE_ELASTICA2 = 3 };
+
+/* Last error message (empty if none). */
+const char *oracle_last_error(void);
+
+/* Selling stencils of a Riemannian metric field.
+ *   dim: 2 or 3; dims[dim]; gridScales[dim];
+ *   metric: N x K (K = 3 or 6, components xx,xy,yy[,xz,yz,zz]); dual=1 if it is the dual metric.
+ *   offsets: N x K x dim int8 (reference slot order); weights: N x K doubles (after zeroing).
+ * Returns 0, or -1 on a non positive-definite tensor / non-terminating reduction. */
+int oracle_riemann_stencils(int dim, const int *dims, const double *gridScales,
+                            const double *metric, int metric_is_field, int dual,
+                            int8_t *offsets, double *weights);
+
+/* Shared per-theta stencils of a curvature model (ORACLE_RSF2 or ORACLE_ELASTICA2).
+ *   ntheta: dims[2]; gridScale; xi; eps.
+ *   offsets: ntheta x nDiff x 3 int8; weights: ntheta x nDiff doubles; symmetric: nDiff flags.
+ *   nDiff = 4 (RSF2: 3 forward + 1 symmetric) or 27 (Elastica2<5>: 27 forward). */
+int oracle_curvature_stencils(int model, int ntheta, double gridScale, double xi, double eps,
+                              int8_t *offsets, double *weights, int *symmetric);
+
+/* Full Fast-Marching solve.
+ *   model: ORACLE_*; dims[dim];
+ *   Riemann: gridScales[dim], metric (N x K or 1 x K if !metric_is_field), dual flag;
+ *   curvature: gridScale = gridScales[0], xi, eps, cost (N or 1 value if !cost_is_field).
+ *   seeds: nseeds x dim physical points; seedValues: nseeds (or NULL -> 0).
+ *   origin: dim (or NULL -> 0; for curvature models only the two physical components are read).
+ * Outputs (any may be NULL): values N (FP64, +inf where unreached), activeNeighs N (bitsets as
+ * in the reference's ActiveNeighFlagType), flow N x dim (geodesicFlow, already -ReDim'd),
+ * timings[2] = {StencilCPUTime, FMCPUTime} in seconds of process CPU time (clock()).
+ * Returns 0 or -1. */
+int oracle_solve(int model, const int *dims, const double *gridScales, const double *origin,
+                 const double *metric, int metric_is_field, int dual,
+                 double xi, double eps, const double *cost, int cost_is_field,
+                 int nseeds, const double *seeds, const double *seedValues,
+                 double *values, uint64_t *activeNeighs, double *flow, double *timings);
+
+/* Same as oracle_solve, then BackwardVariation for one group of targets.
+ *   targets: ntargets x dim physical points; tweights: ntargets.
+ *   costSensitivity: N doubles (zero outside the dependency cone).
+ *   seedSens: receives up to nseeds_max (index linear, weight) pairs; *nseedSens = count. */
+int oracle_solve_backward(int model, const int *dims, const double *gridScales, const double *origin,
+                          const double *metric, int metric_is_field, int dual,
+                          double xi, double eps, const double *cost, int cost_is_field,
+                          int nseeds, const double *seeds, const double *seedValues,
+                          int ntargets, const double *targets, const double *tweights,
+                          double *values, double *costSensitivity,
+                          int64_t *seedSensIndex, double *seedSensWeight, int nseeds_max,
+                          int *nseedSens);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
