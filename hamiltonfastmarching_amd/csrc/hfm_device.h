@@ -1,0 +1,166 @@
+// hfm_device.h -- data layout and per-point numerics shared by the HIP kernels.
+//
+// Layout in HBM.  A grid of dims (n0, n1[, n2]) is cut into tiles of 64 points, one wavefront
+// per tile: 8x8 in 2-D, 4x4x4 in 3-D.  Every per-point array is stored tile-major
+// ("tiled index" p = tile * 64 + local), so one wave reads a tile's 64 values of a field with a
+// single coalesced 256 B (FP32) or 512 B (FP64) access.  Points of partial edge tiles that fall
+// outside the grid are "padding": fixed at +inf and never updated.
+//
+// Stencils.  Riemann models keep one stencil per point (nDiff symmetric differences: offsets
+// packed as int8x3 in a uint32, weights in the update precision).  The curvature models
+// (ReedsSheppForward2, Elastica2) share one stencil per theta index (a small table) and keep
+// a per-point speed (1/cost); the weight of a difference is baseWeight * speed^2
+// (Headers/Base/EulerianStencil.h:84-97).
+#ifndef HFM_AMD_DEVICE_H
+#define HFM_AMD_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hfm {
+
+enum ModelId { MODEL_RIEMANN2 = 0, MODEL_RIEMANN3 = 1, MODEL_RSF2 = 2, MODEL_ELASTICA2 = 3 };
+
+constexpr int TILE_POINTS = 64;
+constexpr int MAX_DIFF = 27;  // Elastica2<5>: 27 forward differences
+
+// Grid geometry, passed by value to every kernel.
+struct Geom {
+    int dim;
+    int n[3];      // grid extents (n[2] = 1 in 2-D)
+    int sh[3];     // log2 of the tile extent per axis: (3,3,0) in 2-D, (2,2,2) in 3-D
+    int nt[3];     // tiles per axis
+    int ntiles;
+    int periodic;  // bit k set: axis k is periodic (theta axis of R^2 x S^1)
+};
+
+__host__ __device__ inline int tile_extent(const Geom &g, int k) { return 1 << g.sh[k]; }
+
+// Coordinates of the point at tiled index (tile, local).
+__host__ __device__ inline void tiled_coords(const Geom &g, int tile, int local, int *c) {
+    const int t2 = tile % g.nt[2];
+    const int t01 = tile / g.nt[2];
+    const int t1 = t01 % g.nt[1];
+    const int t0 = t01 / g.nt[1];
+    const int l2 = local & ((1 << g.sh[2]) - 1);
+    const int l1 = (local >> g.sh[2]) & ((1 << g.sh[1]) - 1);
+    const int l0 = local >> (g.sh[1] + g.sh[2]);
+    c[0] = (t0 << g.sh[0]) + l0;
+    c[1] = (t1 << g.sh[1]) + l1;
+    c[2] = (t2 << g.sh[2]) + l2;
+}
+
+// Tile id and local index of in-grid coordinates.
+__host__ __device__ inline int tile_of(const Geom &g, const int *c) {
+    return ((c[0] >> g.sh[0]) * g.nt[1] + (c[1] >> g.sh[1])) * g.nt[2] + (c[2] >> g.sh[2]);
+}
+__host__ __device__ inline int local_of(const Geom &g, const int *c) {
+    const int l0 = c[0] & ((1 << g.sh[0]) - 1);
+    const int l1 = c[1] & ((1 << g.sh[1]) - 1);
+    const int l2 = c[2] & ((1 << g.sh[2]) - 1);
+    return (((l0 << g.sh[1]) | l1) << g.sh[2]) | l2;
+}
+__host__ __device__ inline bool in_grid(const Geom &g, const int *c) {
+    return c[0] < g.n[0] && c[1] < g.n[1] && c[2] < g.n[2];
+}
+
+// Shift coordinates by s*offset with the reference's boundary rules: closed axes invalidate,
+// periodic axes wrap (Headers/Base/Implementation/PeriodicGrid.hxx:120-140).
+__host__ __device__ inline bool shift_coords(const Geom &g, const int *c, int o0, int o1, int o2, int *r) {
+    r[0] = c[0] + o0;
+    r[1] = c[1] + o1;
+    r[2] = c[2] + o2;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (g.periodic >> k & 1) {
+            int v = r[k] % g.n[k];
+            r[k] = v < 0 ? v + g.n[k] : v;
+        } else if (r[k] < 0 || r[k] >= g.n[k]) {
+            return false;
+        }
+    }
+    return true;
+}
+
+__host__ __device__ inline uint32_t pack_offset(int o0, int o1, int o2) {
+    return (uint32_t)(uint8_t)(int8_t)o0 | ((uint32_t)(uint8_t)(int8_t)o1 << 8) |
+           ((uint32_t)(uint8_t)(int8_t)o2 << 16);
+}
+__host__ __device__ inline void unpack_offset(uint32_t p, int &o0, int &o1, int &o2) {
+    o0 = (int)(int8_t)(p & 0xff);
+    o1 = (int)(int8_t)((p >> 8) & 0xff);
+    o2 = (int)(int8_t)((p >> 16) & 0xff);
+}
+
+// Static description of a model's stencil shape.
+template <int ModelT> struct ModelShape;
+template <> struct ModelShape<MODEL_RIEMANN2> {
+    static constexpr int DIM = 2, NFWD = 0, NSYM = 3, SHARED = 0;
+};
+template <> struct ModelShape<MODEL_RIEMANN3> {
+    static constexpr int DIM = 3, NFWD = 0, NSYM = 6, SHARED = 0;
+};
+template <> struct ModelShape<MODEL_RSF2> {
+    static constexpr int DIM = 3, NFWD = 3, NSYM = 1, SHARED = 1;
+};
+template <> struct ModelShape<MODEL_ELASTICA2> {
+    static constexpr int DIM = 3, NFWD = 27, NSYM = 0, SHARED = 1;
+};
+
+// Device-side stencil data.
+template <typename T> struct StencilData {
+    // per-point (Riemann): offs[k * np + p], w[k * np + p]
+    const uint32_t *__restrict__ offs;
+    const T *__restrict__ w;
+    // shared per theta (curvature): tab_offs[theta * ndiff + k], tab_w[theta * ndiff + k]
+    const uint32_t *__restrict__ tab_offs;
+    const double *__restrict__ tab_w;
+    const T *__restrict__ speed;  // per point, tiled
+    long long np;                 // number of tiled points (ntiles * 64)
+};
+
+template <typename T> __device__ inline T dev_inf();
+template <> __device__ inline float dev_inf<float>() { return __builtin_huge_valf(); }
+template <> __device__ inline double dev_inf<double>() { return __builtin_huge_val(); }
+
+// Local Hopf-Lax solve of sum_k w_k max(0, u - v_k)^2 = 1 over NC candidates.
+// Candidates are taken in increasing v (ties: lower slot first) and accumulated while
+// v < current root, in the reference's arithmetic (Headers/Base/Implementation/
+// EulerianStencil.hxx:30-72): with m the smallest accepted value, a += w, b += w (v-m),
+// c += (w (v-m)) (v-m) starting from c = -1, root = m + (b + sqrt(max(0, b^2 - a c))) / a.
+// Returns the root (+inf if no finite candidate) and the bitmask of accumulated slots.
+template <int NC, typename T>
+__device__ inline T local_solve(const T (&v)[NC], const T (&w)[NC], uint32_t &used) {
+    const T inf = dev_inf<T>();
+    T r = inf, m = 0, a = 0, b = 0, c = -1;
+    uint32_t sel = 0;
+    for (int step = 0; step < NC; ++step) {
+        T bv = inf, bw = 0;
+        int bk = -1;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            const bool ok = !((sel >> k) & 1u) && v[k] < bv;
+            bv = ok ? v[k] : bv;
+            bw = ok ? w[k] : bw;
+            bk = ok ? k : bk;
+        }
+        if (bk < 0 || !(bv < r)) break;
+        if (step == 0) m = bv;
+        const T vr = bv - m;
+        const T wv = bw * vr;
+        const T wv2 = (bw * vr) * vr;
+        a += bw;
+        b += wv;
+        c += wv2;
+        const T d2 = b * b - a * c;
+        const T d = sqrt(d2 > T(0) ? d2 : T(0));
+        r = m + (b + d) / a;
+        sel |= 1u << bk;
+    }
+    used = sel;
+    return r;
+}
+
+}  // namespace hfm
+
+#endif

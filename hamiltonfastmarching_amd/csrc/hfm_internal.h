@@ -1,0 +1,102 @@
+// hfm_internal.h -- host-side internals shared by the plan, adjoint and IO translation units.
+#ifndef HFM_AMD_INTERNAL_H
+#define HFM_AMD_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/hfm_amd.h"
+#include "hfm_device.h"
+
+namespace hfm {
+
+void set_error(const std::string &m);
+
+struct HipFail {
+    std::string msg;
+};
+
+#define HIPCHECK(expr)                                                                     \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            throw hfm::HipFail{std::string(#expr) + " failed: " + hipGetErrorString(e_)};        \
+    } while (0)
+
+template <typename T> T *dalloc(size_t n) {
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    HIPCHECK(hipMalloc(&p, n * sizeof(T)));
+    return static_cast<T *>(p);
+}
+
+inline void dfree(void *p) {
+    if (p) (void)hipFree(p);
+}
+
+
+}  // namespace hfm
+
+struct hfm_plan {
+    using Geom = hfm::Geom;
+    template <typename T> using StencilData = hfm::StencilData<T>;
+    int model = 0;
+    int precision = 32;
+    Geom g{};
+    long long np = 0;  // tiled points
+    long long N = 0;   // grid points
+    double h[3] = {1, 1, 1};
+    double origin[3] = {0, 0, 0};
+    double dependScale = 1;
+    double xi = 1, eps = 0.1;
+    int ndiff = 0;
+    bool shared = false;
+
+    // device buffers
+    void *u = nullptr;
+    uint32_t *offs = nullptr;
+    void *w = nullptr;
+    void *speed = nullptr;
+    uint32_t *tab_offs = nullptr;
+    double *tab_w = nullptr;
+    unsigned long long *fixedmask = nullptr;
+    int *dep_cnt = nullptr, *dep_start = nullptr, *dep_fill = nullptr, *deps = nullptr;
+    long long deps_cap = 0, ndeps = 0;
+    int *lists[2] = {nullptr, nullptr};
+    int *flags[2] = {nullptr, nullptr};
+    int *counts = nullptr;  // 3 rotating counters
+    int *status = nullptr;  // stencil status
+    unsigned long long *stats = nullptr;
+    long long *seed_p = nullptr;
+    double *seed_v = nullptr;
+    int seed_cap = 0;
+    // pinned host mirrors
+    int *h_count = nullptr;
+    unsigned long long *h_stats = nullptr;
+
+    bool stencil_ready = false;
+    bool solved = false;
+    bool timing = false;
+    int max_stencil_width = 0;
+    std::vector<long long> last_seeds;  // tiled indices of the last solve's seeds
+    hfm_plan_stats st{};
+
+    template <typename T> StencilData<T> sd() const {
+        StencilData<T> s;
+        s.offs = offs;
+        s.w = static_cast<const T *>(w);
+        s.tab_offs = tab_offs;
+        s.tab_w = tab_w;
+        s.speed = static_cast<const T *>(speed);
+        s.np = np;
+        return s;
+    }
+};
+
+int hfm_plan_adjoint_impl(hfm_plan *p, int ntargets, const double *targets, const double *weights,
+                          double *d_sensitivity, int64_t *seed_index, double *seed_weight, int max_seeds,
+                          int *nseed_out, void *stream);
+
+#endif

@@ -1,0 +1,611 @@
+// hfm_io.cpp -- the key/value IO object and the hfm_run() driver (C ABI hfm_io_*, hfm_run).
+//
+// hfm_run reproduces the phases and keys of HFMInterface<Traits>::Run
+// (Headers/Base/Implementation/HFMInterface.hxx:296-579) for the four models of this library,
+// on top of the device plan (hfm_plan_*):
+//   SetupIO       verbosity, arrayOrdering                                  (:311-341)
+//   SetupStencil  dims, gridScale(s), origin, metric | dualMetric, cost | speed, xi, eps
+//                 (Riemannian.h:46-61, Curvature2.h:104-108,219-224, HFM_StencilShare.hxx:61-71)
+//   SetupSolver   MaxStencilWidth, seeds / seedValues                      (:354-410)
+//   RunSolver     FMCPUTime (wall-clock seconds of the GPU solve here)     (:422-450)
+//   ExportData    values, activeNeighs, geodesicFlow                        (:526-579)
+//   Finally       inspectSensitivity -> costSensitivity_<i>, seedSensitivity_<i>
+//                 (Headers/ExtraAlgorithms/FirstVariation.h:60-102)
+// Arrays are FP64, stored row-major with element components last; arrayOrdering permutes the
+// grid axes exactly like IO_::GetArray / SetArray (JMM_CPPLibs/JMM_CPPLibs/Output/IO.hxx:68-110,
+// 153-240): n = P(s) with P = identity, T (swap axes 0,1), R (reverse) or T o R.
+// Unused / defaulted keys are reported like BaseIO::UsageReport (BaseIO.hxx:64-104).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <set>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/hfm_amd.h"
+
+namespace {
+
+struct IOError {
+    std::string msg;
+};
+
+struct Field {
+    bool is_string = false;
+    std::string str;
+    std::vector<int64_t> dims;  // empty: scalar
+    std::vector<double> data;
+    bool computed = false;
+};
+
+enum Ordering { ROW_MAJOR = 0, YXZ_ROW_MAJOR = 1, COLUMN_MAJOR = 2, YXZ_COLUMN_MAJOR = 3 };
+
+}  // namespace
+
+struct hfm_io {
+    std::map<std::string, Field> fields;
+    std::set<std::string> used, defaulted, visitedUnset;
+    std::string err;
+    std::vector<std::string> computed_order;
+    int verbosity = 1;
+    Ordering ordering = ROW_MAJOR;
+    std::string log;
+
+    bool has(const std::string &k) const { return fields.count(k) > 0; }
+    const Field &raw(const std::string &k) {
+        auto it = fields.find(k);
+        if (it == fields.end()) {
+            visitedUnset.insert(k);
+            throw IOError{"Field " + k + " not found"};
+        }
+        used.insert(k);
+        return it->second;
+    }
+    double scalar(const std::string &k) {
+        const Field &f = raw(k);
+        if (f.is_string) throw IOError{"IO error : key " + k + " is a string, not a numerical field."};
+        if (f.data.size() != 1) throw IOError{"IO error : field " + k + " is not a scalar"};
+        return f.data[0];
+    }
+    double scalar(const std::string &k, double def) {
+        if (!has(k)) {
+            defaulted.insert(k);
+            return def;
+        }
+        return scalar(k);
+    }
+    std::string str(const std::string &k) {
+        const Field &f = raw(k);
+        if (!f.is_string) throw IOError{"IO error : key " + k + " is a numerical field, not a string"};
+        return f.str;
+    }
+    std::string str(const std::string &k, const std::string &def) {
+        if (!has(k)) {
+            defaulted.insert(k);
+            return def;
+        }
+        return str(k);
+    }
+    std::vector<double> vec(const std::string &k) {
+        const Field &f = raw(k);
+        if (f.is_string) throw IOError{"IO error : key " + k + " is a string, not a numerical field."};
+        return f.data;
+    }
+    void set_computed(const std::string &k, Field f) {
+        f.computed = true;
+        if (!fields.count(k) || !fields[k].computed) computed_order.push_back(k);
+        fields[k] = std::move(f);
+    }
+    void set_scalar(const std::string &k, double v) {
+        Field f;
+        f.data = {v};
+        set_computed(k, std::move(f));
+    }
+    void set_string(const std::string &k, const std::string &v) {
+        Field f;
+        f.is_string = true;
+        f.str = v;
+        set_computed(k, std::move(f));
+    }
+};
+
+namespace {
+
+// Index map between a stored array (as the user wrote it) and the internal row-major grid
+// array: internal multi-index n = P(s) of stored multi-index s (only the first g axes move).
+std::vector<int64_t> permute_dims(const std::vector<int64_t> &d, Ordering o, bool inverse = false) {
+    std::vector<int64_t> r = d;
+    auto T = [](std::vector<int64_t> &v) {
+        if (v.size() >= 2) std::swap(v[0], v[1]);
+    };
+    auto R = [](std::vector<int64_t> &v) {
+        const size_t n = v.size();
+        for (size_t i = 0; i < n / 2; ++i) std::swap(v[i], v[n - 1 - i]);
+    };
+    switch (o) {
+        case ROW_MAJOR: break;
+        case YXZ_ROW_MAJOR: T(r); break;
+        case COLUMN_MAJOR: R(r); break;
+        case YXZ_COLUMN_MAJOR:  // P = T o R, P^-1 = R o T
+            if (inverse) { T(r); R(r); } else { R(r); T(r); }
+            break;
+    }
+    return r;
+}
+
+// Convert a field whose first axes are grid axes (ncomp trailing component values per point)
+// between the stored layout (as the user wrote it) and the internal row-major grid layout.
+// Internal multi-index n = P(s) for stored multi-index s.  dims_in: grid dims of src;
+// dims_out receives the grid dims of the result.
+std::vector<double> reorder(const std::vector<double> &src, const std::vector<int64_t> &dims_in, int64_t ncomp,
+                            Ordering o, bool to_internal, std::vector<int64_t> &dims_out) {
+    const std::vector<int64_t> stored = to_internal ? dims_in : permute_dims(dims_in, o, true);
+    const std::vector<int64_t> internal = to_internal ? permute_dims(dims_in, o) : dims_in;
+    dims_out = to_internal ? internal : stored;
+    if (o == ROW_MAJOR || stored.size() < 2) return src;
+    const size_t g = stored.size();
+    int64_t total = 1;
+    for (auto v : stored) total *= v;
+    std::vector<double> dst(src.size());
+    std::vector<int64_t> s(g);
+    for (int64_t si = 0; si < total; ++si) {
+        int64_t r = si;
+        for (size_t k = g; k-- > 0;) {
+            s[k] = r % stored[k];
+            r /= stored[k];
+        }
+        const std::vector<int64_t> n = permute_dims(s, o);
+        int64_t ni = 0;
+        for (size_t k = 0; k < g; ++k) ni = ni * internal[k] + n[k];
+        const int64_t a = to_internal ? si : ni;
+        const int64_t b = to_internal ? ni : si;
+        for (int64_t c = 0; c < ncomp; ++c) dst[b * ncomp + c] = src[a * ncomp + c];
+    }
+    return dst;
+}
+
+Ordering parse_ordering(const std::string &s) {
+    if (s == "RowMajor") return ROW_MAJOR;
+    if (s == "YXZ_RowMajor") return YXZ_ROW_MAJOR;
+    if (s == "ColumnMajor") return COLUMN_MAJOR;
+    if (s == "YXZ_ColumnMajor") return YXZ_COLUMN_MAJOR;
+    throw IOError{"Error : unrecognized array ordering"};
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    explicit DevBuf(size_t bytes) {
+        if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) throw IOError{"hipMalloc failed"};
+    }
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    double *d() const { return static_cast<double *>(p); }
+};
+
+struct PlanGuard {
+    hfm_plan *p = nullptr;
+    ~PlanGuard() { hfm_plan_destroy(p); }
+};
+
+void check(int rc) {
+    if (rc) throw IOError{hfm_last_error()};
+}
+
+// A grid field of element size ncomp: constant (0-D element), full grid, or for the curvature
+// models a field over (x, y) only or over theta only (HFMInterface.hxx:23-32).  Returns the
+// internal row-major full-grid array (or a single element if constant).
+std::vector<double> grid_field(hfm_io *io, const std::string &key, const std::vector<int64_t> &dims, int64_t ncomp,
+                               bool bundle, bool &is_field) {
+    const Field &f = io->raw(key);
+    if (f.is_string) throw IOError{"IO error : key " + key + " is a string, not a numerical field."};
+    std::vector<int64_t> gd = f.dims;
+    if (ncomp > 1) {
+        if (gd.empty() || gd.back() != ncomp)
+            throw IOError{"BaseIO input error first dimension " + std::to_string(gd.empty() ? 0 : gd.back()) +
+                          " of field " + key + " does not match expected value " + std::to_string(ncomp) + "."};
+        gd.pop_back();
+    }
+    if (gd.empty()) {
+        is_field = false;
+        return f.data;
+    }
+    std::vector<int64_t> internal;
+    std::vector<double> v = reorder(f.data, gd, ncomp, io->ordering, true, internal);
+    is_field = true;
+    const size_t D = dims.size();
+    if (internal.size() == D) {
+        if (internal != dims) throw IOError{"Error : Field " + key + " has inconsistent dimensions."};
+        return v;
+    }
+    if (bundle && (internal.size() == D - 1 || internal.size() == 1)) {
+        const bool indep = internal.size() == D - 1;  // (x, y) field broadcast over theta
+        if (indep && (internal[0] != dims[0] || internal[1] != dims[1]))
+            throw IOError{"Error : Field " + key + " has inconsistent dimensions."};
+        if (!indep && internal[0] != dims[2]) throw IOError{"Error : Field " + key + " has inconsistent dimensions."};
+        std::vector<double> full((size_t)(dims[0] * dims[1] * dims[2]) * ncomp);
+        for (int64_t i = 0; i < dims[0]; ++i)
+            for (int64_t j = 0; j < dims[1]; ++j)
+                for (int64_t t = 0; t < dims[2]; ++t) {
+                    const int64_t dst = (i * dims[1] + j) * dims[2] + t;
+                    const int64_t src = indep ? i * dims[1] + j : t;
+                    for (int64_t c = 0; c < ncomp; ++c) full[dst * ncomp + c] = v[src * ncomp + c];
+                }
+        return full;
+    }
+    throw IOError{"Field " + key + " has incorrect depth.\n"};
+}
+
+void set_grid_array(hfm_io *io, const std::string &key, const std::vector<double> &internal,
+                    const std::vector<int64_t> &dims, int64_t ncomp) {
+    Field f;
+    std::vector<int64_t> stored;
+    f.data = reorder(internal, dims, ncomp, io->ordering, false, stored);
+    f.dims = stored;
+    if (ncomp > 1) f.dims.push_back(ncomp);
+    io->set_computed(key, std::move(f));
+}
+
+double wall() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void run_impl(hfm_io *io) {
+    io->computed_order.clear();
+    io->used.clear();
+    io->defaulted.clear();
+    io->visitedUnset.clear();
+    // Run_SetupIO
+    io->verbosity = (int)io->scalar("verbosity", 1);
+    io->ordering = parse_ordering(io->str("arrayOrdering", "RowMajor"));
+    // Dispatch on the model key (DispatchAndRun.h:224-353)
+    const std::string model = io->str("model");
+    const bool riemann = (model == "Riemann2" || model == "Riemann3");
+    const bool bundle = (model == "ReedsSheppForward2" || model == "Elastica2" || model == "Elastica2_5");
+    if (!riemann && !bundle) throw IOError{"Unrecognized model " + model};
+    const int dim = model == "Riemann2" ? 2 : 3;
+    // Run_SetupStencil
+    const std::vector<double> dv = io->vec("dims");
+    if ((int)dv.size() != dim) throw IOError{"IO error : field dims has the wrong size"};
+    std::vector<int64_t> dims(dim);
+    for (int i = 0; i < dim; ++i) dims[i] = (int64_t)dv[i];
+    double h[3] = {0, 0, 0};
+    if (riemann && io->has("gridScales")) {
+        const std::vector<double> g = io->vec("gridScales");
+        if ((int)g.size() != dim) throw IOError{"IO error : field gridScales has the wrong size"};
+        for (int i = 0; i < dim; ++i) h[i] = g[i];
+    } else {
+        const double gs = io->scalar("gridScale");
+        for (int i = 0; i < dim; ++i) h[i] = gs;
+    }
+    std::vector<double> origin;
+    if (io->has("origin")) {
+        origin = io->vec("origin");
+        const size_t want = riemann ? dim : 2;
+        if (origin.size() != want) throw IOError{"IO error : field origin has the wrong size"};
+    } else {
+        io->defaulted.insert("origin");
+    }
+    const int precision = (int)io->scalar("precision", 64.);
+    const double xi = bundle ? io->scalar("xi", 1.) : 1.;
+    const double eps = bundle ? io->scalar("eps", 0.1) : 0.1;
+    PlanGuard pg;
+    check(hfm_plan_create(&pg.p, model.c_str(), dims.data(), h, origin.empty() ? nullptr : origin.data(), precision,
+                          xi, eps));
+    hfm_plan *plan = pg.p;
+    int64_t N = 1;
+    for (auto d : dims) N *= d;
+
+    const double t0 = wall();
+    if (riemann) {
+        const bool hasDual = io->has("dualMetric");
+        if (hasDual && io->has("metric")) throw IOError{"Error: both primal and dual metric provided"};
+        const std::string mkey = hasDual ? "dualMetric" : "metric";
+        bool is_field = false;
+        const int64_t K = dim * (dim + 1) / 2;
+        std::vector<double> m = grid_field(io, mkey, dims, K, false, is_field);
+        DevBuf dm(m.size() * sizeof(double));
+        if (hipMemcpy(dm.p, m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            throw IOError{"hipMemcpy failed"};
+        check(hfm_plan_set_metric(plan, dm.d(), is_field ? 1 : 0, hasDual ? 1 : 0, nullptr));
+    } else {
+        bool is_field = false;
+        std::vector<double> c;
+        if (io->has("speed")) {
+            c = grid_field(io, "speed", dims, 1, true, is_field);
+            for (auto &v : c) v = 1. / v;
+        } else {
+            c = grid_field(io, "cost", dims, 1, true, is_field);
+        }
+        DevBuf dc(c.size() * sizeof(double));
+        if (hipMemcpy(dc.p, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            throw IOError{"hipMemcpy failed"};
+        check(hfm_plan_set_cost(plan, dc.d(), is_field ? 1 : 0, nullptr));
+    }
+    const double t1 = wall();
+    io->set_scalar("StencilCPUTime", t1 - t0);
+    hfm_plan_stats st;
+    check(hfm_plan_get_stats(plan, &st));
+    // Run_SetupSolver
+    io->set_scalar("MaxStencilWidth", (double)st.max_stencil_width);
+    if (io->has("order") && io->scalar("order") != 1.)
+        throw IOError{"Error : only order=1 is supported by this library"};
+    if (!io->has("seeds")) {
+        io->log += "No seeds to run fast marching solver, and missing values or activeNeighs to compute geodesics.\n";
+        return;
+    }
+    std::vector<double> seeds = io->vec("seeds");
+    if (seeds.size() % dim) throw IOError{"IO error : seeds has the wrong size"};
+    const int nseeds = (int)(seeds.size() / dim);
+    std::vector<double> sv;
+    if (io->has("seedValues")) {
+        sv = io->vec("seedValues");
+        if ((int)sv.size() != nseeds) throw IOError{"Error : Inconsistent size of seedValues."};
+    }
+    // Run_RunSolver
+    const double t2 = wall();
+    check(hfm_plan_solve(plan, nseeds, seeds.data(), sv.empty() ? nullptr : sv.data(), nullptr));
+    const double t3 = wall();
+    check(hfm_plan_get_stats(plan, &st));
+    io->set_scalar("FMCPUTime", t3 - t2);
+    io->set_scalar("FMGPUTime", st.solve_ms * 1e-3);
+    io->set_scalar("nIterations", (double)st.iterations);
+    io->set_scalar("nTileVisits", (double)st.tile_visits);
+    if (io->verbosity >= 1) {
+        std::ostringstream os;
+        os << "Fast marching solver completed in " << (t3 - t2) << " s.\n";
+        io->log += os.str();
+    }
+    // Run_ExportData
+    const bool exportValues = io->scalar("exportValues", 0.) != 0.;
+    const bool exportActive = io->scalar("exportActiveNeighs", 0.) != 0.;
+    const bool exportFlow = io->scalar("exportGeodesicFlow", 0.) != 0.;
+    if (exportValues) {
+        DevBuf dvv(N * sizeof(double));
+        check(hfm_plan_values(plan, dvv.d(), nullptr));
+        std::vector<double> v(N);
+        if (hipMemcpy(v.data(), dvv.p, N * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+            throw IOError{"hipMemcpy failed"};
+        set_grid_array(io, "values", v, dims, 1);
+    }
+    if (exportActive || exportFlow) {
+        DevBuf dfl(exportFlow ? N * dim * sizeof(double) : 8);
+        DevBuf dac(exportActive ? N * sizeof(double) : 8);
+        check(hfm_plan_flow(plan, exportFlow ? dfl.d() : nullptr, exportActive ? dac.d() : nullptr, nullptr));
+        if (exportActive) {
+            std::vector<double> a(N);
+            if (hipMemcpy(a.data(), dac.p, N * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+                throw IOError{"hipMemcpy failed"};
+            set_grid_array(io, "activeNeighs", a, dims, 1);
+        }
+        if (exportFlow) {
+            std::vector<double> f(N * dim);
+            if (hipMemcpy(f.data(), dfl.p, N * dim * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+                throw IOError{"hipMemcpy failed"};
+            set_grid_array(io, "geodesicFlow", f, dims, dim);
+        }
+    }
+    // FirstVariation::Finally (backward part)
+    if (io->has("inspectSensitivity")) {
+        const std::vector<double> pts = io->vec("inspectSensitivity");
+        if (pts.size() % dim) throw IOError{"IO error : inspectSensitivity has the wrong size"};
+        const size_t npts = pts.size() / dim;
+        std::vector<double> lengths;
+        if (io->has("inspectSensitivityLengths")) {
+            lengths = io->vec("inspectSensitivityLengths");
+            double tot = 0;
+            for (double l : lengths) tot += l;
+            if (tot != (double)npts) throw IOError{"Error : inconsistent inspectSensitivityLengths"};
+        } else {
+            lengths.assign(npts, 1.);
+        }
+        std::vector<double> weights;
+        if (io->has("inspectSensitivityWeights")) {
+            weights = io->vec("inspectSensitivityWeights");
+            if (weights.size() != npts) throw IOError{"Error: Inconsistent inspectSensitivityWeights size"};
+        } else {
+            weights.assign(npts, 1.);
+        }
+        DevBuf dsens(N * sizeof(double));
+        size_t pos = 0;
+        for (size_t i = 0; i < lengths.size(); ++i) {
+            const int cnt = (int)lengths[i];
+            const int maxs = 4096;
+            std::vector<int64_t> si(maxs);
+            std::vector<double> sw(maxs);
+            int ns = 0;
+            check(hfm_plan_adjoint(plan, cnt, pts.data() + pos * dim, weights.data() + pos, dsens.d(), si.data(),
+                                   sw.data(), maxs, &ns, nullptr));
+            pos += cnt;
+            std::vector<double> s(N);
+            if (hipMemcpy(s.data(), dsens.p, N * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+                throw IOError{"hipMemcpy failed"};
+            set_grid_array(io, "costSensitivity_" + std::to_string(i), s, dims, 1);
+            // seedSensitivity: (ReDim(PointFromIndex(index)), weight) pairs (FirstVariation.h:90-98)
+            Field f;
+            const int n = std::min(ns, maxs);
+            f.dims = {(int64_t)n, (int64_t)(dim + 1)};
+            for (int k = 0; k < n; ++k) {
+                int64_t r = si[k];
+                int64_t idx[3] = {0, 0, 0};
+                for (int q = dim - 1; q >= 0; --q) {
+                    idx[q] = r % dims[q];
+                    r /= dims[q];
+                }
+                for (int q = 0; q < dim; ++q) {
+                    const double pt = idx[q] + 0.5;
+                    double o;
+                    if (bundle && q == 2) o = pt * (2 * 3.141592653589793238462643383279502884L / dims[2]) +
+                                              (-(2 * 3.141592653589793238462643383279502884L / dims[2]) / 2.);
+                    else o = pt * h[q] + (origin.empty() ? 0. : origin[q]);
+                    f.data.push_back(o);
+                }
+                f.data.push_back(sw[k]);
+            }
+            io->set_computed("seedSensitivity_" + std::to_string(i), std::move(f));
+        }
+    }
+}
+
+void usage_report(hfm_io *io) {
+    std::ostringstream unusedUser, defaulted, visited;
+    for (const auto &kv : io->fields)
+        if (!kv.second.computed && !io->used.count(kv.first)) unusedUser << kv.first << " ";
+    for (const auto &k : io->defaulted) defaulted << k << " ";
+    for (const auto &k : io->visitedUnset) visited << k << " ";
+    if (!unusedUser.str().empty()) io->set_string("unusedFromUser", unusedUser.str());
+    if (!defaulted.str().empty()) io->set_string("defaulted", defaulted.str());
+    if (!visited.str().empty()) io->set_string("visitedUnset", visited.str());
+}
+
+long long copy_out(const std::string &s, char *buf, size_t bufsize) {
+    if (buf && bufsize) {
+        const size_t n = std::min(bufsize - 1, s.size());
+        std::memcpy(buf, s.data(), n);
+        buf[n] = 0;
+    }
+    return (long long)s.size();
+}
+
+}  // namespace
+
+extern "C" {
+
+hfm_io *hfm_io_create(void) { return new hfm_io; }
+void hfm_io_destroy(hfm_io *io) { delete io; }
+
+int hfm_io_set_string(hfm_io *io, const char *key, const char *value) {
+    if (!io || !key || !value) return 1;
+    Field f;
+    f.is_string = true;
+    f.str = value;
+    io->fields[key] = std::move(f);
+    return 0;
+}
+
+int hfm_io_set_scalar(hfm_io *io, const char *key, double value) {
+    if (!io || !key) return 1;
+    Field f;
+    f.data = {value};
+    io->fields[key] = std::move(f);
+    return 0;
+}
+
+int hfm_io_set_array(hfm_io *io, const char *key, const double *data, int ndim, const int64_t *shape) {
+    if (!io || !key || ndim < 0 || (ndim > 0 && !shape)) return 1;
+    Field f;
+    int64_t n = 1;
+    for (int i = 0; i < ndim; ++i) {
+        if (shape[i] < 0) {
+            io->err = "negative dimension";
+            return 1;
+        }
+        f.dims.push_back(shape[i]);
+        n *= shape[i];
+    }
+    if (n > 0 && !data) return 1;
+    f.data.assign(data, data + n);
+    io->fields[key] = std::move(f);
+    return 0;
+}
+
+int hfm_io_has_field(const hfm_io *io, const char *key) { return (io && key && io->has(key)) ? 1 : 0; }
+
+int hfm_io_erase_field(hfm_io *io, const char *key) {
+    if (!io || !key) return 1;
+    io->fields.erase(key);
+    return 0;
+}
+
+int hfm_io_field_kind(const hfm_io *io, const char *key) {
+    if (!io || !key) return 0;
+    auto it = io->fields.find(key);
+    if (it == io->fields.end()) return 0;
+    return it->second.is_string ? 1 : 2;
+}
+
+long long hfm_io_get_string(hfm_io *io, const char *key, char *buf, size_t bufsize) {
+    if (!io || !key) return -1;
+    auto it = io->fields.find(key);
+    if (it == io->fields.end() || !it->second.is_string) {
+        io->err = std::string("Field ") + key + " not found or not a string";
+        return -1;
+    }
+    return copy_out(it->second.str, buf, bufsize);
+}
+
+int hfm_io_get_scalar(hfm_io *io, const char *key, double *value) {
+    if (!io || !key || !value) return 1;
+    auto it = io->fields.find(key);
+    if (it == io->fields.end() || it->second.is_string || it->second.data.size() != 1) {
+        io->err = std::string("Field ") + key + " not found or not a scalar";
+        return 1;
+    }
+    *value = it->second.data[0];
+    return 0;
+}
+
+int hfm_io_get_array_shape(hfm_io *io, const char *key, int *ndim, int64_t *shape, int max_ndim) {
+    if (!io || !key || !ndim) return 1;
+    auto it = io->fields.find(key);
+    if (it == io->fields.end() || it->second.is_string) {
+        io->err = std::string("Field ") + key + " not found or not numerical";
+        return 1;
+    }
+    const auto &d = it->second.dims;
+    *ndim = (int)d.size();
+    if ((int)d.size() > max_ndim) return 1;
+    for (size_t i = 0; i < d.size(); ++i) shape[i] = d[i];
+    return 0;
+}
+
+int hfm_io_get_array(hfm_io *io, const char *key, double *out, size_t count) {
+    if (!io || !key || (!out && count)) return 1;
+    auto it = io->fields.find(key);
+    if (it == io->fields.end() || it->second.is_string) {
+        io->err = std::string("Field ") + key + " not found or not numerical";
+        return 1;
+    }
+    const auto &v = it->second.data;
+    if (count != v.size()) {
+        io->err = "hfm_io_get_array: size mismatch";
+        return 1;
+    }
+    std::memcpy(out, v.data(), count * sizeof(double));
+    return 0;
+}
+
+long long hfm_io_computed_keys(hfm_io *io, char *buf, size_t bufsize) {
+    if (!io) return -1;
+    std::string s;
+    for (const auto &k : io->computed_order) {
+        if (!s.empty()) s += " ";
+        s += k;
+    }
+    return copy_out(s, buf, bufsize);
+}
+
+const char *hfm_io_last_error(const hfm_io *io) { return io ? io->err.c_str() : "null io"; }
+
+int hfm_run(hfm_io *io) {
+    if (!io) return 1;
+    io->err.clear();
+    io->log.clear();
+    try {
+        run_impl(io);
+        usage_report(io);
+    } catch (const IOError &e) {
+        io->err = e.msg;
+        return 1;
+    } catch (const std::exception &e) {
+        io->err = e.what();
+        return 1;
+    }
+    return 0;
+}
+
+}  // extern "C"

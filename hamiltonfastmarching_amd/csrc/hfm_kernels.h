@@ -1,0 +1,61 @@
+// hfm_kernels.h -- host-side launchers of the HIP kernels (see hfm_kernels.hip).
+#ifndef HFM_AMD_KERNELS_H
+#define HFM_AMD_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hfm_device.h"
+
+namespace hfm {
+
+// 1/(h_i h_j) in packed symmetric order (Headers/Specializations/Riemannian.h:50-52).
+struct InvHH {
+    double v[6];
+};
+
+// Per-axis scale applied by ReDim to a vector (h_i, or the theta step on the bundle axis).
+struct FlowScale {
+    double h[3];
+};
+
+template <typename T> struct FimArgs {
+    Geom g;
+    StencilData<T> sd;
+    T *u;
+    const unsigned long long *fixedmask;
+    const int *dep_start;
+    const int *deps;
+    const int *list_in;
+    const int *count_in;
+    int *list_out;
+    int *count_out;
+    int *count_reset;
+    int *flag_in;
+    int *flag_out;
+    unsigned long long *stats;  // [0] tile visits, [1] sweeps (summed over visits)
+    unsigned grid;
+};
+
+template <typename T>
+hipError_t launch_riemann_stencil(const Geom &g, const double *metric, int is_field, int dual, const InvHH &ih,
+                                  uint32_t *offs, T *w, long long np, int *status, hipStream_t s);
+template <typename T>
+hipError_t launch_tile_deps(int model, const Geom &g, const StencilData<T> &sd, int pass, int *cnt, const int *start,
+                            int *fill, int *deps, int *ovf, hipStream_t s);
+hipError_t launch_exclusive_scan(const int *cnt, int *start, int n, hipStream_t s);
+template <typename T>
+hipError_t launch_init(const Geom &g, T *u, unsigned long long *fixedmask, int *f0, int *f1, long long np,
+                       const long long *seed_p, const double *seed_v, int nseeds, hipStream_t s);
+hipError_t launch_seed_activate(const long long *seed_p, int nseeds, const int *dep_start, const int *deps,
+                                int *flag_out, int *list_out, int *count_out, hipStream_t s);
+template <typename T> hipError_t launch_fim(int model, const FimArgs<T> &a, hipStream_t s);
+template <typename T>
+hipError_t launch_export_values(const Geom &g, const T *u, double *out, long long np, hipStream_t s);
+template <typename T>
+hipError_t launch_flow(int model, const Geom &g, const StencilData<T> &sd, const T *u, const unsigned long long *fm,
+                       const FlowScale &fs, double *flow, double *act, long long np, hipStream_t s);
+
+}  // namespace hfm
+
+#endif

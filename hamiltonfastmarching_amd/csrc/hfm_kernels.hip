@@ -1,0 +1,569 @@
+// hfm_kernels.hip -- HIP kernels of the GPU Fast-Marching library (gfx950, wave64).
+//
+//   k_riemann_stencil   per-point FP64 Selling reduction -> int8 offsets + weights (tiled)
+//   k_tile_deps         tile dependency graph: which tiles read a tile (one wave per tile)
+//   k_fim               one round of the block fast-iterative method over the active-tile list
+//   k_export_values     tiled values -> row-major FP64 'values'
+//   k_flow              geodesic flow / active-neighbour bits from the converged values
+//   k_adjoint_*         backward differentiation on the same tiles (reverse causal order)
+//
+// All kernels are launched through the host functions at the bottom of this file, which are the
+// only symbols the plan code (hfm_plan.hip) uses.
+#include "hfm_kernels.h"
+
+#include "hfm_device.h"
+#include "selling.h"
+
+namespace hfm {
+
+// ---------------------------------------------------------------------------------------------
+// Stencil construction (Riemann2 / Riemann3).
+// ---------------------------------------------------------------------------------------------
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_riemann_stencil(Geom g, const double *__restrict__ metric, int metric_is_field,
+                                                         int dual, InvHH ih, uint32_t *__restrict__ offs,
+                                                         T *__restrict__ w, long long np, int *__restrict__ status) {
+    constexpr int K = D * (D + 1) / 2;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    int c[3];
+    tiled_coords(g, (int)(p >> 6), (int)(p & 63), c);
+    if (!in_grid(g, c)) {
+        for (int k = 0; k < K; ++k) {
+            offs[k * np + p] = 0;
+            w[k * np + p] = T(0);
+        }
+        return;
+    }
+    const long long lin = ((long long)c[0] * g.n[1] + c[1]) * g.n[2] + c[2];
+    double mc[K];
+    const double *src = metric + (metric_is_field ? lin * K : 0);
+    for (int k = 0; k < K; ++k) mc[k] = src[k];
+    int8_t o8[K][D];
+    double wd[K];
+    int mx = 0;
+    const int st = riemann_point_stencil<D>(mc, dual != 0, ih.v, o8, wd, &mx);
+    for (int k = 0; k < K; ++k) {
+        offs[k * np + p] = pack_offset(o8[k][0], D > 1 ? o8[k][1] : 0, D > 2 ? o8[k][2] : 0);
+        w[k * np + p] = (T)wd[k];
+    }
+    if (st != SELLING_OK) {
+        // status[0]: error code, status[1]: smallest failing row-major index
+        atomicMax(&status[0], st);
+        atomicMin(&status[1], (int)lin);
+    }
+    // status[2]: max L1 norm of a nonzero-weight offset (MaxStencilWidth, HFMInterface.hxx:357)
+    int l1max = 0;
+    for (int k = 0; k < K; ++k) {
+        if (wd[k] == 0.) continue;
+        int l1 = 0;
+        for (int q = 0; q < D; ++q) l1 += o8[k][q] < 0 ? -o8[k][q] : o8[k][q];
+        l1max = l1 > l1max ? l1 : l1max;
+    }
+    atomicMax(&status[2], l1max);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Per-lane stencil access, shared by the tile kernels.
+// ---------------------------------------------------------------------------------------------
+template <int ModelT, typename T> struct LaneStencil {
+    using S = ModelShape<ModelT>;
+    static constexpr int NC = S::NFWD + S::NSYM;
+    uint32_t off[NC];
+    T w[NC];
+
+    __device__ void load(const Geom &g, const StencilData<T> &sd, long long p, const int *c) {
+        if (!S::SHARED) {
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                off[k] = sd.offs[k * sd.np + p];
+                w[k] = sd.w[k * sd.np + p];
+            }
+        } else {
+            int th = c[2] < g.n[2] ? c[2] : g.n[2] - 1;
+            const T s = sd.speed[p];
+            const T s2 = s * s;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                off[k] = sd.tab_offs[th * NC + k];
+                w[k] = (T)sd.tab_w[th * NC + k] * s2;
+            }
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Tile dependency graph.  For tile T, its sources are the tiles holding x +- e_k for the
+// points x of T (nonzero weights only).  deps(S) = tiles T != S having S as a source: the tiles
+// to re-run when S changes.  Pass 0 counts, pass 1 fills (CSR offsets from an exclusive scan).
+// ---------------------------------------------------------------------------------------------
+constexpr int HASH_SLOTS = 2048;
+
+template <int ModelT, typename T>
+__global__ __launch_bounds__(64) void k_tile_deps(Geom g, StencilData<T> sd, int pass, int *__restrict__ dep_count,
+                                                  const int *__restrict__ dep_start, int *__restrict__ dep_fill,
+                                                  int *__restrict__ deps, int *__restrict__ overflow) {
+    using S = ModelShape<ModelT>;
+    constexpr int NC = S::NFWD + S::NSYM;
+    __shared__ int set[HASH_SLOTS];
+    const int tile = blockIdx.x;
+    const int lane = threadIdx.x;
+    for (int i = lane; i < HASH_SLOTS; i += 64) set[i] = -1;
+    __syncthreads();
+    const long long p = (long long)tile * 64 + lane;
+    int c[3];
+    tiled_coords(g, tile, lane, c);
+    if (in_grid(g, c)) {
+        LaneStencil<ModelT, T> st;
+        st.load(g, sd, p, c);
+        for (int k = 0; k < NC; ++k) {
+            if (st.w[k] == T(0)) continue;
+            int o0, o1, o2;
+            unpack_offset(st.off[k], o0, o1, o2);
+            const int nsides = (k >= S::NFWD) ? 2 : 1;
+            for (int s = 0; s < nsides; ++s) {
+                const int sg = s == 0 ? 1 : -1;
+                int r[3];
+                if (!shift_coords(g, c, sg * o0, sg * o1, sg * o2, r)) continue;
+                const int t = tile_of(g, r);
+                if (t == tile) continue;
+                unsigned h = ((unsigned)t * 2654435761u) & (HASH_SLOTS - 1);
+                for (int probe = 0; probe < HASH_SLOTS; ++probe) {
+                    const int prev = atomicCAS(&set[h], -1, t);
+                    if (prev == -1 || prev == t) break;
+                    h = (h + 1) & (HASH_SLOTS - 1);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = lane; i < HASH_SLOTS; i += 64) {
+        const int src = set[i];
+        if (src < 0) continue;
+        if (pass == 0) {
+            atomicAdd(&dep_count[src], 1);
+        } else {
+            const int pos = atomicAdd(&dep_fill[src], 1);
+            deps[dep_start[src] + pos] = tile;
+        }
+    }
+    (void)overflow;
+}
+
+// Exclusive scan of n counts into start[0..n] (one block; setup-time only).
+__global__ __launch_bounds__(1024) void k_exclusive_scan(const int *__restrict__ cnt, int *__restrict__ start, int n) {
+    __shared__ int part[1024];
+    const int tid = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int b = tid * per, e = min(n, b + per);
+    int s = 0;
+    for (int i = b; i < e; ++i) s += cnt[i];
+    part[tid] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    int run = tid == 0 ? 0 : part[tid - 1];
+    for (int i = b; i < e; ++i) {
+        start[i] = run;
+        run += cnt[i];
+    }
+    if (tid == 1023) start[n] = part[1023];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Solve initialisation.
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_init(Geom g, T *__restrict__ u, unsigned long long *__restrict__ fixedmask,
+                                              int *__restrict__ flags0, int *__restrict__ flags1, long long np) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    u[p] = dev_inf<T>();
+    int c[3];
+    tiled_coords(g, (int)(p >> 6), (int)(p & 63), c);
+    const unsigned long long pad = __ballot(!in_grid(g, c));
+    if ((threadIdx.x & 63) == 0) {
+        const int tile = (int)(p >> 6);
+        fixedmask[tile] = pad;
+        flags0[tile] = 0;
+        flags1[tile] = 0;
+    }
+}
+
+template <typename T>
+__global__ void k_set_seeds(const long long *__restrict__ seed_p, const double *__restrict__ seed_v, int nseeds,
+                            T *__restrict__ u, unsigned long long *__restrict__ fixedmask) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nseeds) return;
+    const long long p = seed_p[i];
+    u[p] = (T)seed_v[i];
+    atomicOr(&fixedmask[p >> 6], 1ull << (p & 63));
+}
+
+// Wave-aggregated push of candidate tile ids (d < 0: none) into the next list.
+__device__ inline void push_tiles(int d, int lane, int *__restrict__ flag_out, int *__restrict__ list_out,
+                                  int *__restrict__ count_out) {
+    const bool want = d >= 0 && atomicExch(&flag_out[d], 1) == 0;
+    const unsigned long long mask = __ballot(want);
+    if (mask == 0) return;
+    int base = 0;
+    if (lane == 0) base = atomicAdd(count_out, __popcll(mask));
+    base = __shfl(base, 0);
+    if (want) {
+        const unsigned long long below = lane == 0 ? 0ull : (mask & (~0ull >> (64 - lane)));
+        list_out[base + __popcll(below)] = d;
+    }
+}
+
+// Seed tiles and their dependents form the first active list.
+__global__ __launch_bounds__(64) void k_seed_activate(const long long *__restrict__ seed_p, int nseeds,
+                                                      const int *__restrict__ dep_start, const int *__restrict__ deps,
+                                                      int *__restrict__ flag_out, int *__restrict__ list_out,
+                                                      int *__restrict__ count_out) {
+    const int lane = threadIdx.x;
+    for (int i = 0; i < nseeds; ++i) {
+        const int tile = (int)(seed_p[i] >> 6);
+        push_tiles(lane == 0 ? tile : -1, lane, flag_out, list_out, count_out);
+        const int b = dep_start[tile], e = dep_start[tile + 1];
+        for (int j0 = b; j0 < e; j0 += 64) {
+            const int j = j0 + lane;
+            push_tiles(j < e ? deps[j] : -1, lane, flag_out, list_out, count_out);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// One round of the block fast-iterative method.
+//
+// Each wave takes a tile from the active list, stages the tile's values in LDS, caches the
+// values of neighbours outside the tile in registers, and runs Jacobi sweeps of the local solve
+// (values only decrease) until no lane changes (wave ballot) or MAXSWEEP sweeps.  If the tile
+// changed, its dependent tiles are appended to the next list; if it did not converge it
+// re-appends itself.  Values of other tiles may be read while they are being lowered by another
+// wave: any value read is an upper bound of the solution and the scheme is monotone, and the
+// writer re-activates its dependents, so the fixed point is unaffected.
+// ---------------------------------------------------------------------------------------------
+template <int ModelT, typename T, int MAXSWEEP>
+__global__ __launch_bounds__(256) void k_fim(Geom g, StencilData<T> sd, T *__restrict__ u,
+                                             const unsigned long long *__restrict__ fixedmask,
+                                             const int *__restrict__ dep_start, const int *__restrict__ deps,
+                                             const int *__restrict__ list_in, const int *__restrict__ count_in,
+                                             int *__restrict__ list_out, int *__restrict__ count_out,
+                                             int *__restrict__ count_reset, int *__restrict__ flag_in,
+                                             int *__restrict__ flag_out, unsigned long long *__restrict__ stats) {
+    using S = ModelShape<ModelT>;
+    constexpr int NF = S::NFWD, NS = S::NSYM, NC = NF + NS;
+    __shared__ T su[4][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const T inf = dev_inf<T>();
+    const int n = *count_in;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *count_reset = 0;
+    unsigned long long my_sweeps = 0, my_visits = 0;
+    for (int i = blockIdx.x * 4 + wv; i < n; i += gridDim.x * 4) {
+        const int tile = list_in[i];
+        if (lane == 0) flag_in[tile] = 0;
+        const long long p = (long long)tile * 64 + lane;
+        int c[3];
+        tiled_coords(g, tile, lane, c);
+        const bool fixed = (fixedmask[tile] >> lane) & 1ull;
+        const T u0 = u[p];
+        LaneStencil<ModelT, T> st;
+        st.load(g, sd, p, c);
+        // Neighbour slots: forward k -> one side, symmetric k -> two sides.
+        int la[NC], lb[NC];
+        T oa[NC], ob[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            int o0, o1, o2;
+            unpack_offset(st.off[k], o0, o1, o2);
+            la[k] = -1;
+            lb[k] = -1;
+            oa[k] = inf;
+            ob[k] = inf;
+            if (fixed || st.w[k] == T(0)) continue;
+            int r[3];
+            if (shift_coords(g, c, o0, o1, o2, r)) {
+                const int t = tile_of(g, r), l = local_of(g, r);
+                if (t == tile) la[k] = l;
+                else oa[k] = u[(long long)t * 64 + l];
+            }
+            if (k >= NF) {
+                if (shift_coords(g, c, -o0, -o1, -o2, r)) {
+                    const int t = tile_of(g, r), l = local_of(g, r);
+                    if (t == tile) lb[k] = l;
+                    else ob[k] = u[(long long)t * 64 + l];
+                }
+            }
+        }
+        T ucur = u0;
+        su[wv][lane] = ucur;
+        bool converged = false;
+        int sweeps = 0;
+        for (int it = 0; it < MAXSWEEP; ++it) {
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            T v[NC];
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                const T va = la[k] >= 0 ? su[wv][la[k]] : oa[k];
+                const T vb = lb[k] >= 0 ? su[wv][lb[k]] : ob[k];
+                v[k] = va < vb ? va : vb;
+            }
+            uint32_t used;
+            const T r = local_solve<NC, T>(v, st.w, used);
+            const bool ch = !fixed && r < ucur;
+            ucur = ch ? r : ucur;
+            ++sweeps;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            su[wv][lane] = ucur;
+            if (__ballot(ch) == 0ull) {
+                converged = true;
+                break;
+            }
+        }
+        const bool lowered = ucur < u0;
+        if (lowered) u[p] = ucur;
+        const bool tile_changed = __ballot(lowered) != 0ull;
+        if (tile_changed) {
+            const int b = dep_start[tile], e = dep_start[tile + 1];
+            for (int j0 = b; j0 < e; j0 += 64) {
+                const int j = j0 + lane;
+                push_tiles(j < e ? deps[j] : -1, lane, flag_out, list_out, count_out);
+            }
+        }
+        if (!converged) push_tiles(lane == 0 ? tile : -1, lane, flag_out, list_out, count_out);
+        my_sweeps += sweeps;
+        my_visits += 1;
+    }
+    if (lane == 0 && my_visits) {
+        atomicAdd(&stats[0], my_visits);
+        atomicAdd(&stats[1], my_sweeps);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Exports.
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_export_values(Geom g, const T *__restrict__ u, double *__restrict__ out,
+                                                       long long np) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    int c[3];
+    tiled_coords(g, (int)(p >> 6), (int)(p & 63), c);
+    if (!in_grid(g, c)) return;
+    const long long lin = ((long long)c[0] * g.n[1] + c[1]) * g.n[2] + c[2];
+    out[lin] = (double)u[p];
+}
+
+// Geodesic flow and active-neighbour bits, recomputed from the converged values with the
+// accepted (v < u) differences (Headers/Base/Implementation/HamiltonFastMarching.hxx:187-272,
+// 363-378, EulerianStencil.hxx:155-233).  flow = -ReDim(sum_k w_k (u - v_k)^+ e_k).
+template <int ModelT, typename T>
+__global__ __launch_bounds__(256) void k_flow(Geom g, StencilData<T> sd, const T *__restrict__ u,
+                                              const unsigned long long *__restrict__ fixedmask, FlowScale fs,
+                                              double *__restrict__ flow, double *__restrict__ active_out,
+                                              long long np) {
+    using S = ModelShape<ModelT>;
+    constexpr int NF = S::NFWD, NC = NF + S::NSYM;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int tile = (int)(p >> 6), lane = (int)(p & 63);
+    int c[3];
+    tiled_coords(g, tile, lane, c);
+    if (!in_grid(g, c)) return;
+    const long long lin = ((long long)c[0] * g.n[1] + c[1]) * g.n[2] + c[2];
+    const T inf = dev_inf<T>();
+    LaneStencil<ModelT, T> st;
+    st.load(g, sd, p, c);
+    T v[NC];
+    int side[NC];
+    for (int k = 0; k < NC; ++k) {
+        int o0, o1, o2;
+        unpack_offset(st.off[k], o0, o1, o2);
+        v[k] = inf;
+        side[k] = 0;
+        if (st.w[k] == T(0)) continue;
+        int r[3];
+        T va = inf, vb = inf;
+        if (shift_coords(g, c, o0, o1, o2, r)) va = u[(long long)tile_of(g, r) * 64 + local_of(g, r)];
+        if (k >= NF && shift_coords(g, c, -o0, -o1, -o2, r)) vb = u[(long long)tile_of(g, r) * 64 + local_of(g, r)];
+        v[k] = va <= vb ? va : vb;
+        side[k] = va <= vb ? 1 : -1;
+    }
+    const T uu = u[p];
+    const bool fixed = (fixedmask[tile] >> lane) & 1ull;
+    uint32_t used = 0;
+    if (!fixed && uu < inf) {
+        // differences with v < u, in the local solver's order
+        T vv[NC];
+        for (int k = 0; k < NC; ++k) vv[k] = v[k] < uu ? v[k] : inf;
+        (void)local_solve<NC, T>(vv, st.w, used);
+    }
+    double f[3] = {0., 0., 0.};
+    double bits = 0.;
+    if (used) {
+        // Recompute the value from the used differences (minVal = their smallest value).
+        double m = __builtin_huge_val();
+        for (int k = 0; k < NC; ++k)
+            if (used >> k & 1u) m = (double)v[k] < m ? (double)v[k] : m;
+        double a = 0, b = 0, cc = -1;
+        for (int k = 0; k < NC; ++k) {
+            if (!(used >> k & 1u)) continue;
+            const double w = (double)st.w[k], vr = (double)v[k] - m;
+            a += w;
+            b += w * vr;
+            cc += (w * vr) * vr;
+        }
+        const double d2 = b * b - a * cc;
+        const double value = m + (b + sqrt(d2 > 0. ? d2 : 0.)) / a;
+        unsigned long long ab = 0;
+        for (int k = 0; k < NC; ++k) {
+            if (!(used >> k & 1u)) continue;
+            int o0, o1, o2;
+            unpack_offset(st.off[k], o0, o1, o2);
+            const double wpd = (double)st.w[k] * fmax(0., value - (double)v[k]);
+            f[0] += wpd * (double)(side[k] * o0);
+            f[1] += wpd * (double)(side[k] * o1);
+            f[2] += wpd * (double)(side[k] * o2);
+            const int bit = k < NF ? k : NF + 2 * (k - NF) + (side[k] > 0 ? 0 : 1);
+            ab |= 1ull << bit;
+        }
+        bits = (double)ab;
+    }
+    if (flow) {
+        for (int q = 0; q < g.dim; ++q) flow[lin * g.dim + q] = -(f[q] * fs.h[q]);
+    }
+    if (active_out) active_out[lin] = bits;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Host launchers.
+// ---------------------------------------------------------------------------------------------
+static inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+template <typename T>
+hipError_t launch_riemann_stencil(const Geom &g, const double *metric, int is_field, int dual, const InvHH &ih,
+                                  uint32_t *offs, T *w, long long np, int *status, hipStream_t s) {
+    if (g.dim == 2)
+        hipLaunchKernelGGL((k_riemann_stencil<2, T>), dim3(blocks_for(np, 256)), dim3(256), 0, s, g, metric, is_field,
+                           dual, ih, offs, w, np, status);
+    else
+        hipLaunchKernelGGL((k_riemann_stencil<3, T>), dim3(blocks_for(np, 256)), dim3(256), 0, s, g, metric, is_field,
+                           dual, ih, offs, w, np, status);
+    return hipGetLastError();
+}
+template hipError_t launch_riemann_stencil<float>(const Geom &, const double *, int, int, const InvHH &, uint32_t *,
+                                                  float *, long long, int *, hipStream_t);
+template hipError_t launch_riemann_stencil<double>(const Geom &, const double *, int, int, const InvHH &, uint32_t *,
+                                                   double *, long long, int *, hipStream_t);
+
+template <int M, typename T>
+static void deps_pass(const Geom &g, const StencilData<T> &sd, int pass, int *cnt, const int *start, int *fill,
+                      int *deps, int *ovf, hipStream_t s) {
+    hipLaunchKernelGGL((k_tile_deps<M, T>), dim3(g.ntiles), dim3(64), 0, s, g, sd, pass, cnt, start, fill, deps, ovf);
+}
+
+template <typename T>
+hipError_t launch_tile_deps(int model, const Geom &g, const StencilData<T> &sd, int pass, int *cnt, const int *start,
+                            int *fill, int *deps, int *ovf, hipStream_t s) {
+    switch (model) {
+        case MODEL_RIEMANN2: deps_pass<MODEL_RIEMANN2, T>(g, sd, pass, cnt, start, fill, deps, ovf, s); break;
+        case MODEL_RIEMANN3: deps_pass<MODEL_RIEMANN3, T>(g, sd, pass, cnt, start, fill, deps, ovf, s); break;
+        case MODEL_RSF2: deps_pass<MODEL_RSF2, T>(g, sd, pass, cnt, start, fill, deps, ovf, s); break;
+        case MODEL_ELASTICA2: deps_pass<MODEL_ELASTICA2, T>(g, sd, pass, cnt, start, fill, deps, ovf, s); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+template hipError_t launch_tile_deps<float>(int, const Geom &, const StencilData<float> &, int, int *, const int *,
+                                            int *, int *, int *, hipStream_t);
+template hipError_t launch_tile_deps<double>(int, const Geom &, const StencilData<double> &, int, int *, const int *,
+                                             int *, int *, int *, hipStream_t);
+
+hipError_t launch_exclusive_scan(const int *cnt, int *start, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_exclusive_scan, dim3(1), dim3(1024), 0, s, cnt, start, n);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_init(const Geom &g, T *u, unsigned long long *fixedmask, int *f0, int *f1, long long np,
+                       const long long *seed_p, const double *seed_v, int nseeds, hipStream_t s) {
+    hipLaunchKernelGGL((k_init<T>), dim3(blocks_for(np, 256)), dim3(256), 0, s, g, u, fixedmask, f0, f1, np);
+    if (nseeds > 0)
+        hipLaunchKernelGGL((k_set_seeds<T>), dim3(blocks_for(nseeds, 64)), dim3(64), 0, s, seed_p, seed_v, nseeds, u,
+                           fixedmask);
+    return hipGetLastError();
+}
+template hipError_t launch_init<float>(const Geom &, float *, unsigned long long *, int *, int *, long long,
+                                       const long long *, const double *, int, hipStream_t);
+template hipError_t launch_init<double>(const Geom &, double *, unsigned long long *, int *, int *, long long,
+                                        const long long *, const double *, int, hipStream_t);
+
+hipError_t launch_seed_activate(const long long *seed_p, int nseeds, const int *dep_start, const int *deps,
+                                int *flag_out, int *list_out, int *count_out, hipStream_t s) {
+    hipLaunchKernelGGL(k_seed_activate, dim3(1), dim3(64), 0, s, seed_p, nseeds, dep_start, deps, flag_out, list_out,
+                       count_out);
+    return hipGetLastError();
+}
+
+template <int M, typename T>
+static void fim_one(const FimArgs<T> &a, hipStream_t s) {
+    hipLaunchKernelGGL((k_fim<M, T, 32>), dim3(a.grid), dim3(256), 0, s, a.g, a.sd, a.u, a.fixedmask, a.dep_start,
+                       a.deps, a.list_in, a.count_in, a.list_out, a.count_out, a.count_reset, a.flag_in, a.flag_out,
+                       a.stats);
+}
+
+template <typename T>
+hipError_t launch_fim(int model, const FimArgs<T> &a, hipStream_t s) {
+    switch (model) {
+        case MODEL_RIEMANN2: fim_one<MODEL_RIEMANN2, T>(a, s); break;
+        case MODEL_RIEMANN3: fim_one<MODEL_RIEMANN3, T>(a, s); break;
+        case MODEL_RSF2: fim_one<MODEL_RSF2, T>(a, s); break;
+        case MODEL_ELASTICA2: fim_one<MODEL_ELASTICA2, T>(a, s); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+template hipError_t launch_fim<float>(int, const FimArgs<float> &, hipStream_t);
+template hipError_t launch_fim<double>(int, const FimArgs<double> &, hipStream_t);
+
+template <typename T>
+hipError_t launch_export_values(const Geom &g, const T *u, double *out, long long np, hipStream_t s) {
+    hipLaunchKernelGGL((k_export_values<T>), dim3(blocks_for(np, 256)), dim3(256), 0, s, g, u, out, np);
+    return hipGetLastError();
+}
+template hipError_t launch_export_values<float>(const Geom &, const float *, double *, long long, hipStream_t);
+template hipError_t launch_export_values<double>(const Geom &, const double *, double *, long long, hipStream_t);
+
+template <int M, typename T>
+static void flow_one(const Geom &g, const StencilData<T> &sd, const T *u, const unsigned long long *fm,
+                     const FlowScale &fs, double *flow, double *act, long long np, hipStream_t s) {
+    hipLaunchKernelGGL((k_flow<M, T>), dim3(blocks_for(np, 256)), dim3(256), 0, s, g, sd, u, fm, fs, flow, act, np);
+}
+
+template <typename T>
+hipError_t launch_flow(int model, const Geom &g, const StencilData<T> &sd, const T *u, const unsigned long long *fm,
+                       const FlowScale &fs, double *flow, double *act, long long np, hipStream_t s) {
+    switch (model) {
+        case MODEL_RIEMANN2: flow_one<MODEL_RIEMANN2, T>(g, sd, u, fm, fs, flow, act, np, s); break;
+        case MODEL_RIEMANN3: flow_one<MODEL_RIEMANN3, T>(g, sd, u, fm, fs, flow, act, np, s); break;
+        case MODEL_RSF2: flow_one<MODEL_RSF2, T>(g, sd, u, fm, fs, flow, act, np, s); break;
+        case MODEL_ELASTICA2: flow_one<MODEL_ELASTICA2, T>(g, sd, u, fm, fs, flow, act, np, s); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+template hipError_t launch_flow<float>(int, const Geom &, const StencilData<float> &, const float *,
+                                       const unsigned long long *, const FlowScale &, double *, double *, long long,
+                                       hipStream_t);
+template hipError_t launch_flow<double>(int, const Geom &, const StencilData<double> &, const double *,
+                                        const unsigned long long *, const FlowScale &, double *, double *, long long,
+                                        hipStream_t);
+
+}  // namespace hfm

@@ -1,0 +1,695 @@
+// hfm_plan.hip -- host orchestration of the device-resident solver plan (C ABI hfm_plan_*).
+//
+// A plan owns every device buffer of one grid: per-point stencils (or the shared per-theta
+// table and the speed field), the tiled values, the tile dependency graph, the two active-tile
+// lists with their flags and counters.  Buffers are allocated once at plan creation (or when
+// the dependency graph grows) and reused by every solve.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/hfm_amd.h"
+#include "hfm_internal.h"
+#include "hfm_kernels.h"
+#include "selling.h"
+
+using namespace hfm;
+
+namespace hfm {
+
+thread_local std::string g_last_error;
+
+void set_error(const std::string &m) { g_last_error = m; }
+
+}  // namespace hfm
+
+// speed = 1 / cost in the tiled layout (HFMInterface.hxx:131-150 DataSource_Inverse).
+__global__ void k_speed_from_cost_f(Geom g, const double *cost, int is_field, float *speed, long long np) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    int c[3];
+    tiled_coords(g, (int)(p >> 6), (int)(p & 63), c);
+    if (!in_grid(g, c)) { speed[p] = 0.f; return; }
+    const long long lin = ((long long)c[0] * g.n[1] + c[1]) * g.n[2] + c[2];
+    speed[p] = (float)(1. / cost[is_field ? lin : 0]);
+}
+__global__ void k_speed_from_cost_d(Geom g, const double *cost, int is_field, double *speed, long long np) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    int c[3];
+    tiled_coords(g, (int)(p >> 6), (int)(p & 63), c);
+    if (!in_grid(g, c)) { speed[p] = 0.; return; }
+    const long long lin = ((long long)c[0] * g.n[1] + c[1]) * g.n[2] + c[2];
+    speed[p] = 1. / cost[is_field ? lin : 0];
+}
+
+namespace {
+
+const double kPi = 3.141592653589793238462643383279502884L;
+
+int model_from_name(const std::string &m) {
+    if (m == "Riemann2") return MODEL_RIEMANN2;
+    if (m == "Riemann3") return MODEL_RIEMANN3;
+    if (m == "ReedsSheppForward2") return MODEL_RSF2;
+    if (m == "Elastica2" || m == "Elastica2_5") return MODEL_ELASTICA2;
+    return -1;
+}
+
+int model_ndiff(int model) {
+    switch (model) {
+        case MODEL_RIEMANN2: return 3;
+        case MODEL_RIEMANN3: return 6;
+        case MODEL_RSF2: return 4;
+        case MODEL_ELASTICA2: return 27;
+    }
+    return 0;
+}
+
+}  // namespace
+
+
+namespace {
+
+void plan_free(hfm_plan *p) {
+    dfree(p->u);
+    dfree(p->offs);
+    dfree(p->w);
+    dfree(p->speed);
+    dfree(p->tab_offs);
+    dfree(p->tab_w);
+    dfree(p->fixedmask);
+    dfree(p->dep_cnt);
+    dfree(p->dep_start);
+    dfree(p->dep_fill);
+    dfree(p->deps);
+    dfree(p->lists[0]);
+    dfree(p->lists[1]);
+    dfree(p->flags[0]);
+    dfree(p->flags[1]);
+    dfree(p->counts);
+    dfree(p->status);
+    dfree(p->stats);
+    dfree(p->seed_p);
+    dfree(p->seed_v);
+    if (p->h_count) (void)hipHostFree(p->h_count);
+    if (p->h_stats) (void)hipHostFree(p->h_stats);
+}
+
+// Build the tile dependency graph from the stencils now on the device.
+template <typename T> void build_deps(hfm_plan *p, hipStream_t s) {
+    const int nt = p->g.ntiles;
+    HIPCHECK(hipMemsetAsync(p->dep_cnt, 0, sizeof(int) * nt, s));
+    HIPCHECK(launch_tile_deps<T>(p->model, p->g, p->sd<T>(), 0, p->dep_cnt, nullptr, nullptr, nullptr, nullptr, s));
+    HIPCHECK(launch_exclusive_scan(p->dep_cnt, p->dep_start, nt, s));
+    int total = 0;
+    HIPCHECK(hipMemcpyAsync(p->h_count, p->dep_start + nt, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    total = *p->h_count;
+    if (total > p->deps_cap) {
+        dfree(p->deps);
+        p->deps = nullptr;
+        p->deps_cap = std::max<long long>(total, 1);
+        p->deps = dalloc<int>(p->deps_cap);
+    }
+    p->ndeps = total;
+    HIPCHECK(hipMemsetAsync(p->dep_fill, 0, sizeof(int) * nt, s));
+    HIPCHECK(launch_tile_deps<T>(p->model, p->g, p->sd<T>(), 1, p->dep_cnt, p->dep_start, p->dep_fill, p->deps,
+                                 nullptr, s));
+}
+
+// Voronoi1Vec (Headers/Specializations/CommonTraits.h:118-161) on the host: the shared
+// curvature stencils are 64-odd small decompositions, computed once per plan.
+template <int Dv> void voronoi1vec(const double *v, double eps, int (*off)[3], double *w) {
+    constexpr int K = Dv * (Dv + 1) / 2;
+    const double angleTol = 1. / std::sqrt(double(Dv - 1));
+    const double weightTol = 1000 * 2.220446049250313080847e-16;
+    SymPacked<Dv> relaxed;
+    double sqn = 0.;
+    for (int i = 0; i < Dv; ++i) sqn += v[i] * v[i];
+    for (int i = 0; i < Dv; ++i)
+        for (int j = 0; j <= i; ++j) {
+            const double exact = v[i] * v[j];
+            const double id = (i == j) ? 1. : 0.;
+            relaxed.m[i * (i + 1) / 2 + j] = exact * (1. - eps * eps) + (sqn * id) * (eps * eps);
+        }
+    int o[K][Dv];
+    double wr[K];
+    if (selling_decompose<Dv>(relaxed, o, wr) != SELLING_OK)
+        throw HipFail{"ObtuseSuperbase error : input must be positive definite"};
+    const double wTol = sqn * weightTol;
+    const double angTol = (1 + angleTol * angleTol) / sqn;
+    for (int k = 0; k < K; ++k) {
+        double scal = 0., offn = 0.;
+        for (int c = 0; c < Dv; ++c) scal += v[c] * double(o[k][c]);
+        for (int c = 0; c < Dv; ++c) offn += double(o[k][c]) * double(o[k][c]);
+        w[k] = (wr[k] < wTol || (scal * scal) * angTol < offn) ? 0. : wr[k];
+        off[k][0] = off[k][1] = off[k][2] = 0;
+        for (int c = 0; c < Dv; ++c) off[k][c] = (int)(int8_t)(scal <= 0 ? o[k][c] : -o[k][c]);
+    }
+}
+
+// Shared stencils, forward differences first then symmetric ones, like the reference layout
+// (EulerianStencil.h:23-40): ReedsSheppForward2 (Curvature2.h:88-102) and Elastica2<5>
+// (Curvature2.h:178-217, Fejer weights :233-234).
+void curvature_table(hfm_plan *p, std::vector<uint32_t> &toff, std::vector<double> &tw, int &maxw) {
+    const int nth = p->g.n[2];
+    const int nd = p->ndiff;
+    toff.assign((size_t)nth * nd, 0);
+    tw.assign((size_t)nth * nd, 0.);
+    maxw = 0;
+    const double gs = p->h[0];
+    for (int t = 0; t < nth; ++t) {
+        int off[27][3];
+        double w[27];
+        if (p->model == MODEL_RSF2) {
+            const double theta = t * p->dependScale;
+            double v[2] = {std::cos(theta), std::sin(theta)};
+            const double b = 1. / gs;
+            v[0] *= b;
+            v[1] *= b;
+            voronoi1vec<2>(v, p->eps, off, w);
+            off[3][0] = 0;
+            off[3][1] = 0;
+            off[3][2] = -1;
+            const double xs = p->xi * p->dependScale;
+            w[3] = 1. / (xs * xs);
+        } else {
+            const int nFejer = 5;
+            const double fejer[5] = {0.167781, 0.525552, 0.613333, 0.525552, 0.167781};
+            const double theta = (2. * kPi * t) / nth;
+            for (int l = 0; l < nFejer / 2; ++l) {
+                const double phi = kPi * (l + 0.5) / nFejer;
+                const double v[3] = {std::sin(phi) * std::cos(theta) / gs, std::sin(phi) * std::sin(theta) / gs,
+                                     std::cos(phi) / (p->xi * p->dependScale)};
+                const int s0 = 2 * l, s1 = 2 * l + 1;
+                voronoi1vec<3>(v, p->eps, &off[6 * s0], &w[6 * s0]);
+                for (int i = 0; i < 6; ++i) {
+                    w[6 * s0 + i] *= fejer[l];
+                    w[6 * s1 + i] = w[6 * s0 + i];
+                    off[6 * s1 + i][0] = off[6 * s0 + i][0];
+                    off[6 * s1 + i][1] = off[6 * s0 + i][1];
+                    off[6 * s1 + i][2] = (int)(int8_t)(-off[6 * s0 + i][2]);
+                }
+            }
+            const double v2[2] = {std::cos(theta) / gs, std::sin(theta) / gs};
+            const int s = nFejer - 1;
+            voronoi1vec<2>(v2, p->eps, &off[6 * s], &w[6 * s]);
+            for (int i = 0; i < 3; ++i) w[6 * s + i] *= fejer[nFejer / 2];
+        }
+        for (int k = 0; k < nd; ++k) {
+            toff[(size_t)t * nd + k] = pack_offset(off[k][0], off[k][1], off[k][2]);
+            tw[(size_t)t * nd + k] = w[k];
+            if (w[k] != 0.) maxw = std::max(maxw, std::abs(off[k][0]) + std::abs(off[k][1]) + std::abs(off[k][2]));
+        }
+    }
+}
+
+
+// Seed placement (StaticFactoring.hxx:242-393): ADim, floor, out-of-range seeds skipped,
+// the first value kept for a repeated index.  Returns tiled indices in row-major index order.
+void place_seeds(const hfm_plan *p, int nseeds, const double *seeds, const double *sv,
+                 std::vector<long long> &tp, std::vector<double> &vals) {
+    std::map<long long, std::pair<long long, double>> bylin;
+    const int dim = p->g.dim;
+    for (int s = 0; s < nseeds; ++s) {
+        int c[3] = {0, 0, 0};
+        bool ok = true;
+        for (int i = 0; i < dim; ++i) {
+            double q;
+            if (p->shared && i == 2) q = (seeds[s * dim + i] - p->origin[i]) / p->dependScale;
+            else q = (seeds[s * dim + i] - p->origin[i]) / p->h[i];
+            const double f = std::floor(q);
+            if (!(f > -2e9 && f < 2e9)) { ok = false; break; }
+            int ci = (int)f;
+            if (p->g.periodic >> i & 1) {
+                ci %= p->g.n[i];
+                if (ci < 0) ci += p->g.n[i];
+            } else if (ci < 0 || ci >= p->g.n[i]) {
+                ok = false;
+                break;
+            }
+            c[i] = ci;
+        }
+        if (!ok) continue;
+        const long long lin = ((long long)c[0] * p->g.n[1] + c[1]) * p->g.n[2] + c[2];
+        const long long tpi = (long long)tile_of(p->g, c) * 64 + local_of(p->g, c);
+        bylin.insert({lin, {tpi, sv ? sv[s] : 0.}});
+    }
+    tp.clear();
+    vals.clear();
+    for (const auto &kv : bylin) {
+        tp.push_back(kv.second.first);
+        vals.push_back(kv.second.second);
+    }
+}
+
+template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    HIPCHECK(hipEventRecord(e0, s));
+    const int nseeds = (int)p->last_seeds.size();
+    HIPCHECK(launch_init<T>(p->g, static_cast<T *>(p->u), p->fixedmask, p->flags[0], p->flags[1], p->np, p->seed_p,
+                            p->seed_v, nseeds, s));
+    HIPCHECK(hipMemsetAsync(p->counts, 0, 3 * sizeof(int), s));
+    HIPCHECK(hipMemsetAsync(p->stats, 0, 2 * sizeof(unsigned long long), s));
+    HIPCHECK(launch_seed_activate(p->seed_p, nseeds, p->dep_start, p->deps, p->flags[0], p->lists[0], &p->counts[0], s));
+
+    FimArgs<T> a;
+    a.g = p->g;
+    a.sd = p->sd<T>();
+    a.u = static_cast<T *>(p->u);
+    a.fixedmask = p->fixedmask;
+    a.dep_start = p->dep_start;
+    a.deps = p->deps;
+    a.stats = p->stats;
+    a.grid = (unsigned)std::min<long long>((p->g.ntiles + 3) / 4, 4096);
+
+    std::vector<hipEvent_t> tev;
+    long long k = 0;
+    const int batch = 8;
+    const long long max_rounds = 4LL * p->g.ntiles + 4096;
+    while (true) {
+        for (int b = 0; b < batch; ++b, ++k) {
+            a.list_in = p->lists[k & 1];
+            a.list_out = p->lists[(k + 1) & 1];
+            a.count_in = &p->counts[k % 3];
+            a.count_out = &p->counts[(k + 1) % 3];
+            a.count_reset = &p->counts[(k + 2) % 3];
+            a.flag_in = p->flags[k & 1];
+            a.flag_out = p->flags[(k + 1) & 1];
+            if (p->timing) {
+                hipEvent_t ea, eb;
+                HIPCHECK(hipEventCreate(&ea));
+                HIPCHECK(hipEventCreate(&eb));
+                HIPCHECK(hipEventRecord(ea, s));
+                HIPCHECK(launch_fim<T>(p->model, a, s));
+                HIPCHECK(hipEventRecord(eb, s));
+                tev.push_back(ea);
+                tev.push_back(eb);
+            } else {
+                HIPCHECK(launch_fim<T>(p->model, a, s));
+            }
+        }
+        HIPCHECK(hipMemcpyAsync(p->h_count, &p->counts[k % 3], sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        if (*p->h_count == 0) break;
+        if (k > max_rounds) throw HipFail{"block fast-iterative method did not converge"};
+    }
+    HIPCHECK(hipEventRecord(e1, s));
+    HIPCHECK(hipMemcpyAsync(p->h_stats, p->stats, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+    p->st.solve_ms = ms;
+    p->st.iterations = k;
+    p->st.tile_visits = (int64_t)p->h_stats[0];
+    p->st.sweeps = (int64_t)p->h_stats[1];
+    double fim = 0;
+    // Each pair brackets one launch; launches whose list was empty still count as launches.
+    for (size_t i = 0; i + 1 < tev.size(); i += 2) {
+        float d = 0;
+        HIPCHECK(hipEventElapsedTime(&d, tev[i], tev[i + 1]));
+        fim += d;
+        (void)hipEventDestroy(tev[i]);
+        (void)hipEventDestroy(tev[i + 1]);
+    }
+    p->st.fim_ms = p->timing ? fim : 0.;
+    p->st.fim_launches = p->timing ? (int64_t)(tev.size() / 2) : 0;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
+template <typename T> void set_metric_impl(hfm_plan *p, const double *d_metric, int is_field, int dual, hipStream_t s) {
+    InvHH ih{};
+    const int D = p->g.dim;
+    for (int i = 0; i < D; ++i)
+        for (int j = 0; j <= i; ++j) ih.v[i * (i + 1) / 2 + j] = 1. / (p->h[i] * p->h[j]);
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    HIPCHECK(hipEventRecord(e0, s));
+    int init[3] = {0, 0x7fffffff, 0};
+    HIPCHECK(hipMemcpyAsync(p->status, init, sizeof(init), hipMemcpyHostToDevice, s));
+    HIPCHECK(launch_riemann_stencil<T>(p->g, d_metric, is_field, dual, ih, p->offs, static_cast<T *>(p->w), p->np,
+                                       p->status, s));
+    int hs[3];
+    HIPCHECK(hipMemcpyAsync(hs, p->status, sizeof(hs), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    if (hs[0] == SELLING_NOT_SPD)
+        throw HipFail{"ObtuseSuperbase error : input must be positive definite (grid point " + std::to_string(hs[1]) +
+                      ")"};
+    if (hs[0] == SELLING_NO_CONVERGENCE)
+        throw HipFail{"ObtuseSuperbase error : did not terminate in 200 iterations (grid point " +
+                      std::to_string(hs[1]) + ")"};
+    p->max_stencil_width = hs[2];
+    build_deps<T>(p, s);
+    HIPCHECK(hipEventRecord(e1, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+    p->st.stencil_ms = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *hfm_last_error(void) { return g_last_error.c_str(); }
+
+int hfm_plan_create(hfm_plan **out, const char *model, const int64_t *dims, const double *gridScales,
+                    const double *origin, int precision, double xi, double eps) {
+    if (!out || !model || !dims || !gridScales) {
+        set_error("hfm_plan_create: null argument");
+        return 1;
+    }
+    *out = nullptr;
+    const int m = model_from_name(model);
+    if (m < 0) {
+        set_error(std::string("Unrecognized model ") + model);
+        return 1;
+    }
+    if (precision != 32 && precision != 64) {
+        set_error("precision must be 32 or 64");
+        return 1;
+    }
+    hfm_plan *p = new hfm_plan;
+    try {
+        p->model = m;
+        p->precision = precision;
+        p->shared = (m == MODEL_RSF2 || m == MODEL_ELASTICA2);
+        p->ndiff = model_ndiff(m);
+        const int dim = (m == MODEL_RIEMANN2) ? 2 : 3;
+        Geom &g = p->g;
+        g.dim = dim;
+        g.n[2] = 1;
+        p->N = 1;
+        for (int i = 0; i < dim; ++i) {
+            if (dims[i] <= 0) throw HipFail{"BaseGrid error : domain size must be positive"};
+            if (dims[i] > (1 << 24)) throw HipFail{"grid extent too large"};
+            g.n[i] = (int)dims[i];
+            p->N *= dims[i];
+        }
+        if (dim == 2) {
+            g.sh[0] = 3; g.sh[1] = 3; g.sh[2] = 0;
+        } else {
+            g.sh[0] = 2; g.sh[1] = 2; g.sh[2] = 2;
+        }
+        long long nt = 1;
+        for (int i = 0; i < 3; ++i) {
+            g.nt[i] = (g.n[i] + (1 << g.sh[i]) - 1) >> g.sh[i];
+            nt *= g.nt[i];
+        }
+        if (nt > (1LL << 26)) throw HipFail{"grid too large"};
+        g.ntiles = (int)nt;
+        g.periodic = p->shared ? 4 : 0;
+        p->np = nt * 64;
+        if (p->shared) {
+            if (!(gridScales[0] > 0)) throw HipFail{"Parametrization error : gridscale should be positive"};
+            p->h[0] = p->h[1] = gridScales[0];
+            p->dependScale = 2 * kPi / g.n[2];
+            p->h[2] = p->dependScale;
+            p->origin[0] = origin ? origin[0] : 0.;
+            p->origin[1] = origin ? origin[1] : 0.;
+            p->origin[2] = -p->dependScale / 2.;
+            p->xi = xi;
+            p->eps = eps;
+        } else {
+            for (int i = 0; i < dim; ++i) {
+                if (!(gridScales[i] > 0)) throw HipFail{"Parametrization error : gridscales should be positive"};
+                p->h[i] = gridScales[i];
+                p->origin[i] = origin ? origin[i] : 0.;
+            }
+        }
+        const size_t ts = precision == 32 ? 4 : 8;
+        p->u = dalloc<char>(p->np * ts);
+        if (!p->shared) {
+            p->offs = dalloc<uint32_t>(p->np * p->ndiff);
+            p->w = dalloc<char>(p->np * p->ndiff * ts);
+        } else {
+            p->speed = dalloc<char>(p->np * ts);
+            p->tab_offs = dalloc<uint32_t>((size_t)g.n[2] * p->ndiff);
+            p->tab_w = dalloc<double>((size_t)g.n[2] * p->ndiff);
+        }
+        p->fixedmask = dalloc<unsigned long long>(nt);
+        p->dep_cnt = dalloc<int>(nt);
+        p->dep_start = dalloc<int>(nt + 1);
+        p->dep_fill = dalloc<int>(nt);
+        p->lists[0] = dalloc<int>(nt);
+        p->lists[1] = dalloc<int>(nt);
+        p->flags[0] = dalloc<int>(nt);
+        p->flags[1] = dalloc<int>(nt);
+        p->counts = dalloc<int>(4);
+        p->status = dalloc<int>(4);
+        p->stats = dalloc<unsigned long long>(4);
+        HIPCHECK(hipHostMalloc((void **)&p->h_count, sizeof(int) * 4, hipHostMallocDefault));
+        HIPCHECK(hipHostMalloc((void **)&p->h_stats, sizeof(unsigned long long) * 4, hipHostMallocDefault));
+        p->st.n_points = p->N;
+        p->st.n_tiles = nt;
+    } catch (const HipFail &e) {
+        set_error(e.msg);
+        plan_free(p);
+        delete p;
+        return 1;
+    }
+    *out = p;
+    return 0;
+}
+
+void hfm_plan_destroy(hfm_plan *plan) {
+    if (!plan) return;
+    plan_free(plan);
+    delete plan;
+}
+
+int hfm_plan_set_metric(hfm_plan *p, const double *d_metric, int is_field, int dual, void *stream) {
+    if (!p || !d_metric) {
+        set_error("hfm_plan_set_metric: null argument");
+        return 1;
+    }
+    if (p->shared) {
+        set_error("hfm_plan_set_metric: model has no metric (use hfm_plan_set_cost)");
+        return 1;
+    }
+    try {
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        p->stencil_ready = false;
+        p->solved = false;
+        if (p->precision == 32) set_metric_impl<float>(p, d_metric, is_field, dual, s);
+        else set_metric_impl<double>(p, d_metric, is_field, dual, s);
+        p->st.max_stencil_width = p->max_stencil_width;
+        p->st.n_deps = p->ndeps;
+        p->stencil_ready = true;
+    } catch (const HipFail &e) {
+        set_error(e.msg);
+        return 1;
+    }
+    return 0;
+}
+
+int hfm_plan_set_cost(hfm_plan *p, const double *d_cost, int is_field, void *stream) {
+    if (!p || !d_cost) {
+        set_error("hfm_plan_set_cost: null argument");
+        return 1;
+    }
+    if (!p->shared) {
+        set_error("hfm_plan_set_cost: model has no cost field (use hfm_plan_set_metric)");
+        return 1;
+    }
+    try {
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        p->stencil_ready = false;
+        p->solved = false;
+        hipEvent_t e0, e1;
+        HIPCHECK(hipEventCreate(&e0));
+        HIPCHECK(hipEventCreate(&e1));
+        HIPCHECK(hipEventRecord(e0, s));
+        std::vector<uint32_t> toff;
+        std::vector<double> tw;
+        int maxw = 0;
+        curvature_table(p, toff, tw, maxw);
+        HIPCHECK(hipMemcpyAsync(p->tab_offs, toff.data(), toff.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(p->tab_w, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        const unsigned nb = (unsigned)((p->np + 255) / 256);
+        if (p->precision == 32) {
+            hipLaunchKernelGGL(k_speed_from_cost_f, dim3(nb), dim3(256), 0, s, p->g, d_cost, is_field,
+                               static_cast<float *>(p->speed), p->np);
+            HIPCHECK(hipGetLastError());
+            build_deps<float>(p, s);
+        } else {
+            hipLaunchKernelGGL(k_speed_from_cost_d, dim3(nb), dim3(256), 0, s, p->g, d_cost, is_field,
+                               static_cast<double *>(p->speed), p->np);
+            HIPCHECK(hipGetLastError());
+            build_deps<double>(p, s);
+        }
+        HIPCHECK(hipEventRecord(e1, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+        p->st.stencil_ms = ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        p->max_stencil_width = maxw;
+        p->st.max_stencil_width = maxw;
+        p->st.n_deps = p->ndeps;
+        p->stencil_ready = true;
+    } catch (const HipFail &e) {
+        set_error(e.msg);
+        return 1;
+    }
+    return 0;
+}
+
+int hfm_plan_solve(hfm_plan *p, int nseeds, const double *seeds, const double *seedValues, void *stream) {
+    if (!p || (nseeds > 0 && !seeds)) {
+        set_error("hfm_plan_solve: null argument");
+        return 1;
+    }
+    if (!p->stencil_ready) {
+        set_error("hfm_plan_solve: stencils not set (call hfm_plan_set_metric / hfm_plan_set_cost first)");
+        return 1;
+    }
+    try {
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        std::vector<long long> tp;
+        std::vector<double> vals;
+        place_seeds(p, nseeds, seeds, seedValues, tp, vals);
+        if (tp.empty()) throw HipFail{nseeds > 0 ? "Error : seeds incorrectly set" : "No seeds to run fast marching solver"};
+        if ((int)tp.size() > p->seed_cap) {
+            dfree(p->seed_p);
+            dfree(p->seed_v);
+            p->seed_p = nullptr;
+            p->seed_v = nullptr;
+            p->seed_cap = std::max<int>((int)tp.size(), 16);
+            p->seed_p = dalloc<long long>(p->seed_cap);
+            p->seed_v = dalloc<double>(p->seed_cap);
+        }
+        HIPCHECK(hipMemcpyAsync(p->seed_p, tp.data(), tp.size() * sizeof(long long), hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(p->seed_v, vals.data(), vals.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        p->last_seeds = tp;
+        if (p->precision == 32) run_fim<float>(p, s);
+        else run_fim<double>(p, s);
+        p->solved = true;
+    } catch (const HipFail &e) {
+        set_error(e.msg);
+        return 1;
+    }
+    return 0;
+}
+
+int hfm_plan_values(hfm_plan *p, double *d_values, void *stream) {
+    if (!p || !d_values) {
+        set_error("hfm_plan_values: null argument");
+        return 1;
+    }
+    if (!p->solved) {
+        set_error("hfm_plan_values: no solve has run");
+        return 1;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e = p->precision == 32
+                       ? launch_export_values<float>(p->g, static_cast<const float *>(p->u), d_values, p->np, s)
+                       : launch_export_values<double>(p->g, static_cast<const double *>(p->u), d_values, p->np, s);
+    if (e != hipSuccess) {
+        set_error(std::string("hfm_plan_values: ") + hipGetErrorString(e));
+        return 1;
+    }
+    return 0;
+}
+
+int hfm_plan_flow(hfm_plan *p, double *d_flow, double *d_active, void *stream) {
+    if (!p) {
+        set_error("hfm_plan_flow: null argument");
+        return 1;
+    }
+    if (!p->solved) {
+        set_error("hfm_plan_flow: no solve has run");
+        return 1;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    FlowScale fs{{p->h[0], p->h[1], p->h[2]}};
+    hipError_t e = p->precision == 32
+                       ? launch_flow<float>(p->model, p->g, p->sd<float>(), static_cast<const float *>(p->u),
+                                            p->fixedmask, fs, d_flow, d_active, p->np, s)
+                       : launch_flow<double>(p->model, p->g, p->sd<double>(), static_cast<const double *>(p->u),
+                                             p->fixedmask, fs, d_flow, d_active, p->np, s);
+    if (e != hipSuccess) {
+        set_error(std::string("hfm_plan_flow: ") + hipGetErrorString(e));
+        return 1;
+    }
+    return 0;
+}
+
+int hfm_plan_get_stats(const hfm_plan *p, hfm_plan_stats *st) {
+    if (!p || !st) {
+        set_error("hfm_plan_get_stats: null argument");
+        return 1;
+    }
+    *st = p->st;
+    return 0;
+}
+
+int hfm_plan_get_stencils(hfm_plan *p, int8_t *h_offsets, double *h_weights) {
+    if (!p || !h_offsets || !h_weights) {
+        set_error("hfm_plan_get_stencils: null argument");
+        return 1;
+    }
+    if (p->shared || !p->stencil_ready) {
+        set_error("hfm_plan_get_stencils: no per-point stencils (Riemann plan after hfm_plan_set_metric)");
+        return 1;
+    }
+    try {
+        const int K = p->ndiff, D = p->g.dim;
+        std::vector<uint32_t> offs((size_t)p->np * K);
+        HIPCHECK(hipMemcpy(offs.data(), p->offs, offs.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        std::vector<double> wd((size_t)p->np * K);
+        if (p->precision == 32) {
+            std::vector<float> wf((size_t)p->np * K);
+            HIPCHECK(hipMemcpy(wf.data(), p->w, wf.size() * sizeof(float), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < wf.size(); ++i) wd[i] = wf[i];
+        } else {
+            HIPCHECK(hipMemcpy(wd.data(), p->w, wd.size() * sizeof(double), hipMemcpyDeviceToHost));
+        }
+        for (long long tp = 0; tp < p->np; ++tp) {
+            int c[3];
+            tiled_coords(p->g, (int)(tp >> 6), (int)(tp & 63), c);
+            if (!in_grid(p->g, c)) continue;
+            const long long lin = ((long long)c[0] * p->g.n[1] + c[1]) * p->g.n[2] + c[2];
+            for (int k = 0; k < K; ++k) {
+                int o[3];
+                unpack_offset(offs[(size_t)k * p->np + tp], o[0], o[1], o[2]);
+                for (int q = 0; q < D; ++q) h_offsets[(lin * K + k) * D + q] = (int8_t)o[q];
+                h_weights[lin * K + k] = wd[(size_t)k * p->np + tp];
+            }
+        }
+    } catch (const HipFail &e) {
+        set_error(e.msg);
+        return 1;
+    }
+    return 0;
+}
+
+int hfm_plan_set_timing(hfm_plan *p, int enable) {
+    if (!p) {
+        set_error("hfm_plan_set_timing: null argument");
+        return 1;
+    }
+    p->timing = enable != 0;
+    return 0;
+}
+
+int hfm_plan_adjoint(hfm_plan *p, int ntargets, const double *targets, const double *weights, double *d_sensitivity,
+                     int64_t *seed_index, double *seed_weight, int max_seeds, int *nseed_out, void *stream) {
+    return hfm_plan_adjoint_impl(p, ntargets, targets, weights, d_sensitivity, seed_index, seed_weight, max_seeds,
+                                 nseed_out, stream);
+}
+
+}  // extern "C"

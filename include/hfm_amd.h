@@ -1,0 +1,154 @@
+/*
+ * hfm_amd.h -- C ABI of the MI355X (gfx950) Fast-Marching library libhfm_amd.so.
+ *
+ * Drop-in boundary for HamiltonFastMarching's CPU solve (Mirebeau/HamiltonFastMarching,
+ * conda hfm 0.2.6).  Two layers:
+ *
+ *  1. The key/value IO object and hfm_run(), which replace the reference's
+ *        void Run(IO&)                         Headers/DispatchAndRun.h:224-353
+ *     as called by FileHFM (Interfaces/FileHFM/FileHFM.cpp:12-26) and by the pybind module
+ *     HFMIO (Interfaces/PythonHFM/src/main.cpp:11-47).  Keys, shapes, array orderings and
+ *     outputs follow the reference (JMM_CPPLibs/JMM_CPPLibs/Output/IO.h:77-118,
+ *     Headers/Base/Implementation/HFMInterface.hxx:296-579).  The model is chosen at run time
+ *     from the "model" key (Riemann2, Riemann3, ReedsSheppForward2, Elastica2) instead of at
+ *     compile time (DispatchAndRun.h:12,227-244).
+ *
+ *  2. A device-resident plan (hfm_plan_*) for callers that keep fields in HBM: stencils,
+ *     values and scratch stay on the GPU between calls.  It has no single reference
+ *     counterpart; it splits HFMInterface::Run into its phases (Run_SetupStencil :343-352,
+ *     Run_RunSolver :422-450, Run_ExportData :526-579, FirstVariation::Finally
+ *     Headers/ExtraAlgorithms/FirstVariation.h:60-102) over device pointers.
+ *
+ * Conventions: every function returns 0 on success and a nonzero code on failure; no C++
+ * exception crosses this boundary.  The message of the last failure is returned by
+ * hfm_io_last_error() for IO calls and by hfm_last_error() (per thread) for plan calls;
+ * it carries the reference's wording where the reference raises the same error
+ * (JMM_CPPLibs/JMM_CPPLibs/Macros/Exception.h:9-19).  Arrays crossing the IO layer are FP64
+ * like the reference's RawElement (BaseIO.hxx:10-33) and are copied in and out.  Pointers
+ * documented as "device" are HIP device pointers; "stream" is a hipStream_t (NULL = default).
+ */
+#ifndef HFM_AMD_H
+#define HFM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ----------------------------------------------------------------------------------------- */
+/* IO dictionary (replaces IO_<Base>: IO.h:77-118; pybind HFMIO main.cpp:35-47)              */
+/* ----------------------------------------------------------------------------------------- */
+typedef struct hfm_io hfm_io;
+
+/* new IO object (IO_<FileIO>/IO_<PythonIO> construction) */
+hfm_io *hfm_io_create(void);
+void hfm_io_destroy(hfm_io *io);
+/* set_string (IO.h:98 SetString; main.cpp:37) */
+int hfm_io_set_string(hfm_io *io, const char *key, const char *value);
+/* set_scalar (IO.h:103 Set<ScalarType>; main.cpp:39) */
+int hfm_io_set_scalar(hfm_io *io, const char *key, double value);
+/* set_array (IO.h:105 SetArray; main.cpp:41): row-major data of the given shape as stored */
+int hfm_io_set_array(hfm_io *io, const char *key, const double *data, int ndim, const int64_t *shape);
+/* has_field (IO.h:92 HasField; main.cpp:43): 1 if present, 0 otherwise */
+int hfm_io_has_field(const hfm_io *io, const char *key);
+/* erase_field (main.cpp:45) */
+int hfm_io_erase_field(hfm_io *io, const char *key);
+/* 0: absent, 1: string, 2: scalar or array */
+int hfm_io_field_kind(const hfm_io *io, const char *key);
+/* get_string (IO.h:97 GetString; main.cpp:36): copies into buf (NUL-terminated, truncated to
+ * bufsize); returns the full length, or -1 if missing / not a string */
+long long hfm_io_get_string(hfm_io *io, const char *key, char *buf, size_t bufsize);
+/* get_scalar (IO.h:88 Get<ScalarType>; main.cpp:38) */
+int hfm_io_get_scalar(hfm_io *io, const char *key, double *value);
+/* shape of an array field (ndim = 0 for a scalar); shape must hold max_ndim entries */
+int hfm_io_get_array_shape(hfm_io *io, const char *key, int *ndim, int64_t *shape, int max_ndim);
+/* get_array (IO.h:91 GetArray; main.cpp:40): copies count values (row-major as stored) */
+int hfm_io_get_array(hfm_io *io, const char *key, double *out, size_t count);
+/* computed_keys (main.cpp:42): space-separated keys written by hfm_run; returns full length */
+long long hfm_io_computed_keys(hfm_io *io, char *buf, size_t bufsize);
+const char *hfm_io_last_error(const hfm_io *io);
+
+/* Run the solve described by io (Run(IO&), Headers/DispatchAndRun.h:224-353).  Synchronous. */
+int hfm_run(hfm_io *io);
+
+/* ----------------------------------------------------------------------------------------- */
+/* Device-resident plan (phases of HFMInterface::Run over device pointers)                     */
+/* ----------------------------------------------------------------------------------------- */
+typedef struct hfm_plan hfm_plan;
+
+typedef struct hfm_plan_stats {
+    int64_t iterations;     /* block-FIM rounds of the last solve */
+    int64_t tile_visits;    /* tiles processed (64 points each) over the last solve */
+    int64_t sweeps;         /* local sweeps summed over tile visits */
+    int64_t max_stencil_width; /* MaxStencilWidth (HFMInterface.hxx:357) */
+    int64_t n_points;       /* grid points */
+    int64_t n_tiles;        /* tiles of 64 points */
+    int64_t n_deps;         /* entries of the tile dependency graph */
+    double stencil_ms;      /* device time of the last stencil phase (hipEvent) */
+    double solve_ms;        /* device time of the last solve (hipEvent) */
+    double fim_ms;          /* summed duration of the update-kernel launches of the last solve
+                               (hipEvent pairs around each launch on the plan's stream) */
+    int64_t fim_launches;   /* update-kernel launches of the last solve */
+} hfm_plan_stats;
+
+/* model: "Riemann2" | "Riemann3" | "ReedsSheppForward2" | "Elastica2".
+ * dims: grid extents (2 or 3).  gridScales: one per axis for Riemann (Riemannian.h:46-61,
+ * HFM_ParamDefault.hxx:68-102); gridScales[0] for the curvature models (the theta step is
+ * 2 pi / dims[2], Curvature2.h:104-108).  origin: NULL or one per axis (only the two physical
+ * components for the curvature models).  precision: 32 or 64 (update arithmetic).
+ * xi, eps: curvature-model parameters (Curvature2.h:104-108; ignored for Riemann). */
+int hfm_plan_create(hfm_plan **out, const char *model, const int64_t *dims, const double *gridScales,
+                    const double *origin, int precision, double xi, double eps);
+void hfm_plan_destroy(hfm_plan *plan);
+
+/* Riemann: metric (device, FP64, row-major (dims..., d(d+1)/2) or one tensor if !is_field),
+ * primal ("metric") or dual ("dualMetric").  Builds the per-point Selling stencils
+ * (StencilRiemann::SetStencil, Riemannian.h:41-44) and the tile dependency graph.
+ * Fails with the reference's message on a non positive-definite tensor. */
+int hfm_plan_set_metric(hfm_plan *plan, const double *d_metric, int is_field, int dual, void *stream);
+
+/* Curvature models: cost (device, FP64, row-major dims or one value if !is_field);
+ * speed = 1/cost (HFM_StencilShare.hxx:61-71). */
+int hfm_plan_set_cost(hfm_plan *plan, const double *d_cost, int is_field, void *stream);
+
+/* Solve from seeds (host, physical coordinates, nseeds x dim; seedValues NULL -> 0), the
+ * seed placement of StaticFactoring::SetSeeds (StaticFactoring.hxx:242-393).  Runs the
+ * block fast-iterative method to its fixed point on the plan's stream. */
+int hfm_plan_solve(hfm_plan *plan, int nseeds, const double *seeds, const double *seedValues, void *stream);
+
+/* Arrival times of the last solve, row-major FP64 (device) -- the 'values' output. */
+int hfm_plan_values(hfm_plan *plan, double *d_values, void *stream);
+
+/* geodesicFlow (N x dim, FP64, device, may be NULL) and activeNeighs (N, FP64 holding the
+ * bitset like the reference's export, device, may be NULL) (HFMInterface.hxx:536-553). */
+int hfm_plan_flow(hfm_plan *plan, double *d_flow, double *d_active, void *stream);
+
+/* Backward differentiation (FirstVariation::BackwardVariation, FirstVariation.h:341-385):
+ * gradient of sum_t weights[t] * u(targets[t]) w.r.t. the cost field (row-major FP64 device
+ * array costSensitivity; zero outside the targets' dependency cone).  targets are host
+ * physical points (ntargets x dim).  Seeds reached are returned in seed_index (row-major
+ * linear index) / seed_weight, up to max_seeds entries; *nseed_out = number reached. */
+int hfm_plan_adjoint(hfm_plan *plan, int ntargets, const double *targets, const double *weights,
+                     double *d_sensitivity, int64_t *seed_index, double *seed_weight, int max_seeds,
+                     int *nseed_out, void *stream);
+
+int hfm_plan_get_stats(const hfm_plan *plan, hfm_plan_stats *stats);
+
+/* Per-point stencils of a Riemann plan, copied to host buffers in row-major point order:
+ * offsets N x K x dim int8 and weights N x K (FP64 copy of the stored weights), in the
+ * reference's slot order (what getStencils prints, HFMInterface.hxx:360-375). */
+int hfm_plan_get_stencils(hfm_plan *plan, int8_t *h_offsets, double *h_weights);
+
+/* Per-launch timing of the update kernel (hipEvent pairs around every launch; off by default
+ * because the events add a few microseconds per launch).  Fills fim_ms / fim_launches. */
+int hfm_plan_set_timing(hfm_plan *plan, int enable);
+
+/* Message of the last failing plan call on this thread ("" if none). */
+const char *hfm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,78 @@
+"""GPU parity: the HIP library against the oracle (FP64 restatement of the reference, itself
+bit-exact against the reference binaries) on the same seeded inputs."""
+import numpy as np
+import pytest
+
+import oracle
+import hamiltonfastmarching_amd as hfm
+from hamiltonfastmarching_amd import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+
+
+def _floor_rel(u, ref):
+    """Max relative error over cells with ref >= the smallest nonzero value (floor near seeds)."""
+    fin = np.isfinite(ref)
+    pos = ref[fin & (ref > 0)]
+    floor = pos.min() if pos.size else 0.0
+    m = fin & (ref >= floor) & (ref > 0)
+    return float(np.max(np.abs(u[m] - ref[m]) / ref[m])) if m.any() else 0.0
+
+
+CASES = [
+    ("riemann2_48", lambda: syn.riemann2(48, seed=1)),
+    ("riemann3_20", lambda: syn.riemann3(20, seed=2)),
+    ("rsf2_24x24x16", lambda: syn.rsf2(24, 16, seed=3)),
+    ("elastica2_20x20x16", lambda: syn.elastica2(20, 16, seed=4)),
+]
+
+
+@pytest.mark.parametrize("name,make", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("precision", [64, 32])
+def test_values_match_oracle(name, make, precision):
+    print(f"case {name} precision {precision}", flush=True)
+    p = make()
+    for k in ("tips", "inspectSensitivity", "inspectSensitivityWeights", "inspectSensitivityLengths"):
+        p.pop(k, None)
+    ref = oracle.solve(p)["values"]
+    q = dict(p)
+    q["precision"] = float(precision)
+    out = hfm.run(q)
+    u = out["values"]
+    assert u.shape == ref.shape
+    fin = np.isfinite(ref)
+    assert np.array_equal(np.isfinite(u), fin)
+    err = _floor_rel(u, ref)
+    tol = 1e-12 if precision == 64 else 1e-5
+    print(f"{name} fp{precision}: max rel err {err:.3e}, bitexact {np.mean(u[fin] == ref[fin]):.4f}")
+    assert err <= tol
+
+
+@pytest.mark.parametrize("n,dim", [(64, 2), (24, 3)])
+@pytest.mark.parametrize("precision", [64, 32])
+def test_stencil_offsets_bitexact(n, dim, precision):
+    import torch
+    print(f"stencils dim {dim} n {n} precision {precision}", flush=True)
+    p = syn.riemann2(n, seed=5) if dim == 2 else syn.riemann3(n, seed=6)
+    plan = hfm.Plan(p["model"], [n] * dim, p["gridScale"], precision=precision)
+    plan.set_metric(torch.from_numpy(p["metric"]))
+    off, w = plan.stencils()
+    roff, rw = oracle.riemann_stencils([n] * dim, p["gridScale"], p["metric"])
+    assert np.array_equal(off, roff)
+    if precision == 64:
+        assert np.array_equal(w, rw)
+    else:
+        assert np.array_equal(w, rw.astype(np.float32).astype(np.float64))
+
+
+@pytest.mark.parametrize("n", [48, 64])
+def test_riemann3_fp32_error(n):
+    print(f"riemann3 n {n} fp32", flush=True)
+    p = syn.riemann3(n, seed=7)
+    ref = oracle.solve(p)["values"]
+    q = dict(p)
+    q["precision"] = 32.0
+    u = hfm.run(q)["values"]
+    err = _floor_rel(u, ref)
+    print(f"riemann3 {n}^3 fp32: max rel err {err:.3e}", flush=True)
+    assert err <= 1e-5
