@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Benchmark: Riemann3 128^3 single solve on one MI355X vs the reference CPU Fast Marching.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+A step is one complete solve of the Riemann3 workload (128^3 grid, h = 1/128, one seed at the
+centre, SPD metric R diag(1, a^2, b^2) R^T with a, b in [1, 20] from smooth seeded fields):
+FP64 Selling stencils from the HBM-resident metric, block fast-iterative FP32 update to the
+fixed point, and the row-major FP64 'values' output, all on the device.  Inputs are resident in
+HBM before the timed region; the timed region is bracketed by torch.cuda.synchronize().
+
+Printed JSON fields beyond the driver's contract:
+  roofline      update kernel (k_fim): algorithmic bytes per launch / mean launch duration
+                (hipEvent pairs around every launch on the plan's stream, in a separate
+                instrumented solve), against the 8 TB/s HBM peak; traffic from rocprofv3
+                counters when profiles/fim_riemann3_128_traffic.json exists.
+  cpu_baseline  the reference's FileHFM_Riemann3, compiled from its sources into oracle/_ref,
+                timed on this host (one full solve, 1 core; StencilCPUTime + FMCPUTime), or the
+                oracle's C++ restatement when the reference binary is absent.
+  accuracy      max relative error of the FP32 values vs the reference FP64 values (floored at
+                the smallest nonzero reference value), from the CPU-baseline run.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ms/solve Riemann3 128^3 vs HFM CPU; served solves/s + p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per point for one tile visit of the FP32 Riemann3 update: u read 4 +
+# u write 4 + 6 FP32 weights 24 + 6 x 3 int8 offsets 18 (neighbour reads are cache hits).
+BYTES_PER_POINT_VISIT = 50
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    return ap.parse_args()
+
+
+def cpu_baseline(params):
+    """Time one full CPU solve: the reference binary if built, else the oracle restatement."""
+    import oracle
+    cores = 1
+    if oracle.ref_available(params["model"]):
+        t = time.time()
+        out = oracle.run_reference(params)
+        wall = time.time() - t
+        if out.get("retcode", 1) != 0:
+            raise RuntimeError("reference run failed: " + out.get("log", "")[-500:])
+        cpu_s = out["StencilCPUTime"] + out["FMCPUTime"]
+        kind = "reference"
+        values = out["values"]
+        fm_s = out["FMCPUTime"]
+    else:
+        t = time.time()
+        out = oracle.solve(params)
+        wall = time.time() - t
+        cpu_s = out["StencilCPUTime"] + out["FMCPUTime"]
+        kind = "port"
+        values = out["values"]
+        fm_s = out["FMCPUTime"]
+    return {
+        "value": round(cpu_s * 1e3, 1), "unit": "ms/solve", "cores": cores, "kind": kind,
+        "sample": f"1 full Riemann3 {params['dims'][0]:.0f}^3 solve (StencilCPUTime + FMCPUTime, "
+                  f"clock() CPU time), wall {wall:.1f} s",
+        "fm_ms": round(fm_s * 1e3, 1), "nproc": os.cpu_count(),
+    }, values
+
+
+def floor_rel(u, ref):
+    fin = np.isfinite(ref)
+    pos = ref[fin & (ref > 0)]
+    floor = pos.min()
+    m = fin & (ref >= floor)
+    return float(np.max(np.abs(u[m] - ref[m]) / ref[m]))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "a multi-GPU layout is described but not built.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    import hamiltonfastmarching_amd as hfm
+    from hamiltonfastmarching_amd import synthetic as syn
+
+    n = args.n
+    params = syn.riemann3(n, seed=0)
+    metric = torch.from_numpy(params["metric"]).cuda()
+    seeds = params["seeds"]
+    plan = hfm.Plan("Riemann3", (n, n, n), params["gridScale"], precision=32)
+    out = torch.empty((n, n, n), dtype=torch.float64, device="cuda")
+
+    def step():
+        plan.set_metric(metric)
+        plan.solve(seeds)
+        plan.values(out)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / max(args.steps, 1)
+    st = plan.stats()
+
+    # Instrumented solve: per-launch hipEvent timing of the update kernel.
+    plan.set_timing(True)
+    step()
+    torch.cuda.synchronize()
+    sti = plan.stats()
+    plan.set_timing(False)
+    launches = max(sti["fim_launches"], 1)
+    bytes_total = sti["tile_visits"] * 64 * BYTES_PER_POINT_VISIT
+    mean_launch_ms = sti["fim_ms"] / launches
+    achieved = (bytes_total / launches) / (mean_launch_ms * 1e-3) / 1e9 if mean_launch_ms > 0 else 0.0
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "fim_riemann3_128_traffic.json")
+    if os.path.exists(tpath) and n == 128:
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    roofline = {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+        "kernel": "k_fim<Riemann3,float>", "launches": sti["fim_launches"],
+        "mean_launch_us": round(mean_launch_ms * 1e3, 2),
+        "tile_visits": sti["tile_visits"], "bytes_per_point_visit": BYTES_PER_POINT_VISIT,
+    }
+    result = {
+        "metric": METRIC, "value": round(ms, 3), "unit": "ms/solve", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded smooth SPD field, anisotropy up to 20)",
+        "config": {"workload": f"Riemann3 {n}^3 single solve, h=1/{n}, centre seed, FP32 update, FP64 Selling",
+                   "grid": [n, n, n], "seeds": 1},
+        "solves_per_s": round(1e3 / ms, 2),
+        "solve_breakdown_ms": {"stencil_and_deps": round(st["stencil_ms"], 3), "fim": round(st["solve_ms"], 3)},
+        "fim_iterations": st["iterations"], "tile_visits": st["tile_visits"], "sweeps": st["sweeps"],
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        cb, ref_values = cpu_baseline(params)
+        result["cpu_baseline"] = cb
+        u = out.cpu().numpy()
+        result["accuracy"] = {"max_rel_err_vs_fp64_reference": floor_rel(u, ref_values),
+                              "tolerance": 1e-5}
+        result["speedup_vs_cpu"] = round(cb["value"] / ms, 1)
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

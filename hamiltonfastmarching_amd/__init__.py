@@ -260,6 +260,10 @@ class Plan:
                                                   _dptr(w)))
         return off, w
 
+    def set_delta(self, factor: float):
+        self._check(self._L.hfm_plan_set_delta(self._h, float(factor)))
+        return self
+
     def set_timing(self, enable: bool = True):
         self._check(self._L.hfm_plan_set_timing(self._h, int(enable)))
         return self

@@ -20,7 +20,8 @@ class HfmPlanStats(ctypes.Structure):
     _fields_ = [("iterations", c_int64), ("tile_visits", c_int64), ("sweeps", c_int64),
                 ("max_stencil_width", c_int64), ("n_points", c_int64), ("n_tiles", c_int64),
                 ("n_deps", c_int64), ("stencil_ms", c_double), ("solve_ms", c_double),
-                ("fim_ms", c_double), ("fim_launches", c_int64)]
+                ("fim_ms", c_double), ("fim_launches", c_int64), ("tiles_carried", c_int64),
+                ("delta", c_double), ("adjoint_rounds", c_int64)]
 
 
 # (name, restype, argtypes) of every exported symbol; tests check that each one is exported.
@@ -52,6 +53,7 @@ SIGNATURES = [
                                  c_int, P(c_int), c_void_p]),
     ("hfm_plan_get_stats", c_int, [c_void_p, P(HfmPlanStats)]),
     ("hfm_plan_set_timing", c_int, [c_void_p, c_int]),
+    ("hfm_plan_set_delta", c_int, [c_void_p, c_double]),
     ("hfm_plan_get_stencils", c_int, [c_void_p, P(ctypes.c_int8), P(c_double)]),
     ("hfm_last_error", ctypes.c_char_p, []),
 ]

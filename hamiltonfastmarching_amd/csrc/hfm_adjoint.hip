@@ -1,8 +1,379 @@
-// hfm_adjoint.hip -- backward differentiation (placeholder until the adjoint kernels land).
-#include "hfm_internal.h"
+// hfm_adjoint.hip -- backward differentiation of the arrival times w.r.t. the cost field.
+//
+// The reference (FirstVariation::BackwardVariation, Headers/ExtraAlgorithms/FirstVariation.h:
+// 341-385, with ValueVariation :198-251) sweeps the points in strictly decreasing value order
+// and, at each point x with adjoint weight lambda(x):
+//   w'_k = Weight_k * (u(x) - v_k)^+ over x's active differences, S = sum_k w'_k,
+//   costSensitivity(x) = lambda(x) * var(x),   var = (mult * sum_k w'_k (u - v_k)) / S
+//                                               (mult = speed = 1/cost; 1 for Riemann models),
+//   lambda(neighbour_k) += lambda(x) * w'_k / S,
+//   a point without active differences (a seed) is reported in seedSensitivity.
+// The active differences are the ones the forward solve accepted (v_k < u(x)), recomputed here
+// from the converged values with the forward kernels' local solver.
+//
+// The sweep order is a topological order of the DAG x -> neighbour_k (edges go to strictly
+// smaller values).  On the GPU the same sums are accumulated with Kahn's algorithm: every point
+// counts its predecessors (in-degree); a round processes the points whose predecessors have all
+// been processed (their lambda is final), pushes lambda(x) w'_k / S with atomic adds, then
+// decrements the neighbours' in-degrees and appends those reaching zero to the next round.
+// The result equals the reference's up to the summation order of the atomic adds.
+#include <algorithm>
+#include <map>
+#include <vector>
 
-int hfm_plan_adjoint_impl(hfm_plan *, int, const double *, const double *, double *, int64_t *, double *, int, int *,
-                          void *) {
-    hfm::set_error("hfm_plan_adjoint: not available in this build");
-    return 1;
+#include "hfm_internal.h"
+#include "hfm_kernels.h"
+
+namespace hfm {
+
+// Active differences of tiled point p recomputed from converged values.  Returns the number of
+// active entries; nb[] their tiled indices, wp[] the w'_k, and *wsum, *var.
+template <int ModelT, typename T>
+__device__ inline int active_set(const Geom &g, const StencilData<T> &sd, const T *__restrict__ u, long long p,
+                                 const int *c, bool fixed, long long *nb, double *wp, double *wsum, double *var) {
+    using S = ModelShape<ModelT>;
+    constexpr int NF = S::NFWD, NC = NF + S::NSYM;
+    const T inf = dev_inf<T>();
+    const T uu = u[p];
+    *wsum = 0.;
+    *var = 0.;
+    if (fixed || !(uu < inf)) return 0;
+    LaneStencil<ModelT, T> st;
+    st.load(g, sd, p, c);
+    T v[NC];
+    long long idx[NC];
+    for (int k = 0; k < NC; ++k) {
+        int o0, o1, o2;
+        unpack_offset(st.off[k], o0, o1, o2);
+        v[k] = inf;
+        idx[k] = -1;
+        if (st.w[k] == T(0)) continue;
+        int r[3];
+        T va = inf, vb = inf;
+        long long ia = -1, ib = -1;
+        if (shift_coords(g, c, o0, o1, o2, r)) {
+            ia = (long long)tile_of(g, r) * 64 + local_of(g, r);
+            va = u[ia];
+        }
+        if (k >= NF && shift_coords(g, c, -o0, -o1, -o2, r)) {
+            ib = (long long)tile_of(g, r) * 64 + local_of(g, r);
+            vb = u[ib];
+        }
+        const bool a_first = va <= vb;
+        v[k] = a_first ? va : vb;
+        idx[k] = a_first ? ia : ib;
+        if (!(v[k] < uu)) v[k] = inf;
+    }
+    uint32_t used = 0;
+    (void)local_solve<NC, T>(v, st.w, used);
+    const double mult = S::SHARED ? (double)sd.speed[p] : 1.;
+    int n = 0;
+    double ws = 0., acc = 0.;
+    for (int k = 0; k < NC; ++k) {
+        if (!(used >> k & 1u)) continue;
+        const double d = fmax(0., (double)uu - (double)v[k]);
+        const double w = (double)st.w[k] * d;
+        ws += w;
+        acc += w * d;
+        nb[n] = idx[k];
+        wp[n] = w;
+        ++n;
+    }
+    *wsum = ws;
+    *var = ws > 0. ? (mult * acc) / ws : 0.;
+    return n;
+}
+
+template <int ModelT, typename T>
+__global__ __launch_bounds__(256) void k_adj_prepare(Geom g, StencilData<T> sd, const T *__restrict__ u,
+                                                     const unsigned long long *__restrict__ fixedmask,
+                                                     int *__restrict__ indeg, double *__restrict__ lambda,
+                                                     double *__restrict__ sens, long long np) {
+    constexpr int NC = ModelShape<ModelT>::NFWD + ModelShape<ModelT>::NSYM;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    lambda[p] = 0.;
+    sens[p] = 0.;
+    int c[3];
+    tiled_coords(g, (int)(p >> 6), (int)(p & 63), c);
+    if (!in_grid(g, c)) return;
+    const bool fixed = (fixedmask[p >> 6] >> (p & 63)) & 1ull;
+    long long nb[NC];
+    double wp[NC], ws, var;
+    const int n = active_set<ModelT, T>(g, sd, u, p, c, fixed, nb, wp, &ws, &var);
+    for (int i = 0; i < n; ++i) atomicAdd(&indeg[nb[i]], 1);
+}
+
+// Wave-aggregated append of tiled indices into a list.
+__device__ inline void adj_push(long long q, int lane, long long *__restrict__ list, int *__restrict__ count) {
+    const bool want = q >= 0;
+    const unsigned long long mask = __ballot(want);
+    if (mask == 0) return;
+    int base = 0;
+    if (lane == 0) base = atomicAdd(count, __popcll(mask));
+    base = __shfl(base, 0);
+    if (want) {
+        const unsigned long long below = lane == 0 ? 0ull : (mask & (~0ull >> (64 - lane)));
+        list[base + __popcll(below)] = q;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_adj_frontier(Geom g, const int *__restrict__ indeg, long long np,
+                                                      long long *__restrict__ list, int *__restrict__ count) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;  // np is a multiple of 64: whole waves leave together
+    int c[3];
+    tiled_coords(g, (int)(p >> 6), (int)(p & 63), c);
+    const bool start = in_grid(g, c) && indeg[p] == 0;
+    adj_push(start ? p : -1, threadIdx.x & 63, list, count);
+}
+
+__global__ void k_adj_targets(const long long *__restrict__ tp, const double *__restrict__ tw, int n,
+                              double *__restrict__ lambda) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) lambda[tp[i]] = tw[i];
+}
+
+template <int ModelT, typename T>
+__global__ __launch_bounds__(256) void k_adj_round(Geom g, StencilData<T> sd, const T *__restrict__ u,
+                                                   const unsigned long long *__restrict__ fixedmask,
+                                                   int *__restrict__ indeg, double *__restrict__ lambda,
+                                                   double *__restrict__ sens, const long long *__restrict__ list_in,
+                                                   const int *__restrict__ count_in, long long *__restrict__ list_out,
+                                                   int *__restrict__ count_out, int *__restrict__ count_reset,
+                                                   long long *__restrict__ seeds_idx, double *__restrict__ seeds_w,
+                                                   int *__restrict__ nseeds, int max_seeds) {
+    constexpr int NC = ModelShape<ModelT>::NFWD + ModelShape<ModelT>::NSYM;
+    const int n = *count_in;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *count_reset = 0;
+    const int lane = threadIdx.x & 63;
+    // Uniform trip count per wave so that the wave-level pushes see every lane.
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (int base = wave * 64; base < n; base += nwaves * 64) {
+        const int i = base + lane;
+        long long nb[NC];
+        double wp[NC], ws = 0., var = 0.;
+        int na = 0;
+        double lam = 0.;
+        if (i < n) {
+            const long long p = list_in[i];
+            int c[3];
+            tiled_coords(g, (int)(p >> 6), (int)(p & 63), c);
+            const bool fixed = (fixedmask[p >> 6] >> (p & 63)) & 1ull;
+            na = active_set<ModelT, T>(g, sd, u, p, c, fixed, nb, wp, &ws, &var);
+            lam = lambda[p];
+            if (lam != 0.) {
+                sens[p] = lam * var;
+                if (na == 0) {
+                    const int s = atomicAdd(nseeds, 1);
+                    if (s < max_seeds) {
+                        seeds_idx[s] = p;
+                        seeds_w[s] = lam;
+                    }
+                } else if (ws > 0.) {
+                    const double inv = 1. / ws;
+                    for (int k = 0; k < na; ++k) atomicAdd(&lambda[nb[k]], lam * (wp[k] * inv));
+                }
+            }
+        }
+        __threadfence();
+        for (int k = 0; k < NC; ++k) {
+            long long q = -1;
+            if (k < na && atomicSub(&indeg[nb[k]], 1) == 1) q = nb[k];
+            adj_push(q, lane, list_out, count_out);
+        }
+    }
+}
+
+__global__ void k_export_sens(Geom g, const double *__restrict__ s, double *__restrict__ out, long long np) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    int c[3];
+    tiled_coords(g, (int)(p >> 6), (int)(p & 63), c);
+    if (!in_grid(g, c)) return;
+    out[((long long)c[0] * g.n[1] + c[1]) * g.n[2] + c[2]] = s[p];
+}
+
+}  // namespace hfm
+
+using namespace hfm;
+
+namespace {
+
+template <int M, typename T> void adjoint_run(hfm_plan *p, const std::vector<long long> &tp,
+                                              const std::vector<double> &tw, double *d_out, hipStream_t s,
+                                              std::vector<long long> &sidx, std::vector<double> &sw, int max_seeds,
+                                              int *nseed_out) {
+    const long long np = p->np;
+    const unsigned nb = (unsigned)((np + 255) / 256);
+    const T *u = static_cast<const T *>(p->u);
+    HIPCHECK(hipMemsetAsync(p->adj_indeg, 0, sizeof(int) * np, s));
+    hipLaunchKernelGGL((k_adj_prepare<M, T>), dim3(nb), dim3(256), 0, s, p->g, p->sd<T>(), u, p->fixedmask,
+                       p->adj_indeg, p->adj_lambda, p->adj_sens, np);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(p->adj_tp, tp.data(), tp.size() * sizeof(long long), hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(p->adj_tw, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    if (!tp.empty()) {
+        hipLaunchKernelGGL(k_adj_targets, dim3((unsigned)((tp.size() + 63) / 64)), dim3(64), 0, s, p->adj_tp,
+                           p->adj_tw, (int)tp.size(), p->adj_lambda);
+        HIPCHECK(hipGetLastError());
+    }
+    HIPCHECK(hipMemsetAsync(p->adj_counts, 0, 4 * sizeof(int), s));
+    hipLaunchKernelGGL(k_adj_frontier, dim3(nb), dim3(256), 0, s, p->g, p->adj_indeg, np, p->adj_list[0],
+                       &p->adj_counts[0]);
+    HIPCHECK(hipGetLastError());
+    const unsigned grid = (unsigned)std::min<long long>((np + 255) / 256, 2048);
+    long long k = 0;
+    const long long max_rounds = np + 16;
+    while (true) {
+        for (int b = 0; b < 8; ++b, ++k) {
+            hipLaunchKernelGGL((k_adj_round<M, T>), dim3(grid), dim3(256), 0, s, p->g, p->sd<T>(), u, p->fixedmask,
+                               p->adj_indeg, p->adj_lambda, p->adj_sens, p->adj_list[k & 1], &p->adj_counts[k % 3],
+                               p->adj_list[(k + 1) & 1], &p->adj_counts[(k + 1) % 3], &p->adj_counts[(k + 2) % 3],
+                               p->adj_seed_idx, p->adj_seed_w, &p->adj_counts[3], max_seeds);
+            HIPCHECK(hipGetLastError());
+        }
+        HIPCHECK(hipMemcpyAsync(p->h_count, &p->adj_counts[k % 3], sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        if (*p->h_count == 0) break;
+        if (k > max_rounds) throw HipFail{"adjoint sweep did not terminate"};
+    }
+    p->st.adjoint_rounds = k;
+    hipLaunchKernelGGL(k_export_sens, dim3(nb), dim3(256), 0, s, p->g, p->adj_sens, d_out, np);
+    HIPCHECK(hipGetLastError());
+    int ns = 0;
+    HIPCHECK(hipMemcpyAsync(&ns, &p->adj_counts[3], sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    const int nget = std::min(ns, max_seeds);
+    std::vector<long long> ti(nget);
+    std::vector<double> tv(nget);
+    if (nget > 0) {
+        HIPCHECK(hipMemcpy(ti.data(), p->adj_seed_idx, nget * sizeof(long long), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(tv.data(), p->adj_seed_w, nget * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    // Tiled -> row-major index; reported in the reference's processing order: decreasing
+    // (value, index) (BackwardVariation pops the largest key of its (value, index) map).
+    std::vector<std::pair<std::pair<double, long long>, double>> keyed;
+    for (int i = 0; i < nget; ++i) {
+        int c[3];
+        tiled_coords(p->g, (int)(ti[i] >> 6), (int)(ti[i] & 63), c);
+        T uv;
+        HIPCHECK(hipMemcpy(&uv, u + ti[i], sizeof(T), hipMemcpyDeviceToHost));
+        keyed.push_back({{(double)uv, ((long long)c[0] * p->g.n[1] + c[1]) * p->g.n[2] + c[2]}, tv[i]});
+    }
+    std::sort(keyed.begin(), keyed.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+    sidx.clear();
+    sw.clear();
+    for (auto &pr : keyed) {
+        sidx.push_back(pr.first.second);
+        sw.push_back(pr.second);
+    }
+    *nseed_out = ns;
+}
+
+}  // namespace
+
+int hfm_plan_adjoint_impl(hfm_plan *p, int ntargets, const double *targets, const double *weights,
+                          double *d_sensitivity, int64_t *seed_index, double *seed_weight, int max_seeds,
+                          int *nseed_out, void *stream) {
+    if (!p || !d_sensitivity || (ntargets > 0 && !targets) || !nseed_out || (max_seeds > 0 && (!seed_index || !seed_weight))) {
+        set_error("hfm_plan_adjoint: null argument");
+        return 1;
+    }
+    if (!p->solved) {
+        set_error("hfm_plan_adjoint: no solve has run");
+        return 1;
+    }
+    try {
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        // Targets: ADim + floor; out of range is an error; a repeated index keeps its first
+        // weight (std::map::insert in BackwardVariation, FirstVariation.h:351-354).
+        std::map<long long, std::pair<long long, double>> seen;
+        std::vector<long long> tp;
+        std::vector<double> tw;
+        const int dim = p->g.dim;
+        for (int t = 0; t < ntargets; ++t) {
+            int c[3] = {0, 0, 0};
+            for (int i = 0; i < dim; ++i) {
+                double q;
+                if (p->shared && i == 2) q = (targets[t * dim + i] - p->origin[i]) / p->dependScale;
+                else q = (targets[t * dim + i] - p->origin[i]) / p->h[i];
+                const double f = std::floor(q);
+                if (!(f > -2e9 && f < 2e9)) throw HipFail{"Error : inspectSensitivity data points are out of range"};
+                int ci = (int)f;
+                if (p->g.periodic >> i & 1) {
+                    ci %= p->g.n[i];
+                    if (ci < 0) ci += p->g.n[i];
+                } else if (ci < 0 || ci >= p->g.n[i]) {
+                    throw HipFail{"Error : inspectSensitivity data points are out of range"};
+                }
+                c[i] = ci;
+            }
+            const long long lin = ((long long)c[0] * p->g.n[1] + c[1]) * p->g.n[2] + c[2];
+            const long long tpi = (long long)tile_of(p->g, c) * 64 + local_of(p->g, c);
+            if (seen.insert({lin, {tpi, weights ? weights[t] : 1.}}).second) {
+                tp.push_back(tpi);
+                tw.push_back(weights ? weights[t] : 1.);
+            }
+        }
+        if (!p->adj_lambda) {
+            p->adj_lambda = dalloc<double>(p->np);
+            p->adj_sens = dalloc<double>(p->np);
+            p->adj_indeg = dalloc<int>(p->np);
+            p->adj_list[0] = dalloc<long long>(p->np);
+            p->adj_list[1] = dalloc<long long>(p->np);
+            p->adj_counts = dalloc<int>(4);
+        }
+        const int cap = std::max(max_seeds, 1);
+        if (cap > p->adj_seed_cap) {
+            dfree(p->adj_seed_idx);
+            dfree(p->adj_seed_w);
+            p->adj_seed_idx = nullptr;
+            p->adj_seed_w = nullptr;
+            p->adj_seed_idx = dalloc<long long>(cap);
+            p->adj_seed_w = dalloc<double>(cap);
+            p->adj_seed_cap = cap;
+        }
+        if ((int)tp.size() > p->adj_t_cap) {
+            dfree(p->adj_tp);
+            dfree(p->adj_tw);
+            p->adj_tp = nullptr;
+            p->adj_tw = nullptr;
+            p->adj_t_cap = std::max<int>((int)tp.size(), 64);
+            p->adj_tp = dalloc<long long>(p->adj_t_cap);
+            p->adj_tw = dalloc<double>(p->adj_t_cap);
+        }
+        std::vector<long long> sidx;
+        std::vector<double> sw;
+        int ns = 0;
+        const bool f32 = p->precision == 32;
+        switch (p->model) {
+            case MODEL_RIEMANN2:
+                if (f32) adjoint_run<MODEL_RIEMANN2, float>(p, tp, tw, d_sensitivity, s, sidx, sw, cap, &ns);
+                else adjoint_run<MODEL_RIEMANN2, double>(p, tp, tw, d_sensitivity, s, sidx, sw, cap, &ns);
+                break;
+            case MODEL_RIEMANN3:
+                if (f32) adjoint_run<MODEL_RIEMANN3, float>(p, tp, tw, d_sensitivity, s, sidx, sw, cap, &ns);
+                else adjoint_run<MODEL_RIEMANN3, double>(p, tp, tw, d_sensitivity, s, sidx, sw, cap, &ns);
+                break;
+            case MODEL_RSF2:
+                if (f32) adjoint_run<MODEL_RSF2, float>(p, tp, tw, d_sensitivity, s, sidx, sw, cap, &ns);
+                else adjoint_run<MODEL_RSF2, double>(p, tp, tw, d_sensitivity, s, sidx, sw, cap, &ns);
+                break;
+            case MODEL_ELASTICA2:
+                if (f32) adjoint_run<MODEL_ELASTICA2, float>(p, tp, tw, d_sensitivity, s, sidx, sw, cap, &ns);
+                else adjoint_run<MODEL_ELASTICA2, double>(p, tp, tw, d_sensitivity, s, sidx, sw, cap, &ns);
+                break;
+        }
+        for (size_t i = 0; i < sidx.size() && (int)i < max_seeds; ++i) {
+            seed_index[i] = sidx[i];
+            seed_weight[i] = sw[i];
+        }
+        *nseed_out = ns;
+    } catch (const HipFail &e) {
+        set_error(e.msg);
+        return 1;
+    }
+    return 0;
 }

@@ -161,6 +161,91 @@ __device__ inline T local_solve(const T (&v)[NC], const T (&w)[NC], uint32_t &us
     return r;
 }
 
+// Per-lane stencil access, shared by the tile kernels.
+template <int ModelT, typename T> struct LaneStencil {
+    using S = ModelShape<ModelT>;
+    static constexpr int NC = S::NFWD + S::NSYM;
+    uint32_t off[NC];
+    T w[NC];
+
+    __device__ void load(const Geom &g, const StencilData<T> &sd, long long p, const int *c) {
+        if (!S::SHARED) {
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                off[k] = sd.offs[k * sd.np + p];
+                w[k] = sd.w[k * sd.np + p];
+            }
+        } else {
+            int th = c[2] < g.n[2] ? c[2] : g.n[2] - 1;
+            const T s = sd.speed[p];
+            const T s2 = s * s;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                off[k] = sd.tab_offs[th * NC + k];
+                w[k] = (T)sd.tab_w[th * NC + k] * s2;
+            }
+        }
+    }
+};
+
+// Local solve used by the update kernel.  FP64: exactly local_solve (the reference's order of
+// operations, so converged values reproduce the reference's FP64 values).  FP32: the same
+// candidates in the same order, but the test "next v below the current root" is evaluated
+// without the root as f(v) < 1, f(t) = sum_accepted w (t - v_i)^2 (f is increasing above the
+// accepted values and f(root) = 1), so the square root and division run once per solve.
+template <int NC, typename T> __device__ inline T local_solve_update(const T (&v)[NC], const T (&w)[NC]) {
+    uint32_t used;
+    return local_solve<NC, T>(v, w, used);
+}
+template <int NC> __device__ inline float local_solve_update(const float (&v)[NC], const float (&w)[NC]) {
+    const float inf = __builtin_huge_valf();
+    float m = 0.f, a = 0.f, b = 0.f, c = 0.f;
+    uint32_t sel = 0;
+    bool any = false;
+    for (int step = 0; step < NC; ++step) {
+        float bv = inf, bw = 0.f;
+        int bk = -1;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            const bool ok = !((sel >> k) & 1u) && v[k] < bv;
+            bv = ok ? v[k] : bv;
+            bw = ok ? w[k] : bw;
+            bk = ok ? k : bk;
+        }
+        if (bk < 0) break;
+        if (step == 0) {
+            m = bv;
+        } else {
+            const float t = bv - m;
+            if (!(t * (a * t - 2.f * b) + c < 1.f)) break;
+        }
+        const float vr = bv - m;
+        const float wv = bw * vr;
+        a += bw;
+        b += wv;
+        c += wv * vr;
+        sel |= 1u << bk;
+        any = true;
+    }
+    if (!any) return inf;
+    const float cc = c - 1.f;
+    const float d2 = b * b - a * cc;
+    return m + (b + sqrtf(d2 > 0.f ? d2 : 0.f)) / a;
+}
+
+// Wave-reduced accumulation of (sum, count) into tau_acc[0..1]; every lane of the wave calls it.
+__device__ inline void wave_add_tau(double v, double c, double *tau_acc) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        v += __shfl_xor(v, o);
+        c += __shfl_xor(c, o);
+    }
+    if ((threadIdx.x & 63) == 0 && c > 0.) {
+        atomicAdd(&tau_acc[0], v);
+        atomicAdd(&tau_acc[1], c);
+    }
+}
+
 }  // namespace hfm
 
 #endif

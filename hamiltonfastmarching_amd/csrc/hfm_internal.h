@@ -64,9 +64,15 @@ struct hfm_plan {
     unsigned long long *fixedmask = nullptr;
     int *dep_cnt = nullptr, *dep_start = nullptr, *dep_fill = nullptr, *deps = nullptr;
     long long deps_cap = 0, ndeps = 0;
-    int *lists[2] = {nullptr, nullptr};
-    int *flags[2] = {nullptr, nullptr};
-    int *counts = nullptr;  // 3 rotating counters
+    int *lists[1] = {nullptr};     // tiles listed by the current round
+    int *counts = nullptr;         // 2 alternating list counters
+    unsigned *keys = nullptr;      // per-tile scheduling key (KEY_NONE: not pending)
+    unsigned *tmax = nullptr;      // per-tile largest value (as key) after its last visit
+    unsigned *nextmin = nullptr;   // 2 alternating smallest pending keys
+    float *bounds = nullptr;       // 2 alternating band bounds
+    double *tau_acc = nullptr;     // (sum, count) of one-step times
+    double tau = 0.;               // mean one-step time of the current stencils
+    double delta_factor = 8.;      // band width in units of tau (<= 0: unordered)
     int *status = nullptr;  // stencil status
     unsigned long long *stats = nullptr;
     long long *seed_p = nullptr;
@@ -75,6 +81,14 @@ struct hfm_plan {
     // pinned host mirrors
     int *h_count = nullptr;
     unsigned long long *h_stats = nullptr;
+
+    // adjoint scratch (allocated on the first hfm_plan_adjoint call)
+    double *adj_lambda = nullptr, *adj_sens = nullptr;
+    int *adj_indeg = nullptr, *adj_counts = nullptr;
+    long long *adj_list[2] = {nullptr, nullptr};
+    long long *adj_seed_idx = nullptr, *adj_tp = nullptr;
+    double *adj_seed_w = nullptr, *adj_tw = nullptr;
+    int adj_seed_cap = 0, adj_t_cap = 0;
 
     bool stencil_ready = false;
     bool solved = false;

@@ -19,6 +19,21 @@ struct FlowScale {
     double h[3];
 };
 
+constexpr unsigned KEY_NONE = 0xffffffffu;
+constexpr int PEND_SLOTS = 32;  // spread of the per-round minimum-key and statistics counters
+
+// One scheduling round (see k_schedule): reads the pending minimum and the previous bound,
+// writes this round's bound, lists the tiles it takes, accumulates the keys it leaves.
+struct SchedRound {
+    const unsigned *pend_in;   // PEND_SLOTS words
+    unsigned *pend_out;        // PEND_SLOTS words
+    const float *bound_in;
+    float *bound_out;
+    int *list;
+    int *count;
+    float delta;  // band width; <= 0 lists every pending tile
+};
+
 template <typename T> struct FimArgs {
     Geom g;
     StencilData<T> sd;
@@ -26,29 +41,30 @@ template <typename T> struct FimArgs {
     const unsigned long long *fixedmask;
     const int *dep_start;
     const int *deps;
-    const int *list_in;
-    const int *count_in;
-    int *list_out;
-    int *count_out;
+    const int *list;
+    const int *count;
     int *count_reset;
-    int *flag_in;
-    int *flag_out;
-    unsigned long long *stats;  // [0] tile visits, [1] sweeps (summed over visits)
+    unsigned *keys;
+    unsigned *tmax;
+    unsigned *pend_out;
+    unsigned *pend_reset;
+    unsigned long long *stats;  // PEND_SLOTS pairs: (tiles processed, sweeps summed)
     unsigned grid;
 };
 
 template <typename T>
 hipError_t launch_riemann_stencil(const Geom &g, const double *metric, int is_field, int dual, const InvHH &ih,
-                                  uint32_t *offs, T *w, long long np, int *status, hipStream_t s);
+                                  uint32_t *offs, T *w, long long np, int *status, double *tau_acc, hipStream_t s);
 template <typename T>
 hipError_t launch_tile_deps(int model, const Geom &g, const StencilData<T> &sd, int pass, int *cnt, const int *start,
                             int *fill, int *deps, int *ovf, hipStream_t s);
 hipError_t launch_exclusive_scan(const int *cnt, int *start, int n, hipStream_t s);
 template <typename T>
-hipError_t launch_init(const Geom &g, T *u, unsigned long long *fixedmask, int *f0, int *f1, long long np,
-                       const long long *seed_p, const double *seed_v, int nseeds, hipStream_t s);
-hipError_t launch_seed_activate(const long long *seed_p, int nseeds, const int *dep_start, const int *deps,
-                                int *flag_out, int *list_out, int *count_out, hipStream_t s);
+hipError_t launch_init(const Geom &g, T *u, unsigned long long *fixedmask, unsigned *keys, unsigned *tmax,
+                       long long np, const long long *seed_p, const double *seed_v, int nseeds, hipStream_t s);
+hipError_t launch_seed_activate(const long long *seed_p, const double *seed_v, int nseeds, const int *dep_start,
+                                const int *deps, unsigned *keys, const unsigned *tmax, unsigned *pend, hipStream_t s);
+hipError_t launch_schedule(int ntiles, unsigned *keys, const SchedRound &sr, hipStream_t s);
 template <typename T> hipError_t launch_fim(int model, const FimArgs<T> &a, hipStream_t s);
 template <typename T>
 hipError_t launch_export_values(const Geom &g, const T *u, double *out, long long np, hipStream_t s);

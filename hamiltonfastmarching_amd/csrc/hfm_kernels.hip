@@ -22,75 +22,56 @@ namespace hfm {
 template <int D, typename T>
 __global__ __launch_bounds__(256) void k_riemann_stencil(Geom g, const double *__restrict__ metric, int metric_is_field,
                                                          int dual, InvHH ih, uint32_t *__restrict__ offs,
-                                                         T *__restrict__ w, long long np, int *__restrict__ status) {
+                                                         T *__restrict__ w, long long np, int *__restrict__ status,
+                                                         double *__restrict__ tau_acc) {
     constexpr int K = D * (D + 1) / 2;
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     int c[3];
     tiled_coords(g, (int)(p >> 6), (int)(p & 63), c);
+    double tau_v = 0., tau_c = 0.;
     if (!in_grid(g, c)) {
         for (int k = 0; k < K; ++k) {
             offs[k * np + p] = 0;
             w[k * np + p] = T(0);
         }
-        return;
-    }
-    const long long lin = ((long long)c[0] * g.n[1] + c[1]) * g.n[2] + c[2];
-    double mc[K];
-    const double *src = metric + (metric_is_field ? lin * K : 0);
-    for (int k = 0; k < K; ++k) mc[k] = src[k];
-    int8_t o8[K][D];
-    double wd[K];
-    int mx = 0;
-    const int st = riemann_point_stencil<D>(mc, dual != 0, ih.v, o8, wd, &mx);
-    for (int k = 0; k < K; ++k) {
-        offs[k * np + p] = pack_offset(o8[k][0], D > 1 ? o8[k][1] : 0, D > 2 ? o8[k][2] : 0);
-        w[k * np + p] = (T)wd[k];
-    }
-    if (st != SELLING_OK) {
-        // status[0]: error code, status[1]: smallest failing row-major index
-        atomicMax(&status[0], st);
-        atomicMin(&status[1], (int)lin);
-    }
-    // status[2]: max L1 norm of a nonzero-weight offset (MaxStencilWidth, HFMInterface.hxx:357)
-    int l1max = 0;
-    for (int k = 0; k < K; ++k) {
-        if (wd[k] == 0.) continue;
-        int l1 = 0;
-        for (int q = 0; q < D; ++q) l1 += o8[k][q] < 0 ? -o8[k][q] : o8[k][q];
-        l1max = l1 > l1max ? l1 : l1max;
-    }
-    atomicMax(&status[2], l1max);
-}
-
-// ---------------------------------------------------------------------------------------------
-// Per-lane stencil access, shared by the tile kernels.
-// ---------------------------------------------------------------------------------------------
-template <int ModelT, typename T> struct LaneStencil {
-    using S = ModelShape<ModelT>;
-    static constexpr int NC = S::NFWD + S::NSYM;
-    uint32_t off[NC];
-    T w[NC];
-
-    __device__ void load(const Geom &g, const StencilData<T> &sd, long long p, const int *c) {
-        if (!S::SHARED) {
-#pragma unroll
-            for (int k = 0; k < NC; ++k) {
-                off[k] = sd.offs[k * sd.np + p];
-                w[k] = sd.w[k * sd.np + p];
-            }
-        } else {
-            int th = c[2] < g.n[2] ? c[2] : g.n[2] - 1;
-            const T s = sd.speed[p];
-            const T s2 = s * s;
-#pragma unroll
-            for (int k = 0; k < NC; ++k) {
-                off[k] = sd.tab_offs[th * NC + k];
-                w[k] = (T)sd.tab_w[th * NC + k] * s2;
-            }
+    } else {
+        const long long lin = ((long long)c[0] * g.n[1] + c[1]) * g.n[2] + c[2];
+        double mc[K];
+        const double *src = metric + (metric_is_field ? lin * K : 0);
+        for (int k = 0; k < K; ++k) mc[k] = src[k];
+        int8_t o8[K][D];
+        double wd[K];
+        int mx = 0;
+        const int st = riemann_point_stencil<D>(mc, dual != 0, ih.v, o8, wd, &mx);
+        for (int k = 0; k < K; ++k) {
+            offs[k * np + p] = pack_offset(o8[k][0], D > 1 ? o8[k][1] : 0, D > 2 ? o8[k][2] : 0);
+            w[k * np + p] = (T)wd[k];
+        }
+        if (st != SELLING_OK) {
+            // status[0]: error code, status[1]: smallest failing row-major index
+            atomicMax(&status[0], st);
+            atomicMin(&status[1], (int)lin);
+        }
+        // status[2]: max L1 norm of a nonzero-weight offset (MaxStencilWidth, HFMInterface.hxx:357)
+        int l1max = 0;
+        for (int k = 0; k < K; ++k) {
+            if (wd[k] == 0.) continue;
+            int l1 = 0;
+            for (int q = 0; q < D; ++q) l1 += o8[k][q] < 0 ? -o8[k][q] : o8[k][q];
+            l1max = l1 > l1max ? l1 : l1max;
+        }
+        atomicMax(&status[2], l1max);
+        // Mean one-step time 1/sqrt(sum_k w_k): sets the width of the scheduling bands.
+        double sw = 0.;
+        for (int k = 0; k < K; ++k) sw += wd[k];
+        if (sw > 0.) {
+            tau_v = 1. / sqrt(sw);
+            tau_c = 1.;
         }
     }
-};
+    wave_add_tau(tau_v, tau_c, tau_acc);
+}
 
 // ---------------------------------------------------------------------------------------------
 // Tile dependency graph.  For tile T, its sources are the tiles holding x +- e_k for the
@@ -179,7 +160,7 @@ __global__ __launch_bounds__(1024) void k_exclusive_scan(const int *__restrict__
 // ---------------------------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_init(Geom g, T *__restrict__ u, unsigned long long *__restrict__ fixedmask,
-                                              int *__restrict__ flags0, int *__restrict__ flags1, long long np) {
+                                              unsigned *__restrict__ keys, unsigned *__restrict__ tmax, long long np) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     u[p] = dev_inf<T>();
@@ -189,8 +170,8 @@ __global__ __launch_bounds__(256) void k_init(Geom g, T *__restrict__ u, unsigne
     if ((threadIdx.x & 63) == 0) {
         const int tile = (int)(p >> 6);
         fixedmask[tile] = pad;
-        flags0[tile] = 0;
-        flags1[tile] = 0;
+        keys[tile] = KEY_NONE;
+        tmax[tile] = KEY_NONE;
     }
 }
 
@@ -204,68 +185,154 @@ __global__ void k_set_seeds(const long long *__restrict__ seed_p, const double *
     atomicOr(&fixedmask[p >> 6], 1ull << (p & 63));
 }
 
-// Wave-aggregated push of candidate tile ids (d < 0: none) into the next list.
-__device__ inline void push_tiles(int d, int lane, int *__restrict__ flag_out, int *__restrict__ list_out,
-                                  int *__restrict__ count_out) {
-    const bool want = d >= 0 && atomicExch(&flag_out[d], 1) == 0;
-    const unsigned long long mask = __ballot(want);
-    if (mask == 0) return;
-    int base = 0;
-    if (lane == 0) base = atomicAdd(count_out, __popcll(mask));
-    base = __shfl(base, 0);
-    if (want) {
-        const unsigned long long below = lane == 0 ? 0ull : (mask & (~0ull >> (64 - lane)));
-        list_out[base + __popcll(below)] = d;
+// Scheduling keys: float values mapped to order-preserving uint32 so that atomicMin works for
+// negative seed values too.  KEY_NONE (0xffffffff) means "nothing pending".
+__device__ inline unsigned key_of(float f) {
+    const unsigned b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ inline float key_value(unsigned k) {
+    if (k == KEY_NONE) return __builtin_huge_valf();
+    const unsigned b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    return __uint_as_float(b);
+}
+__device__ inline unsigned wave_min_u32(unsigned v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned w = (unsigned)__shfl_xor((int)v, o);
+        v = w < v ? w : v;
     }
+    return v;
+}
+__device__ inline unsigned wave_max_u32(unsigned v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned w = (unsigned)__shfl_xor((int)v, o);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+__device__ inline unsigned relaxed_load(const unsigned *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Seed tiles and their dependents form the first active list.
-__global__ __launch_bounds__(64) void k_seed_activate(const long long *__restrict__ seed_p, int nseeds,
+// Mark tile d (d < 0: none) pending with key `key`: lowers keys[d] unless the tile's largest
+// value is already <= key (then no point of d can use a value >= key).  Returns the smallest key
+// marked by the wave (KEY_NONE if none); every lane of the wave calls it.
+__device__ inline unsigned mark_pending(int d, unsigned key, unsigned *__restrict__ keys,
+                                        const unsigned *__restrict__ tmax) {
+    bool lowered = false;
+    if (d >= 0 && relaxed_load(&tmax[d]) > key && relaxed_load(&keys[d]) > key) {
+        atomicMin(&keys[d], key);
+        lowered = true;
+    }
+    return wave_min_u32(lowered ? key : KEY_NONE);
+}
+
+// The smallest pending key is accumulated in PEND_SLOTS words (one per wave id modulo
+// PEND_SLOTS) so that no single address takes an atomic from every wave.
+__device__ inline unsigned read_pend(const unsigned *slots) {
+    unsigned m = KEY_NONE;
+    for (int i = 0; i < PEND_SLOTS; ++i) {
+        const unsigned v = slots[i];
+        m = v < m ? v : m;
+    }
+    return m;
+}
+
+// Seed tiles and their dependents are the first pending tiles, keyed by the seed value.
+__global__ __launch_bounds__(64) void k_seed_activate(const long long *__restrict__ seed_p,
+                                                      const double *__restrict__ seed_v, int nseeds,
                                                       const int *__restrict__ dep_start, const int *__restrict__ deps,
-                                                      int *__restrict__ flag_out, int *__restrict__ list_out,
-                                                      int *__restrict__ count_out) {
+                                                      unsigned *__restrict__ keys, const unsigned *__restrict__ tmax,
+                                                      unsigned *__restrict__ pend_slots) {
     const int lane = threadIdx.x;
+    unsigned wmin = KEY_NONE;
     for (int i = 0; i < nseeds; ++i) {
         const int tile = (int)(seed_p[i] >> 6);
-        push_tiles(lane == 0 ? tile : -1, lane, flag_out, list_out, count_out);
+        const unsigned key = key_of((float)seed_v[i]);
+        unsigned m = mark_pending(lane == 0 ? tile : -1, key, keys, tmax);
+        wmin = m < wmin ? m : wmin;
         const int b = dep_start[tile], e = dep_start[tile + 1];
         for (int j0 = b; j0 < e; j0 += 64) {
             const int j = j0 + lane;
-            push_tiles(j < e ? deps[j] : -1, lane, flag_out, list_out, count_out);
+            m = mark_pending(j < e ? deps[j] : -1, key, keys, tmax);
+            wmin = m < wmin ? m : wmin;
         }
+    }
+    if (lane == 0 && wmin != KEY_NONE) atomicMin(&pend_slots[0], wmin);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Round scheduling, ordered by value bands.
+//
+// Every tile has a key: KEY_NONE, or a lower bound of the values its next visit can produce
+// (the smallest new value among the changes that made it pending).  Round k lists the pending
+// tiles whose key lies below the band bound B_k and consumes their keys; B_k stays at B_{k-1}
+// while a pending key lies below it, and otherwise moves to (smallest pending key + delta).
+// delta <= 0 lists every pending tile.  The ordering decides when work is done, never what the
+// fixed point is.  The smallest pending key of round k+1 is accumulated in pend[(k+1)&1] by the
+// keys this kernel leaves pending and by the marks of the update kernel of round k.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_schedule(int ntiles, unsigned *__restrict__ keys, SchedRound sr) {
+    const float pend = key_value(read_pend(sr.pend_in));
+    const float bprev = *sr.bound_in;
+    float B = __builtin_huge_valf();
+    if (sr.delta > 0.f && pend < __builtin_huge_valf()) B = pend < bprev ? bprev : pend + sr.delta;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *sr.bound_out = B;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    unsigned k = KEY_NONE;
+    if (t < ntiles) k = keys[t];
+    const bool take = k != KEY_NONE && (key_value(k) < B || !(B < __builtin_huge_valf()));
+    const unsigned left = (!take && k != KEY_NONE) ? k : KEY_NONE;
+    if (take) keys[t] = KEY_NONE;
+    const unsigned lmin = wave_min_u32(left);
+    if (lane == 0 && lmin != KEY_NONE) atomicMin(&sr.pend_out[(t >> 6) & (PEND_SLOTS - 1)], lmin);
+    const unsigned long long mask = __ballot(take);
+    if (mask == 0) return;
+    int base = 0;
+    if (lane == 0) base = atomicAdd(sr.count, __popcll(mask));
+    base = __shfl(base, 0);
+    if (take) {
+        const unsigned long long below = lane == 0 ? 0ull : (mask & (~0ull >> (64 - lane)));
+        sr.list[base + __popcll(below)] = t;
     }
 }
 
 // ---------------------------------------------------------------------------------------------
-// One round of the block fast-iterative method.
+// Update kernel of one round of the block fast-iterative method.
 //
-// Each wave takes a tile from the active list, stages the tile's values in LDS, caches the
-// values of neighbours outside the tile in registers, and runs Jacobi sweeps of the local solve
-// (values only decrease) until no lane changes (wave ballot) or MAXSWEEP sweeps.  If the tile
-// changed, its dependent tiles are appended to the next list; if it did not converge it
-// re-appends itself.  Values of other tiles may be read while they are being lowered by another
-// wave: any value read is an upper bound of the solution and the scheme is monotone, and the
-// writer re-activates its dependents, so the fixed point is unaffected.
+// Each wave takes a listed tile, stages its values in LDS (one wavefront, 64 points), caches
+// the values of neighbours outside the tile in registers, and runs Jacobi sweeps of the local
+// solve until no lane changes (wave ballot) or MAXSWEEP sweeps.  If the tile changed, its
+// dependent tiles are marked pending with key = smallest lowered value (the tile itself too if
+// it did not converge).  Values of other tiles may be read while another wave lowers them: any
+// value read is an upper bound of the solution and the scheme is monotone, and the writer marks
+// its dependents pending, so the fixed point is unaffected.
 // ---------------------------------------------------------------------------------------------
 template <int ModelT, typename T, int MAXSWEEP>
 __global__ __launch_bounds__(256) void k_fim(Geom g, StencilData<T> sd, T *__restrict__ u,
                                              const unsigned long long *__restrict__ fixedmask,
                                              const int *__restrict__ dep_start, const int *__restrict__ deps,
-                                             const int *__restrict__ list_in, const int *__restrict__ count_in,
-                                             int *__restrict__ list_out, int *__restrict__ count_out,
-                                             int *__restrict__ count_reset, int *__restrict__ flag_in,
-                                             int *__restrict__ flag_out, unsigned long long *__restrict__ stats) {
+                                             const int *__restrict__ list, const int *__restrict__ count,
+                                             int *__restrict__ count_reset, unsigned *__restrict__ keys,
+                                             unsigned *__restrict__ tmax, unsigned *__restrict__ pend_out,
+                                             unsigned *__restrict__ pend_reset, unsigned long long *__restrict__ stats) {
     using S = ModelShape<ModelT>;
     constexpr int NF = S::NFWD, NS = S::NSYM, NC = NF + NS;
     __shared__ T su[4][64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const T inf = dev_inf<T>();
-    const int n = *count_in;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *count_reset = 0;
+    const int n = *count;
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0) *count_reset = 0;
+        if (threadIdx.x < PEND_SLOTS) pend_reset[threadIdx.x] = KEY_NONE;
+    }
     unsigned long long my_sweeps = 0, my_visits = 0;
+    unsigned wmin = KEY_NONE;
     for (int i = blockIdx.x * 4 + wv; i < n; i += gridDim.x * 4) {
-        const int tile = list_in[i];
-        if (lane == 0) flag_in[tile] = 0;
+        const int tile = list[i];
         const long long p = (long long)tile * 64 + lane;
         int c[3];
         tiled_coords(g, tile, lane, c);
@@ -313,8 +380,7 @@ __global__ __launch_bounds__(256) void k_fim(Geom g, StencilData<T> sd, T *__res
                 const T vb = lb[k] >= 0 ? su[wv][lb[k]] : ob[k];
                 v[k] = va < vb ? va : vb;
             }
-            uint32_t used;
-            const T r = local_solve<NC, T>(v, st.w, used);
+            const T r = local_solve_update<NC, T>(v, st.w);
             const bool ch = !fixed && r < ucur;
             ucur = ch ? r : ucur;
             ++sweeps;
@@ -330,19 +396,28 @@ __global__ __launch_bounds__(256) void k_fim(Geom g, StencilData<T> sd, T *__res
         if (lowered) u[p] = ucur;
         const bool tile_changed = __ballot(lowered) != 0ull;
         if (tile_changed) {
+            const unsigned tm = wave_max_u32(key_of((float)ucur));
+            if (lane == 0) tmax[tile] = tm;
+            const unsigned kmin = wave_min_u32(lowered ? key_of((float)ucur) : KEY_NONE);
             const int b = dep_start[tile], e = dep_start[tile + 1];
             for (int j0 = b; j0 < e; j0 += 64) {
                 const int j = j0 + lane;
-                push_tiles(j < e ? deps[j] : -1, lane, flag_out, list_out, count_out);
+                const unsigned m = mark_pending(j < e ? deps[j] : -1, kmin, keys, tmax);
+                wmin = m < wmin ? m : wmin;
+            }
+            if (!converged) {
+                const unsigned m = mark_pending(lane == 0 ? tile : -1, kmin, keys, tmax);
+                wmin = m < wmin ? m : wmin;
             }
         }
-        if (!converged) push_tiles(lane == 0 ? tile : -1, lane, flag_out, list_out, count_out);
         my_sweeps += sweeps;
         my_visits += 1;
     }
+    const int slot = ((blockIdx.x * 4 + wv) & (PEND_SLOTS - 1));
+    if (lane == 0 && wmin != KEY_NONE) atomicMin(&pend_out[slot], wmin);
     if (lane == 0 && my_visits) {
-        atomicAdd(&stats[0], my_visits);
-        atomicAdd(&stats[1], my_sweeps);
+        atomicAdd(&stats[2 * slot], my_visits);
+        atomicAdd(&stats[2 * slot + 1], my_sweeps);
     }
 }
 
@@ -449,19 +524,19 @@ static inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + 
 
 template <typename T>
 hipError_t launch_riemann_stencil(const Geom &g, const double *metric, int is_field, int dual, const InvHH &ih,
-                                  uint32_t *offs, T *w, long long np, int *status, hipStream_t s) {
+                                  uint32_t *offs, T *w, long long np, int *status, double *tau_acc, hipStream_t s) {
     if (g.dim == 2)
         hipLaunchKernelGGL((k_riemann_stencil<2, T>), dim3(blocks_for(np, 256)), dim3(256), 0, s, g, metric, is_field,
-                           dual, ih, offs, w, np, status);
+                           dual, ih, offs, w, np, status, tau_acc);
     else
         hipLaunchKernelGGL((k_riemann_stencil<3, T>), dim3(blocks_for(np, 256)), dim3(256), 0, s, g, metric, is_field,
-                           dual, ih, offs, w, np, status);
+                           dual, ih, offs, w, np, status, tau_acc);
     return hipGetLastError();
 }
 template hipError_t launch_riemann_stencil<float>(const Geom &, const double *, int, int, const InvHH &, uint32_t *,
-                                                  float *, long long, int *, hipStream_t);
+                                                  float *, long long, int *, double *, hipStream_t);
 template hipError_t launch_riemann_stencil<double>(const Geom &, const double *, int, int, const InvHH &, uint32_t *,
-                                                   double *, long long, int *, hipStream_t);
+                                                   double *, long long, int *, double *, hipStream_t);
 
 template <int M, typename T>
 static void deps_pass(const Geom &g, const StencilData<T> &sd, int pass, int *cnt, const int *start, int *fill,
@@ -492,31 +567,35 @@ hipError_t launch_exclusive_scan(const int *cnt, int *start, int n, hipStream_t 
 }
 
 template <typename T>
-hipError_t launch_init(const Geom &g, T *u, unsigned long long *fixedmask, int *f0, int *f1, long long np,
-                       const long long *seed_p, const double *seed_v, int nseeds, hipStream_t s) {
-    hipLaunchKernelGGL((k_init<T>), dim3(blocks_for(np, 256)), dim3(256), 0, s, g, u, fixedmask, f0, f1, np);
+hipError_t launch_init(const Geom &g, T *u, unsigned long long *fixedmask, unsigned *keys, unsigned *tmax,
+                       long long np, const long long *seed_p, const double *seed_v, int nseeds, hipStream_t s) {
+    hipLaunchKernelGGL((k_init<T>), dim3(blocks_for(np, 256)), dim3(256), 0, s, g, u, fixedmask, keys, tmax, np);
     if (nseeds > 0)
         hipLaunchKernelGGL((k_set_seeds<T>), dim3(blocks_for(nseeds, 64)), dim3(64), 0, s, seed_p, seed_v, nseeds, u,
                            fixedmask);
     return hipGetLastError();
 }
-template hipError_t launch_init<float>(const Geom &, float *, unsigned long long *, int *, int *, long long,
-                                       const long long *, const double *, int, hipStream_t);
-template hipError_t launch_init<double>(const Geom &, double *, unsigned long long *, int *, int *, long long,
-                                        const long long *, const double *, int, hipStream_t);
+template hipError_t launch_init<float>(const Geom &, float *, unsigned long long *, unsigned *, unsigned *,
+                                       long long, const long long *, const double *, int, hipStream_t);
+template hipError_t launch_init<double>(const Geom &, double *, unsigned long long *, unsigned *, unsigned *,
+                                        long long, const long long *, const double *, int, hipStream_t);
 
-hipError_t launch_seed_activate(const long long *seed_p, int nseeds, const int *dep_start, const int *deps,
-                                int *flag_out, int *list_out, int *count_out, hipStream_t s) {
-    hipLaunchKernelGGL(k_seed_activate, dim3(1), dim3(64), 0, s, seed_p, nseeds, dep_start, deps, flag_out, list_out,
-                       count_out);
+hipError_t launch_seed_activate(const long long *seed_p, const double *seed_v, int nseeds, const int *dep_start,
+                                const int *deps, unsigned *keys, const unsigned *tmax, unsigned *pend, hipStream_t s) {
+    hipLaunchKernelGGL(k_seed_activate, dim3(1), dim3(64), 0, s, seed_p, seed_v, nseeds, dep_start, deps, keys, tmax,
+                       pend);
+    return hipGetLastError();
+}
+
+hipError_t launch_schedule(int ntiles, unsigned *keys, const SchedRound &sr, hipStream_t s) {
+    hipLaunchKernelGGL(k_schedule, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, ntiles, keys, sr);
     return hipGetLastError();
 }
 
 template <int M, typename T>
 static void fim_one(const FimArgs<T> &a, hipStream_t s) {
     hipLaunchKernelGGL((k_fim<M, T, 32>), dim3(a.grid), dim3(256), 0, s, a.g, a.sd, a.u, a.fixedmask, a.dep_start,
-                       a.deps, a.list_in, a.count_in, a.list_out, a.count_out, a.count_reset, a.flag_in, a.flag_out,
-                       a.stats);
+                       a.deps, a.list, a.count, a.count_reset, a.keys, a.tmax, a.pend_out, a.pend_reset, a.stats);
 }
 
 template <typename T>

@@ -30,23 +30,22 @@ void set_error(const std::string &m) { g_last_error = m; }
 }  // namespace hfm
 
 // speed = 1 / cost in the tiled layout (HFMInterface.hxx:131-150 DataSource_Inverse).
-__global__ void k_speed_from_cost_f(Geom g, const double *cost, int is_field, float *speed, long long np) {
+template <typename T>
+__global__ void k_speed_from_cost(Geom g, const double *cost, int is_field, T *speed, long long np, double *tau_acc) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     int c[3];
     tiled_coords(g, (int)(p >> 6), (int)(p & 63), c);
-    if (!in_grid(g, c)) { speed[p] = 0.f; return; }
-    const long long lin = ((long long)c[0] * g.n[1] + c[1]) * g.n[2] + c[2];
-    speed[p] = (float)(1. / cost[is_field ? lin : 0]);
-}
-__global__ void k_speed_from_cost_d(Geom g, const double *cost, int is_field, double *speed, long long np) {
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= np) return;
-    int c[3];
-    tiled_coords(g, (int)(p >> 6), (int)(p & 63), c);
-    if (!in_grid(g, c)) { speed[p] = 0.; return; }
-    const long long lin = ((long long)c[0] * g.n[1] + c[1]) * g.n[2] + c[2];
-    speed[p] = 1. / cost[is_field ? lin : 0];
+    double cv = 0., cc = 0.;
+    if (!in_grid(g, c)) {
+        speed[p] = T(0);
+    } else {
+        const long long lin = ((long long)c[0] * g.n[1] + c[1]) * g.n[2] + c[2];
+        cv = cost[is_field ? lin : 0];
+        cc = 1.;
+        speed[p] = (T)(1. / cv);
+    }
+    wave_add_tau(cv, cc, tau_acc);  // mean cost
 }
 
 namespace {
@@ -89,10 +88,23 @@ void plan_free(hfm_plan *p) {
     dfree(p->dep_fill);
     dfree(p->deps);
     dfree(p->lists[0]);
-    dfree(p->lists[1]);
-    dfree(p->flags[0]);
-    dfree(p->flags[1]);
+
     dfree(p->counts);
+    dfree(p->keys);
+    dfree(p->tmax);
+    dfree(p->nextmin);
+    dfree(p->bounds);
+    dfree(p->tau_acc);
+    dfree(p->adj_lambda);
+    dfree(p->adj_sens);
+    dfree(p->adj_indeg);
+    dfree(p->adj_counts);
+    dfree(p->adj_list[0]);
+    dfree(p->adj_list[1]);
+    dfree(p->adj_seed_idx);
+    dfree(p->adj_seed_w);
+    dfree(p->adj_tp);
+    dfree(p->adj_tw);
     dfree(p->status);
     dfree(p->stats);
     dfree(p->seed_p);
@@ -255,11 +267,15 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
     HIPCHECK(hipEventCreate(&e1));
     HIPCHECK(hipEventRecord(e0, s));
     const int nseeds = (int)p->last_seeds.size();
-    HIPCHECK(launch_init<T>(p->g, static_cast<T *>(p->u), p->fixedmask, p->flags[0], p->flags[1], p->np, p->seed_p,
+    HIPCHECK(launch_init<T>(p->g, static_cast<T *>(p->u), p->fixedmask, p->keys, p->tmax, p->np, p->seed_p,
                             p->seed_v, nseeds, s));
-    HIPCHECK(hipMemsetAsync(p->counts, 0, 3 * sizeof(int), s));
-    HIPCHECK(hipMemsetAsync(p->stats, 0, 2 * sizeof(unsigned long long), s));
-    HIPCHECK(launch_seed_activate(p->seed_p, nseeds, p->dep_start, p->deps, p->flags[0], p->lists[0], &p->counts[0], s));
+    HIPCHECK(hipMemsetAsync(p->counts, 0, 2 * sizeof(int), s));
+    HIPCHECK(hipMemsetAsync(p->stats, 0, 2 * PEND_SLOTS * sizeof(unsigned long long), s));
+    HIPCHECK(hipMemsetAsync(p->nextmin, 0xff, 2 * PEND_SLOTS * sizeof(unsigned), s));
+    const float binit[2] = {0.f, -__builtin_huge_valf()};
+    HIPCHECK(hipMemcpyAsync(p->bounds, binit, sizeof(binit), hipMemcpyHostToDevice, s));
+    HIPCHECK(launch_seed_activate(p->seed_p, p->seed_v, nseeds, p->dep_start, p->deps, p->keys, p->tmax,
+                                  &p->nextmin[0], s));
 
     FimArgs<T> a;
     a.g = p->g;
@@ -268,22 +284,27 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
     a.fixedmask = p->fixedmask;
     a.dep_start = p->dep_start;
     a.deps = p->deps;
+    a.keys = p->keys;
+    a.tmax = p->tmax;
+    a.list = p->lists[0];
     a.stats = p->stats;
     a.grid = (unsigned)std::min<long long>((p->g.ntiles + 3) / 4, 4096);
+    const float delta = (float)(p->delta_factor > 0 ? p->delta_factor * p->tau : 0.);
 
     std::vector<hipEvent_t> tev;
     long long k = 0;
     const int batch = 8;
-    const long long max_rounds = 4LL * p->g.ntiles + 4096;
+    const long long max_rounds = 8LL * p->g.ntiles + 65536;
     while (true) {
         for (int b = 0; b < batch; ++b, ++k) {
-            a.list_in = p->lists[k & 1];
-            a.list_out = p->lists[(k + 1) & 1];
-            a.count_in = &p->counts[k % 3];
-            a.count_out = &p->counts[(k + 1) % 3];
-            a.count_reset = &p->counts[(k + 2) % 3];
-            a.flag_in = p->flags[k & 1];
-            a.flag_out = p->flags[(k + 1) & 1];
+            // round k: schedule lists tiles into lists[0] / counts[k&1], update consumes them
+            SchedRound sr{&p->nextmin[(k & 1) * PEND_SLOTS], &p->nextmin[((k + 1) & 1) * PEND_SLOTS],
+                          &p->bounds[(k + 1) & 1], &p->bounds[k & 1], p->lists[0], &p->counts[k & 1], delta};
+            HIPCHECK(launch_schedule(p->g.ntiles, p->keys, sr, s));
+            a.count = &p->counts[k & 1];
+            a.count_reset = &p->counts[(k + 1) & 1];
+            a.pend_out = &p->nextmin[((k + 1) & 1) * PEND_SLOTS];
+            a.pend_reset = &p->nextmin[(k & 1) * PEND_SLOTS];
             if (p->timing) {
                 hipEvent_t ea, eb;
                 HIPCHECK(hipEventCreate(&ea));
@@ -297,20 +318,28 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
                 HIPCHECK(launch_fim<T>(p->model, a, s));
             }
         }
-        HIPCHECK(hipMemcpyAsync(p->h_count, &p->counts[k % 3], sizeof(int), hipMemcpyDeviceToHost, s));
+        // count of the last scheduled round (the update kernel cleared only the other slot)
+        HIPCHECK(hipMemcpyAsync(p->h_count, &p->counts[(k - 1) & 1], sizeof(int), hipMemcpyDeviceToHost, s));
         HIPCHECK(hipStreamSynchronize(s));
         if (*p->h_count == 0) break;
         if (k > max_rounds) throw HipFail{"block fast-iterative method did not converge"};
     }
     HIPCHECK(hipEventRecord(e1, s));
-    HIPCHECK(hipMemcpyAsync(p->h_stats, p->stats, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(p->h_stats, p->stats, 2 * PEND_SLOTS * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                            s));
     HIPCHECK(hipStreamSynchronize(s));
     float ms = 0;
     HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
     p->st.solve_ms = ms;
     p->st.iterations = k;
-    p->st.tile_visits = (int64_t)p->h_stats[0];
-    p->st.sweeps = (int64_t)p->h_stats[1];
+    p->st.tile_visits = 0;
+    p->st.sweeps = 0;
+    for (int i = 0; i < PEND_SLOTS; ++i) {
+        p->st.tile_visits += (int64_t)p->h_stats[2 * i];
+        p->st.sweeps += (int64_t)p->h_stats[2 * i + 1];
+    }
+    p->st.tiles_carried = 0;
+    p->st.delta = delta;
     double fim = 0;
     // Each pair brackets one launch; launches whose list was empty still count as launches.
     for (size_t i = 0; i + 1 < tev.size(); i += 2) {
@@ -337,11 +366,15 @@ template <typename T> void set_metric_impl(hfm_plan *p, const double *d_metric, 
     HIPCHECK(hipEventRecord(e0, s));
     int init[3] = {0, 0x7fffffff, 0};
     HIPCHECK(hipMemcpyAsync(p->status, init, sizeof(init), hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemsetAsync(p->tau_acc, 0, 2 * sizeof(double), s));
     HIPCHECK(launch_riemann_stencil<T>(p->g, d_metric, is_field, dual, ih, p->offs, static_cast<T *>(p->w), p->np,
-                                       p->status, s));
+                                       p->status, p->tau_acc, s));
     int hs[3];
+    double acc[2];
     HIPCHECK(hipMemcpyAsync(hs, p->status, sizeof(hs), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(acc, p->tau_acc, sizeof(acc), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
+    p->tau = acc[1] > 0 ? acc[0] / acc[1] : 0.;
     if (hs[0] == SELLING_NOT_SPD)
         throw HipFail{"ObtuseSuperbase error : input must be positive definite (grid point " + std::to_string(hs[1]) +
                       ")"};
@@ -444,14 +477,18 @@ int hfm_plan_create(hfm_plan **out, const char *model, const int64_t *dims, cons
         p->dep_start = dalloc<int>(nt + 1);
         p->dep_fill = dalloc<int>(nt);
         p->lists[0] = dalloc<int>(nt);
-        p->lists[1] = dalloc<int>(nt);
-        p->flags[0] = dalloc<int>(nt);
-        p->flags[1] = dalloc<int>(nt);
+
         p->counts = dalloc<int>(4);
+        p->keys = dalloc<unsigned>(nt);
+        p->tmax = dalloc<unsigned>(nt);
+        p->nextmin = dalloc<unsigned>(2 * PEND_SLOTS);
+        p->bounds = dalloc<float>(4);
+        p->tau_acc = dalloc<double>(2);
         p->status = dalloc<int>(4);
-        p->stats = dalloc<unsigned long long>(4);
+        p->stats = dalloc<unsigned long long>(2 * PEND_SLOTS);
         HIPCHECK(hipHostMalloc((void **)&p->h_count, sizeof(int) * 4, hipHostMallocDefault));
-        HIPCHECK(hipHostMalloc((void **)&p->h_stats, sizeof(unsigned long long) * 4, hipHostMallocDefault));
+        HIPCHECK(hipHostMalloc((void **)&p->h_stats, sizeof(unsigned long long) * 2 * PEND_SLOTS,
+                               hipHostMallocDefault));
         p->st.n_points = p->N;
         p->st.n_tiles = nt;
     } catch (const HipFail &e) {
@@ -519,17 +556,30 @@ int hfm_plan_set_cost(hfm_plan *p, const double *d_cost, int is_field, void *str
         HIPCHECK(hipMemcpyAsync(p->tab_offs, toff.data(), toff.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
         HIPCHECK(hipMemcpyAsync(p->tab_w, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice, s));
         const unsigned nb = (unsigned)((p->np + 255) / 256);
+        HIPCHECK(hipMemsetAsync(p->tau_acc, 0, 2 * sizeof(double), s));
         if (p->precision == 32) {
-            hipLaunchKernelGGL(k_speed_from_cost_f, dim3(nb), dim3(256), 0, s, p->g, d_cost, is_field,
-                               static_cast<float *>(p->speed), p->np);
+            hipLaunchKernelGGL((k_speed_from_cost<float>), dim3(nb), dim3(256), 0, s, p->g, d_cost, is_field,
+                               static_cast<float *>(p->speed), p->np, p->tau_acc);
             HIPCHECK(hipGetLastError());
             build_deps<float>(p, s);
         } else {
-            hipLaunchKernelGGL(k_speed_from_cost_d, dim3(nb), dim3(256), 0, s, p->g, d_cost, is_field,
-                               static_cast<double *>(p->speed), p->np);
+            hipLaunchKernelGGL((k_speed_from_cost<double>), dim3(nb), dim3(256), 0, s, p->g, d_cost, is_field,
+                               static_cast<double *>(p->speed), p->np, p->tau_acc);
             HIPCHECK(hipGetLastError());
             build_deps<double>(p, s);
         }
+        // one-step time ~ cost / sqrt(mean over theta of sum_k baseWeight_k)
+        double mw = 0.;
+        for (int t = 0; t < p->g.n[2]; ++t) {
+            double sw = 0.;
+            for (int k = 0; k < p->ndiff; ++k) sw += tw[(size_t)t * p->ndiff + k];
+            mw += sw;
+        }
+        mw /= p->g.n[2];
+        double acc[2];
+        HIPCHECK(hipMemcpyAsync(acc, p->tau_acc, sizeof(acc), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        p->tau = (acc[1] > 0 && mw > 0) ? (acc[0] / acc[1]) / std::sqrt(mw) : 0.;
         HIPCHECK(hipEventRecord(e1, s));
         HIPCHECK(hipStreamSynchronize(s));
         float ms = 0;
@@ -674,6 +724,15 @@ int hfm_plan_get_stencils(hfm_plan *p, int8_t *h_offsets, double *h_weights) {
         set_error(e.msg);
         return 1;
     }
+    return 0;
+}
+
+int hfm_plan_set_delta(hfm_plan *p, double factor) {
+    if (!p) {
+        set_error("hfm_plan_set_delta: null argument");
+        return 1;
+    }
+    p->delta_factor = factor;
     return 0;
 }
 

@@ -91,6 +91,9 @@ typedef struct hfm_plan_stats {
     double fim_ms;          /* summed duration of the update-kernel launches of the last solve
                                (hipEvent pairs around each launch on the plan's stream) */
     int64_t fim_launches;   /* update-kernel launches of the last solve */
+    int64_t tiles_carried;  /* listed tiles deferred to a later band (not processed) */
+    double delta;           /* band width used by the last solve (0: unordered) */
+    int64_t adjoint_rounds; /* rounds of the last backward sweep */
 } hfm_plan_stats;
 
 /* model: "Riemann2" | "Riemann3" | "ReedsSheppForward2" | "Elastica2".
@@ -140,6 +143,11 @@ int hfm_plan_get_stats(const hfm_plan *plan, hfm_plan_stats *stats);
  * offsets N x K x dim int8 and weights N x K (FP64 copy of the stored weights), in the
  * reference's slot order (what getStencils prints, HFMInterface.hxx:360-375). */
 int hfm_plan_get_stencils(hfm_plan *plan, int8_t *h_offsets, double *h_weights);
+
+/* Width of the value bands that order the block fast-iterative rounds, in units of the mean
+ * one-step time 1/sqrt(sum_k w_k) of the stencils (default 8; <= 0 processes every active
+ * tile in every round).  It changes the amount of work, never the fixed point. */
+int hfm_plan_set_delta(hfm_plan *plan, double factor);
 
 /* Per-launch timing of the update kernel (hipEvent pairs around every launch; off by default
  * because the events add a few microseconds per launch).  Fills fim_ms / fim_launches. */

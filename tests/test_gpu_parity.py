@@ -76,3 +76,36 @@ def test_riemann3_fp32_error(n):
     err = _floor_rel(u, ref)
     print(f"riemann3 {n}^3 fp32: max rel err {err:.3e}", flush=True)
     assert err <= 1e-5
+
+
+ADJ_CASES = [
+    ("elastica2_20x20x16", lambda: syn.elastica2(20, 16, seed=4, ntargets=8)),
+    ("rsf2_24x24x16", lambda: dict(syn.rsf2(24, 16, seed=3), inspectSensitivity=syn.rsf2(24, 16, seed=3)["tips"][:6])),
+    ("riemann2_40", lambda: dict(syn.riemann2(40, seed=8), inspectSensitivity=np.array([[0.1, 0.2], [0.8, 0.9], [0.3, 0.7]]))),
+    ("riemann3_16", lambda: dict(syn.riemann3(16, seed=9), inspectSensitivity=np.array([[0.1, 0.2, 0.3], [0.9, 0.8, 0.7]]))),
+]
+
+
+@pytest.mark.parametrize("name,make", ADJ_CASES, ids=[c[0] for c in ADJ_CASES])
+@pytest.mark.parametrize("precision", [64, 32])
+def test_adjoint_matches_oracle(name, make, precision):
+    print(f"adjoint {name} precision {precision}", flush=True)
+    p = make()
+    p.pop("tips", None)
+    tg = np.asarray(p["inspectSensitivity"], np.float64)
+    tw = np.asarray(p.get("inspectSensitivityWeights", np.ones(len(tg))), np.float64)
+    p["inspectSensitivityWeights"] = tw
+    p["inspectSensitivityLengths"] = np.array([float(len(tg))])
+    _, sref, (sidx, sw) = oracle.solve_backward(p, tg, tw)
+    q = dict(p)
+    q["precision"] = float(precision)
+    out = hfm.run(q)
+    s = out["costSensitivity_0"]
+    scale = np.abs(sref).max()
+    err = np.abs(s - sref).max() / scale
+    print(f"{name} fp{precision}: adjoint rel err {err:.3e} (scale {scale:.3e})", flush=True)
+    tol = 1e-9 if precision == 64 else 2e-3
+    assert err <= tol
+    ss = out["seedSensitivity_0"]
+    assert ss.shape[0] == len(sidx)
+    np.testing.assert_allclose(ss[:, -1], sw, rtol=1e-9 if precision == 64 else 2e-3)
