@@ -293,6 +293,22 @@ void run_impl(hfm_io *io) {
     const int precision = (int)io->scalar("precision", 64.);
     const double xi = bundle ? io->scalar("xi", 1.) : 1.;
     const double eps = bundle ? io->scalar("eps", 0.1) : 0.1;
+    // Read and validate every input field on the host before any device work.
+    const int64_t K = dim * (dim + 1) / 2;
+    bool is_field = false, hasDual = false;
+    std::vector<double> field;
+    if (riemann) {
+        hasDual = io->has("dualMetric");
+        if (hasDual && io->has("metric")) throw IOError{"Error: both primal and dual metric provided"};
+        field = grid_field(io, hasDual ? "dualMetric" : "metric", dims, K, false, is_field);
+    } else if (io->has("speed")) {
+        field = grid_field(io, "speed", dims, 1, true, is_field);
+        for (auto &v : field) v = 1. / v;
+    } else {
+        field = grid_field(io, "cost", dims, 1, true, is_field);
+    }
+    if (precision != 32 && precision != 64) throw IOError{"precision must be 32 or 64"};
+
     PlanGuard pg;
     check(hfm_plan_create(&pg.p, model.c_str(), dims.data(), h, origin.empty() ? nullptr : origin.data(), precision,
                           xi, eps));
@@ -301,30 +317,12 @@ void run_impl(hfm_io *io) {
     for (auto d : dims) N *= d;
 
     const double t0 = wall();
-    if (riemann) {
-        const bool hasDual = io->has("dualMetric");
-        if (hasDual && io->has("metric")) throw IOError{"Error: both primal and dual metric provided"};
-        const std::string mkey = hasDual ? "dualMetric" : "metric";
-        bool is_field = false;
-        const int64_t K = dim * (dim + 1) / 2;
-        std::vector<double> m = grid_field(io, mkey, dims, K, false, is_field);
-        DevBuf dm(m.size() * sizeof(double));
-        if (hipMemcpy(dm.p, m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    {
+        DevBuf dm(field.size() * sizeof(double));
+        if (hipMemcpy(dm.p, field.data(), field.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
             throw IOError{"hipMemcpy failed"};
-        check(hfm_plan_set_metric(plan, dm.d(), is_field ? 1 : 0, hasDual ? 1 : 0, nullptr));
-    } else {
-        bool is_field = false;
-        std::vector<double> c;
-        if (io->has("speed")) {
-            c = grid_field(io, "speed", dims, 1, true, is_field);
-            for (auto &v : c) v = 1. / v;
-        } else {
-            c = grid_field(io, "cost", dims, 1, true, is_field);
-        }
-        DevBuf dc(c.size() * sizeof(double));
-        if (hipMemcpy(dc.p, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
-            throw IOError{"hipMemcpy failed"};
-        check(hfm_plan_set_cost(plan, dc.d(), is_field ? 1 : 0, nullptr));
+        if (riemann) check(hfm_plan_set_metric(plan, dm.d(), is_field ? 1 : 0, hasDual ? 1 : 0, nullptr));
+        else check(hfm_plan_set_cost(plan, dm.d(), is_field ? 1 : 0, nullptr));
     }
     const double t1 = wall();
     io->set_scalar("StencilCPUTime", t1 - t0);

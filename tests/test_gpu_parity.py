@@ -109,3 +109,97 @@ def test_adjoint_matches_oracle(name, make, precision):
     ss = out["seedSensitivity_0"]
     assert ss.shape[0] == len(sidx)
     np.testing.assert_allclose(ss[:, -1], sw, rtol=1e-9 if precision == 64 else 2e-3)
+
+
+# ------------------------------------------------------------------------------------------
+# Against the golden fixtures written by the reference executables (tests/golden/)
+# ------------------------------------------------------------------------------------------
+import os
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _gold(name):
+    return np.load(os.path.join(GOLD, name + ".npz"))
+
+
+@pytest.mark.parametrize("name,make", [("riemann2_64", lambda: syn.riemann2(64, seed=1)),
+                                       ("riemann3_24", lambda: syn.riemann3(24, seed=2))])
+@pytest.mark.parametrize("precision", [64, 32])
+def test_riemann_vs_golden(name, make, precision):
+    print(f"golden {name} precision {precision}", flush=True)
+    g = _gold(name)
+    p = make()
+    q = dict(p, precision=float(precision), exportActiveNeighs=1.0, exportGeodesicFlow=1.0)
+    out = hfm.run(q)
+    ref = g["values"]
+    err = _floor_rel(out["values"], ref)
+    print(f"{name} fp{precision}: values rel err {err:.3e}", flush=True)
+    assert err <= (1e-12 if precision == 64 else 1e-5)
+    if precision == 64:
+        act_eq = np.mean(out["activeNeighs"] == g["activeNeighs"])
+        fl = out["geodesicFlow"]
+        gf = g["geodesicFlow"]
+        ferr = np.max(np.abs(fl - gf)) / np.max(np.abs(gf))
+        print(f"{name}: activeNeighs equal {act_eq:.5f}, flow rel err {ferr:.3e}", flush=True)
+        assert act_eq >= 0.999
+        assert ferr <= 1e-10
+    import torch
+    dims = p["metric"].shape[:-1]
+    plan = hfm.Plan(p["model"], dims, p["gridScale"], precision=precision)
+    plan.set_metric(torch.from_numpy(p["metric"]))
+    off, _ = plan.stencils()
+    assert np.array_equal(off, g["offsets"])
+
+
+@pytest.mark.parametrize("key", ["metric", "dualMetric"])
+def test_selling_ties_vs_golden(key):
+    import torch
+    g = _gold("ties3")
+    plan = hfm.Plan("Riemann3", (4, 4, 4), 1.0, precision=64)
+    plan.set_metric(torch.from_numpy(g["tensors"]), dual=(key == "dualMetric"))
+    off, _ = plan.stencils()
+    assert np.array_equal(off, g[key + "_offsets"])
+
+
+@pytest.mark.parametrize("name,make", [("rsf2_32x32x16", lambda: syn.rsf2(32, 16, seed=3)),
+                                       ("elastica2_24x24x16", lambda: syn.elastica2(24, 16, seed=4))])
+@pytest.mark.parametrize("precision", [64, 32])
+def test_curvature_vs_golden(name, make, precision):
+    print(f"golden {name} precision {precision}", flush=True)
+    g = _gold(name)
+    p = make()
+    p.pop("tips", None)
+    q = dict(p, precision=float(precision), exportActiveNeighs=1.0, exportGeodesicFlow=1.0)
+    out = hfm.run(q)
+    err = _floor_rel(out["values"], g["values"])
+    print(f"{name} fp{precision}: values rel err {err:.3e}", flush=True)
+    assert err <= (1e-12 if precision == 64 else 1e-5)
+    if "costSensitivity_0" in g:
+        s, gs = out["costSensitivity_0"], g["costSensitivity_0"]
+        serr = np.max(np.abs(s - gs)) / np.max(np.abs(gs))
+        print(f"{name} fp{precision}: costSensitivity rel err {serr:.3e}", flush=True)
+        assert serr <= (1e-9 if precision == 64 else 2e-3)
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_riemann3_128_vs_reference_samples(precision):
+    """Benchmark size: 4096 seeded samples of the reference's FP64 values."""
+    import torch
+    print(f"riemann3 128 precision {precision}", flush=True)
+    g = _gold("riemann3_128")
+    p = syn.riemann3(128, seed=0)
+    plan = hfm.Plan("Riemann3", (128,) * 3, p["gridScale"], precision=precision)
+    plan.set_metric(torch.from_numpy(p["metric"]))
+    plan.solve(p["seeds"])
+    u = plan.values().cpu().numpy().ravel()
+    idx, ref = g["sample_index"], g["sample_values"]
+    us = u[idx]
+    fin = np.isfinite(ref)
+    assert np.array_equal(np.isfinite(us), fin)
+    assert int(np.isfinite(u).sum()) == int(g["n_finite"])
+    floor = ref[fin & (ref > 0)].min()
+    m = fin & (ref > 0)
+    err = float(np.max(np.abs(us[m] - ref[m]) / ref[m]))
+    print(f"riemann3 128^3 fp{precision}: sample rel err {err:.3e} (floor {floor:.3e})", flush=True)
+    assert err <= (1e-12 if precision == 64 else 1e-5)

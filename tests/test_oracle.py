@@ -1,0 +1,264 @@
+"""CPU tests of the oracle (the FP64 restatement that the GPU library is checked against).
+
+Pins, in order of strength:
+  * golden fixtures written by the reference's own FileHFM executables (tests/golden/): values,
+    activeNeighs, geodesicFlow, Selling offsets, shared curvature stencils, costSensitivity --
+    all required bit-exact;
+  * closed forms (identity metric, constant metrics), invariants (Selling reconstruction,
+    non-negative weights, unimodular superbase; the discrete eikonal residual), symmetry and a
+    brute-force Gauss-Seidel fixed point on tiny grids;
+  * planted mistakes: perturbed outputs must fail the same checks.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from hamiltonfastmarching_amd import synthetic as syn
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    return np.load(os.path.join(GOLD, name + ".npz"))
+
+
+def sha(a):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+# ------------------------------------------------------------------------------------------
+# Golden fixtures from the reference
+# ------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("name,make", [("riemann2_64", lambda: syn.riemann2(64, seed=1)),
+                                       ("riemann3_24", lambda: syn.riemann3(24, seed=2))])
+def test_riemann_golden(name, make):
+    g = load(name)
+    p = make()
+    assert sha(p["metric"]) == str(g["metric_sha256"]), "synthetic generator changed"
+    out = oracle.solve(p, flow=True, active=True)
+    assert np.array_equal(out["values"], g["values"])
+    assert np.array_equal(out["activeNeighs"], g["activeNeighs"].astype(np.uint64))
+    assert np.array_equal(out["geodesicFlow"], g["geodesicFlow"])
+    dims = p["metric"].shape[:-1]
+    off, w = oracle.riemann_stencils(dims, p["gridScale"], p["metric"])
+    assert np.array_equal(off, g["offsets"])
+    np.testing.assert_allclose(w, g["weights_printed"], rtol=1e-5, atol=1e-12)
+    nz = w > 0
+    l1 = np.abs(off.astype(int)).sum(-1)
+    assert int(l1[nz].max()) == int(g["MaxStencilWidth"])
+
+
+@pytest.mark.parametrize("key", ["metric", "dualMetric"])
+def test_selling_ties_golden(key):
+    g = load("ties3")
+    t = g["tensors"].reshape(-1, 6)
+    off, _ = oracle.riemann_stencils((4, 4, 4), 1.0, t, dual=(key == "dualMetric"))
+    assert np.array_equal(off, g[key + "_offsets"])
+
+
+@pytest.mark.parametrize("name,model,make", [
+    ("rsf2_32x32x16", "ReedsSheppForward2", lambda: syn.rsf2(32, 16, seed=3)),
+    ("elastica2_24x24x16", "Elastica2", lambda: syn.elastica2(24, 16, seed=4))])
+def test_curvature_golden(name, model, make):
+    g = load(name)
+    p = make()
+    assert sha(p["cost"]) == str(g["cost_sha256"])
+    p.pop("tips", None)
+    out = oracle.solve(p, flow=True, active=True)
+    assert np.array_equal(out["values"], g["values"])
+    assert np.array_equal(out["activeNeighs"], g["activeNeighs"].astype(np.uint64))
+    assert np.array_equal(out["geodesicFlow"], g["geodesicFlow"])
+    off, w, sym = oracle.curvature_stencils(model, 16, p["gridScale"], p["xi"], p["eps"])
+    assert np.array_equal(off, g["table_offsets"])
+    np.testing.assert_allclose(w, g["table_weights_printed"], rtol=1e-5, atol=1e-12)
+    if "costSensitivity_0" in g:
+        v, s, (si, sw) = oracle.solve_backward(p, p["inspectSensitivity"], p["inspectSensitivityWeights"])
+        assert np.array_equal(s, g["costSensitivity_0"])
+        ss = g["seedSensitivity_0"]
+        assert np.array_equal(sw, ss[:, -1])
+
+
+def test_riemann3_128_digest():
+    """Benchmark-size values against the reference's digest and 4096 seeded samples."""
+    g = load("riemann3_128")
+    p = syn.riemann3(128, seed=0)
+    assert sha(p["metric"]) == str(g["metric_sha256"])
+    v = oracle.solve(p)["values"]
+    assert np.array_equal(v.ravel()[g["sample_index"]], g["sample_values"])
+    assert sha(v) == str(g["values_sha256"])
+
+
+# ------------------------------------------------------------------------------------------
+# Closed forms and invariants
+# ------------------------------------------------------------------------------------------
+def _sym3(M):
+    return np.array([M[0, 0], M[1, 0], M[1, 1], M[2, 0], M[2, 1], M[2, 2]])
+
+
+def _unsym(c, d):
+    if d == 2:
+        return np.array([[c[0], c[1]], [c[1], c[2]]])
+    return np.array([[c[0], c[1], c[3]], [c[1], c[2], c[4]], [c[3], c[4], c[5]]])
+
+
+def test_identity_2d_closed_form():
+    """D = I: offsets (0,1), (-1,0) with weight 1 and (1,-1) with weight 0 (the 4-point scheme);
+    values along the axes are exactly k h; at a diagonal neighbour (a+b+sqrt(2h^2-(a-b)^2))/2."""
+    off, w = oracle.riemann_stencils((1, 1), 1.0, np.array([1.0, 0.0, 1.0]))
+    assert sorted(map(tuple, off[0].tolist())) == [(-1, 0), (0, 1), (1, -1)]
+    assert np.allclose(sorted(w[0]), [0, 1, 1])
+    n, h = 21, 0.1
+    p = {"model": "Riemann2", "dims": [n, n], "gridScale": h, "metric": np.array([1.0, 0.0, 1.0]),
+         "seeds": np.array([[(n // 2 + 0.5) * h] * 2])}
+    u = oracle.solve(p)["values"]
+    c = n // 2
+    for k in range(n - c):
+        assert u[c + k, c] == pytest.approx(k * h, rel=1e-14, abs=1e-15)
+        assert u[c, c - k] == pytest.approx(k * h, rel=1e-14, abs=1e-15)
+    a = u[c + 1, c]
+    assert u[c + 1, c + 1] == pytest.approx((a + a + np.sqrt(2 * h * h)) / 2, rel=1e-14)
+
+
+def test_selling_invariants_random():
+    rng = np.random.Generator(np.random.PCG64(5))
+    for d in (2, 3):
+        K = 3 if d == 2 else 6
+        A = rng.standard_normal((500, d, d))
+        M = A @ np.swapaxes(A, 1, 2) + 0.05 * np.eye(d)
+        comps = np.array([[m[0, 0], m[1, 0], m[1, 1]] if d == 2 else _sym3(m) for m in M])
+        off, w = oracle.riemann_stencils((500,) + (1,) * (d - 1), 1.0, comps, dual=True)
+        assert (w >= 0).all()
+        for i in range(500):
+            D = _unsym(comps[i], d)
+            R = sum(w[i, k] * np.outer(off[i, k], off[i, k]) for k in range(K))
+            # reconstruction up to the zeroed weights (< 1000 eps Tr D each)
+            assert np.allclose(R, D, atol=K * 1000 * 2.3e-16 * np.trace(D) + 1e-12 * np.abs(D).max())
+
+
+def test_planted_selling_mistake_is_caught():
+    """A transposed offset or a wrong sign in a weight breaks the reconstruction check."""
+    comps = np.array([2.0, 0.7, 1.5, 0.3, -0.2, 1.1])
+    off, w = oracle.riemann_stencils((1, 1, 1), 1.0, comps, dual=True)
+    D = _unsym(comps, 3)
+    good = sum(w[0, k] * np.outer(off[0, k], off[0, k]) for k in range(6))
+    assert np.allclose(good, D)
+    bad_off = off.copy()
+    bad_off[0, 0] = bad_off[0, 0][::-1]
+    bad = sum(w[0, k] * np.outer(bad_off[0, k], bad_off[0, k]) for k in range(6))
+    w2 = w.copy()
+    w2[0, 1] = -w2[0, 1]
+    bad2 = sum(w2[0, k] * np.outer(off[0, k], off[0, k]) for k in range(6))
+    assert not (np.allclose(bad, D) and np.allclose(bad2, D))
+
+
+def residual(p, u):
+    """max |sum_k w_k max(0, u - u(x+e), u - u(x-e))^2 - 1| over non-seed reachable points."""
+    dims = u.shape
+    d = len(dims)
+    off, w = oracle.riemann_stencils(dims, p["gridScale"], p["metric"])
+    N = u.size
+    uf = u.ravel()
+    idx = np.array(np.unravel_index(np.arange(N), dims)).T
+    res = np.zeros(N)
+    for k in range(off.shape[1]):
+        best = np.zeros(N)
+        for s in (1, -1):
+            nb = idx + s * off[:, k, :].astype(int)
+            ok = np.all((nb >= 0) & (nb < np.array(dims)), axis=1)
+            vv = np.full(N, np.inf)
+            vv[ok] = uf[np.ravel_multi_index(nb[ok].T, dims)]
+            best = np.maximum(best, np.where(np.isfinite(vv), np.maximum(0, uf - vv), 0))
+        res += w[:, k] * best ** 2
+    return res
+
+
+def test_residual_invariant_and_planted_perturbation():
+    p = syn.riemann3(12, seed=3)
+    u = oracle.solve(p)["values"]
+    r = residual(p, u)
+    seed_idx = np.ravel_multi_index((6, 6, 6), u.shape)
+    fin = np.isfinite(u.ravel())
+    mask = fin.copy()
+    mask[seed_idx] = False
+    assert np.max(np.abs(r[mask] - 1)) < 1e-9
+    u2 = u.copy()
+    u2[3, 4, 5] *= 1 + 1e-6
+    r2 = residual(p, u2)
+    assert np.max(np.abs(r2[mask] - 1)) > 1e-7
+
+
+def test_bruteforce_gauss_seidel_fixed_point():
+    """FP64 Gauss-Seidel sweeps of the local solve to a fixed point equal the FM values."""
+    p = syn.riemann2(10, seed=4)
+    u_fm = oracle.solve(p)["values"]
+    dims = u_fm.shape
+    off, w = oracle.riemann_stencils(dims, p["gridScale"], p["metric"])
+    u = np.full(dims, np.inf)
+    u[5, 5] = 0.0
+    for _ in range(200):
+        changed = False
+        for i in range(dims[0]):
+            for j in range(dims[1]):
+                if (i, j) == (5, 5):
+                    continue
+                lin = i * dims[1] + j
+                cand = []
+                for k in range(3):
+                    if w[lin, k] == 0:
+                        continue
+                    vs = []
+                    for s in (1, -1):
+                        a, b = i + s * off[lin, k, 0], j + s * off[lin, k, 1]
+                        if 0 <= a < dims[0] and 0 <= b < dims[1]:
+                            vs.append(u[a, b])
+                    if vs:
+                        cand.append((min(vs), w[lin, k]))
+                cand.sort()
+                r, A, B, C = np.inf, 0.0, 0.0, -1.0
+                m = None
+                for v, wk in cand:
+                    if not v < r:
+                        break
+                    m = v if m is None else m
+                    vr = v - m
+                    A += wk
+                    B += wk * vr
+                    C += (wk * vr) * vr
+                    r = m + (B + np.sqrt(max(0.0, B * B - A * C))) / A
+                if r < u[i, j]:
+                    u[i, j] = r
+                    changed = True
+        if not changed:
+            break
+    fin = np.isfinite(u_fm)
+    assert np.array_equal(np.isfinite(u), fin)
+    assert np.max(np.abs(u[fin] - u_fm[fin])) < 1e-13
+
+
+def test_reflection_symmetry():
+    """A metric field symmetric under x -> -x with a centred seed gives symmetric values."""
+    n = 17
+    h = 1.0 / n
+    x = (np.arange(n) + 0.5) * h - 0.5
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    a = 1 + 3 * np.exp(-((X ** 2 + 2 * Y ** 2) * 10))
+    m = np.stack([a, 0.3 * X * Y * 0 + 0.0, 1 + Y ** 2], axis=-1)
+    p = {"model": "Riemann2", "dims": [n, n], "gridScale": h, "metric": m, "seeds": np.array([[0.5, 0.5]])}
+    u = oracle.solve(p)["values"]
+    assert np.allclose(u, u[::-1, :], rtol=1e-13)
+    assert np.allclose(u, u[:, ::-1], rtol=1e-13)
+
+
+def test_reference_and_port_agree_where_reference_built():
+    """When the reference binaries are present (development container), the restatement is
+    bit-exact against a fresh reference run on an unseen input."""
+    if not oracle.ref_available("Riemann3"):
+        pytest.skip("reference binaries not built here")
+    p = syn.riemann3(14, seed=99, amax=30.0)
+    r = oracle.run_reference(dict(p, exportActiveNeighs=1.0))
+    mine = oracle.solve(p, active=True)
+    assert np.array_equal(r["values"], mine["values"])
+    assert np.array_equal(r["activeNeighs"].astype(np.uint64), mine["activeNeighs"])
