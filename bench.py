@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one plain 1 GiB device-to-device copy (reference kernel for the "
+                         "FETCH_SIZE / WRITE_SIZE counter scale)")
     return ap.parse_args()
 
 
@@ -158,6 +161,13 @@ def main():
         "fim_iterations": st["iterations"], "tile_visits": st["tile_visits"], "sweeps": st["sweeps"],
         "roofline": roofline,
     }
+    if args.copy_calibration:
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        result["copy_calibration_bytes"] = src.numel() * 4
+        del src, dst
     if not args.no_cpu:
         cb, ref_values = cpu_baseline(params)
         result["cpu_baseline"] = cb
