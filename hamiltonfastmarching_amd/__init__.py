@@ -264,6 +264,14 @@ class Plan:
         self._check(self._L.hfm_plan_set_delta(self._h, float(factor)))
         return self
 
+    def set_grid(self, blocks: int):
+        self._check(self._L.hfm_plan_set_grid(self._h, int(blocks)))
+        return self
+
+    def set_round_target(self, tiles: int):
+        self._check(self._L.hfm_plan_set_round_target(self._h, int(tiles)))
+        return self
+
     def set_timing(self, enable: bool = True):
         self._check(self._L.hfm_plan_set_timing(self._h, int(enable)))
         return self

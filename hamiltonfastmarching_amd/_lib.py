@@ -24,6 +24,24 @@ class HfmPlanStats(ctypes.Structure):
                 ("delta", c_double), ("adjoint_rounds", c_int64)]
 
 
+class HfmRequest(ctypes.Structure):
+    _fields_ = [("model", ctypes.c_char_p), ("dims", c_int64 * 3), ("gridScale", c_double),
+                ("metric", c_void_p), ("metric_is_field", c_int), ("metric_on_device", c_int), ("dual", c_int),
+                ("nseeds", c_int), ("seeds", P(c_double)), ("seed_values", P(c_double)),
+                ("values_out", P(c_double))]
+
+
+class HfmRequestResult(ctypes.Structure):
+    _fields_ = [("status", c_int), ("latency_ms", c_double), ("queue_ms", c_double), ("setup_ms", c_double),
+                ("rounds", c_int64), ("tile_visits", c_int64), ("error", ctypes.c_char * 256)]
+
+
+class HfmServerStats(ctypes.Structure):
+    _fields_ = [("submitted", c_int64), ("completed", c_int64), ("failed", c_int64), ("resident", c_int64),
+                ("max_resident_seen", c_int64), ("batches", c_int64), ("rounds", c_int64),
+                ("hbm_in_use", c_double), ("hbm_budget", c_double)]
+
+
 # (name, restype, argtypes) of every exported symbol; tests check that each one is exported.
 SIGNATURES = [
     ("hfm_io_create", c_void_p, []),
@@ -54,7 +72,15 @@ SIGNATURES = [
     ("hfm_plan_get_stats", c_int, [c_void_p, P(HfmPlanStats)]),
     ("hfm_plan_set_timing", c_int, [c_void_p, c_int]),
     ("hfm_plan_set_delta", c_int, [c_void_p, c_double]),
+    ("hfm_plan_set_round_target", c_int, [c_void_p, c_int]),
+    ("hfm_plan_set_grid", c_int, [c_void_p, c_int]),
     ("hfm_plan_get_stencils", c_int, [c_void_p, P(ctypes.c_int8), P(c_double)]),
+    ("hfm_server_create", c_int, [P(c_void_p), c_double, c_int]),
+    ("hfm_server_submit", c_int, [c_void_p, P(HfmRequest), P(c_int64)]),
+    ("hfm_server_poll", c_int, [c_void_p, c_int64, P(c_int)]),
+    ("hfm_server_wait", c_int, [c_void_p, c_int64, c_double, P(HfmRequestResult)]),
+    ("hfm_server_get_stats", c_int, [c_void_p, P(HfmServerStats)]),
+    ("hfm_server_destroy", None, [c_void_p]),
     ("hfm_last_error", ctypes.c_char_p, []),
 ]
 

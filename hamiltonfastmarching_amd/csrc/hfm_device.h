@@ -94,18 +94,47 @@ __host__ __device__ inline void unpack_offset(uint32_t p, int &o0, int &o1, int 
 
 // Static description of a model's stencil shape.
 template <int ModelT> struct ModelShape;
+// PERIODIC: bitmask of periodic axes (the theta axis of R^2 x S^1, Curvature2.h:24-25).
 template <> struct ModelShape<MODEL_RIEMANN2> {
-    static constexpr int DIM = 2, NFWD = 0, NSYM = 3, SHARED = 0;
+    static constexpr int DIM = 2, NFWD = 0, NSYM = 3, SHARED = 0, PERIODIC = 0;
 };
 template <> struct ModelShape<MODEL_RIEMANN3> {
-    static constexpr int DIM = 3, NFWD = 0, NSYM = 6, SHARED = 0;
+    static constexpr int DIM = 3, NFWD = 0, NSYM = 6, SHARED = 0, PERIODIC = 0;
 };
 template <> struct ModelShape<MODEL_RSF2> {
-    static constexpr int DIM = 3, NFWD = 3, NSYM = 1, SHARED = 1;
+    static constexpr int DIM = 3, NFWD = 3, NSYM = 1, SHARED = 1, PERIODIC = 4;
 };
 template <> struct ModelShape<MODEL_ELASTICA2> {
-    static constexpr int DIM = 3, NFWD = 27, NSYM = 0, SHARED = 1;
+    static constexpr int DIM = 3, NFWD = 27, NSYM = 0, SHARED = 1, PERIODIC = 4;
 };
+
+// shift_coords with the model's dimension and periodic axes known at compile time.
+template <int ModelT>
+__host__ __device__ inline bool shift_coords_m(const Geom &g, const int *c, int o0, int o1, int o2, int *r) {
+    constexpr int DIM = ModelShape<ModelT>::DIM, PER = ModelShape<ModelT>::PERIODIC;
+    const int o[3] = {o0, o1, o2};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k >= DIM) {
+            r[k] = c[k];
+            continue;
+        }
+        int v = c[k] + o[k];
+        if (PER >> k & 1) {
+            const int n = g.n[k];
+            if (v < 0) v += n;
+            else if (v >= n) v -= n;
+            if (v < 0 || v >= n) {
+                v %= n;
+                if (v < 0) v += n;
+            }
+        } else if (v < 0 || v >= g.n[k]) {
+            return false;
+        }
+        r[k] = v;
+    }
+    return true;
+}
 
 // Device-side stencil data.
 template <typename T> struct StencilData {
@@ -197,40 +226,79 @@ template <int NC, typename T> __device__ inline T local_solve_update(const T (&v
     uint32_t used;
     return local_solve<NC, T>(v, w, used);
 }
-template <int NC> __device__ inline float local_solve_update(const float (&v)[NC], const float (&w)[NC]) {
+template <int NC> __device__ inline float local_solve_update(const float (&v0)[NC], const float (&w0)[NC]) {
     const float inf = __builtin_huge_valf();
-    float m = 0.f, a = 0.f, b = 0.f, c = 0.f;
-    uint32_t sel = 0;
-    bool any = false;
-    for (int step = 0; step < NC; ++step) {
-        float bv = inf, bw = 0.f;
-        int bk = -1;
+    float v[NC], w[NC];
 #pragma unroll
-        for (int k = 0; k < NC; ++k) {
-            const bool ok = !((sel >> k) & 1u) && v[k] < bv;
-            bv = ok ? v[k] : bv;
-            bw = ok ? w[k] : bw;
-            bk = ok ? k : bk;
-        }
-        if (bk < 0) break;
-        if (step == 0) {
-            m = bv;
-        } else {
-            const float t = bv - m;
-            if (!(t * (a * t - 2.f * b) + c < 1.f)) break;
-        }
-        const float vr = bv - m;
-        const float wv = bw * vr;
-        a += bw;
-        b += wv;
-        c += wv * vr;
-        sel |= 1u << bk;
-        any = true;
+    for (int k = 0; k < NC; ++k) {
+        v[k] = v0[k];
+        w[k] = w0[k];
     }
-    if (!any) return inf;
-    const float cc = c - 1.f;
-    const float d2 = b * b - a * cc;
-    return m + (b + sqrtf(d2 > 0.f ? d2 : 0.f)) / a;
+    if constexpr (NC <= 8) {
+        // odd-even transposition network: NC rounds of compare-exchange on (v, w) pairs
+#pragma unroll
+        for (int rnd = 0; rnd < NC; ++rnd) {
+#pragma unroll
+            for (int j = (rnd & 1); j + 1 < NC; j += 2) {
+                const float a0 = v[j], a1 = v[j + 1], b0 = w[j], b1 = w[j + 1];
+                const bool sw = a1 < a0;
+                v[j] = fminf(a0, a1);
+                v[j + 1] = fmaxf(a0, a1);
+                w[j] = sw ? b1 : b0;
+                w[j + 1] = sw ? b0 : b1;
+            }
+        }
+        if (!(v[0] < inf)) return inf;
+        const float m = v[0];
+        float a = w[0], b = 0.f, c = 0.f;
+        bool go = true;
+#pragma unroll
+        for (int k = 1; k < NC; ++k) {
+            const float t = v[k] - m;
+            go = go && (t * (a * t - 2.f * b) + c < 1.f);
+            const float wk = go ? w[k] : 0.f;
+            const float tk = go ? t : 0.f;
+            a += wk;
+            b += wk * tk;
+            c += (wk * tk) * tk;
+        }
+        // hardware square root and reciprocal (1 ulp each): they perturb only the increment
+        // over the smallest neighbour value, far below the FP32 rounding of u itself
+        const float d2 = b * b - a * (c - 1.f);
+        return m + (b + __builtin_amdgcn_sqrtf(d2 > 0.f ? d2 : 0.f)) * __builtin_amdgcn_rcpf(a);
+    } else {
+        float m = 0.f, a = 0.f, b = 0.f, c = 0.f;
+        uint32_t sel = 0;
+        bool any = false;
+        for (int step = 0; step < NC; ++step) {
+            float bv = inf, bw = 0.f;
+            int bk = -1;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                const bool ok = !((sel >> k) & 1u) && v[k] < bv;
+                bv = ok ? v[k] : bv;
+                bw = ok ? w[k] : bw;
+                bk = ok ? k : bk;
+            }
+            if (bk < 0) break;
+            if (step == 0) {
+                m = bv;
+            } else {
+                const float t = bv - m;
+                if (!(t * (a * t - 2.f * b) + c < 1.f)) break;
+            }
+            const float vr = bv - m;
+            const float wv = bw * vr;
+            a += bw;
+            b += wv;
+            c += wv * vr;
+            sel |= 1u << bk;
+            any = true;
+        }
+        if (!any) return inf;
+        const float d2 = b * b - a * (c - 1.f);
+        return m + (b + sqrtf(d2 > 0.f ? d2 : 0.f)) / a;
+    }
 }
 
 // Wave-reduced accumulation of (sum, count) into tau_acc[0..1]; every lane of the wave calls it.

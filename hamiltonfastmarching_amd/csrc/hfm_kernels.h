@@ -31,7 +31,11 @@ struct SchedRound {
     float *bound_out;
     int *list;
     int *count;
-    float delta;  // band width; <= 0 lists every pending tile
+    float delta;            // initial band width; <= 0 lists every pending tile
+    const float *delta_in;  // band width of the previous round
+    float *delta_out;       // band width of this round
+    const int *prev_count;  // tiles the previous round listed
+    int target;             // tiles per round the band width is steered to
 };
 
 template <typename T> struct FimArgs {

@@ -13,6 +13,7 @@
 
 #include "hfm_device.h"
 #include "selling.h"
+#include "hfm_tile.h"
 
 namespace hfm {
 
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(64) void k_tile_deps(Geom g, StencilData<T> sd, int
             for (int s = 0; s < nsides; ++s) {
                 const int sg = s == 0 ? 1 : -1;
                 int r[3];
-                if (!shift_coords(g, c, sg * o0, sg * o1, sg * o2, r)) continue;
+                if (!shift_coords_m<ModelT>(g, c, sg * o0, sg * o1, sg * o2, r)) continue;
                 const int t = tile_of(g, r);
                 if (t == tile) continue;
                 unsigned h = ((unsigned)t * 2654435761u) & (HASH_SLOTS - 1);
@@ -185,61 +186,6 @@ __global__ void k_set_seeds(const long long *__restrict__ seed_p, const double *
     atomicOr(&fixedmask[p >> 6], 1ull << (p & 63));
 }
 
-// Scheduling keys: float values mapped to order-preserving uint32 so that atomicMin works for
-// negative seed values too.  KEY_NONE (0xffffffff) means "nothing pending".
-__device__ inline unsigned key_of(float f) {
-    const unsigned b = __float_as_uint(f);
-    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
-}
-__device__ inline float key_value(unsigned k) {
-    if (k == KEY_NONE) return __builtin_huge_valf();
-    const unsigned b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
-    return __uint_as_float(b);
-}
-__device__ inline unsigned wave_min_u32(unsigned v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const unsigned w = (unsigned)__shfl_xor((int)v, o);
-        v = w < v ? w : v;
-    }
-    return v;
-}
-__device__ inline unsigned wave_max_u32(unsigned v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const unsigned w = (unsigned)__shfl_xor((int)v, o);
-        v = w > v ? w : v;
-    }
-    return v;
-}
-__device__ inline unsigned relaxed_load(const unsigned *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Mark tile d (d < 0: none) pending with key `key`: lowers keys[d] unless the tile's largest
-// value is already <= key (then no point of d can use a value >= key).  Returns the smallest key
-// marked by the wave (KEY_NONE if none); every lane of the wave calls it.
-__device__ inline unsigned mark_pending(int d, unsigned key, unsigned *__restrict__ keys,
-                                        const unsigned *__restrict__ tmax) {
-    bool lowered = false;
-    if (d >= 0 && relaxed_load(&tmax[d]) > key && relaxed_load(&keys[d]) > key) {
-        atomicMin(&keys[d], key);
-        lowered = true;
-    }
-    return wave_min_u32(lowered ? key : KEY_NONE);
-}
-
-// The smallest pending key is accumulated in PEND_SLOTS words (one per wave id modulo
-// PEND_SLOTS) so that no single address takes an atomic from every wave.
-__device__ inline unsigned read_pend(const unsigned *slots) {
-    unsigned m = KEY_NONE;
-    for (int i = 0; i < PEND_SLOTS; ++i) {
-        const unsigned v = slots[i];
-        m = v < m ? v : m;
-    }
-    return m;
-}
-
 // Seed tiles and their dependents are the first pending tiles, keyed by the seed value.
 __global__ __launch_bounds__(64) void k_seed_activate(const long long *__restrict__ seed_p,
                                                       const double *__restrict__ seed_v, int nseeds,
@@ -275,16 +221,18 @@ __global__ __launch_bounds__(64) void k_seed_activate(const long long *__restric
 // keys this kernel leaves pending and by the marks of the update kernel of round k.
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_schedule(int ntiles, unsigned *__restrict__ keys, SchedRound sr) {
-    const float pend = key_value(read_pend(sr.pend_in));
-    const float bprev = *sr.bound_in;
-    float B = __builtin_huge_valf();
-    if (sr.delta > 0.f && pend < __builtin_huge_valf()) B = pend < bprev ? bprev : pend + sr.delta;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *sr.bound_out = B;
+    float delta = sr.delta;
+    if (delta > 0.f) delta = adapt_delta(*sr.delta_in, *sr.prev_count, sr.target, 0.5f * sr.delta, 64.f * sr.delta);
+    const float B = band_bound(key_value(read_pend(sr.pend_in)), *sr.bound_in, delta);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *sr.bound_out = B;
+        *sr.delta_out = delta;
+    }
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     unsigned k = KEY_NONE;
     if (t < ntiles) k = keys[t];
-    const bool take = k != KEY_NONE && (key_value(k) < B || !(B < __builtin_huge_valf()));
+    const bool take = band_take(k, B);
     const unsigned left = (!take && k != KEY_NONE) ? k : KEY_NONE;
     if (take) keys[t] = KEY_NONE;
     const unsigned lmin = wave_min_u32(left);
@@ -301,15 +249,7 @@ __global__ __launch_bounds__(256) void k_schedule(int ntiles, unsigned *__restri
 }
 
 // ---------------------------------------------------------------------------------------------
-// Update kernel of one round of the block fast-iterative method.
-//
-// Each wave takes a listed tile, stages its values in LDS (one wavefront, 64 points), caches
-// the values of neighbours outside the tile in registers, and runs Jacobi sweeps of the local
-// solve until no lane changes (wave ballot) or MAXSWEEP sweeps.  If the tile changed, its
-// dependent tiles are marked pending with key = smallest lowered value (the tile itself too if
-// it did not converge).  Values of other tiles may be read while another wave lowers them: any
-// value read is an upper bound of the solution and the scheme is monotone, and the writer marks
-// its dependents pending, so the fixed point is unaffected.
+// Update kernel of one round: every wave processes listed tiles with fim_tile (hfm_tile.h).
 // ---------------------------------------------------------------------------------------------
 template <int ModelT, typename T, int MAXSWEEP>
 __global__ __launch_bounds__(256) void k_fim(Geom g, StencilData<T> sd, T *__restrict__ u,
@@ -319,98 +259,19 @@ __global__ __launch_bounds__(256) void k_fim(Geom g, StencilData<T> sd, T *__res
                                              int *__restrict__ count_reset, unsigned *__restrict__ keys,
                                              unsigned *__restrict__ tmax, unsigned *__restrict__ pend_out,
                                              unsigned *__restrict__ pend_reset, unsigned long long *__restrict__ stats) {
-    using S = ModelShape<ModelT>;
-    constexpr int NF = S::NFWD, NS = S::NSYM, NC = NF + NS;
     __shared__ T su[4][64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const T inf = dev_inf<T>();
     const int n = *count;
     if (blockIdx.x == 0) {
         if (threadIdx.x == 0) *count_reset = 0;
         if (threadIdx.x < PEND_SLOTS) pend_reset[threadIdx.x] = KEY_NONE;
     }
+    const TileCtx<T> ctx{g, sd, u, fixedmask, dep_start, deps, keys, tmax};
     unsigned long long my_sweeps = 0, my_visits = 0;
     unsigned wmin = KEY_NONE;
     for (int i = blockIdx.x * 4 + wv; i < n; i += gridDim.x * 4) {
         const int tile = list[i];
-        const long long p = (long long)tile * 64 + lane;
-        int c[3];
-        tiled_coords(g, tile, lane, c);
-        const bool fixed = (fixedmask[tile] >> lane) & 1ull;
-        const T u0 = u[p];
-        LaneStencil<ModelT, T> st;
-        st.load(g, sd, p, c);
-        // Neighbour slots: forward k -> one side, symmetric k -> two sides.
-        int la[NC], lb[NC];
-        T oa[NC], ob[NC];
-#pragma unroll
-        for (int k = 0; k < NC; ++k) {
-            int o0, o1, o2;
-            unpack_offset(st.off[k], o0, o1, o2);
-            la[k] = -1;
-            lb[k] = -1;
-            oa[k] = inf;
-            ob[k] = inf;
-            if (fixed || st.w[k] == T(0)) continue;
-            int r[3];
-            if (shift_coords(g, c, o0, o1, o2, r)) {
-                const int t = tile_of(g, r), l = local_of(g, r);
-                if (t == tile) la[k] = l;
-                else oa[k] = u[(long long)t * 64 + l];
-            }
-            if (k >= NF) {
-                if (shift_coords(g, c, -o0, -o1, -o2, r)) {
-                    const int t = tile_of(g, r), l = local_of(g, r);
-                    if (t == tile) lb[k] = l;
-                    else ob[k] = u[(long long)t * 64 + l];
-                }
-            }
-        }
-        T ucur = u0;
-        su[wv][lane] = ucur;
-        bool converged = false;
-        int sweeps = 0;
-        for (int it = 0; it < MAXSWEEP; ++it) {
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            T v[NC];
-#pragma unroll
-            for (int k = 0; k < NC; ++k) {
-                const T va = la[k] >= 0 ? su[wv][la[k]] : oa[k];
-                const T vb = lb[k] >= 0 ? su[wv][lb[k]] : ob[k];
-                v[k] = va < vb ? va : vb;
-            }
-            const T r = local_solve_update<NC, T>(v, st.w);
-            const bool ch = !fixed && r < ucur;
-            ucur = ch ? r : ucur;
-            ++sweeps;
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            su[wv][lane] = ucur;
-            if (__ballot(ch) == 0ull) {
-                converged = true;
-                break;
-            }
-        }
-        const bool lowered = ucur < u0;
-        if (lowered) u[p] = ucur;
-        const bool tile_changed = __ballot(lowered) != 0ull;
-        if (tile_changed) {
-            const unsigned tm = wave_max_u32(key_of((float)ucur));
-            if (lane == 0) tmax[tile] = tm;
-            const unsigned kmin = wave_min_u32(lowered ? key_of((float)ucur) : KEY_NONE);
-            const int b = dep_start[tile], e = dep_start[tile + 1];
-            for (int j0 = b; j0 < e; j0 += 64) {
-                const int j = j0 + lane;
-                const unsigned m = mark_pending(j < e ? deps[j] : -1, kmin, keys, tmax);
-                wmin = m < wmin ? m : wmin;
-            }
-            if (!converged) {
-                const unsigned m = mark_pending(lane == 0 ? tile : -1, kmin, keys, tmax);
-                wmin = m < wmin ? m : wmin;
-            }
-        }
-        my_sweeps += sweeps;
+        my_sweeps += fim_tile<ModelT, T, MAXSWEEP>(ctx, tile, lane, su[wv], wmin);
         my_visits += 1;
     }
     const int slot = ((blockIdx.x * 4 + wv) & (PEND_SLOTS - 1));
@@ -466,8 +327,8 @@ __global__ __launch_bounds__(256) void k_flow(Geom g, StencilData<T> sd, const T
         if (st.w[k] == T(0)) continue;
         int r[3];
         T va = inf, vb = inf;
-        if (shift_coords(g, c, o0, o1, o2, r)) va = u[(long long)tile_of(g, r) * 64 + local_of(g, r)];
-        if (k >= NF && shift_coords(g, c, -o0, -o1, -o2, r)) vb = u[(long long)tile_of(g, r) * 64 + local_of(g, r)];
+        if (shift_coords_m<ModelT>(g, c, o0, o1, o2, r)) va = u[(long long)tile_of(g, r) * 64 + local_of(g, r)];
+        if (k >= NF && shift_coords_m<ModelT>(g, c, -o0, -o1, -o2, r)) vb = u[(long long)tile_of(g, r) * 64 + local_of(g, r)];
         v[k] = va <= vb ? va : vb;
         side[k] = va <= vb ? 1 : -1;
     }

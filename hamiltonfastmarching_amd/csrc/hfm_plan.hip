@@ -272,8 +272,10 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
     HIPCHECK(hipMemsetAsync(p->counts, 0, 2 * sizeof(int), s));
     HIPCHECK(hipMemsetAsync(p->stats, 0, 2 * PEND_SLOTS * sizeof(unsigned long long), s));
     HIPCHECK(hipMemsetAsync(p->nextmin, 0xff, 2 * PEND_SLOTS * sizeof(unsigned), s));
-    const float binit[2] = {0.f, -__builtin_huge_valf()};
+    const float delta0 = (float)(p->delta_factor > 0 ? p->delta_factor * p->tau : 0.);
+    const float binit[4] = {0.f, -__builtin_huge_valf(), delta0, delta0};
     HIPCHECK(hipMemcpyAsync(p->bounds, binit, sizeof(binit), hipMemcpyHostToDevice, s));
+    const int target = p->round_target > 0 ? p->round_target : std::max(64, std::min(256, p->g.ntiles / 4));
     HIPCHECK(launch_seed_activate(p->seed_p, p->seed_v, nseeds, p->dep_start, p->deps, p->keys, p->tmax,
                                   &p->nextmin[0], s));
 
@@ -288,8 +290,8 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
     a.tmax = p->tmax;
     a.list = p->lists[0];
     a.stats = p->stats;
-    a.grid = (unsigned)std::min<long long>((p->g.ntiles + 3) / 4, 4096);
-    const float delta = (float)(p->delta_factor > 0 ? p->delta_factor * p->tau : 0.);
+    a.grid = (unsigned)std::min<long long>((p->g.ntiles + 3) / 4, p->fim_grid > 0 ? p->fim_grid : 1024);
+    const float delta = delta0;
 
     std::vector<hipEvent_t> tev;
     long long k = 0;
@@ -298,8 +300,17 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
     while (true) {
         for (int b = 0; b < batch; ++b, ++k) {
             // round k: schedule lists tiles into lists[0] / counts[k&1], update consumes them
-            SchedRound sr{&p->nextmin[(k & 1) * PEND_SLOTS], &p->nextmin[((k + 1) & 1) * PEND_SLOTS],
-                          &p->bounds[(k + 1) & 1], &p->bounds[k & 1], p->lists[0], &p->counts[k & 1], delta};
+            SchedRound sr{&p->nextmin[(k & 1) * PEND_SLOTS],
+                          &p->nextmin[((k + 1) & 1) * PEND_SLOTS],
+                          &p->bounds[(k + 1) & 1],
+                          &p->bounds[k & 1],
+                          p->lists[0],
+                          &p->counts[k & 1],
+                          delta,
+                          &p->bounds[2 + ((k + 1) & 1)],
+                          &p->bounds[2 + (k & 1)],
+                          &p->counts[(k + 1) & 1],
+                          target};
             HIPCHECK(launch_schedule(p->g.ntiles, p->keys, sr, s));
             a.count = &p->counts[k & 1];
             a.count_reset = &p->counts[(k + 1) & 1];
@@ -733,6 +744,24 @@ int hfm_plan_set_delta(hfm_plan *p, double factor) {
         return 1;
     }
     p->delta_factor = factor;
+    return 0;
+}
+
+int hfm_plan_set_grid(hfm_plan *p, int blocks) {
+    if (!p) {
+        set_error("hfm_plan_set_grid: null argument");
+        return 1;
+    }
+    p->fim_grid = blocks;
+    return 0;
+}
+
+int hfm_plan_set_round_target(hfm_plan *p, int tiles) {
+    if (!p) {
+        set_error("hfm_plan_set_round_target: null argument");
+        return 1;
+    }
+    p->round_target = tiles;
     return 0;
 }
 

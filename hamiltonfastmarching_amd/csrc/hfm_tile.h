@@ -1,0 +1,212 @@
+// hfm_tile.h -- per-tile update of the block fast-iterative method and its scheduling helpers,
+// shared by the single-solve kernels (hfm_kernels.hip) and the multi-solve serving kernels
+// (hfm_server.hip).
+#ifndef HFM_AMD_TILE_H
+#define HFM_AMD_TILE_H
+
+#include "hfm_device.h"
+#include "hfm_kernels.h"
+
+namespace hfm {
+
+// Scheduling keys: float values mapped to order-preserving uint32 so that atomicMin works for
+// negative seed values too.  KEY_NONE (0xffffffff) means "nothing pending".
+__device__ inline unsigned key_of(float f) {
+    const unsigned b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ inline float key_value(unsigned k) {
+    if (k == KEY_NONE) return __builtin_huge_valf();
+    const unsigned b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    return __uint_as_float(b);
+}
+__device__ inline unsigned wave_min_u32(unsigned v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned w = (unsigned)__shfl_xor((int)v, o);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+__device__ inline unsigned wave_max_u32(unsigned v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned w = (unsigned)__shfl_xor((int)v, o);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+__device__ inline unsigned relaxed_load(const unsigned *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Mark tile d (d < 0: none) pending with key `key`: lowers keys[d] unless the tile's largest
+// value is already <= key (then no point of d can use a value >= key).  Returns the smallest key
+// marked by the wave (KEY_NONE if none); every lane of the wave calls it.
+__device__ inline unsigned mark_pending(int d, unsigned key, unsigned *__restrict__ keys,
+                                        const unsigned *__restrict__ tmax) {
+    bool lowered = false;
+    if (d >= 0 && relaxed_load(&tmax[d]) > key && relaxed_load(&keys[d]) > key) {
+        atomicMin(&keys[d], key);
+        lowered = true;
+    }
+    return wave_min_u32(lowered ? key : KEY_NONE);
+}
+
+// The smallest pending key is accumulated in PEND_SLOTS words (one per wave id modulo
+// PEND_SLOTS) so that no single address takes an atomic from every wave.
+__device__ inline unsigned read_pend(const unsigned *slots) {
+    unsigned m = KEY_NONE;
+    for (int i = 0; i < PEND_SLOTS; ++i) {
+        const unsigned v = slots[i];
+        m = v < m ? v : m;
+    }
+    return m;
+}
+
+// Band bound of a round: stays at the previous bound while a pending key lies below it,
+// otherwise the smallest pending key + delta; +inf (every pending tile) when delta <= 0.
+__device__ inline float band_bound(float pend, float bprev, float delta) {
+    float B = __builtin_huge_valf();
+    if (delta > 0.f && pend < __builtin_huge_valf()) B = pend < bprev ? bprev : pend + delta;
+    return B;
+}
+
+// Band width of this round, adapted to the number of tiles the previous round listed: x1.5
+// below target/2, /1.5 above 2*target, kept within [dmin, dmax].  Every block of a round
+// computes the same value from the same inputs.
+__device__ inline float adapt_delta(float dprev, int prev_count, int target, float dmin, float dmax) {
+    float d = dprev;
+    if (prev_count < target / 2) d *= 1.5f;
+    else if (prev_count > 2 * target) d *= (1.f / 1.5f);
+    return fminf(fmaxf(d, dmin), dmax);
+}
+
+// Whether a tile with key k is taken by a round of bound B.
+__device__ inline bool band_take(unsigned k, float B) {
+    return k != KEY_NONE && (key_value(k) < B || !(B < __builtin_huge_valf()));
+}
+
+template <typename T> struct TileCtx {
+    Geom g;
+    StencilData<T> sd;
+    T *__restrict__ u;
+    const unsigned long long *__restrict__ fixedmask;
+    const int *__restrict__ dep_start;
+    const int *__restrict__ deps;
+    unsigned *__restrict__ keys;
+    unsigned *__restrict__ tmax;
+};
+
+// One visit of `tile` by one wavefront (lane = 0..63, su = the wave's 64-entry LDS row).
+//
+// The tile's values are staged in LDS, the values of neighbours outside the tile are cached in
+// registers, and Jacobi sweeps of the local solve run until no lane changes (wave ballot) or
+// MAXSWEEP sweeps.  If the tile changed, its dependent tiles are marked pending with key =
+// smallest lowered value (the tile itself too if it did not converge); wmin is lowered to the
+// smallest key marked.  Values of other tiles may be read while another wave lowers them: any
+// value read is an upper bound of the solution and the scheme is monotone, and the writer marks
+// its dependents pending, so the fixed point is unaffected.  Returns the number of sweeps.
+template <int ModelT, typename T, int MAXSWEEP>
+__device__ inline int fim_tile(const TileCtx<T> &ctx, int tile_in, int lane, T *su, unsigned &wmin) {
+    const int tile = __builtin_amdgcn_readfirstlane(tile_in);  // uniform: scalar address arithmetic
+    using S = ModelShape<ModelT>;
+    constexpr int NF = S::NFWD, NS = S::NSYM, NC = NF + NS;
+    const T inf = dev_inf<T>();
+    const Geom &g = ctx.g;
+    T *__restrict__ u = ctx.u;
+    const long long p = (long long)tile * 64 + lane;
+    int c[3];
+    tiled_coords(g, tile, lane, c);
+    const bool fixed = (ctx.fixedmask[tile] >> lane) & 1ull;
+    // Issue the loads the pending marks will need now, so that their latency overlaps the
+    // neighbour loads and the sweeps.  Within one update launch keys and tile maxima only
+    // decrease, so values read early are upper bounds and filter conservatively.
+    const int db = ctx.dep_start[tile], de = ctx.dep_start[tile + 1];
+    const int d0 = db + lane < de ? ctx.deps[db + lane] : -1;
+    const unsigned tmax0 = d0 >= 0 ? ctx.tmax[d0] : 0u;
+    const unsigned key0 = d0 >= 0 ? ctx.keys[d0] : 0u;
+    const T u0 = u[p];
+    LaneStencil<ModelT, T> st;
+    st.load(g, ctx.sd, p, c);
+    // Neighbour slots: forward k -> one side, symmetric k -> two sides.
+    int la[NC], lb[NC];
+    T oa[NC], ob[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        int o0, o1, o2;
+        unpack_offset(st.off[k], o0, o1, o2);
+        la[k] = -1;
+        lb[k] = -1;
+        oa[k] = inf;
+        ob[k] = inf;
+        if (fixed || st.w[k] == T(0)) continue;
+        int r[3];
+        if (shift_coords_m<ModelT>(g, c, o0, o1, o2, r)) {
+            const int t = tile_of(g, r), l = local_of(g, r);
+            if (t == tile) la[k] = l;
+            else oa[k] = u[(long long)t * 64 + l];
+        }
+        if (k >= NF) {
+            if (shift_coords_m<ModelT>(g, c, -o0, -o1, -o2, r)) {
+                const int t = tile_of(g, r), l = local_of(g, r);
+                if (t == tile) lb[k] = l;
+                else ob[k] = u[(long long)t * 64 + l];
+            }
+        }
+    }
+    T ucur = u0;
+    su[lane] = ucur;
+    bool converged = false;
+    int sweeps = 0;
+    for (int it = 0; it < MAXSWEEP; ++it) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        T v[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            const T va = la[k] >= 0 ? su[la[k]] : oa[k];
+            const T vb = lb[k] >= 0 ? su[lb[k]] : ob[k];
+            v[k] = va < vb ? va : vb;
+        }
+        const T r = local_solve_update<NC>(v, st.w);
+        const bool ch = !fixed && r < ucur;
+        ucur = ch ? r : ucur;
+        ++sweeps;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        su[lane] = ucur;
+        if (__ballot(ch) == 0ull) {
+            converged = true;
+            break;
+        }
+    }
+    const bool lowered = ucur < u0;
+    if (lowered) u[p] = ucur;
+    const bool tile_changed = __ballot(lowered) != 0ull;
+    if (tile_changed) {
+        const unsigned tm = wave_max_u32(key_of((float)ucur));
+        if (lane == 0) ctx.tmax[tile] = tm;
+        const unsigned kmin = wave_min_u32(lowered ? key_of((float)ucur) : KEY_NONE);
+        {
+            const bool lw = d0 >= 0 && tmax0 > kmin && key0 > kmin;
+            if (lw) atomicMin(&ctx.keys[d0], kmin);
+            const unsigned m = wave_min_u32(lw ? kmin : KEY_NONE);
+            wmin = m < wmin ? m : wmin;
+        }
+        for (int j0 = db + 64; j0 < de; j0 += 64) {
+            const int j = j0 + lane;
+            const unsigned m = mark_pending(j < de ? ctx.deps[j] : -1, kmin, ctx.keys, ctx.tmax);
+            wmin = m < wmin ? m : wmin;
+        }
+        if (!converged) {
+            const unsigned m = mark_pending(lane == 0 ? tile : -1, kmin, ctx.keys, ctx.tmax);
+            wmin = m < wmin ? m : wmin;
+        }
+    }
+    return sweeps;
+}
+
+}  // namespace hfm
+
+#endif

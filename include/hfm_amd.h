@@ -149,9 +149,72 @@ int hfm_plan_get_stencils(hfm_plan *plan, int8_t *h_offsets, double *h_weights);
  * tile in every round).  It changes the amount of work, never the fixed point. */
 int hfm_plan_set_delta(hfm_plan *plan, double factor);
 
+/* Tiles per round that the band width is steered to (doubling / halving each round; 0: a
+ * default of n_tiles / 8 clamped to [64, 8192]).  Changes the work, never the fixed point. */
+int hfm_plan_set_round_target(hfm_plan *plan, int tiles);
+
+/* Blocks (of 4 wavefronts, one tile per wavefront at a time) of each update-kernel launch
+ * (0: 1024).  A tuning knob; results do not depend on it. */
+int hfm_plan_set_grid(hfm_plan *plan, int blocks);
+
 /* Per-launch timing of the update kernel (hipEvent pairs around every launch; off by default
  * because the events add a few microseconds per launch).  Fills fim_ms / fim_launches. */
 int hfm_plan_set_timing(hfm_plan *plan, int enable);
+
+/* ----------------------------------------------------------------------------------------- */
+/* Continuous-batching solve server (no reference counterpart: the reference solves one      */
+/* problem per Run(IO&) call; this serves many Riemann2 / Riemann3 solves concurrently)      */
+/* ----------------------------------------------------------------------------------------- */
+typedef struct hfm_server hfm_server;
+
+typedef struct hfm_request {
+    const char *model;          /* "Riemann2" | "Riemann3" */
+    int64_t dims[3];            /* grid extents (dims[2] ignored in 2-D) */
+    double gridScale;           /* h, the same on every axis */
+    const double *metric;       /* FP64 row-major (dims..., d(d+1)/2), or one tensor if !metric_is_field */
+    int metric_is_field;
+    int metric_on_device;       /* 1: device pointer the caller keeps alive until completion;
+                                   0: host pointer, copied by hfm_server_submit */
+    int dual;                   /* 1: metric is the dual metric (dualMetric key) */
+    int nseeds;
+    const double *seeds;        /* host, nseeds x dim physical points (copied) */
+    const double *seed_values;  /* host, nseeds values or NULL (copied) */
+    double *values_out;         /* host buffer of prod(dims) doubles written on completion, or NULL */
+} hfm_request;
+
+typedef struct hfm_request_result {
+    int status;                 /* 0: solved; nonzero: failed (message in error) */
+    double latency_ms;          /* submit -> values available on the host */
+    double queue_ms;            /* submit -> admitted (HBM budget and resident-slot limit) */
+    double setup_ms;            /* admitted -> stencils and tile graph ready */
+    int64_t rounds;             /* scheduling rounds the solve took part in */
+    int64_t tile_visits;        /* tiles processed for this solve */
+    char error[256];
+} hfm_request_result;
+
+typedef struct hfm_server_stats {
+    int64_t submitted, completed, failed;
+    int64_t resident;           /* solves currently on the device */
+    int64_t max_resident_seen;
+    int64_t batches, rounds;    /* kernel rounds over the union of resident solves */
+    double hbm_in_use;          /* bytes reserved by admitted solves */
+    double hbm_budget;
+} hfm_server_stats;
+
+/* Create a server.  hbm_budget_bytes: device bytes that resident solves may reserve (requests
+ * wait until they fit; a request larger than the budget fails).  max_resident: cap on solves
+ * sharing the device.  The update runs in FP32 with FP64 Selling stencils. */
+int hfm_server_create(hfm_server **out, double hbm_budget_bytes, int max_resident);
+/* Enqueue a solve; returns at once with a ticket.  Thread-safe. */
+int hfm_server_submit(hfm_server *srv, const hfm_request *req, int64_t *ticket);
+/* *done = 1 once the ticket's result is available. */
+int hfm_server_poll(hfm_server *srv, int64_t ticket, int *done);
+/* Block until the ticket completes (timeout_ms < 0: no limit); fills res.  Returns nonzero on
+ * timeout or unknown ticket. */
+int hfm_server_wait(hfm_server *srv, int64_t ticket, double timeout_ms, hfm_request_result *res);
+int hfm_server_get_stats(hfm_server *srv, hfm_server_stats *st);
+/* Stop the worker threads (waits for in-flight solves) and free every device buffer. */
+void hfm_server_destroy(hfm_server *srv);
 
 /* Message of the last failing plan call on this thread ("" if none). */
 const char *hfm_last_error(void);
