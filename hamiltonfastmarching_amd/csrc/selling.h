@@ -76,31 +76,59 @@ template <int D> HFM_HD double sym_dot(const SymPacked<D> &c, const int *u, cons
     return sum;
 }
 
+// D v for an integer vector v held in doubles, with the reference's summation order: entry i is
+// sum_j D_ij v_j accumulated from 0 (the inner sum "sumi" of SymmetricMatrixType.hxx:18-30).
+template <int D> HFM_HD void sym_apply(const SymPacked<D> &c, const double *v, double *out) {
+    for (int i = 0; i < D; ++i) {
+        double sumi = 0.;
+        for (int j = 0; j < D; ++j) sumi += c(i, j) * v[j];
+        out[i] = sumi;
+    }
+}
+
+// <u, D v> = sum_i u_i (D v)_i accumulated from 0 -- the reference's outer sum.
+template <int D> HFM_HD double dot_cached(const double *u, const double *Dv) {
+    double sum = 0.;
+    for (int i = 0; i < D; ++i) sum += u[i] * Dv[i];
+    return sum;
+}
+
 // Tensor decomposition: offsets off[K][D] (int, before narrowing) and weights w[K].
+// The superbase vectors are small integers held exactly in doubles, and D sb_j is cached per
+// vector and refreshed when the vector changes; every scalar product therefore performs exactly
+// the floating-point operations of the reference's ScalarProduct, in the same order (negating a
+// vector negates its cached product exactly).
 template <int D>
 HFM_HD int selling_decompose(const SymPacked<D> &m, int (*off)[D], double *w, int maxIt = 200) {
-    constexpr int K = D * (D + 1) / 2;
-    int sb[D + 1][D];
+    double sb[D + 1][D];
+    double Dsb[D + 1][D];
     for (int i = 0; i <= D; ++i)
-        for (int j = 0; j < D; ++j) sb[i][j] = i < D ? (i == j) : -1;
+        for (int j = 0; j < D; ++j) sb[i][j] = i < D ? (i == j ? 1. : 0.) : -1.;
     if (sym_det<D>(m) <= 0 || sym_trace<D>(m) <= 0) return SELLING_NOT_SPD;
+    for (int i = 0; i <= D; ++i) sym_apply<D>(m, sb[i], Dsb[i]);
     bool reduced = true;
     for (int it = 0; reduced && it < maxIt; ++it) {
         reduced = false;
         for (int i = 0; i < D; ++i)
             for (int j = i + 1; j <= D; ++j) {
-                if (sym_dot<D>(m, sb[i], sb[j]) > 0) {
+                if (dot_cached<D>(sb[i], Dsb[j]) > 0) {
                     reduced = true;
                     if (D == 2) {
                         const int k = 3 - i - j;
                         for (int c = 0; c < D; ++c) sb[k][c] = sb[i][c] - sb[j][c];
+                        sym_apply<D>(m, sb[k], Dsb[k]);
                     } else {
                         int k0 = -1, k1 = -1;
                         for (int q = 0; q <= D; ++q)
                             if (q != i && q != j) { if (k0 < 0) k0 = q; else k1 = q; }
                         for (int c = 0; c < D; ++c) { sb[k0][c] += sb[i][c]; sb[k1][c] += sb[i][c]; }
+                        sym_apply<D>(m, sb[k0], Dsb[k0]);
+                        sym_apply<D>(m, sb[k1], Dsb[k1]);
                     }
-                    for (int c = 0; c < D; ++c) sb[i][c] = -sb[i][c];
+                    for (int c = 0; c < D; ++c) {
+                        sb[i][c] = -sb[i][c];
+                        Dsb[i][c] = -Dsb[i][c];
+                    }
                 }
             }
     }
@@ -108,9 +136,9 @@ HFM_HD int selling_decompose(const SymPacked<D> &m, int (*off)[D], double *w, in
     if (D == 2) {
         for (int i = 0; i < 3; ++i) {
             const int j = (i + 1) % 3, k = (i + 2) % 3;
-            off[i][0] = -sb[i][1];
-            off[i][1] = sb[i][0];
-            w[i] = -sym_dot<D>(m, sb[j], sb[k]);
+            off[i][0] = -(int)sb[i][1];
+            off[i][1] = (int)sb[i][0];
+            w[i] = -dot_cached<D>(sb[j], Dsb[k]);
         }
     } else {
         int s = 0;
@@ -121,13 +149,12 @@ HFM_HD int selling_decompose(const SymPacked<D> &m, int (*off)[D], double *w, in
                     if (q != i && q != j) { if (r0 < 0) r0 = q; else r1 = q; }
                 for (int c = 0; c < 3; ++c) {
                     const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
-                    off[s][c] = sb[r0][c1] * sb[r1][c2] - sb[r0][c2] * sb[r1][c1];
+                    off[s][c] = (int)sb[r0][c1] * (int)sb[r1][c2] - (int)sb[r0][c2] * (int)sb[r1][c1];
                 }
-                w[s] = -sym_dot<D>(m, sb[i], sb[j]);
+                w[s] = -dot_cached<D>(sb[i], Dsb[j]);
                 ++s;
             }
     }
-    (void)K;
     return SELLING_OK;
 }
 
