@@ -231,6 +231,14 @@ class Plan:
                                           _stream_ptr(torch)))
         return fl, ac
 
+    def active_offsets(self):
+        """activeOffsets (HFMInterface.hxx:554-577): dims + (nslots,) FP64 on the device."""
+        torch = _torch()
+        ns = self._L.hfm_plan_n_active_slots(self._h)
+        out = torch.empty(self.dims + (ns,), dtype=torch.float64, device="cuda")
+        self._check(self._L.hfm_plan_active_offsets(self._h, ctypes.c_void_p(out.data_ptr()), _stream_ptr(torch)))
+        return out
+
     def adjoint(self, targets, weights=None, max_seeds=4096):
         torch = _torch()
         t = np.ascontiguousarray(np.asarray(targets, np.float64).reshape(-1, self.dim))

@@ -157,6 +157,7 @@ template <> __device__ inline double dev_inf<double>() { return __builtin_huge_v
 // v < current root, in the reference's arithmetic (Headers/Base/Implementation/
 // EulerianStencil.hxx:30-72): with m the smallest accepted value, a += w, b += w (v-m),
 // c += (w (v-m)) (v-m) starting from c = -1, root = m + (b + sqrt(max(0, b^2 - a c))) / a.
+// Zero-weight candidates are accepted like the others (the root stays +inf until a > 0).
 // Returns the root (+inf if no finite candidate) and the bitmask of accumulated slots.
 template <int NC, typename T>
 __device__ inline T local_solve(const T (&v)[NC], const T (&w)[NC], uint32_t &used) {
@@ -183,7 +184,7 @@ __device__ inline T local_solve(const T (&v)[NC], const T (&w)[NC], uint32_t &us
         c += wv2;
         const T d2 = b * b - a * c;
         const T d = sqrt(d2 > T(0) ? d2 : T(0));
-        r = m + (b + d) / a;
+        r = a > T(0) ? m + (b + d) / a : inf;  // no weight yet: +inf (QuadraticMax::Solve skips a == 0)
         sel |= 1u << bk;
     }
     used = sel;
@@ -222,6 +223,45 @@ template <int ModelT, typename T> struct LaneStencil {
 // candidates in the same order, but the test "next v below the current root" is evaluated
 // without the root as f(v) < 1, f(t) = sum_accepted w (t - v_i)^2 (f is increasing above the
 // accepted values and f(root) = 1), so the square root and division run once per solve.
+// Compare-exchange networks that sort NC values (checked exhaustively with the 0-1 principle and
+// all permutations): the optimal 3-, 5- and 12-comparator networks for 3, 4 and 6 inputs, and
+// odd-even transposition for other sizes up to 8.
+template <int NC> struct SortNet {
+    static constexpr int N = NC * (NC - 1) / 2;  // NC rounds of alternately floor(NC/2), floor((NC-1)/2) pairs
+    __host__ __device__ static constexpr int pair(int q, int side) {
+        // odd-even transposition: rounds r = 0..NC-1, pairs (j, j+1), j = r&1, r&1 + 2, ...
+        int r = 0;
+        while (true) {
+            const int cnt = (NC - (r & 1)) / 2;
+            if (q < cnt) return (r & 1) + 2 * q + side;
+            q -= cnt;
+            ++r;
+        }
+    }
+};
+template <> struct SortNet<3> {
+    static constexpr int N = 3;
+    __host__ __device__ static constexpr int pair(int q, int side) {
+        constexpr int p[3][2] = {{1, 2}, {0, 2}, {0, 1}};
+        return p[q][side];
+    }
+};
+template <> struct SortNet<4> {
+    static constexpr int N = 5;
+    __host__ __device__ static constexpr int pair(int q, int side) {
+        constexpr int p[5][2] = {{0, 1}, {2, 3}, {0, 2}, {1, 3}, {1, 2}};
+        return p[q][side];
+    }
+};
+template <> struct SortNet<6> {
+    static constexpr int N = 12;
+    __host__ __device__ static constexpr int pair(int q, int side) {
+        constexpr int p[12][2] = {{0, 5}, {1, 3}, {2, 4}, {1, 2}, {3, 4}, {0, 3},
+                                  {2, 5}, {0, 1}, {2, 3}, {4, 5}, {1, 2}, {3, 4}};
+        return p[q][side];
+    }
+};
+
 template <int NC, typename T> __device__ inline T local_solve_update(const T (&v)[NC], const T (&w)[NC]) {
     uint32_t used;
     return local_solve<NC, T>(v, w, used);
@@ -235,36 +275,39 @@ template <int NC> __device__ inline float local_solve_update(const float (&v0)[N
         w[k] = w0[k];
     }
     if constexpr (NC <= 8) {
-        // odd-even transposition network: NC rounds of compare-exchange on (v, w) pairs
+        // sort (v, w) pairs by v with a compare-exchange network (SortNet<NC>)
 #pragma unroll
-        for (int rnd = 0; rnd < NC; ++rnd) {
-#pragma unroll
-            for (int j = (rnd & 1); j + 1 < NC; j += 2) {
-                const float a0 = v[j], a1 = v[j + 1], b0 = w[j], b1 = w[j + 1];
-                const bool sw = a1 < a0;
-                v[j] = fminf(a0, a1);
-                v[j + 1] = fmaxf(a0, a1);
-                w[j] = sw ? b1 : b0;
-                w[j + 1] = sw ? b0 : b1;
-            }
+        for (int q = 0; q < SortNet<NC>::N; ++q) {
+            const int i = SortNet<NC>::pair(q, 0), j = SortNet<NC>::pair(q, 1);
+            const float a0 = v[i], a1 = v[j], b0 = w[i], b1 = w[j];
+            const bool sw = a1 < a0;
+            v[i] = sw ? a1 : a0;
+            v[j] = sw ? a0 : a1;
+            w[i] = sw ? b1 : b0;
+            w[j] = sw ? b0 : b1;
         }
         if (!(v[0] < inf)) return inf;
+        // Accept sorted neighbours while the quadratic through the accepted ones stays below 1
+        // at the next value: f(t) = a t^2 - 2 b t + c < 1 (t relative to the smallest value m),
+        // the square-root-free form of "next value below the current root".  Explicit fused
+        // multiply-adds: FP32 only, never on the FP64 path.
         const float m = v[0];
         float a = w[0], b = 0.f, c = 0.f;
         bool go = true;
 #pragma unroll
         for (int k = 1; k < NC; ++k) {
             const float t = v[k] - m;
-            go = go && (t * (a * t - 2.f * b) + c < 1.f);
+            go = go && (__builtin_fmaf(t, __builtin_fmaf(a, t, -2.f * b), c) < 1.f);
             const float wk = go ? w[k] : 0.f;
             const float tk = go ? t : 0.f;
+            const float wt = wk * tk;
             a += wk;
-            b += wk * tk;
-            c += (wk * tk) * tk;
+            b += wt;
+            c = __builtin_fmaf(wt, tk, c);
         }
         // hardware square root and reciprocal (1 ulp each): they perturb only the increment
         // over the smallest neighbour value, far below the FP32 rounding of u itself
-        const float d2 = b * b - a * (c - 1.f);
+        const float d2 = __builtin_fmaf(-a, c - 1.f, b * b);
         return m + (b + __builtin_amdgcn_sqrtf(d2 > 0.f ? d2 : 0.f)) * __builtin_amdgcn_rcpf(a);
     } else {
         float m = 0.f, a = 0.f, b = 0.f, c = 0.f;
@@ -301,17 +344,63 @@ template <int NC> __device__ inline float local_solve_update(const float (&v0)[N
     }
 }
 
-// Wave-reduced accumulation of (sum, count) into tau_acc[0..1]; every lane of the wave calls it.
-__device__ inline void wave_add_tau(double v, double c, double *tau_acc) {
+// Launch-wide reductions (mean one-step time, widest stencil) go to RED_SLOTS accumulators, each
+// on its own 256-byte line, chosen by the global wave index.  A single accumulator would
+// serialise every wave of the launch on one address (tens of thousands of same-address atomics
+// per 128^3 grid); the host sums the slots after the launch.
+constexpr int RED_SLOTS = 256;
+struct alignas(256) RedSlot {
+    double sum;
+    double count;
+    int l1max;
+    int pad_[59];
+};
+static_assert(sizeof(RedSlot) == 256, "RedSlot is one 256-byte line");
+
+#if defined(__HIPCC__)
+__device__ inline RedSlot *red_slot(RedSlot *red) {
+    const unsigned wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    return red + (wave & (RED_SLOTS - 1));
+}
+
+// Wave-reduced accumulation of (sum, count) into this wave's slot; every lane of the wave calls it.
+__device__ inline void wave_add_tau(double v, double c, RedSlot *red) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         v += __shfl_xor(v, o);
         c += __shfl_xor(c, o);
     }
     if ((threadIdx.x & 63) == 0 && c > 0.) {
-        atomicAdd(&tau_acc[0], v);
-        atomicAdd(&tau_acc[1], c);
+        RedSlot *r = red_slot(red);
+        atomicAdd(&r->sum, v);
+        atomicAdd(&r->count, c);
     }
+}
+
+// Wave-reduced maximum into this wave's slot; every lane of the wave calls it.
+__device__ inline void wave_max_l1(int v, RedSlot *red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const int t = __shfl_xor(v, o);
+        v = t > v ? t : v;
+    }
+    if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(&red_slot(red)->l1max, v);
+}
+#endif
+
+// Host-side total of the slots (copied back after the launch).
+struct RedTotals {
+    double sum = 0., count = 0.;
+    int l1max = 0;
+};
+inline RedTotals red_total(const RedSlot *h) {
+    RedTotals t;
+    for (int i = 0; i < RED_SLOTS; ++i) {
+        t.sum += h[i].sum;
+        t.count += h[i].count;
+        t.l1max = h[i].l1max > t.l1max ? h[i].l1max : t.l1max;
+    }
+    return t;
 }
 
 }  // namespace hfm

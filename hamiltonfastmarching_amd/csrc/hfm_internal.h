@@ -70,12 +70,13 @@ struct hfm_plan {
     unsigned *tmax = nullptr;      // per-tile largest value (as key) after its last visit
     unsigned *nextmin = nullptr;   // 2 alternating smallest pending keys
     float *bounds = nullptr;       // 2 alternating band bounds + 2 alternating band widths
-    double *tau_acc = nullptr;     // (sum, count) of one-step times
+    hfm::RedSlot *red = nullptr;   // spread launch-wide reductions (mean one-step time, widest stencil)
+    std::vector<hfm::RedSlot> h_red;  // host copy of the slots
     double tau = 0.;               // mean one-step time of the current stencils
     double delta_factor = 8.;      // initial band width in units of tau (<= 0: unordered)
     int fim_grid = 0;              // update-kernel blocks of 4 waves (0: 1024)
     int round_target = 0;          // tiles per round the band width is steered to (0: ntiles/8)
-    int *status = nullptr;  // stencil status
+    int *status = nullptr;  // stencil status: worst Selling error, smallest failing row-major index
     unsigned long long *stats = nullptr;
     long long *seed_p = nullptr;
     double *seed_v = nullptr;

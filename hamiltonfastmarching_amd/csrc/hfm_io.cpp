@@ -8,7 +8,7 @@
 //                 (Riemannian.h:46-61, Curvature2.h:104-108,219-224, HFM_StencilShare.hxx:61-71)
 //   SetupSolver   MaxStencilWidth, seeds / seedValues                      (:354-410)
 //   RunSolver     FMCPUTime (wall-clock seconds of the GPU solve here)     (:422-450)
-//   ExportData    values, activeNeighs, geodesicFlow                        (:526-579)
+//   ExportData    values, activeNeighs, geodesicFlow, activeOffsets         (:526-579)
 //   Finally       inspectSensitivity -> costSensitivity_<i>, seedSensitivity_<i>
 //                 (Headers/ExtraAlgorithms/FirstVariation.h:60-102)
 // Arrays are FP64, stored row-major with element components last; arrayOrdering permutes the
@@ -386,6 +386,15 @@ void run_impl(hfm_io *io) {
                 throw IOError{"hipMemcpy failed"};
             set_grid_array(io, "geodesicFlow", f, dims, dim);
         }
+    }
+    if (io->scalar("exportActiveOffsets", 0.) != 0.) {
+        const int ns = hfm_plan_n_active_slots(plan);
+        DevBuf dao(N * ns * sizeof(double));
+        check(hfm_plan_active_offsets(plan, dao.d(), nullptr));
+        std::vector<double> a(N * ns);
+        if (hipMemcpy(a.data(), dao.p, N * ns * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+            throw IOError{"hipMemcpy failed"};
+        set_grid_array(io, "activeOffsets", a, dims, ns);
     }
     // FirstVariation::Finally (backward part)
     if (io->has("inspectSensitivity")) {

@@ -58,7 +58,7 @@ template <typename T> struct FimArgs {
 
 template <typename T>
 hipError_t launch_riemann_stencil(const Geom &g, const double *metric, int is_field, int dual, const InvHH &ih,
-                                  uint32_t *offs, T *w, long long np, int *status, double *tau_acc, hipStream_t s);
+                                  uint32_t *offs, T *w, long long np, int *status, RedSlot *red, hipStream_t s);
 template <typename T>
 hipError_t launch_tile_deps(int model, const Geom &g, const StencilData<T> &sd, int pass, int *cnt, const int *start,
                             int *fill, int *deps, int *ovf, hipStream_t s);
@@ -74,7 +74,7 @@ template <typename T>
 hipError_t launch_export_values(const Geom &g, const T *u, double *out, long long np, hipStream_t s);
 template <typename T>
 hipError_t launch_flow(int model, const Geom &g, const StencilData<T> &sd, const T *u, const unsigned long long *fm,
-                       const FlowScale &fs, double *flow, double *act, long long np, hipStream_t s);
+                       const FlowScale &fs, double *flow, double *act, double *aoffs, long long np, hipStream_t s);
 
 }  // namespace hfm
 

@@ -24,13 +24,14 @@ template <int D, typename T>
 __global__ __launch_bounds__(256) void k_riemann_stencil(Geom g, const double *__restrict__ metric, int metric_is_field,
                                                          int dual, InvHH ih, uint32_t *__restrict__ offs,
                                                          T *__restrict__ w, long long np, int *__restrict__ status,
-                                                         double *__restrict__ tau_acc) {
+                                                         RedSlot *__restrict__ red) {
     constexpr int K = D * (D + 1) / 2;
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     int c[3];
     tiled_coords(g, (int)(p >> 6), (int)(p & 63), c);
     double tau_v = 0., tau_c = 0.;
+    int l1max = 0;
     if (!in_grid(g, c)) {
         for (int k = 0; k < K; ++k) {
             offs[k * np + p] = 0;
@@ -54,15 +55,13 @@ __global__ __launch_bounds__(256) void k_riemann_stencil(Geom g, const double *_
             atomicMax(&status[0], st);
             atomicMin(&status[1], (int)lin);
         }
-        // status[2]: max L1 norm of a nonzero-weight offset (MaxStencilWidth, HFMInterface.hxx:357)
-        int l1max = 0;
+        // max L1 norm of a nonzero-weight offset (MaxStencilWidth, HFMInterface.hxx:357)
         for (int k = 0; k < K; ++k) {
             if (wd[k] == 0.) continue;
             int l1 = 0;
             for (int q = 0; q < D; ++q) l1 += o8[k][q] < 0 ? -o8[k][q] : o8[k][q];
             l1max = l1 > l1max ? l1 : l1max;
         }
-        atomicMax(&status[2], l1max);
         // Mean one-step time 1/sqrt(sum_k w_k): sets the width of the scheduling bands.
         double sw = 0.;
         for (int k = 0; k < K; ++k) sw += wd[k];
@@ -71,7 +70,8 @@ __global__ __launch_bounds__(256) void k_riemann_stencil(Geom g, const double *_
             tau_c = 1.;
         }
     }
-    wave_add_tau(tau_v, tau_c, tau_acc);
+    wave_add_tau(tau_v, tau_c, red);
+    wave_max_l1(l1max, red);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -119,41 +119,83 @@ __global__ __launch_bounds__(64) void k_tile_deps(Geom g, StencilData<T> sd, int
         }
     }
     __syncthreads();
-    for (int i = lane; i < HASH_SLOTS; i += 64) {
+    // Compact the set to the front of the table, so the global atomics below take
+    // ceil(sources / 64) passes with every lane busy instead of one pass per 64 slots.
+    int mine[HASH_SLOTS / 64];
+    int nmine = 0;
+#pragma unroll
+    for (int j = 0; j < HASH_SLOTS / 64; ++j) {
+        const int v = set[j * 64 + lane];
+        mine[j] = v;
+        nmine += v >= 0;
+    }
+    int incl = nmine;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    const int nsrc = __shfl(incl, 63);
+    __syncthreads();
+    int pos = incl - nmine;
+#pragma unroll
+    for (int j = 0; j < HASH_SLOTS / 64; ++j)
+        if (mine[j] >= 0) set[pos++] = mine[j];
+    __syncthreads();
+    for (int i = lane; i < nsrc; i += 64) {
         const int src = set[i];
-        if (src < 0) continue;
         if (pass == 0) {
             atomicAdd(&dep_count[src], 1);
         } else {
-            const int pos = atomicAdd(&dep_fill[src], 1);
-            deps[dep_start[src] + pos] = tile;
+            const int q = atomicAdd(&dep_fill[src], 1);
+            deps[dep_start[src] + q] = tile;
         }
     }
     (void)overflow;
 }
 
-// Exclusive scan of n counts into start[0..n] (one block; setup-time only).
+// Exclusive scan of n counts into start[0..n] (one block of 1024; setup-time only).  The array is
+// swept in coalesced chunks of 4096 (four consecutive counts per thread); each chunk is scanned
+// with wave shuffles plus one LDS pass over the 16 wave totals, and carries into the next chunk.
 __global__ __launch_bounds__(1024) void k_exclusive_scan(const int *__restrict__ cnt, int *__restrict__ start, int n) {
-    __shared__ int part[1024];
-    const int tid = threadIdx.x;
-    const int per = (n + 1023) / 1024;
-    const int b = tid * per, e = min(n, b + per);
-    int s = 0;
-    for (int i = b; i < e; ++i) s += cnt[i];
-    part[tid] = s;
+    __shared__ int wsum[16];
+    __shared__ int carry_s;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    if (tid == 0) carry_s = 0;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int v = tid >= off ? part[tid - off] : 0;
+    for (int base = 0; base < n; base += 4096) {
+        const int i0 = base + tid * 4;
+        int v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = i0 + k < n ? cnt[i0 + k] : 0;
+        const int tsum = v[0] + v[1] + v[2] + v[3];
+        int incl = tsum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(incl, o);
+            if (lane >= o) incl += t;
+        }
+        if (lane == 63) wsum[wid] = incl;
+        const int carry = carry_s;
         __syncthreads();
-        part[tid] += v;
+        int wpre = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < 16; ++w) {
+            const int x = wsum[w];
+            wpre += w < wid ? x : 0;
+            total += x;
+        }
+        int run = carry + wpre + incl - tsum;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (i0 + k < n) start[i0 + k] = run;
+            run += v[k];
+        }
+        __syncthreads();
+        if (tid == 0) carry_s = carry + total;
         __syncthreads();
     }
-    int run = tid == 0 ? 0 : part[tid - 1];
-    for (int i = b; i < e; ++i) {
-        start[i] = run;
-        run += cnt[i];
-    }
-    if (tid == 1023) start[n] = part[1023];
+    if (tid == 0) start[n] = carry_s;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -259,7 +301,7 @@ __global__ __launch_bounds__(256) void k_fim(Geom g, StencilData<T> sd, T *__res
                                              int *__restrict__ count_reset, unsigned *__restrict__ keys,
                                              unsigned *__restrict__ tmax, unsigned *__restrict__ pend_out,
                                              unsigned *__restrict__ pend_reset, unsigned long long *__restrict__ stats) {
-    __shared__ T su[4][64];
+    __shared__ T su[4][SU_ROW];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int n = *count;
     if (blockIdx.x == 0) {
@@ -300,11 +342,26 @@ __global__ __launch_bounds__(256) void k_export_values(Geom g, const T *__restri
 // Geodesic flow and active-neighbour bits, recomputed from the converged values with the
 // accepted (v < u) differences (Headers/Base/Implementation/HamiltonFastMarching.hxx:187-272,
 // 363-378, EulerianStencil.hxx:155-233).  flow = -ReDim(sum_k w_k (u - v_k)^+ e_k).
+// Whether a nonzero-weight difference of the lane's stencil reaches offset (o0, o1, o2): a
+// forward one with that offset, or a symmetric one with that offset or its opposite.
+template <int ModelT, typename T>
+__device__ inline bool shares_offset(const LaneStencil<ModelT, T> &st, int k, int o0, int o1, int o2) {
+    constexpr int NF = ModelShape<ModelT>::NFWD, NC = NF + ModelShape<ModelT>::NSYM;
+    for (int j = 0; j < NC; ++j) {
+        if (j == k || st.w[j] == T(0)) continue;
+        int p0, p1, p2;
+        unpack_offset(st.off[j], p0, p1, p2);
+        if (p0 == o0 && p1 == o1 && p2 == o2) return true;
+        if (j >= NF && p0 == -o0 && p1 == -o1 && p2 == -o2) return true;
+    }
+    return false;
+}
+
 template <int ModelT, typename T>
 __global__ __launch_bounds__(256) void k_flow(Geom g, StencilData<T> sd, const T *__restrict__ u,
                                               const unsigned long long *__restrict__ fixedmask, FlowScale fs,
                                               double *__restrict__ flow, double *__restrict__ active_out,
-                                              long long np) {
+                                              double *__restrict__ aoffs_out, long long np) {
     using S = ModelShape<ModelT>;
     constexpr int NF = S::NFWD, NC = NF + S::NSYM;
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -324,11 +381,16 @@ __global__ __launch_bounds__(256) void k_flow(Geom g, StencilData<T> sd, const T
         unpack_offset(st.off[k], o0, o1, o2);
         v[k] = inf;
         side[k] = 0;
-        if (st.w[k] == T(0)) continue;
+        // A neighbour enters the reversed stencil only through a nonzero-weight difference
+        // (HFM_StencilRecompute.hxx:92, HFM_StencilShare.hxx:153), but its acceptance activates
+        // every difference with that offset (EulerianStencil.hxx:114-137): a zero-weight
+        // difference is active only if a nonzero-weight one shares its (signed) offset.
+        const bool pa = st.w[k] != T(0) || shares_offset<ModelT>(st, k, o0, o1, o2);
+        const bool pb = k >= NF && (st.w[k] != T(0) || shares_offset<ModelT>(st, k, -o0, -o1, -o2));
         int r[3];
         T va = inf, vb = inf;
-        if (shift_coords_m<ModelT>(g, c, o0, o1, o2, r)) va = u[(long long)tile_of(g, r) * 64 + local_of(g, r)];
-        if (k >= NF && shift_coords_m<ModelT>(g, c, -o0, -o1, -o2, r)) vb = u[(long long)tile_of(g, r) * 64 + local_of(g, r)];
+        if (pa && shift_coords_m<ModelT>(g, c, o0, o1, o2, r)) va = u[(long long)tile_of(g, r) * 64 + local_of(g, r)];
+        if (pb && shift_coords_m<ModelT>(g, c, -o0, -o1, -o2, r)) vb = u[(long long)tile_of(g, r) * 64 + local_of(g, r)];
         v[k] = va <= vb ? va : vb;
         side[k] = va <= vb ? 1 : -1;
     }
@@ -376,6 +438,22 @@ __global__ __launch_bounds__(256) void k_flow(Geom g, StencilData<T> sd, const T
         for (int q = 0; q < g.dim; ++q) flow[lin * g.dim + q] = -(f[q] * fs.h[q]);
     }
     if (active_out) active_out[lin] = bits;
+    if (aoffs_out) {
+        // activeOffsets (HFMInterface.hxx:554-577): the active differences in slot order (a
+        // symmetric one with the sign of its active side), each offset packed big-endian as
+        // sum_q uint8(o_q) << 8 (DIM - 1 - q) into a double; unused entries 0.
+        int j = 0;
+        for (int k = 0; k < NC; ++k) {
+            if (!(used >> k & 1u)) continue;
+            int o[3];
+            unpack_offset(st.off[k], o[0], o[1], o[2]);
+            unsigned long long packed = 0;
+            for (int q = 0; q < S::DIM; ++q)
+                packed |= (unsigned long long)(uint8_t)(int8_t)(side[k] * o[q]) << (8 * (S::DIM - 1 - q));
+            aoffs_out[lin * NC + j++] = (double)packed;
+        }
+        for (; j < NC; ++j) aoffs_out[lin * NC + j] = 0.;
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -385,19 +463,19 @@ static inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + 
 
 template <typename T>
 hipError_t launch_riemann_stencil(const Geom &g, const double *metric, int is_field, int dual, const InvHH &ih,
-                                  uint32_t *offs, T *w, long long np, int *status, double *tau_acc, hipStream_t s) {
+                                  uint32_t *offs, T *w, long long np, int *status, RedSlot *red, hipStream_t s) {
     if (g.dim == 2)
         hipLaunchKernelGGL((k_riemann_stencil<2, T>), dim3(blocks_for(np, 256)), dim3(256), 0, s, g, metric, is_field,
-                           dual, ih, offs, w, np, status, tau_acc);
+                           dual, ih, offs, w, np, status, red);
     else
         hipLaunchKernelGGL((k_riemann_stencil<3, T>), dim3(blocks_for(np, 256)), dim3(256), 0, s, g, metric, is_field,
-                           dual, ih, offs, w, np, status, tau_acc);
+                           dual, ih, offs, w, np, status, red);
     return hipGetLastError();
 }
 template hipError_t launch_riemann_stencil<float>(const Geom &, const double *, int, int, const InvHH &, uint32_t *,
-                                                  float *, long long, int *, double *, hipStream_t);
+                                                  float *, long long, int *, RedSlot *, hipStream_t);
 template hipError_t launch_riemann_stencil<double>(const Geom &, const double *, int, int, const InvHH &, uint32_t *,
-                                                   double *, long long, int *, double *, hipStream_t);
+                                                   double *, long long, int *, RedSlot *, hipStream_t);
 
 template <int M, typename T>
 static void deps_pass(const Geom &g, const StencilData<T> &sd, int pass, int *cnt, const int *start, int *fill,
@@ -483,27 +561,28 @@ template hipError_t launch_export_values<double>(const Geom &, const double *, d
 
 template <int M, typename T>
 static void flow_one(const Geom &g, const StencilData<T> &sd, const T *u, const unsigned long long *fm,
-                     const FlowScale &fs, double *flow, double *act, long long np, hipStream_t s) {
-    hipLaunchKernelGGL((k_flow<M, T>), dim3(blocks_for(np, 256)), dim3(256), 0, s, g, sd, u, fm, fs, flow, act, np);
+                     const FlowScale &fs, double *flow, double *act, double *aoffs, long long np, hipStream_t s) {
+    hipLaunchKernelGGL((k_flow<M, T>), dim3(blocks_for(np, 256)), dim3(256), 0, s, g, sd, u, fm, fs, flow, act, aoffs,
+                       np);
 }
 
 template <typename T>
 hipError_t launch_flow(int model, const Geom &g, const StencilData<T> &sd, const T *u, const unsigned long long *fm,
-                       const FlowScale &fs, double *flow, double *act, long long np, hipStream_t s) {
+                       const FlowScale &fs, double *flow, double *act, double *aoffs, long long np, hipStream_t s) {
     switch (model) {
-        case MODEL_RIEMANN2: flow_one<MODEL_RIEMANN2, T>(g, sd, u, fm, fs, flow, act, np, s); break;
-        case MODEL_RIEMANN3: flow_one<MODEL_RIEMANN3, T>(g, sd, u, fm, fs, flow, act, np, s); break;
-        case MODEL_RSF2: flow_one<MODEL_RSF2, T>(g, sd, u, fm, fs, flow, act, np, s); break;
-        case MODEL_ELASTICA2: flow_one<MODEL_ELASTICA2, T>(g, sd, u, fm, fs, flow, act, np, s); break;
+        case MODEL_RIEMANN2: flow_one<MODEL_RIEMANN2, T>(g, sd, u, fm, fs, flow, act, aoffs, np, s); break;
+        case MODEL_RIEMANN3: flow_one<MODEL_RIEMANN3, T>(g, sd, u, fm, fs, flow, act, aoffs, np, s); break;
+        case MODEL_RSF2: flow_one<MODEL_RSF2, T>(g, sd, u, fm, fs, flow, act, aoffs, np, s); break;
+        case MODEL_ELASTICA2: flow_one<MODEL_ELASTICA2, T>(g, sd, u, fm, fs, flow, act, aoffs, np, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 template hipError_t launch_flow<float>(int, const Geom &, const StencilData<float> &, const float *,
-                                       const unsigned long long *, const FlowScale &, double *, double *, long long,
-                                       hipStream_t);
+                                       const unsigned long long *, const FlowScale &, double *, double *, double *,
+                                       long long, hipStream_t);
 template hipError_t launch_flow<double>(int, const Geom &, const StencilData<double> &, const double *,
-                                        const unsigned long long *, const FlowScale &, double *, double *, long long,
-                                        hipStream_t);
+                                        const unsigned long long *, const FlowScale &, double *, double *, double *,
+                                        long long, hipStream_t);
 
 }  // namespace hfm

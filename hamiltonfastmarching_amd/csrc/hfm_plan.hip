@@ -31,7 +31,7 @@ void set_error(const std::string &m) { g_last_error = m; }
 
 // speed = 1 / cost in the tiled layout (HFMInterface.hxx:131-150 DataSource_Inverse).
 template <typename T>
-__global__ void k_speed_from_cost(Geom g, const double *cost, int is_field, T *speed, long long np, double *tau_acc) {
+__global__ void k_speed_from_cost(Geom g, const double *cost, int is_field, T *speed, long long np, RedSlot *red) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     int c[3];
@@ -45,7 +45,7 @@ __global__ void k_speed_from_cost(Geom g, const double *cost, int is_field, T *s
         cc = 1.;
         speed[p] = (T)(1. / cv);
     }
-    wave_add_tau(cv, cc, tau_acc);  // mean cost
+    wave_add_tau(cv, cc, red);  // mean cost
 }
 
 namespace {
@@ -94,7 +94,7 @@ void plan_free(hfm_plan *p) {
     dfree(p->tmax);
     dfree(p->nextmin);
     dfree(p->bounds);
-    dfree(p->tau_acc);
+    dfree(p->red);
     dfree(p->adj_lambda);
     dfree(p->adj_sens);
     dfree(p->adj_indeg);
@@ -375,24 +375,24 @@ template <typename T> void set_metric_impl(hfm_plan *p, const double *d_metric, 
     HIPCHECK(hipEventCreate(&e0));
     HIPCHECK(hipEventCreate(&e1));
     HIPCHECK(hipEventRecord(e0, s));
-    int init[3] = {0, 0x7fffffff, 0};
+    int init[2] = {0, 0x7fffffff};
     HIPCHECK(hipMemcpyAsync(p->status, init, sizeof(init), hipMemcpyHostToDevice, s));
-    HIPCHECK(hipMemsetAsync(p->tau_acc, 0, 2 * sizeof(double), s));
+    HIPCHECK(hipMemsetAsync(p->red, 0, RED_SLOTS * sizeof(RedSlot), s));
     HIPCHECK(launch_riemann_stencil<T>(p->g, d_metric, is_field, dual, ih, p->offs, static_cast<T *>(p->w), p->np,
-                                       p->status, p->tau_acc, s));
-    int hs[3];
-    double acc[2];
+                                       p->status, p->red, s));
+    int hs[2];
     HIPCHECK(hipMemcpyAsync(hs, p->status, sizeof(hs), hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipMemcpyAsync(acc, p->tau_acc, sizeof(acc), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(p->h_red.data(), p->red, RED_SLOTS * sizeof(RedSlot), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
-    p->tau = acc[1] > 0 ? acc[0] / acc[1] : 0.;
+    const RedTotals tot = red_total(p->h_red.data());
+    p->tau = tot.count > 0 ? tot.sum / tot.count : 0.;
     if (hs[0] == SELLING_NOT_SPD)
         throw HipFail{"ObtuseSuperbase error : input must be positive definite (grid point " + std::to_string(hs[1]) +
                       ")"};
     if (hs[0] == SELLING_NO_CONVERGENCE)
         throw HipFail{"ObtuseSuperbase error : did not terminate in 200 iterations (grid point " +
                       std::to_string(hs[1]) + ")"};
-    p->max_stencil_width = hs[2];
+    p->max_stencil_width = tot.l1max;
     build_deps<T>(p, s);
     HIPCHECK(hipEventRecord(e1, s));
     HIPCHECK(hipStreamSynchronize(s));
@@ -494,7 +494,8 @@ int hfm_plan_create(hfm_plan **out, const char *model, const int64_t *dims, cons
         p->tmax = dalloc<unsigned>(nt);
         p->nextmin = dalloc<unsigned>(2 * PEND_SLOTS);
         p->bounds = dalloc<float>(4);
-        p->tau_acc = dalloc<double>(2);
+        p->red = dalloc<RedSlot>(RED_SLOTS);
+        p->h_red.resize(RED_SLOTS);
         p->status = dalloc<int>(4);
         p->stats = dalloc<unsigned long long>(2 * PEND_SLOTS);
         HIPCHECK(hipHostMalloc((void **)&p->h_count, sizeof(int) * 4, hipHostMallocDefault));
@@ -567,15 +568,15 @@ int hfm_plan_set_cost(hfm_plan *p, const double *d_cost, int is_field, void *str
         HIPCHECK(hipMemcpyAsync(p->tab_offs, toff.data(), toff.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
         HIPCHECK(hipMemcpyAsync(p->tab_w, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice, s));
         const unsigned nb = (unsigned)((p->np + 255) / 256);
-        HIPCHECK(hipMemsetAsync(p->tau_acc, 0, 2 * sizeof(double), s));
+        HIPCHECK(hipMemsetAsync(p->red, 0, RED_SLOTS * sizeof(RedSlot), s));
         if (p->precision == 32) {
             hipLaunchKernelGGL((k_speed_from_cost<float>), dim3(nb), dim3(256), 0, s, p->g, d_cost, is_field,
-                               static_cast<float *>(p->speed), p->np, p->tau_acc);
+                               static_cast<float *>(p->speed), p->np, p->red);
             HIPCHECK(hipGetLastError());
             build_deps<float>(p, s);
         } else {
             hipLaunchKernelGGL((k_speed_from_cost<double>), dim3(nb), dim3(256), 0, s, p->g, d_cost, is_field,
-                               static_cast<double *>(p->speed), p->np, p->tau_acc);
+                               static_cast<double *>(p->speed), p->np, p->red);
             HIPCHECK(hipGetLastError());
             build_deps<double>(p, s);
         }
@@ -587,10 +588,10 @@ int hfm_plan_set_cost(hfm_plan *p, const double *d_cost, int is_field, void *str
             mw += sw;
         }
         mw /= p->g.n[2];
-        double acc[2];
-        HIPCHECK(hipMemcpyAsync(acc, p->tau_acc, sizeof(acc), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(p->h_red.data(), p->red, RED_SLOTS * sizeof(RedSlot), hipMemcpyDeviceToHost, s));
         HIPCHECK(hipStreamSynchronize(s));
-        p->tau = (acc[1] > 0 && mw > 0) ? (acc[0] / acc[1]) / std::sqrt(mw) : 0.;
+        const RedTotals tot = red_total(p->h_red.data());
+        p->tau = (tot.count > 0 && mw > 0) ? (tot.sum / tot.count) / std::sqrt(mw) : 0.;
         HIPCHECK(hipEventRecord(e1, s));
         HIPCHECK(hipStreamSynchronize(s));
         float ms = 0;
@@ -666,28 +667,42 @@ int hfm_plan_values(hfm_plan *p, double *d_values, void *stream) {
     return 0;
 }
 
-int hfm_plan_flow(hfm_plan *p, double *d_flow, double *d_active, void *stream) {
+static int plan_flow(const char *who, hfm_plan *p, double *d_flow, double *d_active, double *d_aoffs, void *stream) {
     if (!p) {
-        set_error("hfm_plan_flow: null argument");
+        set_error(std::string(who) + ": null argument");
         return 1;
     }
     if (!p->solved) {
-        set_error("hfm_plan_flow: no solve has run");
+        set_error(std::string(who) + ": no solve has run");
         return 1;
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
     FlowScale fs{{p->h[0], p->h[1], p->h[2]}};
     hipError_t e = p->precision == 32
                        ? launch_flow<float>(p->model, p->g, p->sd<float>(), static_cast<const float *>(p->u),
-                                            p->fixedmask, fs, d_flow, d_active, p->np, s)
+                                            p->fixedmask, fs, d_flow, d_active, d_aoffs, p->np, s)
                        : launch_flow<double>(p->model, p->g, p->sd<double>(), static_cast<const double *>(p->u),
-                                             p->fixedmask, fs, d_flow, d_active, p->np, s);
+                                             p->fixedmask, fs, d_flow, d_active, d_aoffs, p->np, s);
     if (e != hipSuccess) {
-        set_error(std::string("hfm_plan_flow: ") + hipGetErrorString(e));
+        set_error(std::string(who) + ": " + hipGetErrorString(e));
         return 1;
     }
     return 0;
 }
+
+int hfm_plan_flow(hfm_plan *p, double *d_flow, double *d_active, void *stream) {
+    return plan_flow("hfm_plan_flow", p, d_flow, d_active, nullptr, stream);
+}
+
+int hfm_plan_active_offsets(hfm_plan *p, double *d_offsets, void *stream) {
+    if (!d_offsets) {
+        set_error("hfm_plan_active_offsets: null argument");
+        return 1;
+    }
+    return plan_flow("hfm_plan_active_offsets", p, nullptr, nullptr, d_offsets, stream);
+}
+
+int hfm_plan_n_active_slots(const hfm_plan *p) { return p ? p->ndiff : 0; }
 
 int hfm_plan_get_stats(const hfm_plan *p, hfm_plan_stats *st) {
     if (!p || !st) {

@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void k_mschedule(const SolveDesc *__restrict__
 __global__ __launch_bounds__(256) void k_mfim(const SolveDesc *__restrict__ descs, int ndesc, int parity,
                                               const int *__restrict__ list, const int *__restrict__ list_count,
                                               int *__restrict__ list_count_reset) {
-    __shared__ float su[4][64];
+    __shared__ float su[4][SU_ROW];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int n = *list_count;
     if (blockIdx.x == 0) {
@@ -168,7 +168,6 @@ struct Slot {
     double *seed_v = nullptr;
     double *values_dev = nullptr;
     int *status = nullptr;
-    double *tau_acc = nullptr;
     float delta = 0;
     int64_t rounds = 0;
     hipEvent_t export_done = nullptr;
@@ -197,7 +196,6 @@ size_t arena_bytes(const RequestRec &r) {
     add(r.seed_v.size() * 8);
     add(r.N * 8);           // values_dev
     add(16);                // status
-    add(16);                // tau
     return b;
 }
 
@@ -219,6 +217,7 @@ struct hfm_server {
 
     static constexpr int NSETUP = 4;  // setup threads, one stream each
     hipStream_t s_setup[NSETUP] = {}, s_comp = nullptr;
+    RedSlot *d_red[NSETUP] = {};  // per setup thread: stencil reductions (mean one-step time)
     std::thread th_setup[NSETUP], th_comp;
 
     // compute-thread state
@@ -261,7 +260,7 @@ void release_budget(hfm_server *S, size_t bytes) {
 }
 
 // Build stencils and tile graph of one admitted request on the setup stream.
-Slot *setup_solve(hfm_server *S, const std::shared_ptr<RequestRec> &r, hipStream_t s) {
+Slot *setup_solve(hfm_server *S, const std::shared_ptr<RequestRec> &r, hipStream_t s, RedSlot *red) {
     Slot *sl = new Slot;
     sl->rec = r;
     const size_t ab = arena_bytes(*r);
@@ -280,7 +279,6 @@ Slot *setup_solve(hfm_server *S, const std::shared_ptr<RequestRec> &r, hipStream
     sl->seed_v = carve<double>(p, r->seed_v.size());
     sl->values_dev = carve<double>(p, r->N);
     sl->status = carve<int>(p, 4);
-    sl->tau_acc = carve<double>(p, 2);
     const double *metric = r->metric_dev;
     const size_t mcount = r->metric_is_field ? (size_t)r->N * r->K : (size_t)r->K;
     if (!metric) {
@@ -291,24 +289,24 @@ Slot *setup_solve(hfm_server *S, const std::shared_ptr<RequestRec> &r, hipStream
     }
     HIPCHECK(hipMemcpyAsync(sl->seed_p, r->seed_tp.data(), r->seed_tp.size() * 8, hipMemcpyHostToDevice, s));
     HIPCHECK(hipMemcpyAsync(sl->seed_v, r->seed_v.data(), r->seed_v.size() * 8, hipMemcpyHostToDevice, s));
-    int init[4] = {0, 0x7fffffff, 0, 0};
+    int init[2] = {0, 0x7fffffff};
     HIPCHECK(hipMemcpyAsync(sl->status, init, sizeof(init), hipMemcpyHostToDevice, s));
-    HIPCHECK(hipMemsetAsync(sl->tau_acc, 0, 16, s));
+    HIPCHECK(hipMemsetAsync(red, 0, RED_SLOTS * sizeof(RedSlot), s));
     InvHH ih{};
     const int D = r->g.dim;
     for (int i = 0; i < D; ++i)
         for (int j = 0; j <= i; ++j) ih.v[i * (i + 1) / 2 + j] = 1. / (r->h * r->h);
     HIPCHECK(launch_riemann_stencil<float>(r->g, metric, r->metric_is_field, r->dual, ih, sl->offs, sl->w, r->np,
-                                           sl->status, sl->tau_acc, s));
+                                           sl->status, red, s));
     StencilData<float> sd{sl->offs, sl->w, nullptr, nullptr, nullptr, r->np};
     HIPCHECK(hipMemsetAsync(sl->dep_cnt, 0, sizeof(int) * r->g.ntiles, s));
     HIPCHECK(launch_tile_deps<float>(r->model, r->g, sd, 0, sl->dep_cnt, nullptr, nullptr, nullptr, nullptr, s));
     HIPCHECK(launch_exclusive_scan(sl->dep_cnt, sl->dep_start, r->g.ntiles, s));
-    int hs[4];
-    double acc[2];
+    int hs[2];
+    std::vector<RedSlot> h_red(RED_SLOTS);
     int total = 0;
     HIPCHECK(hipMemcpyAsync(hs, sl->status, sizeof(hs), hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipMemcpyAsync(acc, sl->tau_acc, sizeof(acc), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(h_red.data(), red, RED_SLOTS * sizeof(RedSlot), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipMemcpyAsync(&total, sl->dep_start + r->g.ntiles, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
     if (hs[0] != SELLING_OK) {
@@ -328,7 +326,8 @@ Slot *setup_solve(hfm_server *S, const std::shared_ptr<RequestRec> &r, hipStream
         sl->metric_stage = nullptr;
     }
     HIPCHECK(hipStreamSynchronize(s));
-    const double tau = acc[1] > 0 ? acc[0] / acc[1] : 0.;
+    const RedTotals tot = red_total(h_red.data());
+    const double tau = tot.count > 0 ? tot.sum / tot.count : 0.;
     sl->delta = (float)(8. * tau);
     return sl;
 }
@@ -357,7 +356,7 @@ void setup_loop(hfm_server *S, int which) {
         r->res.queue_ms = ms_since(r->t_submit, r->t_admit);
         Slot *sl = nullptr;
         try {
-            sl = setup_solve(S, r, stream);
+            sl = setup_solve(S, r, stream, S->d_red[which]);
         } catch (const HipFail &e) {
             release_budget(S, r->bytes);
             fail_request(S, r, e.msg);
@@ -560,8 +559,10 @@ int hfm_server_create(hfm_server **out, double hbm_budget_bytes, int max_residen
         S->budget = hbm_budget_bytes;
         S->max_resident = std::min(max_resident, MAX_SLOTS);
         S->stats.hbm_budget = hbm_budget_bytes;
-        for (int i = 0; i < hfm_server::NSETUP; ++i)
+        for (int i = 0; i < hfm_server::NSETUP; ++i) {
             HIPCHECK(hipStreamCreateWithFlags(&S->s_setup[i], hipStreamNonBlocking));
+            S->d_red[i] = dalloc<RedSlot>(RED_SLOTS);
+        }
         HIPCHECK(hipStreamCreateWithFlags(&S->s_comp, hipStreamNonBlocking));
         S->d_desc = dalloc<SolveDesc>(MAX_SLOTS);
         HIPCHECK(hipHostMalloc((void **)&S->h_desc, sizeof(SolveDesc) * MAX_SLOTS, hipHostMallocDefault));
@@ -741,6 +742,7 @@ void hfm_server_destroy(hfm_server *S) {
     for (int i = 0; i < hfm_server::NSETUP; ++i)
         if (S->s_setup[i]) (void)hipStreamSynchronize(S->s_setup[i]);
     if (S->d_list) (void)hipFree(S->d_list);
+    for (int i = 0; i < hfm_server::NSETUP; ++i) dfree(S->d_red[i]);
     dfree(S->d_desc);
     dfree(S->d_lc);
     dfree(S->d_count);

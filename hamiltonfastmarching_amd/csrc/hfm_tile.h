@@ -87,6 +87,13 @@ __device__ inline bool band_take(unsigned k, float B) {
     return k != KEY_NONE && (key_value(k) < B || !(B < __builtin_huge_valf()));
 }
 
+// LDS row of one wave: the tile's 64 values, then a +inf sentinel read by out-of-tile sides.
+constexpr int SU_SENTINEL = 64;
+constexpr int SU_ROW = 65;
+
+// Minimum without NaN handling (no operand is NaN here): a compare and a select.
+template <typename T> __device__ inline T min_nn(T a, T b) { return b < a ? b : a; }
+
 template <typename T> struct TileCtx {
     Geom g;
     StencilData<T> sd;
@@ -98,7 +105,7 @@ template <typename T> struct TileCtx {
     unsigned *__restrict__ tmax;
 };
 
-// One visit of `tile` by one wavefront (lane = 0..63, su = the wave's 64-entry LDS row).
+// One visit of `tile` by one wavefront (lane = 0..63, su = the wave's SU_ROW-entry LDS row).
 //
 // The tile's values are staged in LDS, the values of neighbours outside the tile are cached in
 // registers, and Jacobi sweeps of the local solve run until no lane changes (wave ballot) or
@@ -129,34 +136,40 @@ __device__ inline int fim_tile(const TileCtx<T> &ctx, int tile_in, int lane, T *
     const T u0 = u[p];
     LaneStencil<ModelT, T> st;
     st.load(g, ctx.sd, p, c);
-    // Neighbour slots: forward k -> one side, symmetric k -> two sides.
-    int la[NC], lb[NC];
-    T oa[NC], ob[NC];
+    // Neighbour slots: forward k -> one side, symmetric k -> two sides.  A side inside the tile
+    // reads the wave's LDS row (ia/ib = its lane); a side outside reads the row's +inf sentinel
+    // (SU_SENTINEL) and contributes its value, frozen for the visit, through hk = the smaller of
+    // the outside values.  Each sweep then takes v_k = min(su[ia_k], su[ib_k], hk) without a
+    // branch, which is the smaller neighbour value of difference k exactly.
+    int ia[NC], ib[NC];
+    T hk[NC];
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
         int o0, o1, o2;
         unpack_offset(st.off[k], o0, o1, o2);
-        la[k] = -1;
-        lb[k] = -1;
-        oa[k] = inf;
-        ob[k] = inf;
-        if (fixed || st.w[k] == T(0)) continue;
-        int r[3];
-        if (shift_coords_m<ModelT>(g, c, o0, o1, o2, r)) {
-            const int t = tile_of(g, r), l = local_of(g, r);
-            if (t == tile) la[k] = l;
-            else oa[k] = u[(long long)t * 64 + l];
-        }
-        if (k >= NF) {
-            if (shift_coords_m<ModelT>(g, c, -o0, -o1, -o2, r)) {
+        ia[k] = SU_SENTINEL;
+        ib[k] = SU_SENTINEL;
+        T oa = inf, ob = inf;
+        if (!fixed && st.w[k] != T(0)) {
+            int r[3];
+            if (shift_coords_m<ModelT>(g, c, o0, o1, o2, r)) {
                 const int t = tile_of(g, r), l = local_of(g, r);
-                if (t == tile) lb[k] = l;
-                else ob[k] = u[(long long)t * 64 + l];
+                if (t == tile) ia[k] = l;
+                else oa = u[(long long)t * 64 + l];
+            }
+            if (k >= NF) {
+                if (shift_coords_m<ModelT>(g, c, -o0, -o1, -o2, r)) {
+                    const int t = tile_of(g, r), l = local_of(g, r);
+                    if (t == tile) ib[k] = l;
+                    else ob = u[(long long)t * 64 + l];
+                }
             }
         }
+        hk[k] = oa < ob ? oa : ob;
     }
     T ucur = u0;
     su[lane] = ucur;
+    if (lane == 0) su[SU_SENTINEL] = inf;
     bool converged = false;
     int sweeps = 0;
     for (int it = 0; it < MAXSWEEP; ++it) {
@@ -165,9 +178,9 @@ __device__ inline int fim_tile(const TileCtx<T> &ctx, int tile_in, int lane, T *
         T v[NC];
 #pragma unroll
         for (int k = 0; k < NC; ++k) {
-            const T va = la[k] >= 0 ? su[la[k]] : oa[k];
-            const T vb = lb[k] >= 0 ? su[lb[k]] : ob[k];
-            v[k] = va < vb ? va : vb;
+            const T va = su[ia[k]];
+            const T vab = (k >= NF) ? min_nn(va, su[ib[k]]) : va;
+            v[k] = min_nn(vab, hk[k]);
         }
         const T r = local_solve_update<NC>(v, st.w);
         const bool ch = !fixed && r < ucur;

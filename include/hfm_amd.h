@@ -128,6 +128,13 @@ int hfm_plan_values(hfm_plan *plan, double *d_values, void *stream);
  * bitset like the reference's export, device, may be NULL) (HFMInterface.hxx:536-553). */
 int hfm_plan_flow(hfm_plan *plan, double *d_flow, double *d_active, void *stream);
 
+/* activeOffsets (HFMInterface.hxx:554-577, exportActiveOffsets): N x nslots FP64 (device),
+ * nslots = hfm_plan_n_active_slots(plan).  Per point, the offsets of the active differences in
+ * stencil slot order (a symmetric difference with the sign of its active side), each packed
+ * as sum_q uint8(offset_q) << 8 (dim - 1 - q) and stored as a double; remaining entries 0. */
+int hfm_plan_active_offsets(hfm_plan *plan, double *d_offsets, void *stream);
+int hfm_plan_n_active_slots(const hfm_plan *plan);
+
 /* Backward differentiation (FirstVariation::BackwardVariation, FirstVariation.h:341-385):
  * gradient of sum_t weights[t] * u(targets[t]) w.r.t. the cost field (row-major FP64 device
  * array costSensitivity; zero outside the targets' dependency cone).  targets are host

@@ -9,11 +9,11 @@ sources exist (the development container):
     make -C oracle ref && python tests/golden/make_golden.py
 
 Fixtures (all row-major):
-  riemann2_64.npz    values, activeNeighs, geodesicFlow, per-point Selling offsets/weights
-  riemann3_24.npz    values, activeNeighs, geodesicFlow, per-point Selling offsets/weights
+  riemann2_64.npz    values, activeNeighs, geodesicFlow, activeOffsets, per-point Selling offsets/weights
+  riemann3_24.npz    values, activeNeighs, geodesicFlow, activeOffsets, per-point Selling offsets/weights
   ties3.npz          Selling offsets of near-tie / degenerate-direction 3x3 tensors
-  rsf2_32x32x16.npz  values, activeNeighs, geodesicFlow, shared stencils (getStencils)
-  elastica2_24x24x16.npz  values, costSensitivity_0, seedSensitivity_0, shared stencils
+  rsf2_32x32x16.npz  values, activeNeighs, geodesicFlow, activeOffsets, shared stencils (getStencils)
+  elastica2_24x24x16.npz  values, activeOffsets, costSensitivity_0, seedSensitivity_0, shared stencils
   riemann3_128.npz   digests of values at the benchmark size + 4096 seeded sample points
 Inputs are regenerated by hamiltonfastmarching_amd.synthetic; the SHA-256 of each generated
 input is stored so that a test can tell a generator change from a numerical one.
@@ -62,7 +62,8 @@ def cell_centres(dims, h):
 def riemann_case(name, params):
     dims = [int(n) for n in params["dims"]]
     q = dict(params)
-    q.update(exportActiveNeighs=1.0, exportGeodesicFlow=1.0, getStencils=cell_centres(dims, params["gridScale"]))
+    q.update(exportActiveNeighs=1.0, exportGeodesicFlow=1.0, exportActiveOffsets=1.0,
+             getStencils=cell_centres(dims, params["gridScale"]))
     r = oracle.run_reference(q)
     assert r["retcode"] == 0, r["log"]
     st = parse_stencils(r["stencils"])
@@ -77,7 +78,7 @@ def riemann_case(name, params):
             off[lin, k] = ok
             w[lin, k] = wk
     np.savez_compressed(os.path.join(HERE, name + ".npz"), values=r["values"], activeNeighs=r["activeNeighs"],
-                        geodesicFlow=r["geodesicFlow"], offsets=off, weights_printed=w,
+                        geodesicFlow=r["geodesicFlow"], activeOffsets=r["activeOffsets"], offsets=off, weights_printed=w,
                         metric_sha256=sha(params["metric"]), MaxStencilWidth=r["MaxStencilWidth"])
     print(name, "ok", r["values"].shape)
 
@@ -128,7 +129,7 @@ def curvature_case(name, params, sensitivity=False):
     q.pop("tips", None)
     dims = [int(n) for n in params["dims"]]
     nth = dims[2]
-    q.update(exportActiveNeighs=1.0, exportGeodesicFlow=1.0)
+    q.update(exportActiveNeighs=1.0, exportGeodesicFlow=1.0, exportActiveOffsets=1.0)
     # stencils of every theta at the domain centre
     q["getStencils"] = np.array([[0.5, 0.5, t * 2 * np.pi / nth] for t in range(nth)])
     r = oracle.run_reference(q)
@@ -147,7 +148,8 @@ def curvature_case(name, params, sensitivity=False):
         extra["costSensitivity_0"] = r["costSensitivity_0"]
         extra["seedSensitivity_0"] = r["seedSensitivity_0"]
     np.savez_compressed(os.path.join(HERE, name + ".npz"), values=r["values"], activeNeighs=r["activeNeighs"],
-                        geodesicFlow=r["geodesicFlow"], table_offsets=off, table_weights_printed=w,
+                        geodesicFlow=r["geodesicFlow"], activeOffsets=r["activeOffsets"], table_offsets=off,
+                        table_weights_printed=w,
                         cost_sha256=sha(params["cost"]), **extra)
     print(name, "ok")
 

@@ -123,6 +123,22 @@ def _gold(name):
     return np.load(os.path.join(GOLD, name + ".npz"))
 
 
+def _check_active_offsets(name, out, g, precision):
+    """activeOffsets: exactly the reference's row wherever the active set agrees with the
+    reference's activeNeighs; the active set agrees almost everywhere in FP64."""
+    a, ga = out["activeNeighs"], g["activeNeighs"]
+    ao, gao = out["activeOffsets"], g["activeOffsets"]
+    assert ao.shape == gao.shape
+    same = a == ga
+    rows_eq = np.all(ao == gao, axis=-1)
+    frac = float(np.mean(same))
+    print(f"{name} fp{precision}: activeNeighs equal {frac:.5f}, activeOffsets rows equal {np.mean(rows_eq):.5f}",
+          flush=True)
+    assert np.all(rows_eq[same])
+    if precision == 64:
+        assert frac >= 0.999
+
+
 @pytest.mark.parametrize("name,make", [("riemann2_64", lambda: syn.riemann2(64, seed=1)),
                                        ("riemann3_24", lambda: syn.riemann3(24, seed=2))])
 @pytest.mark.parametrize("precision", [64, 32])
@@ -130,8 +146,10 @@ def test_riemann_vs_golden(name, make, precision):
     print(f"golden {name} precision {precision}", flush=True)
     g = _gold(name)
     p = make()
-    q = dict(p, precision=float(precision), exportActiveNeighs=1.0, exportGeodesicFlow=1.0)
+    q = dict(p, precision=float(precision), exportActiveNeighs=1.0, exportGeodesicFlow=1.0,
+             exportActiveOffsets=1.0)
     out = hfm.run(q)
+    _check_active_offsets(name, out, g, precision)
     ref = g["values"]
     err = _floor_rel(out["values"], ref)
     print(f"{name} fp{precision}: values rel err {err:.3e}", flush=True)
@@ -170,11 +188,18 @@ def test_curvature_vs_golden(name, make, precision):
     g = _gold(name)
     p = make()
     p.pop("tips", None)
-    q = dict(p, precision=float(precision), exportActiveNeighs=1.0, exportGeodesicFlow=1.0)
+    q = dict(p, precision=float(precision), exportActiveNeighs=1.0, exportGeodesicFlow=1.0,
+             exportActiveOffsets=1.0)
     out = hfm.run(q)
+    _check_active_offsets(name, out, g, precision)
     err = _floor_rel(out["values"], g["values"])
     print(f"{name} fp{precision}: values rel err {err:.3e}", flush=True)
     assert err <= (1e-12 if precision == 64 else 1e-5)
+    if precision == 64:
+        fl, gf = out["geodesicFlow"], g["geodesicFlow"]
+        ferr = np.max(np.abs(fl - gf)) / np.max(np.abs(gf))
+        print(f"{name}: flow rel err {ferr:.3e}", flush=True)
+        assert ferr <= 1e-10
     if "costSensitivity_0" in g:
         s, gs = out["costSensitivity_0"], g["costSensitivity_0"]
         serr = np.max(np.abs(s - gs)) / np.max(np.abs(gs))

@@ -2,7 +2,8 @@
 
 Pins, in order of strength:
   * golden fixtures written by the reference's own FileHFM executables (tests/golden/): values,
-    activeNeighs, geodesicFlow, Selling offsets, shared curvature stencils, costSensitivity --
+    activeNeighs, geodesicFlow, activeOffsets, Selling offsets, shared curvature stencils,
+    costSensitivity --
     all required bit-exact;
   * closed forms (identity metric, constant metrics), invariants (Selling reconstruction,
     non-negative weights, unimodular superbase; the discrete eikonal residual), symmetry and a
@@ -16,6 +17,7 @@ import pytest
 
 import oracle
 from hamiltonfastmarching_amd import synthetic as syn
+from pack_utils import pack_active_offsets
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -79,6 +81,41 @@ def test_curvature_golden(name, model, make):
         assert np.array_equal(s, g["costSensitivity_0"])
         ss = g["seedSensitivity_0"]
         assert np.array_equal(sw, ss[:, -1])
+
+
+@pytest.mark.parametrize("name,make,model", [("riemann2_64", lambda: syn.riemann2(64, seed=1), "Riemann2"),
+                                             ("riemann3_24", lambda: syn.riemann3(24, seed=2), "Riemann3"),
+                                             ("rsf2_32x32x16", lambda: syn.rsf2(32, 16, seed=3),
+                                              "ReedsSheppForward2"),
+                                             ("elastica2_24x24x16", lambda: syn.elastica2(24, 16, seed=4),
+                                              "Elastica2")])
+def test_active_offsets_golden(name, make, model):
+    """The reference's activeOffsets export equals the packing of the oracle's active set over
+    the oracle's stencils (which are themselves pinned above)."""
+    g = load(name)
+    p = make()
+    p.pop("tips", None)
+    out = oracle.solve(p, active=True)
+    if model.startswith("Riemann"):
+        dims = p["metric"].shape[:-1]
+        offs, _ = oracle.riemann_stencils(dims, p["gridScale"], p["metric"])
+        nfwd, dim = 0, len(dims)
+    else:
+        dims = p["cost"].shape
+        tab, _, _ = oracle.curvature_stencils(model, dims[2], p["gridScale"], p["xi"], p["eps"])
+        offs = np.broadcast_to(tab[None, None], dims[:2] + tab.shape).reshape(-1, tab.shape[1], 3)
+        nfwd, dim = (3 if model == "ReedsSheppForward2" else 27), 3
+    mine = pack_active_offsets(out["activeNeighs"], offs, nfwd, dim)
+    ref = g["activeOffsets"].reshape(mine.shape)
+    assert np.array_equal(mine, ref)
+    # planted mistakes: little-endian packing, or dropping the side sign, must fail
+    rev = pack_active_offsets(out["activeNeighs"], offs[..., ::-1].copy(), nfwd, dim)
+    assert not np.array_equal(rev, ref)
+    if nfwd < offs.shape[1]:
+        act = out["activeNeighs"].astype(np.uint64).ravel()
+        b = np.uint64(nfwd + 1)
+        flipped = act ^ (((act >> b) & np.uint64(1)) << b) ^ (((act >> b) & np.uint64(1)) << (b - np.uint64(1)))
+        assert not np.array_equal(pack_active_offsets(flipped, offs, nfwd, dim), ref)
 
 
 def test_riemann3_128_digest():
