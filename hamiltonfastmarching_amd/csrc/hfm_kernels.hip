@@ -262,7 +262,7 @@ __global__ __launch_bounds__(64) void k_seed_activate(const long long *__restric
 // fixed point is.  The smallest pending key of round k+1 is accumulated in pend[(k+1)&1] by the
 // keys this kernel leaves pending and by the marks of the update kernel of round k.
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_schedule(int ntiles, unsigned *__restrict__ keys, SchedRound sr) {
+__global__ __launch_bounds__(SCHED_BLOCK) void k_schedule(int ntiles, unsigned *__restrict__ keys, SchedRound sr) {
     float delta = sr.delta;
     if (delta > 0.f) delta = adapt_delta(*sr.delta_in, *sr.prev_count, sr.target, 0.5f * sr.delta, 64.f * sr.delta);
     const float B = band_bound(key_value(read_pend(sr.pend_in)), *sr.bound_in, delta);
@@ -271,23 +271,13 @@ __global__ __launch_bounds__(256) void k_schedule(int ntiles, unsigned *__restri
         *sr.delta_out = delta;
     }
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63;
     unsigned k = KEY_NONE;
     if (t < ntiles) k = keys[t];
     const bool take = band_take(k, B);
     const unsigned left = (!take && k != KEY_NONE) ? k : KEY_NONE;
     if (take) keys[t] = KEY_NONE;
-    const unsigned lmin = wave_min_u32(left);
-    if (lane == 0 && lmin != KEY_NONE) atomicMin(&sr.pend_out[(t >> 6) & (PEND_SLOTS - 1)], lmin);
-    const unsigned long long mask = __ballot(take);
-    if (mask == 0) return;
-    int base = 0;
-    if (lane == 0) base = atomicAdd(sr.count, __popcll(mask));
-    base = __shfl(base, 0);
-    if (take) {
-        const unsigned long long below = lane == 0 ? 0ull : (mask & (~0ull >> (64 - lane)));
-        sr.list[base + __popcll(below)] = t;
-    }
+    const int pos = block_take(take, left, sr.count, nullptr, &sr.pend_out[blockIdx.x & (PEND_SLOTS - 1)]);
+    if (take) sr.list[pos] = t;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -527,7 +517,7 @@ hipError_t launch_seed_activate(const long long *seed_p, const double *seed_v, i
 }
 
 hipError_t launch_schedule(int ntiles, unsigned *keys, const SchedRound &sr, hipStream_t s) {
-    hipLaunchKernelGGL(k_schedule, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, ntiles, keys, sr);
+    hipLaunchKernelGGL(k_schedule, dim3(blocks_for(ntiles, SCHED_BLOCK)), dim3(SCHED_BLOCK), 0, s, ntiles, keys, sr);
     return hipGetLastError();
 }
 

@@ -34,6 +34,7 @@ namespace hfm {
 
 constexpr int SOLVE_SHIFT = 20;  // list entry = solve << SOLVE_SHIFT | tile
 constexpr int MAX_SLOTS = 512;   // descriptor table size (hard cap on resident solves)
+constexpr int ROUND_TILES = 4096;  // tiles per round shared by the resident solves (~ one per wave of a full update grid)
 
 struct SolveDesc {
     TileCtx<float> ctx;
@@ -47,43 +48,33 @@ struct SolveDesc {
     unsigned long long *stats;    // PEND_SLOTS pairs (visits, sweeps)
 };
 
-// Scheduling round over every resident solve: grid.y = descriptor index, grid.x = tile blocks.
-__global__ __launch_bounds__(256) void k_mschedule(const SolveDesc *__restrict__ descs, int parity,
-                                                   int *__restrict__ list, int *__restrict__ list_count) {
-    const int s = blockIdx.y;
+// Scheduling round over every running solve.  Block b covers tiles [bmap[b].y, bmap[b].y +
+// SCHED_BLOCK) of solve bmap[b].x: the grid has one block per SCHED_BLOCK tiles of each running solve.
+__global__ __launch_bounds__(SCHED_BLOCK) void k_mschedule(const SolveDesc *__restrict__ descs,
+                                                           const int2 *__restrict__ bmap, int parity,
+                                                           int *__restrict__ list, int *__restrict__ list_count) {
+    const int2 bm = bmap[blockIdx.x];
+    const int s = bm.x;
     const SolveDesc &d = descs[s];
-    if (!d.active) return;
     const int nt = d.ctx.g.ntiles;
-    if ((int)(blockIdx.x * blockDim.x) >= nt) return;
     float delta = d.delta;
     if (delta > 0.f)
-        delta = adapt_delta(d.bound[2 + (parity ^ 1)], d.count[parity ^ 1], d.target, 0.5f * d.delta, 64.f * d.delta);
+        delta = adapt_delta(d.bound[2 + (parity ^ 1)], d.count[parity ^ 1], d.target, d.delta * (1.f / 16.f),
+                            64.f * d.delta);
     const float B = band_bound(key_value(read_pend(d.pend + parity * PEND_SLOTS)), d.bound[parity ^ 1], delta);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (bm.y == 0 && threadIdx.x == 0) {
         d.bound[parity] = B;
         d.bound[2 + parity] = delta;
     }
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63;
+    const int t = bm.y + threadIdx.x;
     unsigned k = KEY_NONE;
     if (t < nt) k = d.ctx.keys[t];
     const bool take = band_take(k, B);
     const unsigned left = (!take && k != KEY_NONE) ? k : KEY_NONE;
     if (take) d.ctx.keys[t] = KEY_NONE;
-    const unsigned lmin = wave_min_u32(left);
-    if (lane == 0 && lmin != KEY_NONE) atomicMin(&d.pend[(parity ^ 1) * PEND_SLOTS + ((t >> 6) & (PEND_SLOTS - 1))], lmin);
-    const unsigned long long mask = __ballot(take);
-    if (mask == 0) return;
-    int base = 0;
-    if (lane == 0) {
-        base = atomicAdd(list_count, __popcll(mask));
-        atomicAdd(&d.count[parity], __popcll(mask));
-    }
-    base = __shfl(base, 0);
-    if (take) {
-        const unsigned long long below = lane == 0 ? 0ull : (mask & (~0ull >> (64 - lane)));
-        list[base + __popcll(below)] = (s << SOLVE_SHIFT) | t;
-    }
+    const int pos = block_take(take, left, list_count, &d.count[parity],
+                               &d.pend[(parity ^ 1) * PEND_SLOTS + (blockIdx.x & (PEND_SLOTS - 1))]);
+    if (take) list[pos] = (s << SOLVE_SHIFT) | t;
 }
 
 // Update round over the packed list of (solve, tile) entries of every resident solve.
@@ -121,6 +112,39 @@ __global__ __launch_bounds__(256) void k_mfim(const SolveDesc *__restrict__ desc
     }
 }
 
+// Setup-time clears of one solve in one launch: the stencil status words, the reduction slots,
+// the tile-graph counters and the seed key word.
+__global__ __launch_bounds__(256) void k_setup_clear(int *__restrict__ status, RedSlot *__restrict__ red,
+                                                     int *__restrict__ dep_cnt, int *__restrict__ dep_fill,
+                                                     unsigned *__restrict__ seed_key, int ntiles) {
+    const int i0 = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    if (i0 == 0) {
+        status[0] = 0;
+        status[1] = 0x7fffffff;
+        *seed_key = KEY_NONE;
+    }
+    int *rw = reinterpret_cast<int *>(red);
+    for (int i = i0; i < RED_SLOTS * (int)(sizeof(RedSlot) / sizeof(int)); i += stride) rw[i] = 0;
+    for (int i = i0; i < ntiles; i += stride) {
+        dep_cnt[i] = 0;
+        dep_fill[i] = 0;
+    }
+}
+
+// Scheduling state of a solve joining the running set at a round of the given parity: the seed
+// key (smallest key the seed marks left pending) becomes the pending minimum of that round.
+__global__ __launch_bounds__(64) void k_join_state(unsigned *__restrict__ pend, int *__restrict__ count,
+                                                   unsigned long long *__restrict__ stats, float *__restrict__ bound,
+                                                   const unsigned *__restrict__ seed_key, int parity, float delta) {
+    const int i = threadIdx.x;
+    for (int j = i; j < 2 * PEND_SLOTS; j += 64) {
+        pend[j] = (j == parity * PEND_SLOTS) ? *seed_key : KEY_NONE;
+        stats[j] = 0ull;
+    }
+    if (i < 2) count[i] = 0;
+    if (i < 4) bound[i] = i < 2 ? -__builtin_huge_valf() : delta;
+}
+
 }  // namespace hfm
 
 using namespace hfm;
@@ -145,6 +169,7 @@ struct RequestRec {
     int metric_is_field = 1, dual = 0;
     std::vector<long long> seed_tp;
     std::vector<double> seed_v;
+    std::vector<long long> seed_blob;  // seed_tp then the bits of seed_v: one host-to-device copy
     double *values_out = nullptr;
     size_t bytes = 0;
     Clock::time_point t_submit, t_admit, t_ready, t_done;
@@ -155,6 +180,7 @@ struct RequestRec {
 struct Slot {
     std::shared_ptr<RequestRec> rec;
     int idx = -1;  // descriptor index
+    long long first_batch = 0;  // first batch of rounds that includes the solve
     char *arena = nullptr;
     int *deps = nullptr;
     double *metric_stage = nullptr;
@@ -168,6 +194,7 @@ struct Slot {
     double *seed_v = nullptr;
     double *values_dev = nullptr;
     int *status = nullptr;
+    unsigned *seed_key = nullptr;  // smallest key the seed marks left pending
     float delta = 0;
     int64_t rounds = 0;
     hipEvent_t export_done = nullptr;
@@ -192,10 +219,11 @@ size_t arena_bytes(const RequestRec &r) {
     add(nt * 4);            // dep_cnt
     add((nt + 1) * 4);      // dep_start
     add(nt * 4);            // dep_fill
-    add(r.seed_tp.size() * 8);
-    add(r.seed_v.size() * 8);
+    add(r.seed_tp.size() * 16);  // seed points and values, one block
+
     add(r.N * 8);           // values_dev
     add(16);                // status
+    add(4);                 // seed key
     return b;
 }
 
@@ -215,14 +243,27 @@ struct hfm_server {
     bool stop = false;
     hfm_server_stats stats{};
 
-    static constexpr int NSETUP = 4;  // setup threads, one stream each
+    // Setup threads, one stream each.  With the compute stream and the caller's stream this
+    // stays within the 4 hardware queues HIP creates by default, so no setup stream shares a
+    // queue with the compute stream.
+    static constexpr int NSETUP = 2;
+    static constexpr int SETUP_GROUP = 8;  // requests set up per pass (two synchronisations)
     hipStream_t s_setup[NSETUP] = {}, s_comp = nullptr;
-    RedSlot *d_red[NSETUP] = {};  // per setup thread: stencil reductions (mean one-step time)
+    RedSlot *d_red[NSETUP] = {};           // per thread: SETUP_GROUP x RED_SLOTS stencil reductions
+    RedSlot *h_red[NSETUP] = {};           // pinned copies
+    int *h_status[NSETUP] = {};            // pinned SETUP_GROUP x 2
+    int *h_total[NSETUP] = {};             // pinned SETUP_GROUP dependency totals
     std::thread th_setup[NSETUP], th_comp;
 
-    // compute-thread state
+    // compute-thread state.  One batch of rounds stays in flight while the host reads the
+    // counters of the previous one, so the host mirrors the device reads from or writes to
+    // are double-buffered by batch parity.
     SolveDesc *d_desc = nullptr;
-    SolveDesc *h_desc = nullptr;  // pinned
+    SolveDesc *h_desc = nullptr;         // host master copy of the descriptor table
+    SolveDesc *h_desc_stage[2] = {};     // pinned, source of the per-batch upload
+    int2 *d_bmap = nullptr;              // k_mschedule block table
+    int2 *h_bmap_stage[2] = {};          // pinned
+    long long bmap_cap = 0;
     int *d_list = nullptr;
     long long list_cap = 0;
     int *d_lc = nullptr;          // 2 alternating list counters
@@ -230,13 +271,15 @@ struct hfm_server {
     unsigned *d_pend = nullptr;   // MAX_SLOTS * 2 * PEND_SLOTS
     float *d_bound = nullptr;     // MAX_SLOTS * 2
     unsigned long long *d_stats = nullptr;  // MAX_SLOTS * 2 * PEND_SLOTS
-    int *h_count = nullptr;       // pinned MAX_SLOTS * 2
-    unsigned long long *h_stats = nullptr;  // pinned
+    int *h_count[2] = {};                   // pinned MAX_SLOTS * 2, per batch parity
+    unsigned long long *h_stats[2] = {};    // pinned, per batch parity
+    hipEvent_t batch_done[2] = {};
     std::vector<Slot *> running;
     std::vector<Slot *> exporting;
     std::vector<int> free_idx;
     int ndesc = 0;
     long long round = 0;
+    long long nlaunched = 0;  // batches launched
 };
 
 namespace {
@@ -259,114 +302,154 @@ void release_budget(hfm_server *S, size_t bytes) {
     S->cv_submit.notify_all();
 }
 
-// Build stencils and tile graph of one admitted request on the setup stream.
-Slot *setup_solve(hfm_server *S, const std::shared_ptr<RequestRec> &r, hipStream_t s, RedSlot *red) {
-    Slot *sl = new Slot;
-    sl->rec = r;
-    const size_t ab = arena_bytes(*r);
-    HIPCHECK(hipMallocAsync((void **)&sl->arena, ab, s));
-    char *p = sl->arena;
-    sl->u = carve<float>(p, r->np);
-    sl->offs = carve<uint32_t>(p, r->np * r->K);
-    sl->w = carve<float>(p, r->np * r->K);
-    sl->fixedmask = carve<unsigned long long>(p, r->g.ntiles);
-    sl->keys = carve<unsigned>(p, r->g.ntiles);
-    sl->tmax = carve<unsigned>(p, r->g.ntiles);
-    sl->dep_cnt = carve<int>(p, r->g.ntiles);
-    sl->dep_start = carve<int>(p, r->g.ntiles + 1);
-    sl->dep_fill = carve<int>(p, r->g.ntiles);
-    sl->seed_p = carve<long long>(p, r->seed_tp.size());
-    sl->seed_v = carve<double>(p, r->seed_v.size());
-    sl->values_dev = carve<double>(p, r->N);
-    sl->status = carve<int>(p, 4);
-    const double *metric = r->metric_dev;
-    const size_t mcount = r->metric_is_field ? (size_t)r->N * r->K : (size_t)r->K;
-    if (!metric) {
-        HIPCHECK(hipMallocAsync((void **)&sl->metric_stage, mcount * sizeof(double), s));
-        HIPCHECK(hipMemcpyAsync(sl->metric_stage, r->metric_host.data(), mcount * sizeof(double),
-                                hipMemcpyHostToDevice, s));
-        metric = sl->metric_stage;
-    }
-    HIPCHECK(hipMemcpyAsync(sl->seed_p, r->seed_tp.data(), r->seed_tp.size() * 8, hipMemcpyHostToDevice, s));
-    HIPCHECK(hipMemcpyAsync(sl->seed_v, r->seed_v.data(), r->seed_v.size() * 8, hipMemcpyHostToDevice, s));
-    int init[2] = {0, 0x7fffffff};
-    HIPCHECK(hipMemcpyAsync(sl->status, init, sizeof(init), hipMemcpyHostToDevice, s));
-    HIPCHECK(hipMemsetAsync(red, 0, RED_SLOTS * sizeof(RedSlot), s));
-    InvHH ih{};
-    const int D = r->g.dim;
-    for (int i = 0; i < D; ++i)
-        for (int j = 0; j <= i; ++j) ih.v[i * (i + 1) / 2 + j] = 1. / (r->h * r->h);
-    HIPCHECK(launch_riemann_stencil<float>(r->g, metric, r->metric_is_field, r->dual, ih, sl->offs, sl->w, r->np,
-                                           sl->status, red, s));
-    StencilData<float> sd{sl->offs, sl->w, nullptr, nullptr, nullptr, r->np};
-    HIPCHECK(hipMemsetAsync(sl->dep_cnt, 0, sizeof(int) * r->g.ntiles, s));
-    HIPCHECK(launch_tile_deps<float>(r->model, r->g, sd, 0, sl->dep_cnt, nullptr, nullptr, nullptr, nullptr, s));
-    HIPCHECK(launch_exclusive_scan(sl->dep_cnt, sl->dep_start, r->g.ntiles, s));
-    int hs[2];
-    std::vector<RedSlot> h_red(RED_SLOTS);
-    int total = 0;
-    HIPCHECK(hipMemcpyAsync(hs, sl->status, sizeof(hs), hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipMemcpyAsync(h_red.data(), red, RED_SLOTS * sizeof(RedSlot), hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipMemcpyAsync(&total, sl->dep_start + r->g.ntiles, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipStreamSynchronize(s));
-    if (hs[0] != SELLING_OK) {
-        if (sl->metric_stage) HIPCHECK(hipFreeAsync(sl->metric_stage, s));
-        HIPCHECK(hipFreeAsync(sl->arena, s));
-        HIPCHECK(hipStreamSynchronize(s));
-        delete sl;
-        throw HipFail{hs[0] == SELLING_NOT_SPD ? "ObtuseSuperbase error : input must be positive definite"
-                                              : "ObtuseSuperbase error : did not terminate in 200 iterations"};
-    }
-    HIPCHECK(hipMallocAsync((void **)&sl->deps, sizeof(int) * std::max(total, 1), s));
-    HIPCHECK(hipMemsetAsync(sl->dep_fill, 0, sizeof(int) * r->g.ntiles, s));
-    HIPCHECK(launch_tile_deps<float>(r->model, r->g, sd, 1, sl->dep_cnt, sl->dep_start, sl->dep_fill, sl->deps,
-                                     nullptr, s));
-    if (sl->metric_stage) {
-        HIPCHECK(hipFreeAsync(sl->metric_stage, s));
-        sl->metric_stage = nullptr;
+// Setup of a group of admitted requests on one setup stream: stencils, tile graph, initial
+// values and seed marks.  Phase 1 launches every request's stencil, dependency count and scan
+// and copies back its status, reductions and dependency total; one synchronisation; phase 2
+// allocates and fills the dependency lists, initialises values and marks the seeds; one
+// synchronisation.  Failed requests are reported in `err` (empty: ready).
+struct SetupItem {
+    std::shared_ptr<RequestRec> r;
+    Slot *sl = nullptr;
+    std::string err;
+};
+
+void setup_group(hfm_server *S, std::vector<SetupItem> &items, int which) {
+    hipStream_t s = S->s_setup[which];
+    const int n = (int)items.size();
+    for (int g = 0; g < n; ++g) {
+        const std::shared_ptr<RequestRec> &r = items[g].r;
+        Slot *sl = new Slot;
+        items[g].sl = sl;
+        sl->rec = r;
+        HIPCHECK(hipMallocAsync((void **)&sl->arena, arena_bytes(*r), s));
+        char *p = sl->arena;
+        sl->u = carve<float>(p, r->np);
+        sl->offs = carve<uint32_t>(p, r->np * r->K);
+        sl->w = carve<float>(p, r->np * r->K);
+        sl->fixedmask = carve<unsigned long long>(p, r->g.ntiles);
+        sl->keys = carve<unsigned>(p, r->g.ntiles);
+        sl->tmax = carve<unsigned>(p, r->g.ntiles);
+        sl->dep_cnt = carve<int>(p, r->g.ntiles);
+        sl->dep_start = carve<int>(p, r->g.ntiles + 1);
+        sl->dep_fill = carve<int>(p, r->g.ntiles);
+        const size_t ns = r->seed_tp.size();
+        sl->seed_p = carve<long long>(p, 2 * ns);
+        sl->seed_v = reinterpret_cast<double *>(sl->seed_p + ns);
+        sl->values_dev = carve<double>(p, r->N);
+        sl->status = carve<int>(p, 4);
+        sl->seed_key = carve<unsigned>(p, 1);
+        const double *metric = r->metric_dev;
+        if (!metric) {
+            const size_t mcount = r->metric_is_field ? (size_t)r->N * r->K : (size_t)r->K;
+            HIPCHECK(hipMallocAsync((void **)&sl->metric_stage, mcount * sizeof(double), s));
+            HIPCHECK(hipMemcpyAsync(sl->metric_stage, r->metric_host.data(), mcount * sizeof(double),
+                                    hipMemcpyHostToDevice, s));
+            metric = sl->metric_stage;
+        }
+        HIPCHECK(hipMemcpyAsync(sl->seed_p, r->seed_blob.data(), r->seed_blob.size() * 8, hipMemcpyHostToDevice, s));
+        RedSlot *red = S->d_red[which] + (size_t)g * RED_SLOTS;
+        hipLaunchKernelGGL(k_setup_clear, dim3((unsigned)std::min<long long>((r->g.ntiles + 255) / 256 + 16, 1024)),
+                           dim3(256), 0, s, sl->status, red, sl->dep_cnt, sl->dep_fill, sl->seed_key, r->g.ntiles);
+        HIPCHECK(hipGetLastError());
+        InvHH ih{};
+        const int D = r->g.dim;
+        for (int i = 0; i < D; ++i)
+            for (int j = 0; j <= i; ++j) ih.v[i * (i + 1) / 2 + j] = 1. / (r->h * r->h);
+        HIPCHECK(launch_riemann_stencil<float>(r->g, metric, r->metric_is_field, r->dual, ih, sl->offs, sl->w, r->np,
+                                               sl->status, red, s));
+        StencilData<float> sd{sl->offs, sl->w, nullptr, nullptr, nullptr, r->np};
+        HIPCHECK(launch_tile_deps<float>(r->model, r->g, sd, 0, sl->dep_cnt, nullptr, nullptr, nullptr, nullptr, s));
+        HIPCHECK(launch_exclusive_scan(sl->dep_cnt, sl->dep_start, r->g.ntiles, s));
+        HIPCHECK(hipMemcpyAsync(S->h_status[which] + 2 * g, sl->status, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(S->h_red[which] + (size_t)g * RED_SLOTS, red, RED_SLOTS * sizeof(RedSlot),
+                                hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(S->h_total[which] + g, sl->dep_start + r->g.ntiles, sizeof(int),
+                                hipMemcpyDeviceToHost, s));
     }
     HIPCHECK(hipStreamSynchronize(s));
-    const RedTotals tot = red_total(h_red.data());
-    const double tau = tot.count > 0 ? tot.sum / tot.count : 0.;
-    sl->delta = (float)(8. * tau);
-    return sl;
+    for (int g = 0; g < n; ++g) {
+        const std::shared_ptr<RequestRec> &r = items[g].r;
+        Slot *sl = items[g].sl;
+        const int st = S->h_status[which][2 * g];
+        if (st != SELLING_OK) {
+            items[g].err = st == SELLING_NOT_SPD ? "ObtuseSuperbase error : input must be positive definite"
+                                                 : "ObtuseSuperbase error : did not terminate in 200 iterations";
+            if (sl->metric_stage) HIPCHECK(hipFreeAsync(sl->metric_stage, s));
+            HIPCHECK(hipFreeAsync(sl->arena, s));
+            delete sl;
+            items[g].sl = nullptr;
+            continue;
+        }
+        const int total = S->h_total[which][g];
+        HIPCHECK(hipMallocAsync((void **)&sl->deps, sizeof(int) * std::max(total, 1), s));
+        StencilData<float> sd{sl->offs, sl->w, nullptr, nullptr, nullptr, r->np};
+        HIPCHECK(launch_tile_deps<float>(r->model, r->g, sd, 1, sl->dep_cnt, sl->dep_start, sl->dep_fill, sl->deps,
+                                         nullptr, s));
+        const int nseeds = (int)r->seed_tp.size();
+        HIPCHECK(launch_init<float>(r->g, sl->u, sl->fixedmask, sl->keys, sl->tmax, r->np, sl->seed_p, sl->seed_v,
+                                    nseeds, s));
+        HIPCHECK(launch_seed_activate(sl->seed_p, sl->seed_v, nseeds, sl->dep_start, sl->deps, sl->keys, sl->tmax,
+                                      sl->seed_key, s));
+        if (sl->metric_stage) {
+            HIPCHECK(hipFreeAsync(sl->metric_stage, s));
+            sl->metric_stage = nullptr;
+        }
+        const RedTotals tot = red_total(S->h_red[which] + (size_t)g * RED_SLOTS);
+        const double tau = tot.count > 0 ? tot.sum / tot.count : 0.;
+        sl->delta = (float)(8. * tau);
+    }
+    HIPCHECK(hipStreamSynchronize(s));
 }
 
 void setup_loop(hfm_server *S, int which) {
     if (hipSetDevice(0) != hipSuccess) return;
-    hipStream_t stream = S->s_setup[which];
     while (true) {
-        std::shared_ptr<RequestRec> r;
+        std::vector<SetupItem> items;
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            S->cv_submit.wait(lk, [&] {
-                if (S->stop) return true;
+            auto admissible = [&] {
                 if (S->queue.empty()) return false;
                 const auto &h = S->queue.front();
                 return S->resident < S->max_resident && S->in_use + (double)h->bytes <= S->budget;
-            });
+            };
+            S->cv_submit.wait(lk, [&] { return S->stop || admissible(); });
             if (S->stop) return;
-            r = S->queue.front();
-            S->queue.pop_front();
-            S->in_use += (double)r->bytes;
-            S->resident++;
-            S->stats.max_resident_seen = std::max<int64_t>(S->stats.max_resident_seen, S->resident);
+            while ((int)items.size() < hfm_server::SETUP_GROUP && admissible()) {
+                SetupItem it;
+                it.r = S->queue.front();
+                S->queue.pop_front();
+                S->in_use += (double)it.r->bytes;
+                S->resident++;
+                S->stats.max_resident_seen = std::max<int64_t>(S->stats.max_resident_seen, S->resident);
+                items.push_back(std::move(it));
+            }
         }
-        r->t_admit = Clock::now();
-        r->res.queue_ms = ms_since(r->t_submit, r->t_admit);
-        Slot *sl = nullptr;
+        const auto t_admit = Clock::now();
+        for (auto &it : items) {
+            it.r->t_admit = t_admit;
+            it.r->res.queue_ms = ms_since(it.r->t_submit, t_admit);
+        }
+        std::string group_err;
         try {
-            sl = setup_solve(S, r, stream, S->d_red[which]);
+            setup_group(S, items, which);
         } catch (const HipFail &e) {
-            release_budget(S, r->bytes);
-            fail_request(S, r, e.msg);
-            continue;
+            group_err = e.msg;
         }
-        r->t_ready = Clock::now();
-        r->res.setup_ms = ms_since(r->t_admit, r->t_ready);
+        const auto t_ready = Clock::now();
+        std::vector<Slot *> ok;
+        for (auto &it : items) {
+            if (!group_err.empty() || !it.err.empty() || !it.sl) {
+                release_budget(S, it.r->bytes);
+                fail_request(S, it.r, group_err.empty() ? it.err : group_err);
+                continue;
+            }
+            it.r->t_ready = t_ready;
+            it.r->res.setup_ms = ms_since(it.r->t_admit, t_ready);
+            ok.push_back(it.sl);
+        }
         std::lock_guard<std::mutex> lk(S->mu);
-        S->ready.push_back(sl);
-        S->cv_ready.notify_all();
+        for (Slot *sl : ok) S->ready.push_back(sl);
+        if (!ok.empty()) S->cv_ready.notify_all();
     }
 }
 
@@ -381,27 +464,21 @@ void join_solve(hfm_server *S, Slot *sl) {
         idx = S->ndesc++;
     }
     sl->idx = idx;
+    sl->first_batch = S->nlaunched;
     const int parity = (int)(S->round & 1);
     unsigned *pend = S->d_pend + (size_t)idx * 2 * PEND_SLOTS;
     float *bound = S->d_bound + (size_t)idx * 4;
     int *count = S->d_count + (size_t)idx * 2;
     unsigned long long *st = S->d_stats + (size_t)idx * 2 * PEND_SLOTS;
-    HIPCHECK(launch_init<float>(r.g, sl->u, sl->fixedmask, sl->keys, sl->tmax, r.np, sl->seed_p, sl->seed_v,
-                                (int)r.seed_tp.size(), s));
-    HIPCHECK(hipMemsetAsync(pend, 0xff, 2 * PEND_SLOTS * sizeof(unsigned), s));
-    HIPCHECK(hipMemsetAsync(count, 0, 2 * sizeof(int), s));
-    HIPCHECK(hipMemsetAsync(st, 0, 2 * PEND_SLOTS * sizeof(unsigned long long), s));
-    const float binit[4] = {-__builtin_huge_valf(), -__builtin_huge_valf(), sl->delta, sl->delta};
-    HIPCHECK(hipMemcpyAsync(bound, binit, sizeof(binit), hipMemcpyHostToDevice, s));
-    HIPCHECK(launch_seed_activate(sl->seed_p, sl->seed_v, (int)r.seed_tp.size(), sl->dep_start, sl->deps, sl->keys,
-                                  sl->tmax, pend + parity * PEND_SLOTS, s));
+    hipLaunchKernelGGL(k_join_state, dim3(1), dim3(64), 0, s, pend, count, st, bound, sl->seed_key, parity, sl->delta);
+    HIPCHECK(hipGetLastError());
     SolveDesc &d = S->h_desc[idx];
     d.ctx = TileCtx<float>{r.g, StencilData<float>{sl->offs, sl->w, nullptr, nullptr, nullptr, r.np}, sl->u,
                            sl->fixedmask, sl->dep_start, sl->deps, sl->keys, sl->tmax};
     d.model = r.model;
     d.active = 1;
     d.delta = sl->delta;
-    d.target = std::max(64, std::min(256, r.g.ntiles / 4));
+    d.target = 256;  // set per batch from the number of running solves
     d.pend = pend;
     d.bound = bound;
     d.count = count;
@@ -409,30 +486,155 @@ void join_solve(hfm_server *S, Slot *sl) {
     S->running.push_back(sl);
 }
 
-void finish_export(hfm_server *S, Slot *sl) {
+void finish_export(Slot *sl, const unsigned long long *h_stats) {
     RequestRec &r = *sl->rec;
-    unsigned long long v = 0, sw = 0;
-    const unsigned long long *hs = S->h_stats + (size_t)sl->idx * 2 * PEND_SLOTS;
-    for (int i = 0; i < PEND_SLOTS; ++i) {
-        v += hs[2 * i];
-        sw += hs[2 * i + 1];
-    }
-    (void)sw;
+    unsigned long long v = 0;
+    const unsigned long long *hs = h_stats + (size_t)sl->idx * 2 * PEND_SLOTS;
+    for (int i = 0; i < PEND_SLOTS; ++i) v += hs[2 * i];
     r.res.tile_visits = (int64_t)v;
     r.res.rounds = sl->rounds;
 }
 
+// Launch one batch of `batch` rounds over the running solves, then the copies of their
+// counters into the host mirrors of parity q = batch index & 1, and record batch_done[q].
+void launch_batch(hfm_server *S, long long q, int batch) {
+    hipStream_t s = S->s_comp;
+    const int par = (int)(q & 1);
+    long long tiles = 0;
+    std::vector<int2> bm;
+    for (Slot *sl : S->running) {
+        const int nt = sl->rec->g.ntiles;
+        tiles += nt;
+        for (int b = 0; b < nt; b += SCHED_BLOCK) bm.push_back(int2{sl->idx, b});
+    }
+    if (tiles > S->list_cap) {
+        if (S->d_list) HIPCHECK(hipFreeAsync(S->d_list, s));
+        S->list_cap = std::max<long long>(tiles, 2 * S->list_cap);
+        HIPCHECK(hipMallocAsync((void **)&S->d_list, S->list_cap * sizeof(int), s));
+    }
+    if ((long long)bm.size() > S->bmap_cap) {
+        // The staging buffer of the other parity may still feed an in-flight copy: wait for it.
+        HIPCHECK(hipStreamSynchronize(s));
+        if (S->d_bmap) HIPCHECK(hipFree(S->d_bmap));
+        for (int i = 0; i < 2; ++i)
+            if (S->h_bmap_stage[i]) HIPCHECK(hipHostFree(S->h_bmap_stage[i]));
+        S->bmap_cap = std::max<long long>((long long)bm.size(), 2 * S->bmap_cap);
+        HIPCHECK(hipMalloc((void **)&S->d_bmap, S->bmap_cap * sizeof(int2)));
+        for (int i = 0; i < 2; ++i)
+            HIPCHECK(hipHostMalloc((void **)&S->h_bmap_stage[i], S->bmap_cap * sizeof(int2), hipHostMallocDefault));
+    }
+    // Tiles per round each solve's band is steered to: the resident solves share ROUND_TILES;
+    // a busy server runs narrow bands (fewer redundant tile visits, the extra rounds are shared
+    // by all solves), a lone solve runs wide ones (fewer rounds).
+    const int tgt = std::max(16, std::min(256, ROUND_TILES / (int)S->running.size()));
+    for (int i = 0; i < S->ndesc; ++i) S->h_desc[i].target = tgt;
+    std::memcpy(S->h_desc_stage[par], S->h_desc, sizeof(SolveDesc) * S->ndesc);
+    std::memcpy(S->h_bmap_stage[par], bm.data(), bm.size() * sizeof(int2));
+    HIPCHECK(hipMemcpyAsync(S->d_desc, S->h_desc_stage[par], sizeof(SolveDesc) * S->ndesc, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(S->d_bmap, S->h_bmap_stage[par], bm.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+    const unsigned grid = (unsigned)std::min<long long>((tiles + 3) / 4, 2048);
+    for (int b = 0; b < batch; ++b) {
+        const int parity = (int)(S->round & 1);
+        hipLaunchKernelGGL(k_mschedule, dim3((unsigned)bm.size()), dim3(SCHED_BLOCK), 0, s, S->d_desc, S->d_bmap,
+                           parity, S->d_list, &S->d_lc[parity]);
+        HIPCHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_mfim, dim3(grid), dim3(256), 0, s, S->d_desc, S->ndesc, parity, S->d_list,
+                           &S->d_lc[parity], &S->d_lc[parity ^ 1]);
+        HIPCHECK(hipGetLastError());
+        S->round++;
+    }
+    for (Slot *sl : S->running) sl->rounds += batch;
+    HIPCHECK(hipMemcpyAsync(S->h_count[par], S->d_count, sizeof(int) * 2 * S->ndesc, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(S->h_stats[par], S->d_stats, sizeof(unsigned long long) * 2 * PEND_SLOTS * S->ndesc,
+                            hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipEventRecord(S->batch_done[par], s));
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stats.batches++;
+    S->stats.rounds += batch;
+}
+
+// Read the counters of batch q (after its event): solves whose last round listed no tile have
+// converged; export them.  Only solves still running and included in batch q are looked at.
+// Their descriptors are switched off for the batches launched from now on; a batch already in
+// flight may still list them and finds nothing pending.
+void process_batch(hfm_server *S, long long q, int last_parity) {
+    const int par = (int)(q & 1);
+    HIPCHECK(hipEventSynchronize(S->batch_done[par]));
+    std::vector<Slot *> done, still;
+    for (Slot *sl : S->running) {
+        const bool member = sl->first_batch <= q;
+        const bool finished = member && S->h_count[par][2 * sl->idx + last_parity] == 0;
+        const bool runaway = member && sl->rounds > 8LL * sl->rec->g.ntiles + 65536;
+        if (!finished && !runaway) {
+            still.push_back(sl);
+            continue;
+        }
+        if (!finished) {
+            sl->rec->res.status = 2;
+            std::snprintf(sl->rec->res.error, sizeof(sl->rec->res.error),
+                          "block fast-iterative method did not converge");
+        }
+        done.push_back(sl);
+    }
+    if (done.empty()) return;
+    S->running.swap(still);
+    for (Slot *sl : done) {
+        S->h_desc[sl->idx].active = 0;
+        finish_export(sl, S->h_stats[par]);
+        HIPCHECK(launch_export_values<float>(sl->rec->g, sl->u, sl->values_dev, sl->rec->np, S->s_comp));
+        if (sl->rec->values_out)
+            HIPCHECK(hipMemcpyAsync(sl->rec->values_out, sl->values_dev, sl->rec->N * sizeof(double),
+                                    hipMemcpyDeviceToHost, S->s_comp));
+        // Recorded after every batch launched so far: once it completes, no kernel reads the
+        // solve's memory any more.
+        HIPCHECK(hipEventCreateWithFlags(&sl->export_done, hipEventDisableTiming));
+        HIPCHECK(hipEventRecord(sl->export_done, S->s_comp));
+        S->exporting.push_back(sl);
+    }
+}
+
+void retire_exports(hfm_server *S) {
+    std::vector<Slot *> pending;
+    for (Slot *sl : S->exporting) {
+        if (hipEventQuery(sl->export_done) != hipSuccess) {
+            pending.push_back(sl);
+            continue;
+        }
+        (void)hipEventDestroy(sl->export_done);
+        HIPCHECK(hipFreeAsync(sl->deps, S->s_comp));
+        HIPCHECK(hipFreeAsync(sl->arena, S->s_comp));
+        S->free_idx.push_back(sl->idx);
+        auto r = sl->rec;
+        const size_t bytes = r->bytes;
+        delete sl;
+        release_budget(S, bytes);
+        std::lock_guard<std::mutex> lk(S->mu);
+        r->t_done = Clock::now();
+        r->res.latency_ms = ms_since(r->t_submit, r->t_done);
+        r->done = true;
+        if (r->res.status == 0) S->stats.completed++;
+        else S->stats.failed++;
+        S->cv_done.notify_all();
+    }
+    S->exporting.swap(pending);
+}
+
+// The compute thread: joins set-up solves, keeps one batch of rounds in flight over the running
+// solves while it reads the previous batch's counters, exports converged solves, frees them.
 void compute_loop(hfm_server *S) {
     if (hipSetDevice(0) != hipSuccess) return;
     const int batch = 8;
+    long long processed = 0;
+    int last_parity[2] = {0, 0};
     while (true) {
-        // joins
         std::vector<Slot *> joining;
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            if (S->running.empty() && S->exporting.empty())
+            if (S->running.empty() && S->exporting.empty() && processed == S->nlaunched)
                 S->cv_ready.wait(lk, [&] { return S->stop || !S->ready.empty(); });
-            if (S->stop && S->running.empty() && S->exporting.empty() && S->ready.empty()) return;
+            if (S->stop && S->running.empty() && S->exporting.empty() && S->ready.empty() &&
+                processed == S->nlaunched)
+                return;
             while (!S->ready.empty()) {
                 joining.push_back(S->ready.front());
                 S->ready.pop_front();
@@ -441,93 +643,17 @@ void compute_loop(hfm_server *S) {
         try {
             for (Slot *sl : joining) join_solve(S, sl);
             if (!S->running.empty()) {
-                // list capacity: total tiles of the resident solves
-                long long tiles = 0;
-                int maxblocks = 0;
-                for (Slot *sl : S->running) {
-                    tiles += sl->rec->g.ntiles;
-                    maxblocks = std::max(maxblocks, (sl->rec->g.ntiles + 255) / 256);
-                }
-                if (tiles > S->list_cap) {
-                    if (S->d_list) HIPCHECK(hipFreeAsync(S->d_list, S->s_comp));
-                    S->list_cap = std::max<long long>(tiles, 2 * S->list_cap);
-                    HIPCHECK(hipMallocAsync((void **)&S->d_list, S->list_cap * sizeof(int), S->s_comp));
-                }
-                HIPCHECK(hipMemcpyAsync(S->d_desc, S->h_desc, sizeof(SolveDesc) * S->ndesc, hipMemcpyHostToDevice,
-                                        S->s_comp));
-                const unsigned grid = (unsigned)std::min<long long>((tiles + 3) / 4, 2048);
-                for (int b = 0; b < batch; ++b) {
-                    const int parity = (int)(S->round & 1);
-                    hipLaunchKernelGGL(k_mschedule, dim3(maxblocks, S->ndesc), dim3(256), 0, S->s_comp, S->d_desc,
-                                       parity, S->d_list, &S->d_lc[parity]);
-                    HIPCHECK(hipGetLastError());
-                    hipLaunchKernelGGL(k_mfim, dim3(grid), dim3(256), 0, S->s_comp, S->d_desc, S->ndesc, parity,
-                                       S->d_list, &S->d_lc[parity], &S->d_lc[parity ^ 1]);
-                    HIPCHECK(hipGetLastError());
-                    S->round++;
-                }
-                for (Slot *sl : S->running) sl->rounds += batch;
-                HIPCHECK(hipMemcpyAsync(S->h_count, S->d_count, sizeof(int) * 2 * S->ndesc, hipMemcpyDeviceToHost,
-                                        S->s_comp));
-                HIPCHECK(hipMemcpyAsync(S->h_stats, S->d_stats,
-                                        sizeof(unsigned long long) * 2 * PEND_SLOTS * S->ndesc,
-                                        hipMemcpyDeviceToHost, S->s_comp));
-                HIPCHECK(hipStreamSynchronize(S->s_comp));
-                {
-                    std::lock_guard<std::mutex> lk(S->mu);
-                    S->stats.batches++;
-                    S->stats.rounds += batch;
-                }
-                const int last = (int)((S->round - 1) & 1);
-                std::vector<Slot *> still;
-                for (Slot *sl : S->running) {
-                    const bool finished = S->h_count[2 * sl->idx + last] == 0;
-                    const bool runaway = sl->rounds > 8LL * sl->rec->g.ntiles + 65536;
-                    if (!finished && !runaway) {
-                        still.push_back(sl);
-                        continue;
-                    }
-                    S->h_desc[sl->idx].active = 0;
-                    finish_export(S, sl);
-                    if (runaway) {
-                        sl->rec->res.status = 2;
-                        std::snprintf(sl->rec->res.error, sizeof(sl->rec->res.error),
-                                      "block fast-iterative method did not converge");
-                    }
-                    HIPCHECK(launch_export_values<float>(sl->rec->g, sl->u, sl->values_dev, sl->rec->np, S->s_comp));
-                    if (sl->rec->values_out)
-                        HIPCHECK(hipMemcpyAsync(sl->rec->values_out, sl->values_dev, sl->rec->N * sizeof(double),
-                                                hipMemcpyDeviceToHost, S->s_comp));
-                    HIPCHECK(hipEventCreateWithFlags(&sl->export_done, hipEventDisableTiming));
-                    HIPCHECK(hipEventRecord(sl->export_done, S->s_comp));
-                    S->exporting.push_back(sl);
-                }
-                S->running.swap(still);
+                const long long q = S->nlaunched;
+                launch_batch(S, q, batch);
+                last_parity[q & 1] = (int)((S->round - 1) & 1);
+                S->nlaunched = q + 1;
             }
-            // completed exports
-            std::vector<Slot *> pending;
-            for (Slot *sl : S->exporting) {
-                if (hipEventQuery(sl->export_done) != hipSuccess) {
-                    pending.push_back(sl);
-                    continue;
-                }
-                (void)hipEventDestroy(sl->export_done);
-                HIPCHECK(hipFreeAsync(sl->deps, S->s_comp));
-                HIPCHECK(hipFreeAsync(sl->arena, S->s_comp));
-                S->free_idx.push_back(sl->idx);
-                auto r = sl->rec;
-                const size_t bytes = r->bytes;
-                delete sl;
-                release_budget(S, bytes);
-                std::lock_guard<std::mutex> lk(S->mu);
-                r->t_done = Clock::now();
-                r->res.latency_ms = ms_since(r->t_submit, r->t_done);
-                r->done = true;
-                if (r->res.status == 0) S->stats.completed++;
-                else S->stats.failed++;
-                S->cv_done.notify_all();
+            // keep the batch just launched in flight; drain everything when nothing was launched
+            while (processed < S->nlaunched - (S->running.empty() ? 0 : 1)) {
+                const long long q = processed++;
+                process_batch(S, q, last_parity[q & 1]);
             }
-            S->exporting.swap(pending);
+            retire_exports(S);
             if (S->running.empty() && !S->exporting.empty()) std::this_thread::sleep_for(std::chrono::microseconds(50));
         } catch (const HipFail &e) {
             // A device error poisons every resident solve: fail them all and stop serving.
@@ -561,21 +687,33 @@ int hfm_server_create(hfm_server **out, double hbm_budget_bytes, int max_residen
         S->stats.hbm_budget = hbm_budget_bytes;
         for (int i = 0; i < hfm_server::NSETUP; ++i) {
             HIPCHECK(hipStreamCreateWithFlags(&S->s_setup[i], hipStreamNonBlocking));
-            S->d_red[i] = dalloc<RedSlot>(RED_SLOTS);
+            S->d_red[i] = dalloc<RedSlot>((size_t)RED_SLOTS * hfm_server::SETUP_GROUP);
+            HIPCHECK(hipHostMalloc((void **)&S->h_red[i], sizeof(RedSlot) * RED_SLOTS * hfm_server::SETUP_GROUP,
+                                   hipHostMallocDefault));
+            HIPCHECK(hipHostMalloc((void **)&S->h_status[i], sizeof(int) * 2 * hfm_server::SETUP_GROUP,
+                                   hipHostMallocDefault));
+            HIPCHECK(hipHostMalloc((void **)&S->h_total[i], sizeof(int) * hfm_server::SETUP_GROUP,
+                                   hipHostMallocDefault));
         }
         HIPCHECK(hipStreamCreateWithFlags(&S->s_comp, hipStreamNonBlocking));
         S->d_desc = dalloc<SolveDesc>(MAX_SLOTS);
-        HIPCHECK(hipHostMalloc((void **)&S->h_desc, sizeof(SolveDesc) * MAX_SLOTS, hipHostMallocDefault));
+        S->h_desc = new SolveDesc[MAX_SLOTS];
         std::memset(S->h_desc, 0, sizeof(SolveDesc) * MAX_SLOTS);
+        for (int i = 0; i < 2; ++i) {
+            HIPCHECK(hipHostMalloc((void **)&S->h_desc_stage[i], sizeof(SolveDesc) * MAX_SLOTS, hipHostMallocDefault));
+            HIPCHECK(hipEventCreateWithFlags(&S->batch_done[i], hipEventDisableTiming));
+        }
         S->d_lc = dalloc<int>(2);
         HIPCHECK(hipMemset(S->d_lc, 0, 2 * sizeof(int)));
         S->d_count = dalloc<int>(MAX_SLOTS * 2);
         S->d_pend = dalloc<unsigned>(MAX_SLOTS * 2 * PEND_SLOTS);
         S->d_bound = dalloc<float>(MAX_SLOTS * 4);
         S->d_stats = dalloc<unsigned long long>(MAX_SLOTS * 2 * PEND_SLOTS);
-        HIPCHECK(hipHostMalloc((void **)&S->h_count, sizeof(int) * MAX_SLOTS * 2, hipHostMallocDefault));
-        HIPCHECK(hipHostMalloc((void **)&S->h_stats, sizeof(unsigned long long) * MAX_SLOTS * 2 * PEND_SLOTS,
-                               hipHostMallocDefault));
+        for (int i = 0; i < 2; ++i) {
+            HIPCHECK(hipHostMalloc((void **)&S->h_count[i], sizeof(int) * MAX_SLOTS * 2, hipHostMallocDefault));
+            HIPCHECK(hipHostMalloc((void **)&S->h_stats[i], sizeof(unsigned long long) * MAX_SLOTS * 2 * PEND_SLOTS,
+                                   hipHostMallocDefault));
+        }
         for (int i = 0; i < hfm_server::NSETUP; ++i) S->th_setup[i] = std::thread(setup_loop, S, i);
         S->th_comp = std::thread(compute_loop, S);
     } catch (const HipFail &e) {
@@ -663,6 +801,12 @@ int hfm_server_submit(hfm_server *S, const hfm_request *q, int64_t *ticket) {
         r->seed_tp.push_back(kv.second.first);
         r->seed_v.push_back(kv.second.second);
     }
+    r->seed_blob = r->seed_tp;
+    for (double v : r->seed_v) {
+        long long bits;
+        std::memcpy(&bits, &v, sizeof(bits));
+        r->seed_blob.push_back(bits);
+    }
     r->values_out = q->values_out;
     r->bytes = arena_bytes(*r) + (size_t)nt * 64 * 4 + (q->metric_on_device ? 0 : r->metric_host.size() * 8);
     std::lock_guard<std::mutex> lk(S->mu);
@@ -742,16 +886,27 @@ void hfm_server_destroy(hfm_server *S) {
     for (int i = 0; i < hfm_server::NSETUP; ++i)
         if (S->s_setup[i]) (void)hipStreamSynchronize(S->s_setup[i]);
     if (S->d_list) (void)hipFree(S->d_list);
-    for (int i = 0; i < hfm_server::NSETUP; ++i) dfree(S->d_red[i]);
+    for (int i = 0; i < hfm_server::NSETUP; ++i) {
+        dfree(S->d_red[i]);
+        if (S->h_red[i]) (void)hipHostFree(S->h_red[i]);
+        if (S->h_status[i]) (void)hipHostFree(S->h_status[i]);
+        if (S->h_total[i]) (void)hipHostFree(S->h_total[i]);
+    }
     dfree(S->d_desc);
     dfree(S->d_lc);
     dfree(S->d_count);
     dfree(S->d_pend);
     dfree(S->d_bound);
     dfree(S->d_stats);
-    if (S->h_desc) (void)hipHostFree(S->h_desc);
-    if (S->h_count) (void)hipHostFree(S->h_count);
-    if (S->h_stats) (void)hipHostFree(S->h_stats);
+    delete[] S->h_desc;
+    if (S->d_bmap) (void)hipFree(S->d_bmap);
+    for (int i = 0; i < 2; ++i) {
+        if (S->h_desc_stage[i]) (void)hipHostFree(S->h_desc_stage[i]);
+        if (S->h_bmap_stage[i]) (void)hipHostFree(S->h_bmap_stage[i]);
+        if (S->h_count[i]) (void)hipHostFree(S->h_count[i]);
+        if (S->h_stats[i]) (void)hipHostFree(S->h_stats[i]);
+        if (S->batch_done[i]) (void)hipEventDestroy(S->batch_done[i]);
+    }
     if (S->s_comp) (void)hipStreamDestroy(S->s_comp);
     for (int i = 0; i < hfm_server::NSETUP; ++i)
         if (S->s_setup[i]) (void)hipStreamDestroy(S->s_setup[i]);

@@ -87,6 +87,47 @@ __device__ inline bool band_take(unsigned k, float B) {
     return k != KEY_NONE && (key_value(k) < B || !(B < __builtin_huge_valf()));
 }
 
+// Scheduling kernels run blocks of SCHED_BLOCK threads, one tile per thread.
+constexpr int SCHED_BLOCK = 1024;
+
+// Block-level aggregation of a scheduling pass (every thread of the block calls it once):
+// `take` = the thread's tile is listed this round, `left` = the key it leaves pending
+// (KEY_NONE if none).  One atomicAdd per block reserves the block's list entries (and adds them
+// to *solve_count if given), one atomicMin folds the block's smallest left key into *pend_slot.
+// Returns the list position of a taken tile.  Per-wave atomics on the shared list counter
+// would serialise thousands of waves on one address.
+__device__ inline int block_take(bool take, unsigned left, int *__restrict__ list_count,
+                                 int *__restrict__ solve_count, unsigned *__restrict__ pend_slot) {
+    constexpr int NW = SCHED_BLOCK / 64;
+    __shared__ int wcnt[NW];
+    __shared__ unsigned wmn[NW];
+    __shared__ int base_s;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const unsigned long long mask = __ballot(take);
+    const unsigned lmin = wave_min_u32(left);
+    if (lane == 0) {
+        wcnt[wid] = __popcll(mask);
+        wmn[wid] = lmin;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        unsigned m = KEY_NONE;
+        for (int w = 0; w < NW; ++w) {
+            tot += wcnt[w];
+            m = wmn[w] < m ? wmn[w] : m;
+        }
+        base_s = tot ? atomicAdd(list_count, tot) : 0;
+        if (tot && solve_count) atomicAdd(solve_count, tot);
+        if (m != KEY_NONE) atomicMin(pend_slot, m);
+    }
+    __syncthreads();
+    int pre = 0;
+    for (int w = 0; w < wid; ++w) pre += wcnt[w];
+    const unsigned long long below = lane == 0 ? 0ull : (mask & (~0ull >> (64 - lane)));
+    return base_s + pre + __popcll(below);
+}
+
 // LDS row of one wave: the tile's 64 values, then a +inf sentinel read by out-of-tile sides.
 constexpr int SU_SENTINEL = 64;
 constexpr int SU_ROW = 65;
