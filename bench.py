@@ -12,8 +12,20 @@ HBM before the timed region; the timed region is bracketed by torch.cuda.synchro
 Printed JSON fields beyond the driver's contract:
   roofline      update kernel (k_fim): algorithmic bytes per launch / mean launch duration
                 (hipEvent pairs around every launch on the plan's stream, in a separate
-                instrumented solve), against the 8 TB/s HBM peak; traffic from rocprofv3
-                counters when profiles/fim_riemann3_128_traffic.json exists.
+                instrumented solve), against the 8 TB/s HBM peak.  The unit of work is the
+                point-update (one local solve at one point in one tile sweep) at 50 B each;
+                achieved_per_visit counts each tile visit once instead (its data is read from
+                HBM once and swept in LDS/registers), and solve_effective_gbs divides the
+                stencil + value bytes of the whole grid by the solve time.  traffic = HBM bytes
+                per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by a 1 GiB copy
+                (profiles/fim_riemann3_128_traffic.json).
+  serving       the continuous-batching server on a stream of mixed Riemann2 / Riemann3
+                requests (70 % 2-D, side 64-1024 log-uniform; 30 % 3-D, side 32-160; 1-16
+                seeds; anisotropy 1-50): sustained solves/s with 128 requests kept in flight
+                (closed loop, window excluding the first and last 128 completions), the rate
+                of a burst with every request submitted at once (makespan), p50 / p99
+                latency at an offered Poisson load of 5x the one-at-a-time rate, and the
+                one-at-a-time plain GPU solves/s of the same requests.
   cpu_baseline  the reference's FileHFM_Riemann3, compiled from its sources into oracle/_ref,
                 timed on this host (one full solve, 1 core; StencilCPUTime + FMCPUTime), or the
                 oracle's C++ restatement when the reference binary is absent.
@@ -35,9 +47,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ms/solve Riemann3 128^3 vs HFM CPU; served solves/s + p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per point for one tile visit of the FP32 Riemann3 update: u read 4 +
-# u write 4 + 6 FP32 weights 24 + 6 x 3 int8 offsets 18 (neighbour reads are cache hits).
-BYTES_PER_POINT_VISIT = 50
+# Algorithmic bytes per point-update of the FP32 Riemann3 update: u read 4 + u write 4 +
+# 6 FP32 weights 24 + 6 x 3 int8 offsets 18 (neighbour reads are cache hits).
+BYTES_PER_POINT_UPDATE = 50
+# Whole-grid bytes of one solve: metric in 48 + stencil out 42 (stencil kernel), stencil in 42
+# + value out 4 (update), values out 8 (export).
+BYTES_PER_POINT_SOLVE = 48 + 42 + 42 + 4 + 8
 
 
 def parse():
@@ -47,6 +62,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serving-requests", type=int, default=300)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 1 GiB device-to-device copy (reference kernel for the "
                          "FETCH_SIZE / WRITE_SIZE counter scale)")
@@ -81,6 +98,44 @@ def cpu_baseline(params):
                   f"clock() CPU time), wall {wall:.1f} s",
         "fm_ms": round(fm_s * 1e3, 1), "nproc": os.cpu_count(),
     }, values
+
+
+def serving_leg(count):
+    """Continuous-batching server vs one-at-a-time plain GPU solves on the same request stream."""
+    import torch
+    from hamiltonfastmarching_amd import serving
+    reqs = serving.make_requests(count, seed=11)
+    serving.run_sequential(reqs[:4])
+    seq = serving.run_sequential(reqs)
+    srv = serving.Server(hbm_budget_bytes=64e9, max_resident=256)
+    try:
+        serving.run_load(srv, reqs[:16], qps=0)
+        burst = serving.run_load(srv, reqs, qps=0)
+        closed = serving.run_closed_loop(srv, reqs, concurrency=128, rounds=3)
+        offered = 5.0 * seq["solves_per_s"]
+        load = serving.run_load(srv, reqs, qps=offered)
+        stats = srv.stats()
+    finally:
+        srv.close()
+    torch.cuda.synchronize()
+    n2 = sum(r["model"] == "Riemann2" for r in reqs)
+    sustained = closed["sustained_solves_per_s"]
+    return {
+        "requests": count, "mix": {"Riemann2": n2, "Riemann3": count - n2},
+        "solves_per_s": round(sustained, 1),
+        "solves_per_s_definition": "sustained completion rate with 128 requests kept in flight (closed "
+                                   "loop over 3x the request stream; completions 128 .. 3n-128)",
+        "closed_loop_latency": {"p50_ms": round(closed["p50_ms"], 3), "p99_ms": round(closed["p99_ms"], 3)},
+        "sequential_solves_per_s": round(seq["solves_per_s"], 1),
+        "speedup_vs_sequential": round(sustained / seq["solves_per_s"], 2),
+        "burst": {"solves_per_s": round(burst["solves_per_s"], 1),
+                  "speedup_vs_sequential": round(burst["solves_per_s"] / seq["solves_per_s"], 2),
+                  **{k: round(burst[k], 3) for k in ("p50_ms", "p99_ms", "makespan_s")}},
+        "at_5x_sequential_rate": {"qps_offered": round(offered, 1), "solves_per_s": round(load["solves_per_s"], 1),
+                                  "p50_ms": round(load["p50_ms"], 3), "p99_ms": round(load["p99_ms"], 3)},
+        "sequential_latency": {"p50_ms": round(seq["p50_ms"], 3), "p99_ms": round(seq["p99_ms"], 3)},
+        "max_resident_seen": stats["max_resident_seen"],
+    }
 
 
 def floor_rel(u, ref):
@@ -134,9 +189,13 @@ def main():
     sti = plan.stats()
     plan.set_timing(False)
     launches = max(sti["fim_launches"], 1)
-    bytes_total = sti["tile_visits"] * 64 * BYTES_PER_POINT_VISIT
     mean_launch_ms = sti["fim_ms"] / launches
-    achieved = (bytes_total / launches) / (mean_launch_ms * 1e-3) / 1e9 if mean_launch_ms > 0 else 0.0
+
+    def gbs(total_bytes):
+        return (total_bytes / launches) / (mean_launch_ms * 1e-3) / 1e9 if mean_launch_ms > 0 else 0.0
+
+    achieved = gbs(sti["sweeps"] * 64 * BYTES_PER_POINT_UPDATE)
+    achieved_visit = gbs(sti["tile_visits"] * 64 * BYTES_PER_POINT_UPDATE)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "fim_riemann3_128_traffic.json")
     if os.path.exists(tpath) and n == 128:
@@ -146,8 +205,10 @@ def main():
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
         "kernel": "k_fim<Riemann3,float>", "launches": sti["fim_launches"],
-        "mean_launch_us": round(mean_launch_ms * 1e3, 2),
-        "tile_visits": sti["tile_visits"], "bytes_per_point_visit": BYTES_PER_POINT_VISIT,
+        "mean_launch_us": round(mean_launch_ms * 1e3, 2), "unit_of_work": "point-update",
+        "bytes_per_point_update": BYTES_PER_POINT_UPDATE, "point_updates": sti["sweeps"] * 64,
+        "tile_visits": sti["tile_visits"], "achieved_per_visit": round(achieved_visit, 1),
+        "solve_effective_gbs": round(n ** 3 * BYTES_PER_POINT_SOLVE / (ms * 1e-3) / 1e9, 1),
     }
     result = {
         "metric": METRIC, "value": round(ms, 3), "unit": "ms/solve", "n_gpus": 1,
@@ -161,6 +222,8 @@ def main():
         "fim_iterations": st["iterations"], "tile_visits": st["tile_visits"], "sweeps": st["sweeps"],
         "roofline": roofline,
     }
+    if not args.no_serving:
+        result["serving"] = serving_leg(args.serving_requests)
     if args.copy_calibration:
         src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
         dst = torch.empty_like(src)

@@ -69,7 +69,9 @@ hipError_t launch_init(const Geom &g, T *u, unsigned long long *fixedmask, unsig
 hipError_t launch_seed_activate(const long long *seed_p, const double *seed_v, int nseeds, const int *dep_start,
                                 const int *deps, unsigned *keys, const unsigned *tmax, unsigned *pend, hipStream_t s);
 hipError_t launch_schedule(int ntiles, unsigned *keys, const SchedRound &sr, hipStream_t s);
-template <typename T> hipError_t launch_fim(int model, const FimArgs<T> &a, hipStream_t s);
+template <typename T>
+hipError_t launch_fim(int model, const FimArgs<T> &a, hipStream_t s, hipEvent_t ev0 = nullptr,
+                      hipEvent_t ev1 = nullptr);
 template <typename T>
 hipError_t launch_export_values(const Geom &g, const T *u, double *out, long long np, hipStream_t s);
 template <typename T>

@@ -11,6 +11,8 @@
 // only symbols the plan code (hfm_plan.hip) uses.
 #include "hfm_kernels.h"
 
+#include <hip/hip_ext.h>
+
 #include "hfm_device.h"
 #include "selling.h"
 #include "hfm_tile.h"
@@ -521,25 +523,34 @@ hipError_t launch_schedule(int ntiles, unsigned *keys, const SchedRound &sr, hip
     return hipGetLastError();
 }
 
+// With ev0/ev1 the launch goes through hipExtLaunchKernelGGL, whose events take the begin and
+// end timestamps of the dispatch itself (the kernel's execution time, without the dispatch gap
+// that an event recorded before the launch would include).
 template <int M, typename T>
-static void fim_one(const FimArgs<T> &a, hipStream_t s) {
-    hipLaunchKernelGGL((k_fim<M, T, 32>), dim3(a.grid), dim3(256), 0, s, a.g, a.sd, a.u, a.fixedmask, a.dep_start,
-                       a.deps, a.list, a.count, a.count_reset, a.keys, a.tmax, a.pend_out, a.pend_reset, a.stats);
+static void fim_one(const FimArgs<T> &a, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+    if (ev0)
+        hipExtLaunchKernelGGL((k_fim<M, T, 32>), dim3(a.grid), dim3(256), 0, s, ev0, ev1, 0, a.g, a.sd, a.u,
+                              a.fixedmask, a.dep_start, a.deps, a.list, a.count, a.count_reset, a.keys, a.tmax,
+                              a.pend_out, a.pend_reset, a.stats);
+    else
+        hipLaunchKernelGGL((k_fim<M, T, 32>), dim3(a.grid), dim3(256), 0, s, a.g, a.sd, a.u, a.fixedmask,
+                           a.dep_start, a.deps, a.list, a.count, a.count_reset, a.keys, a.tmax, a.pend_out,
+                           a.pend_reset, a.stats);
 }
 
 template <typename T>
-hipError_t launch_fim(int model, const FimArgs<T> &a, hipStream_t s) {
+hipError_t launch_fim(int model, const FimArgs<T> &a, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
     switch (model) {
-        case MODEL_RIEMANN2: fim_one<MODEL_RIEMANN2, T>(a, s); break;
-        case MODEL_RIEMANN3: fim_one<MODEL_RIEMANN3, T>(a, s); break;
-        case MODEL_RSF2: fim_one<MODEL_RSF2, T>(a, s); break;
-        case MODEL_ELASTICA2: fim_one<MODEL_ELASTICA2, T>(a, s); break;
+        case MODEL_RIEMANN2: fim_one<MODEL_RIEMANN2, T>(a, s, ev0, ev1); break;
+        case MODEL_RIEMANN3: fim_one<MODEL_RIEMANN3, T>(a, s, ev0, ev1); break;
+        case MODEL_RSF2: fim_one<MODEL_RSF2, T>(a, s, ev0, ev1); break;
+        case MODEL_ELASTICA2: fim_one<MODEL_ELASTICA2, T>(a, s, ev0, ev1); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
-template hipError_t launch_fim<float>(int, const FimArgs<float> &, hipStream_t);
-template hipError_t launch_fim<double>(int, const FimArgs<double> &, hipStream_t);
+template hipError_t launch_fim<float>(int, const FimArgs<float> &, hipStream_t, hipEvent_t, hipEvent_t);
+template hipError_t launch_fim<double>(int, const FimArgs<double> &, hipStream_t, hipEvent_t, hipEvent_t);
 
 template <typename T>
 hipError_t launch_export_values(const Geom &g, const T *u, double *out, long long np, hipStream_t s) {

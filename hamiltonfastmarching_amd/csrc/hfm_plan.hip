@@ -320,9 +320,7 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
                 hipEvent_t ea, eb;
                 HIPCHECK(hipEventCreate(&ea));
                 HIPCHECK(hipEventCreate(&eb));
-                HIPCHECK(hipEventRecord(ea, s));
-                HIPCHECK(launch_fim<T>(p->model, a, s));
-                HIPCHECK(hipEventRecord(eb, s));
+                HIPCHECK(launch_fim<T>(p->model, a, s, ea, eb));
                 tev.push_back(ea);
                 tev.push_back(eb);
             } else {

@@ -17,7 +17,7 @@ import numpy as np
 from . import _lib
 from . import PyExp, Plan
 
-__all__ = ["Server", "make_requests", "run_load", "run_sequential", "latency_summary"]
+__all__ = ["Server", "make_requests", "run_load", "run_closed_loop", "run_sequential", "latency_summary"]
 
 
 class Server:
@@ -212,10 +212,57 @@ def run_load(server: Server, requests, qps: float, seed: int = 1, collect_values
     ok = [x for x in res if x["status"] == 0]
     out = {"qps_offered": qps, "requests": len(requests), "completed": len(ok),
            "solves_per_s": len(ok) / (t1 - t0), "makespan_s": t1 - t0}
+
     out.update(latency_summary([x["latency_ms"] for x in ok]))
     out["queue_p50_ms"] = float(np.percentile([x["queue_ms"] for x in ok], 50))
     if collect_values:
         out["values"] = [o.reshape(r["dims"]) for o, r in zip(outs, requests)]
+    return out
+
+
+def run_closed_loop(server: Server, requests, concurrency: int = 128, rounds: int = 3):
+    """Closed-loop load: keep `concurrency` requests in flight, submitting the next one (cycling
+    through `requests`, `rounds` times) whenever one completes.  The sustained rate counts the
+    completions between the concurrency-th and the (total - concurrency)-th, a window in which
+    the server always holds `concurrency` requests and whose mix is that of the stream."""
+    total = len(requests) * rounds
+    inflight = {}
+    done_times = []
+    lat = []
+    nxt = 0
+    t0 = time.perf_counter()
+
+    def submit_next():
+        nonlocal nxt
+        r = requests[nxt % len(requests)]
+        inflight[server.submit(r["model"], r["dims"], r["gridScale"], r["metric"], r["seeds"])] = nxt
+        nxt += 1
+
+    while nxt < min(concurrency, total):
+        submit_next()
+    while inflight:
+        finished = [t for t in list(inflight) if server.poll(t)]
+        if not finished:
+            time.sleep(0.0002)
+            continue
+        now = time.perf_counter()
+        for t in finished:
+            res = server.wait(t)
+            del inflight[t]
+            if res["status"] != 0:
+                raise PyExp("request failed: " + res["error"])
+            done_times.append(now)
+            lat.append(res["latency_ms"])
+            if nxt < total:
+                submit_next()
+    t1 = time.perf_counter()
+    done_times = np.sort(np.asarray(done_times))
+    a, b = concurrency, total - concurrency
+    out = {"requests": total, "concurrency": concurrency, "makespan_s": t1 - t0,
+           "solves_per_s": len(done_times) / (t1 - t0)}
+    if b > a + 8:
+        out["sustained_solves_per_s"] = (b - a) / max(done_times[b - 1] - done_times[a - 1], 1e-9)
+    out.update(latency_summary(lat))
     return out
 
 
