@@ -284,6 +284,11 @@ class Plan:
         self._check(self._L.hfm_plan_set_timing(self._h, int(enable)))
         return self
 
+    def set_graph(self, enable: bool = True):
+        """Run the update loop as a replayed HIP graph (default) or as individual launches."""
+        self._check(self._L.hfm_plan_set_graph(self._h, int(enable)))
+        return self
+
 
 class EikonalSolve:
     """``EikonalSolve.apply(cost, plan, seeds, targets, weights)`` -> sum_t weights[t] u(targets[t]).

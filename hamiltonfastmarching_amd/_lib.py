@@ -73,6 +73,7 @@ SIGNATURES = [
                                  c_int, P(c_int), c_void_p]),
     ("hfm_plan_get_stats", c_int, [c_void_p, P(HfmPlanStats)]),
     ("hfm_plan_set_timing", c_int, [c_void_p, c_int]),
+    ("hfm_plan_set_graph", c_int, [c_void_p, c_int]),
     ("hfm_plan_set_delta", c_int, [c_void_p, c_double]),
     ("hfm_plan_set_round_target", c_int, [c_void_p, c_int]),
     ("hfm_plan_set_grid", c_int, [c_void_p, c_int]),

@@ -96,6 +96,13 @@ struct hfm_plan {
     bool stencil_ready = false;
     bool solved = false;
     bool timing = false;
+    // The update loop runs as a captured HIP graph of one batch of rounds (schedule + update
+    // launches and the read-back of the last round's count), replayed until convergence.  The
+    // graph is captured on graph_stream and re-captured when its launch arguments change.
+    bool use_graph = true;
+    hipStream_t graph_stream = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<char> graph_key;
     int max_stencil_width = 0;
     std::vector<long long> last_seeds;  // tiled indices of the last solve's seeds
     hfm_plan_stats st{};

@@ -31,7 +31,7 @@ struct SchedRound {
     float *bound_out;
     int *list;
     int *count;
-    float delta;            // initial band width; <= 0 lists every pending tile
+    const float *delta0;    // initial band width (device word); <= 0 lists every pending tile
     const float *delta_in;  // band width of the previous round
     float *delta_out;       // band width of this round
     const int *prev_count;  // tiles the previous round listed

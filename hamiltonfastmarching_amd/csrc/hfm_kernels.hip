@@ -265,8 +265,9 @@ __global__ __launch_bounds__(64) void k_seed_activate(const long long *__restric
 // keys this kernel leaves pending and by the marks of the update kernel of round k.
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(SCHED_BLOCK) void k_schedule(int ntiles, unsigned *__restrict__ keys, SchedRound sr) {
-    float delta = sr.delta;
-    if (delta > 0.f) delta = adapt_delta(*sr.delta_in, *sr.prev_count, sr.target, 0.5f * sr.delta, 64.f * sr.delta);
+    const float d0 = *sr.delta0;
+    float delta = d0;
+    if (delta > 0.f) delta = adapt_delta(*sr.delta_in, *sr.prev_count, sr.target, 0.5f * d0, 64.f * d0);
     const float B = band_bound(key_value(read_pend(sr.pend_in)), *sr.bound_in, delta);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *sr.bound_out = B;
@@ -295,6 +296,11 @@ __global__ __launch_bounds__(256) void k_fim(Geom g, StencilData<T> sd, T *__res
                                              unsigned *__restrict__ pend_reset, unsigned long long *__restrict__ stats) {
     __shared__ T su[4][SU_ROW];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int stride = gridDim.x * 4;
+    int i = blockIdx.x * 4 + wv;
+    // The list holds ntiles entries: read this wave's first entry together with the count (an
+    // entry past the count is stale and never used) instead of after it.
+    int next = i < g.ntiles ? list[i] : 0;
     const int n = *count;
     if (blockIdx.x == 0) {
         if (threadIdx.x == 0) *count_reset = 0;
@@ -303,8 +309,9 @@ __global__ __launch_bounds__(256) void k_fim(Geom g, StencilData<T> sd, T *__res
     const TileCtx<T> ctx{g, sd, u, fixedmask, dep_start, deps, keys, tmax};
     unsigned long long my_sweeps = 0, my_visits = 0;
     unsigned wmin = KEY_NONE;
-    for (int i = blockIdx.x * 4 + wv; i < n; i += gridDim.x * 4) {
-        const int tile = list[i];
+    for (; i < n; i += stride) {
+        const int tile = next;
+        next = i + stride < n ? list[i + stride] : 0;
         my_sweeps += fim_tile<ModelT, T, MAXSWEEP>(ctx, tile, lane, su[wv], wmin);
         my_visits += 1;
     }

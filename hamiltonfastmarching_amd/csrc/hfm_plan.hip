@@ -111,6 +111,8 @@ void plan_free(hfm_plan *p) {
     dfree(p->seed_v);
     if (p->h_count) (void)hipHostFree(p->h_count);
     if (p->h_stats) (void)hipHostFree(p->h_stats);
+    if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+    if (p->graph_stream) (void)hipStreamDestroy(p->graph_stream);
 }
 
 // Build the tile dependency graph from the stencils now on the device.
@@ -273,13 +275,14 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
     HIPCHECK(hipMemsetAsync(p->stats, 0, 2 * PEND_SLOTS * sizeof(unsigned long long), s));
     HIPCHECK(hipMemsetAsync(p->nextmin, 0xff, 2 * PEND_SLOTS * sizeof(unsigned), s));
     const float delta0 = (float)(p->delta_factor > 0 ? p->delta_factor * p->tau : 0.);
-    const float binit[4] = {0.f, -__builtin_huge_valf(), delta0, delta0};
+    const float binit[5] = {0.f, -__builtin_huge_valf(), delta0, delta0, delta0};
     HIPCHECK(hipMemcpyAsync(p->bounds, binit, sizeof(binit), hipMemcpyHostToDevice, s));
     const int target = p->round_target > 0 ? p->round_target : std::max(64, std::min(256, p->g.ntiles / 4));
     HIPCHECK(launch_seed_activate(p->seed_p, p->seed_v, nseeds, p->dep_start, p->deps, p->keys, p->tmax,
                                   &p->nextmin[0], s));
 
     FimArgs<T> a;
+    std::memset(&a, 0, sizeof(a));  // the bytes of `a` key the captured graph
     a.g = p->g;
     a.sd = p->sd<T>();
     a.u = static_cast<T *>(p->u);
@@ -291,44 +294,69 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
     a.list = p->lists[0];
     a.stats = p->stats;
     a.grid = (unsigned)std::min<long long>((p->g.ntiles + 3) / 4, p->fim_grid > 0 ? p->fim_grid : 1024);
-    const float delta = delta0;
-
     std::vector<hipEvent_t> tev;
     long long k = 0;
-    const int batch = 8;
+    const int batch = 8;  // even: every batch starts on an even round, so all batches are alike
     const long long max_rounds = 8LL * p->g.ntiles + 65536;
-    while (true) {
-        for (int b = 0; b < batch; ++b, ++k) {
-            // round k: schedule lists tiles into lists[0] / counts[k&1], update consumes them
-            SchedRound sr{&p->nextmin[(k & 1) * PEND_SLOTS],
-                          &p->nextmin[((k + 1) & 1) * PEND_SLOTS],
-                          &p->bounds[(k + 1) & 1],
-                          &p->bounds[k & 1],
+    // One batch of rounds k0 .. k0 + batch - 1 on stream st, then the read-back of the count
+    // of its last round (the update kernel cleared only the other slot).
+    auto enqueue_batch = [&](long long k0, hipStream_t st, bool timed) {
+        for (long long kk = k0; kk < k0 + batch; ++kk) {
+            // round kk: schedule lists tiles into lists[0] / counts[kk&1], update consumes them
+            SchedRound sr{&p->nextmin[(kk & 1) * PEND_SLOTS],
+                          &p->nextmin[((kk + 1) & 1) * PEND_SLOTS],
+                          &p->bounds[(kk + 1) & 1],
+                          &p->bounds[kk & 1],
                           p->lists[0],
-                          &p->counts[k & 1],
-                          delta,
-                          &p->bounds[2 + ((k + 1) & 1)],
-                          &p->bounds[2 + (k & 1)],
-                          &p->counts[(k + 1) & 1],
+                          &p->counts[kk & 1],
+                          &p->bounds[4],
+                          &p->bounds[2 + ((kk + 1) & 1)],
+                          &p->bounds[2 + (kk & 1)],
+                          &p->counts[(kk + 1) & 1],
                           target};
-            HIPCHECK(launch_schedule(p->g.ntiles, p->keys, sr, s));
-            a.count = &p->counts[k & 1];
-            a.count_reset = &p->counts[(k + 1) & 1];
-            a.pend_out = &p->nextmin[((k + 1) & 1) * PEND_SLOTS];
-            a.pend_reset = &p->nextmin[(k & 1) * PEND_SLOTS];
-            if (p->timing) {
+            HIPCHECK(launch_schedule(p->g.ntiles, p->keys, sr, st));
+            a.count = &p->counts[kk & 1];
+            a.count_reset = &p->counts[(kk + 1) & 1];
+            a.pend_out = &p->nextmin[((kk + 1) & 1) * PEND_SLOTS];
+            a.pend_reset = &p->nextmin[(kk & 1) * PEND_SLOTS];
+            if (timed) {
                 hipEvent_t ea, eb;
                 HIPCHECK(hipEventCreate(&ea));
                 HIPCHECK(hipEventCreate(&eb));
-                HIPCHECK(launch_fim<T>(p->model, a, s, ea, eb));
+                HIPCHECK(launch_fim<T>(p->model, a, st, ea, eb));
                 tev.push_back(ea);
                 tev.push_back(eb);
             } else {
-                HIPCHECK(launch_fim<T>(p->model, a, s));
+                HIPCHECK(launch_fim<T>(p->model, a, st));
             }
         }
-        // count of the last scheduled round (the update kernel cleared only the other slot)
-        HIPCHECK(hipMemcpyAsync(p->h_count, &p->counts[(k - 1) & 1], sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(p->h_count, &p->counts[(k0 + batch - 1) & 1], sizeof(int), hipMemcpyDeviceToHost,
+                                st));
+    };
+    const bool graph = p->use_graph && !p->timing;
+    if (graph) {
+        // launch arguments of the batch: re-capture when any of them changed
+        std::vector<char> key(sizeof(a) + sizeof(target));
+        std::memcpy(key.data(), &a, sizeof(a));
+        std::memcpy(key.data() + sizeof(a), &target, sizeof(target));
+        if (!p->graph_exec || key != p->graph_key) {
+            if (p->graph_exec) HIPCHECK(hipGraphExecDestroy(p->graph_exec));
+            p->graph_exec = nullptr;
+            if (!p->graph_stream) HIPCHECK(hipStreamCreateWithFlags(&p->graph_stream, hipStreamNonBlocking));
+            hipGraph_t g;
+            HIPCHECK(hipStreamBeginCapture(p->graph_stream, hipStreamCaptureModeThreadLocal));
+            enqueue_batch(0, p->graph_stream, false);
+            HIPCHECK(hipStreamEndCapture(p->graph_stream, &g));
+            const hipError_t ie = hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIPCHECK(ie);
+            p->graph_key = key;
+        }
+    }
+    while (true) {
+        if (graph) HIPCHECK(hipGraphLaunch(p->graph_exec, s));
+        else enqueue_batch(k, s, p->timing);
+        k += batch;
         HIPCHECK(hipStreamSynchronize(s));
         if (*p->h_count == 0) break;
         if (k > max_rounds) throw HipFail{"block fast-iterative method did not converge"};
@@ -348,7 +376,7 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
         p->st.sweeps += (int64_t)p->h_stats[2 * i + 1];
     }
     p->st.tiles_carried = 0;
-    p->st.delta = delta;
+    p->st.delta = delta0;
     double fim = 0;
     // Each pair brackets one launch; launches whose list was empty still count as launches.
     for (size_t i = 0; i + 1 < tev.size(); i += 2) {
@@ -491,7 +519,7 @@ int hfm_plan_create(hfm_plan **out, const char *model, const int64_t *dims, cons
         p->keys = dalloc<unsigned>(nt);
         p->tmax = dalloc<unsigned>(nt);
         p->nextmin = dalloc<unsigned>(2 * PEND_SLOTS);
-        p->bounds = dalloc<float>(4);
+        p->bounds = dalloc<float>(5);  // 2 band bounds, 2 band widths, initial band width
         p->red = dalloc<RedSlot>(RED_SLOTS);
         p->h_red.resize(RED_SLOTS);
         p->status = dalloc<int>(4);
@@ -775,6 +803,15 @@ int hfm_plan_set_round_target(hfm_plan *p, int tiles) {
         return 1;
     }
     p->round_target = tiles;
+    return 0;
+}
+
+int hfm_plan_set_graph(hfm_plan *p, int enable) {
+    if (!p) {
+        set_error("hfm_plan_set_graph: null argument");
+        return 1;
+    }
+    p->use_graph = enable != 0;
     return 0;
 }
 

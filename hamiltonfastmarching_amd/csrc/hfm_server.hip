@@ -79,7 +79,8 @@ __global__ __launch_bounds__(SCHED_BLOCK) void k_mschedule(const SolveDesc *__re
 
 // Update round over the packed list of (solve, tile) entries of every resident solve.
 __global__ __launch_bounds__(256) void k_mfim(const SolveDesc *__restrict__ descs, int ndesc, int parity,
-                                              const int *__restrict__ list, const int *__restrict__ list_count,
+                                              const int *__restrict__ list, long long list_cap,
+                                              const int *__restrict__ list_count,
                                               int *__restrict__ list_count_reset) {
     __shared__ float su[4][SU_ROW];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -95,8 +96,12 @@ __global__ __launch_bounds__(256) void k_mfim(const SolveDesc *__restrict__ desc
     }
     const int gw = blockIdx.x * 4 + wv;
     const int slot = gw & (PEND_SLOTS - 1);
-    for (int i = gw; i < n; i += gridDim.x * 4) {
-        const int e = __builtin_amdgcn_readfirstlane(list[i]);
+    const int stride = gridDim.x * 4;
+    // read this wave's first entry together with the count (entries past it are never used)
+    int next = gw < list_cap ? list[gw] : 0;
+    for (int i = gw; i < n; i += stride) {
+        const int e = __builtin_amdgcn_readfirstlane(next);
+        next = i + stride < n ? list[i + stride] : 0;
         const int s = e >> SOLVE_SHIFT;
         const int tile = e & ((1 << SOLVE_SHIFT) - 1);
         const SolveDesc &d = descs[s];
@@ -538,7 +543,7 @@ void launch_batch(hfm_server *S, long long q, int batch) {
         hipLaunchKernelGGL(k_mschedule, dim3((unsigned)bm.size()), dim3(SCHED_BLOCK), 0, s, S->d_desc, S->d_bmap,
                            parity, S->d_list, &S->d_lc[parity]);
         HIPCHECK(hipGetLastError());
-        hipLaunchKernelGGL(k_mfim, dim3(grid), dim3(256), 0, s, S->d_desc, S->ndesc, parity, S->d_list,
+        hipLaunchKernelGGL(k_mfim, dim3(grid), dim3(256), 0, s, S->d_desc, S->ndesc, parity, S->d_list, S->list_cap,
                            &S->d_lc[parity], &S->d_lc[parity ^ 1]);
         HIPCHECK(hipGetLastError());
         S->round++;

@@ -164,9 +164,15 @@ int hfm_plan_set_round_target(hfm_plan *plan, int tiles);
  * (0: 1024).  A tuning knob; results do not depend on it. */
 int hfm_plan_set_grid(hfm_plan *plan, int blocks);
 
-/* Per-launch timing of the update kernel (hipEvent pairs around every launch; off by default
- * because the events add a few microseconds per launch).  Fills fim_ms / fim_launches. */
+/* Per-launch timing of the update kernel (start/stop events taken from each dispatch's own
+ * timestamps; off by default).  Fills fim_ms / fim_launches.  Timed solves run the rounds as
+ * individual launches, not as the captured graph. */
 int hfm_plan_set_timing(hfm_plan *plan, int enable);
+
+/* Run the update loop as a captured HIP graph of one batch of rounds, replayed until
+ * convergence (default on), or as individual launches.  Both compute the same fixed point.
+ * No counterpart in the reference (its loop is the serial heap of HamiltonFastMarching.hxx). */
+int hfm_plan_set_graph(hfm_plan *plan, int enable);
 
 /* ----------------------------------------------------------------------------------------- */
 /* Continuous-batching solve server (no reference counterpart: the reference solves one      */
