@@ -56,6 +56,11 @@ def lib():
                                    P(d), P(ctypes.c_uint64), P(d), P(d)]
         L.oracle_solve_backward.argtypes = [i32, P(i32), P(d), P(d), P(d), i32, i32, d, d, P(d), i32, i32,
                                             P(d), P(d), i32, P(d), P(d), P(d), P(d), P(i64), P(d), i32, P(i32)]
+        L.oracle_geodesics.argtypes = [i32, P(i32), P(d), P(d), P(d), i32, i32, d, d, P(d), i32, i32, P(d), P(d),
+                                       i32, P(d), d, d, d, P(d), ctypes.c_long, P(i64), P(ctypes.c_long)]
+        L.oracle_geodesics_given.argtypes = [i32, P(i32), P(d), P(d), P(d), i32, i32, d, d, P(d), i32, P(d),
+                                             P(ctypes.c_uint64), i32, P(d), d, d, d, P(d), ctypes.c_long, P(i64),
+                                             P(ctypes.c_long)]
         _lib = L
     return _lib
 
@@ -192,6 +197,57 @@ def solve_backward(params, targets, weights=None):
         raise RuntimeError(_err())
     n = min(ns.value, nmax)
     return values.reshape(dims), sens.reshape(dims), (si[:n].copy(), sw[:n].copy())
+
+
+def geodesics(params, tips, step=0.25, threshold=0.001, volume_bound=-1.0):
+    """FM + the discrete geodesic solver from each tip: (points (M x dim), lengths (ntips))."""
+    (model, dims, dim, N, h, origin, metric, mif, dual, cost, cif, seeds, sv, xi, eps) = _common_args(params)
+    tips = np.ascontiguousarray(np.asarray(tips, np.float64).reshape(-1, dim))
+    lengths = np.zeros(len(tips), np.int64)
+    npts = ctypes.c_long(0)
+    dimsa = np.array(dims, np.int32)
+    cap = 1 << 16
+    while True:
+        pts = np.zeros((cap, dim))
+        r = lib().oracle_geodesics(MODELS[model], _ptr(dimsa, ctypes.c_int), _ptr(h, ctypes.c_double),
+                                   _ptr(origin, ctypes.c_double), _ptr(metric, ctypes.c_double), mif, dual,
+                                   xi, eps, _ptr(cost, ctypes.c_double), cif, len(seeds),
+                                   _ptr(seeds, ctypes.c_double), _ptr(sv, ctypes.c_double), len(tips),
+                                   _ptr(tips, ctypes.c_double), float(step), float(threshold),
+                                   float(volume_bound), _ptr(pts, ctypes.c_double), cap,
+                                   _ptr(lengths, ctypes.c_int64), ctypes.byref(npts))
+        if r:
+            raise RuntimeError(_err())
+        if npts.value <= cap:
+            return pts[:npts.value].copy(), lengths.copy()
+        cap = npts.value
+
+
+def geodesics_given(params, values, active, tips, step=0.25, threshold=0.001, volume_bound=-1.0):
+    """The discrete geodesic solver on given values / activeNeighs (e.g. another solver's)."""
+    (model, dims, dim, N, h, origin, metric, mif, dual, cost, cif, seeds, sv, xi, eps) = _common_args(params)
+    tips = np.ascontiguousarray(np.asarray(tips, np.float64).reshape(-1, dim))
+    vals = np.ascontiguousarray(np.asarray(values, np.float64).ravel())
+    act = np.ascontiguousarray(np.asarray(active).astype(np.uint64).ravel())
+    assert vals.size == N and act.size == N
+    lengths = np.zeros(len(tips), np.int64)
+    npts = ctypes.c_long(0)
+    dimsa = np.array(dims, np.int32)
+    cap = 1 << 16
+    while True:
+        pts = np.zeros((cap, dim))
+        r = lib().oracle_geodesics_given(MODELS[model], _ptr(dimsa, ctypes.c_int), _ptr(h, ctypes.c_double),
+                                         _ptr(origin, ctypes.c_double), _ptr(metric, ctypes.c_double), mif, dual,
+                                         xi, eps, _ptr(cost, ctypes.c_double), cif, _ptr(vals, ctypes.c_double),
+                                         _ptr(act, ctypes.c_uint64), len(tips), _ptr(tips, ctypes.c_double),
+                                         float(step), float(threshold), float(volume_bound),
+                                         _ptr(pts, ctypes.c_double), cap, _ptr(lengths, ctypes.c_int64),
+                                         ctypes.byref(npts))
+        if r:
+            raise RuntimeError(_err())
+        if npts.value <= cap:
+            return pts[:npts.value].copy(), lengths.copy()
+        cap = npts.value
 
 
 # ------------------------------------------------------------------------------------------

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <ctime>
 #include <limits>
+#include <array>
 #include <map>
 #include <queue>
 #include <string>
@@ -590,6 +591,209 @@ void backward_variation(const Problem &P, const FMState &S, const std::vector<st
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Geodesic backtracking, discrete solver (GeodesicDiscreteSolver.h:14-24, .hxx:17-231): the unit
+// mass at the tip is split among the 2^d surrounding grid points and transported down the
+// discrete flow (Recompute, order 1) in decreasing (value, index) order; the running weighted
+// mean of the mass positions gives the geodesic points.  Arithmetic follows the reference:
+// vector / scalar is a multiplication by the reciprocal (VectorType.h:37-42), symmetric
+// matrices are packed, rank-one tensors are u_i u_j.
+// ---------------------------------------------------------------------------------------------
+struct GeoParams {
+    double step = 0.25, threshold = 0.001, volumeBound = -1;  // volumeBound <= 0: 5 * 1.3^d
+};
+using Idx3 = std::array<int, 3>;
+
+// Discrete flow of HopfLaxRecompute (EulerianStencil.hxx:155-233) at order 1: the active
+// differences in stencil order, weight w_i max(0, r - v_i) with r the recomputed value.
+int discrete_flow(const Problem &P, const FMState &S, long x, int (*off)[3], double *w) {
+    ActiveDiff ad[2 * kMaxDiff];
+    const int n = S.active[x] ? active_diffs(P, S, x, ad) : 0;
+    if (n == 0) return 0;
+    double vals[2 * kMaxDiff];
+    Quad q; q.minVal = kInf;
+    for (int i = 0; i < n; ++i) { vals[i] = S.values[ad[i].neigh]; q.minVal = std::min(q.minVal, vals[i]); }
+    for (int i = 0; i < n; ++i) q.add(vals[i], 1. * ad[i].w);
+    const double value = q.solve();
+    for (int i = 0; i < n; ++i) {
+        w[i] = ad[i].w * std::max(0., value - vals[i]);
+        for (int c = 0; c < 3; ++c) off[i][c] = c < P.dim ? ad[i].s * ad[i].off[c] : 0;
+    }
+    return n;
+}
+
+struct GeoSums {
+    int d = 2;
+    double weightSum = 0, indexSum[3] = {0, 0, 0}, cov[6] = {0, 0, 0, 0, 0, 0};
+    void update(double weight, const Idx3 &ind) {
+        weightSum += weight;
+        double v[3];
+        for (int i = 0; i < d; ++i) v[i] = double(ind[i]);
+        for (int i = 0; i < d; ++i) indexSum[i] += v[i] * weight;
+        for (int i = 0; i < d; ++i)
+            for (int j = 0; j <= i; ++j) cov[i * (i + 1) / 2 + j] += (v[i] * v[j]) * weight;
+    }
+    void mean(double *m) const {
+        const double b = 1. / weightSum;
+        for (int i = 0; i < d; ++i) m[i] = indexSum[i] * b;
+    }
+};
+
+double det_sym(int d, const double *c) {
+    auto at = [&](int i, int j) { return i >= j ? c[i * (i + 1) / 2 + j] : c[j * (j + 1) / 2 + i]; };
+    if (d == 2) return at(0, 0) * at(1, 1) - at(1, 0) * at(0, 1);
+    return at(0, 0) * (at(1, 1) * at(2, 2) - at(1, 2) * at(2, 1)) +
+           at(0, 1) * (2. * at(1, 2) * at(2, 0) - at(1, 0) * at(2, 2)) - at(0, 2) * (at(1, 1) * at(2, 0));
+}
+
+// One pass of GeodesicDiscrete from geodesic.back(); returns true when a restart is needed.
+bool geodesic_pass(const Problem &P, const FMState &S, std::vector<std::array<double, 3>> &geodesic,
+                   const GeoParams &gp, double volumeBound) {
+    const int d = P.dim;
+    std::map<std::pair<double, Idx3>, double> geo;
+    GeoSums sums;
+    sums.d = d;
+    std::vector<std::pair<double, std::array<double, 3>>> steps;
+    double stepNorm = 0;
+    double indexAvg[3] = {0, 0, 0}, indexBase[3] = {0, 0, 0};
+    auto value_at = [&](Idx3 ind) {
+        P.periodize(ind.data());
+        return S.values[P.ravel(ind.data())];
+    };
+    auto push_point = [&](const double *q) {
+        std::array<double, 3> pt{0, 0, 0};
+        for (int i = 0; i < d; ++i) pt[i] = q[i] + 0.5;
+        geodesic.push_back(pt);
+    };
+    {
+        const std::array<double, 3> tip = geodesic.back();
+        Idx3 indexRef{0, 0, 0};
+        double pointRef[3] = {0, 0, 0};
+        for (int j = 0; j < d; ++j) {
+            indexRef[j] = (int)std::floor(tip[j]);
+            pointRef[j] = indexRef[j] + 0.5;
+        }
+        for (int i = 0; i < (1 << d); ++i) {
+            double weight = 1;
+            Idx3 index = indexRef;
+            for (int j = 0; j < d; ++j) {
+                const double diff = tip[j] - pointRef[j];
+                if (i & (1 << j)) {
+                    index[j] += diff > 0 ? 1 : -1;
+                    weight *= std::abs(diff);
+                } else {
+                    weight *= 1. - std::abs(diff);
+                }
+            }
+            Idx3 per = index;
+            if (!P.periodize(per.data())) continue;
+            geo[{S.values[P.ravel(per.data())], index}] = weight;
+            sums.update(weight, index);
+        }
+        if (sums.weightSum == 0) return false;
+        sums.mean(indexAvg);
+        std::copy(indexAvg, indexAvg + 3, indexBase);
+    }
+    const double weightSumLowerBound = 1e-5;
+    int off[2 * kMaxDiff][3];
+    double fw[2 * kMaxDiff];
+    while (!geo.empty() && sums.weightSum > weightSumLowerBound) {
+        const auto it = std::prev(geo.end());
+        const double value = it->first.first;
+        const Idx3 index = it->first.second;
+        const double weight = it->second;
+        geo.erase(it);
+        if (weight == 0) continue;
+        sums.update(-weight, index);
+        Idx3 per = index;
+        P.periodize(per.data());
+        const int nf = discrete_flow(P, S, P.ravel(per.data()), off, fw);
+        double wSum = 0;
+        for (int k = 0; k < nf; ++k) wSum += fw[k];
+        if (wSum == 0) continue;
+        for (int k = 0; k < nf; ++k) {
+            const double w = weight * fw[k] / wSum;
+            Idx3 ind = index;
+            for (int c = 0; c < d; ++c) ind[c] += off[k][c];
+            const double val = value_at(ind);
+            const auto jt = geo.find({val, ind});
+            if (jt == geo.end()) {
+                if (w < sums.weightSum * gp.threshold) continue;
+                geo[{val, ind}] = w;
+            } else {
+                jt->second += w;
+            }
+            sums.update(w, ind);
+        }
+        if (sums.weightSum <= 0) {
+            push_point(indexAvg);
+            return false;
+        }
+        double oldAvg[3];
+        std::copy(indexAvg, indexAvg + 3, oldAvg);
+        sums.mean(indexAvg);
+        std::array<double, 3> step{0, 0, 0};
+        for (int i = 0; i < d; ++i) step[i] = indexAvg[i] - oldAvg[i];
+        steps.push_back({value, step});
+        double sq = 0;
+        for (int i = 0; i < d; ++i) sq += step[i] * step[i];
+        stepNorm += std::sqrt(sq);
+        if (geo.size() == 1) {
+            if (nf == 0) break;
+            push_point(indexAvg);
+            std::copy(indexAvg, indexAvg + 3, indexBase);
+            stepNorm = 0;
+            steps.clear();
+            continue;
+        }
+        if (stepNorm < gp.step && geo.size() > 1) continue;
+        const double valInc = steps.front().first - value;
+        if (valInc <= 0) continue;
+        const double delta = valInc / d;
+        double p[3];
+        std::copy(indexBase, indexBase + 3, p);
+        for (const auto &vs : steps) {
+            const double a = (vs.first - value) + delta;
+            const double rb = 1. / (valInc + delta);
+            for (int i = 0; i < d; ++i) p[i] += (vs.second[i] * a) * rb;
+        }
+        push_point(p);
+        std::copy(indexAvg, indexAvg + 3, indexBase);
+        stepNorm = 0;
+        steps.clear();
+        double var[6], m[3];
+        const double b = 1. / sums.weightSum;
+        for (int k = 0; k < d * (d + 1) / 2; ++k) var[k] = sums.cov[k] * b;
+        sums.mean(m);
+        for (int i = 0; i < d; ++i)
+            for (int j = 0; j <= i; ++j) {
+                var[i * (i + 1) / 2 + j] -= m[i] * m[j];
+                var[i * (i + 1) / 2 + j] += (i == j) ? 1. : 0.;
+            }
+        if (det_sym(d, var) > volumeBound * volumeBound) {
+            push_point(indexAvg);
+            return true;
+        }
+    }
+    return false;
+}
+
+// GeodesicDiscreteSolver::Run (GeodesicDiscreteSolver.hxx:41-54): restarts from the last point
+// while the mass spreads too much, the volume bound growing 1.3x from the 5th restart; gives up
+// after 50.  Returns true on failure.
+bool geodesic_run(const Problem &P, const FMState &S, std::vector<std::array<double, 3>> &geodesic,
+                  const GeoParams &gp) {
+    const double vb0 = gp.volumeBound > 0 ? gp.volumeBound : 5. * std::pow(1.3, P.dim);
+    int nRestarts = 0;
+    double vb = vb0;
+    while (nRestarts < 50) {
+        if (!geodesic_pass(P, S, geodesic, gp, vb)) break;
+        ++nRestarts;
+        if (nRestarts >= 5) vb *= 1.3;
+    }
+    return nRestarts == 50;
+}
+
 // Seeds: ADim then floor, out-of-range seeds dropped, first value kept for duplicates
 // (StaticFactoring.hxx:242-393, BaseGrid.h:61-67).
 std::vector<std::pair<long, double>> make_seeds(const Problem &P, int nseeds, const double *seeds,
@@ -721,6 +925,77 @@ int oracle_solve_backward(int model, const int *dims, const double *gridScales, 
         }
         *nseedSens = n;
         return 0;
+    } catch (const OracleError &e) {
+        g_err = e.msg;
+        return -1;
+    }
+}
+
+static int geodesics_impl(Problem &P, FMState &S, int ntips, const double *tips, double step, double threshold,
+                          double volumeBound, double *points, long max_points, int64_t *lengths, long *npoints) {
+    GeoParams gp;
+    gp.step = step;
+    gp.threshold = threshold;
+    gp.volumeBound = volumeBound;
+    long np = 0;
+    for (int t = 0; t < ntips; ++t) {
+        std::vector<std::array<double, 3>> geodesic;
+        std::array<double, 3> tip{0, 0, 0};
+        P.adim(tips + t * P.dim, tip.data());
+        geodesic.push_back(tip);
+        geodesic_run(P, S, geodesic, gp);
+        lengths[t] = (int64_t)geodesic.size();
+        for (const auto &q : geodesic) {
+            if (np < max_points)
+                for (int c = 0; c < P.dim; ++c) {
+                    // ReDim (HFM_ParamDefault.hxx:86-89 and :137-148)
+                    const double sc = (P.shared && c == 2) ? P.dependScale : P.h[c];
+                    points[np * P.dim + c] = q[c] * sc + P.origin[c];
+                }
+            ++np;
+        }
+    }
+    *npoints = np;
+    return 0;
+}
+
+int oracle_geodesics(int model, const int *dims, const double *gridScales, const double *origin,
+                     const double *metric, int metric_is_field, int dual, double xi, double eps,
+                     const double *cost, int cost_is_field, int nseeds, const double *seeds,
+                     const double *seedValues, int ntips, const double *tips, double step, double threshold,
+                     double volumeBound, double *points, long max_points, int64_t *lengths, long *npoints) {
+    try {
+        g_err.clear();
+        Problem P;
+        setup_problem(P, model, dims, gridScales, origin, metric, metric_is_field, dual, xi, eps, cost,
+                      cost_is_field);
+        const auto sd = make_seeds(P, nseeds, seeds, seedValues);
+        FMState S;
+        fast_marching(P, sd, S, nullptr, nullptr);
+        return geodesics_impl(P, S, ntips, tips, step, threshold, volumeBound, points, max_points, lengths,
+                              npoints);
+    } catch (const OracleError &e) {
+        g_err = e.msg;
+        return -1;
+    }
+}
+
+int oracle_geodesics_given(int model, const int *dims, const double *gridScales, const double *origin,
+                           const double *metric, int metric_is_field, int dual, double xi, double eps,
+                           const double *cost, int cost_is_field, const double *values,
+                           const uint64_t *activeNeighs, int ntips, const double *tips, double step,
+                           double threshold, double volumeBound, double *points, long max_points,
+                           int64_t *lengths, long *npoints) {
+    try {
+        g_err.clear();
+        Problem P;
+        setup_problem(P, model, dims, gridScales, origin, metric, metric_is_field, dual, xi, eps, cost,
+                      cost_is_field);
+        FMState S;
+        S.values.assign(values, values + P.N);
+        S.active.assign(activeNeighs, activeNeighs + P.N);
+        return geodesics_impl(P, S, ntips, tips, step, threshold, volumeBound, points, max_points, lengths,
+                              npoints);
     } catch (const OracleError &e) {
         g_err = e.msg;
         return -1;

@@ -82,6 +82,26 @@ int oracle_solve_backward(int model, const int *dims, const double *gridScales, 
                           int64_t *seedSensIndex, double *seedSensWeight, int nseeds_max,
                           int *nseedSens);
 
+/* Same as oracle_solve, then the discrete geodesic solver (GeodesicDiscreteSolver.hxx) from
+ * each tip (ntips x dim physical points) with geodesicStep `step`, geodesicWeightThreshold
+ * `threshold`, geodesicVolumeBound `volumeBound` (<= 0: 5 * 1.3^dim).  Points are written
+ * ReDim'd and concatenated (up to max_points points of dim coordinates); lengths[t] = number of
+ * points of tip t's geodesic; *npoints = total. */
+int oracle_geodesics(int model, const int *dims, const double *gridScales, const double *origin,
+                     const double *metric, int metric_is_field, int dual, double xi, double eps,
+                     const double *cost, int cost_is_field, int nseeds, const double *seeds,
+                     const double *seedValues, int ntips, const double *tips, double step, double threshold,
+                     double volumeBound, double *points, long max_points, int64_t *lengths, long *npoints);
+
+/* The discrete geodesic solver alone, on given arrival times (values, N) and active sets
+ * (activeNeighs, N bitsets) instead of the oracle's own fast marching. */
+int oracle_geodesics_given(int model, const int *dims, const double *gridScales, const double *origin,
+                           const double *metric, int metric_is_field, int dual, double xi, double eps,
+                           const double *cost, int cost_is_field, const double *values,
+                           const uint64_t *activeNeighs, int ntips, const double *tips, double step,
+                           double threshold, double volumeBound, double *points, long max_points,
+                           int64_t *lengths, long *npoints);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,6 +14,7 @@ Fixtures (all row-major):
   ties3.npz          Selling offsets of near-tie / degenerate-direction 3x3 tensors
   rsf2_32x32x16.npz  values, activeNeighs, geodesicFlow, activeOffsets, shared stencils (getStencils)
   elastica2_24x24x16.npz  values, activeOffsets, costSensitivity_0, seedSensitivity_0, shared stencils
+  geodesics.npz      geodesicPoints / geodesicLengths from `tips` (discrete solver), four models
   riemann3_128.npz   digests of values at the benchmark size + 4096 seeded sample points
 Inputs are regenerated by hamiltonfastmarching_amd.synthetic; the SHA-256 of each generated
 input is stored so that a test can tell a generator change from a numerical one.
@@ -154,6 +155,34 @@ def curvature_case(name, params, sensitivity=False):
     print(name, "ok")
 
 
+def geodesic_cases():
+    """geodesicPoints / geodesicLengths of the discrete geodesic solver from `tips`, per model."""
+    rng = np.random.Generator(np.random.PCG64(9))
+    cases = {
+        "rsf2": syn.rsf2(32, 16, seed=3),
+        "riemann2": syn.riemann2(64, seed=1),
+        "riemann3": syn.riemann3(24, seed=2),
+        "elastica2": syn.elastica2(24, 16, seed=4),
+    }
+    out = {}
+    for name, p in cases.items():
+        q = {k: v for k, v in p.items() if k not in ("tips", "inspectSensitivity", "inspectSensitivityWeights",
+                                                     "inspectSensitivityLengths")}
+        if "tips" in p:
+            tips = np.asarray(p["tips"], float)
+        elif "inspectSensitivity" in p:
+            tips = np.asarray(p["inspectSensitivity"], float)
+        else:
+            tips = rng.uniform(0.05, 0.95, size=(8, len(p["dims"])))
+        r = oracle.run_reference(dict(q, tips=tips))
+        assert r["retcode"] == 0, r["log"]
+        out[name + "_tips"] = tips
+        out[name + "_points"] = r["geodesicPoints"]
+        out[name + "_lengths"] = r["geodesicLengths"]
+    np.savez_compressed(os.path.join(HERE, "geodesics.npz"), **out)
+    print("geodesics ok")
+
+
 def big_case():
     p = syn.riemann3(128, seed=0)
     r = oracle.run_reference(p)
@@ -177,6 +206,7 @@ def main():
     ties_case()
     curvature_case("rsf2_32x32x16", syn.rsf2(32, 16, seed=3))
     curvature_case("elastica2_24x24x16", syn.elastica2(24, 16, seed=4), sensitivity=True)
+    geodesic_cases()
     if "--big" in sys.argv:
         big_case()
 

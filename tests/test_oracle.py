@@ -299,3 +299,88 @@ def test_reference_and_port_agree_where_reference_built():
     mine = oracle.solve(p, active=True)
     assert np.array_equal(r["values"], mine["values"])
     assert np.array_equal(r["activeNeighs"].astype(np.uint64), mine["activeNeighs"])
+
+
+# ------------------------------------------------------------------------------------------
+# Geodesic backtracking (discrete solver)
+# ------------------------------------------------------------------------------------------
+GEO_CASES = {
+    "rsf2": lambda: syn.rsf2(32, 16, seed=3),
+    "riemann2": lambda: syn.riemann2(64, seed=1),
+    "riemann3": lambda: syn.riemann3(24, seed=2),
+    "elastica2": lambda: syn.elastica2(24, 16, seed=4),
+}
+
+
+def _strip(p):
+    return {k: v for k, v in p.items() if k not in ("tips", "inspectSensitivity", "inspectSensitivityWeights",
+                                                    "inspectSensitivityLengths")}
+
+
+@pytest.mark.parametrize("name", list(GEO_CASES))
+def test_geodesics_golden(name):
+    """geodesicPoints / geodesicLengths bit-exact against the reference's discrete solver."""
+    g = load("geodesics")
+    p = _strip(GEO_CASES[name]())
+    pts, lens = oracle.geodesics(p, g[name + "_tips"])
+    assert np.array_equal(lens, g[name + "_lengths"].astype(np.int64))
+    assert np.array_equal(pts, g[name + "_points"])
+    # the same solver on given values / active sets (the entry the GPU kernel is checked with)
+    o = oracle.solve(p, active=True)
+    pts2, lens2 = oracle.geodesics_given(p, o["values"], o["activeNeighs"], g[name + "_tips"])
+    assert np.array_equal(pts2, pts) and np.array_equal(lens2, lens)
+    # planted mistake: another geodesicStep gives another curve
+    pts3, _ = oracle.geodesics(p, g[name + "_tips"], step=0.3)
+    assert pts3.shape != pts.shape or not np.array_equal(pts3, pts)
+
+
+def test_geodesics_isotropic_straight():
+    """Identity metric, seed at the centre: each geodesic runs from its tip to the seed along the
+    straight segment (the discrete mass transport stays within a cell of it)."""
+    n = 64
+    h = 1.0 / n
+    p = {"model": "Riemann2", "dims": np.array([n, n], float), "gridScale": h, "origin": np.zeros(2),
+         "seeds": np.array([[0.5, 0.5]]), "metric": np.array([1.0, 0.0, 1.0]), "arrayOrdering": "RowMajor"}
+    rng = np.random.Generator(np.random.PCG64(3))
+    tips = rng.uniform(0.1, 0.9, size=(12, 2))
+    pts, lens = oracle.geodesics(p, tips)
+    seed = np.array([0.5, 0.5]) + 0.5 * h  # centre of the seed cell
+    start = 0
+    for t, L in zip(tips, lens):
+        c = pts[start:start + L]
+        start += L
+        assert np.max(np.abs(c[0] - t)) < 1e-12
+        assert np.linalg.norm(c[-1] - seed) < 1.5 * h
+        d = seed - t
+        rel = c - t
+        s = np.clip(rel @ d / (d @ d), 0, 1)
+        dist = np.linalg.norm(rel - s[:, None] * d, axis=1)
+        assert dist.max() < 1.0 * h, (t, dist.max() / h)
+
+
+@pytest.mark.parametrize("name", ["riemann2", "riemann3", "rsf2"])
+def test_geodesics_descend_arrival_times(name):
+    """Along each geodesic the arrival time at the nearest cell falls from the tip towards a seed
+    (cell-rounded values of the averaged positions fluctuate, so only the trend is checked)."""
+    g = load("geodesics")
+    p = _strip(GEO_CASES[name]())
+    u = oracle.solve(p)["values"]
+    dims = np.array(u.shape)
+    h = np.array([p["gridScale"]] * len(dims), float)
+    origin = np.zeros(len(dims))
+    if name == "rsf2":
+        h[2] = 2 * np.pi / dims[2]
+        origin[2] = -h[2] / 2
+    pts, lens = g[name + "_points"], g[name + "_lengths"].astype(int)
+    start = 0
+    for L in lens:
+        c = pts[start:start + L]
+        start += L
+        idx = np.floor((c - origin) / h).astype(int)
+        if name == "rsf2":
+            idx[:, 2] %= dims[2]
+        idx = np.clip(idx, 0, dims - 1)
+        val = u[tuple(idx.T)]
+        assert val[-1] < 0.5 * val[0]
+        if L >= 8:
+            assert np.corrcoef(np.arange(L), val)[0, 1] < -0.5
