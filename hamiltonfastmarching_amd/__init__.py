@@ -239,6 +239,25 @@ class Plan:
         self._check(self._L.hfm_plan_active_offsets(self._h, ctypes.c_void_p(out.data_ptr()), _stream_ptr(torch)))
         return out
 
+    def geodesics(self, tips, step=0.25, threshold=0.001, volume_bound=0.0):
+        """Geodesics backtracked from `tips` (ntips x dim physical points) by the discrete solver
+        (GeodesicDiscreteSolver.hxx): (geodesicPoints (M x dim), geodesicLengths (ntips))."""
+        torch = _torch()
+        t = np.ascontiguousarray(np.asarray(tips, np.float64).reshape(-1, self.dim))
+        lengths = np.zeros(len(t), np.int64)
+        npts = ctypes.c_int64(0)
+        nfail = ctypes.c_int(0)
+        cap = 1 << 14
+        while True:
+            pts = np.zeros((cap, self.dim))
+            self._check(self._L.hfm_plan_geodesics(self._h, len(t), _dptr(t), float(step), float(threshold),
+                                                   float(volume_bound), _dptr(pts), cap,
+                                                   lengths.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                                   ctypes.byref(npts), ctypes.byref(nfail), _stream_ptr(torch)))
+            if npts.value <= cap:
+                return pts[:npts.value].copy(), lengths.copy()
+            cap = npts.value
+
     def adjoint(self, targets, weights=None, max_seeds=4096):
         torch = _torch()
         t = np.ascontiguousarray(np.asarray(targets, np.float64).reshape(-1, self.dim))

@@ -8,6 +8,7 @@
 //                 (Riemannian.h:46-61, Curvature2.h:104-108,219-224, HFM_StencilShare.hxx:61-71)
 //   SetupSolver   MaxStencilWidth, seeds / seedValues                      (:354-410)
 //   RunSolver     FMCPUTime (wall-clock seconds of the GPU solve here)     (:422-450)
+//   ExtractGeodesics  tips -> geodesicPoints, geodesicLengths, GeodesicCPUTime (:452-523)
 //   ExportData    values, activeNeighs, geodesicFlow, activeOffsets         (:526-579)
 //   Finally       inspectSensitivity -> costSensitivity_<i>, seedSensitivity_<i>
 //                 (Headers/ExtraAlgorithms/FirstVariation.h:60-102)
@@ -357,6 +358,41 @@ void run_impl(hfm_io *io) {
         std::ostringstream os;
         os << "Fast marching solver completed in " << (t3 - t2) << " s.\n";
         io->log += os.str();
+    }
+    // Run_ExtractGeodesics (HFMInterface.hxx:452-523): tips -> geodesicPoints, geodesicLengths
+    if (io->has("tips")) {
+        const double tg = wall();
+        const std::string solver = io->str("geodesicSolver", "Discrete");
+        if (solver != "Discrete")
+            throw IOError{"Error : geodesic solver " + solver + " is not available (only 'Discrete')."};
+        const std::vector<double> tips = io->vec("tips");
+        if (tips.size() % dim) throw IOError{"IO error : tips has the wrong size"};
+        const int ntips = (int)(tips.size() / dim);
+        const double step = io->scalar("geodesicStep", 0.25);
+        const double thr = io->scalar("geodesicWeightThreshold", 0.001);
+        const double vb = io->scalar("geodesicVolumeBound", 5. * std::pow(1.3, dim));
+        std::vector<int64_t> lens(ntips);
+        std::vector<double> pts((size_t)4096 * dim);
+        int64_t npts = 0;
+        int nfail = 0;
+        check(hfm_plan_geodesics(plan, ntips, tips.data(), step, thr, vb, pts.data(), 4096, lens.data(), &npts,
+                                 &nfail, nullptr));
+        if (npts > 4096) {
+            pts.resize((size_t)npts * dim);
+            check(hfm_plan_geodesics(plan, ntips, tips.data(), step, thr, vb, pts.data(), npts, lens.data(), &npts,
+                                     &nfail, nullptr));
+        }
+        pts.resize((size_t)npts * dim);
+        if (nfail && io->verbosity >= 1) io->log += "Geodesic extraction failed for " + std::to_string(nfail) + " tip(s)\n";
+        Field fp;
+        fp.dims = {npts, (int64_t)dim};
+        fp.data = std::move(pts);
+        io->set_computed("geodesicPoints", std::move(fp));
+        Field fl;
+        fl.dims = {(int64_t)ntips};
+        for (int64_t l : lens) fl.data.push_back((double)l);
+        io->set_computed("geodesicLengths", std::move(fl));
+        io->set_scalar("GeodesicCPUTime", wall() - tg);
     }
     // Run_ExportData
     const bool exportValues = io->scalar("exportValues", 0.) != 0.;

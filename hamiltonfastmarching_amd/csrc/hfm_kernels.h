@@ -56,6 +56,21 @@ template <typename T> struct FimArgs {
     unsigned grid;
 };
 
+// Geodesic backtracking (hfm_geodesic.hip).  Tips are ADim'd (grid-index units); points are
+// written ReDim'd, max_points per tip, tip t at points[t * max_points * dim ...].
+struct GeoParamsHost {
+    double step = 0.25, threshold = 0.001, volumeBound = 0.;
+    double scale[3] = {1., 1., 1.}, origin[3] = {0., 0., 0.};
+    long long max_points = 0;  // per tip
+    long long max_iter = 0;    // per pass
+};
+enum GeoStatusHost { GEO_STATUS_OK = 0, GEO_STATUS_FAILED = 1, GEO_STATUS_SET_OVERFLOW = 2,
+                     GEO_STATUS_STEPS_OVERFLOW = 3, GEO_STATUS_NO_END = 4 };
+template <typename T>
+hipError_t launch_geodesics(int model, const Geom &g, const StencilData<T> &sd, const T *u,
+                            const unsigned long long *fm, const GeoParamsHost &h, int ntips, const double *tips,
+                            double *points, long long *lengths, int *status, hipStream_t s);
+
 template <typename T>
 hipError_t launch_riemann_stencil(const Geom &g, const double *metric, int is_field, int dual, const InvHH &ih,
                                   uint32_t *offs, T *w, long long np, int *status, RedSlot *red, hipStream_t s);

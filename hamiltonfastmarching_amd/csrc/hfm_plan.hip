@@ -730,6 +730,107 @@ int hfm_plan_active_offsets(hfm_plan *p, double *d_offsets, void *stream) {
 
 int hfm_plan_n_active_slots(const hfm_plan *p) { return p ? p->ndiff : 0; }
 
+int hfm_plan_geodesics(hfm_plan *p, int ntips, const double *tips, double step, double threshold,
+                       double volumeBound, double *points, int64_t max_points, int64_t *lengths, int64_t *npoints,
+                       int *nfailed, void *stream) {
+    if (!p || (ntips > 0 && (!tips || !lengths)) || !npoints) {
+        set_error("hfm_plan_geodesics: null argument");
+        return 1;
+    }
+    if (!p->solved) {
+        set_error("hfm_plan_geodesics: no solve has run");
+        return 1;
+    }
+    *npoints = 0;
+    if (nfailed) *nfailed = 0;
+    if (ntips <= 0) return 0;
+    try {
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const int dim = p->g.dim;
+        // ADim (HFM_ParamDefault.hxx:70-72 and :107-119), the same as the seeds' placement
+        std::vector<double> tad((size_t)ntips * dim);
+        GeoParamsHost h;
+        for (int i = 0; i < dim; ++i) {
+            h.scale[i] = (p->shared && i == 2) ? p->dependScale : p->h[i];
+            h.origin[i] = p->origin[i];
+        }
+        for (int t = 0; t < ntips; ++t)
+            for (int i = 0; i < dim; ++i) tad[(size_t)t * dim + i] = (tips[(size_t)t * dim + i] - h.origin[i]) / h.scale[i];
+        h.step = step;
+        h.threshold = threshold;
+        h.volumeBound = volumeBound > 0 ? volumeBound : 5. * std::pow(1.3, dim);
+        h.max_iter = 4 * p->np + 4096;
+        long long cap = 1024;
+        for (int i = 0; i < dim; ++i) cap += 8LL * p->g.n[i];
+        double *d_tips = nullptr, *d_pts = nullptr;
+        long long *d_len = nullptr;
+        int *d_st = nullptr;
+        std::vector<long long> hlen(ntips);
+        std::vector<int> hst(ntips);
+        HIPCHECK(hipMallocAsync((void **)&d_tips, tad.size() * sizeof(double), s));
+        HIPCHECK(hipMallocAsync((void **)&d_len, ntips * sizeof(long long), s));
+        HIPCHECK(hipMallocAsync((void **)&d_st, ntips * sizeof(int), s));
+        HIPCHECK(hipMemcpyAsync(d_tips, tad.data(), tad.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        for (int attempt = 0;; ++attempt) {
+            h.max_points = cap;
+            HIPCHECK(hipMallocAsync((void **)&d_pts, (size_t)ntips * cap * dim * sizeof(double), s));
+            const hipError_t e =
+                p->precision == 32
+                    ? launch_geodesics<float>(p->model, p->g, p->sd<float>(), static_cast<const float *>(p->u),
+                                              p->fixedmask, h, ntips, d_tips, d_pts, d_len, d_st, s)
+                    : launch_geodesics<double>(p->model, p->g, p->sd<double>(), static_cast<const double *>(p->u),
+                                               p->fixedmask, h, ntips, d_tips, d_pts, d_len, d_st, s);
+            HIPCHECK(e);
+            HIPCHECK(hipMemcpyAsync(hlen.data(), d_len, ntips * sizeof(long long), hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipMemcpyAsync(hst.data(), d_st, ntips * sizeof(int), hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            long long mx = 0;
+            for (long long l : hlen) mx = std::max(mx, l);
+            if (mx <= cap || attempt >= 2) break;
+            HIPCHECK(hipFreeAsync(d_pts, s));
+            cap = mx;
+        }
+        int failed = 0;
+        std::string err;
+        for (int t = 0; t < ntips; ++t) {
+            if (hst[t] == GEO_STATUS_FAILED) ++failed;
+            else if (hst[t] == GEO_STATUS_SET_OVERFLOW)
+                err = "geodesic extraction: the transported mass spread over more than 256 grid points";
+            else if (hst[t] == GEO_STATUS_STEPS_OVERFLOW)
+                err = "geodesic extraction: more than 256 steps between two geodesic points";
+            else if (hst[t] == GEO_STATUS_NO_END)
+                err = "geodesic extraction: mass transport did not end";
+            else if (hlen[t] > cap)
+                err = "geodesic extraction: geodesic longer than the point buffer";
+        }
+        long long total = 0;
+        if (err.empty()) {
+            for (int t = 0; t < ntips; ++t) {
+                lengths[t] = hlen[t];
+                const long long n = hlen[t];
+                if (points && total < max_points) {
+                    const long long take = std::min<long long>(n, max_points - total);
+                    HIPCHECK(hipMemcpyAsync(points + (size_t)total * dim, d_pts + (size_t)t * cap * dim,
+                                            (size_t)take * dim * sizeof(double), hipMemcpyDeviceToHost, s));
+                }
+                total += n;
+            }
+        }
+        HIPCHECK(hipFreeAsync(d_pts, s));
+        HIPCHECK(hipFreeAsync(d_tips, s));
+        HIPCHECK(hipFreeAsync(d_len, s));
+        HIPCHECK(hipFreeAsync(d_st, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        if (!err.empty()) throw HipFail{err};
+        *npoints = total;
+        if (nfailed) *nfailed = failed;
+    } catch (const HipFail &e) {
+        set_error(e.msg);
+        return 1;
+    }
+    return 0;
+}
+
 int hfm_plan_get_stats(const hfm_plan *p, hfm_plan_stats *st) {
     if (!p || !st) {
         set_error("hfm_plan_get_stats: null argument");

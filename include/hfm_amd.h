@@ -135,6 +135,18 @@ int hfm_plan_flow(hfm_plan *plan, double *d_flow, double *d_active, void *stream
 int hfm_plan_active_offsets(hfm_plan *plan, double *d_offsets, void *stream);
 int hfm_plan_n_active_slots(const hfm_plan *plan);
 
+/* Geodesic backtracking from tips (GeodesicDiscreteSolver.hxx:17-231, driver
+ * HFMInterface.hxx:452-487): the discrete solver, one wavefront per tip, on the last solve.
+ *   tips: ntips x dim host physical points; step / threshold / volumeBound: geodesicStep
+ *   (0.25), geodesicWeightThreshold (0.001), geodesicVolumeBound (<= 0: 5 * 1.3^dim).
+ *   points: host, receives up to max_points ReDim'd points (x dim), the geodesics concatenated;
+ *   lengths: host, ntips point counts (geodesicLengths); *npoints = total;
+ *   *nfailed (may be NULL) = tips whose extraction gave up after 50 restarts (their points are
+ *   still returned, as the reference does). */
+int hfm_plan_geodesics(hfm_plan *plan, int ntips, const double *tips, double step, double threshold,
+                       double volumeBound, double *points, int64_t max_points, int64_t *lengths, int64_t *npoints,
+                       int *nfailed, void *stream);
+
 /* Backward differentiation (FirstVariation::BackwardVariation, FirstVariation.h:341-385):
  * gradient of sum_t weights[t] * u(targets[t]) w.r.t. the cost field (row-major FP64 device
  * array costSensitivity; zero outside the targets' dependency cone).  targets are host
