@@ -1,6 +1,6 @@
 """Summaries of rocprofv3 result databases (rocpd SQLite) as short text tables.
 
-    python -m hamiltonfastmarching_amd.profiling trace  <results.db>
+    python -m hamiltonfastmarching_amd.profiling trace  <results.db> [--filter SUBSTRING]
     python -m hamiltonfastmarching_amd.profiling counters <fetch_results.db> <write_results.db> \
         [--kernel k_fim] [--copy-bytes N]
 
@@ -18,11 +18,13 @@ import sys
 import numpy as np
 
 
-def kernel_trace(db: str):
+def kernel_trace(db: str, substring: str = ""):
     c = sqlite3.connect(db)
     rows = c.execute("select name, duration from kernels").fetchall()
     by = {}
     for name, d in rows:
+        if substring and substring not in name:
+            continue
         by.setdefault(name, []).append(d)
     out = []
     for name, ds in by.items():
@@ -71,7 +73,8 @@ def counters(fetch_db: str, write_db: str, kernel: str = "k_fim", copy_bytes: fl
 
 def main(argv):
     if len(argv) >= 2 and argv[0] == "trace":
-        print(kernel_trace(argv[1]))
+        sub = argv[argv.index("--filter") + 1] if "--filter" in argv else ""
+        print(kernel_trace(argv[1], sub))
     elif len(argv) >= 3 and argv[0] == "counters":
         kernel = "k_fim"
         copy_bytes = float(1 << 30)

@@ -290,3 +290,29 @@ def run_sequential(requests, collect_values: bool = False):
     if collect_values:
         out["values"] = vals
     return out
+
+
+def main(argv=None):
+    """python -m hamiltonfastmarching_amd.serving [--requests N] [--concurrency C] [--rounds R]
+    [--burst] [--seed S]: the closed-loop sustained rate (or a burst) on the request mix, as JSON."""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--requests", type=int, default=200)
+    ap.add_argument("--concurrency", type=int, default=128)
+    ap.add_argument("--rounds", type=int, default=3)
+    ap.add_argument("--burst", action="store_true")
+    ap.add_argument("--seed", type=int, default=11)
+    a = ap.parse_args(argv)
+    reqs = make_requests(a.requests, seed=a.seed)
+    srv = Server(hbm_budget_bytes=64e9, max_resident=256)
+    try:
+        run_load(srv, reqs[:16], qps=0)
+        out = run_load(srv, reqs, qps=0) if a.burst else run_closed_loop(srv, reqs, a.concurrency, a.rounds)
+    finally:
+        srv.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
