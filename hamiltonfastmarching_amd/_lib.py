@@ -58,6 +58,9 @@ SIGNATURES = [
     ("hfm_io_get_array", c_int, [c_void_p, ctypes.c_char_p, P(c_double), c_size_t]),
     ("hfm_io_computed_keys", ctypes.c_longlong, [c_void_p, ctypes.c_char_p, c_size_t]),
     ("hfm_io_last_error", ctypes.c_char_p, [c_void_p]),
+    ("hfm_io_messages", ctypes.c_char_p, [c_void_p]),
+    ("hfm_io_read_files", c_int, [c_void_p, ctypes.c_char_p]),
+    ("hfm_io_write_files", c_int, [c_void_p, ctypes.c_char_p]),
     ("hfm_run", c_int, [c_void_p]),
     ("hfm_plan_create", c_int, [P(c_void_p), ctypes.c_char_p, P(c_int64), P(c_double), P(c_double), c_int,
                                 c_double, c_double]),

@@ -11,6 +11,7 @@ Every HIP translation unit is compiled with
 from __future__ import annotations
 
 import os
+import shutil
 import subprocess
 import sys
 from concurrent.futures import ThreadPoolExecutor
@@ -57,9 +58,33 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    build_file_executables()
     if verbose:
         print(f"built {LIB}")
     return LIB
+
+
+FILE_MODELS = ["Riemann2", "Riemann3", "ReedsSheppForward2", "Elastica2"]
+
+
+def build_file_executables() -> str:
+    """The FileHFM-compatible executable (host code over libhfm_amd.so): _lib/FileHFM_AMD, plus
+    one copy per model named FileHFM_<Model> (the name fixes the model, as the reference's
+    per-model executables do)."""
+    exe = os.path.join(OUT_DIR, "FileHFM_AMD")
+    tmp = exe + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", f"-I{INCLUDE}", os.path.join(CSRC, "hfm_file_main.cpp"), "-o", tmp,
+           f"-L{OUT_DIR}", "-lhfm_amd", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("FileHFM executable failed to build:\n" + r.stdout + r.stderr)
+    os.replace(tmp, exe)
+    for m in FILE_MODELS:
+        dst = os.path.join(OUT_DIR, f"FileHFM_{m}")
+        shutil.copyfile(exe, dst + ".tmp")
+        os.chmod(dst + ".tmp", 0o755)
+        os.replace(dst + ".tmp", dst)
+    return exe
 
 
 if __name__ == "__main__":

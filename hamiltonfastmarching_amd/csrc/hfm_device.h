@@ -218,11 +218,6 @@ template <int ModelT, typename T> struct LaneStencil {
     }
 };
 
-// Local solve used by the update kernel.  FP64: exactly local_solve (the reference's order of
-// operations, so converged values reproduce the reference's FP64 values).  FP32: the same
-// candidates in the same order, but the test "next v below the current root" is evaluated
-// without the root as f(v) < 1, f(t) = sum_accepted w (t - v_i)^2 (f is increasing above the
-// accepted values and f(root) = 1), so the square root and division run once per solve.
 // Compare-exchange networks that sort NC values (checked exhaustively with the 0-1 principle and
 // all permutations): the optimal 3-, 5- and 12-comparator networks for 3, 4 and 6 inputs, and
 // odd-even transposition for other sizes up to 8.
@@ -262,6 +257,11 @@ template <> struct SortNet<6> {
     }
 };
 
+// Local solve used by the update kernel.  FP64: exactly local_solve (the reference's order of
+// operations, so converged values reproduce the reference's FP64 values).  FP32: the same
+// candidates in the same order, but the test "next v below the current root" is evaluated
+// without the root as f(v) < 1, f(t) = sum_accepted w (t - v_i)^2 (f is increasing above the
+// accepted values and f(root) = 1), so the square root and division run once per solve.
 template <int NC, typename T> __device__ inline T local_solve_update(const T (&v)[NC], const T (&w)[NC]) {
     uint32_t used;
     return local_solve<NC, T>(v, w, used);

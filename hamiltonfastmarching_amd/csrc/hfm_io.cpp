@@ -20,6 +20,8 @@
 
 #include <chrono>
 #include <cmath>
+#include <limits>
+#include <fstream>
 #include <cstring>
 #include <map>
 #include <set>
@@ -633,6 +635,125 @@ long long hfm_io_computed_keys(hfm_io *io, char *buf, size_t bufsize) {
 }
 
 const char *hfm_io_last_error(const hfm_io *io) { return io ? io->err.c_str() : "null io"; }
+
+const char *hfm_io_messages(const hfm_io *io) { return io ? io->log.c_str() : ""; }
+
+// FileHFM wire format (JMM_CPPLibs/JMM_CPPLibs/Output/FileIO.hxx:7-93): <prefix>_Format.txt lists,
+// per key, the key, the number of dims (-1: a string on the next line, 0: a scalar), one dim per
+// line and a blank line; <prefix>_Data.dat holds every numerical field's FP64 values in order.
+int hfm_io_read_files(hfm_io *io, const char *prefix) {
+    if (!io || !prefix) return 1;
+    io->err.clear();
+    try {
+        const std::string pre(prefix);
+        std::ifstream ifs(pre + "_Format.txt");
+        if (!ifs.is_open())
+            throw IOError{"FileIO error : Input format file could not be opened " + pre + "_Format.txt."};
+        auto next_line = [&] { ifs.ignore(std::numeric_limits<std::streamsize>::max(), ifs.widen('\n')); };
+        struct Pending {
+            std::string key;
+            Field f;
+            size_t begin = 0, size = 0;
+        };
+        std::vector<Pending> items;
+        size_t pos = 0;
+        while (!ifs.eof()) {
+            std::string key;
+            ifs >> key;
+            next_line();
+            if (key.empty()) {
+                if (ifs.eof()) break;
+                throw IOError{"FileIO Error empty field name"};
+            }
+            if (ifs.eof() || ifs.fail()) throw IOError{"FileIO error : invalid input format file."};
+            Pending it;
+            it.key = key;
+            long long ndims = 0;
+            ifs >> ndims;
+            next_line();
+            if (ifs.eof() || ifs.fail() || ndims < -1)
+                throw IOError{"FileIO error : invalid dimension value (" + std::to_string(ndims) + ") for key " + key +
+                              "in input format file."};
+            if (ndims == -1) {
+                it.f.is_string = true;
+                ifs >> it.f.str;
+                next_line();
+                if (ifs.eof() || ifs.fail())
+                    throw IOError{"FileIO error : invalid string value for key " + key + "in input format file."};
+            } else {
+                size_t size = 1;
+                for (long long i = 0; i < ndims; ++i) {
+                    long long d = 0;
+                    ifs >> d;
+                    next_line();
+                    if (ifs.eof() || ifs.fail() || d < 0)
+                        throw IOError{"FileIO error : invalid dimension for key " + key + " in input format file."};
+                    it.f.dims.push_back(d);
+                    size *= (size_t)d;
+                }
+                it.begin = pos;
+                it.size = size;
+                pos += size;
+            }
+            next_line();
+            items.push_back(std::move(it));
+        }
+        std::vector<double> data(pos);
+        if (pos > 0) {
+            std::ifstream dfs(pre + "_Data.dat", std::ios::binary);
+            if (!dfs.is_open()) throw IOError{"FileIO error : input data file could not be opened " + pre + "_Data.dat"};
+            dfs.seekg(0, dfs.end);
+            if ((size_t)dfs.tellg() != pos * sizeof(double))
+                throw IOError{"FileIO error : input data file has incorrect size."};
+            dfs.seekg(0, dfs.beg);
+            dfs.read(reinterpret_cast<char *>(data.data()), (std::streamsize)(pos * sizeof(double)));
+            if (dfs.gcount() != (std::streamsize)(pos * sizeof(double)))
+                throw IOError{"FileIO error : could not read all data"};
+        }
+        for (auto &it : items) {
+            if (!it.f.is_string) it.f.data.assign(data.begin() + it.begin, data.begin() + it.begin + it.size);
+            io->fields[it.key] = std::move(it.f);
+        }
+    } catch (const IOError &e) {
+        io->err = e.msg;
+        return 1;
+    }
+    return 0;
+}
+
+// Output pair of FileIO::~FileIO (FileIO.hxx:96-132): the computed keys only, in key order.
+int hfm_io_write_files(hfm_io *io, const char *prefix) {
+    if (!io || !prefix) return 1;
+    io->err.clear();
+    const std::string pre(prefix);
+    std::ofstream ofs(pre + "_Format.txt");
+    if (!ofs.is_open()) {
+        io->err = "FileIO error : output format file could not be opened " + pre + "_Format.txt";
+        return 1;
+    }
+    std::vector<double> out;
+    for (const auto &kv : io->fields) {
+        const Field &f = kv.second;
+        if (!f.computed) continue;
+        ofs << kv.first << "\n";
+        if (f.is_string) {
+            ofs << "-1\n" << f.str << "\n\n";
+        } else {
+            ofs << f.dims.size() << "\n";
+            for (int64_t d : f.dims) ofs << d << "\n";
+            ofs << "\n";
+            out.insert(out.end(), f.data.begin(), f.data.end());
+        }
+    }
+    ofs.close();
+    std::ofstream dfs(pre + "_Data.dat", std::ios::binary);
+    if (!dfs.is_open()) {
+        io->err = "FileIO error : output data file could not be opened " + pre + "_Data.dat";
+        return 1;
+    }
+    dfs.write(reinterpret_cast<const char *>(out.data()), (std::streamsize)(out.size() * sizeof(double)));
+    return dfs.good() ? 0 : 1;
+}
 
 int hfm_run(hfm_io *io) {
     if (!io) return 1;

@@ -70,6 +70,17 @@ int hfm_io_get_array(hfm_io *io, const char *key, double *out, size_t count);
 long long hfm_io_computed_keys(hfm_io *io, char *buf, size_t bufsize);
 const char *hfm_io_last_error(const hfm_io *io);
 
+/* The FileHFM wire format (JMM_CPPLibs/JMM_CPPLibs/Output/FileIO.hxx:7-132): read the input
+ * pair <prefix>_Format.txt / <prefix>_Data.dat into the IO object (replaces FileIO::FileIO), and
+ * write its computed keys as an output pair (replaces FileIO::~FileIO).  0 or 1
+ * (hfm_io_last_error: the reference's messages). */
+int hfm_io_read_files(hfm_io *io, const char *prefix);
+int hfm_io_write_files(hfm_io *io, const char *prefix);
+
+/* Messages of the last hfm_run at the requested verbosity (the reference prints them with
+ * Msg(), HFMInterface.hxx:442-447 and elsewhere). */
+const char *hfm_io_messages(const hfm_io *io);
+
 /* Run the solve described by io (Run(IO&), Headers/DispatchAndRun.h:224-353).  Synchronous. */
 int hfm_run(hfm_io *io);
 
