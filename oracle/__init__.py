@@ -58,6 +58,8 @@ def lib():
                                             P(d), P(d), i32, P(d), P(d), P(d), P(d), P(i64), P(d), i32, P(i32)]
         L.oracle_geodesics.argtypes = [i32, P(i32), P(d), P(d), P(d), i32, i32, d, d, P(d), i32, i32, P(d), P(d),
                                        i32, P(d), d, d, d, P(d), ctypes.c_long, P(i64), P(ctypes.c_long)]
+        L.oracle_forward_variation.argtypes = [i32, P(i32), P(d), P(d), P(d), i32, i32, d, d, P(d), i32, i32, P(d),
+                                               P(d), i32, P(d), P(d), P(d), P(d)]
         L.oracle_geodesics_given.argtypes = [i32, P(i32), P(d), P(d), P(d), i32, i32, d, d, P(d), i32, P(d),
                                              P(ctypes.c_uint64), i32, P(d), d, d, d, P(d), ctypes.c_long, P(i64),
                                              P(ctypes.c_long)]
@@ -221,6 +223,32 @@ def geodesics(params, tips, step=0.25, threshold=0.001, volume_bound=-1.0):
         if npts.value <= cap:
             return pts[:npts.value].copy(), lengths.copy()
         cap = npts.value
+
+
+def forward_variation(params, cost_variation=None, seed_value_variation=None):
+    """FM + ForwardVariation: (values, valueVariation of shape dims + (nVar,)).
+    cost_variation: dims + (nVar,); seed_value_variation: (nVar, nseeds)."""
+    (model, dims, dim, N, h, origin, metric, mif, dual, cost, cif, seeds, sv, xi, eps) = _common_args(params)
+    cvar = svar = None
+    nvar = 0
+    if cost_variation is not None:
+        cvar = np.ascontiguousarray(np.asarray(cost_variation, np.float64).reshape(N, -1))
+        nvar = cvar.shape[1]
+    if seed_value_variation is not None:
+        svar = np.ascontiguousarray(np.asarray(seed_value_variation, np.float64).reshape(-1, len(seeds)))
+        nvar = max(nvar, svar.shape[0])
+    values = np.zeros(N)
+    out = np.zeros((N, nvar))
+    dimsa = np.array(dims, np.int32)
+    r = lib().oracle_forward_variation(MODELS[model], _ptr(dimsa, ctypes.c_int), _ptr(h, ctypes.c_double),
+                                       _ptr(origin, ctypes.c_double), _ptr(metric, ctypes.c_double), mif, dual,
+                                       xi, eps, _ptr(cost, ctypes.c_double), cif, len(seeds),
+                                       _ptr(seeds, ctypes.c_double), _ptr(sv, ctypes.c_double), nvar,
+                                       _ptr(cvar, ctypes.c_double), _ptr(svar, ctypes.c_double),
+                                       _ptr(values, ctypes.c_double), _ptr(out, ctypes.c_double))
+    if r:
+        raise RuntimeError(_err())
+    return values.reshape(dims), out.reshape(tuple(dims) + (nvar,))
 
 
 def geodesics_given(params, values, active, tips, step=0.25, threshold=0.001, volume_bound=-1.0):

@@ -591,6 +591,26 @@ void backward_variation(const Problem &P, const FMState &S, const std::vector<st
     }
 }
 
+// FirstVariation::ForwardVariation (FirstVariation.h:389-429): points in increasing
+// (value, index) order; result(x, k) += var(x) * costVariation(x, k) + sum_i w_i result(n_i, k)
+// over the normalised active neighbours of ValueVariation.  result starts at the seeds' value
+// variations (FirstVariation.h:103-124), zero elsewhere.  Arrays are (N, nVar) row-major.
+void forward_variation(const Problem &P, const FMState &S, int nVar, const double *costVar, double *result) {
+    std::vector<std::pair<double, long>> order(P.N);
+    for (long i = 0; i < P.N; ++i) order[i] = {S.values[i], i};
+    std::sort(order.begin(), order.end());
+    std::vector<std::pair<long, double>> neighbors;
+    for (const auto &vi : order) {
+        const long x = vi.second;
+        const double var = value_variation(P, S, x, neighbors);
+        for (int k = 0; k < nVar; ++k) {
+            double &res = result[x * nVar + k];
+            res += var * (costVar ? costVar[x * nVar + k] : 0.);
+            for (const auto &nb : neighbors) res += nb.second * result[nb.first * nVar + k];
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Geodesic backtracking, discrete solver (GeodesicDiscreteSolver.h:14-24, .hxx:17-231): the unit
 // mass at the tip is split among the 2^d surrounding grid points and transported down the
@@ -996,6 +1016,41 @@ int oracle_geodesics_given(int model, const int *dims, const double *gridScales,
         S.active.assign(activeNeighs, activeNeighs + P.N);
         return geodesics_impl(P, S, ntips, tips, step, threshold, volumeBound, points, max_points, lengths,
                               npoints);
+    } catch (const OracleError &e) {
+        g_err = e.msg;
+        return -1;
+    }
+}
+
+int oracle_forward_variation(int model, const int *dims, const double *gridScales, const double *origin,
+                             const double *metric, int metric_is_field, int dual, double xi, double eps,
+                             const double *cost, int cost_is_field, int nseeds, const double *seeds,
+                             const double *seedValues, int nVar, const double *costVariation,
+                             const double *seedValueVariation, double *values, double *valueVariation) {
+    try {
+        g_err.clear();
+        Problem P;
+        setup_problem(P, model, dims, gridScales, origin, metric, metric_is_field, dual, xi, eps, cost,
+                      cost_is_field);
+        const auto sd = make_seeds(P, nseeds, seeds, seedValues);
+        FMState S;
+        fast_marching(P, sd, S, nullptr, nullptr);
+        if (values) std::copy(S.values.begin(), S.values.end(), values);
+        std::fill(valueVariation, valueVariation + P.N * nVar, 0.);
+        if (seedValueVariation) {
+            // (nVar, nseeds) row-major; each seed's index from ADim + floor, later seeds overwrite
+            for (int i = 0; i < nseeds; ++i) {
+                double q[3];
+                P.adim(seeds + i * P.dim, q);
+                int idx[3] = {0, 0, 0};
+                for (int c = 0; c < P.dim; ++c) idx[c] = (int)std::floor(q[c]);
+                if (!P.periodize(idx)) continue;
+                const long x = P.ravel(idx);
+                for (int k = 0; k < nVar; ++k) valueVariation[x * nVar + k] = seedValueVariation[k * nseeds + i];
+            }
+        }
+        forward_variation(P, S, nVar, costVariation, valueVariation);
+        return 0;
     } catch (const OracleError &e) {
         g_err = e.msg;
         return -1;

@@ -102,6 +102,15 @@ int oracle_geodesics_given(int model, const int *dims, const double *gridScales,
                            double threshold, double volumeBound, double *points, long max_points,
                            int64_t *lengths, long *npoints);
 
+/* Same as oracle_solve, then FirstVariation::ForwardVariation: valueVariation (N x nVar,
+ * row-major) from costVariation (N x nVar or NULL) and seedValueVariation (nVar x nseeds or
+ * NULL). */
+int oracle_forward_variation(int model, const int *dims, const double *gridScales, const double *origin,
+                             const double *metric, int metric_is_field, int dual, double xi, double eps,
+                             const double *cost, int cost_is_field, int nseeds, const double *seeds,
+                             const double *seedValues, int nVar, const double *costVariation,
+                             const double *seedValueVariation, double *values, double *valueVariation);
+
 #ifdef __cplusplus
 }
 #endif

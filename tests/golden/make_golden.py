@@ -15,6 +15,7 @@ Fixtures (all row-major):
   rsf2_32x32x16.npz  values, activeNeighs, geodesicFlow, activeOffsets, shared stencils (getStencils)
   elastica2_24x24x16.npz  values, activeOffsets, costSensitivity_0, seedSensitivity_0, shared stencils
   geodesics.npz      geodesicPoints / geodesicLengths from `tips` (discrete solver), four models
+  forward_variation.npz  valueVariation from costVariation / seedValueVariation, four models
   riemann3_128.npz   digests of values at the benchmark size + 4096 seeded sample points
 Inputs are regenerated by hamiltonfastmarching_amd.synthetic; the SHA-256 of each generated
 input is stored so that a test can tell a generator change from a numerical one.
@@ -183,6 +184,34 @@ def geodesic_cases():
     print("geodesics ok")
 
 
+FWD_CASES = {
+    "riemann2": lambda: syn.riemann2(32, seed=1),
+    "riemann3": lambda: syn.riemann3(12, seed=2),
+    "rsf2": lambda: syn.rsf2(16, 8, seed=3),
+    "elastica2": lambda: syn.elastica2(16, 8, seed=4),
+}
+
+
+def forward_cases():
+    """valueVariation (FirstVariation.h:103-124, 389-429) from seeded costVariation and
+    seedValueVariation, two variations per model."""
+    rng = np.random.Generator(np.random.PCG64(5))
+    out = {}
+    for name, make in FWD_CASES.items():
+        p = {k: v for k, v in make().items() if k not in ("tips", "inspectSensitivity", "inspectSensitivityWeights",
+                                                         "inspectSensitivityLengths")}
+        dims = tuple(int(x) for x in p["dims"])
+        cv = rng.normal(size=dims + (2,))
+        sv = rng.normal(size=(2, len(np.asarray(p["seeds"]).reshape(-1, len(dims)))))
+        r = oracle.run_reference(dict(p, costVariation=cv, seedValueVariation=sv))
+        assert r["retcode"] == 0, r["log"]
+        out[name + "_costVariation"] = cv
+        out[name + "_seedValueVariation"] = sv
+        out[name + "_valueVariation"] = r["valueVariation"]
+    np.savez_compressed(os.path.join(HERE, "forward_variation.npz"), **out)
+    print("forward variation ok")
+
+
 def big_case():
     p = syn.riemann3(128, seed=0)
     r = oracle.run_reference(p)
@@ -207,6 +236,7 @@ def main():
     curvature_case("rsf2_32x32x16", syn.rsf2(32, 16, seed=3))
     curvature_case("elastica2_24x24x16", syn.elastica2(24, 16, seed=4), sensitivity=True)
     geodesic_cases()
+    forward_cases()
     if "--big" in sys.argv:
         big_case()
 

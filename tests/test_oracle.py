@@ -384,3 +384,64 @@ def test_geodesics_descend_arrival_times(name):
         assert val[-1] < 0.5 * val[0]
         if L >= 8:
             assert np.corrcoef(np.arange(L), val)[0, 1] < -0.5
+
+
+# ------------------------------------------------------------------------------------------
+# Forward variation (tangent mode)
+# ------------------------------------------------------------------------------------------
+FWD = {
+    "riemann2": lambda: syn.riemann2(32, seed=1),
+    "riemann3": lambda: syn.riemann3(12, seed=2),
+    "rsf2": lambda: syn.rsf2(16, 8, seed=3),
+    "elastica2": lambda: syn.elastica2(16, 8, seed=4),
+}
+
+
+@pytest.mark.parametrize("name", list(FWD))
+def test_forward_variation_golden(name):
+    g = load("forward_variation")
+    p = _strip(FWD[name]())
+    _, vv = oracle.forward_variation(p, g[name + "_costVariation"], g[name + "_seedValueVariation"])
+    assert np.array_equal(vv, g[name + "_valueVariation"])
+    # planted mistake: the seed variations' layout transposed
+    sv = g[name + "_seedValueVariation"]
+    if sv.shape[1] > 1:
+        _, bad = oracle.forward_variation(p, g[name + "_costVariation"], sv[::-1].copy())
+        assert not np.array_equal(bad, g[name + "_valueVariation"])
+
+
+def _scaled(p, eps, dc):
+    """The input with cost -> cost (1 + eps dc) (curvature models), metric -> (1 + eps dc)^2 metric
+    (Riemann models: arrival times scale like a cost)."""
+    q = dict(p)
+    if "cost" in p:
+        q["cost"] = p["cost"] * (1 + eps * dc)
+    else:
+        q["metric"] = p["metric"] * ((1 + eps * dc) ** 2)[..., None]
+    return q
+
+
+@pytest.mark.parametrize("name", ["riemann2", "rsf2"])
+def test_forward_variation_finite_differences(name):
+    """valueVariation is the derivative of the arrival times along the cost / seed-value
+    direction: central differences with eps = 1e-7 agree (a larger step crosses the kinks where
+    active sets change)."""
+    p = _strip(FWD[name]())
+    dims = tuple(int(x) for x in p["dims"])
+    rng = np.random.Generator(np.random.PCG64(11))
+    dc = rng.uniform(-1, 1, size=dims)
+    cost = p["cost"] if "cost" in p else np.ones(dims)
+    _, vv = oracle.forward_variation(p, (dc * cost)[..., None])
+    eps = 1e-7
+    up = oracle.solve(_scaled(p, eps, dc))["values"]
+    dn = oracle.solve(_scaled(p, -eps, dc))["values"]
+    fd = (up - dn) / (2 * eps)
+    fin = np.isfinite(fd) & np.isfinite(vv[..., 0])
+    err = np.abs(fd[fin] - vv[..., 0][fin])
+    scale = np.abs(vv[..., 0][fin]).max()
+    assert np.quantile(err, 0.99) < 1e-6 * scale
+    # seed values: a unit variation of every seed value shifts every reached value by 1
+    ns = len(np.asarray(p["seeds"]).reshape(-1, len(dims)))
+    _, vs = oracle.forward_variation(p, None, np.ones((1, ns)))
+    reached = np.isfinite(oracle.solve(p)["values"])
+    assert np.allclose(vs[..., 0][reached], 1.0, atol=1e-12)
