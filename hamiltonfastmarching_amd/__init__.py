@@ -272,6 +272,36 @@ class Plan:
         n = min(ns.value, max_seeds)
         return out, (si[:n].copy(), sw[:n].copy())
 
+    def forward_variation(self, cost_variation=None, seed_value_variation=None, nvar=None):
+        """valueVariation (FirstVariation::ForwardVariation, FirstVariation.h:389-429) of the last
+        solve: dims + (nvar,) FP64 on the device.  cost_variation: dims + (nvar,) (device tensor or
+        array) or None; seed_value_variation: (nvar, nseeds) for the seeds of the last solve, or
+        None."""
+        torch = _torch()
+        cv = None
+        if cost_variation is not None:
+            cv = torch.as_tensor(cost_variation, dtype=torch.float64).to("cuda").contiguous()
+            if tuple(cv.shape[:-1]) != tuple(self.dims):
+                raise PyExp("Forward variation error : inconsistent input sizes")
+        svv = None
+        if seed_value_variation is not None:
+            svv = np.ascontiguousarray(np.asarray(seed_value_variation, np.float64))
+            if svv.ndim != 2:
+                raise PyExp("Error : Inconsistent size of seedValueVariation")
+        nv = nvar
+        for n in ([cv.shape[-1]] if cv is not None else []) + ([svv.shape[0]] if svv is not None else []):
+            if nv is not None and n != nv:
+                raise PyExp("Forward variation error : inconsistent input sizes")
+            nv = n
+        if nv is None:
+            raise PyExp("Forward variation error : inconsistent input sizes")
+        out = torch.empty(tuple(self.dims) + (nv,), dtype=torch.float64, device="cuda")
+        self._check(self._L.hfm_plan_forward_variation(
+            self._h, int(nv), ctypes.c_void_p(cv.data_ptr()) if cv is not None else None,
+            int(svv.shape[1]) if svv is not None else 0, _dptr(svv) if svv is not None else None,
+            ctypes.c_void_p(out.data_ptr()), _stream_ptr(torch)))
+        return out
+
     def stats(self) -> dict:
         st = _lib.HfmPlanStats()
         self._check(self._L.hfm_plan_get_stats(self._h, ctypes.byref(st)))

@@ -21,7 +21,8 @@ class HfmPlanStats(ctypes.Structure):
                 ("max_stencil_width", c_int64), ("n_points", c_int64), ("n_tiles", c_int64),
                 ("n_deps", c_int64), ("stencil_ms", c_double), ("solve_ms", c_double),
                 ("fim_ms", c_double), ("fim_launches", c_int64), ("tiles_carried", c_int64),
-                ("delta", c_double), ("adjoint_rounds", c_int64)]
+                ("delta", c_double), ("adjoint_rounds", c_int64),
+                ("forward_rounds", c_int64)]
 
 
 class HfmRequest(ctypes.Structure):
@@ -76,6 +77,7 @@ SIGNATURES = [
     ("hfm_plan_n_active_slots", c_int, [c_void_p]),
     ("hfm_plan_adjoint", c_int, [c_void_p, c_int, P(c_double), P(c_double), c_void_p, P(c_int64), P(c_double),
                                  c_int, P(c_int), c_void_p]),
+    ("hfm_plan_forward_variation", c_int, [c_void_p, c_int, c_void_p, c_int, P(c_double), c_void_p, c_void_p]),
     ("hfm_plan_get_stats", c_int, [c_void_p, P(HfmPlanStats)]),
     ("hfm_plan_set_timing", c_int, [c_void_p, c_int]),
     ("hfm_plan_set_graph", c_int, [c_void_p, c_int]),

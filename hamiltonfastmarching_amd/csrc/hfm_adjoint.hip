@@ -80,7 +80,8 @@ __device__ inline int active_set(const Geom &g, const StencilData<T> &sd, const 
         ++n;
     }
     *wsum = ws;
-    *var = ws > 0. ? (mult * acc) / ws : 0.;
+    // ValueVariation (FirstVariation.h:231-250): invWeight * (mult * sum), invWeight = 1 / S
+    *var = ws > 0. ? (1. / ws) * (mult * acc) : 0.;
     return n;
 }
 
@@ -195,11 +196,154 @@ __global__ void k_export_sens(Geom g, const double *__restrict__ s, double *__re
     out[((long long)c[0] * g.n[1] + c[1]) * g.n[2] + c[2]] = s[p];
 }
 
+// ---------------------------------------------------------------------------------------------
+// Forward differentiation (FirstVariation::ForwardVariation, FirstVariation.h:389-429).  The
+// reference visits the points in increasing (value, index) order and sets, for k < nVar,
+//   res(x, k) += var(x) * costVariation(x, k) + sum_i w_i / S * res(neighbour_i, k)
+// over x's active differences in stencil order (ValueVariation, :198-251).  A point only reads
+// its own active neighbours, all with strictly smaller values, so any topological order of the
+// DAG x -> neighbour_i gives the same sums.  The GPU runs Kahn's algorithm in pull form: a
+// point is processed once all its active neighbours are (pending count zero), gathers their
+// final results in stencil order (the reference's operation order, no atomics on the sums),
+// then decrements the pending counts of its dependants, found through a reverse adjacency
+// (CSR of the edges neighbour -> x) built once per call.
+// ---------------------------------------------------------------------------------------------
+
+// Pending count of every point (its number of active neighbours) and in-degree of the reverse
+// adjacency.
+template <int ModelT, typename T>
+__global__ __launch_bounds__(256) void k_fwd_prepare(Geom g, StencilData<T> sd, const T *__restrict__ u,
+                                                     const unsigned long long *__restrict__ fixedmask,
+                                                     int *__restrict__ pending, int *__restrict__ rcount,
+                                                     long long np) {
+    constexpr int NC = ModelShape<ModelT>::NFWD + ModelShape<ModelT>::NSYM;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    int c[3];
+    tiled_coords(g, (int)(p >> 6), (int)(p & 63), c);
+    if (!in_grid(g, c)) {
+        pending[p] = 0;
+        return;
+    }
+    const bool fixed = (fixedmask[p >> 6] >> (p & 63)) & 1ull;
+    long long nb[NC];
+    double wp[NC], ws, var;
+    const int n = active_set<ModelT, T>(g, sd, u, p, c, fixed, nb, wp, &ws, &var);
+    pending[p] = n;
+    for (int i = 0; i < n; ++i) atomicAdd(&rcount[nb[i]], 1);
+}
+
+// Reverse adjacency: dependants of point q in rev[rstart[q] .. rstart[q + 1]).  rcount (the
+// in-degrees from k_fwd_prepare) is counted back down to zero as the slots are taken.
+template <int ModelT, typename T>
+__global__ __launch_bounds__(256) void k_fwd_fill(Geom g, StencilData<T> sd, const T *__restrict__ u,
+                                                  const unsigned long long *__restrict__ fixedmask,
+                                                  const int *__restrict__ rstart, int *__restrict__ rcount,
+                                                  int *__restrict__ rev, long long np) {
+    constexpr int NC = ModelShape<ModelT>::NFWD + ModelShape<ModelT>::NSYM;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    int c[3];
+    tiled_coords(g, (int)(p >> 6), (int)(p & 63), c);
+    if (!in_grid(g, c)) return;
+    const bool fixed = (fixedmask[p >> 6] >> (p & 63)) & 1ull;
+    long long nb[NC];
+    double wp[NC], ws, var;
+    const int n = active_set<ModelT, T>(g, sd, u, p, c, fixed, nb, wp, &ws, &var);
+    for (int i = 0; i < n; ++i) {
+        const int slot = atomicSub(&rcount[nb[i]], 1) - 1;
+        rev[rstart[nb[i]] + slot] = (int)p;
+    }
+}
+
+// Initial results: zero, then the seeds' value variations (later seeds overwrite earlier ones
+// at the same point, as the reference's loop over the seeds does, FirstVariation.h:116-119; the
+// host passes one entry per point, the last one).
+__global__ void k_fwd_seeds(const long long *__restrict__ sp, const double *__restrict__ sv, int nseeds, int nvar,
+                            double *__restrict__ res) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nseeds * nvar) return;
+    const int s = i / nvar, k = i - s * nvar;
+    res[sp[s] * nvar + k] = sv[(long long)s * nvar + k];
+}
+
+template <int ModelT, typename T>
+__global__ __launch_bounds__(256) void k_fwd_round(Geom g, StencilData<T> sd, const T *__restrict__ u,
+                                                   const unsigned long long *__restrict__ fixedmask,
+                                                   int *__restrict__ pending, const int *__restrict__ rstart,
+                                                   const int *__restrict__ rev, const double *__restrict__ cv,
+                                                   int nvar, double *__restrict__ res,
+                                                   const long long *__restrict__ list_in,
+                                                   const int *__restrict__ count_in, long long *__restrict__ list_out,
+                                                   int *__restrict__ count_out, int *__restrict__ count_reset) {
+    constexpr int NC = ModelShape<ModelT>::NFWD + ModelShape<ModelT>::NSYM;
+    const int n = *count_in;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *count_reset = 0;
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (int base = wave * 64; base < n; base += nwaves * 64) {
+        const int i = base + lane;
+        int beg = 0, cnt = 0;
+        if (i < n) {
+            const long long p = list_in[i];
+            int c[3];
+            tiled_coords(g, (int)(p >> 6), (int)(p & 63), c);
+            const bool fixed = (fixedmask[p >> 6] >> (p & 63)) & 1ull;
+            long long nb[NC];
+            double wp[NC], ws = 0., var = 0.;
+            const int na = active_set<ModelT, T>(g, sd, u, p, c, fixed, nb, wp, &ws, &var);
+            // neighbour weights w_i / S as the reference normalises them (weight *= 1 / S)
+            const double inv = ws > 0. ? 1. / ws : 0.;
+            const long long lin = ((long long)c[0] * g.n[1] + c[1]) * g.n[2] + c[2];
+            for (int k = 0; k < nvar; ++k) {
+                double r = res[p * nvar + k];
+                r += var * (cv ? cv[lin * nvar + k] : 0.);
+                if (ws > 0.)
+                    for (int j = 0; j < na; ++j) r += (wp[j] * inv) * res[nb[j] * nvar + k];
+                res[p * nvar + k] = r;
+            }
+            beg = rstart[p];
+            cnt = rstart[p + 1] - beg;
+        }
+        __threadfence();
+        // dependants: wave-uniform trip count so that every lane takes part in the pushes
+        for (int j = 0; __ballot(j < cnt) != 0ull; ++j) {
+            long long q = -1;
+            if (j < cnt) {
+                const int d = rev[beg + j];
+                if (atomicSub(&pending[d], 1) == 1) q = d;
+            }
+            adj_push(q, lane, list_out, count_out);
+        }
+    }
+}
+
+__global__ void k_fwd_export(Geom g, const double *__restrict__ res, int nvar, double *__restrict__ out,
+                             long long np) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    int c[3];
+    tiled_coords(g, (int)(p >> 6), (int)(p & 63), c);
+    if (!in_grid(g, c)) return;
+    const long long lin = ((long long)c[0] * g.n[1] + c[1]) * g.n[2] + c[2];
+    for (int k = 0; k < nvar; ++k) out[lin * nvar + k] = res[p * nvar + k];
+}
+
 }  // namespace hfm
 
 using namespace hfm;
 
 namespace {
+
+// Round lists shared by the backward and forward sweeps.
+void ensure_lists(hfm_plan *p) {
+    if (!p->adj_counts) {
+        p->adj_list[0] = dalloc<long long>(p->np);
+        p->adj_list[1] = dalloc<long long>(p->np);
+        p->adj_counts = dalloc<int>(4);
+    }
+}
 
 template <int M, typename T> void adjoint_run(hfm_plan *p, const std::vector<long long> &tp,
                                               const std::vector<double> &tw, double *d_out, hipStream_t s,
@@ -272,6 +416,63 @@ template <int M, typename T> void adjoint_run(hfm_plan *p, const std::vector<lon
     *nseed_out = ns;
 }
 
+template <int M, typename T> void forward_run(hfm_plan *p, int nvar, const double *cv,
+                                              const std::vector<long long> &sp, const std::vector<double> &sv,
+                                              double *d_out, hipStream_t s) {
+    const long long np = p->np;
+    const unsigned nb = (unsigned)((np + 255) / 256);
+    const T *u = static_cast<const T *>(p->u);
+    HIPCHECK(hipMemsetAsync(p->fwd_rcount, 0, sizeof(int) * np, s));
+    hipLaunchKernelGGL((k_fwd_prepare<M, T>), dim3(nb), dim3(256), 0, s, p->g, p->sd<T>(), u, p->fixedmask,
+                       p->fwd_pending, p->fwd_rcount, np);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(launch_exclusive_scan(p->fwd_rcount, p->fwd_rstart, (int)np, s));
+    int nedges = 0;
+    HIPCHECK(hipMemcpyAsync(&nedges, p->fwd_rstart + np, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    if (nedges > p->fwd_rev_cap) {
+        dfree(p->fwd_rev);
+        p->fwd_rev = nullptr;
+        p->fwd_rev_cap = std::max<long long>(nedges, np);
+        p->fwd_rev = dalloc<int>(p->fwd_rev_cap);
+    }
+    hipLaunchKernelGGL((k_fwd_fill<M, T>), dim3(nb), dim3(256), 0, s, p->g, p->sd<T>(), u, p->fixedmask,
+                       p->fwd_rstart, p->fwd_rcount, p->fwd_rev, np);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemsetAsync(p->fwd_res, 0, sizeof(double) * np * nvar, s));
+    if (!sp.empty()) {
+        HIPCHECK(hipMemcpyAsync(p->fwd_sp, sp.data(), sp.size() * sizeof(long long), hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(p->fwd_sv, sv.data(), sv.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        const int n = (int)sp.size() * nvar;
+        hipLaunchKernelGGL(k_fwd_seeds, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p->fwd_sp, p->fwd_sv,
+                           (int)sp.size(), nvar, p->fwd_res);
+        HIPCHECK(hipGetLastError());
+    }
+    HIPCHECK(hipMemsetAsync(p->adj_counts, 0, 4 * sizeof(int), s));
+    hipLaunchKernelGGL(k_adj_frontier, dim3(nb), dim3(256), 0, s, p->g, p->fwd_pending, np, p->adj_list[0],
+                       &p->adj_counts[0]);
+    HIPCHECK(hipGetLastError());
+    const unsigned grid = (unsigned)std::min<long long>((np + 255) / 256, 2048);
+    long long k = 0;
+    const long long max_rounds = np + 16;
+    while (true) {
+        for (int b = 0; b < 8; ++b, ++k) {
+            hipLaunchKernelGGL((k_fwd_round<M, T>), dim3(grid), dim3(256), 0, s, p->g, p->sd<T>(), u, p->fixedmask,
+                               p->fwd_pending, p->fwd_rstart, p->fwd_rev, cv, nvar, p->fwd_res, p->adj_list[k & 1],
+                               &p->adj_counts[k % 3], p->adj_list[(k + 1) & 1], &p->adj_counts[(k + 1) % 3],
+                               &p->adj_counts[(k + 2) % 3]);
+            HIPCHECK(hipGetLastError());
+        }
+        HIPCHECK(hipMemcpyAsync(p->h_count, &p->adj_counts[k % 3], sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        if (*p->h_count == 0) break;
+        if (k > max_rounds) throw HipFail{"forward variation sweep did not terminate"};
+    }
+    p->st.forward_rounds = k;
+    hipLaunchKernelGGL(k_fwd_export, dim3(nb), dim3(256), 0, s, p->g, p->fwd_res, nvar, d_out, np);
+    HIPCHECK(hipGetLastError());
+}
+
 }  // namespace
 
 int hfm_plan_adjoint_impl(hfm_plan *p, int ntargets, const double *targets, const double *weights,
@@ -321,10 +522,8 @@ int hfm_plan_adjoint_impl(hfm_plan *p, int ntargets, const double *targets, cons
             p->adj_lambda = dalloc<double>(p->np);
             p->adj_sens = dalloc<double>(p->np);
             p->adj_indeg = dalloc<int>(p->np);
-            p->adj_list[0] = dalloc<long long>(p->np);
-            p->adj_list[1] = dalloc<long long>(p->np);
-            p->adj_counts = dalloc<int>(4);
         }
+        ensure_lists(p);
         const int cap = std::max(max_seeds, 1);
         if (cap > p->adj_seed_cap) {
             dfree(p->adj_seed_idx);
@@ -371,6 +570,86 @@ int hfm_plan_adjoint_impl(hfm_plan *p, int ntargets, const double *targets, cons
             seed_weight[i] = sw[i];
         }
         *nseed_out = ns;
+    } catch (const HipFail &e) {
+        set_error(e.msg);
+        return 1;
+    }
+    return 0;
+}
+
+int hfm_plan_forward_variation_impl(hfm_plan *p, int nvar, const double *d_cost_variation, int nseeds,
+                                    const double *seed_value_variation, double *d_value_variation, void *stream) {
+    if (!p || !d_value_variation || (seed_value_variation && nseeds < 0)) {
+        set_error("hfm_plan_forward_variation: null argument");
+        return 1;
+    }
+    if (!p->solved) {
+        set_error("hfm_plan_forward_variation: no solve has run");
+        return 1;
+    }
+    if (nvar <= 0) {
+        set_error("Forward variation error : inconsistent input sizes");
+        return 1;
+    }
+    try {
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        // seedValueVariation (nVar x nseeds, row-major) onto the seeds of the last solve, in the
+        // order they were passed; a later seed overwrites an earlier one at the same point.
+        std::vector<long long> sp;
+        std::vector<double> sv;
+        if (seed_value_variation) {
+            if (nseeds != (int)p->seed_inputs.size()) throw HipFail{"Error : Inconsistent size of seedValueVariation"};
+            std::map<long long, int> last;
+            for (int i = 0; i < nseeds; ++i)
+                if (p->seed_inputs[i] >= 0) last[p->seed_inputs[i]] = i;
+            for (const auto &kv : last) {
+                sp.push_back(kv.first);
+                for (int k = 0; k < nvar; ++k) sv.push_back(seed_value_variation[(size_t)k * nseeds + kv.second]);
+            }
+        }
+        ensure_lists(p);
+        if (!p->fwd_pending) {
+            p->fwd_pending = dalloc<int>(p->np);
+            p->fwd_rcount = dalloc<int>(p->np);
+            p->fwd_rstart = dalloc<int>(p->np + 1);
+        }
+        if ((long long)nvar * p->np > p->fwd_res_cap) {
+            dfree(p->fwd_res);
+            p->fwd_res = nullptr;
+            p->fwd_res_cap = (long long)nvar * p->np;
+            p->fwd_res = dalloc<double>(p->fwd_res_cap);
+        }
+        if ((int)sp.size() > p->fwd_seed_cap) {
+            dfree(p->fwd_sp);
+            p->fwd_sp = nullptr;
+            p->fwd_seed_cap = std::max<int>((int)sp.size(), 16);
+            p->fwd_sp = dalloc<long long>(p->fwd_seed_cap);
+        }
+        if ((long long)sv.size() > p->fwd_sv_cap) {
+            dfree(p->fwd_sv);
+            p->fwd_sv = nullptr;
+            p->fwd_sv_cap = std::max<long long>((long long)sv.size(), 64);
+            p->fwd_sv = dalloc<double>(p->fwd_sv_cap);
+        }
+        const bool f32 = p->precision == 32;
+        switch (p->model) {
+            case MODEL_RIEMANN2:
+                if (f32) forward_run<MODEL_RIEMANN2, float>(p, nvar, d_cost_variation, sp, sv, d_value_variation, s);
+                else forward_run<MODEL_RIEMANN2, double>(p, nvar, d_cost_variation, sp, sv, d_value_variation, s);
+                break;
+            case MODEL_RIEMANN3:
+                if (f32) forward_run<MODEL_RIEMANN3, float>(p, nvar, d_cost_variation, sp, sv, d_value_variation, s);
+                else forward_run<MODEL_RIEMANN3, double>(p, nvar, d_cost_variation, sp, sv, d_value_variation, s);
+                break;
+            case MODEL_RSF2:
+                if (f32) forward_run<MODEL_RSF2, float>(p, nvar, d_cost_variation, sp, sv, d_value_variation, s);
+                else forward_run<MODEL_RSF2, double>(p, nvar, d_cost_variation, sp, sv, d_value_variation, s);
+                break;
+            case MODEL_ELASTICA2:
+                if (f32) forward_run<MODEL_ELASTICA2, float>(p, nvar, d_cost_variation, sp, sv, d_value_variation, s);
+                else forward_run<MODEL_ELASTICA2, double>(p, nvar, d_cost_variation, sp, sv, d_value_variation, s);
+                break;
+        }
     } catch (const HipFail &e) {
         set_error(e.msg);
         return 1;

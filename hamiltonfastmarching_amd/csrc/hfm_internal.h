@@ -92,6 +92,13 @@ struct hfm_plan {
     long long *adj_seed_idx = nullptr, *adj_tp = nullptr;
     double *adj_seed_w = nullptr, *adj_tw = nullptr;
     int adj_seed_cap = 0, adj_t_cap = 0;
+    // forward-variation scratch (first hfm_plan_forward_variation call; lists shared with the adjoint)
+    int *fwd_pending = nullptr, *fwd_rcount = nullptr, *fwd_rstart = nullptr, *fwd_rev = nullptr;
+    long long fwd_rev_cap = 0;
+    double *fwd_res = nullptr, *fwd_sv = nullptr;
+    long long fwd_res_cap = 0, fwd_sv_cap = 0;
+    long long *fwd_sp = nullptr;
+    int fwd_seed_cap = 0;
 
     bool stencil_ready = false;
     bool solved = false;
@@ -105,6 +112,7 @@ struct hfm_plan {
     std::vector<char> graph_key;
     int max_stencil_width = 0;
     std::vector<long long> last_seeds;  // tiled indices of the last solve's seeds
+    std::vector<long long> seed_inputs; // tiled index of each seed as passed (-1: out of the grid)
     hfm_plan_stats st{};
 
     template <typename T> StencilData<T> sd() const {
@@ -122,5 +130,8 @@ struct hfm_plan {
 int hfm_plan_adjoint_impl(hfm_plan *p, int ntargets, const double *targets, const double *weights,
                           double *d_sensitivity, int64_t *seed_index, double *seed_weight, int max_seeds,
                           int *nseed_out, void *stream);
+
+int hfm_plan_forward_variation_impl(hfm_plan *p, int nvar, const double *d_cost_variation, int nseeds,
+                                    const double *seed_value_variation, double *d_value_variation, void *stream);
 
 #endif

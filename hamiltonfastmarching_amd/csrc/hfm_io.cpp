@@ -494,6 +494,54 @@ void run_impl(hfm_io *io) {
             io->set_computed("seedSensitivity_" + std::to_string(i), std::move(f));
         }
     }
+    // FirstVariation::Finally (forward part, FirstVariation.h:102-121): costVariation (dims...,
+    // nVar; every axis follows arrayOrdering) and seedValueVariation (nVar x nseeds, read
+    // RowMajor) -> valueVariation (dims..., nVar)
+    if (io->has("costVariation") || io->has("seedValueVariation")) {
+        const size_t D = dims.size();
+        int64_t nvc = -1, nvs = -1;
+        std::vector<double> cv;
+        if (io->has("costVariation")) {
+            const Field &f = io->raw("costVariation");
+            if (f.is_string) throw IOError{"IO error : key costVariation is a string, not a numerical field."};
+            if (f.dims.size() != D + 1)
+                throw IOError{"IO::GetArray error : Field costVariation has depth " + std::to_string(f.dims.size()) +
+                              " instead of expected " + std::to_string(D + 1) + "."};
+            std::vector<int64_t> internal;
+            cv = reorder(f.data, f.dims, 1, io->ordering, true, internal);
+            if (!std::equal(dims.begin(), dims.end(), internal.begin()))
+                throw IOError{"Forward variation error : inconsistent input sizes"};
+            nvc = internal[D];
+        }
+        std::vector<double> svv;
+        if (io->has("seedValueVariation")) {
+            const Field &f = io->raw("seedValueVariation");
+            if (f.is_string) throw IOError{"IO error : key seedValueVariation is a string, not a numerical field."};
+            if (f.dims.size() != 2)
+                throw IOError{"IO::GetArray error : Field seedValueVariation has depth " +
+                              std::to_string(f.dims.size()) + " instead of expected 2."};
+            if (f.dims[1] != nseeds) throw IOError{"Error : Inconsistent size of seedValueVariation"};
+            nvs = f.dims[0];
+            svv = f.data;
+        }
+        if (nvc >= 0 && nvs >= 0 && nvc != nvs) throw IOError{"Forward variation error : inconsistent input sizes"};
+        const int64_t nvar = std::max(nvc, nvs);
+        std::vector<double> vv((size_t)N * nvar, 0.);
+        if (nvar > 0) {
+            DevBuf dcv(cv.empty() ? 8 : cv.size() * sizeof(double));
+            if (!cv.empty() &&
+                hipMemcpy(dcv.p, cv.data(), cv.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+                throw IOError{"hipMemcpy failed"};
+            DevBuf dvv(vv.size() * sizeof(double));
+            check(hfm_plan_forward_variation(plan, (int)nvar, cv.empty() ? nullptr : dcv.d(), nseeds,
+                                             svv.empty() ? nullptr : svv.data(), dvv.d(), nullptr));
+            if (hipMemcpy(vv.data(), dvv.p, vv.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+                throw IOError{"hipMemcpy failed"};
+        }
+        std::vector<int64_t> vdims(dims.begin(), dims.end());
+        vdims.push_back(nvar);
+        set_grid_array(io, "valueVariation", vv, vdims, 1);
+    }
 }
 
 void usage_report(hfm_io *io) {

@@ -105,6 +105,13 @@ void plan_free(hfm_plan *p) {
     dfree(p->adj_seed_w);
     dfree(p->adj_tp);
     dfree(p->adj_tw);
+    dfree(p->fwd_pending);
+    dfree(p->fwd_rcount);
+    dfree(p->fwd_rstart);
+    dfree(p->fwd_rev);
+    dfree(p->fwd_res);
+    dfree(p->fwd_sv);
+    dfree(p->fwd_sp);
     dfree(p->status);
     dfree(p->stats);
     dfree(p->seed_p);
@@ -228,8 +235,9 @@ void curvature_table(hfm_plan *p, std::vector<uint32_t> &toff, std::vector<doubl
 // Seed placement (StaticFactoring.hxx:242-393): ADim, floor, out-of-range seeds skipped,
 // the first value kept for a repeated index.  Returns tiled indices in row-major index order.
 void place_seeds(const hfm_plan *p, int nseeds, const double *seeds, const double *sv,
-                 std::vector<long long> &tp, std::vector<double> &vals) {
+                 std::vector<long long> &tp, std::vector<double> &vals, std::vector<long long> *inputs = nullptr) {
     std::map<long long, std::pair<long long, double>> bylin;
+    if (inputs) inputs->assign(nseeds, -1);
     const int dim = p->g.dim;
     for (int s = 0; s < nseeds; ++s) {
         int c[3] = {0, 0, 0};
@@ -253,6 +261,7 @@ void place_seeds(const hfm_plan *p, int nseeds, const double *seeds, const doubl
         if (!ok) continue;
         const long long lin = ((long long)c[0] * p->g.n[1] + c[1]) * p->g.n[2] + c[2];
         const long long tpi = (long long)tile_of(p->g, c) * 64 + local_of(p->g, c);
+        if (inputs) (*inputs)[s] = tpi;
         bylin.insert({lin, {tpi, sv ? sv[s] : 0.}});
     }
     tp.clear();
@@ -647,9 +656,9 @@ int hfm_plan_solve(hfm_plan *p, int nseeds, const double *seeds, const double *s
     }
     try {
         hipStream_t s = static_cast<hipStream_t>(stream);
-        std::vector<long long> tp;
+        std::vector<long long> tp, inputs;
         std::vector<double> vals;
-        place_seeds(p, nseeds, seeds, seedValues, tp, vals);
+        place_seeds(p, nseeds, seeds, seedValues, tp, vals, &inputs);
         if (tp.empty()) throw HipFail{nseeds > 0 ? "Error : seeds incorrectly set" : "No seeds to run fast marching solver"};
         if ((int)tp.size() > p->seed_cap) {
             dfree(p->seed_p);
@@ -663,6 +672,7 @@ int hfm_plan_solve(hfm_plan *p, int nseeds, const double *seeds, const double *s
         HIPCHECK(hipMemcpyAsync(p->seed_p, tp.data(), tp.size() * sizeof(long long), hipMemcpyHostToDevice, s));
         HIPCHECK(hipMemcpyAsync(p->seed_v, vals.data(), vals.size() * sizeof(double), hipMemcpyHostToDevice, s));
         p->last_seeds = tp;
+        p->seed_inputs = inputs;
         if (p->precision == 32) run_fim<float>(p, s);
         else run_fim<double>(p, s);
         p->solved = true;
@@ -929,6 +939,12 @@ int hfm_plan_adjoint(hfm_plan *p, int ntargets, const double *targets, const dou
                      int64_t *seed_index, double *seed_weight, int max_seeds, int *nseed_out, void *stream) {
     return hfm_plan_adjoint_impl(p, ntargets, targets, weights, d_sensitivity, seed_index, seed_weight, max_seeds,
                                  nseed_out, stream);
+}
+
+int hfm_plan_forward_variation(hfm_plan *p, int nvar, const double *d_cost_variation, int nseeds,
+                               const double *seed_value_variation, double *d_value_variation, void *stream) {
+    return hfm_plan_forward_variation_impl(p, nvar, d_cost_variation, nseeds, seed_value_variation,
+                                           d_value_variation, stream);
 }
 
 }  // extern "C"

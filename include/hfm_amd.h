@@ -105,6 +105,7 @@ typedef struct hfm_plan_stats {
     int64_t tiles_carried;  /* listed tiles deferred to a later band (not processed) */
     double delta;           /* band width used by the last solve (0: unordered) */
     int64_t adjoint_rounds; /* rounds of the last backward sweep */
+    int64_t forward_rounds; /* rounds of the last forward-variation sweep */
 } hfm_plan_stats;
 
 /* model: "Riemann2" | "Riemann3" | "ReedsSheppForward2" | "Elastica2".
@@ -166,6 +167,20 @@ int hfm_plan_geodesics(hfm_plan *plan, int ntips, const double *tips, double ste
 int hfm_plan_adjoint(hfm_plan *plan, int ntargets, const double *targets, const double *weights,
                      double *d_sensitivity, int64_t *seed_index, double *seed_weight, int max_seeds,
                      int *nseed_out, void *stream);
+
+/* Forward differentiation (FirstVariation::ForwardVariation, FirstVariation.h:389-429, driver
+ * FirstVariation.h:102-121): first-order variation of the arrival times of the last solve
+ * along nvar directions of the cost (costVariation) and of the seed values
+ * (seedValueVariation).
+ *   d_cost_variation: device, FP64, row-major (dims..., nvar), or NULL (zero);
+ *   seed_value_variation: host, FP64, nvar x nseeds row-major (the reference reads it RowMajor
+ *   whatever the array ordering), nseeds = the seeds passed to the last hfm_plan_solve, in that
+ *   order (a later seed overwrites an earlier one at the same grid point), or NULL (zero);
+ *   d_value_variation: device, FP64, row-major (dims..., nvar) -- the 'valueVariation' output.
+ * Errors: "Error : Inconsistent size of seedValueVariation" when nseeds differs from the
+ * solve's seed count. */
+int hfm_plan_forward_variation(hfm_plan *plan, int nvar, const double *d_cost_variation, int nseeds,
+                               const double *seed_value_variation, double *d_value_variation, void *stream);
 
 int hfm_plan_get_stats(const hfm_plan *plan, hfm_plan_stats *stats);
 
