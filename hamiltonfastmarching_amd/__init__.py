@@ -12,7 +12,8 @@ Mirrors the reference's Python interface for its hot path:
   ``FileIO.WriteCallRead`` (JMM_CPPLibs/FileIO/FileIO.py:54-73) with FileHFM's RowMajor default.
 * ``Plan`` -- device-resident solver over torch CUDA tensors (hfm_plan_* of the C ABI).
 * ``EikonalSolve`` -- ``torch.autograd.Function``: arrival times at target points as a function of
-  the cost field, backward = the adjoint (FirstVariation::BackwardVariation).
+  the cost field, backward = the adjoint (FirstVariation::BackwardVariation), forward-mode
+  ``jvp`` = the forward variation (FirstVariation::ForwardVariation).
 """
 from __future__ import annotations
 
@@ -343,7 +344,9 @@ class EikonalSolve:
     """``EikonalSolve.apply(cost, plan, seeds, targets, weights)`` -> sum_t weights[t] u(targets[t]).
 
     A ``torch.autograd.Function`` for the curvature models (cost field): forward runs the GPU
-    solve, backward runs the adjoint kernel and returns d/d cost."""
+    solve, backward runs the adjoint kernel and returns d/d cost, and forward-mode AD (``jvp``)
+    runs the forward-variation kernels with the cost tangent as costVariation
+    (FirstVariation::ForwardVariation)."""
 
     _fn = None
 
@@ -369,6 +372,7 @@ def _make_autograd():
             ctx.save_for_backward(sens)
             # value = sum_t w_t u(index(t)), index by the reference's seed/target snapping
             idx = _snap(plan, t)
+            ctx.fwd = (plan, idx, w)
             val = torch.zeros((), dtype=torch.float64, device=u.device)
             for (i, wt) in zip(idx, w):
                 val = val + wt * u[i]
@@ -378,6 +382,15 @@ def _make_autograd():
         def backward(ctx, g):
             (sens,) = ctx.saved_tensors
             return g * sens, None, None, None, None
+
+        @staticmethod
+        def jvp(ctx, cost_t, plan_t, seeds_t, targets_t, weights_t):
+            plan, idx, w = ctx.fwd
+            vv = plan.forward_variation(cost_t.detach().to(torch.float64).contiguous()[..., None])[..., 0]
+            out = torch.zeros((), dtype=torch.float64, device=vv.device)
+            for (i, wt) in zip(idx, w):
+                out = out + wt * vv[i]
+            return out
 
     return _Eik
 
