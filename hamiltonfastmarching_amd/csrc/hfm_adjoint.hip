@@ -119,6 +119,27 @@ __device__ inline void adj_push(long long q, int lane, long long *__restrict__ l
     }
 }
 
+// Wave-aggregated append of up to 8 tiled indices per lane (items[q] for the set bits q of
+// mask): one counter atomic per wave, positions from a prefix sum over the lanes.
+__device__ inline void wave_push8(const long long *items, unsigned mask, int lane, long long *__restrict__ list,
+                                  int *__restrict__ count) {
+    const int c = __popc(mask);
+    int incl = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    const int total = __shfl(incl, 63);
+    if (total == 0) return;  // wave-uniform
+    int base = 0;
+    if (lane == 0) base = atomicAdd(count, total);
+    int pos = __shfl(base, 0) + incl - c;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        if (mask >> q & 1u) list[pos++] = items[q];
+}
+
 __global__ __launch_bounds__(256) void k_adj_frontier(Geom g, const int *__restrict__ indeg, long long np,
                                                       long long *__restrict__ list, int *__restrict__ count) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -178,11 +199,23 @@ __global__ __launch_bounds__(256) void k_adj_round(Geom g, StencilData<T> sd, co
                 }
             }
         }
-        __threadfence();
-        for (int k = 0; k < NC; ++k) {
-            long long q = -1;
-            if (k < na && atomicSub(&indeg[nb[k]], 1) == 1) q = nb[k];
-            adj_push(q, lane, list_out, count_out);
+        // In-degree decrements, eight per lane at a time (independent atomics in flight), then one
+        // aggregated append of the neighbours that became ready.  They are processed by the next
+        // launch, which sees this launch's lambda updates.
+#pragma unroll
+        for (int k0 = 0; k0 < NC; k0 += 8) {
+            if (__ballot(k0 < na) == 0ull) continue;
+            long long items[8];
+            unsigned mask = 0;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int k = k0 + q;
+                items[q] = (k < NC && k < na) ? nb[k < NC ? k : 0] : -1;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (items[q] >= 0 && atomicSub(&indeg[items[q]], 1) == 1) mask |= 1u << q;
+            wave_push8(items, mask, lane, list_out, count_out);
         }
     }
 }
@@ -306,15 +339,18 @@ __global__ __launch_bounds__(256) void k_fwd_round(Geom g, StencilData<T> sd, co
             beg = rstart[p];
             cnt = rstart[p + 1] - beg;
         }
-        __threadfence();
-        // dependants: wave-uniform trip count so that every lane takes part in the pushes
-        for (int j = 0; __ballot(j < cnt) != 0ull; ++j) {
-            long long q = -1;
-            if (j < cnt) {
-                const int d = rev[beg + j];
-                if (atomicSub(&pending[d], 1) == 1) q = d;
-            }
-            adj_push(q, lane, list_out, count_out);
+        // Dependants' pending counts, eight per lane at a time (independent atomics in flight), then
+        // one aggregated append of those that became ready.  They are processed by the next launch,
+        // which sees this launch's results.
+        for (int j0 = 0; __ballot(j0 < cnt) != 0ull; j0 += 8) {
+            long long items[8];
+            unsigned mask = 0;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) items[q] = j0 + q < cnt ? (long long)rev[beg + j0 + q] : -1;
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (items[q] >= 0 && atomicSub(&pending[items[q]], 1) == 1) mask |= 1u << q;
+            wave_push8(items, mask, lane, list_out, count_out);
         }
     }
 }
@@ -367,7 +403,7 @@ template <int M, typename T> void adjoint_run(hfm_plan *p, const std::vector<lon
     hipLaunchKernelGGL(k_adj_frontier, dim3(nb), dim3(256), 0, s, p->g, p->adj_indeg, np, p->adj_list[0],
                        &p->adj_counts[0]);
     HIPCHECK(hipGetLastError());
-    const unsigned grid = (unsigned)std::min<long long>((np + 255) / 256, 2048);
+    const unsigned grid = (unsigned)std::min<long long>((np + 255) / 256, 1024);
     long long k = 0;
     const long long max_rounds = np + 16;
     while (true) {
@@ -426,7 +462,7 @@ template <int M, typename T> void forward_run(hfm_plan *p, int nvar, const doubl
     hipLaunchKernelGGL((k_fwd_prepare<M, T>), dim3(nb), dim3(256), 0, s, p->g, p->sd<T>(), u, p->fixedmask,
                        p->fwd_pending, p->fwd_rcount, np);
     HIPCHECK(hipGetLastError());
-    HIPCHECK(launch_exclusive_scan(p->fwd_rcount, p->fwd_rstart, (int)np, s));
+    HIPCHECK(launch_exclusive_scan(p->fwd_rcount, p->fwd_rstart, (int)np, s, p->fwd_scan_work));
     int nedges = 0;
     HIPCHECK(hipMemcpyAsync(&nedges, p->fwd_rstart + np, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
@@ -452,7 +488,7 @@ template <int M, typename T> void forward_run(hfm_plan *p, int nvar, const doubl
     hipLaunchKernelGGL(k_adj_frontier, dim3(nb), dim3(256), 0, s, p->g, p->fwd_pending, np, p->adj_list[0],
                        &p->adj_counts[0]);
     HIPCHECK(hipGetLastError());
-    const unsigned grid = (unsigned)std::min<long long>((np + 255) / 256, 2048);
+    const unsigned grid = (unsigned)std::min<long long>((np + 255) / 256, 1024);
     long long k = 0;
     const long long max_rounds = np + 16;
     while (true) {
@@ -612,6 +648,7 @@ int hfm_plan_forward_variation_impl(hfm_plan *p, int nvar, const double *d_cost_
             p->fwd_pending = dalloc<int>(p->np);
             p->fwd_rcount = dalloc<int>(p->np);
             p->fwd_rstart = dalloc<int>(p->np + 1);
+            p->fwd_scan_work = dalloc<int>(scan_work_ints(p->np));
         }
         if ((long long)nvar * p->np > p->fwd_res_cap) {
             dfree(p->fwd_res);

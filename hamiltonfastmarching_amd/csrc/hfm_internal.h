@@ -94,6 +94,7 @@ struct hfm_plan {
     int adj_seed_cap = 0, adj_t_cap = 0;
     // forward-variation scratch (first hfm_plan_forward_variation call; lists shared with the adjoint)
     int *fwd_pending = nullptr, *fwd_rcount = nullptr, *fwd_rstart = nullptr, *fwd_rev = nullptr;
+    int *fwd_scan_work = nullptr;
     long long fwd_rev_cap = 0;
     double *fwd_res = nullptr, *fwd_sv = nullptr;
     long long fwd_res_cap = 0, fwd_sv_cap = 0;

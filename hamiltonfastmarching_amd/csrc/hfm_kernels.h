@@ -77,7 +77,10 @@ hipError_t launch_riemann_stencil(const Geom &g, const double *metric, int is_fi
 template <typename T>
 hipError_t launch_tile_deps(int model, const Geom &g, const StencilData<T> &sd, int pass, int *cnt, const int *start,
                             int *fill, int *deps, int *ovf, hipStream_t s);
-hipError_t launch_exclusive_scan(const int *cnt, int *start, int n, hipStream_t s);
+// Exclusive scan of n counts into start[0..n].  With work (scan_work_ints(n) ints of device
+// memory) long arrays are scanned by one block per 4096 counts; without, by a single block.
+long long scan_work_ints(long long n);
+hipError_t launch_exclusive_scan(const int *cnt, int *start, int n, hipStream_t s, int *work = nullptr);
 template <typename T>
 hipError_t launch_init(const Geom &g, T *u, unsigned long long *fixedmask, unsigned *keys, unsigned *tmax,
                        long long np, const long long *seed_p, const double *seed_v, int nseeds, hipStream_t s);

@@ -156,48 +156,70 @@ __global__ __launch_bounds__(64) void k_tile_deps(Geom g, StencilData<T> sd, int
     (void)overflow;
 }
 
-// Exclusive scan of n counts into start[0..n] (one block of 1024; setup-time only).  The array is
-// swept in coalesced chunks of 4096 (four consecutive counts per thread); each chunk is scanned
-// with wave shuffles plus one LDS pass over the 16 wave totals, and carries into the next chunk.
-__global__ __launch_bounds__(1024) void k_exclusive_scan(const int *__restrict__ cnt, int *__restrict__ start, int n) {
-    __shared__ int wsum[16];
-    __shared__ int carry_s;
+// Exclusive scans of int counts.  The unit of work is a chunk of 4096 consecutive counts taken
+// by a block of 1024 threads (four consecutive counts per thread, coalesced), scanned with wave
+// shuffles plus one LDS pass over the 16 wave totals.
+constexpr int SCAN_CHUNK = 4096;
+
+// Scans chunk [base, base + 4096) of cnt into start with the given carry; returns the chunk total
+// (on every thread).  Contains barriers: every thread of the block calls it.
+__device__ inline int scan_chunk(const int *__restrict__ cnt, int *__restrict__ start, int n, int base, int carry,
+                                 int *wsum) {
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    if (tid == 0) carry_s = 0;
+    const int i0 = base + tid * 4;
+    int v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = i0 + k < n ? cnt[i0 + k] : 0;
+    const int tsum = v[0] + v[1] + v[2] + v[3];
+    int incl = tsum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) wsum[wid] = incl;
     __syncthreads();
-    for (int base = 0; base < n; base += 4096) {
-        const int i0 = base + tid * 4;
-        int v[4];
+    int wpre = 0, total = 0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) v[k] = i0 + k < n ? cnt[i0 + k] : 0;
-        const int tsum = v[0] + v[1] + v[2] + v[3];
-        int incl = tsum;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int t = __shfl_up(incl, o);
-            if (lane >= o) incl += t;
-        }
-        if (lane == 63) wsum[wid] = incl;
-        const int carry = carry_s;
-        __syncthreads();
-        int wpre = 0, total = 0;
-#pragma unroll
-        for (int w = 0; w < 16; ++w) {
-            const int x = wsum[w];
-            wpre += w < wid ? x : 0;
-            total += x;
-        }
+    for (int w = 0; w < 16; ++w) {
+        const int x = wsum[w];
+        wpre += w < wid ? x : 0;
+        total += x;
+    }
+    if (start) {
         int run = carry + wpre + incl - tsum;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             if (i0 + k < n) start[i0 + k] = run;
             run += v[k];
         }
-        __syncthreads();
-        if (tid == 0) carry_s = carry + total;
-        __syncthreads();
     }
-    if (tid == 0) start[n] = carry_s;
+    __syncthreads();  // wsum is reused by the next chunk
+    return total;
+}
+
+// One block of 1024 sweeps the whole array chunk by chunk, carrying the running total; writes
+// start[0..n].  Used for short arrays and for the chunk totals of the multi-block scan.
+__global__ __launch_bounds__(1024) void k_exclusive_scan(const int *__restrict__ cnt, int *__restrict__ start, int n) {
+    __shared__ int wsum[16];
+    int carry = 0;
+    for (int base = 0; base < n; base += SCAN_CHUNK) carry += scan_chunk(cnt, start, n, base, carry, wsum);
+    if (threadIdx.x == 0) start[n] = carry;
+}
+
+// Multi-block scan, pass 1: the total of every chunk.
+__global__ __launch_bounds__(1024) void k_scan_totals(const int *__restrict__ cnt, int n, int *__restrict__ totals) {
+    __shared__ int wsum[16];
+    const int t = scan_chunk(cnt, nullptr, n, blockIdx.x * SCAN_CHUNK, 0, wsum);
+    if (threadIdx.x == 0) totals[blockIdx.x] = t;
+}
+
+// Pass 3: every chunk rescanned from its carry (the scanned chunk totals); start[n] = total.
+__global__ __launch_bounds__(1024) void k_scan_apply(const int *__restrict__ cnt, const int *__restrict__ carries,
+                                                     int *__restrict__ start, int n) {
+    __shared__ int wsum[16];
+    (void)scan_chunk(cnt, start, n, blockIdx.x * SCAN_CHUNK, carries[blockIdx.x], wsum);
+    if (blockIdx.x == 0 && threadIdx.x == 0) start[n] = carries[gridDim.x];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -499,8 +521,18 @@ template hipError_t launch_tile_deps<float>(int, const Geom &, const StencilData
 template hipError_t launch_tile_deps<double>(int, const Geom &, const StencilData<double> &, int, int *, const int *,
                                              int *, int *, int *, hipStream_t);
 
-hipError_t launch_exclusive_scan(const int *cnt, int *start, int n, hipStream_t s) {
-    hipLaunchKernelGGL(k_exclusive_scan, dim3(1), dim3(1024), 0, s, cnt, start, n);
+long long scan_work_ints(long long n) { return 2 * ((n + SCAN_CHUNK - 1) / SCAN_CHUNK) + 1; }
+
+hipError_t launch_exclusive_scan(const int *cnt, int *start, int n, hipStream_t s, int *work) {
+    const int nchunks = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    if (!work || nchunks <= 8) {
+        hipLaunchKernelGGL(k_exclusive_scan, dim3(1), dim3(1024), 0, s, cnt, start, n);
+        return hipGetLastError();
+    }
+    int *totals = work, *carries = work + nchunks;  // nchunks + (nchunks + 1) ints
+    hipLaunchKernelGGL(k_scan_totals, dim3(nchunks), dim3(1024), 0, s, cnt, n, totals);
+    hipLaunchKernelGGL(k_exclusive_scan, dim3(1), dim3(1024), 0, s, totals, carries, nchunks);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nchunks), dim3(1024), 0, s, cnt, carries, start, n);
     return hipGetLastError();
 }
 

@@ -108,6 +108,7 @@ void plan_free(hfm_plan *p) {
     dfree(p->fwd_pending);
     dfree(p->fwd_rcount);
     dfree(p->fwd_rstart);
+    dfree(p->fwd_scan_work);
     dfree(p->fwd_rev);
     dfree(p->fwd_res);
     dfree(p->fwd_sv);
