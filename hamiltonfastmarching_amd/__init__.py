@@ -318,6 +318,10 @@ class Plan:
                                                   _dptr(w)))
         return off, w
 
+    def set_max_sweeps(self, sweeps: int):
+        self._check(self._L.hfm_plan_set_max_sweeps(self._h, int(sweeps)))
+        return self
+
     def set_delta(self, factor: float):
         self._check(self._L.hfm_plan_set_delta(self._h, float(factor)))
         return self

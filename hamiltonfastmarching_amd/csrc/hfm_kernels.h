@@ -54,6 +54,7 @@ template <typename T> struct FimArgs {
     unsigned *pend_reset;
     unsigned long long *stats;  // PEND_SLOTS pairs: (tiles processed, sweeps summed)
     unsigned grid;
+    int max_sweeps;  // sweeps per tile visit before the tile is handed back to the schedule: 8, 16 or 32
 };
 
 // Geodesic backtracking (hfm_geodesic.hip).  Tips are ADim'd (grid-index units); points are

@@ -565,16 +565,23 @@ hipError_t launch_schedule(int ntiles, unsigned *keys, const SchedRound &sr, hip
 // With ev0/ev1 the launch goes through hipExtLaunchKernelGGL, whose events take the begin and
 // end timestamps of the dispatch itself (the kernel's execution time, without the dispatch gap
 // that an event recorded before the launch would include).
-template <int M, typename T>
-static void fim_one(const FimArgs<T> &a, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+template <int M, typename T, int MS>
+static void fim_sweeps(const FimArgs<T> &a, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
     if (ev0)
-        hipExtLaunchKernelGGL((k_fim<M, T, 32>), dim3(a.grid), dim3(256), 0, s, ev0, ev1, 0, a.g, a.sd, a.u,
+        hipExtLaunchKernelGGL((k_fim<M, T, MS>), dim3(a.grid), dim3(256), 0, s, ev0, ev1, 0, a.g, a.sd, a.u,
                               a.fixedmask, a.dep_start, a.deps, a.list, a.count, a.count_reset, a.keys, a.tmax,
                               a.pend_out, a.pend_reset, a.stats);
     else
-        hipLaunchKernelGGL((k_fim<M, T, 32>), dim3(a.grid), dim3(256), 0, s, a.g, a.sd, a.u, a.fixedmask,
+        hipLaunchKernelGGL((k_fim<M, T, MS>), dim3(a.grid), dim3(256), 0, s, a.g, a.sd, a.u, a.fixedmask,
                            a.dep_start, a.deps, a.list, a.count, a.count_reset, a.keys, a.tmax, a.pend_out,
                            a.pend_reset, a.stats);
+}
+
+template <int M, typename T>
+static void fim_one(const FimArgs<T> &a, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+    if (a.max_sweeps <= 8) fim_sweeps<M, T, 8>(a, s, ev0, ev1);
+    else if (a.max_sweeps <= 16) fim_sweeps<M, T, 16>(a, s, ev0, ev1);
+    else fim_sweeps<M, T, 32>(a, s, ev0, ev1);
 }
 
 template <typename T>

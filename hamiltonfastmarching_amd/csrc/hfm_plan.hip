@@ -304,6 +304,7 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
     a.list = p->lists[0];
     a.stats = p->stats;
     a.grid = (unsigned)std::min<long long>((p->g.ntiles + 3) / 4, p->fim_grid > 0 ? p->fim_grid : 1024);
+    a.max_sweeps = p->max_sweeps;
     std::vector<hipEvent_t> tev;
     long long k = 0;
     const int batch = 8;  // even: every batch starts on an even round, so all batches are alike
@@ -924,6 +925,15 @@ int hfm_plan_set_graph(hfm_plan *p, int enable) {
         return 1;
     }
     p->use_graph = enable != 0;
+    return 0;
+}
+
+int hfm_plan_set_max_sweeps(hfm_plan *p, int sweeps) {
+    if (!p) {
+        set_error("hfm_plan_set_max_sweeps: null argument");
+        return 1;
+    }
+    p->max_sweeps = sweeps <= 8 ? 8 : sweeps <= 16 ? 16 : 32;
     return 0;
 }
 

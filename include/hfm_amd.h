@@ -202,6 +202,11 @@ int hfm_plan_set_round_target(hfm_plan *plan, int tiles);
  * (0: 1024).  A tuning knob; results do not depend on it. */
 int hfm_plan_set_grid(hfm_plan *plan, int blocks);
 
+/* Jacobi sweeps a tile visit may run before an unconverged tile is handed back to the
+ * schedule (8, 16 or 32; other values round up; default 8).  A tuning knob: it changes the
+ * work per visit and the number of visits, never the fixed point. */
+int hfm_plan_set_max_sweeps(hfm_plan *plan, int sweeps);
+
 /* Per-launch timing of the update kernel (start/stop events taken from each dispatch's own
  * timestamps; off by default).  Fills fim_ms / fim_launches.  Timed solves run the rounds as
  * individual launches, not as the captured graph. */

@@ -60,6 +60,8 @@ def lib():
                                        i32, P(d), d, d, d, P(d), ctypes.c_long, P(i64), P(ctypes.c_long)]
         L.oracle_forward_variation.argtypes = [i32, P(i32), P(d), P(d), P(d), i32, i32, d, d, P(d), i32, i32, P(d),
                                                P(d), i32, P(d), P(d), P(d), P(d)]
+        L.oracle_set_walls.argtypes = [ctypes.c_long, P(ctypes.c_ubyte), i32]
+        L.oracle_set_walls.restype = None
         L.oracle_geodesics_given.argtypes = [i32, P(i32), P(d), P(d), P(d), i32, i32, d, d, P(d), i32, P(d),
                                              P(ctypes.c_uint64), i32, P(d), d, d, d, P(d), ctypes.c_long, P(i64),
                                              P(ctypes.c_long)]
@@ -149,7 +151,41 @@ def _common_args(params):
     sv = None if sv is None else np.ascontiguousarray(sv, np.float64).ravel()
     xi = float(params.get("xi", 1.0))
     eps = float(params.get("eps", 0.1))
+    _set_walls(params, dims, model)
     return model, dims, dim, N, h, origin, metric, metric_is_field, dual, cost, cost_is_field, seeds, sv, xi, eps
+
+
+def grid_walls(params, dims, model):
+    """The `walls` field as one byte per point (nonzero = wall), or None: a constant, a full
+    field, or for the curvature models an (x, y) field or a theta field broadcast
+    (GetIntegralField, HFMInterface.hxx:33-42)."""
+    if "walls" not in params:
+        return None
+    w = np.asarray(params["walls"], np.float64)
+    if w.ndim == 0:
+        full = np.full(tuple(dims), w != 0)
+    elif w.ndim == len(dims):
+        full = w != 0
+    elif model not in ("Riemann2", "Riemann3") and w.ndim == 2:
+        full = np.broadcast_to((w != 0)[:, :, None], tuple(dims))
+    elif model not in ("Riemann2", "Riemann3") and w.ndim == 1:
+        full = np.broadcast_to((w != 0)[None, None, :], tuple(dims))
+    else:
+        raise RuntimeError("Field walls has incorrect depth.")
+    if full.shape != tuple(dims):
+        raise RuntimeError("Error : Field walls has inconsistent dimensions.")
+    return np.ascontiguousarray(full, np.uint8).ravel()
+
+
+def _set_walls(params, dims, model):
+    w = grid_walls(params, dims, model)
+    refine = int(float(params.get("refineStencilAtWallBoundary", 0)) != 0)
+    refine |= 2 if params.get("_plant_no_wall_walk") else 0  # planted mistake, tests only
+    if w is None:
+        lib().oracle_set_walls(0, None, 0)
+    else:
+        lib().oracle_set_walls(w.size, _ptr(w, ctypes.c_ubyte), refine)
+        _set_walls.keep = w  # the library copies the bytes; keep them alive across the call anyway
 
 
 def solve(params, flow=False, active=False):

@@ -21,6 +21,10 @@
 namespace {
 
 thread_local std::string g_err;
+// Walls for the next problem set up (oracle_set_walls): one byte per point, empty for none.
+thread_local std::vector<uint8_t> g_walls;
+thread_local bool g_refine_walls = false;
+thread_local bool g_plant_no_walk = false;  // planted mistake for the tests: no line walk
 const double kInf = std::numeric_limits<double>::infinity();
 const double kPi = 3.141592653589793238462643383279502884L;  // CommonTraits.h:46 (long double -> double)
 const double kEps = std::numeric_limits<double>::epsilon();
@@ -318,6 +322,10 @@ struct Problem {
         }
     }
     double redim_vec(double v, int i) const { return (shared && i == 2) ? v * dependScale : v * h[i]; }
+    // Obstacles (ExtraAlgorithms/WallObstructionTest.h): empty when there are none.
+    std::vector<uint8_t> walls;
+    bool refineWalls = false;  // refineStencilAtWallBoundary
+    bool plantNoWalk = false;  // planted mistake (tests only): skip the line walk
 };
 
 void setup_problem(Problem &P, int model, const int *dims, const double *gridScales, const double *origin,
@@ -364,6 +372,92 @@ void setup_problem(Problem &P, int model, const int *dims, const double *gridSca
         P.speed.resize(P.N);
         for (long x = 0; x < P.N; ++x) P.speed[x] = 1. / cost[cost_is_field ? x : 0];  // DataSource_Inverse
     }
+    if (!g_walls.empty()) {
+        if ((long)g_walls.size() != P.N) throw OracleError{"Error : Field walls has inconsistent dimensions.\n"};
+        P.walls = g_walls;
+        P.refineWalls = g_refine_walls;
+        P.plantNoWalk = g_plant_no_walk;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Walls (WallObstructionTest.h:38-150).  The distance to the walls is the grid L1 distance
+// (unit steps along the axes, periodic axes wrapped) up to maxDist = MaxStencilWidth, stored
+// as the reference's ShortType int8 (127 = farther); a difference from an accepted point y to
+// the point x = y + v is visible unless the discrete segment from y to x meets a wall.
+// ---------------------------------------------------------------------------------------------
+constexpr int kWallFar = 127;  // std::numeric_limits<int_least8_t>::max()
+
+// ImplementIn (:38-88): a Dijkstra from the walls with unit steps, i.e. a breadth-first search.
+std::vector<int8_t> wall_distance(const Problem &P, int maxWidth) {
+    std::vector<int8_t> dist(P.N, (int8_t)kWallFar);
+    const int maxDist = (int8_t)maxWidth;  // narrowed to ShortType as in the reference
+    std::vector<long> fifo;
+    for (long x = 0; x < P.N; ++x)
+        if (P.walls[x]) {
+            dist[x] = 0;
+            fifo.push_back(x);
+        }
+    for (size_t head = 0; head < fifo.size(); ++head) {
+        const long x = fifo[head];
+        const int nb = dist[x] + 1;
+        if (nb > maxDist) continue;
+        int idx[3] = {0, 0, 0};
+        P.unravel(x, idx);
+        for (int i = 0; i < P.dim; ++i)
+            for (int e = -1; e <= 1; e += 2) {
+                int n[3] = {idx[0], idx[1], idx[2]};
+                n[i] += e;
+                if (!P.periodize(n)) continue;
+                const long y = P.ravel(n);
+                if (dist[y] <= nb) continue;
+                dist[y] = (int8_t)nb;
+                fifo.push_back(y);
+            }
+    }
+    return dist;
+}
+
+// Visible(base, v, sum) (:96-150): base = accepted point, sum = base + v (periodized).
+bool wall_visible(const Problem &P, const std::vector<int8_t> &dist, const int *base, const int *v, long sum) {
+    int wd = dist[sum];
+    if (wd == kWallFar) return true;
+    if (wd == 0) return false;
+    if (P.refineWalls && dist[P.ravel(base)] == 1) return true;
+    int sign[3] = {0, 0, 0};
+    int vSum = 0, vProd = 1, vMax = 0;
+    for (int i = 0; i < P.dim; ++i)
+        if (v[i] != 0) {
+            sign[i] = v[i] > 0 ? 1 : -1;
+            const int a = std::abs(v[i]);
+            vSum += a;
+            vProd *= a;
+            vMax = std::max(vMax, a);
+        }
+    if (vSum <= wd || P.plantNoWalk) return true;
+    // Straight discrete path from base towards sum: axis j steps when its next crossing
+    // (vProd / |v_j| * (2 m + 1)) is reached by the threshold 2 i vProd / vMax.
+    int pos[3] = {base[0], base[1], base[2]};
+    long long next[3], inc[3];
+    for (int i = 0; i < P.dim; ++i) {
+        next[i] = v[i] == 0 ? (long long)std::numeric_limits<int>::max() : vProd / std::abs(v[i]);
+        inc[i] = 2 * next[i];
+    }
+    for (int i = 1; i < vMax; ++i) {
+        const long long threshold = (2LL * i * vProd) / vMax;
+        for (int j = 0; j < P.dim; ++j)
+            if (next[j] <= threshold) {
+                next[j] += inc[j];
+                --vSum;
+                pos[j] += sign[j];
+            }
+        int pp[3] = {pos[0], pos[1], pos[2]};
+        if (!P.periodize(pp)) return true;  // not reached: the segment stays in the domain
+        wd = dist[P.ravel(pp)];
+        if (vSum <= wd) return true;
+        if (wd == 0) return false;
+    }
+    return true;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -450,6 +544,12 @@ void fast_marching(const Problem &P, const std::vector<std::pair<long, double>> 
     std::vector<long> start;
     std::vector<Off3> offs;
     reversed_offsets(P, start, offs);
+    std::vector<int8_t> wdist;
+    if (!P.walls.empty()) {
+        int maxw = 0;  // MaxStencilWidth (HamiltonFastMarching.hxx:37-45)
+        for (const Off3 &e : offs) maxw = std::max(maxw, std::abs(e.o[0]) + std::abs(e.o[1]) + std::abs(e.o[2]));
+        wdist = wall_distance(P, maxw);
+    }
     const clock_t t1 = clock();
     S.values.assign(P.N, kInf);
     S.active.assign(P.N, 0);
@@ -472,6 +572,12 @@ void fast_marching(const Problem &P, const std::vector<std::pair<long, double>> 
             // ConditionalUpdate (HamiltonFastMarching.hxx:136-154)
             const long x = P.shift(y, e.o, -1);
             if (x < 0) continue;
+            if (!wdist.empty()) {  // VisibleOffset (HamiltonFastMarching.hxx:160-170)
+                int yi[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+                P.unravel(y, yi);
+                for (int c = 0; c < P.dim; ++c) v[c] = -e.o[c];
+                if (!wall_visible(P, wdist, yi, v, x)) continue;
+            }
             if (S.accepted[x]) continue;
             if (S.active[x] == 0 && S.values[x] != kInf) continue;  // seeds are never updated
             if (S.values[x] <= top.value) continue;
@@ -836,6 +942,13 @@ std::vector<std::pair<long, double>> make_seeds(const Problem &P, int nseeds, co
 extern "C" {
 
 const char *oracle_last_error(void) { return g_err.c_str(); }
+
+// Walls of the problems set up next on this thread: n bytes (nonzero = wall), n = 0 for none.
+void oracle_set_walls(long n, const unsigned char *walls, int refine) {
+    g_walls.assign(walls, walls + (walls ? n : 0));
+    g_refine_walls = (refine & 1) != 0;
+    g_plant_no_walk = (refine & 2) != 0;
+}
 
 int oracle_riemann_stencils(int dim, const int *dims, const double *gridScales, const double *metric,
                             int metric_is_field, int dual, int8_t *offsets, double *weights) {

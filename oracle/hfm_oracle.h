@@ -37,6 +37,12 @@ enum { ORACLE_RIEMANN2 = 0, ORACLE_RIEMANN3 = 1, ORACLE_RSF2 = 2, ORACLE_ELASTIC
 /* Last error message (empty if none). */
 const char *oracle_last_error(void);
 
+/* Walls (ExtraAlgorithms/WallObstructionTest.h) of the problems set up next on this thread:
+ * n bytes in row-major point order, nonzero = wall; n = 0 for none.  refine bit 0 =
+ * refineStencilAtWallBoundary; bit 1 plants a mistake for the tests (the line walk of the
+ * visibility test is skipped). */
+void oracle_set_walls(long n, const unsigned char *walls, int refine);
+
 /* Selling stencils of a Riemannian metric field.
  *   dim: 2 or 3; dims[dim]; gridScales[dim];
  *   metric: N x K (K = 3 or 6, components xx,xy,yy[,xz,yz,zz]); dual=1 if it is the dual metric.

@@ -212,6 +212,56 @@ def forward_cases():
     print("forward variation ok")
 
 
+def wall_fields():
+    """Obstacle fields of the wall cases (WallObstructionTest.h), deterministic."""
+    w2 = np.zeros((48, 48))
+    w2[6:40, 31] = 1        # a long wall right of the seed
+    w2[10:14, 10:16] = 1    # a block
+    w3 = np.zeros((20, 20, 20))
+    w3[14, :, :] = 1        # a plane with a square hole
+    w3[14, 8:12, 6:10] = 0
+    wxy = np.zeros((24, 24))  # (x, y) field, broadcast over theta
+    wxy[4:18, 16] = 1
+    wxy[17, 6:17] = 1
+    rng = np.random.Generator(np.random.PCG64(99))
+    we = (rng.random((20, 20, 16)) < 0.04).astype(float)
+    we[9:12, 9:12, :] = 0   # keep the seed's neighbourhood open
+    return {"riemann2": w2, "riemann3": w3, "rsf2": wxy, "elastica2": we}
+
+
+WALL_CASES = {
+    "riemann2": (lambda: syn.riemann2(48, seed=1), {}),
+    "riemann2_refine": (lambda: syn.riemann2(48, seed=1), {"refineStencilAtWallBoundary": 1.0}),
+    "riemann3": (lambda: syn.riemann3(20, seed=2), {}),
+    "rsf2": (lambda: syn.rsf2(24, 16, seed=3), {}),
+    "elastica2": (lambda: syn.elastica2(20, 16, seed=4), {}),
+}
+
+
+def wall_cases():
+    """values / activeNeighs / geodesicFlow with walls (WallObstructionTest.h:38-150), and
+    geodesics from tips behind the walls."""
+    fields = wall_fields()
+    out = {}
+    for name, (make, extra) in WALL_CASES.items():
+        p = {k: v for k, v in make().items()
+             if k not in ("tips", "inspectSensitivity", "inspectSensitivityWeights", "inspectSensitivityLengths")}
+        base = name.split("_")[0]
+        walls = fields[base]
+        dim = len(p["dims"])
+        tips = {"riemann2": [[0.9, 0.5], [0.7, 0.3]], "riemann3": [[0.9, 0.5, 0.5], [0.95, 0.1, 0.9]],
+                "rsf2": [[0.8, 0.2, 1.0], [0.5, 0.9, 4.0]], "elastica2": [[0.2, 0.8, 0.5], [0.85, 0.15, 2.0]]}[base]
+        q = dict(p, walls=walls, exportActiveNeighs=1.0, exportGeodesicFlow=1.0, tips=np.array(tips), **extra)
+        r = oracle.run_reference(q)
+        assert r["retcode"] == 0, r["log"]
+        out[name + "_walls"] = walls
+        for k in ("values", "activeNeighs", "geodesicFlow", "geodesicPoints", "geodesicLengths", "MaxStencilWidth"):
+            out[name + "_" + k] = np.asarray(r[k])
+        assert np.asarray(r["geodesicPoints"]).shape[1] == dim
+    np.savez_compressed(os.path.join(HERE, "walls.npz"), **out)
+    print("walls ok")
+
+
 def big_case():
     p = syn.riemann3(128, seed=0)
     r = oracle.run_reference(p)
@@ -230,6 +280,12 @@ def main():
     for m in ("Riemann2", "Riemann3", "ReedsSheppForward2", "Elastica2"):
         if not oracle.ref_available(m):
             sys.exit(f"reference binary for {m} missing: run `make -C oracle ref` first")
+    only = [a for a in sys.argv[1:] if not a.startswith("--")]
+    if only:  # regenerate the named fixture groups only
+        groups = {"geodesics": geodesic_cases, "forward": forward_cases, "walls": wall_cases, "big": big_case}
+        for a in only:
+            groups[a]()
+        return
     riemann_case("riemann2_64", syn.riemann2(64, seed=1))
     riemann_case("riemann3_24", syn.riemann3(24, seed=2))
     ties_case()
@@ -237,6 +293,7 @@ def main():
     curvature_case("elastica2_24x24x16", syn.elastica2(24, 16, seed=4), sensitivity=True)
     geodesic_cases()
     forward_cases()
+    wall_cases()
     if "--big" in sys.argv:
         big_case()
 

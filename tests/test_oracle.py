@@ -445,3 +445,67 @@ def test_forward_variation_finite_differences(name):
     _, vs = oracle.forward_variation(p, None, np.ones((1, ns)))
     reached = np.isfinite(oracle.solve(p)["values"])
     assert np.allclose(vs[..., 0][reached], 1.0, atol=1e-12)
+
+
+# ------------------------------------------------------------------------------------------
+# Walls (WallObstructionTest.h:38-150)
+# ------------------------------------------------------------------------------------------
+WALLS = {
+    "riemann2": (lambda: syn.riemann2(48, seed=1), "riemann2", {}),
+    "riemann2_refine": (lambda: syn.riemann2(48, seed=1), "riemann2", {"refineStencilAtWallBoundary": 1.0}),
+    "riemann3": (lambda: syn.riemann3(20, seed=2), "riemann3", {}),
+    "rsf2": (lambda: syn.rsf2(24, 16, seed=3), "rsf2", {}),
+    "elastica2": (lambda: syn.elastica2(20, 16, seed=4), "elastica2", {}),
+}
+WALL_TIPS = {"riemann2": [[0.9, 0.5], [0.7, 0.3]], "riemann3": [[0.9, 0.5, 0.5], [0.95, 0.1, 0.9]],
+             "rsf2": [[0.8, 0.2, 1.0], [0.5, 0.9, 4.0]], "elastica2": [[0.2, 0.8, 0.5], [0.85, 0.15, 2.0]]}
+
+
+def wall_params(name):
+    g = load("walls")
+    make, base, extra = WALLS[name]
+    return dict(_strip(make()), walls=g[name + "_walls"], **extra), g, base
+
+
+@pytest.mark.parametrize("name", list(WALLS))
+def test_walls_golden(name):
+    """values, activeNeighs, geodesicFlow and geodesics with walls equal the reference's."""
+    p, g, base = wall_params(name)
+    r = oracle.solve(p, flow=True, active=True)
+    assert np.array_equal(r["values"], g[name + "_values"])
+    assert np.array_equal(r["activeNeighs"].astype(np.float64), g[name + "_activeNeighs"])
+    assert np.array_equal(r["geodesicFlow"], g[name + "_geodesicFlow"])
+    pts, lens = oracle.geodesics(p, WALL_TIPS[base])
+    assert np.array_equal(lens, g[name + "_geodesicLengths"].astype(np.int64))
+    assert np.array_equal(pts, g[name + "_geodesicPoints"])
+    # the walls matter here: without them the values differ
+    assert not np.array_equal(oracle.solve(_strip(WALLS[name][0]()))["values"], r["values"])
+
+
+def test_walls_refine_changes_the_result():
+    """refineStencilAtWallBoundary lets differences from points next to a wall through."""
+    g = load("walls")
+    assert not np.array_equal(g["riemann2_values"], g["riemann2_refine_values"])
+
+
+def test_walls_identities():
+    """No wall cells = no walls; wall cells are never reached; a closed box around the seed leaves
+    everything outside unreached; a planted mistake (walls ignored beyond the first step of the
+    line walk) changes the fixture's result."""
+    p = _strip(syn.riemann2(32, seed=5))
+    u0 = oracle.solve(p)["values"]
+    assert np.array_equal(oracle.solve(dict(p, walls=np.zeros((32, 32))))["values"], u0)
+    w = np.zeros((32, 32))
+    w[8, 8:25] = w[24, 8:25] = w[8:25, 8] = w[8:25, 24] = 1
+    u = oracle.solve(dict(p, walls=w))["values"]
+    inside = np.zeros((32, 32), bool)
+    inside[9:24, 9:24] = True
+    assert np.all(np.isinf(u[w != 0]))
+    assert np.all(np.isinf(u[~inside & (w == 0)]))
+    assert np.all(np.isfinite(u[inside]))
+    # planted mistake in the oracle: the visibility test without its line walk (only the end
+    # point's distance to the walls) disagrees with the reference's fixtures
+    for name in ("riemann2", "riemann3"):
+        q, g, _ = wall_params(name)
+        bad = oracle.solve(dict(q, _plant_no_wall_walk=True))["values"]
+        assert not np.array_equal(bad, g[name + "_values"])
