@@ -208,6 +208,23 @@ class Plan:
                                               _stream_ptr(torch)))
         return self
 
+    def set_walls(self, walls, refine: bool = False):
+        """Obstacles (walls key): a boolean/numeric field of the grid's shape (nonzero = wall), or
+        None to remove them; refine = refineStencilAtWallBoundary."""
+        torch = _torch()
+        if walls is None:
+            self._check(self._L.hfm_plan_set_walls(self._h, None, int(refine), _stream_ptr(torch)))
+            self._walls = None
+            return self
+        w = torch.as_tensor(walls).to("cuda")
+        if tuple(w.shape) != tuple(self.dims):
+            raise PyExp("Error : Field walls has inconsistent dimensions.")
+        w = (w != 0).to(torch.uint8).contiguous()
+        self._check(self._L.hfm_plan_set_walls(self._h, ctypes.c_void_p(w.data_ptr()), int(refine),
+                                               _stream_ptr(torch)))
+        self._walls = w  # the copy is made on the stream; keep the source alive until then
+        return self
+
     def solve(self, seeds, seed_values=None):
         torch = _torch()
         s = np.ascontiguousarray(np.asarray(seeds, np.float64).reshape(-1, self.dim))

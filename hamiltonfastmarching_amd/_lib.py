@@ -85,6 +85,7 @@ SIGNATURES = [
     ("hfm_plan_set_round_target", c_int, [c_void_p, c_int]),
     ("hfm_plan_set_grid", c_int, [c_void_p, c_int]),
     ("hfm_plan_set_max_sweeps", c_int, [c_void_p, c_int]),
+    ("hfm_plan_set_walls", c_int, [c_void_p, c_void_p, c_int, c_void_p]),
     ("hfm_plan_get_stencils", c_int, [c_void_p, P(ctypes.c_int8), P(c_double)]),
     ("hfm_server_create", c_int, [P(c_void_p), c_double, c_int]),
     ("hfm_server_submit", c_int, [c_void_p, P(HfmRequest), P(c_int64)]),

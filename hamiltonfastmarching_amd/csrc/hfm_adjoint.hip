@@ -51,11 +51,11 @@ __device__ inline int active_set(const Geom &g, const StencilData<T> &sd, const 
         int r[3];
         T va = inf, vb = inf;
         long long ia = -1, ib = -1;
-        if (shift_coords_m<ModelT>(g, c, o0, o1, o2, r)) {
+        if (st.visible(k, 0) && shift_coords_m<ModelT>(g, c, o0, o1, o2, r)) {
             ia = (long long)tile_of(g, r) * 64 + local_of(g, r);
             va = u[ia];
         }
-        if (k >= NF && shift_coords_m<ModelT>(g, c, -o0, -o1, -o2, r)) {
+        if (k >= NF && st.visible(k, 1) && shift_coords_m<ModelT>(g, c, -o0, -o1, -o2, r)) {
             ib = (long long)tile_of(g, r) * 64 + local_of(g, r);
             vb = u[ib];
         }

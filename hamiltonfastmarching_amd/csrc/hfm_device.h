@@ -146,7 +146,14 @@ template <typename T> struct StencilData {
     const double *__restrict__ tab_w;
     const T *__restrict__ speed;  // per point, tiled
     long long np;                 // number of tiled points (ntiles * 64)
+    // Walls: per point (tiled), bit side_bit(k, s) set when side s of difference k is visible
+    // (WallObstructionTest.h:96-150); null when there are no walls (everything visible).
+    const uint32_t *__restrict__ vis = nullptr;
 };
+
+// Bit of side s (0: x + e_k, 1: x - e_k) of difference k, in the reference's activeNeighs layout
+// (EulerianStencil.hxx:114-141): forward differences first, then two bits per symmetric one.
+template <int NF> __host__ __device__ constexpr int side_bit(int k, int s) { return k < NF ? k : NF + 2 * (k - NF) + s; }
 
 template <typename T> __device__ inline T dev_inf();
 template <> __device__ inline float dev_inf<float>() { return __builtin_huge_valf(); }
@@ -197,8 +204,12 @@ template <int ModelT, typename T> struct LaneStencil {
     static constexpr int NC = S::NFWD + S::NSYM;
     uint32_t off[NC];
     T w[NC];
+    uint32_t vis;  // visible sides (side_bit layout); all ones without walls
+
+    __device__ bool visible(int k, int s) const { return (vis >> side_bit<S::NFWD>(k, s)) & 1u; }
 
     __device__ void load(const Geom &g, const StencilData<T> &sd, long long p, const int *c) {
+        vis = sd.vis ? sd.vis[p] : 0xffffffffu;
         if (!S::SHARED) {
 #pragma unroll
             for (int k = 0; k < NC; ++k) {

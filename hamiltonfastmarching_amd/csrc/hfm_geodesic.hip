@@ -178,8 +178,10 @@ __device__ int discrete_flow(const Geom &g, const StencilData<T> &sd, const T *u
             return false;
         };
         const bool nz = L.s_w[k] != 0.;
-        const bool pa = nz || (S::SHARED && shares(o0, o1, o2));
-        const bool pb = k >= NF && (nz || (S::SHARED && shares(-o0, -o1, -o2)));
+        // walls: sides that are not visible never enter the active set
+        const uint32_t vm = sd.vis ? sd.vis[p] : 0xffffffffu;
+        const bool pa = (vm >> side_bit<NF>(k, 0) & 1u) && (nz || (S::SHARED && shares(o0, o1, o2)));
+        const bool pb = k >= NF && (vm >> side_bit<NF>(k, 1) & 1u) && (nz || (S::SHARED && shares(-o0, -o1, -o2)));
         T va = inf, vb = inf;
         int r[3];
         if (pa && shift_coords_m<ModelT>(g, c, o0, o1, o2, r)) va = u[(long long)tile_of(g, r) * 64 + local_of(g, r)];

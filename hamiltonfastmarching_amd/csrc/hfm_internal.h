@@ -102,6 +102,14 @@ struct hfm_plan {
     long long *fwd_sp = nullptr;
     int fwd_seed_cap = 0;
 
+    // walls (hfm_plan_set_walls): row-major bytes, the visibility mask derived from them and the
+    // current stencils (rebuilt when either changes), scratch for the wall distances
+    uint8_t *walls = nullptr;
+    int walls_refine = 0;
+    uint32_t *vis = nullptr;
+    int8_t *wdist[2] = {nullptr, nullptr};
+    bool vis_dirty = true;
+
     bool stencil_ready = false;
     bool solved = false;
     bool timing = false;
@@ -125,6 +133,7 @@ struct hfm_plan {
         s.tab_w = tab_w;
         s.speed = static_cast<const T *>(speed);
         s.np = np;
+        s.vis = walls ? vis : nullptr;
         return s;
     }
 };

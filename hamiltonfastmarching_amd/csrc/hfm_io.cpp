@@ -10,8 +10,10 @@
 //   RunSolver     FMCPUTime (wall-clock seconds of the GPU solve here)     (:422-450)
 //   ExtractGeodesics  tips -> geodesicPoints, geodesicLengths, GeodesicCPUTime (:452-523)
 //   ExportData    values, activeNeighs, geodesicFlow, activeOffsets         (:526-579)
-//   Finally       inspectSensitivity -> costSensitivity_<i>, seedSensitivity_<i>
-//                 (Headers/ExtraAlgorithms/FirstVariation.h:60-102)
+//   Extra algorithms  walls, refineStencilAtWallBoundary (ExtraAlgorithms/WallObstructionTest.h:30-36)
+//   Finally       inspectSensitivity -> costSensitivity_<i>, seedSensitivity_<i>;
+//                 costVariation, seedValueVariation -> valueVariation
+//                 (Headers/ExtraAlgorithms/FirstVariation.h:60-124)
 // Arrays are FP64, stored row-major with element components last; arrayOrdering permutes the
 // grid axes exactly like IO_::GetArray / SetArray (JMM_CPPLibs/JMM_CPPLibs/Output/IO.hxx:68-110,
 // 153-240): n = P(s) with P = identity, T (swap axes 0,1), R (reverse) or T o R.
@@ -311,6 +313,18 @@ void run_impl(hfm_io *io) {
         field = grid_field(io, "cost", dims, 1, true, is_field);
     }
     if (precision != 32 && precision != 64) throw IOError{"precision must be 32 or 64"};
+    // Walls (WallObstructionTest.h:30-36): an integral field read as bool (nonzero = wall),
+    // constant, full, or for the curvature models over (x, y) or theta only.
+    std::vector<uint8_t> walls;
+    if (io->has("walls")) {
+        bool wfield = false;
+        const std::vector<double> w = grid_field(io, "walls", dims, 1, bundle, wfield);
+        int64_t n = 1;
+        for (auto d : dims) n *= d;
+        walls.resize((size_t)n);
+        for (int64_t i = 0; i < n; ++i) walls[i] = w[wfield ? i : 0] != 0. ? 1 : 0;
+    }
+    const bool refineWalls = io->scalar("refineStencilAtWallBoundary", 0.) != 0.;
 
     PlanGuard pg;
     check(hfm_plan_create(&pg.p, model.c_str(), dims.data(), h, origin.empty() ? nullptr : origin.data(), precision,
@@ -326,6 +340,13 @@ void run_impl(hfm_io *io) {
             throw IOError{"hipMemcpy failed"};
         if (riemann) check(hfm_plan_set_metric(plan, dm.d(), is_field ? 1 : 0, hasDual ? 1 : 0, nullptr));
         else check(hfm_plan_set_cost(plan, dm.d(), is_field ? 1 : 0, nullptr));
+    }
+    if (!walls.empty()) {
+        DevBuf dw(walls.size());
+        if (hipMemcpy(dw.p, walls.data(), walls.size(), hipMemcpyHostToDevice) != hipSuccess)
+            throw IOError{"hipMemcpy failed"};
+        check(hfm_plan_set_walls(plan, static_cast<const uint8_t *>(dw.p), refineWalls ? 1 : 0, nullptr));
+        if (hipDeviceSynchronize() != hipSuccess) throw IOError{"hipDeviceSynchronize failed"};
     }
     const double t1 = wall();
     io->set_scalar("StencilCPUTime", t1 - t0);

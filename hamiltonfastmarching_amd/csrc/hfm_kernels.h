@@ -78,6 +78,14 @@ hipError_t launch_riemann_stencil(const Geom &g, const double *metric, int is_fi
 template <typename T>
 hipError_t launch_tile_deps(int model, const Geom &g, const StencilData<T> &sd, int pass, int *cnt, const int *start,
                             int *fill, int *deps, int *ovf, hipStream_t s);
+// Walls (hfm_walls.hip): per-point visible-side bitmask (side_bit layout) from a row-major byte
+// field of walls, the distance to the walls being capped at maxDist (MaxStencilWidth narrowed to
+// int8); dist0 / dist1: np bytes of scratch each.
+template <typename T>
+hipError_t launch_wall_visibility(int model, const Geom &g, const StencilData<T> &sd, const uint8_t *walls,
+                                  int maxDist, int refine, int8_t *dist0, int8_t *dist1, uint32_t *vis, long long np,
+                                  hipStream_t s);
+
 // Exclusive scan of n counts into start[0..n].  With work (scan_work_ints(n) ints of device
 // memory) long arrays are scanned by one block per 4096 counts; without, by a single block.
 long long scan_work_ints(long long n);

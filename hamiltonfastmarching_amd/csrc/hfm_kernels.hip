@@ -406,8 +406,10 @@ __global__ __launch_bounds__(256) void k_flow(Geom g, StencilData<T> sd, const T
         // (HFM_StencilRecompute.hxx:92, HFM_StencilShare.hxx:153), but its acceptance activates
         // every difference with that offset (EulerianStencil.hxx:114-137): a zero-weight
         // difference is active only if a nonzero-weight one shares its (signed) offset.
-        const bool pa = st.w[k] != T(0) || shares_offset<ModelT>(st, k, o0, o1, o2);
-        const bool pb = k >= NF && (st.w[k] != T(0) || shares_offset<ModelT>(st, k, -o0, -o1, -o2));
+        // Walls: an invisible side is never updated from (VisibleOffset).
+        const bool pa = st.visible(k, 0) && (st.w[k] != T(0) || shares_offset<ModelT>(st, k, o0, o1, o2));
+        const bool pb = k >= NF && st.visible(k, 1) &&
+                        (st.w[k] != T(0) || shares_offset<ModelT>(st, k, -o0, -o1, -o2));
         int r[3];
         T va = inf, vb = inf;
         if (pa && shift_coords_m<ModelT>(g, c, o0, o1, o2, r)) va = u[(long long)tile_of(g, r) * 64 + local_of(g, r)];

@@ -105,6 +105,10 @@ void plan_free(hfm_plan *p) {
     dfree(p->adj_seed_w);
     dfree(p->adj_tp);
     dfree(p->adj_tw);
+    dfree(p->walls);
+    dfree(p->vis);
+    dfree(p->wdist[0]);
+    dfree(p->wdist[1]);
     dfree(p->fwd_pending);
     dfree(p->fwd_rcount);
     dfree(p->fwd_rstart);
@@ -574,6 +578,7 @@ int hfm_plan_set_metric(hfm_plan *p, const double *d_metric, int is_field, int d
         p->st.max_stencil_width = p->max_stencil_width;
         p->st.n_deps = p->ndeps;
         p->stencil_ready = true;
+        p->vis_dirty = true;
     } catch (const HipFail &e) {
         set_error(e.msg);
         return 1;
@@ -640,6 +645,7 @@ int hfm_plan_set_cost(hfm_plan *p, const double *d_cost, int is_field, void *str
         p->st.max_stencil_width = maxw;
         p->st.n_deps = p->ndeps;
         p->stencil_ready = true;
+        p->vis_dirty = true;
     } catch (const HipFail &e) {
         set_error(e.msg);
         return 1;
@@ -675,6 +681,22 @@ int hfm_plan_solve(hfm_plan *p, int nseeds, const double *seeds, const double *s
         HIPCHECK(hipMemcpyAsync(p->seed_v, vals.data(), vals.size() * sizeof(double), hipMemcpyHostToDevice, s));
         p->last_seeds = tp;
         p->seed_inputs = inputs;
+        if (p->walls && p->vis_dirty) {
+            if (!p->vis) {
+                p->vis = dalloc<uint32_t>(p->np);
+                p->wdist[0] = dalloc<int8_t>(p->np);
+                p->wdist[1] = dalloc<int8_t>(p->np);
+            }
+            // maxDist = MaxStencilWidth as the reference's int8 ShortType (WallObstructionTest.h:47)
+            const int maxDist = (int)(int8_t)p->max_stencil_width;
+            if (p->precision == 32)
+                HIPCHECK(launch_wall_visibility<float>(p->model, p->g, p->sd<float>(), p->walls, maxDist,
+                                                       p->walls_refine, p->wdist[0], p->wdist[1], p->vis, p->np, s));
+            else
+                HIPCHECK(launch_wall_visibility<double>(p->model, p->g, p->sd<double>(), p->walls, maxDist,
+                                                        p->walls_refine, p->wdist[0], p->wdist[1], p->vis, p->np, s));
+            p->vis_dirty = false;
+        }
         if (p->precision == 32) run_fim<float>(p, s);
         else run_fim<double>(p, s);
         p->solved = true;
@@ -925,6 +947,31 @@ int hfm_plan_set_graph(hfm_plan *p, int enable) {
         return 1;
     }
     p->use_graph = enable != 0;
+    return 0;
+}
+
+int hfm_plan_set_walls(hfm_plan *p, const uint8_t *d_walls, int refine, void *stream) {
+    if (!p) {
+        set_error("hfm_plan_set_walls: null argument");
+        return 1;
+    }
+    try {
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        if (!d_walls) {
+            dfree(p->walls);
+            p->walls = nullptr;
+        } else {
+            long long n = 1;
+            for (int i = 0; i < p->g.dim; ++i) n *= p->g.n[i];
+            if (!p->walls) p->walls = dalloc<uint8_t>(n);
+            HIPCHECK(hipMemcpyAsync(p->walls, d_walls, (size_t)n, hipMemcpyDeviceToDevice, s));
+        }
+        p->walls_refine = refine != 0;
+        p->vis_dirty = true;
+    } catch (const HipFail &e) {
+        set_error(e.msg);
+        return 1;
+    }
     return 0;
 }
 

@@ -193,13 +193,13 @@ __device__ inline int fim_tile(const TileCtx<T> &ctx, int tile_in, int lane, T *
         T oa = inf, ob = inf;
         if (!fixed && st.w[k] != T(0)) {
             int r[3];
-            if (shift_coords_m<ModelT>(g, c, o0, o1, o2, r)) {
+            if (st.visible(k, 0) && shift_coords_m<ModelT>(g, c, o0, o1, o2, r)) {
                 const int t = tile_of(g, r), l = local_of(g, r);
                 if (t == tile) ia[k] = l;
                 else oa = u[(long long)t * 64 + l];
             }
             if (k >= NF) {
-                if (shift_coords_m<ModelT>(g, c, -o0, -o1, -o2, r)) {
+                if (st.visible(k, 1) && shift_coords_m<ModelT>(g, c, -o0, -o1, -o2, r)) {
                     const int t = tile_of(g, r), l = local_of(g, r);
                     if (t == tile) ib[k] = l;
                     else ob = u[(long long)t * 64 + l];

@@ -128,6 +128,14 @@ int hfm_plan_set_metric(hfm_plan *plan, const double *d_metric, int is_field, in
  * speed = 1/cost (HFM_StencilShare.hxx:61-71). */
 int hfm_plan_set_cost(hfm_plan *plan, const double *d_cost, int is_field, void *stream);
 
+/* Walls (WallObstructionTest.h): d_walls = device bytes in row-major point order, nonzero = wall,
+ * or NULL to remove them; refine = refineStencilAtWallBoundary.  A difference from an accepted
+ * point to the updated one is used only if the discrete segment between them meets no wall
+ * (Visible, WallObstructionTest.h:96-150; the distance to the walls is capped at
+ * MaxStencilWidth).  Applies to the next solves and to everything computed from them (flow,
+ * activeNeighs, geodesics, adjoint, forward variation); kept across set_metric / set_cost. */
+int hfm_plan_set_walls(hfm_plan *plan, const uint8_t *d_walls, int refine, void *stream);
+
 /* Solve from seeds (host, physical coordinates, nseeds x dim; seedValues NULL -> 0), the
  * seed placement of StaticFactoring::SetSeeds (StaticFactoring.hxx:242-393).  Runs the
  * block fast-iterative method to its fixed point on the plan's stream. */

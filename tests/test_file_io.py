@@ -97,3 +97,14 @@ def test_executable_vs_golden(tmp_path, model, name, make):
     assert np.max(np.abs(out["geodesicPoints"] - geo[key + "_points"])) <= 1e-9
     for k in ("FMCPUTime", "StencilCPUTime", "GeodesicCPUTime"):
         assert k in out
+
+
+def test_walls_input_errors():
+    """The walls field is validated on the host before any device work (GetIntegralField,
+    HFMInterface.hxx:33-42,288-292): wrong extent or depth is the reference's error."""
+    import hamiltonfastmarching_amd as hfm
+    p = syn.riemann2(16, seed=3)
+    with pytest.raises(hfm.PyExp, match="walls has inconsistent dimensions"):
+        hfm.run(dict(p, walls=np.zeros((16, 15))))
+    with pytest.raises(hfm.PyExp, match="walls has incorrect depth"):
+        hfm.run(dict(p, walls=np.zeros((16, 16, 2))))
