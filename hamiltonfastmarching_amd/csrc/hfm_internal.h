@@ -142,6 +142,11 @@ int hfm_plan_adjoint_impl(hfm_plan *p, int ntargets, const double *targets, cons
                           double *d_sensitivity, int64_t *seed_index, double *seed_weight, int max_seeds,
                           int *nseed_out, void *stream);
 
+// hfm_plan_geodesics with tips already in grid units (ADim'd) when tips_adim.
+int hfm_plan_geodesics_impl(hfm_plan *p, int ntips, const double *tips, bool tips_adim, double step,
+                            double threshold, double volumeBound, double *points, int64_t max_points,
+                            int64_t *lengths, int64_t *npoints, int *nfailed, void *stream);
+
 int hfm_plan_forward_variation_impl(hfm_plan *p, int nvar, const double *d_cost_variation, int nseeds,
                                     const double *seed_value_variation, double *d_value_variation, void *stream);
 

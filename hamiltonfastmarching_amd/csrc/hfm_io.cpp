@@ -32,6 +32,12 @@
 #include <vector>
 
 #include "../../include/hfm_amd.h"
+#include "selling.h"
+
+// Geodesics from tips already in grid units (hfm_plan.hip).
+int hfm_plan_geodesics_impl(hfm_plan *p, int ntips, const double *tips, bool tips_adim, double step,
+                            double threshold, double volumeBound, double *points, int64_t max_points,
+                            int64_t *lengths, int64_t *npoints, int *nfailed, void *stream);
 
 namespace {
 
@@ -255,6 +261,179 @@ void set_grid_array(hfm_io *io, const std::string &key, const std::vector<double
     io->set_computed(key, std::move(f));
 }
 
+// Seed spreading (StaticFactoring.hxx:284-345), Riemann models: every grid point within
+// `radius` (grid units) of a seed p, and p's own cell, becomes a seed with value
+// v + (|p - q|_{M(p)} + |p - q|_{M(q)}) / 2, where M(p) is the metric interpolated at p from the
+// 2^d surrounding grid points (BaseGrid.h:80-100, out-of-domain corners weighted 0 without
+// renormalisation, PeriodicGrid.hxx:28-40; MapWeightedSum.h) and M(q) the metric at q, both primal
+// and scaled by h_i h_j (Riemannian.h:62-71).  Candidates come from the box of half-width
+// 1 + ceil(radius) in row-major order; a point is taken once per seed.  `seeds` (physical) and
+// `values` are replaced; spreadedSeeds / spreadedSeedValues are exported.
+template <int D>
+void spread_seeds_d(hfm_io *io, const std::vector<int64_t> &dims, const double *h, const std::vector<double> &origin,
+                    const std::vector<double> &metric, bool is_field, bool dual, double radius,
+                    std::vector<double> &seeds, std::vector<double> &values) {
+    using Sym = hfm::SymPacked<D>;
+    constexpr int K = Sym::K;
+    auto org = [&](int i) { return origin.empty() ? 0. : origin[i]; };
+    auto in_domain = [&](const int64_t *idx) {
+        for (int i = 0; i < D; ++i)
+            if (idx[i] < 0 || idx[i] >= dims[i]) return false;
+        return true;
+    };
+    auto metric_at = [&](const int64_t *idx) {
+        Sym m;
+        int64_t lin = 0;
+        for (int i = 0; i < D; ++i) lin = lin * dims[i] + idx[i];
+        for (int k = 0; k < K; ++k) m.m[k] = metric[(is_field ? lin : 0) * K + k];
+        return m;
+    };
+    // Rescale<false>: primal metric times the rank-one tensor h h^T, component by component
+    auto rescale = [&](Sym m) {
+        if (dual) m = hfm::sym_inverse<D>(m);
+        for (int i = 0; i < D; ++i)
+            for (int j = 0; j <= i; ++j) m.m[Sym::lin(i, j)] *= h[i] * h[j];
+        return m;
+    };
+    auto norm = [&](const Sym &m, const double *v) {  // sqrt(<v, M v>) in the reference's order
+        double sum = 0.;
+        for (int i = 0; i < D; ++i) {
+            double sumi = 0.;
+            for (int j = 0; j < D; ++j) sumi += m(i, j) * v[j];
+            sum += v[i] * sumi;
+        }
+        return std::sqrt(sum);
+    };
+    const int n_in = (int)(seeds.size() / D);
+    std::vector<double> newPts, newVals;
+    const int64_t bd = 1 + (int64_t)std::ceil(radius);
+    const int64_t side = 1 + 2 * bd;
+    int64_t boxn = 1;
+    for (int i = 0; i < D; ++i) boxn *= side;
+    for (int s = 0; s < n_in; ++s) {
+        double p[D];
+        for (int i = 0; i < D; ++i) p[i] = (seeds[(size_t)s * D + i] - org(i)) / h[i];
+        const double value = values[s];
+        // GetGuess(p): interpolated metric
+        int64_t nearest[D];
+        int dir[D];
+        double wgt[D];
+        for (int i = 0; i < D; ++i) {
+            nearest[i] = (int64_t)std::floor(p[i]);
+            const double diff = p[i] - (double)nearest[i] - 0.5;
+            dir[i] = diff > 0 ? 1 : -1;
+            wgt[i] = std::abs(diff);
+        }
+        Sym mp{};
+        for (int c = 0; c < (1 << D); ++c) {
+            int64_t q[D];
+            double w = 1.;
+            for (int j = 0; j < D; ++j) {
+                q[j] = nearest[j];
+                if (c >> j & 1) {
+                    w *= wgt[j];
+                    q[j] += dir[j];
+                } else {
+                    w *= 1. - wgt[j];
+                }
+            }
+            if (!in_domain(q)) w = 0.;
+            if (c == 0) {
+                // the reference evaluates the first corner unconditionally; outside the domain there is
+                // no metric, and no corner at all is valid (PeriodicGrid.hxx:37-38)
+                if (!in_domain(q)) throw IOError{"Error : point has no valid neighbors in the grid domain"};
+                const Sym m = metric_at(q);
+                for (int k = 0; k < K; ++k) mp.m[k] = w * m.m[k];
+            } else if (w != 0.) {
+                const Sym m = metric_at(q);
+                for (int k = 0; k < K; ++k) mp.m[k] += w * m.m[k];
+            }
+        }
+        const Sym distp = rescale(mp);
+        std::set<std::vector<int64_t>> seen;
+        auto insert = [&](const int64_t *index, double rad) {
+            double v[D], q[D];
+            for (int i = 0; i < D; ++i) {
+                q[i] = (double)index[i] + 0.5;
+                v[i] = p[i] - q[i];
+            }
+            double e2 = 0.;
+            for (int i = 0; i < D; ++i) e2 += v[i] * v[i];
+            if (std::sqrt(e2) >= rad) return;
+            if (!seen.insert(std::vector<int64_t>(index, index + D)).second) return;
+            if (!in_domain(index)) return;
+            const Sym distq = rescale(metric_at(index));
+            for (int i = 0; i < D; ++i) newPts.push_back(q[i]);
+            newVals.push_back(value + 0.5 * (norm(distp, v) + norm(distq, v)));
+        };
+        insert(nearest, std::numeric_limits<double>::infinity());
+        for (int64_t b = 0; b < boxn; ++b) {
+            int64_t idx[D];
+            int64_t r = b;
+            for (int i = D - 1; i >= 0; --i) {
+                idx[i] = nearest[i] + (r % side) - bd;
+                r /= side;
+            }
+            insert(idx, radius);
+        }
+    }
+    const size_t n = newVals.size();
+    seeds.resize(n * D);
+    Field fp, fv;
+    fp.dims = {(int64_t)n, (int64_t)D};
+    for (size_t k = 0; k < n; ++k)
+        for (int i = 0; i < D; ++i) {
+            const double x = newPts[k * D + i] * h[i] + org(i);  // ReDim
+            seeds[k * D + i] = x;
+            fp.data.push_back(x);
+        }
+    values = newVals;
+    fv.dims = {(int64_t)n};
+    fv.data = newVals;
+    io->set_computed("spreadedSeeds", std::move(fp));
+    io->set_computed("spreadedSeedValues", std::move(fv));
+}
+
+// seedValueVariation is given for the `seeds` as passed; the solver may run on more seeds
+// (spread or unoriented).  Each solver seed takes the variation of the last given seed at its
+// grid point (FirstVariation.h:110-119 writes them in order), zero otherwise.
+std::vector<double> seed_variation_for_solver(const std::vector<double> &svv, int64_t nvar, int nin,
+                                              const std::vector<double> &given, const std::vector<double> &solver,
+                                              int dim, const std::vector<int64_t> &dims, const double *h,
+                                              const std::vector<double> &origin, bool bundle, double thetaStep) {
+    auto index_of = [&](const double *pt, std::vector<int64_t> &idx) {
+        idx.assign(dim, 0);
+        for (int i = 0; i < dim; ++i) {
+            const bool th = bundle && i == 2;
+            const double o = th ? -thetaStep / 2. : (origin.empty() ? 0. : origin[i]);
+            const double q = std::floor((pt[i] - o) / (th ? thetaStep : h[i]));
+            if (!(q > -2e9 && q < 2e9)) return false;
+            int64_t c = (int64_t)q;
+            if (th) {
+                c %= dims[2];
+                if (c < 0) c += dims[2];
+            } else if (c < 0 || c >= dims[i]) {
+                return false;
+            }
+            idx[i] = c;
+        }
+        return true;
+    };
+    std::map<std::vector<int64_t>, int> last;
+    std::vector<int64_t> idx;
+    for (int i = 0; i < nin; ++i)
+        if (index_of(&given[(size_t)i * dim], idx)) last[idx] = i;
+    const int nsol = (int)(solver.size() / dim);
+    std::vector<double> out((size_t)nvar * nsol, 0.);
+    for (int j = 0; j < nsol; ++j) {
+        if (!index_of(&solver[(size_t)j * dim], idx)) continue;
+        const auto it = last.find(idx);
+        if (it == last.end()) continue;
+        for (int64_t k = 0; k < nvar; ++k) out[(size_t)k * nsol + j] = svv[(size_t)k * nin + it->second];
+    }
+    return out;
+}
+
 double wall() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -356,21 +535,61 @@ void run_impl(hfm_io *io) {
     io->set_scalar("MaxStencilWidth", (double)st.max_stencil_width);
     if (io->has("order") && io->scalar("order") != 1.)
         throw IOError{"Error : only order=1 is supported by this library"};
-    if (!io->has("seeds")) {
+    // Seeds (StaticFactoring::SetSeeds, StaticFactoring.hxx:242-393): `seeds` / `seedValues`,
+    // optionally spread over the grid points within `seedRadius` (Riemann models), then for the
+    // curvature models `seeds_Unoriented` / `seedValues_Unoriented` at every theta.
+    const double thetaStep = bundle ? 2 * 3.141592653589793238462643383279502884L / dims[2] : 0.;
+    auto redim = [&](double q, int i) {
+        const double o = bundle && i == 2 ? -thetaStep / 2. : (origin.empty() ? 0. : origin[i]);
+        return q * (bundle && i == 2 ? thetaStep : h[i]) + o;
+    };
+    std::vector<double> seeds, sv;   // physical points and values handed to the solver
+    int nseeds_in = 0;               // number of `seeds` as given
+    std::vector<double> seeds_given; // `seeds` as passed
+    if (io->has("seeds")) {
+        seeds = io->vec("seeds");
+        if (seeds.size() % dim) throw IOError{"IO error : seeds has the wrong size"};
+        nseeds_in = (int)(seeds.size() / dim);
+        seeds_given = seeds;
+        if (io->has("seedValues")) {
+            sv = io->vec("seedValues");
+            if ((int)sv.size() != nseeds_in) throw IOError{"Error : Inconsistent size of seedValues."};
+        } else {
+            sv.assign(nseeds_in, 0.);
+        }
+        const double seedRadius = io->scalar("seedRadius", 0.);
+        if (seedRadius != 0.) {
+            if (!riemann) throw IOError{"Equation factoring error : no guess"};
+            if (seedRadius < 0.)
+                throw IOError{"Error : seedRadius < 0 (spreading along the reversed stencils) is not supported"};
+            if (dim == 2) spread_seeds_d<2>(io, dims, h, origin, field, is_field, hasDual, seedRadius, seeds, sv);
+            else spread_seeds_d<3>(io, dims, h, origin, field, is_field, hasDual, seedRadius, seeds, sv);
+        }
+    }
+    if (bundle && io->has("seeds_Unoriented")) {
+        const std::vector<double> up = io->vec("seeds_Unoriented");
+        if (up.size() % 2) throw IOError{"IO error : seeds_Unoriented has the wrong size"};
+        const size_t nu = up.size() / 2;
+        std::vector<double> uv(nu, 0.);
+        if (io->has("seedValues_Unoriented")) uv = io->vec("seedValues_Unoriented");
+        if (uv.size() != nu) throw IOError{"Error : Inconsistent size of seedValues_Unoriented."};
+        // EquivalentPoints (HFMInterface.hxx:67-84): the base point at every theta index
+        for (size_t i = 0; i < nu; ++i)
+            for (int64_t t = 0; t < dims[2]; ++t) {
+                seeds.push_back(up[2 * i]);
+                seeds.push_back(up[2 * i + 1]);
+                seeds.push_back(redim((double)t + 0.5, 2));
+                sv.push_back(uv[i]);
+            }
+    }
+    if (seeds.empty()) {
         io->log += "No seeds to run fast marching solver, and missing values or activeNeighs to compute geodesics.\n";
         return;
     }
-    std::vector<double> seeds = io->vec("seeds");
-    if (seeds.size() % dim) throw IOError{"IO error : seeds has the wrong size"};
     const int nseeds = (int)(seeds.size() / dim);
-    std::vector<double> sv;
-    if (io->has("seedValues")) {
-        sv = io->vec("seedValues");
-        if ((int)sv.size() != nseeds) throw IOError{"Error : Inconsistent size of seedValues."};
-    }
     // Run_RunSolver
     const double t2 = wall();
-    check(hfm_plan_solve(plan, nseeds, seeds.data(), sv.empty() ? nullptr : sv.data(), nullptr));
+    check(hfm_plan_solve(plan, nseeds, seeds.data(), sv.data(), nullptr));
     const double t3 = wall();
     check(hfm_plan_get_stats(plan, &st));
     io->set_scalar("FMCPUTime", t3 - t2);
@@ -382,14 +601,26 @@ void run_impl(hfm_io *io) {
         os << "Fast marching solver completed in " << (t3 - t2) << " s.\n";
         io->log += os.str();
     }
-    // Run_ExtractGeodesics (HFMInterface.hxx:452-523): tips -> geodesicPoints, geodesicLengths
-    if (io->has("tips")) {
-        const double tg = wall();
+    // nAccepted (CommonStoppingCriteria.h:162-165): without stopping criteria, every point the
+    // heap accepts is a reached one, i.e. a finite value
+    {
+        DevBuf dv(N * sizeof(double));
+        check(hfm_plan_values(plan, dv.d(), nullptr));
+        std::vector<double> v(N);
+        if (hipMemcpy(v.data(), dv.p, N * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+            throw IOError{"hipMemcpy failed"};
+        int64_t reached = 0;
+        for (double x : v) reached += std::isfinite(x) ? 1 : 0;
+        io->set_scalar("nAccepted", (double)reached);
+    }
+    // Run_ExtractGeodesics (HFMInterface.hxx:452-523): tips -> geodesicPoints, geodesicLengths;
+    // for the curvature models tips_Unoriented -> geodesicPoints_Unoriented, geodesicLengths_Unoriented
+    const double tg = wall();
+    bool anyGeodesic = false;
+    auto export_geodesics = [&](const std::string &suffix, const std::vector<double> &tips, bool tips_adim) {
         const std::string solver = io->str("geodesicSolver", "Discrete");
         if (solver != "Discrete")
             throw IOError{"Error : geodesic solver " + solver + " is not available (only 'Discrete')."};
-        const std::vector<double> tips = io->vec("tips");
-        if (tips.size() % dim) throw IOError{"IO error : tips has the wrong size"};
         const int ntips = (int)(tips.size() / dim);
         const double step = io->scalar("geodesicStep", 0.25);
         const double thr = io->scalar("geodesicWeightThreshold", 0.001);
@@ -398,25 +629,61 @@ void run_impl(hfm_io *io) {
         std::vector<double> pts((size_t)4096 * dim);
         int64_t npts = 0;
         int nfail = 0;
-        check(hfm_plan_geodesics(plan, ntips, tips.data(), step, thr, vb, pts.data(), 4096, lens.data(), &npts,
-                                 &nfail, nullptr));
+        check(hfm_plan_geodesics_impl(plan, ntips, tips.data(), tips_adim, step, thr, vb, pts.data(), 4096,
+                                      lens.data(), &npts, &nfail, nullptr));
         if (npts > 4096) {
             pts.resize((size_t)npts * dim);
-            check(hfm_plan_geodesics(plan, ntips, tips.data(), step, thr, vb, pts.data(), npts, lens.data(), &npts,
-                                     &nfail, nullptr));
+            check(hfm_plan_geodesics_impl(plan, ntips, tips.data(), tips_adim, step, thr, vb, pts.data(), npts,
+                                          lens.data(), &npts, &nfail, nullptr));
         }
         pts.resize((size_t)npts * dim);
-        if (nfail && io->verbosity >= 1) io->log += "Geodesic extraction failed for " + std::to_string(nfail) + " tip(s)\n";
+        if (nfail && io->verbosity >= 1)
+            io->log += "Geodesic extraction failed for " + std::to_string(nfail) + " tip(s)\n";
+        const std::string sep = suffix.empty() ? "" : "_";
         Field fp;
         fp.dims = {npts, (int64_t)dim};
         fp.data = std::move(pts);
-        io->set_computed("geodesicPoints", std::move(fp));
+        io->set_computed("geodesicPoints" + sep + suffix, std::move(fp));
         Field fl;
         fl.dims = {(int64_t)ntips};
         for (int64_t l : lens) fl.data.push_back((double)l);
-        io->set_computed("geodesicLengths", std::move(fl));
-        io->set_scalar("GeodesicCPUTime", wall() - tg);
+        io->set_computed("geodesicLengths" + sep + suffix, std::move(fl));
+        anyGeodesic = true;
+    };
+    if (io->has("tips")) {
+        const std::vector<double> tips = io->vec("tips");
+        if (tips.size() % dim) throw IOError{"IO error : tips has the wrong size"};
+        export_geodesics("", tips, false);
     }
+    if (bundle && io->has("tips_Unoriented")) {
+        // each base point starts at the theta of smallest arrival time (the last one among equal
+        // values), in grid units (HFMInterface.hxx:502-521)
+        const std::vector<double> up = io->vec("tips_Unoriented");
+        if (up.size() % 2) throw IOError{"IO error : tips_Unoriented has the wrong size"};
+        DevBuf dvals(N * sizeof(double));
+        check(hfm_plan_values(plan, dvals.d(), nullptr));
+        std::vector<double> vals(N);
+        if (hipMemcpy(vals.data(), dvals.p, N * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+            throw IOError{"hipMemcpy failed"};
+        std::vector<double> tips;
+        for (size_t i = 0; i + 1 < up.size(); i += 2) {
+            const double ax = (up[i] - (origin.empty() ? 0. : origin[0])) / h[0];
+            const double ay = (up[i + 1] - (origin.empty() ? 0. : origin[1])) / h[1];
+            double best[3] = {ax, ay, 0.5};
+            double vmin = std::numeric_limits<double>::infinity();
+            const double fx = std::floor(ax), fy = std::floor(ay);
+            const bool inside = fx >= 0 && fx < (double)dims[0] && fy >= 0 && fy < (double)dims[1];
+            for (int64_t t = 0; inside && t < dims[2]; ++t) {
+                const double v = vals[((int64_t)fx * dims[1] + (int64_t)fy) * dims[2] + t];
+                if (vmin < v) continue;
+                best[2] = (double)t + 0.5;
+                vmin = v;
+            }
+            tips.insert(tips.end(), best, best + 3);
+        }
+        export_geodesics("Unoriented", tips, true);
+    }
+    if (anyGeodesic) io->set_scalar("GeodesicCPUTime", wall() - tg);
     // Run_ExportData
     const bool exportValues = io->scalar("exportValues", 0.) != 0.;
     const bool exportActive = io->scalar("exportActiveNeighs", 0.) != 0.;
@@ -541,9 +808,12 @@ void run_impl(hfm_io *io) {
             if (f.dims.size() != 2)
                 throw IOError{"IO::GetArray error : Field seedValueVariation has depth " +
                               std::to_string(f.dims.size()) + " instead of expected 2."};
-            if (f.dims[1] != nseeds) throw IOError{"Error : Inconsistent size of seedValueVariation"};
+            if (f.dims[1] != nseeds_in) throw IOError{"Error : Inconsistent size of seedValueVariation"};
             nvs = f.dims[0];
             svv = f.data;
+            if (nseeds != nseeds_in)
+                svv = seed_variation_for_solver(svv, nvs, nseeds_in, seeds_given, seeds, dim, dims, h, origin, bundle,
+                                                thetaStep);
         }
         if (nvc >= 0 && nvs >= 0 && nvc != nvs) throw IOError{"Forward variation error : inconsistent input sizes"};
         const int64_t nvar = std::max(nvc, nvs);

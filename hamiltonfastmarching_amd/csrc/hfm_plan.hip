@@ -767,6 +767,15 @@ int hfm_plan_n_active_slots(const hfm_plan *p) { return p ? p->ndiff : 0; }
 int hfm_plan_geodesics(hfm_plan *p, int ntips, const double *tips, double step, double threshold,
                        double volumeBound, double *points, int64_t max_points, int64_t *lengths, int64_t *npoints,
                        int *nfailed, void *stream) {
+    return hfm_plan_geodesics_impl(p, ntips, tips, false, step, threshold, volumeBound, points, max_points, lengths,
+                                   npoints, nfailed, stream);
+}
+
+}  // extern "C"
+
+int hfm_plan_geodesics_impl(hfm_plan *p, int ntips, const double *tips, bool tips_adim, double step,
+                            double threshold, double volumeBound, double *points, int64_t max_points,
+                            int64_t *lengths, int64_t *npoints, int *nfailed, void *stream) {
     if (!p || (ntips > 0 && (!tips || !lengths)) || !npoints) {
         set_error("hfm_plan_geodesics: null argument");
         return 1;
@@ -789,7 +798,10 @@ int hfm_plan_geodesics(hfm_plan *p, int ntips, const double *tips, double step, 
             h.origin[i] = p->origin[i];
         }
         for (int t = 0; t < ntips; ++t)
-            for (int i = 0; i < dim; ++i) tad[(size_t)t * dim + i] = (tips[(size_t)t * dim + i] - h.origin[i]) / h.scale[i];
+            for (int i = 0; i < dim; ++i) {
+                const double x = tips[(size_t)t * dim + i];
+                tad[(size_t)t * dim + i] = tips_adim ? x : (x - h.origin[i]) / h.scale[i];
+            }
         h.step = step;
         h.threshold = threshold;
         h.volumeBound = volumeBound > 0 ? volumeBound : 5. * std::pow(1.3, dim);
@@ -864,6 +876,8 @@ int hfm_plan_geodesics(hfm_plan *p, int ntips, const double *tips, double step, 
     }
     return 0;
 }
+
+extern "C" {
 
 int hfm_plan_get_stats(const hfm_plan *p, hfm_plan_stats *st) {
     if (!p || !st) {

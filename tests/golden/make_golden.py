@@ -262,6 +262,74 @@ def wall_cases():
     print("walls ok")
 
 
+SEED_CASES = {
+    # seed spreading over the cells within seedRadius (StaticFactoring.hxx:284-345)
+    "riemann2_radius": (lambda: syn.riemann2(40, seed=6),
+                        {"seeds": np.array([[0.31, 0.52], [0.7, 0.66]]), "seedValues": np.array([0.0, 0.05]),
+                         "seedRadius": 2.5,
+                         "costVariation": None, "seedValueVariation": np.array([[1.0, -2.0], [0.5, 0.25]])}),
+    "riemann2_radius_dual": (lambda: syn.riemann2(40, seed=6),
+                             {"seeds": np.array([[0.43, 0.61]]), "seedRadius": 1.7, "dual": True}),
+    "riemann3_radius": (lambda: syn.riemann3(16, seed=7),
+                        {"seeds": np.array([[0.47, 0.52, 0.55]]), "seedRadius": 1.5}),
+    # unoriented seeds and tips of the curvature models (StaticFactoring.hxx:351-380,
+    # HFMInterface.hxx:498-522)
+    "rsf2_unoriented": (lambda: syn.rsf2(24, 16, seed=8),
+                        {"seeds_Unoriented": np.array([[0.3, 0.4], [0.72, 0.6]]),
+                         "seedValues_Unoriented": np.array([0.0, 0.1]),
+                         "tips_Unoriented": np.array([[0.8, 0.2], [0.15, 0.9], [0.5, 0.5]])}),
+    "elastica2_unoriented": (lambda: syn.elastica2(20, 16, seed=9),
+                             {"seeds": np.array([[0.5, 0.5, 1.0]]),
+                              "seeds_Unoriented": np.array([[0.25, 0.3]]),
+                              "tips_Unoriented": np.array([[0.8, 0.75], [0.1, 0.85]])}),
+}
+
+
+def seed_case_params(name):
+    make, extra = SEED_CASES[name]
+    p = {k: v for k, v in make().items()
+         if k not in ("tips", "seeds", "inspectSensitivity", "inspectSensitivityWeights", "inspectSensitivityLengths")}
+    e = dict(extra)
+    if e.pop("dual", False):
+        p["dualMetric"] = np.linalg.inv(_unpack_sym(p.pop("metric"))).reshape(p["dims"].astype(int).tolist() + [-1])
+        p["dualMetric"] = _pack_sym(p["dualMetric"])
+    if e.get("costVariation", 1) is None:
+        e.pop("costVariation")
+    p.update(e)
+    if "seeds" not in p and "seeds_Unoriented" not in p:
+        p["seeds"] = make()["seeds"]
+    return p
+
+
+def _unpack_sym(m):
+    """(..., 3) packed xx, xy, yy -> (..., 2, 2)."""
+    a = np.empty(m.shape[:-1] + (2, 2))
+    a[..., 0, 0], a[..., 1, 0], a[..., 1, 1] = m[..., 0], m[..., 1], m[..., 2]
+    a[..., 0, 1] = a[..., 1, 0]
+    return a
+
+
+def _pack_sym(a):
+    a = a.reshape(a.shape[:-1] + (2, 2)) if a.shape[-1] == 4 else a
+    return np.stack([a[..., 0, 0], a[..., 1, 0], a[..., 1, 1]], axis=-1)
+
+
+def seed_cases():
+    out = {}
+    for name in SEED_CASES:
+        p = seed_case_params(name)
+        q = dict(p, exportValues=1.0)
+        r = oracle.run_reference(q)
+        assert r["retcode"] == 0, r["log"]
+        for k in ("values", "spreadedSeeds", "spreadedSeedValues", "geodesicPoints_Unoriented",
+                  "geodesicLengths_Unoriented", "valueVariation", "nAccepted"):
+            if k in r:
+                out[name + "_" + k] = np.asarray(r[k])
+        print(name, sorted(k for k in r if not k.endswith("Time")))
+    np.savez_compressed(os.path.join(HERE, "seeds.npz"), **out)
+    print("seeds ok")
+
+
 def big_case():
     p = syn.riemann3(128, seed=0)
     r = oracle.run_reference(p)
@@ -282,7 +350,8 @@ def main():
             sys.exit(f"reference binary for {m} missing: run `make -C oracle ref` first")
     only = [a for a in sys.argv[1:] if not a.startswith("--")]
     if only:  # regenerate the named fixture groups only
-        groups = {"geodesics": geodesic_cases, "forward": forward_cases, "walls": wall_cases, "big": big_case}
+        groups = {"geodesics": geodesic_cases, "forward": forward_cases, "walls": wall_cases, "seeds": seed_cases,
+                  "big": big_case}
         for a in only:
             groups[a]()
         return
@@ -294,6 +363,7 @@ def main():
     geodesic_cases()
     forward_cases()
     wall_cases()
+    seed_cases()
     if "--big" in sys.argv:
         big_case()
 

@@ -1,0 +1,67 @@
+"""Seed set-up variants against the fixtures the reference executables wrote
+(tests/golden/seeds.npz, cases built by tests/golden/make_golden.py):
+
+* seedRadius spreading (StaticFactoring.hxx:284-345): spreadedSeeds / spreadedSeedValues exactly,
+  values 1e-12 relative, valueVariation of the given seeds 1e-11 (primal and dual metrics,
+  2-D and 3-D);
+* seeds_Unoriented / seedValues_Unoriented (StaticFactoring.hxx:351-380) and tips_Unoriented
+  (HFMInterface.hxx:498-522) of the curvature models: values 1e-12, geodesicLengths_Unoriented
+  equal, geodesicPoints_Unoriented within 1e-6 of a grid step;
+* nAccepted (CommonStoppingCriteria.h:162-165) equal.
+FP64 solves."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import hamiltonfastmarching_amd as hfm
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+sys.path.insert(0, GOLD)
+import make_golden as mg  # noqa: E402
+
+
+def _floor_rel(u, ref):
+    fin = np.isfinite(ref)
+    pos = ref[fin & (ref > 0)]
+    floor = pos.min() if pos.size else 0.0
+    m = fin & (ref >= floor) & (ref > 0)
+    return float(np.max(np.abs(u[m] - ref[m]) / ref[m])) if m.any() else 0.0
+
+
+@pytest.mark.parametrize("name", list(mg.SEED_CASES))
+def test_seed_variants_vs_golden(name):
+    print(f"seeds {name}", flush=True)
+    g = np.load(os.path.join(GOLD, "seeds.npz"))
+    p = dict(mg.seed_case_params(name), exportValues=1.0)
+    out = hfm.run(p)
+    u, ref = out["values"], g[name + "_values"]
+    assert np.array_equal(np.isfinite(u), np.isfinite(ref))
+    err = _floor_rel(u, ref)
+    print(f"{name}: values rel err {err:.3e}", flush=True)
+    assert err <= 1e-12
+    assert out["nAccepted"] == float(g[name + "_nAccepted"])
+    if name + "_spreadedSeeds" in g.files:
+        assert np.array_equal(out["spreadedSeeds"], g[name + "_spreadedSeeds"])
+        assert np.array_equal(out["spreadedSeedValues"], g[name + "_spreadedSeedValues"])
+    if name + "_valueVariation" in g.files:
+        vv, vref = out["valueVariation"], g[name + "_valueVariation"]
+        verr = float(np.abs(vv - vref).max() / np.abs(vref).max())
+        print(f"{name}: valueVariation rel err {verr:.3e}", flush=True)
+        assert verr <= 1e-11
+    if name + "_geodesicPoints_Unoriented" in g.files:
+        assert np.array_equal(out["geodesicLengths_Unoriented"], g[name + "_geodesicLengths_Unoriented"])
+        h = float(np.atleast_1d(p["gridScale"])[0])
+        perr = float(np.abs(out["geodesicPoints_Unoriented"] - g[name + "_geodesicPoints_Unoriented"]).max()) / h
+        print(f"{name}: unoriented geodesic points err {perr:.3e} h", flush=True)
+        assert perr <= 1e-6
+
+
+def test_seed_radius_on_curvature_model_is_the_reference_error():
+    """The curvature models have no distance guess (HamiltonFastMarching.h:203-204)."""
+    p = mg.seed_case_params("rsf2_unoriented")
+    with pytest.raises(hfm.PyExp, match="no guess"):
+        hfm.run(dict(p, seeds=np.array([[0.5, 0.5, 1.0]]), seedRadius=2.0))
