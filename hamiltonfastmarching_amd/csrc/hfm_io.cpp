@@ -535,6 +535,14 @@ void run_impl(hfm_io *io) {
     io->set_scalar("MaxStencilWidth", (double)st.max_stencil_width);
     if (io->has("order") && io->scalar("order") != 1.)
         throw IOError{"Error : only order=1 is supported by this library"};
+    // Static factoring and the early-stopping criteria change the values through the serial
+    // heap's acceptance order (StaticFactoring.hxx, CommonStoppingCriteria.h); refused rather
+    // than ignored.
+    if ((io->has("factoringRadius") && io->scalar("factoringRadius") != 0.) || io->has("factoringValues"))
+        throw IOError{"Error : static factoring is not supported by this library"};
+    for (const char *k : {"stopWhenAnyAccepted", "stopWhenAllAccepted", "stopWhenAnyAccepted_Unoriented",
+                          "stopWhenAllAccepted_Unoriented", "stopAtDistance", "nMaxAccepted"})
+        if (io->has(k)) throw IOError{std::string("Error : stopping criterion ") + k + " is not supported by this library"};
     // Seeds (StaticFactoring::SetSeeds, StaticFactoring.hxx:242-393): `seeds` / `seedValues`,
     // optionally spread over the grid points within `seedRadius` (Riemann models), then for the
     // curvature models `seeds_Unoriented` / `seedValues_Unoriented` at every theta.
