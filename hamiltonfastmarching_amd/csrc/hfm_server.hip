@@ -48,6 +48,9 @@ struct SolveDesc {
     unsigned long long *stats;    // PEND_SLOTS pairs (visits, sweeps)
 };
 
+// Jacobi sweeps per tile visit in the serving update kernel.
+constexpr int SERVE_SWEEPS = 32;
+
 // Scheduling round over every running solve.  Block b covers tiles [bmap[b].y, bmap[b].y +
 // SCHED_BLOCK) of solve bmap[b].x: the grid has one block per SCHED_BLOCK tiles of each running solve.
 __global__ __launch_bounds__(SCHED_BLOCK) void k_mschedule(const SolveDesc *__restrict__ descs,
@@ -107,8 +110,9 @@ __global__ __launch_bounds__(256) void k_mfim(const SolveDesc *__restrict__ desc
         const SolveDesc &d = descs[s];
         unsigned wm = KEY_NONE;
         int sw;
-        if (d.model == MODEL_RIEMANN3) sw = fim_tile<MODEL_RIEMANN3, float, 32>(d.ctx, tile, lane, su[wv], wm);
-        else sw = fim_tile<MODEL_RIEMANN2, float, 32>(d.ctx, tile, lane, su[wv], wm);
+        // at most SERVE_SWEEPS sweeps per visit; an unconverged tile goes back to its solve's schedule
+        if (d.model == MODEL_RIEMANN3) sw = fim_tile<MODEL_RIEMANN3, float, SERVE_SWEEPS>(d.ctx, tile, lane, su[wv], wm);
+        else sw = fim_tile<MODEL_RIEMANN2, float, SERVE_SWEEPS>(d.ctx, tile, lane, su[wv], wm);
         if (lane == 0) {
             if (wm != KEY_NONE) atomicMin(&d.pend[(parity ^ 1) * PEND_SLOTS + slot], wm);
             atomicAdd(&d.stats[2 * slot], 1ull);
