@@ -534,7 +534,8 @@ void run_impl(hfm_io *io) {
     // Run_SetupSolver
     io->set_scalar("MaxStencilWidth", (double)st.max_stencil_width);
     if (io->has("order") && io->scalar("order") != 1.)
-        throw IOError{"Error : only order=1 is supported by this library"};
+        throw IOError{"Error : order " + std::to_string(io->scalar("order")) +
+                      " is not supported by this library (order 1 only)"};
     // Static factoring and the early-stopping criteria change the values through the serial
     // heap's acceptance order (StaticFactoring.hxx, CommonStoppingCriteria.h); refused rather
     // than ignored.
