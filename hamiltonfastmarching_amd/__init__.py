@@ -233,6 +233,17 @@ class Plan:
                                            _stream_ptr(torch)))
         return self
 
+    def load_values(self, values, active):
+        """Use cached arrival times and activeNeighs (grid-shaped) instead of a solve."""
+        torch = _torch()
+        v = torch.as_tensor(values, dtype=torch.float64).to("cuda").contiguous()
+        a = torch.as_tensor(active, dtype=torch.float64).to("cuda").contiguous()
+        if tuple(v.shape) != tuple(self.dims) or tuple(a.shape) != tuple(self.dims):
+            raise PyExp("hfm_plan_load_values: values and activeNeighs must have the grid's shape")
+        self._check(self._L.hfm_plan_load_values(self._h, ctypes.c_void_p(v.data_ptr()),
+                                                 ctypes.c_void_p(a.data_ptr()), _stream_ptr(torch)))
+        return self
+
     def values(self, out=None):
         torch = _torch()
         if out is None:

@@ -70,6 +70,7 @@ SIGNATURES = [
     ("hfm_plan_set_cost", c_int, [c_void_p, c_void_p, c_int, c_void_p]),
     ("hfm_plan_solve", c_int, [c_void_p, c_int, P(c_double), P(c_double), c_void_p]),
     ("hfm_plan_values", c_int, [c_void_p, c_void_p, c_void_p]),
+    ("hfm_plan_load_values", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     ("hfm_plan_flow", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     ("hfm_plan_active_offsets", c_int, [c_void_p, c_void_p, c_void_p]),
     ("hfm_plan_geodesics", c_int, [c_void_p, c_int, P(c_double), c_double, c_double, c_double, P(c_double),

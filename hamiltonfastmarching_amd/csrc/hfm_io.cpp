@@ -555,72 +555,90 @@ void run_impl(hfm_io *io) {
     std::vector<double> seeds, sv;   // physical points and values handed to the solver
     int nseeds_in = 0;               // number of `seeds` as given
     std::vector<double> seeds_given; // `seeds` as passed
-    if (io->has("seeds")) {
-        seeds = io->vec("seeds");
-        if (seeds.size() % dim) throw IOError{"IO error : seeds has the wrong size"};
-        nseeds_in = (int)(seeds.size() / dim);
-        seeds_given = seeds;
-        if (io->has("seedValues")) {
-            sv = io->vec("seedValues");
-            if ((int)sv.size() != nseeds_in) throw IOError{"Error : Inconsistent size of seedValues."};
-        } else {
-            sv.assign(nseeds_in, 0.);
-        }
-        const double seedRadius = io->scalar("seedRadius", 0.);
-        if (seedRadius != 0.) {
-            if (!riemann) throw IOError{"Equation factoring error : no guess"};
-            if (seedRadius < 0.)
-                throw IOError{"Error : seedRadius < 0 (spreading along the reversed stencils) is not supported"};
-            if (dim == 2) spread_seeds_d<2>(io, dims, h, origin, field, is_field, hasDual, seedRadius, seeds, sv);
-            else spread_seeds_d<3>(io, dims, h, origin, field, is_field, hasDual, seedRadius, seeds, sv);
-        }
-    }
-    if (bundle && io->has("seeds_Unoriented")) {
-        const std::vector<double> up = io->vec("seeds_Unoriented");
-        if (up.size() % 2) throw IOError{"IO error : seeds_Unoriented has the wrong size"};
-        const size_t nu = up.size() / 2;
-        std::vector<double> uv(nu, 0.);
-        if (io->has("seedValues_Unoriented")) uv = io->vec("seedValues_Unoriented");
-        if (uv.size() != nu) throw IOError{"Error : Inconsistent size of seedValues_Unoriented."};
-        // EquivalentPoints (HFMInterface.hxx:67-84): the base point at every theta index
-        for (size_t i = 0; i < nu; ++i)
-            for (int64_t t = 0; t < dims[2]; ++t) {
-                seeds.push_back(up[2 * i]);
-                seeds.push_back(up[2 * i + 1]);
-                seeds.push_back(redim((double)t + 0.5, 2));
-                sv.push_back(uv[i]);
-            }
-    }
-    if (seeds.empty()) {
-        io->log += "No seeds to run fast marching solver, and missing values or activeNeighs to compute geodesics.\n";
-        return;
-    }
-    const int nseeds = (int)(seeds.size() / dim);
-    // Run_RunSolver
-    const double t2 = wall();
-    check(hfm_plan_solve(plan, nseeds, seeds.data(), sv.data(), nullptr));
-    const double t3 = wall();
-    check(hfm_plan_get_stats(plan, &st));
-    io->set_scalar("FMCPUTime", t3 - t2);
-    io->set_scalar("FMGPUTime", st.solve_ms * 1e-3);
-    io->set_scalar("nIterations", (double)st.iterations);
-    io->set_scalar("nTileVisits", (double)st.tile_visits);
-    if (io->verbosity >= 1) {
-        std::ostringstream os;
-        os << "Fast marching solver completed in " << (t3 - t2) << " s.\n";
-        io->log += os.str();
-    }
-    // nAccepted (CommonStoppingCriteria.h:162-165): without stopping criteria, every point the
-    // heap accepts is a reached one, i.e. a finite value
-    {
-        DevBuf dv(N * sizeof(double));
-        check(hfm_plan_values(plan, dv.d(), nullptr));
-        std::vector<double> v(N);
-        if (hipMemcpy(v.data(), dv.p, N * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    int nseeds = 0;
+    // Run_RunSolver (HFMInterface.hxx:422-450): given both values and activeNeighs, the solver is
+    // bypassed and everything downstream (geodesics, exports, differentiation) uses them.
+    const bool bypass = io->has("values") && io->has("activeNeighs");
+    if (bypass) {
+        bool vf = false, af = false;
+        const std::vector<double> vals = grid_field(io, "values", dims, 1, false, vf);
+        const std::vector<double> act = grid_field(io, "activeNeighs", dims, 1, false, af);
+        if (!vf || !af) throw IOError{"Error : values and activeNeighs must be fields over the grid"};
+        DevBuf dv(vals.size() * sizeof(double)), da(act.size() * sizeof(double));
+        if (hipMemcpy(dv.p, vals.data(), vals.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(da.p, act.data(), act.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
             throw IOError{"hipMemcpy failed"};
-        int64_t reached = 0;
-        for (double x : v) reached += std::isfinite(x) ? 1 : 0;
-        io->set_scalar("nAccepted", (double)reached);
+        check(hfm_plan_load_values(plan, dv.d(), da.d(), nullptr));
+        if (io->verbosity >= 1) io->log += "Bypassing fast marching solver based on cached data.\n";
+        io->set_scalar("nAccepted", 0.);  // nothing accepted: the solver did not run
+    } else {
+        if (io->has("seeds")) {
+            seeds = io->vec("seeds");
+            if (seeds.size() % dim) throw IOError{"IO error : seeds has the wrong size"};
+            nseeds_in = (int)(seeds.size() / dim);
+            seeds_given = seeds;
+            if (io->has("seedValues")) {
+                sv = io->vec("seedValues");
+                if ((int)sv.size() != nseeds_in) throw IOError{"Error : Inconsistent size of seedValues."};
+            } else {
+                sv.assign(nseeds_in, 0.);
+            }
+            const double seedRadius = io->scalar("seedRadius", 0.);
+            if (seedRadius != 0.) {
+                if (!riemann) throw IOError{"Equation factoring error : no guess"};
+                if (seedRadius < 0.)
+                    throw IOError{"Error : seedRadius < 0 (spreading along the reversed stencils) is not supported"};
+                if (dim == 2) spread_seeds_d<2>(io, dims, h, origin, field, is_field, hasDual, seedRadius, seeds, sv);
+                else spread_seeds_d<3>(io, dims, h, origin, field, is_field, hasDual, seedRadius, seeds, sv);
+            }
+        }
+        if (bundle && io->has("seeds_Unoriented")) {
+            const std::vector<double> up = io->vec("seeds_Unoriented");
+            if (up.size() % 2) throw IOError{"IO error : seeds_Unoriented has the wrong size"};
+            const size_t nu = up.size() / 2;
+            std::vector<double> uv(nu, 0.);
+            if (io->has("seedValues_Unoriented")) uv = io->vec("seedValues_Unoriented");
+            if (uv.size() != nu) throw IOError{"Error : Inconsistent size of seedValues_Unoriented."};
+            // EquivalentPoints (HFMInterface.hxx:67-84): the base point at every theta index
+            for (size_t i = 0; i < nu; ++i)
+                for (int64_t t = 0; t < dims[2]; ++t) {
+                    seeds.push_back(up[2 * i]);
+                    seeds.push_back(up[2 * i + 1]);
+                    seeds.push_back(redim((double)t + 0.5, 2));
+                    sv.push_back(uv[i]);
+                }
+        }
+        if (seeds.empty()) {
+            io->log += "No seeds to run fast marching solver, and missing values or activeNeighs to compute geodesics.\n";
+            return;
+        }
+        nseeds = (int)(seeds.size() / dim);
+        // Run_RunSolver
+        const double t2 = wall();
+        check(hfm_plan_solve(plan, nseeds, seeds.data(), sv.data(), nullptr));
+        const double t3 = wall();
+        check(hfm_plan_get_stats(plan, &st));
+        io->set_scalar("FMCPUTime", t3 - t2);
+        io->set_scalar("FMGPUTime", st.solve_ms * 1e-3);
+        io->set_scalar("nIterations", (double)st.iterations);
+        io->set_scalar("nTileVisits", (double)st.tile_visits);
+        if (io->verbosity >= 1) {
+            std::ostringstream os;
+            os << "Fast marching solver completed in " << (t3 - t2) << " s.\n";
+            io->log += os.str();
+        }
+        // nAccepted (CommonStoppingCriteria.h:162-165): without stopping criteria, every point the
+        // heap accepts is a reached one, i.e. a finite value
+        {
+            DevBuf dv(N * sizeof(double));
+            check(hfm_plan_values(plan, dv.d(), nullptr));
+            std::vector<double> v(N);
+            if (hipMemcpy(v.data(), dv.p, N * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+                throw IOError{"hipMemcpy failed"};
+            int64_t reached = 0;
+            for (double x : v) reached += std::isfinite(x) ? 1 : 0;
+            io->set_scalar("nAccepted", (double)reached);
+        }
     }
     // Run_ExtractGeodesics (HFMInterface.hxx:452-523): tips -> geodesicPoints, geodesicLengths;
     // for the curvature models tips_Unoriented -> geodesicPoints_Unoriented, geodesicLengths_Unoriented

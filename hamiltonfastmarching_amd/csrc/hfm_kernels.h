@@ -78,6 +78,11 @@ hipError_t launch_riemann_stencil(const Geom &g, const double *metric, int is_fi
 template <typename T>
 hipError_t launch_tile_deps(int model, const Geom &g, const StencilData<T> &sd, int pass, int *cnt, const int *start,
                             int *fill, int *deps, int *ovf, hipStream_t s);
+// Solver state (values, fixed mask) from row-major FP64 arrival times and activeNeighs.
+template <typename T>
+hipError_t launch_load_state(const Geom &g, const double *values, const double *active, T *u,
+                             unsigned long long *fixedmask, long long np, hipStream_t s);
+
 // Walls (hfm_walls.hip): per-point visible-side bitmask (side_bit layout) from a row-major byte
 // field of walls, the distance to the walls being capped at maxDist (MaxStencilWidth narrowed to
 // int8); dist0 / dist1: np bytes of scratch each.

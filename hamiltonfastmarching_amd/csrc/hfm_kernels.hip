@@ -49,7 +49,7 @@ __global__ __launch_bounds__(256) void k_riemann_stencil(Geom g, const double *_
         int mx = 0;
         const int st = riemann_point_stencil<D>(mc, dual != 0, ih.v, o8, wd, &mx);
         for (int k = 0; k < K; ++k) {
-            offs[k * np + p] = pack_offset(o8[k][0], D > 1 ? o8[k][1] : 0, D > 2 ? o8[k][2] : 0);
+            offs[k * np + p] = pack_offset(o8[k][0], D > 1 ? o8[k][D > 1 ? 1 : 0] : 0, D > 2 ? o8[k][D > 2 ? 2 : 0] : 0);
             w[k * np + p] = (T)wd[k];
         }
         if (st != SELLING_OK) {
@@ -241,6 +241,41 @@ __global__ __launch_bounds__(256) void k_init(Geom g, T *__restrict__ u, unsigne
         tmax[tile] = KEY_NONE;
     }
 }
+
+// Solver state from given arrival times and active sets (row-major FP64): a reached point
+// without active differences is a seed (fixed); padding stays +inf and fixed.
+template <typename T>
+__global__ __launch_bounds__(256) void k_load_state(Geom g, const double *__restrict__ values,
+                                                    const double *__restrict__ active, T *__restrict__ u,
+                                                    unsigned long long *__restrict__ fixedmask, long long np) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;  // np is a multiple of 64: whole waves leave together
+    int c[3];
+    tiled_coords(g, (int)(p >> 6), (int)(p & 63), c);
+    const bool in = in_grid(g, c);
+    double v = __builtin_huge_val();
+    bool fixed = !in;
+    if (in) {
+        const long long lin = ((long long)c[0] * g.n[1] + c[1]) * g.n[2] + c[2];
+        v = values[lin];
+        fixed = active[lin] == 0. && v < __builtin_huge_val();
+    }
+    u[p] = (T)v;
+    const unsigned long long m = __ballot(fixed);
+    if ((threadIdx.x & 63) == 0) fixedmask[p >> 6] = m;
+}
+
+template <typename T>
+hipError_t launch_load_state(const Geom &g, const double *values, const double *active, T *u,
+                             unsigned long long *fixedmask, long long np, hipStream_t s) {
+    hipLaunchKernelGGL((k_load_state<T>), dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, g, values, active, u,
+                       fixedmask, np);
+    return hipGetLastError();
+}
+template hipError_t launch_load_state<float>(const Geom &, const double *, const double *, float *,
+                                             unsigned long long *, long long, hipStream_t);
+template hipError_t launch_load_state<double>(const Geom &, const double *, const double *, double *,
+                                              unsigned long long *, long long, hipStream_t);
 
 template <typename T>
 __global__ void k_set_seeds(const long long *__restrict__ seed_p, const double *__restrict__ seed_v, int nseeds,

@@ -277,6 +277,24 @@ void place_seeds(const hfm_plan *p, int nseeds, const double *seeds, const doubl
     }
 }
 
+// Visibility mask of the walls for the current stencils.
+void build_visibility(hfm_plan *p, hipStream_t s) {
+    if (!p->vis) {
+        p->vis = dalloc<uint32_t>(p->np);
+        p->wdist[0] = dalloc<int8_t>(p->np);
+        p->wdist[1] = dalloc<int8_t>(p->np);
+    }
+    // maxDist = MaxStencilWidth as the reference's int8 ShortType (WallObstructionTest.h:47)
+    const int maxDist = (int)(int8_t)p->max_stencil_width;
+    if (p->precision == 32)
+        HIPCHECK(launch_wall_visibility<float>(p->model, p->g, p->sd<float>(), p->walls, maxDist, p->walls_refine,
+                                               p->wdist[0], p->wdist[1], p->vis, p->np, s));
+    else
+        HIPCHECK(launch_wall_visibility<double>(p->model, p->g, p->sd<double>(), p->walls, maxDist, p->walls_refine,
+                                                p->wdist[0], p->wdist[1], p->vis, p->np, s));
+    p->vis_dirty = false;
+}
+
 template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
     hipEvent_t e0, e1;
     HIPCHECK(hipEventCreate(&e0));
@@ -681,24 +699,38 @@ int hfm_plan_solve(hfm_plan *p, int nseeds, const double *seeds, const double *s
         HIPCHECK(hipMemcpyAsync(p->seed_v, vals.data(), vals.size() * sizeof(double), hipMemcpyHostToDevice, s));
         p->last_seeds = tp;
         p->seed_inputs = inputs;
-        if (p->walls && p->vis_dirty) {
-            if (!p->vis) {
-                p->vis = dalloc<uint32_t>(p->np);
-                p->wdist[0] = dalloc<int8_t>(p->np);
-                p->wdist[1] = dalloc<int8_t>(p->np);
-            }
-            // maxDist = MaxStencilWidth as the reference's int8 ShortType (WallObstructionTest.h:47)
-            const int maxDist = (int)(int8_t)p->max_stencil_width;
-            if (p->precision == 32)
-                HIPCHECK(launch_wall_visibility<float>(p->model, p->g, p->sd<float>(), p->walls, maxDist,
-                                                       p->walls_refine, p->wdist[0], p->wdist[1], p->vis, p->np, s));
-            else
-                HIPCHECK(launch_wall_visibility<double>(p->model, p->g, p->sd<double>(), p->walls, maxDist,
-                                                        p->walls_refine, p->wdist[0], p->wdist[1], p->vis, p->np, s));
-            p->vis_dirty = false;
-        }
+        if (p->walls && p->vis_dirty) build_visibility(p, s);
         if (p->precision == 32) run_fim<float>(p, s);
         else run_fim<double>(p, s);
+        p->solved = true;
+    } catch (const HipFail &e) {
+        set_error(e.msg);
+        return 1;
+    }
+    return 0;
+}
+
+int hfm_plan_load_values(hfm_plan *p, const double *d_values, const double *d_active, void *stream) {
+    if (!p || !d_values || !d_active) {
+        set_error("hfm_plan_load_values: null argument");
+        return 1;
+    }
+    if (!p->stencil_ready) {
+        set_error("hfm_plan_load_values: stencils not set (call hfm_plan_set_metric / hfm_plan_set_cost first)");
+        return 1;
+    }
+    try {
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        if (p->walls && p->vis_dirty) build_visibility(p, s);
+        if (p->precision == 32)
+            HIPCHECK(launch_load_state<float>(p->g, d_values, d_active, static_cast<float *>(p->u), p->fixedmask,
+                                              p->np, s));
+        else
+            HIPCHECK(launch_load_state<double>(p->g, d_values, d_active, static_cast<double *>(p->u), p->fixedmask,
+                                               p->np, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        p->last_seeds.clear();
+        p->seed_inputs.clear();
         p->solved = true;
     } catch (const HipFail &e) {
         set_error(e.msg);

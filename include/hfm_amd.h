@@ -141,6 +141,14 @@ int hfm_plan_set_walls(hfm_plan *plan, const uint8_t *d_walls, int refine, void 
  * block fast-iterative method to its fixed point on the plan's stream. */
 int hfm_plan_solve(hfm_plan *plan, int nseeds, const double *seeds, const double *seedValues, void *stream);
 
+/* Solver state from cached results instead of a solve (the bypass of Run_RunSolver,
+ * HFMInterface.hxx:423-426): d_values (arrival times) and d_active (activeNeighs bitsets as
+ * doubles), both device, FP64, row-major.  A reached point without active differences is taken
+ * as a seed.  Geodesics, flow, activeOffsets, adjoint and forward variation then run on these
+ * values; their active sets are recomputed from the values (they equal d_active when both come
+ * from a solve of the same problem). */
+int hfm_plan_load_values(hfm_plan *plan, const double *d_values, const double *d_active, void *stream);
+
 /* Arrival times of the last solve, row-major FP64 (device) -- the 'values' output. */
 int hfm_plan_values(hfm_plan *plan, double *d_values, void *stream);
 

@@ -330,6 +330,40 @@ def seed_cases():
     print("seeds ok")
 
 
+BYPASS_CASES = {
+    "riemann2": (lambda: syn.riemann2(32, seed=12), [[0.9, 0.2], [0.1, 0.8]]),
+    "rsf2": (lambda: syn.rsf2(16, 8, seed=13), [[0.8, 0.2, 1.0], [0.2, 0.7, 4.0]]),
+}
+
+
+def bypass_params(name):
+    """A solve's values + activeNeighs fed back in (Run_RunSolver bypass, HFMInterface.hxx:423-426),
+    with tips, a flow export and one sensitivity target."""
+    make, tips = BYPASS_CASES[name]
+    p = {k: v for k, v in make().items()
+         if k not in ("tips", "inspectSensitivity", "inspectSensitivityWeights", "inspectSensitivityLengths")}
+    first = oracle.run_reference(dict(p, exportValues=1.0, exportActiveNeighs=1.0))
+    assert first["retcode"] == 0, first["log"]
+    q = dict(p, values=first["values"], activeNeighs=first["activeNeighs"], tips=np.array(tips),
+             exportGeodesicFlow=1.0, inspectSensitivity=np.array(tips[:1]))
+    return q
+
+
+def bypass_cases():
+    out = {}
+    for name in BYPASS_CASES:
+        q = bypass_params(name)
+        r = oracle.run_reference(q)
+        assert r["retcode"] == 0, r["log"]
+        assert "Bypassing" in r["log"], r["log"]
+        out[name + "_values_in"] = q["values"]
+        out[name + "_activeNeighs_in"] = q["activeNeighs"]
+        for k in ("geodesicPoints", "geodesicLengths", "geodesicFlow", "costSensitivity_0", "nAccepted"):
+            out[name + "_" + k] = np.asarray(r[k])
+    np.savez_compressed(os.path.join(HERE, "bypass.npz"), **out)
+    print("bypass ok")
+
+
 def big_case():
     p = syn.riemann3(128, seed=0)
     r = oracle.run_reference(p)
@@ -351,7 +385,7 @@ def main():
     only = [a for a in sys.argv[1:] if not a.startswith("--")]
     if only:  # regenerate the named fixture groups only
         groups = {"geodesics": geodesic_cases, "forward": forward_cases, "walls": wall_cases, "seeds": seed_cases,
-                  "big": big_case}
+                  "bypass": bypass_cases, "big": big_case}
         for a in only:
             groups[a]()
         return
@@ -364,6 +398,7 @@ def main():
     forward_cases()
     wall_cases()
     seed_cases()
+    bypass_cases()
     if "--big" in sys.argv:
         big_case()
 

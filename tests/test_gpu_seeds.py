@@ -75,3 +75,34 @@ def test_unsupported_keys_are_refused(key, value):
     p = syn.riemann2(16, seed=3)
     with pytest.raises(hfm.PyExp, match="not supported"):
         hfm.run(dict(p, **{key: value}))
+
+
+@pytest.mark.parametrize("name", list(mg.BYPASS_CASES))
+def test_bypass_with_cached_values_vs_golden(name):
+    """values + activeNeighs given: the solver is bypassed (HFMInterface.hxx:423-426) and geodesics,
+    geodesicFlow and costSensitivity come from the cached data, as the reference's do."""
+    print(f"bypass {name}", flush=True)
+    g = np.load(os.path.join(GOLD, "bypass.npz"))
+    make, tips = mg.BYPASS_CASES[name]
+    p = {k: v for k, v in make().items()
+         if k not in ("tips", "inspectSensitivity", "inspectSensitivityWeights", "inspectSensitivityLengths")}
+    q = dict(p, values=g[name + "_values_in"], activeNeighs=g[name + "_activeNeighs_in"], tips=np.array(tips),
+             exportGeodesicFlow=1.0, inspectSensitivity=np.array(tips[:1]), verbosity=1.0)
+    io = hfm.HFMIO(default_ordering="RowMajor")
+    for k, v in q.items():
+        io[k] = v
+    io.run()
+    out = {k: io[k] for k in io.computed_keys()}
+    assert out["nAccepted"] == 0.0
+    assert "FMCPUTime" not in out
+    assert np.array_equal(out["geodesicLengths"], g[name + "_geodesicLengths"])
+    h = float(np.atleast_1d(p["gridScale"])[0])
+    perr = float(np.abs(out["geodesicPoints"] - g[name + "_geodesicPoints"]).max()) / h
+    fref = g[name + "_geodesicFlow"]
+    ferr = float(np.abs(out["geodesicFlow"] - fref).max() / np.abs(fref).max())
+    sref = g[name + "_costSensitivity_0"]
+    serr = float(np.abs(out["costSensitivity_0"] - sref).max() / np.abs(sref).max())
+    print(f"{name}: geodesic points {perr:.3e} h, flow {ferr:.3e}, costSensitivity {serr:.3e}", flush=True)
+    assert perr <= 1e-6
+    assert ferr <= 1e-10
+    assert serr <= 1e-9
