@@ -142,6 +142,10 @@ int hfm_plan_adjoint_impl(hfm_plan *p, int ntargets, const double *targets, cons
                           double *d_sensitivity, int64_t *seed_index, double *seed_weight, int max_seeds,
                           int *nseed_out, void *stream);
 
+// Host copy of the per-theta stencil tables of a curvature plan (theta x ndiff, packed offsets
+// and base weights).
+int hfm_plan_curvature_tables(const hfm_plan *p, std::vector<uint32_t> &offs, std::vector<double> &w);
+
 // hfm_plan_geodesics with tips already in grid units (ADim'd) when tips_adim.
 int hfm_plan_geodesics_impl(hfm_plan *p, int ntips, const double *tips, bool tips_adim, double step,
                             double threshold, double volumeBound, double *points, int64_t max_points,

@@ -34,6 +34,9 @@
 #include "../../include/hfm_amd.h"
 #include "selling.h"
 
+// Host copy of a curvature plan's per-theta stencil tables (hfm_plan.hip).
+int hfm_plan_curvature_tables(const hfm_plan *p, std::vector<uint32_t> &offs, std::vector<double> &w);
+
 // Geodesics from tips already in grid units (hfm_plan.hip).
 int hfm_plan_geodesics_impl(hfm_plan *p, int ntips, const double *tips, bool tips_adim, double step,
                             double threshold, double volumeBound, double *points, int64_t max_points,
@@ -533,6 +536,97 @@ void run_impl(hfm_io *io) {
     check(hfm_plan_get_stats(plan, &st));
     // Run_SetupSolver
     io->set_scalar("MaxStencilWidth", (double)st.max_stencil_width);
+    // getStencils (HFMInterface.hxx:360-375): the stencils at the given points, printed as the
+    // reference prints them (EulerianStencil.hxx:76-87, Macros/ExportArrow.h): for each point
+    // {index,{"forward" -> {{diffs}},"symmetric" -> {{diffs}},}}, a difference being
+    // {baseWeight,{offset}}; points out of the domain are skipped with a warning.
+    if (io->has("getStencils")) {
+        const std::vector<double> pts = io->vec("getStencils");
+        if (pts.size() % dim) throw IOError{"IO error : getStencils has the wrong size"};
+        std::vector<uint32_t> toffs;
+        std::vector<double> tw;
+        if (bundle) check(hfm_plan_curvature_tables(plan, toffs, tw));
+        const int nfwd = model == "ReedsSheppForward2" ? 3 : (bundle ? 27 : 0);
+        const int ndiff = bundle ? (int)(tw.size() / dims[2]) : dim * (dim + 1) / 2;
+        std::ostringstream oss;
+        oss << "{";
+        for (size_t q = 0; q + dim <= pts.size(); q += dim) {
+            int64_t idx[3] = {0, 0, 0};
+            bool ok = true;
+            for (int i = 0; i < dim; ++i) {
+                const bool th = bundle && i == 2;
+                const double step = th ? 2 * 3.141592653589793238462643383279502884L / dims[2] : h[i];
+                const double o = th ? -step / 2. : (origin.empty() ? 0. : origin[i]);
+                const double a = std::floor((pts[q + i] - o) / step);
+                if (!(a > -2e9 && a < 2e9)) { ok = false; break; }
+                int64_t c = (int64_t)a;
+                if (th) {
+                    c %= dims[2];
+                    if (c < 0) c += dims[2];
+                } else if (c < 0 || c >= dims[i]) {
+                    ok = false;
+                    break;
+                }
+                idx[i] = c;
+            }
+            if (!ok) {
+                std::ostringstream w;
+                w << "GetStencil error : point {";
+                for (int i = 0; i < dim; ++i) w << (i ? "," : "") << pts[q + i];
+                w << "}is out of range.\n";
+                io->log += w.str();
+                continue;
+            }
+            std::vector<int> off(ndiff * 3, 0);
+            std::vector<double> wt(ndiff, 0.);
+            if (bundle) {
+                for (int k = 0; k < ndiff; ++k) {
+                    const uint32_t pk = toffs[(size_t)idx[2] * ndiff + k];
+                    off[3 * k] = (int8_t)(pk & 0xff);
+                    off[3 * k + 1] = (int8_t)((pk >> 8) & 0xff);
+                    off[3 * k + 2] = (int8_t)((pk >> 16) & 0xff);
+                    wt[k] = tw[(size_t)idx[2] * ndiff + k];
+                }
+            } else {
+                int64_t lin = 0;
+                for (int i = 0; i < dim; ++i) lin = lin * dims[i] + idx[i];
+                const int K = ndiff;
+                const double *mc = field.data() + (is_field ? lin * K : 0);
+                double invhh[6];
+                for (int i = 0; i < dim; ++i)
+                    for (int j = 0; j <= i; ++j) invhh[i * (i + 1) / 2 + j] = 1. / (h[i] * h[j]);
+                int mx = 0;
+                if (dim == 2) {
+                    int8_t o8[3][2];
+                    hfm::riemann_point_stencil<2>(mc, hasDual, invhh, o8, wt.data(), &mx);
+                    for (int k = 0; k < 3; ++k) off[3 * k] = o8[k][0], off[3 * k + 1] = o8[k][1];
+                } else {
+                    int8_t o8[6][3];
+                    hfm::riemann_point_stencil<3>(mc, hasDual, invhh, o8, wt.data(), &mx);
+                    for (int k = 0; k < 6; ++k)
+                        for (int c = 0; c < 3; ++c) off[3 * k + c] = o8[k][c];
+                }
+            }
+            auto diffs = [&](int k0, int k1) {
+                oss << "{{";
+                for (int k = k0; k < k1; ++k) {
+                    oss << (k > k0 ? "," : "") << "{" << wt[k] << ",{";
+                    for (int c = 0; c < dim; ++c) oss << (c ? "," : "") << off[3 * k + c];
+                    oss << "}}";
+                }
+                oss << "}}";
+            };
+            oss << "{{";
+            for (int i = 0; i < dim; ++i) oss << (i ? "," : "") << idx[i];
+            oss << "},{\"forward\" -> ";
+            diffs(0, nfwd);
+            oss << ",\"symmetric\" -> ";
+            diffs(nfwd, ndiff);
+            oss << ",}},";
+        }
+        oss << "}";
+        io->set_string("stencils", oss.str());
+    }
     if (io->has("order") && io->scalar("order") != 1.)
         throw IOError{"Error : order " + std::to_string(io->scalar("order")) +
                       " is not supported by this library (order 1 only)"};

@@ -805,6 +805,22 @@ int hfm_plan_geodesics(hfm_plan *p, int ntips, const double *tips, double step, 
 
 }  // extern "C"
 
+int hfm_plan_curvature_tables(const hfm_plan *p, std::vector<uint32_t> &offs, std::vector<double> &w) {
+    if (!p || !p->shared || !p->stencil_ready) {
+        set_error("hfm_plan_curvature_tables: no curvature stencil tables");
+        return 1;
+    }
+    const size_t n = (size_t)p->g.n[2] * p->ndiff;
+    offs.resize(n);
+    w.resize(n);
+    if (hipMemcpy(offs.data(), p->tab_offs, n * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(w.data(), p->tab_w, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error("hfm_plan_curvature_tables: copy failed");
+        return 1;
+    }
+    return 0;
+}
+
 int hfm_plan_geodesics_impl(hfm_plan *p, int ntips, const double *tips, bool tips_adim, double step,
                             double threshold, double volumeBound, double *points, int64_t max_points,
                             int64_t *lengths, int64_t *npoints, int *nfailed, void *stream) {

@@ -364,6 +364,38 @@ def bypass_cases():
     print("bypass ok")
 
 
+STENCIL_CASES = {
+    "riemann2": (lambda: syn.riemann2(16, seed=21), [[0.3, 0.6], [0.95, 0.05], [1.5, 0.5], [0.0, 0.999]]),
+    "riemann2_dual": (lambda: syn.riemann2(16, seed=22), [[0.41, 0.13]]),
+    "riemann3": (lambda: syn.riemann3(10, seed=23), [[0.5, 0.5, 0.5], [0.05, 0.93, 0.61], [2.0, 0.5, 0.5]]),
+    "rsf2": (lambda: syn.rsf2(12, 8, seed=24), [[0.5, 0.5, 1.0], [0.2, 0.7, 7.5], [0.2, 0.7, -1.0]]),
+    "elastica2": (lambda: syn.elastica2(12, 8, seed=25), [[0.5, 0.5, 2.0], [0.9, 0.1, 5.9]]),
+}
+
+
+def stencil_params(name):
+    make, pts = STENCIL_CASES[name]
+    p = {k: v for k, v in make().items()
+         if k not in ("tips", "inspectSensitivity", "inspectSensitivityWeights", "inspectSensitivityLengths")}
+    if name.endswith("_dual"):
+        p["dualMetric"] = _pack_sym(np.linalg.inv(_unpack_sym(p.pop("metric"))))
+    p["getStencils"] = np.array(pts)
+    return p
+
+
+def stencil_cases():
+    """The 'stencils' string of getStencils (HFMInterface.hxx:360-375), as printed by the reference."""
+    import json
+    out = {}
+    for name in STENCIL_CASES:
+        r = oracle.run_reference(stencil_params(name))
+        assert r["retcode"] == 0, r["log"]
+        out[name] = r["stencils"]
+    with open(os.path.join(HERE, "stencils.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print("stencils ok")
+
+
 def big_case():
     p = syn.riemann3(128, seed=0)
     r = oracle.run_reference(p)
@@ -385,7 +417,7 @@ def main():
     only = [a for a in sys.argv[1:] if not a.startswith("--")]
     if only:  # regenerate the named fixture groups only
         groups = {"geodesics": geodesic_cases, "forward": forward_cases, "walls": wall_cases, "seeds": seed_cases,
-                  "bypass": bypass_cases, "big": big_case}
+                  "bypass": bypass_cases, "stencils": stencil_cases, "big": big_case}
         for a in only:
             groups[a]()
         return
@@ -399,6 +431,7 @@ def main():
     wall_cases()
     seed_cases()
     bypass_cases()
+    stencil_cases()
     if "--big" in sys.argv:
         big_case()
 

@@ -106,3 +106,13 @@ def test_bypass_with_cached_values_vs_golden(name):
     assert perr <= 1e-6
     assert ferr <= 1e-10
     assert serr <= 1e-9
+
+
+@pytest.mark.parametrize("name", list(mg.STENCIL_CASES))
+def test_get_stencils_string_equals_reference(name):
+    """getStencils -> the 'stencils' string, character for character (HFMInterface.hxx:360-375)."""
+    import json
+    with open(os.path.join(GOLD, "stencils.json")) as f:
+        ref = json.load(f)[name]
+    out = hfm.run(mg.stencil_params(name))
+    assert out["stencils"] == ref
