@@ -627,6 +627,35 @@ void run_impl(hfm_io *io) {
         oss << "}";
         io->set_string("stencils", oss.str());
     }
+    // pointToIndex -> indexFromPoint (IndexFromPoint(ADim(p)), unperiodized) and indexToPoint ->
+    // pointFromIndex (ReDim(index + 1/2), the index cast by truncation) (HFMInterface.hxx:377-396)
+    auto axis_scale = [&](int i) { return bundle && i == 2 ? 2 * 3.141592653589793238462643383279502884L / dims[2] : h[i]; };
+    auto axis_origin = [&](int i) {
+        return bundle && i == 2 ? -(double)axis_scale(2) / 2. : (origin.empty() ? 0. : origin[i]);
+    };
+    if (io->has("pointToIndex")) {
+        const std::vector<double> pts = io->vec("pointToIndex");
+        if (pts.size() % dim) throw IOError{"IO error : pointToIndex has the wrong size"};
+        Field f;
+        f.dims = {(int64_t)(pts.size() / dim), (int64_t)dim};
+        for (size_t q = 0; q < pts.size(); ++q) {
+            const int i = (int)(q % dim);
+            f.data.push_back(std::floor((pts[q] - axis_origin(i)) / (double)axis_scale(i)));
+        }
+        io->set_computed("indexFromPoint", std::move(f));
+    }
+    if (io->has("indexToPoint")) {
+        const std::vector<double> idx = io->vec("indexToPoint");
+        if (idx.size() % dim) throw IOError{"IO error : indexToPoint has the wrong size"};
+        Field f;
+        f.dims = {(int64_t)(idx.size() / dim), (int64_t)dim};
+        for (size_t q = 0; q < idx.size(); ++q) {
+            const int i = (int)(q % dim);
+            const double c = (double)(int)idx[q] + 0.5;
+            f.data.push_back(c * (double)axis_scale(i) + axis_origin(i));
+        }
+        io->set_computed("pointFromIndex", std::move(f));
+    }
     if (io->has("order") && io->scalar("order") != 1.)
         throw IOError{"Error : order " + std::to_string(io->scalar("order")) +
                       " is not supported by this library (order 1 only)"};

@@ -391,6 +391,16 @@ def stencil_cases():
         r = oracle.run_reference(stencil_params(name))
         assert r["retcode"] == 0, r["log"]
         out[name] = r["stencils"]
+    # pointToIndex / indexToPoint (HFMInterface.hxx:377-396) on the same problems
+    for name in ("riemann2", "riemann3", "rsf2"):
+        p = stencil_params(name)
+        dim = len(p["dims"])
+        pts = p.pop("getStencils")
+        r = oracle.run_reference(dict(p, pointToIndex=pts, indexToPoint=np.array([[1, 2, 3][:dim], [7, 0, 5][:dim]],
+                                                                               float)))
+        assert r["retcode"] == 0, r["log"]
+        out[name + "_indexFromPoint"] = np.asarray(r["indexFromPoint"]).tolist()
+        out[name + "_pointFromIndex"] = np.asarray(r["pointFromIndex"]).tolist()
     with open(os.path.join(HERE, "stencils.json"), "w") as f:
         json.dump(out, f, indent=1)
     print("stencils ok")

@@ -116,3 +116,17 @@ def test_get_stencils_string_equals_reference(name):
         ref = json.load(f)[name]
     out = hfm.run(mg.stencil_params(name))
     assert out["stencils"] == ref
+
+
+@pytest.mark.parametrize("name", ["riemann2", "riemann3", "rsf2"])
+def test_point_index_conversions_equal_reference(name):
+    """pointToIndex -> indexFromPoint and indexToPoint -> pointFromIndex (HFMInterface.hxx:377-396)."""
+    import json
+    with open(os.path.join(GOLD, "stencils.json")) as f:
+        ref = json.load(f)
+    p = mg.stencil_params(name)
+    dim = len(p["dims"])
+    pts = p.pop("getStencils")
+    out = hfm.run(dict(p, pointToIndex=pts, indexToPoint=np.array([[1, 2, 3][:dim], [7, 0, 5][:dim]], float)))
+    assert np.array_equal(out["indexFromPoint"], np.array(ref[name + "_indexFromPoint"]))
+    assert np.array_equal(out["pointFromIndex"], np.array(ref[name + "_pointFromIndex"]))
