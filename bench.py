@@ -100,7 +100,7 @@ def cpu_baseline(params):
     }, values
 
 
-def serving_leg(count):
+def serving_leg(count, with_cpu=False):
     """Continuous-batching server vs one-at-a-time plain GPU solves on the same request stream."""
     import torch
     from hamiltonfastmarching_amd import serving
@@ -120,6 +120,7 @@ def serving_leg(count):
     torch.cuda.synchronize()
     n2 = sum(r["model"] == "Riemann2" for r in reqs)
     sustained = closed["sustained_solves_per_s"]
+    cpu = cpu_serving(reqs) if with_cpu else None
     return {
         "requests": count, "mix": {"Riemann2": n2, "Riemann3": count - n2},
         "solves_per_s": round(sustained, 1),
@@ -135,7 +136,50 @@ def serving_leg(count):
                                   "p50_ms": round(load["p50_ms"], 3), "p99_ms": round(load["p99_ms"], 3)},
         "sequential_latency": {"p50_ms": round(seq["p50_ms"], 3), "p99_ms": round(seq["p99_ms"], 3)},
         "max_resident_seen": stats["max_resident_seen"],
+        "cpu_aggregate": cpu,
+        "speedup_vs_cpu_aggregate": round(sustained / cpu["solves_per_s"], 1) if cpu and cpu["solves_per_s"] else None,
     }
+
+
+def cpu_serving(reqs, budget_s=15.0, max_workers=16):
+    """The CPU side of the serving comparison: the same request stream solved by independent
+    reference (or oracle) runs on the host cores, one process per core, for about budget_s
+    seconds (workers stop taking requests after that).  Returns the aggregate solves/s."""
+    import threading
+    import oracle
+    workers = max(1, min(max_workers, os.cpu_count() or 1))
+    use_ref = all(oracle.ref_available(m) for m in ("Riemann2", "Riemann3"))
+    host = [{"model": r["model"], "dims": np.array(r["dims"], float), "gridScale": r["gridScale"],
+             "metric": r["metric"].cpu().numpy(), "seeds": np.asarray(r["seeds"])} for r in reqs]
+    lock = threading.Lock()
+    state = {"next": 0, "done": 0, "failed": 0}
+    t0 = time.time()
+
+    def work():
+        while True:
+            with lock:
+                if state["next"] >= len(host) or time.time() - t0 > budget_s:
+                    return
+                i = state["next"]
+                state["next"] += 1
+            if use_ref:
+                ok = oracle.run_reference(host[i]).get("retcode", 1) == 0
+            else:
+                oracle.solve(host[i])
+                ok = True
+            with lock:
+                state["done" if ok else "failed"] += 1
+
+    threads = [threading.Thread(target=work) for _ in range(workers)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    wall = time.time() - t0
+    return {"solves_per_s": round(state["done"] / wall, 2), "workers": workers, "requests": state["done"],
+            "failed": state["failed"], "kind": "reference" if use_ref else "port",
+            "sample": f"first {state['done']} requests of the serving stream, {workers} concurrent single-core "
+                      f"solves, {wall:.1f} s wall"}
 
 
 def floor_rel(u, ref):
@@ -223,7 +267,7 @@ def main():
         "roofline": roofline,
     }
     if not args.no_serving:
-        result["serving"] = serving_leg(args.serving_requests)
+        result["serving"] = serving_leg(args.serving_requests, with_cpu=not args.no_cpu)
     if args.copy_calibration:
         src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
         dst = torch.empty_like(src)
