@@ -136,6 +136,37 @@ __host__ __device__ inline bool shift_coords_m(const Geom &g, const int *c, int 
     return true;
 }
 
+// shift_coords_m without early exits, for gathers that must issue every load before using any:
+// always writes in-grid coordinates to r (clamped where the shift leaves a closed axis) and
+// returns whether the shifted point exists.  Same validity and wrap as shift_coords_m.
+template <int ModelT>
+__device__ inline bool shift_coords_flat(const Geom &g, const int *c, int o0, int o1, int o2, int *r) {
+    constexpr int DIM = ModelShape<ModelT>::DIM, PER = ModelShape<ModelT>::PERIODIC;
+    const int o[3] = {o0, o1, o2};
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k >= DIM) {
+            r[k] = c[k];
+            continue;
+        }
+        const int n = g.n[k];
+        int v = c[k] + o[k];
+        if (PER >> k & 1) {
+            v = v < 0 ? v + n : (v >= n ? v - n : v);
+            if (v < 0 || v >= n) {  // |o| <= 127 may exceed n: full modulo (rare)
+                v %= n;
+                v = v < 0 ? v + n : v;
+            }
+        } else {
+            ok = ok && v >= 0 && v < n;
+            v = v < 0 ? 0 : (v >= n ? n - 1 : v);
+        }
+        r[k] = v;
+    }
+    return ok;
+}
+
 // Device-side stencil data.
 template <typename T> struct StencilData {
     // per-point (Riemann): offs[k * np + p], w[k * np + p]

@@ -367,8 +367,10 @@ __global__ __launch_bounds__(256) void k_fim(Geom g, StencilData<T> sd, T *__res
     unsigned long long my_sweeps = 0, my_visits = 0;
     unsigned wmin = KEY_NONE;
     for (; i < n; i += stride) {
-        const int tile = next;
-        next = i + stride < n ? list[i + stride] : 0;
+        // the current entry goes to a scalar register before the next entry's load is issued,
+        // so that nothing waits on the prefetch until the next iteration
+        const int tile = __builtin_amdgcn_readfirstlane(next);
+        next = list[i + stride < n ? i + stride : i];
         my_sweeps += fim_tile<ModelT, T, MAXSWEEP>(ctx, tile, lane, su[wv], wmin);
         my_visits += 1;
     }

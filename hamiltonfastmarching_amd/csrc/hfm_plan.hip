@@ -309,7 +309,7 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
     const float delta0 = (float)(p->delta_factor > 0 ? p->delta_factor * p->tau : 0.);
     const float binit[5] = {0.f, -__builtin_huge_valf(), delta0, delta0, delta0};
     HIPCHECK(hipMemcpyAsync(p->bounds, binit, sizeof(binit), hipMemcpyHostToDevice, s));
-    const int target = p->round_target > 0 ? p->round_target : std::max(64, std::min(256, p->g.ntiles / 4));
+    const int target = p->round_target > 0 ? p->round_target : std::max(64, std::min(128, p->g.ntiles / 4));
     HIPCHECK(launch_seed_activate(p->seed_p, p->seed_v, nseeds, p->dep_start, p->deps, p->keys, p->tmax,
                                   &p->nextmin[0], s));
 
@@ -325,7 +325,10 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
     a.tmax = p->tmax;
     a.list = p->lists[0];
     a.stats = p->stats;
-    a.grid = (unsigned)std::min<long long>((p->g.ntiles + 3) / 4, p->fim_grid > 0 ? p->fim_grid : 1024);
+    // 640 blocks (2560 waves, 2.5 per SIMD): more waves made the scattered neighbour loads of
+    // every visit slower than the extra concurrency gained (Riemann3 128^3: 3.26 ms per solve
+    // at 640 blocks, 3.39 ms at 1024)
+    a.grid = (unsigned)std::min<long long>((p->g.ntiles + 3) / 4, p->fim_grid > 0 ? p->fim_grid : 640);
     a.max_sweeps = p->max_sweeps;
     std::vector<hipEvent_t> tev;
     long long k = 0;

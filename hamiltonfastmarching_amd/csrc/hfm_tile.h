@@ -64,11 +64,15 @@ __device__ inline unsigned read_pend(const unsigned *slots) {
     return m;
 }
 
-// Band bound of a round: stays at the previous bound while a pending key lies below it,
-// otherwise the smallest pending key + delta; +inf (every pending tile) when delta <= 0.
+// Band bound of a round: a window sliding with the smallest pending key, never moving back --
+// max(previous bound, smallest pending key + delta); +inf (every pending tile) when delta <= 0.
+// A bound that waited at the previous value until every key below it was consumed spent many
+// rounds on the last few tiles of each band (Riemann3 128^3: 104 rounds against 80).  The
+// bound exceeds the smallest pending key, so a round lists at least one tile while any is
+// pending.
 __device__ inline float band_bound(float pend, float bprev, float delta) {
     float B = __builtin_huge_valf();
-    if (delta > 0.f && pend < __builtin_huge_valf()) B = pend < bprev ? bprev : pend + delta;
+    if (delta > 0.f && pend < __builtin_huge_valf()) B = fmaxf(bprev, pend + delta);
     return B;
 }
 
@@ -167,46 +171,74 @@ __device__ inline int fim_tile(const TileCtx<T> &ctx, int tile_in, int lane, T *
     int c[3];
     tiled_coords(g, tile, lane, c);
     const bool fixed = (ctx.fixedmask[tile] >> lane) & 1ull;
-    // Issue the loads the pending marks will need now, so that their latency overlaps the
-    // neighbour loads and the sweeps.  Within one update launch keys and tile maxima only
-    // decrease, so values read early are upper bounds and filter conservatively.
     const int db = ctx.dep_start[tile], de = ctx.dep_start[tile + 1];
-    const int d0 = db + lane < de ? ctx.deps[db + lane] : -1;
-    const unsigned tmax0 = d0 >= 0 ? ctx.tmax[d0] : 0u;
-    const unsigned key0 = d0 >= 0 ? ctx.keys[d0] : 0u;
+    // Loads in order of need, none waiting on another except where an address depends on a
+    // value: own value and stencil; the first 64 dependent tiles; then (below) the neighbour
+    // values, whose addresses come from the stencil, and the dependents' key and tile maximum,
+    // whose addresses come from the dependent list.  The marks after the sweeps use the last
+    // two: within one update launch keys and tile maxima only decrease, so values read early
+    // are upper bounds and filter conservatively.
     const T u0 = u[p];
     LaneStencil<ModelT, T> st;
     st.load(g, ctx.sd, p, c);
+    const bool has_d0 = db + lane < de;
+    int d0raw = tile;
+    if (de > db) d0raw = ctx.deps[has_d0 ? db + lane : de - 1];  // uniform branch
     // Neighbour slots: forward k -> one side, symmetric k -> two sides.  A side inside the tile
     // reads the wave's LDS row (ia/ib = its lane); a side outside reads the row's +inf sentinel
     // (SU_SENTINEL) and contributes its value, frozen for the visit, through hk = the smaller of
     // the outside values.  Each sweep then takes v_k = min(su[ia_k], su[ib_k], hk) without a
     // branch, which is the smaller neighbour value of difference k exactly.
+    //
+    // The gather is straight-line code: every side computes an address (its own point when the
+    // side is absent, inside the tile or invisible), all loads are issued, and only then are the
+    // loaded values selected.  Loads under per-side branches made the compiler wait for each
+    // side's load before the next one, one memory latency per side.
     int ia[NC], ib[NC];
     T hk[NC];
+    unsigned tmax0, key0;
+    {
+        // point indices fit 32 bits: at most 2^26 tiles of 64 points
+        unsigned qa[NC], qb[NC];
+        const unsigned ps = (unsigned)p;
+        bool xa[NC], xb[NC];
 #pragma unroll
-    for (int k = 0; k < NC; ++k) {
-        int o0, o1, o2;
-        unpack_offset(st.off[k], o0, o1, o2);
-        ia[k] = SU_SENTINEL;
-        ib[k] = SU_SENTINEL;
-        T oa = inf, ob = inf;
-        if (!fixed && st.w[k] != T(0)) {
+        for (int k = 0; k < NC; ++k) {
+            int o0, o1, o2;
+            unpack_offset(st.off[k], o0, o1, o2);
+            const bool act = !fixed && st.w[k] != T(0);
             int r[3];
-            if (st.visible(k, 0) && shift_coords_m<ModelT>(g, c, o0, o1, o2, r)) {
-                const int t = tile_of(g, r), l = local_of(g, r);
-                if (t == tile) ia[k] = l;
-                else oa = u[(long long)t * 64 + l];
-            }
+            const bool oka = shift_coords_flat<ModelT>(g, c, o0, o1, o2, r) && act && st.visible(k, 0);
+            const int ta = tile_of(g, r), la = local_of(g, r);
+            ia[k] = oka && ta == tile ? la : SU_SENTINEL;
+            xa[k] = oka && ta != tile;
+            qa[k] = xa[k] ? (unsigned)ta * 64u + (unsigned)la : ps;
             if (k >= NF) {
-                if (st.visible(k, 1) && shift_coords_m<ModelT>(g, c, -o0, -o1, -o2, r)) {
-                    const int t = tile_of(g, r), l = local_of(g, r);
-                    if (t == tile) ib[k] = l;
-                    else ob = u[(long long)t * 64 + l];
-                }
+                const bool okb = shift_coords_flat<ModelT>(g, c, -o0, -o1, -o2, r) && act && st.visible(k, 1);
+                const int tb = tile_of(g, r), lb = local_of(g, r);
+                ib[k] = okb && tb == tile ? lb : SU_SENTINEL;
+                xb[k] = okb && tb != tile;
+                qb[k] = xb[k] ? (unsigned)tb * 64u + (unsigned)lb : ps;
+            } else {
+                ib[k] = SU_SENTINEL;
+                xb[k] = false;
+                qb[k] = ps;
             }
         }
-        hk[k] = oa < ob ? oa : ob;
+        T va[NC], vb[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            va[k] = u[qa[k]];
+            vb[k] = k >= NF ? u[qb[k]] : inf;
+        }
+        const int dq = has_d0 ? d0raw : tile;
+        tmax0 = ctx.tmax[dq];
+        key0 = ctx.keys[dq];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            const T oa = xa[k] ? va[k] : inf, ob = xb[k] ? vb[k] : inf;
+            hk[k] = oa < ob ? oa : ob;
+        }
     }
     T ucur = u0;
     su[lane] = ucur;
@@ -243,8 +275,8 @@ __device__ inline int fim_tile(const TileCtx<T> &ctx, int tile_in, int lane, T *
         if (lane == 0) ctx.tmax[tile] = tm;
         const unsigned kmin = wave_min_u32(lowered ? key_of((float)ucur) : KEY_NONE);
         {
-            const bool lw = d0 >= 0 && tmax0 > kmin && key0 > kmin;
-            if (lw) atomicMin(&ctx.keys[d0], kmin);
+            const bool lw = has_d0 && tmax0 > kmin && key0 > kmin;
+            if (lw) atomicMin(&ctx.keys[d0raw], kmin);
             const unsigned m = wave_min_u32(lw ? kmin : KEY_NONE);
             wmin = m < wmin ? m : wmin;
         }
