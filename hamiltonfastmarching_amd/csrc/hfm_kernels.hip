@@ -322,6 +322,11 @@ __global__ __launch_bounds__(64) void k_seed_activate(const long long *__restric
 // keys this kernel leaves pending and by the marks of the update kernel of round k.
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(SCHED_BLOCK) void k_schedule(int ntiles, unsigned *__restrict__ keys, SchedRound sr) {
+    // the tile's key is loaded first, in parallel with the round parameters (their stores
+    // below could otherwise alias it and order the load after them)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned k = KEY_NONE;
+    if (t < ntiles) k = keys[t];
     const float d0 = *sr.delta0;
     float delta = d0;
     if (delta > 0.f) delta = adapt_delta(*sr.delta_in, *sr.prev_count, sr.target, 0.5f * d0, 64.f * d0);
@@ -330,9 +335,6 @@ __global__ __launch_bounds__(SCHED_BLOCK) void k_schedule(int ntiles, unsigned *
         *sr.bound_out = B;
         *sr.delta_out = delta;
     }
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned k = KEY_NONE;
-    if (t < ntiles) k = keys[t];
     const bool take = band_take(k, B);
     const unsigned left = (!take && k != KEY_NONE) ? k : KEY_NONE;
     if (take) keys[t] = KEY_NONE;

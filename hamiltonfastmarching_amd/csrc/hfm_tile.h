@@ -20,22 +20,29 @@ __device__ inline float key_value(unsigned k) {
     const unsigned b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
     return __uint_as_float(b);
 }
-__device__ inline unsigned wave_min_u32(unsigned v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const unsigned w = (unsigned)__shfl_xor((int)v, o);
-        v = w < v ? w : v;
+// Wave-wide minimum / maximum, returned to every lane; every lane of the wave must be active.
+// DPP inclusive scan: rows of 16 lanes with row_shr 1, 2, 4, 8, then across rows with
+// row_bcast 15 (into rows 1 and 3) and row_bcast 31 (into rows 2 and 3), after which lane 63
+// holds the result.  Lanes without a source keep the identity.  Six dependent VALU steps
+// instead of six LDS permutes, each waiting for the LDS.
+template <bool MAX> __device__ inline unsigned wave_reduce_u32(unsigned v) {
+    constexpr int id = MAX ? 0 : (int)0xffffffffu;
+#define HFM_DPP_STEP(ctrl, rows)                                                                    \
+    {                                                                                              \
+        const unsigned w = (unsigned)__builtin_amdgcn_update_dpp(id, (int)v, ctrl, rows, 0xf, false); \
+        v = MAX ? (w > v ? w : v) : (w < v ? w : v);                                               \
     }
-    return v;
+    HFM_DPP_STEP(0x111, 0xf)  // row_shr:1
+    HFM_DPP_STEP(0x112, 0xf)  // row_shr:2
+    HFM_DPP_STEP(0x114, 0xf)  // row_shr:4
+    HFM_DPP_STEP(0x118, 0xf)  // row_shr:8
+    HFM_DPP_STEP(0x142, 0xa)  // row_bcast:15
+    HFM_DPP_STEP(0x143, 0xc)  // row_bcast:31
+#undef HFM_DPP_STEP
+    return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
 }
-__device__ inline unsigned wave_max_u32(unsigned v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const unsigned w = (unsigned)__shfl_xor((int)v, o);
-        v = w > v ? w : v;
-    }
-    return v;
-}
+__device__ inline unsigned wave_min_u32(unsigned v) { return wave_reduce_u32<false>(v); }
+__device__ inline unsigned wave_max_u32(unsigned v) { return wave_reduce_u32<true>(v); }
 __device__ inline unsigned relaxed_load(const unsigned *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -138,6 +145,7 @@ constexpr int SU_ROW = 65;
 
 // Minimum without NaN handling (no operand is NaN here): a compare and a select.
 template <typename T> __device__ inline T min_nn(T a, T b) { return b < a ? b : a; }
+__device__ inline float min_nn(float a, float b) { return __builtin_fminf(a, b); }  // one v_min_f32
 
 template <typename T> struct TileCtx {
     Geom g;
