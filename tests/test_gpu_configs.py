@@ -1,0 +1,142 @@
+"""The workloads at their full sizes (HFM inputs from the seeded generator):
+
+* Riemann2 256^2, h = 1/256, FP64 update: values against the oracle's FP64 Fast Marching to
+  1e-12 relative, Selling offsets bit-exact.
+* ReedsSheppForward2 256 x 256 x 64, xi = 0.1, 4 seeds, FP32 update: values against the oracle to
+  1e-5 relative; geodesics from the 32 tips, FP64 solve: bit-exact against the oracle's discrete
+  solver on the GPU's own values and active sets, and within one cell of the oracle's curves.
+* Elastica2 192 x 192 x 64, xi = 0.2, 1 seed, 32 targets, FP64: the adjoint (costSensitivity of
+  the sum of the target arrival times) against the oracle's BackwardVariation (which is
+  bit-exact against the reference's) to 1e-9 of the largest magnitude, and against central finite
+  differences (eps = 1e-3 on the cost) at 64 cells drawn from the support of the sensitivity.
+
+The oracle runs on the host (about 6 s for the ReedsSheppForward2 solve and 20 s for the
+Elastica2 solve with its backward sweep)."""
+import numpy as np
+import pytest
+
+import oracle
+import hamiltonfastmarching_amd as hfm
+from hamiltonfastmarching_amd import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+
+SENS_KEYS = ("tips", "inspectSensitivity", "inspectSensitivityWeights", "inspectSensitivityLengths")
+
+
+def _strip(p):
+    return {k: v for k, v in p.items() if k not in SENS_KEYS}
+
+
+def _floor_rel(u, ref):
+    fin = np.isfinite(ref)
+    pos = ref[fin & (ref > 0)]
+    m = fin & (ref >= pos.min())
+    return float(np.max(np.abs(u[m] - ref[m]) / ref[m]))
+
+
+def _split(points, lengths):
+    out, s = [], 0
+    for n in np.asarray(lengths, int):
+        out.append(points[s:s + n])
+        s += n
+    return out
+
+
+def test_riemann2_256_fp64():
+    print("Riemann2 256^2 FP64", flush=True)
+    p = syn.riemann2(256, seed=0)
+    ref = oracle.solve(p)["values"]
+    out = hfm.run(dict(p, precision=64.0))
+    u = out["values"]
+    assert np.array_equal(np.isfinite(u), np.isfinite(ref))
+    err = _floor_rel(u, ref)
+    print(f"  max relative error {err:.2e}", flush=True)
+    assert err <= 1e-12
+    import torch
+    plan = hfm.Plan("Riemann2", (256, 256), p["gridScale"], precision=64)
+    plan.set_metric(torch.from_numpy(p["metric"]).cuda())
+    off, _ = plan.stencils()
+    roff, _ = oracle.riemann_stencils((256, 256), p["gridScale"], p["metric"])
+    assert np.array_equal(off, roff)
+
+
+def test_rsf2_full_values_fp32():
+    print("ReedsSheppForward2 256x256x64 FP32", flush=True)
+    p = _strip(syn.rsf2(256, 64, seed=0))
+    ref = oracle.solve(p)["values"]
+    out = hfm.run(dict(p, precision=32.0))
+    u = out["values"]
+    assert np.array_equal(np.isfinite(u), np.isfinite(ref))
+    err = _floor_rel(u, ref)
+    print(f"  max relative error {err:.2e}", flush=True)
+    assert err <= 1e-5
+
+
+def test_rsf2_full_geodesics_fp64():
+    print("ReedsSheppForward2 256x256x64 geodesics from 32 tips, FP64", flush=True)
+    full = syn.rsf2(256, 64, seed=0)
+    p, tips = _strip(full), full["tips"]
+    out = hfm.run(dict(p, tips=tips, exportValues=1.0, exportActiveNeighs=1.0, precision=64.0))
+    pts, lens = out["geodesicPoints"], out["geodesicLengths"].astype(np.int64)
+    assert len(lens) == 32 and pts.shape == (lens.sum(), 3)
+    # the oracle's discrete solver on this solve's own values and active sets: bit-exact
+    gp, gl = oracle.geodesics_given(p, out["values"], out["activeNeighs"], tips)
+    assert np.array_equal(lens, gl) and np.array_equal(pts, gp)
+    # the oracle's own solve and geodesics: the same curves, point for point within one cell
+    op, ol = oracle.geodesics(p, tips)
+    h = float(p["gridScale"])
+    close = 0
+    for a, b in zip(_split(pts, lens), _split(op, ol)):
+        d = np.sqrt(((a[:, None, :2] - b[None, :, :2]) ** 2).sum(-1)).min(axis=1)
+        close += d.max() < h
+    print(f"  {int(lens.sum())} points; {close}/32 curves within one cell of the oracle's", flush=True)
+    assert close == 32
+
+
+def test_elastica2_full_adjoint_vs_reference_and_fd():
+    import torch
+    print("Elastica2 192x192x64 adjoint, FP64", flush=True)
+    full = syn.elastica2(192, 64, seed=0)
+    p, targets = _strip(full), full["inspectSensitivity"]
+    w = np.ones(len(targets))
+    dims = (192, 192, 64)
+    plan = hfm.Plan("Elastica2", dims, float(p["gridScale"]), origin=p["origin"], precision=64, xi=float(p["xi"]),
+                    eps=float(p["eps"]))
+    cost = torch.tensor(p["cost"], dtype=torch.float64, device="cuda")
+    plan.set_cost(cost).solve(p["seeds"])
+    sens = plan.adjoint(targets, w)[0].cpu().numpy()
+    # the reference's backward differentiation (the oracle's, bit-exact against it)
+    _, sref, _ = oracle.solve_backward(p, targets, w)
+    scale = np.abs(sref).max()
+    err_ref = float(np.abs(sens - sref).max() / scale)
+    print(f"  vs oracle BackwardVariation: max error {err_ref:.2e} of the largest magnitude", flush=True)
+    assert err_ref <= 1e-9
+    # central finite differences of J = sum of the target arrival times at 64 cells of the support
+    idx = hfm._snap(plan, targets)
+    flat = [int(np.ravel_multi_index(i, dims)) for i in idx]
+
+    def J(c):
+        plan.set_cost(c).solve(p["seeds"])
+        return float(plan.values().flatten()[flat].sum())
+
+    rng = np.random.Generator(np.random.PCG64(7))
+    support = np.flatnonzero(np.abs(sens.ravel()) > 1e-6 * scale)
+    cells = rng.choice(support, size=64, replace=False)
+    eps = 1e-3
+    fd = np.empty(64)
+    cf = cost.flatten()
+    for q, cidx in enumerate(cells):
+        cp = cf.clone()
+        cp[cidx] += eps
+        cm = cf.clone()
+        cm[cidx] -= eps
+        fd[q] = (J(cp.view(dims)) - J(cm.view(dims))) / (2 * eps)
+    s = sens.ravel()[cells]
+    rel = np.abs(fd - s) / np.abs(s)
+    print(f"  vs central differences at 64 support cells: median rel {np.median(rel):.2e}, "
+          f"max rel {rel.max():.2e}, max abs {np.abs(fd - s).max():.2e} (largest |sens| {scale:.2e})", flush=True)
+    # a cell whose perturbation switches an upwind choice has a one-sided derivative: allow 2
+    assert np.sum(rel > 1e-3) <= 2
+    assert np.median(rel) <= 1e-5
+    assert np.abs(fd - s).max() <= 1e-5 * scale
