@@ -31,6 +31,9 @@ Printed JSON fields beyond the driver's contract:
                 oracle's C++ restatement when the reference binary is absent.
   accuracy      max relative error of the FP32 values vs the reference FP64 values (floored at
                 the smallest nonzero reference value), from the CPU-baseline run.
+  residual      final FP64 residual check of the last timed solve (hfm_plan_residual): the
+                largest relative deviation between each reached non-seed value and the FP64
+                root of its local scheme, and the finite / infinite mismatches.
 """
 from __future__ import annotations
 
@@ -225,6 +228,8 @@ def main():
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / max(args.steps, 1)
     st = plan.stats()
+    # final FP64 residual check of the last timed solve (outside the timed region)
+    res_dev, res_bad = plan.residual()
 
     # Instrumented solve: per-launch hipEvent timing of the update kernel.
     plan.set_timing(True)
@@ -265,6 +270,9 @@ def main():
         "solve_breakdown_ms": {"stencil_and_deps": round(st["stencil_ms"], 3), "fim": round(st["solve_ms"], 3)},
         "fim_iterations": st["iterations"], "tile_visits": st["tile_visits"], "sweeps": st["sweeps"],
         "roofline": roofline,
+        "residual": {"max_rel_dev": res_dev, "mismatches": res_bad,
+                     "definition": "FP64 recompute of the local scheme at every reached non-seed point: "
+                                   "max |Lambda(u) - u| / |u|"},
     }
     if not args.no_serving:
         result["serving"] = serving_leg(args.serving_requests, with_cpu=not args.no_cpu)

@@ -268,6 +268,15 @@ class Plan:
         self._check(self._L.hfm_plan_active_offsets(self._h, ctypes.c_void_p(out.data_ptr()), _stream_ptr(torch)))
         return out
 
+    def residual(self):
+        """Final FP64 residual check of the last solve: (largest |Lambda(u) - u| / |u| over the
+        reached non-seed points, number of points where exactly one of Lambda(u), u is finite)."""
+        torch = _torch()
+        d = ctypes.c_double(0.0)
+        n = ctypes.c_int64(0)
+        self._check(self._L.hfm_plan_residual(self._h, ctypes.byref(d), ctypes.byref(n), _stream_ptr(torch)))
+        return d.value, n.value
+
     def geodesics(self, tips, step=0.25, threshold=0.001, volume_bound=0.0):
         """Geodesics backtracked from `tips` (ntips x dim physical points) by the discrete solver
         (GeodesicDiscreteSolver.hxx): (geodesicPoints (M x dim), geodesicLengths (ntips))."""

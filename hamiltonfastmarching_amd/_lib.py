@@ -76,6 +76,7 @@ SIGNATURES = [
     ("hfm_plan_geodesics", c_int, [c_void_p, c_int, P(c_double), c_double, c_double, c_double, P(c_double),
                                    c_int64, P(c_int64), P(c_int64), P(c_int), c_void_p]),
     ("hfm_plan_n_active_slots", c_int, [c_void_p]),
+    ("hfm_plan_residual", c_int, [c_void_p, P(c_double), P(c_int64), c_void_p]),
     ("hfm_plan_adjoint", c_int, [c_void_p, c_int, P(c_double), P(c_double), c_void_p, P(c_int64), P(c_double),
                                  c_int, P(c_int), c_void_p]),
     ("hfm_plan_forward_variation", c_int, [c_void_p, c_int, c_void_p, c_int, P(c_double), c_void_p, c_void_p]),

@@ -136,9 +136,10 @@ __host__ __device__ inline bool shift_coords_m(const Geom &g, const int *c, int 
     return true;
 }
 
-// shift_coords_m without early exits, for gathers that must issue every load before using any:
-// always writes in-grid coordinates to r (clamped where the shift leaves a closed axis) and
-// returns whether the shifted point exists.  Same validity and wrap as shift_coords_m.
+// shift_coords_m without early exits, for gathers that issue every load before using any.
+// Writes the shifted coordinates to r (wrapped on periodic axes) and returns whether the
+// shifted point exists.  Coordinates off a closed axis are written unchanged (out of range):
+// whatever is derived from them must not be used for a point that does not exist.
 template <int ModelT>
 __device__ inline bool shift_coords_flat(const Geom &g, const int *c, int o0, int o1, int o2, int *r) {
     constexpr int DIM = ModelShape<ModelT>::DIM, PER = ModelShape<ModelT>::PERIODIC;
@@ -159,8 +160,7 @@ __device__ inline bool shift_coords_flat(const Geom &g, const int *c, int o0, in
                 v = v < 0 ? v + n : v;
             }
         } else {
-            ok = ok && v >= 0 && v < n;
-            v = v < 0 ? 0 : (v >= n ? n - 1 : v);
+            ok = ok && (unsigned)v < (unsigned)n;  // one unsigned compare: 0 <= v < n
         }
         r[k] = v;
     }
@@ -398,6 +398,15 @@ struct alignas(256) RedSlot {
     int pad_[59];
 };
 static_assert(sizeof(RedSlot) == 256, "RedSlot is one 256-byte line");
+
+// Slot of the final residual check (same RED_SLOTS spread, same buffer): largest relative
+// deviation (bits of a nonnegative double) and finite / infinite mismatches.
+struct alignas(256) ResSlot {
+    unsigned long long maxbits;
+    unsigned long long mismatch;
+    unsigned long long pad_[30];
+};
+static_assert(sizeof(ResSlot) == sizeof(RedSlot), "ResSlot reuses the RedSlot buffer");
 
 #if defined(__HIPCC__)
 __device__ inline RedSlot *red_slot(RedSlot *red) {

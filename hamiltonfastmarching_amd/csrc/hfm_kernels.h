@@ -109,6 +109,9 @@ hipError_t launch_export_values(const Geom &g, const T *u, double *out, long lon
 template <typename T>
 hipError_t launch_flow(int model, const Geom &g, const StencilData<T> &sd, const T *u, const unsigned long long *fm,
                        const FlowScale &fs, double *flow, double *act, double *aoffs, long long np, hipStream_t s);
+template <typename T>
+hipError_t launch_residual(int model, const Geom &g, const StencilData<T> &sd, const T *u,
+                           const unsigned long long *fm, long long np, ResSlot *out, hipStream_t s);
 
 }  // namespace hfm
 

@@ -673,4 +673,96 @@ template hipError_t launch_flow<double>(int, const Geom &, const StencilData<dou
                                         const unsigned long long *, const FlowScale &, double *, double *, double *,
                                         long long, hipStream_t);
 
+// ---------------------------------------------------------------------------------------------
+// Final residual check of a converged solve, in FP64.  The discrete solution is the fixed point
+// u = Lambda(u) of the local Hopf-Lax solve (EulerianStencil.hxx:30-72,104-153): at every reached
+// point that is not a seed, u(x) is the root of sum_k w_k max(0, u - v_k)^2 = 1 over the smaller
+// neighbour value v_k of each difference.  This recomputes that root in FP64 (the reference's
+// operation order) from the stored values and weights and reports, over all such points, the
+// largest |Lambda(u) - u| / |u| and the number of points where exactly one of the two is finite.
+// A point the iteration left unconverged shows as a large deviation.
+// ---------------------------------------------------------------------------------------------
+template <int ModelT, typename T>
+__global__ __launch_bounds__(256) void k_residual(Geom g, StencilData<T> sd, const T *__restrict__ u,
+                                                  const unsigned long long *__restrict__ fixedmask, long long np,
+                                                  ResSlot *__restrict__ out) {
+    using S = ModelShape<ModelT>;
+    constexpr int NF = S::NFWD, NC = NF + S::NSYM;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double dev = 0.;
+    int bad = 0;
+    if (p < np) {
+        const int tile = (int)(p >> 6), lane = (int)(p & 63);
+        int c[3];
+        tiled_coords(g, tile, lane, c);
+        const bool fixed = (fixedmask[tile] >> lane) & 1ull;
+        if (in_grid(g, c) && !fixed) {
+            const double inf = __builtin_huge_val();
+            LaneStencil<ModelT, T> st;
+            st.load(g, sd, p, c);
+            double v[NC], w[NC];
+            for (int k = 0; k < NC; ++k) {
+                int o0, o1, o2;
+                unpack_offset(st.off[k], o0, o1, o2);
+                w[k] = (double)st.w[k];
+                v[k] = inf;
+                if (st.w[k] == T(0)) continue;
+                int r[3];
+                if (st.visible(k, 0) && shift_coords_m<ModelT>(g, c, o0, o1, o2, r))
+                    v[k] = (double)u[(long long)tile_of(g, r) * 64 + local_of(g, r)];
+                if (k >= NF && st.visible(k, 1) && shift_coords_m<ModelT>(g, c, -o0, -o1, -o2, r)) {
+                    const double vb = (double)u[(long long)tile_of(g, r) * 64 + local_of(g, r)];
+                    v[k] = vb < v[k] ? vb : v[k];
+                }
+            }
+            uint32_t used = 0;
+            const double lam = local_solve<NC, double>(v, w, used);
+            const double uu = (double)u[p];
+            const bool fl = lam < inf, fu = uu < inf;
+            if (fl != fu) {
+                bad = 1;
+            } else if (fl) {
+                const double den = fmax(fabs(uu), fabs(lam));
+                dev = den > 0. ? fabs(lam - uu) / den : 0.;
+            }
+        }
+    }
+    // nonnegative doubles order like their bit patterns: wave maximum on the bits
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double d = __shfl_xor(dev, o);
+        dev = d > dev ? d : dev;
+        bad += __shfl_xor(bad, o);
+    }
+    if ((threadIdx.x & 63) == 0 && (dev > 0. || bad)) {
+        const unsigned wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+        ResSlot *r = out + (wave & (RED_SLOTS - 1));
+        atomicMax(&r->maxbits, (unsigned long long)__double_as_longlong(dev));
+        atomicAdd(&r->mismatch, (unsigned long long)bad);
+    }
+}
+
+template <int M, typename T>
+static void residual_one(const Geom &g, const StencilData<T> &sd, const T *u, const unsigned long long *fm,
+                         long long np, ResSlot *out, hipStream_t s) {
+    hipLaunchKernelGGL((k_residual<M, T>), dim3(blocks_for(np, 256)), dim3(256), 0, s, g, sd, u, fm, np, out);
+}
+
+template <typename T>
+hipError_t launch_residual(int model, const Geom &g, const StencilData<T> &sd, const T *u,
+                           const unsigned long long *fm, long long np, ResSlot *out, hipStream_t s) {
+    switch (model) {
+        case MODEL_RIEMANN2: residual_one<MODEL_RIEMANN2, T>(g, sd, u, fm, np, out, s); break;
+        case MODEL_RIEMANN3: residual_one<MODEL_RIEMANN3, T>(g, sd, u, fm, np, out, s); break;
+        case MODEL_RSF2: residual_one<MODEL_RSF2, T>(g, sd, u, fm, np, out, s); break;
+        case MODEL_ELASTICA2: residual_one<MODEL_ELASTICA2, T>(g, sd, u, fm, np, out, s); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+template hipError_t launch_residual<float>(int, const Geom &, const StencilData<float> &, const float *,
+                                           const unsigned long long *, long long, ResSlot *, hipStream_t);
+template hipError_t launch_residual<double>(int, const Geom &, const StencilData<double> &, const double *,
+                                            const unsigned long long *, long long, ResSlot *, hipStream_t);
+
 }  // namespace hfm

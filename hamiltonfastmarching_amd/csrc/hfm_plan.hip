@@ -799,6 +799,45 @@ int hfm_plan_active_offsets(hfm_plan *p, double *d_offsets, void *stream) {
 
 int hfm_plan_n_active_slots(const hfm_plan *p) { return p ? p->ndiff : 0; }
 
+int hfm_plan_residual(hfm_plan *p, double *max_rel, int64_t *n_mismatch, void *stream) {
+    if (!p || !max_rel || !n_mismatch) {
+        set_error("hfm_plan_residual: null argument");
+        return 1;
+    }
+    if (!p->solved) {
+        set_error("hfm_plan_residual: no solve has run");
+        return 1;
+    }
+    try {
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        if (p->walls && p->vis_dirty) build_visibility(p, s);
+        ResSlot *slots = reinterpret_cast<ResSlot *>(p->red);
+        HIPCHECK(hipMemsetAsync(slots, 0, RED_SLOTS * sizeof(ResSlot), s));
+        if (p->precision == 32)
+            HIPCHECK(launch_residual<float>(p->model, p->g, p->sd<float>(), static_cast<const float *>(p->u),
+                                            p->fixedmask, p->np, slots, s));
+        else
+            HIPCHECK(launch_residual<double>(p->model, p->g, p->sd<double>(), static_cast<const double *>(p->u),
+                                             p->fixedmask, p->np, slots, s));
+        std::vector<ResSlot> h(RED_SLOTS);
+        HIPCHECK(hipMemcpyAsync(h.data(), slots, RED_SLOTS * sizeof(ResSlot), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        unsigned long long mb = 0, mm = 0;
+        for (const ResSlot &r : h) {
+            mb = r.maxbits > mb ? r.maxbits : mb;
+            mm += r.mismatch;
+        }
+        double d;
+        std::memcpy(&d, &mb, sizeof(d));
+        *max_rel = d;
+        *n_mismatch = (int64_t)mm;
+    } catch (const HipFail &f) {
+        set_error("hfm_plan_residual: " + f.msg);
+        return 1;
+    }
+    return 0;
+}
+
 int hfm_plan_geodesics(hfm_plan *p, int ntips, const double *tips, double step, double threshold,
                        double volumeBound, double *points, int64_t max_points, int64_t *lengths, int64_t *npoints,
                        int *nfailed, void *stream) {

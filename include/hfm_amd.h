@@ -163,6 +163,18 @@ int hfm_plan_flow(hfm_plan *plan, double *d_flow, double *d_active, void *stream
 int hfm_plan_active_offsets(hfm_plan *plan, double *d_offsets, void *stream);
 int hfm_plan_n_active_slots(const hfm_plan *plan);
 
+/* Final residual check of the last solve (or of values loaded with hfm_plan_load_values), in
+ * FP64: at every reached non-seed point, the root of the local scheme
+ * sum_k w_k max(0, u - v_k)^2 = 1 (EulerianStencil.hxx:30-72,104-153; v_k the smaller
+ * neighbour value of difference k, over visible sides) is recomputed in the reference's FP64
+ * operation order from the stored values and weights.  *max_rel receives the largest
+ * |root - u| / max(|u|, |root|), *n_mismatch the number of points where exactly one of root and
+ * u is finite.  A converged FP64 solve gives about 1e-15, an FP32 one the FP32 rounding of the
+ * local solve (about 1e-7); a point left unconverged shows as a large deviation.  Synchronous
+ * on `stream`.  The reference checks nothing of the kind (its heap order makes every accepted
+ * value final); this is the check of the iterative solver's convergence. */
+int hfm_plan_residual(hfm_plan *plan, double *max_rel, int64_t *n_mismatch, void *stream);
+
 /* Geodesic backtracking from tips (GeodesicDiscreteSolver.hxx:17-231, driver
  * HFMInterface.hxx:452-487): the discrete solver, one wavefront per tip, on the last solve.
  *   tips: ntips x dim host physical points; step / threshold / volumeBound: geodesicStep
