@@ -71,7 +71,11 @@ struct hfm_plan {
     unsigned *nextmin = nullptr;   // 2 alternating smallest pending keys
     float *bounds = nullptr;       // 2 alternating band bounds + 2 alternating band widths
     hfm::RedSlot *red = nullptr;   // spread launch-wide reductions (mean one-step time, widest stencil)
-    std::vector<hfm::RedSlot> h_red;  // host copy of the slots
+    hfm::RedSlot *h_red = nullptr;  // pinned host copy of the slots (asynchronous read-back)
+    // stencil-phase events of the last set_metric, read when the statistics are asked for (the
+    // phase ends without a host synchronisation)
+    hipEvent_t stencil_ev[2] = {nullptr, nullptr};
+    bool stencil_ev_pending = false;
     double tau = 0.;               // mean one-step time of the current stencils
     double delta_factor = 8.;      // initial band width in units of tau (<= 0: unordered)
     int fim_grid = 0;              // update-kernel blocks of 4 waves (0: 640)

@@ -81,7 +81,9 @@ __global__ __launch_bounds__(256) void k_riemann_stencil(Geom g, const double *_
 // points x of T (nonzero weights only).  deps(S) = tiles T != S having S as a source: the tiles
 // to re-run when S changes.  Pass 0 counts, pass 1 fills (CSR offsets from an exclusive scan).
 // ---------------------------------------------------------------------------------------------
-constexpr int HASH_SLOTS = 2048;
+// A tile has at most 64 points x 12 sides = 768 distinct source tiles, so 1024 slots always leave
+// a free slot for the linear probing.
+constexpr int HASH_SLOTS = 1024;
 
 template <int ModelT, typename T>
 __global__ __launch_bounds__(64) void k_tile_deps(Geom g, StencilData<T> sd, int pass, int *__restrict__ dep_count,
