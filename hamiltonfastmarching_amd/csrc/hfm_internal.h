@@ -66,6 +66,9 @@ struct hfm_plan {
     long long deps_cap = 0, ndeps = 0;
     int *lists[1] = {nullptr};     // tiles listed by the current round
     int *counts = nullptr;         // 2 alternating list counters
+    int *dep_src = nullptr;        // tile-graph build: each tile's sources (<= 64 kept, 64 per tile)
+    int *dep_nsrc = nullptr;       // tile-graph build: number of sources per tile
+    int *scan_work = nullptr;      // multi-block exclusive scan of the tile-graph counts
     unsigned *keys = nullptr;      // per-tile scheduling key (KEY_NONE: not pending)
     unsigned *tmax = nullptr;      // per-tile largest value (as key) after its last visit
     unsigned *nextmin = nullptr;   // 2 alternating smallest pending keys

@@ -77,7 +77,7 @@ hipError_t launch_riemann_stencil(const Geom &g, const double *metric, int is_fi
                                   uint32_t *offs, T *w, long long np, int *status, RedSlot *red, hipStream_t s);
 template <typename T>
 hipError_t launch_tile_deps(int model, const Geom &g, const StencilData<T> &sd, int pass, int *cnt, const int *start,
-                            int *fill, int *deps, int *ovf, hipStream_t s);
+                            int *fill, int *deps, int *srcbuf, int *srcn, hipStream_t s);
 // Solver state (values, fixed mask) from row-major FP64 arrival times and activeNeighs.
 template <typename T>
 hipError_t launch_load_state(const Geom &g, const double *values, const double *active, T *u,

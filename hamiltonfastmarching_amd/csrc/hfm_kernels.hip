@@ -88,12 +88,26 @@ constexpr int HASH_SLOTS = 1024;
 template <int ModelT, typename T>
 __global__ __launch_bounds__(64) void k_tile_deps(Geom g, StencilData<T> sd, int pass, int *__restrict__ dep_count,
                                                   const int *__restrict__ dep_start, int *__restrict__ dep_fill,
-                                                  int *__restrict__ deps, int *__restrict__ overflow) {
+                                                  int *__restrict__ deps, int *__restrict__ srcbuf,
+                                                  int *__restrict__ srcn) {
     using S = ModelShape<ModelT>;
     constexpr int NC = S::NFWD + S::NSYM;
     __shared__ int set[HASH_SLOTS];
     const int tile = blockIdx.x;
     const int lane = threadIdx.x;
+    // With srcbuf, pass 0 keeps each tile's source list when it has at most 64 entries (64 per
+    // tile), and pass 1 fills from it instead of recomputing the stencil walk and the hash.
+    if (pass == 1 && srcbuf) {
+        const int ns = srcn[tile];
+        if (ns <= 64) {
+            if (lane < ns) {
+                const int src = srcbuf[(long long)tile * 64 + lane];
+                const int q = atomicAdd(&dep_fill[src], 1);
+                deps[dep_start[src] + q] = tile;
+            }
+            return;
+        }
+    }
     for (int i = lane; i < HASH_SLOTS; i += 64) set[i] = -1;
     __syncthreads();
     const long long p = (long long)tile * 64 + lane;
@@ -155,7 +169,10 @@ __global__ __launch_bounds__(64) void k_tile_deps(Geom g, StencilData<T> sd, int
             deps[dep_start[src] + q] = tile;
         }
     }
-    (void)overflow;
+    if (pass == 0 && srcbuf) {
+        if (lane == 0) srcn[tile] = nsrc;
+        if (nsrc <= 64 && lane < nsrc) srcbuf[(long long)tile * 64 + lane] = set[lane];
+    }
 }
 
 // Exclusive scans of int counts.  The unit of work is a chunk of 4096 consecutive counts taken
@@ -543,32 +560,33 @@ template hipError_t launch_riemann_stencil<double>(const Geom &, const double *,
 
 template <int M, typename T>
 static void deps_pass(const Geom &g, const StencilData<T> &sd, int pass, int *cnt, const int *start, int *fill,
-                      int *deps, int *ovf, hipStream_t s) {
-    hipLaunchKernelGGL((k_tile_deps<M, T>), dim3(g.ntiles), dim3(64), 0, s, g, sd, pass, cnt, start, fill, deps, ovf);
+                      int *deps, int *srcbuf, int *srcn, hipStream_t s) {
+    hipLaunchKernelGGL((k_tile_deps<M, T>), dim3(g.ntiles), dim3(64), 0, s, g, sd, pass, cnt, start, fill, deps, srcbuf,
+                       srcn);
 }
 
 template <typename T>
 hipError_t launch_tile_deps(int model, const Geom &g, const StencilData<T> &sd, int pass, int *cnt, const int *start,
-                            int *fill, int *deps, int *ovf, hipStream_t s) {
+                            int *fill, int *deps, int *srcbuf, int *srcn, hipStream_t s) {
     switch (model) {
-        case MODEL_RIEMANN2: deps_pass<MODEL_RIEMANN2, T>(g, sd, pass, cnt, start, fill, deps, ovf, s); break;
-        case MODEL_RIEMANN3: deps_pass<MODEL_RIEMANN3, T>(g, sd, pass, cnt, start, fill, deps, ovf, s); break;
-        case MODEL_RSF2: deps_pass<MODEL_RSF2, T>(g, sd, pass, cnt, start, fill, deps, ovf, s); break;
-        case MODEL_ELASTICA2: deps_pass<MODEL_ELASTICA2, T>(g, sd, pass, cnt, start, fill, deps, ovf, s); break;
+        case MODEL_RIEMANN2: deps_pass<MODEL_RIEMANN2, T>(g, sd, pass, cnt, start, fill, deps, srcbuf, srcn, s); break;
+        case MODEL_RIEMANN3: deps_pass<MODEL_RIEMANN3, T>(g, sd, pass, cnt, start, fill, deps, srcbuf, srcn, s); break;
+        case MODEL_RSF2: deps_pass<MODEL_RSF2, T>(g, sd, pass, cnt, start, fill, deps, srcbuf, srcn, s); break;
+        case MODEL_ELASTICA2: deps_pass<MODEL_ELASTICA2, T>(g, sd, pass, cnt, start, fill, deps, srcbuf, srcn, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 template hipError_t launch_tile_deps<float>(int, const Geom &, const StencilData<float> &, int, int *, const int *,
-                                            int *, int *, int *, hipStream_t);
+                                            int *, int *, int *, int *, hipStream_t);
 template hipError_t launch_tile_deps<double>(int, const Geom &, const StencilData<double> &, int, int *, const int *,
-                                             int *, int *, int *, hipStream_t);
+                                             int *, int *, int *, int *, hipStream_t);
 
 long long scan_work_ints(long long n) { return 2 * ((n + SCAN_CHUNK - 1) / SCAN_CHUNK) + 1; }
 
 hipError_t launch_exclusive_scan(const int *cnt, int *start, int n, hipStream_t s, int *work) {
     const int nchunks = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
-    if (!work || nchunks <= 8) {
+    if (!work || nchunks <= 2) {
         hipLaunchKernelGGL(k_exclusive_scan, dim3(1), dim3(1024), 0, s, cnt, start, n);
         return hipGetLastError();
     }

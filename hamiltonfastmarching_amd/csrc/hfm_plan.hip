@@ -90,6 +90,9 @@ void plan_free(hfm_plan *p) {
     dfree(p->lists[0]);
 
     dfree(p->counts);
+    dfree(p->dep_src);
+    dfree(p->dep_nsrc);
+    dfree(p->scan_work);
     dfree(p->keys);
     dfree(p->tmax);
     dfree(p->nextmin);
@@ -134,8 +137,9 @@ void plan_free(hfm_plan *p) {
 template <typename T> void build_deps(hfm_plan *p, hipStream_t s) {
     const int nt = p->g.ntiles;
     HIPCHECK(hipMemsetAsync(p->dep_cnt, 0, sizeof(int) * nt, s));
-    HIPCHECK(launch_tile_deps<T>(p->model, p->g, p->sd<T>(), 0, p->dep_cnt, nullptr, nullptr, nullptr, nullptr, s));
-    HIPCHECK(launch_exclusive_scan(p->dep_cnt, p->dep_start, nt, s));
+    HIPCHECK(launch_tile_deps<T>(p->model, p->g, p->sd<T>(), 0, p->dep_cnt, nullptr, nullptr, nullptr, p->dep_src,
+                                 p->dep_nsrc, s));
+    HIPCHECK(launch_exclusive_scan(p->dep_cnt, p->dep_start, nt, s, p->scan_work));
     int total = 0;
     HIPCHECK(hipMemcpyAsync(p->h_count, p->dep_start + nt, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
@@ -149,7 +153,7 @@ template <typename T> void build_deps(hfm_plan *p, hipStream_t s) {
     p->ndeps = total;
     HIPCHECK(hipMemsetAsync(p->dep_fill, 0, sizeof(int) * nt, s));
     HIPCHECK(launch_tile_deps<T>(p->model, p->g, p->sd<T>(), 1, p->dep_cnt, p->dep_start, p->dep_fill, p->deps,
-                                 nullptr, s));
+                                 p->dep_src, p->dep_nsrc, s));
 }
 
 // Voronoi1Vec (Headers/Specializations/CommonTraits.h:118-161) on the host: the shared
@@ -551,6 +555,9 @@ int hfm_plan_create(hfm_plan **out, const char *model, const int64_t *dims, cons
         p->lists[0] = dalloc<int>(nt);
 
         p->counts = dalloc<int>(4);
+        p->dep_src = dalloc<int>((size_t)nt * 64);
+        p->dep_nsrc = dalloc<int>(nt);
+        p->scan_work = dalloc<int>(scan_work_ints(nt));
         p->keys = dalloc<unsigned>(nt);
         p->tmax = dalloc<unsigned>(nt);
         p->nextmin = dalloc<unsigned>(2 * PEND_SLOTS);
