@@ -31,6 +31,9 @@ Printed JSON fields beyond the driver's contract:
                 oracle's C++ restatement when the reference binary is absent.
   accuracy      max relative error of the FP32 values vs the reference FP64 values (floored at
                 the smallest nonzero reference value), from the CPU-baseline run.
+  configs       the other workloads on the GPU (ms per solve from HBM-resident inputs; geodesics
+                and adjoint timed separately) beside the reference executable on the same
+                inputs (StencilCPUTime + FMCPUTime, 1 core) and its accuracy.
   residual      final FP64 residual check of the last timed solve (hfm_plan_residual): the
                 largest relative deviation between each reached non-seed value and the FP64
                 root of its local scheme, and the finite / infinite mismatches.
@@ -67,6 +70,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--serving-requests", type=int, default=300)
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other workloads (Riemann2 256^2 FP64, ReedsSheppForward2 256x256x64 with 32 "
+                         "geodesics, Elastica2 192x192x64 with its adjoint)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 1 GiB device-to-device copy (reference kernel for the "
                          "FETCH_SIZE / WRITE_SIZE counter scale)")
@@ -185,6 +191,93 @@ def cpu_serving(reqs, budget_s=15.0, max_workers=16):
                       f"solves, {wall:.1f} s wall"}
 
 
+def _time_gpu(fn, steps, warmup=1):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def configs_leg(with_cpu):
+    """The other workloads, each timed on the GPU from HBM-resident inputs and, with with_cpu,
+    beside the reference executable on the same inputs (StencilCPUTime + FMCPUTime, 1 core):
+      riemann2_256_fp64    Riemann2 256^2, h = 1/256, FP64 update, centre seed
+      rsf2_256x256x64      ReedsSheppForward2, xi = 0.1, FP32, 4 seeds; then 32 geodesics
+      elastica2_192x192x64 Elastica2, xi = 0.2, FP32, 1 seed; then the adjoint of the sum of the
+                           arrival times at 32 targets (costSensitivity)"""
+    import torch
+    import oracle
+    import hamiltonfastmarching_amd as hfm
+    from hamiltonfastmarching_amd import synthetic as syn
+
+    def cpu(params, keys=("StencilCPUTime", "FMCPUTime")):
+        if not with_cpu or not oracle.ref_available(params["model"]):
+            return None, None
+        t = time.time()
+        out = oracle.run_reference(params)
+        wall = time.time() - t
+        if out.get("retcode", 1) != 0:
+            return None, None
+        return {"ms": round(sum(float(out[k]) for k in keys) * 1e3, 1), "keys": "+".join(keys),
+                "wall_ms": round(wall * 1e3, 1), "cores": 1}, out
+
+    res = {}
+    # Riemann2 256^2 FP64
+    p = syn.riemann2(256, seed=0)
+    plan = hfm.Plan("Riemann2", (256, 256), p["gridScale"], precision=64)
+    m = torch.from_numpy(p["metric"]).cuda()
+    out = torch.empty((256, 256), dtype=torch.float64, device="cuda")
+    ms = _time_gpu(lambda: (plan.set_metric(m), plan.solve(p["seeds"]), plan.values(out)), 10)
+    c, ref = cpu(p)
+    r = {"gpu_ms": round(ms, 3), "cpu": c}
+    if ref is not None:
+        r["max_rel_err_vs_reference"] = floor_rel(out.cpu().numpy(), np.asarray(ref["values"]).reshape(256, 256))
+        r["speedup"] = round(c["ms"] / ms, 1)
+    res["riemann2_256_fp64"] = r
+    del plan
+    # ReedsSheppForward2 256 x 256 x 64, FP32, geodesics from 32 tips
+    full = syn.rsf2(256, 64, seed=0)
+    tips = full.pop("tips")
+    dims = (256, 256, 64)
+    plan = hfm.Plan("ReedsSheppForward2", dims, full["gridScale"], origin=full["origin"], precision=32,
+                    xi=float(full["xi"]), eps=float(full["eps"]))
+    cost = torch.from_numpy(full["cost"]).cuda()
+    out = torch.empty(dims, dtype=torch.float64, device="cuda")
+    ms = _time_gpu(lambda: (plan.set_cost(cost), plan.solve(full["seeds"]), plan.values(out)), 3)
+    gms = _time_gpu(lambda: plan.geodesics(tips), 2)
+    c, ref = cpu(dict(full, tips=tips), keys=("StencilCPUTime", "FMCPUTime"))
+    r = {"gpu_ms": round(ms, 3), "geodesics_gpu_ms": round(gms, 3), "cpu": c}
+    if ref is not None:
+        r["geodesics_cpu_ms"] = round(float(ref["GeodesicCPUTime"]) * 1e3, 1)
+        r["max_rel_err_vs_reference"] = floor_rel(out.cpu().numpy(), np.asarray(ref["values"]).reshape(dims))
+        r["speedup"] = round(c["ms"] / ms, 1)
+    res["rsf2_256x256x64"] = r
+    del plan
+    # Elastica2 192 x 192 x 64, FP32, adjoint for 32 targets
+    full = syn.elastica2(192, 64, seed=0)
+    targets = full["inspectSensitivity"]
+    dims = (192, 192, 64)
+    plan = hfm.Plan("Elastica2", dims, full["gridScale"], origin=full["origin"], precision=32,
+                    xi=float(full["xi"]), eps=float(full["eps"]))
+    cost = torch.from_numpy(full["cost"]).cuda()
+    ms = _time_gpu(lambda: (plan.set_cost(cost), plan.solve(full["seeds"])), 3)
+    ams = _time_gpu(lambda: plan.adjoint(targets), 3)
+    c, ref = cpu(full)
+    r = {"gpu_ms": round(ms, 3), "adjoint_gpu_ms": round(ams, 3), "cpu": c}
+    if ref is not None:
+        r["speedup_solve"] = round(c["ms"] / ms, 1)
+        r["cpu_wall_ms_with_backward"] = c["wall_ms"]
+    res["elastica2_192x192x64"] = r
+    del plan
+    torch.cuda.synchronize()
+    return res
+
+
 def floor_rel(u, ref):
     fin = np.isfinite(ref)
     pos = ref[fin & (ref > 0)]
@@ -276,6 +369,8 @@ def main():
     }
     if not args.no_serving:
         result["serving"] = serving_leg(args.serving_requests, with_cpu=not args.no_cpu)
+    if not args.no_configs:
+        result["configs"] = configs_leg(with_cpu=not args.no_cpu)
     if args.copy_calibration:
         src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
         dst = torch.empty_like(src)

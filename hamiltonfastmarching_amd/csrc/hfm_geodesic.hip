@@ -153,6 +153,12 @@ __device__ int discrete_flow(const Geom &g, const StencilData<T> &sd, const T *u
     constexpr int NF = S::NFWD, NC = NF + S::NSYM;
     const int tile = tile_of(g, c), loc = local_of(g, c);
     const long long p = (long long)tile * 64 + loc;
+    // Everything that depends on the point alone is loaded at once (its value, fixed bit and
+    // visible sides with the stencil), so that a step costs two dependent memory latencies: the
+    // stencil, then the neighbour values.
+    const T uu = u[p];
+    const bool fixed = (fixedmask[tile] >> loc) & 1ull;
+    const uint32_t vm = sd.vis ? sd.vis[p] : 0xffffffffu;
     if (lane < NC) {
         uint32_t off;
         T w;
@@ -179,7 +185,6 @@ __device__ int discrete_flow(const Geom &g, const StencilData<T> &sd, const T *u
         };
         const bool nz = L.s_w[k] != 0.;
         // walls: sides that are not visible never enter the active set
-        const uint32_t vm = sd.vis ? sd.vis[p] : 0xffffffffu;
         const bool pa = (vm >> side_bit<NF>(k, 0) & 1u) && (nz || (S::SHARED && shares(o0, o1, o2)));
         const bool pb = k >= NF && (vm >> side_bit<NF>(k, 1) & 1u) && (nz || (S::SHARED && shares(-o0, -o1, -o2)));
         T va = inf, vb = inf;
@@ -190,8 +195,6 @@ __device__ int discrete_flow(const Geom &g, const StencilData<T> &sd, const T *u
         L.s_side[k] = va <= vb ? 1 : -1;
     }
     __syncthreads();
-    const T uu = u[p];
-    const bool fixed = (fixedmask[tile] >> loc) & 1ull;
     if (fixed || !(uu < dev_inf<T>())) return 0;
     // active set: the local solver's accepted differences among those below u (as the flow export)
     T vv[NC], ww[NC];
@@ -225,6 +228,7 @@ __device__ int discrete_flow(const Geom &g, const StencilData<T> &sd, const T *u
             unpack_offset(L.s_off[k], o[0], o[1], o[2]);
             for (int q = 0; q < 3; ++q) L.f_off[n][q] = L.s_side[k] * o[q];
             L.f_w[n] = L.s_w[k] * fmax(0., value - L.s_v[k]);
+            L.f_val[n] = L.s_v[k];  // the value at the flow's target point, as loaded above
         }
         ++n;
     }
@@ -350,14 +354,8 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
             double wSum = 0;
             for (int k = 0; k < nf; ++k) wSum += L.f_w[k];
             if (wSum == 0) continue;
-            // values at the flow's target points, one lane each
-            if (lane < nf) {
-                int ind[3], cc[3];
-                for (int q = 0; q < 3; ++q) ind[q] = index[q] + L.f_off[lane][q];
-                // flow targets are visible neighbours, hence in the domain
-                L.f_val[lane] = periodize<ModelT>(g, ind, cc) ? value_at(g, u, cc) : __builtin_huge_val();
-            }
-            __syncthreads();
+            // the values at the flow's target points (L.f_val) came with the flow: a used
+            // difference's value is its target's, loaded by discrete_flow
             for (int k = 0; k < nf; ++k) {
                 const double w = weight * L.f_w[k] / wSum;
                 int ind[3] = {0, 0, 0};
