@@ -66,6 +66,7 @@ struct hfm_plan {
     long long deps_cap = 0, ndeps = 0;
     int *lists[1] = {nullptr};     // tiles listed by the current round
     int *counts = nullptr;         // 2 alternating list counters
+    long long prev_rounds = 0;     // rounds the last solve on this plan ran
     int *dep_src = nullptr;        // tile-graph build: each tile's sources (<= 64 kept, 64 per tile)
     int *dep_nsrc = nullptr;       // tile-graph build: number of sources per tile
     int *scan_work = nullptr;      // multi-block exclusive scan of the tile-graph counts
