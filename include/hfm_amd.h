@@ -228,7 +228,7 @@ int hfm_plan_set_delta(hfm_plan *plan, double factor);
 int hfm_plan_set_round_target(hfm_plan *plan, int tiles);
 
 /* Blocks (of 4 wavefronts, one tile per wavefront at a time) of each update-kernel launch
- * (0: 640).  A tuning knob; results do not depend on it. */
+ * (0: 640; 384 for ReedsSheppForward2).  A tuning knob; results do not depend on it. */
 int hfm_plan_set_grid(hfm_plan *plan, int blocks);
 
 /* Jacobi sweeps a tile visit may run before an unconverged tile is handed back to the
