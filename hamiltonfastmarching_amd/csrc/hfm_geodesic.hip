@@ -11,16 +11,20 @@
 // geodesicVolumeBound (bound x 1.3 from the 5th restart, at most 50 restarts).
 //
 // Layout.  The live set (at most GEO_CAP points: the reference's std::map keyed by
-// (value, index)) lives in the wave's LDS, GEO_CAP / 64 slots per lane; taking the largest key
-// and finding a key are wave reductions / ballots.  The flow at a point is evaluated with one
-// lane per stencil difference.  All bookkeeping (weight sum, index sum, covariance, steps) is
-// FP64 in the reference's order of operations -- a vector divided by a scalar is a
-// multiplication by the reciprocal (JMM_CPPLibs LinearAlgebra/VectorType.h:37-42) -- and runs
-// uniformly on every lane.  On an FP64 solve the emitted points are those the reference's
-// solver computes from the same values and active sets.
+// (value, index)) lives in the wave's registers, GEO_CAP / 64 slots per lane, with wave-uniform
+// occupancy masks; taking the largest key is a per-lane pass plus two DPP maxima, finding a key
+// a compare and a ballot.  The flow at a point comes from a cache in LDS that holds the flows of
+// every point of the last tiles the mass visited, each tile computed one lane per point when it
+// is first needed: a step then waits on no memory access unless the mass enters a new tile.
+// All bookkeeping (weight sum, index sum, covariance, steps) is FP64 in the reference's order
+// of operations -- a vector divided by a scalar is a multiplication by the reciprocal
+// (JMM_CPPLibs LinearAlgebra/VectorType.h:37-42) -- and runs uniformly on every lane.  On an
+// FP64 solve the emitted points are those the reference's solver computes from the same values
+// and active sets.
 #include "hfm_kernels.h"
 
 #include "hfm_device.h"
+#include "hfm_tile.h"
 
 namespace hfm {
 
@@ -37,40 +41,78 @@ struct Key {
 
 // (value, index) order of std::map<std::pair<double, IndexType>, double> (std::array compares
 // lexicographically).
+// Branch-free: per-lane comparisons become mask operations rather than divergent branches.
 __device__ inline bool key_less(const Key &a, const Key &b) {
-    if (a.val != b.val) return a.val < b.val;
-    if (a.i0 != b.i0) return a.i0 < b.i0;
-    if (a.i1 != b.i1) return a.i1 < b.i1;
-    return a.i2 < b.i2;
-}
-
-__device__ inline Key shfl_key(const Key &k, int lane) {
-    Key r;
-    r.val = __shfl(k.val, lane);
-    r.i0 = __shfl(k.i0, lane);
-    r.i1 = __shfl(k.i1, lane);
-    r.i2 = __shfl(k.i2, lane);
-    r.slot = __shfl(k.slot, lane);
-    return r;
+    return (a.val < b.val) |
+           ((a.val == b.val) & ((a.i0 < b.i0) | ((a.i0 == b.i0) & ((a.i1 < b.i1) | ((a.i1 == b.i1) & (a.i2 < b.i2))))));
 }
 
 struct GeoLDS {
-    double val[GEO_CAP];
-    double w[GEO_CAP];
-    int idx[GEO_CAP][3];
-    int live[GEO_CAP];
+    // the steps since the last emitted point (value, index step); every lane writes the same
+    // entry, so every lane reads back what it wrote itself
     double st_val[GEO_STEPS];
     double st_vec[GEO_STEPS][3];
-    // flow at the current point, one entry per stencil difference
+    // flow at one point evaluated one lane per stencil difference (points whose entries do not
+    // fit in their tile's cache slot)
     uint32_t s_off[MAX_DIFF];
     double s_w[MAX_DIFF];
     double s_v[MAX_DIFF];
     int s_side[MAX_DIFF];
-    // emitted flow entries and the values of their target points
     int f_off[MAX_DIFF][3];
     double f_w[MAX_DIFF];
     double f_val[MAX_DIFF];
 };
+
+// Flow cache: the discrete flow of every point of the last few tiles the mass visited, computed
+// one lane per point when a tile is first needed.  Entries of a tile are packed by point
+// (prefix sum of the per-point counts); a point whose entries would pass the slot's capacity
+// is marked GEO_UNCACHED and evaluated on its own when it is taken out.
+constexpr int GEO_UNCACHED = 0xff;
+template <int ModelT, typename T> struct FlowCacheShape {
+    static constexpr int NC = ModelShape<ModelT>::NFWD + ModelShape<ModelT>::NSYM;
+    static constexpr int ECAP = 64 * (NC < 6 ? NC : 6);  // entries per tile
+    static constexpr int SLOT_BYTES = ECAP * (4 + 8 + (int)sizeof(T)) + 64 * (8 + 4);
+    static constexpr int BUDGET = 64 * 1024;  // with GeoLDS, two workgroups fit in a CU's 160 KB
+    static constexpr int NSLOT = BUDGET / SLOT_BYTES < 12 ? BUDGET / SLOT_BYTES : 12;
+    static_assert(NSLOT >= 4, "flow cache too small");
+};
+template <int ModelT, typename T> struct FlowCache {
+    using CS = FlowCacheShape<ModelT, T>;
+    double wsum[CS::NSLOT][64];         // sum of the point's flow weights, in entry order
+    double e_w[CS::NSLOT][CS::ECAP];    // w_i max(0, r - u_i)
+    T e_v[CS::NSLOT][CS::ECAP];         // value at the flow's target point
+    uint32_t e_off[CS::NSLOT][CS::ECAP];  // signed offset to the target (pack_offset)
+    uint32_t info[CS::NSLOT][64];         // first entry << 8 | number of entries (or GEO_UNCACHED)
+};
+
+// Wave-uniform copies of lane l's value (l uniform).
+__device__ inline int rl_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+// A condition or value every lane holds, marked as such: branches on it are scalar branches and
+// the values merged after them stay in scalar registers.
+__device__ inline bool uni(bool c) { return __builtin_amdgcn_readfirstlane((int)c) != 0; }
+__device__ inline double uni_d(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ inline unsigned long long uni64(unsigned long long x) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)x);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(x >> 32));
+    return (unsigned long long)hi << 32 | lo;
+}
+__device__ inline double rl_d(double v, int l) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// Unsigned 64-bit key in the order of the double values (-0 and +0 equal); 0 is below every value.
+__device__ inline unsigned long long order_key(double v) {
+    v = v + 0.;
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
 
 // FP64 running sums of GeodesicDiscrete's UpdateSums (GeodesicDiscreteSolver.hxx:80-85).
 struct Sums {
@@ -93,7 +135,6 @@ struct Sums {
         for (int i = 0; i < D; ++i) m[i] = indexSum[i] * b;
     }
 };
-
 template <int D> __device__ inline double det_sym(const double *c) {
     auto at = [&](int i, int j) { return i >= j ? c[i * (i + 1) / 2 + j] : c[j * (j + 1) / 2 + i]; };
     if (D == 2) return at(0, 0) * at(1, 1) - at(1, 0) * at(0, 1);
@@ -236,6 +277,129 @@ __device__ int discrete_flow(const Geom &g, const StencilData<T> &sd, const T *u
     return n;
 }
 
+// The discrete flow of the 64 points of a tile, one lane per point, into cache slot `slot`: the
+// same active set and weights as discrete_flow, in the same operations.  Every lane issues its
+// stencil loads, then all its neighbour-value loads, before it uses any of them.
+template <int ModelT, typename T>
+__device__ void fill_flow(const Geom &g, const StencilData<T> &sd, const T *u, const unsigned long long *fixedmask,
+                          int tile, int slot, FlowCache<ModelT, T> &C, int lane) {
+    using S = ModelShape<ModelT>;
+    constexpr int NF = S::NFWD, NC = NF + S::NSYM;
+    using CS = FlowCacheShape<ModelT, T>;
+    const T inf = dev_inf<T>();
+    int c[3];
+    tiled_coords(g, tile, lane, c);
+    const bool inside = c[0] < g.n[0] && c[1] < g.n[1] && c[2] < g.n[2];
+    const long long p = (long long)tile * 64 + lane;
+    const T uu = u[p];
+    const bool fixed = (fixedmask[tile] >> lane) & 1ull;
+    const uint32_t vm = sd.vis ? sd.vis[p] : 0xffffffffu;
+    uint32_t off[NC];
+    T w[NC];
+    const int th = inside ? c[2] : 0;  // theta row of a shared stencil (points past the grid: any row)
+#pragma unroll
+    for (int k = 0; k < NC; ++k) load_slot<ModelT, T>(sd, p, th, k, off[k], w[k]);
+    // zero-weight differences are active only through a nonzero-weight one with their offset
+    // (shared stencils); rare, so the table is read again rather than kept in registers
+    auto shares = [&](int k, int a0, int a1, int a2) {
+        for (int j = 0; j < NC; ++j) {
+            if (j == k) continue;
+            uint32_t oj;
+            T wj;
+            load_slot<ModelT, T>(sd, p, th, j, oj, wj);
+            if ((double)wj == 0.) continue;
+            int p0, p1, p2;
+            unpack_offset(oj, p0, p1, p2);
+            if (p0 == a0 && p1 == a1 && p2 == a2) return true;
+            if (j >= NF && p0 == -a0 && p1 == -a1 && p2 == -a2) return true;
+        }
+        return false;
+    };
+    bool pa[NC], pb[NC];
+    T va[NC], vb[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        int o0, o1, o2, r[3];
+        unpack_offset(off[k], o0, o1, o2);
+        const bool nz = (double)w[k] != 0.;
+        pa[k] = inside && (vm >> side_bit<NF>(k, 0) & 1u) && (nz || (S::SHARED && shares(k, o0, o1, o2)));
+        pb[k] = inside && k >= NF && (vm >> side_bit<NF>(k, 1) & 1u) &&
+                (nz || (S::SHARED && shares(k, -o0, -o1, -o2)));
+        pa[k] = pa[k] && shift_coords_m<ModelT>(g, c, o0, o1, o2, r);
+        va[k] = u[pa[k] ? (long long)tile_of(g, r) * 64 + local_of(g, r) : p];
+        if (k >= NF) {
+            pb[k] = pb[k] && shift_coords_m<ModelT>(g, c, -o0, -o1, -o2, r);
+            vb[k] = u[pb[k] ? (long long)tile_of(g, r) * 64 + local_of(g, r) : p];
+        } else {
+            vb[k] = inf;
+        }
+    }
+    double sv[NC];
+    int side[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        const T a = pa[k] ? va[k] : inf, b = pb[k] ? vb[k] : inf;
+        sv[k] = (double)(a <= b ? a : b);
+        side[k] = a <= b ? 1 : -1;
+    }
+    uint32_t used = 0;
+    if (inside && !fixed && uu < inf) {
+        T vv[NC], ww[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            const T v = (T)sv[k];
+            vv[k] = v < uu ? v : inf;
+            ww[k] = w[k];
+        }
+        (void)local_solve<NC, T>(vv, ww, used);
+    }
+    double value = 0;
+    if (used) {
+        double m = __builtin_huge_val();
+#pragma unroll
+        for (int k = 0; k < NC; ++k)
+            if (used >> k & 1u) m = sv[k] < m ? sv[k] : m;
+        double a = 0, b = 0, cc = -1;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            if (!(used >> k & 1u)) continue;
+            const double wk = (double)w[k], vr = sv[k] - m;
+            a += wk;
+            b += wk * vr;
+            cc += (wk * vr) * vr;
+        }
+        const double d2 = b * b - a * cc;
+        value = m + (b + sqrt(d2 > 0. ? d2 : 0.)) / a;
+    }
+    const int n = __builtin_popcount(used);
+    int incl = n;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(incl, d);
+        if (lane >= d) incl += t;
+    }
+    const int st = incl - n;
+    const bool fits = incl <= CS::ECAP;
+    double ws = 0;
+    int e = st;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        if (!(used >> k & 1u)) continue;
+        int o0, o1, o2;
+        unpack_offset(off[k], o0, o1, o2);
+        const double fw = (double)w[k] * fmax(0., value - sv[k]);
+        ws += fw;
+        if (fits) {
+            C.e_off[slot][e] = pack_offset(side[k] * o0, side[k] * o1, side[k] * o2);
+            C.e_w[slot][e] = fw;
+            C.e_v[slot][e] = (T)sv[k];
+        }
+        ++e;
+    }
+    C.wsum[slot][lane] = ws;
+    C.info[slot][lane] = fits ? (uint32_t)st << 8 | (uint32_t)n : (uint32_t)GEO_UNCACHED;
+}
+
 struct GeoParams {
     double step, threshold, volumeBound;
     double scale[3], origin[3];
@@ -252,9 +416,26 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
                                                  const double *__restrict__ tips, double *__restrict__ points,
                                                  long long *__restrict__ lengths, int *__restrict__ status) {
     constexpr int D = ModelShape<ModelT>::DIM;
-    constexpr int PER_LANE = GEO_CAP / 64;
+    constexpr int ROWS = GEO_CAP / 64;
+    using CS = FlowCacheShape<ModelT, T>;
     __shared__ GeoLDS L;
+    __shared__ FlowCache<ModelT, T> C;
     const int tip = blockIdx.x, lane = threadIdx.x;
+    // The live set: slot 64 j + l is row j of lane l's registers; bit l of live[j] (wave-uniform)
+    // says whether it holds a point.
+    double lv_val[ROWS], lv_w[ROWS];
+    int lv_i[ROWS][3];
+    unsigned long long live[ROWS];
+#pragma unroll
+    for (int j = 0; j < ROWS; ++j) {
+        lv_val[j] = 0.;
+        lv_w[j] = 0.;
+        lv_i[j][0] = lv_i[j][1] = lv_i[j][2] = 0;
+        live[j] = 0;
+    }
+    // Flow cache directory: lane s < NSLOT holds the tile in slot s (-1: empty) and its last use.
+    int my_tag = -1;
+    unsigned my_use = 0, clock = 0;
     long long npts = 0;
     double last[3] = {0., 0., 0.};
     int st = GEO_OK;
@@ -274,8 +455,8 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
     while (nRestarts < 50 && st == GEO_OK) {
         // ---- one pass of GeodesicDiscrete from `last` (GeodesicDiscreteSolver.hxx:56-231)
         bool restart = false;
-        for (int j = 0; j < PER_LANE; ++j) L.live[lane + 64 * j] = 0;
-        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < ROWS; ++j) live[j] = 0;
         int nlive = 0;
         Sums sums;
         double indexAvg[3] = {0., 0., 0.}, indexBase[3] = {0., 0., 0.};
@@ -301,104 +482,174 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
                     }
                 }
                 int c[3];
-                if (!periodize<ModelT>(g, index, c)) continue;
-                if (lane == 0) {
-                    L.val[nlive] = value_at(g, u, c);
-                    L.w[nlive] = weight;
-                    for (int q = 0; q < 3; ++q) L.idx[nlive][q] = q < D ? index[q] : 0;
-                    L.live[nlive] = 1;
+                if (!uni(periodize<ModelT>(g, index, c))) continue;
+                const double v0 = value_at(g, u, c);
+                if (lane == nlive) {  // slot nlive < 8: row 0
+                    lv_val[0] = v0;
+                    lv_w[0] = weight;
+                    for (int q = 0; q < 3; ++q) lv_i[0][q] = q < D ? index[q] : 0;
                 }
+                live[0] = uni64(live[0] | 1ull << nlive);
                 ++nlive;
                 sums.update<D>(weight, index);
             }
-            __syncthreads();
-            if (sums.weightSum == 0) break;
+            if (uni(sums.weightSum == 0)) break;
             sums.mean<D>(indexAvg);
             for (int q = 0; q < 3; ++q) indexBase[q] = indexAvg[q];
         }
         long long iter = 0;
-        while (nlive > 0 && sums.weightSum > 1e-5) {
+        while (uni(nlive > 0 && sums.weightSum > 1e-5)) {
             if (++iter > gp.max_iter) {
                 st = GEO_NO_END;
                 break;
             }
-            // take the live point of largest (value, index)
-            Key best{0., 0, 0, 0, -1};
-            for (int j = 0; j < PER_LANE; ++j) {
-                const int sl = lane + 64 * j;
-                if (!L.live[sl]) continue;
-                const Key k{L.val[sl], L.idx[sl][0], L.idx[sl][1], L.idx[sl][2], sl};
-                if (best.slot < 0 || key_less(best, k)) best = k;
+            // Take the live point of largest (value, index): each lane's largest row, then the
+            // largest value over the wave (two 32-bit DPP maxima of its order-preserving 64-bit
+            // key); equal values in several lanes are ordered by index.
+            int bj = -1, b0 = 0, b1 = 0, b2 = 0;
+            double bv = 0.;
+#pragma unroll
+            for (int j = 0; j < ROWS; ++j) {
+                if (!live[j]) continue;  // empty row (uniform)
+                const Key k{lv_val[j], lv_i[j][0], lv_i[j][1], lv_i[j][2], j};
+                const bool take = (live[j] >> lane & 1ull) & ((bj < 0) | key_less(Key{bv, b0, b1, b2, bj}, k));
+                bj = take ? j : bj;
+                bv = take ? k.val : bv;
+                b0 = take ? k.i0 : b0;
+                b1 = take ? k.i1 : b1;
+                b2 = take ? k.i2 : b2;
             }
-            for (int o = 32; o > 0; o >>= 1) {
-                const Key other = shfl_key(best, lane ^ o);
-                if (other.slot >= 0 && (best.slot < 0 || key_less(best, other))) best = other;
-            }
-            const int slot = best.slot;
-            if (slot < 0) {  // the live count and the live flags disagree: never expected
+            const unsigned long long km = bj >= 0 ? order_key(bv) : 0ull;
+            const unsigned khi = (unsigned)(km >> 32), klo = (unsigned)km;
+            const unsigned mhi = wave_max_u32(khi);
+            const unsigned mlo = wave_max_u32(khi == mhi ? klo : 0u);
+            const unsigned long long tie = __ballot(bj >= 0 && khi == mhi && klo == mlo);
+            if (!tie) {  // the live count and the live bits disagree: never expected
                 st = GEO_NO_END;
                 break;
             }
-            const double value = best.val;
-            const int index[3] = {best.i0, best.i1, best.i2};
-            const double weight = L.w[slot];
-            __syncthreads();
-            if (lane == 0) L.live[slot] = 0;
+            int wl = __ffsll((long long)tie) - 1;
+            if (tie & (tie - 1)) {
+                int w0 = rl_i(b0, wl), w1 = rl_i(b1, wl), w2 = rl_i(b2, wl);
+                for (unsigned long long r = tie & (tie - 1); r; r &= r - 1) {
+                    const int l = __ffsll((long long)r) - 1;
+                    const int c0 = rl_i(b0, l), c1 = rl_i(b1, l), c2 = rl_i(b2, l);
+                    if (c0 != w0 ? w0 < c0 : c1 != w1 ? w1 < c1 : w2 < c2) {
+                        wl = l;
+                        w0 = c0;
+                        w1 = c1;
+                        w2 = c2;
+                    }
+                }
+            }
+            const int wj = rl_i(bj, wl);
+            const double value = rl_d(bv, wl);
+            const int index[3] = {rl_i(b0, wl), rl_i(b1, wl), rl_i(b2, wl)};
+            double myw = 0.;
+#pragma unroll
+            for (int j = 0; j < ROWS; ++j) myw = j == bj ? lv_w[j] : myw;
+            const double weight = rl_d(myw, wl);
+#pragma unroll
+            for (int j = 0; j < ROWS; ++j)
+                if (j == wj) live[j] = uni64(live[j] & ~(1ull << wl));
             --nlive;
-            __syncthreads();
-            if (weight == 0) continue;
+            if (uni(weight == 0)) continue;
             sums.update<D>(-weight, index);
             int c[3];
-            if (!periodize<ModelT>(g, index, c)) continue;  // live points are in the domain
-            const int nf = discrete_flow<ModelT, T>(g, sd, u, fixedmask, c, L, lane);
-            double wSum = 0;
-            for (int k = 0; k < nf; ++k) wSum += L.f_w[k];
-            if (wSum == 0) continue;
-            // the values at the flow's target points (L.f_val) came with the flow: a used
-            // difference's value is its target's, loaded by discrete_flow
-            for (int k = 0; k < nf; ++k) {
-                const double w = weight * L.f_w[k] / wSum;
-                int ind[3] = {0, 0, 0};
-                for (int q = 0; q < D; ++q) ind[q] = index[q] + L.f_off[k][q];
-                const double val = L.f_val[k];
-                int found = -1, freeslot = -1;
-                {
-                    int myf = -1, myfree = -1;
-                    for (int j = 0; j < PER_LANE; ++j) {
-                        const int sl = lane + 64 * j;
-                        if (L.live[sl]) {
-                            if (L.val[sl] == val && L.idx[sl][0] == ind[0] && L.idx[sl][1] == ind[1] &&
-                                L.idx[sl][2] == ind[2])
-                                myf = sl;
-                        } else if (myfree < 0) {
-                            myfree = sl;
-                        }
-                    }
-                    const unsigned long long bf = __ballot(myf >= 0), bfree = __ballot(myfree >= 0);
-                    if (bf) found = __shfl(myf, __ffsll((long long)bf) - 1);
-                    if (bfree) freeslot = __shfl(myfree, __ffsll((long long)bfree) - 1);
+            if (!uni(periodize<ModelT>(g, index, c))) continue;  // live points are in the domain
+            // The point's flow: from its tile's cache slot, the least recently used slot being
+            // filled first when the tile is not cached.
+            const int tile = tile_of(g, c), loc = local_of(g, c);
+            const unsigned long long hit = __ballot(lane < CS::NSLOT && my_tag == tile);
+            int slot;
+            if (hit) {
+                slot = __ffsll((long long)hit) - 1;
+            } else {
+                const unsigned oldest = wave_min_u32(lane < CS::NSLOT ? my_use : 0xffffffffu);
+                slot = __ffsll((long long)__ballot(lane < CS::NSLOT && my_use == oldest)) - 1;
+                fill_flow<ModelT, T>(g, sd, u, fixedmask, tile, slot, C, lane);
+                if (lane == slot) my_tag = tile;
+                __syncthreads();
+            }
+            ++clock;
+            if (lane == slot) my_use = clock;
+            const unsigned info = (unsigned)__builtin_amdgcn_readfirstlane((int)C.info[slot][loc]);
+            double wSum = uni_d(C.wsum[slot][loc]);
+            // the point's flow entries, entry k in lane k
+            int nf;
+            double my_w = 0., my_v = 0.;
+            uint32_t my_off = 0;
+            if (info != (unsigned)GEO_UNCACHED) {
+                nf = (int)(info & 0xffu);
+                const int e = (int)(info >> 8) + lane;
+                if (lane < nf) {
+                    my_w = C.e_w[slot][e];
+                    my_v = (double)C.e_v[slot][e];
+                    my_off = C.e_off[slot][e];
                 }
-                if (found < 0) {
-                    if (w < sums.weightSum * gp.threshold) continue;
-                    if (freeslot < 0) {
+            } else {
+                nf = discrete_flow<ModelT, T>(g, sd, u, fixedmask, c, L, lane);
+                wSum = 0;
+                for (int k = 0; k < nf; ++k) wSum += L.f_w[k];
+                wSum = uni_d(wSum);
+                if (lane < nf) {
+                    my_w = L.f_w[lane];
+                    my_v = L.f_val[lane];
+                    my_off = pack_offset(L.f_off[lane][0], L.f_off[lane][1], L.f_off[lane][2]);
+                }
+            }
+            if (uni(wSum == 0)) continue;
+            const double my_share = weight * my_w / wSum;  // entry k's mass, in lane k
+            // spread the mass over the flow's targets; a flow entry carries its target's value
+            for (int k = 0; k < nf; ++k) {
+                const double w = rl_d(my_share, k), val = rl_d(my_v, k);
+                int o[3];
+                unpack_offset((uint32_t)rl_i((int)my_off, k), o[0], o[1], o[2]);
+                int ind[3] = {0, 0, 0};
+                for (int q = 0; q < D; ++q) ind[q] = index[q] + o[q];
+                int fj = -1;
+#pragma unroll
+                for (int j = 0; j < ROWS; ++j) {
+                    if (!live[j]) continue;
+                    const bool eq = (live[j] >> lane & 1ull) & (lv_val[j] == val) & (lv_i[j][0] == ind[0]) &
+                                    (lv_i[j][1] == ind[1]) & (lv_i[j][2] == ind[2]);
+                    fj = eq ? j : fj;
+                }
+                const unsigned long long bf = __ballot(fj >= 0);
+                if (bf) {
+                    const int fl = __ffsll((long long)bf) - 1, fr = rl_i(fj, fl);
+#pragma unroll
+                    for (int j = 0; j < ROWS; ++j)
+                        if (j == fr && lane == fl) lv_w[j] += w;
+                } else {
+                    if (uni(w < sums.weightSum * gp.threshold)) continue;
+                    int fr = -1, fl = 0;
+#pragma unroll
+                    for (int j = ROWS - 1; j >= 0; --j)
+                        if (~live[j]) {
+                            fr = j;
+                            fl = __ffsll((long long)~live[j]) - 1;
+                        }
+                    if (fr < 0) {
                         st = GEO_SET_OVERFLOW;
                         break;
                     }
-                    if (lane == 0) {
-                        L.val[freeslot] = val;
-                        L.w[freeslot] = w;
-                        for (int q = 0; q < 3; ++q) L.idx[freeslot][q] = ind[q];
-                        L.live[freeslot] = 1;
+#pragma unroll
+                    for (int j = 0; j < ROWS; ++j) {
+                        if (j != fr) continue;
+                        if (lane == fl) {
+                            lv_val[j] = val;
+                            lv_w[j] = w;
+                            for (int q = 0; q < 3; ++q) lv_i[j][q] = ind[q];
+                        }
+                        live[j] = uni64(live[j] | 1ull << fl);
                     }
                     ++nlive;
-                } else if (lane == 0) {
-                    L.w[found] += w;
                 }
-                __syncthreads();
                 sums.update<D>(w, ind);
             }
             if (st != GEO_OK) break;
-            if (sums.weightSum <= 0) {
+            if (uni(sums.weightSum <= 0)) {
                 double q[3];
                 for (int i = 0; i < D; ++i) q[i] = indexAvg[i] + 0.5;
                 emit(q);
@@ -412,12 +663,9 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
                 st = GEO_STEPS_OVERFLOW;
                 break;
             }
-            if (lane == 0) {
-                L.st_val[nsteps] = value;
-                for (int i = 0; i < 3; ++i) L.st_vec[nsteps][i] = step[i];
-            }
+            L.st_val[nsteps] = value;
+            for (int i = 0; i < 3; ++i) L.st_vec[nsteps][i] = step[i];
             ++nsteps;
-            __syncthreads();
             double sq = 0;
             for (int i = 0; i < D; ++i) sq += step[i] * step[i];
             stepNorm += sqrt(sq);
@@ -430,9 +678,9 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
                 nsteps = 0;
                 continue;
             }
-            if (stepNorm < gp.step && nlive > 1) continue;
+            if (uni(stepNorm < gp.step && nlive > 1)) continue;
             const double valInc = L.st_val[0] - value;
-            if (valInc <= 0) continue;
+            if (uni(valInc <= 0)) continue;
             const double delta = valInc / D;
             double p[3] = {indexBase[0], indexBase[1], indexBase[2]};
             for (int s = 0; s < nsteps; ++s) {
@@ -458,7 +706,7 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
                     var[i * (i + 1) / 2 + j] -= mm[i] * mm[j];
                     var[i * (i + 1) / 2 + j] += (i == j) ? 1. : 0.;
                 }
-            if (det_sym<D>(var) > vb * vb) {
+            if (uni(det_sym<D>(var) > vb * vb)) {
                 double q[3];
                 for (int i = 0; i < D; ++i) q[i] = indexAvg[i] + 0.5;
                 emit(q);
