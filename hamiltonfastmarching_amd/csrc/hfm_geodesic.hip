@@ -11,8 +11,8 @@
 // geodesicVolumeBound (bound x 1.3 from the 5th restart, at most 50 restarts).
 //
 // Layout.  The live set (at most GEO_CAP points: the reference's std::map keyed by
-// (value, index)) lives in the wave's registers, GEO_CAP / 64 slots per lane, with wave-uniform
-// occupancy masks; taking the largest key is a per-lane pass plus two DPP maxima, finding a key
+// (value, index)) lives in the wave's registers, GEO_CAP / 64 slots per lane, with a per-lane
+// occupancy mask; taking the largest key is a per-lane pass plus two DPP maxima, finding a key
 // a compare and a ballot.  The flow at a point comes from a cache in LDS that holds the flows of
 // every point of the last tiles the mass visited, each tile computed one lane per point when it
 // is first needed: a step then waits on no memory access unless the mass enters a new tile.
@@ -95,11 +95,6 @@ __device__ inline double uni_d(double v) {
     const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
     const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-__device__ inline unsigned long long uni64(unsigned long long x) {
-    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)x);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(x >> 32));
-    return (unsigned long long)hi << 32 | lo;
 }
 __device__ inline double rl_d(double v, int l) {
     const unsigned long long b = (unsigned long long)__double_as_longlong(v);
@@ -421,17 +416,16 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
     __shared__ GeoLDS L;
     __shared__ FlowCache<ModelT, T> C;
     const int tip = blockIdx.x, lane = threadIdx.x;
-    // The live set: slot 64 j + l is row j of lane l's registers; bit l of live[j] (wave-uniform)
-    // says whether it holds a point.
+    // The live set: slot 64 j + l is row j of lane l's registers; bit j of lane l's my_live says
+    // whether it holds a point.
     double lv_val[ROWS], lv_w[ROWS];
     int lv_i[ROWS][3];
-    unsigned long long live[ROWS];
+    unsigned my_live = 0;
 #pragma unroll
     for (int j = 0; j < ROWS; ++j) {
         lv_val[j] = 0.;
         lv_w[j] = 0.;
         lv_i[j][0] = lv_i[j][1] = lv_i[j][2] = 0;
-        live[j] = 0;
     }
     // Flow cache directory: lane s < NSLOT holds the tile in slot s (-1: empty) and its last use.
     int my_tag = -1;
@@ -455,8 +449,7 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
     while (nRestarts < 50 && st == GEO_OK) {
         // ---- one pass of GeodesicDiscrete from `last` (GeodesicDiscreteSolver.hxx:56-231)
         bool restart = false;
-#pragma unroll
-        for (int j = 0; j < ROWS; ++j) live[j] = 0;
+        my_live = 0;
         int nlive = 0;
         Sums sums;
         double indexAvg[3] = {0., 0., 0.}, indexBase[3] = {0., 0., 0.};
@@ -488,8 +481,8 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
                     lv_val[0] = v0;
                     lv_w[0] = weight;
                     for (int q = 0; q < 3; ++q) lv_i[0][q] = q < D ? index[q] : 0;
+                    my_live |= 1u;
                 }
-                live[0] = uni64(live[0] | 1ull << nlive);
                 ++nlive;
                 sums.update<D>(weight, index);
             }
@@ -510,9 +503,8 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
             double bv = 0.;
 #pragma unroll
             for (int j = 0; j < ROWS; ++j) {
-                if (!live[j]) continue;  // empty row (uniform)
                 const Key k{lv_val[j], lv_i[j][0], lv_i[j][1], lv_i[j][2], j};
-                const bool take = (live[j] >> lane & 1ull) & ((bj < 0) | key_less(Key{bv, b0, b1, b2, bj}, k));
+                const bool take = (my_live >> j & 1u) & ((bj < 0) | key_less(Key{bv, b0, b1, b2, bj}, k));
                 bj = take ? j : bj;
                 bv = take ? k.val : bv;
                 b0 = take ? k.i0 : b0;
@@ -549,9 +541,7 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
 #pragma unroll
             for (int j = 0; j < ROWS; ++j) myw = j == bj ? lv_w[j] : myw;
             const double weight = rl_d(myw, wl);
-#pragma unroll
-            for (int j = 0; j < ROWS; ++j)
-                if (j == wj) live[j] = uni64(live[j] & ~(1ull << wl));
+            if (lane == wl) my_live &= ~(1u << wj);
             --nlive;
             if (uni(weight == 0)) continue;
             sums.update<D>(-weight, index);
@@ -607,11 +597,11 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
                 unpack_offset((uint32_t)rl_i((int)my_off, k), o[0], o[1], o[2]);
                 int ind[3] = {0, 0, 0};
                 for (int q = 0; q < D; ++q) ind[q] = index[q] + o[q];
+                // branch-free over the rows: selects rather than per-row branches
                 int fj = -1;
 #pragma unroll
                 for (int j = 0; j < ROWS; ++j) {
-                    if (!live[j]) continue;
-                    const bool eq = (live[j] >> lane & 1ull) & (lv_val[j] == val) & (lv_i[j][0] == ind[0]) &
+                    const bool eq = (my_live >> j & 1u) & (lv_val[j] == val) & (lv_i[j][0] == ind[0]) &
                                     (lv_i[j][1] == ind[1]) & (lv_i[j][2] == ind[2]);
                     fj = eq ? j : fj;
                 }
@@ -619,31 +609,30 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
                 if (bf) {
                     const int fl = __ffsll((long long)bf) - 1, fr = rl_i(fj, fl);
 #pragma unroll
-                    for (int j = 0; j < ROWS; ++j)
-                        if (j == fr && lane == fl) lv_w[j] += w;
+                    for (int j = 0; j < ROWS; ++j) {
+                        const double add = lv_w[j] + w;
+                        lv_w[j] = (j == fr) & (lane == fl) ? add : lv_w[j];
+                    }
                 } else {
                     if (uni(w < sums.weightSum * gp.threshold)) continue;
-                    int fr = -1, fl = 0;
-#pragma unroll
-                    for (int j = ROWS - 1; j >= 0; --j)
-                        if (~live[j]) {
-                            fr = j;
-                            fl = __ffsll((long long)~live[j]) - 1;
-                        }
-                    if (fr < 0) {
+                    // a free slot: the first free row of the first lane that has one
+                    const int myfree = (my_live & ((1u << ROWS) - 1)) == (1u << ROWS) - 1 ? -1 : __builtin_ctz(~my_live);
+                    const unsigned long long bfree = __ballot(myfree >= 0);
+                    if (!bfree) {
                         st = GEO_SET_OVERFLOW;
                         break;
                     }
+                    const int fl = __ffsll((long long)bfree) - 1, fr = rl_i(myfree, fl);
 #pragma unroll
                     for (int j = 0; j < ROWS; ++j) {
-                        if (j != fr) continue;
-                        if (lane == fl) {
-                            lv_val[j] = val;
-                            lv_w[j] = w;
-                            for (int q = 0; q < 3; ++q) lv_i[j][q] = ind[q];
-                        }
-                        live[j] = uni64(live[j] | 1ull << fl);
+                        const bool mine = (j == fr) & (lane == fl);
+                        lv_val[j] = mine ? val : lv_val[j];
+                        lv_w[j] = mine ? w : lv_w[j];
+                        lv_i[j][0] = mine ? ind[0] : lv_i[j][0];
+                        lv_i[j][1] = mine ? ind[1] : lv_i[j][1];
+                        lv_i[j][2] = mine ? ind[2] : lv_i[j][2];
                     }
+                    my_live |= lane == fl ? 1u << fr : 0u;
                     ++nlive;
                 }
                 sums.update<D>(w, ind);
