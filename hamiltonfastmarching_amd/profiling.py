@@ -33,7 +33,7 @@ def kernel_trace(db: str, substring: str = ""):
     out.sort(reverse=True)
     lines = ["| kernel | calls | total us | avg us | min us | median us | max us |", "|---|---|---|---|---|---|---|"]
     for tot, name, n, avg, mn, med, mx in out:
-        short = name.split("(")[0]
+        short = name.replace("(anonymous namespace)::", "").split("(")[0]
         lines.append(f"| `{short}` | {n} | {tot:.1f} | {avg:.2f} | {mn:.2f} | {med:.2f} | {mx:.2f} |")
     return "\n".join(lines)
 
