@@ -380,6 +380,13 @@ class Plan:
         self._check(self._L.hfm_plan_set_graph(self._h, int(enable)))
         return self
 
+    def set_fused_schedule(self, enable=True):
+        """Rounds list their own tiles inside the update kernel (True), or a separate listing
+        kernel schedules each round (False); None restores the default (the first on 2-D
+        grids, the second on 3-D grids)."""
+        self._check(self._L.hfm_plan_set_fused_schedule(self._h, -1 if enable is None else int(bool(enable))))
+        return self
+
 
 class EikonalSolve:
     """``EikonalSolve.apply(cost, plan, seeds, targets, weights)`` -> sum_t weights[t] u(targets[t]).

@@ -611,7 +611,7 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
 #pragma unroll
                     for (int j = 0; j < ROWS; ++j) {
                         const double add = lv_w[j] + w;
-                        lv_w[j] = (j == fr) & (lane == fl) ? add : lv_w[j];
+                        lv_w[j] = ((j == fr) & (lane == fl)) ? add : lv_w[j];
                     }
                 } else {
                     if (uni(w < sums.weightSum * gp.threshold)) continue;

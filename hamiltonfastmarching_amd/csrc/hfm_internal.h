@@ -65,14 +65,14 @@ struct hfm_plan {
     int *dep_cnt = nullptr, *dep_start = nullptr, *dep_fill = nullptr, *deps = nullptr;
     long long deps_cap = 0, ndeps = 0;
     int *lists[1] = {nullptr};     // tiles listed by the current round
-    int *counts = nullptr;         // 2 alternating list counters
+    int *counts = nullptr;         // 4 rotating list counters
     long long prev_rounds = 0;     // rounds the last solve on this plan ran
     int *dep_src = nullptr;        // tile-graph build: each tile's sources (<= 64 kept, 64 per tile)
     int *dep_nsrc = nullptr;       // tile-graph build: number of sources per tile
     int *scan_work = nullptr;      // multi-block exclusive scan of the tile-graph counts
     unsigned *keys = nullptr;      // per-tile scheduling key (KEY_NONE: not pending)
     unsigned *tmax = nullptr;      // per-tile largest value (as key) after its last visit
-    unsigned *nextmin = nullptr;   // 2 alternating smallest pending keys
+    unsigned *nextmin = nullptr;   // 4 rotating smallest pending keys (PEND_SLOTS words each)
     float *bounds = nullptr;       // 2 alternating band bounds + 2 alternating band widths
     hfm::RedSlot *red = nullptr;   // spread launch-wide reductions (mean one-step time, widest stencil)
     hfm::RedSlot *h_red = nullptr;  // pinned host copy of the slots (asynchronous read-back)
@@ -125,6 +125,7 @@ struct hfm_plan {
     // launches and the read-back of the last round's count), replayed until convergence.  The
     // graph is captured on graph_stream and re-captured when its launch arguments change.
     bool use_graph = true;
+    int fused_schedule = -1;       // rounds list their own tiles (k_fim_sched) instead of k_schedule + k_fim: 1 on, 0 off, -1 on for 2-D grids
     hipStream_t graph_stream = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     std::vector<char> graph_key;

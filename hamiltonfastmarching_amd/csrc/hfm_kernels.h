@@ -55,6 +55,8 @@ template <typename T> struct FimArgs {
     unsigned long long *stats;  // PEND_SLOTS pairs: (tiles processed, sweeps summed)
     unsigned grid;
     int max_sweeps;  // sweeps per tile visit before the tile is handed back to the schedule: 8, 16 or 32
+    int fused;       // 1: k_fim_sched lists its own tiles with the round parameters in sr (no list)
+    SchedRound sr;
 };
 
 // Geodesic backtracking (hfm_geodesic.hip).  Tips are ADim'd (grid-index units); points are

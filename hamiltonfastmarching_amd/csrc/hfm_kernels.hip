@@ -404,6 +404,86 @@ __global__ __launch_bounds__(256) void k_fim(Geom g, StencilData<T> sd, T *__res
 }
 
 // ---------------------------------------------------------------------------------------------
+// Update kernel that schedules its own round: block b owns tiles b, b + grid, b + 2 grid, ...;
+// it lists those whose key lies below the round's band bound (consuming their keys with an
+// atomic exchange, so that a mark made by another block meanwhile is consumed too, never lost)
+// and its four waves visit them.  Every block computes the same bound from the same inputs, and
+// the separate scheduling kernel and its kernel boundary are gone from every round.  A tile
+// marked and then consumed within one round leaves its key in the next round's pending minimum,
+// so one empty round does not prove convergence: two consecutive empty rounds do (the first of
+// them consumed nothing and left the exact minimum of every remaining key).
+// Round k reads pend[k % 4], accumulates pend[(k + 1) % 4] and counts[k % 4], and clears
+// pend[(k + 2) % 4] and counts[(k + 1) % 4], which no block of round k reads or writes.
+// ---------------------------------------------------------------------------------------------
+template <int ModelT, typename T, int MAXSWEEP>
+__global__ __launch_bounds__(256) void k_fim_sched(Geom g, StencilData<T> sd, T *__restrict__ u,
+                                                   const unsigned long long *__restrict__ fixedmask,
+                                                   const int *__restrict__ dep_start, const int *__restrict__ deps,
+                                                   unsigned *__restrict__ keys, unsigned *__restrict__ tmax,
+                                                   SchedRound sr, int *__restrict__ count_reset,
+                                                   unsigned *__restrict__ pend_reset,
+                                                   unsigned long long *__restrict__ stats) {
+    __shared__ T su[4][SU_ROW];
+    __shared__ int blist[256];
+    __shared__ int wcount[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int G = gridDim.x;
+    const int owned = (g.ntiles - (int)blockIdx.x + G - 1) / G;
+    // the first key is loaded before the round parameters it is compared with
+    unsigned k0 = (int)threadIdx.x < owned ? keys[blockIdx.x + threadIdx.x * G] : KEY_NONE;
+    const float d0 = *sr.delta0;
+    float delta = d0;
+    if (delta > 0.f) delta = adapt_delta(*sr.delta_in, *sr.prev_count, sr.target, 0.5f * d0, 64.f * d0);
+    const float B = band_bound(key_value(read_pend(sr.pend_in)), *sr.bound_in, delta);
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0) {
+            *sr.bound_out = B;
+            *sr.delta_out = delta;
+            *count_reset = 0;
+        }
+        if (threadIdx.x < PEND_SLOTS) pend_reset[threadIdx.x] = KEY_NONE;
+    }
+    const TileCtx<T> ctx{g, sd, u, fixedmask, dep_start, deps, keys, tmax};
+    unsigned long long my_sweeps = 0, my_visits = 0;
+    unsigned wmin = KEY_NONE;
+    for (int c0 = 0; c0 < owned; c0 += 256) {
+        const int j = c0 + (int)threadIdx.x;
+        const int t = (int)blockIdx.x + j * G;
+        unsigned k = c0 == 0 ? k0 : (j < owned ? keys[t] : KEY_NONE);
+        const bool take = band_take(k, B);
+        if (take) (void)atomicExch(&keys[t], KEY_NONE);
+        const unsigned left = (!take && k != KEY_NONE) ? k : KEY_NONE;
+        const unsigned long long bm = __ballot(take);
+        const unsigned lmin = wave_min_u32(left);
+        wmin = lmin < wmin ? lmin : wmin;
+        if (lane == 0) wcount[wv] = __popcll(bm);
+        __syncthreads();
+        int base = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            base += w < wv ? wcount[w] : 0;
+            total += wcount[w];
+        }
+        const unsigned long long below = lane == 0 ? 0ull : (bm & (~0ull >> (64 - lane)));
+        if (take) blist[base + __popcll(below)] = t;
+        __syncthreads();
+        if (threadIdx.x == 0 && total) atomicAdd(sr.count, total);
+        for (int e = wv; e < total; e += 4) {
+            const int tile = __builtin_amdgcn_readfirstlane(blist[e]);
+            my_sweeps += fim_tile<ModelT, T, MAXSWEEP>(ctx, tile, lane, su[wv], wmin);
+            my_visits += 1;
+        }
+        __syncthreads();  // blist and wcount are reused by the next chunk
+    }
+    const int slot = ((blockIdx.x * 4 + wv) & (PEND_SLOTS - 1));
+    if (lane == 0 && wmin != KEY_NONE) atomicMin(&sr.pend_out[slot], wmin);
+    if (lane == 0 && my_visits) {
+        atomicAdd(&stats[2 * slot], my_visits);
+        atomicAdd(&stats[2 * slot + 1], my_sweeps);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Exports.
 // ---------------------------------------------------------------------------------------------
 template <typename T>
@@ -628,6 +708,16 @@ hipError_t launch_schedule(int ntiles, unsigned *keys, const SchedRound &sr, hip
 // that an event recorded before the launch would include).
 template <int M, typename T, int MS>
 static void fim_sweeps(const FimArgs<T> &a, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+    if (a.fused) {
+        if (ev0)
+            hipExtLaunchKernelGGL((k_fim_sched<M, T, MS>), dim3(a.grid), dim3(256), 0, s, ev0, ev1, 0, a.g, a.sd, a.u,
+                                  a.fixedmask, a.dep_start, a.deps, a.keys, a.tmax, a.sr, a.count_reset,
+                                  a.pend_reset, a.stats);
+        else
+            hipLaunchKernelGGL((k_fim_sched<M, T, MS>), dim3(a.grid), dim3(256), 0, s, a.g, a.sd, a.u, a.fixedmask,
+                               a.dep_start, a.deps, a.keys, a.tmax, a.sr, a.count_reset, a.pend_reset, a.stats);
+        return;
+    }
     if (ev0)
         hipExtLaunchKernelGGL((k_fim<M, T, MS>), dim3(a.grid), dim3(256), 0, s, ev0, ev1, 0, a.g, a.sd, a.u,
                               a.fixedmask, a.dep_start, a.deps, a.list, a.count, a.count_reset, a.keys, a.tmax,

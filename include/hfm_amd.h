@@ -246,6 +246,13 @@ int hfm_plan_set_timing(hfm_plan *plan, int enable);
  * No counterpart in the reference (its loop is the serial heap of HamiltonFastMarching.hxx). */
 int hfm_plan_set_graph(hfm_plan *plan, int enable);
 
+/* Let every round of the update loop list its own tiles (enable = 1: one kernel a round, each
+ * block taking the pending tiles it owns below the round's band bound, then visiting them), or
+ * schedule each round with a separate listing kernel (0); -1 (the default): the first on 2-D
+ * grids, the second on 3-D grids, whose large rounds balance better over a global list.  Both
+ * compute the same fixed point.  No counterpart in the reference. */
+int hfm_plan_set_fused_schedule(hfm_plan *plan, int enable);
+
 /* ----------------------------------------------------------------------------------------- */
 /* Continuous-batching solve server (no reference counterpart: the reference solves one      */
 /* problem per Run(IO&) call; this serves many Riemann2 / Riemann3 solves concurrently)      */

@@ -1,6 +1,7 @@
 """The block fast-iterative solver's schedule changes the work, never the fixed point: solves with
 unordered rounds (band width 0), narrow and wide value bands, tiny and huge round targets, a
-one-block and an odd-sized update grid and several sweep caps all agree with the oracle's FP64
+one-block and an odd-sized update grid, several sweep caps, and rounds that list their own tiles
+or are listed by a separate kernel all agree with the oracle's FP64
 Fast Marching (1e-12 relative for FP64 plans, 1e-5 for FP32) and with each other."""
 import numpy as np
 import pytest
@@ -11,14 +12,20 @@ from hamiltonfastmarching_amd import synthetic as syn
 
 pytestmark = pytest.mark.gpu
 
-# (band width in mean one-step times, tiles per round, update-kernel blocks, sweeps per visit)
+# (band width in mean one-step times, tiles per round, update-kernel blocks, sweeps per visit,
+#  rounds listing their own tiles)
 SETTINGS = [
-    (8.0, 0, 0, 8),       # defaults
-    (0.0, 0, 0, 8),       # unordered: every pending tile in every round
-    (1.0, 16, 0, 8),      # narrow band, small rounds
-    (64.0, 4096, 0, 32),  # wide band, large rounds, long visits
-    (8.0, 0, 1, 16),      # one block of four waves
-    (8.0, 0, 7, 8),       # an odd grid
+    (8.0, 0, 0, 8, None),       # defaults (rounds list their own tiles on 2-D grids)
+    (8.0, 0, 0, 8, True),       # every round lists its own tiles
+    (8.0, 0, 0, 8, False),      # a separate listing kernel per round
+    (0.0, 0, 0, 8, True),       # unordered: every pending tile in every round
+    (0.0, 0, 0, 8, False),
+    (1.0, 16, 0, 8, True),      # narrow band, small rounds
+    (1.0, 16, 0, 8, False),
+    (64.0, 4096, 0, 32, True),  # wide band, large rounds, long visits
+    (8.0, 0, 1, 16, True),      # one block of four waves (owning every tile)
+    (8.0, 0, 1, 16, False),
+    (8.0, 0, 7, 8, True),       # an odd grid
 ]
 CASES = [
     ("riemann2_96", lambda: syn.riemann2(96, seed=21)),
@@ -52,14 +59,15 @@ def test_fixed_point_independent_of_schedule(name, make, precision):
                         eps=float(p["eps"]))
         plan.set_cost(torch.from_numpy(p["cost"]).cuda())
     outs = []
-    for delta, target, grid, sweeps in SETTINGS:
+    for delta, target, grid, sweeps, fused in SETTINGS:
         plan.set_delta(delta).set_round_target(target).set_grid(grid).set_max_sweeps(sweeps)
+        plan.set_fused_schedule(fused)
         plan.solve(p["seeds"])
         u = plan.values().cpu().numpy()
         st = plan.stats()
         assert np.array_equal(np.isfinite(u), np.isfinite(ref))
         err = _floor_rel(u, ref)
-        print(f"  delta {delta:g} target {target} grid {grid} sweeps {sweeps}: {st['iterations']} rounds, "
+        print(f"  delta {delta:g} target {target} grid {grid} sweeps {sweeps} fused {fused}: {st['iterations']} rounds, "
               f"{st['tile_visits']} visits, max rel err {err:.2e}", flush=True)
         assert err <= TOL[precision]
         outs.append(u)
