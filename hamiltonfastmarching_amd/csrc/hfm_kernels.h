@@ -49,12 +49,13 @@ template <typename T> struct FimArgs {
     const int *count;
     int *count_reset;
     unsigned *keys;
+    unsigned *keys_out;  // fused rounds: the key buffer this round marks into (keys: the one it takes from)
     unsigned *tmax;
     unsigned *pend_out;
     unsigned *pend_reset;
     unsigned long long *stats;  // PEND_SLOTS pairs: (tiles processed, sweeps summed)
     unsigned grid;
-    int max_sweeps;  // sweeps per tile visit before the tile is handed back to the schedule: 8, 16 or 32
+    int max_sweeps;  // sweeps per tile visit before the tile is handed back to the schedule: 4, 8, 16 or 32
     int fused;       // 1: k_fim_sched lists its own tiles with the round parameters in sr (no list)
     SchedRound sr;
 };
@@ -99,7 +100,8 @@ long long scan_work_ints(long long n);
 hipError_t launch_exclusive_scan(const int *cnt, int *start, int n, hipStream_t s, int *work = nullptr);
 template <typename T>
 hipError_t launch_init(const Geom &g, T *u, unsigned long long *fixedmask, unsigned *keys, unsigned *tmax,
-                       long long np, const long long *seed_p, const double *seed_v, int nseeds, hipStream_t s);
+                       long long np, const long long *seed_p, const double *seed_v, int nseeds, hipStream_t s,
+                       int key_buffers = 1);
 hipError_t launch_seed_activate(const long long *seed_p, const double *seed_v, int nseeds, const int *dep_start,
                                 const int *deps, unsigned *keys, const unsigned *tmax, unsigned *pend, hipStream_t s);
 hipError_t launch_schedule(int ntiles, unsigned *keys, const SchedRound &sr, hipStream_t s);

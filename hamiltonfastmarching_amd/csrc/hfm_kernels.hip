@@ -57,9 +57,14 @@ __global__ __launch_bounds__(256) void k_riemann_stencil(Geom g, const double *_
             atomicMax(&status[0], st);
             atomicMin(&status[1], (int)lin);
         }
-        // max L1 norm of a nonzero-weight offset (MaxStencilWidth, HFMInterface.hxx:357)
+        // MaxStencilWidth (HamiltonFastMarching.hxx:36-44): the largest L1 norm among the
+        // reversed offsets, which hold a nonzero-weight side x +- e_k only where x +- e_k lies in
+        // the domain (HFM_StencilRecompute.hxx:91-101; walls are attached later and do not count)
         for (int k = 0; k < K; ++k) {
             if (wd[k] == 0.) continue;
+            const int o0 = o8[k][0], o1 = D > 1 ? o8[k][D > 1 ? 1 : 0] : 0, o2 = D > 2 ? o8[k][D > 2 ? 2 : 0] : 0;
+            int r[3];
+            if (!shift_coords(g, c, o0, o1, o2, r) && !shift_coords(g, c, -o0, -o1, -o2, r)) continue;
             int l1 = 0;
             for (int q = 0; q < D; ++q) l1 += o8[k][q] < 0 ? -o8[k][q] : o8[k][q];
             l1max = l1 > l1max ? l1 : l1max;
@@ -244,9 +249,12 @@ __global__ __launch_bounds__(1024) void k_scan_apply(const int *__restrict__ cnt
 // ---------------------------------------------------------------------------------------------
 // Solve initialisation.
 // ---------------------------------------------------------------------------------------------
+// key_buffers: 1, or 2 for rounds that list their own tiles (keys[0..ntiles) and
+// keys[ntiles..2 ntiles), taken from and marked into by alternate rounds).
 template <typename T>
 __global__ __launch_bounds__(256) void k_init(Geom g, T *__restrict__ u, unsigned long long *__restrict__ fixedmask,
-                                              unsigned *__restrict__ keys, unsigned *__restrict__ tmax, long long np) {
+                                              unsigned *__restrict__ keys, unsigned *__restrict__ tmax, long long np,
+                                              int key_buffers) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     u[p] = dev_inf<T>();
@@ -257,6 +265,7 @@ __global__ __launch_bounds__(256) void k_init(Geom g, T *__restrict__ u, unsigne
         const int tile = (int)(p >> 6);
         fixedmask[tile] = pad;
         keys[tile] = KEY_NONE;
+        if (key_buffers > 1) keys[g.ntiles + tile] = KEY_NONE;
         tmax[tile] = KEY_NONE;
     }
 }
@@ -405,13 +414,23 @@ __global__ __launch_bounds__(256) void k_fim(Geom g, StencilData<T> sd, T *__res
 
 // ---------------------------------------------------------------------------------------------
 // Update kernel that schedules its own round: block b owns tiles b, b + grid, b + 2 grid, ...;
-// it lists those whose key lies below the round's band bound (consuming their keys with an
-// atomic exchange, so that a mark made by another block meanwhile is consumed too, never lost)
-// and its four waves visit them.  Every block computes the same bound from the same inputs, and
-// the separate scheduling kernel and its kernel boundary are gone from every round.  A tile
-// marked and then consumed within one round leaves its key in the next round's pending minimum,
-// so one empty round does not prove convergence: two consecutive empty rounds do (the first of
-// them consumed nothing and left the exact minimum of every remaining key).
+// it lists those whose key lies below the round's band bound and its four waves visit them.
+// Every block computes the same bound from the same inputs, and the separate scheduling kernel
+// and its kernel boundary are gone from every round.
+//
+// Keys are double-buffered: round k takes from keys_in = keys[k % 2] and marks into
+// keys_out = keys[(k + 1) % 2]; a pending tile it does not take moves its key to keys_out.  Only
+// the owning block touches keys_in[t] during round k, so the take is a plain load and store.
+// Every key a round takes was therefore written by an earlier launch, and every value that made
+// it pending was stored before that launch ended: the kernel boundary (L2 write-back on every
+// XCD at the end of a kernel, invalidation at the start of the next) makes those values visible
+// to the visiting wave, with no cache maintenance inside the kernel.  A mark made during round k
+// is consumed at the earliest by round k + 1.  Marking into a buffer nobody takes from in the
+// same launch is what makes this hold: with a single buffer, a block could consume a mark made
+// by a wave on another XCD in the same launch, then read that wave's values through a stale L2.
+//
+// The smallest pending key of round k + 1 is exact (leftover keys and marks both fold into
+// pend_out), and the band bound exceeds it, so an empty round proves convergence.
 // Round k reads pend[k % 4], accumulates pend[(k + 1) % 4] and counts[k % 4], and clears
 // pend[(k + 2) % 4] and counts[(k + 1) % 4], which no block of round k reads or writes.
 // ---------------------------------------------------------------------------------------------
@@ -419,8 +438,9 @@ template <int ModelT, typename T, int MAXSWEEP>
 __global__ __launch_bounds__(256) void k_fim_sched(Geom g, StencilData<T> sd, T *__restrict__ u,
                                                    const unsigned long long *__restrict__ fixedmask,
                                                    const int *__restrict__ dep_start, const int *__restrict__ deps,
-                                                   unsigned *__restrict__ keys, unsigned *__restrict__ tmax,
-                                                   SchedRound sr, int *__restrict__ count_reset,
+                                                   unsigned *__restrict__ keys_in, unsigned *__restrict__ keys_out,
+                                                   unsigned *__restrict__ tmax, SchedRound sr,
+                                                   int *__restrict__ count_reset,
                                                    unsigned *__restrict__ pend_reset,
                                                    unsigned long long *__restrict__ stats) {
     __shared__ T su[4][SU_ROW];
@@ -430,7 +450,7 @@ __global__ __launch_bounds__(256) void k_fim_sched(Geom g, StencilData<T> sd, T 
     const int G = gridDim.x;
     const int owned = (g.ntiles - (int)blockIdx.x + G - 1) / G;
     // the first key is loaded before the round parameters it is compared with
-    unsigned k0 = (int)threadIdx.x < owned ? keys[blockIdx.x + threadIdx.x * G] : KEY_NONE;
+    unsigned k0 = (int)threadIdx.x < owned ? keys_in[blockIdx.x + threadIdx.x * G] : KEY_NONE;
     const float d0 = *sr.delta0;
     float delta = d0;
     if (delta > 0.f) delta = adapt_delta(*sr.delta_in, *sr.prev_count, sr.target, 0.5f * d0, 64.f * d0);
@@ -443,16 +463,17 @@ __global__ __launch_bounds__(256) void k_fim_sched(Geom g, StencilData<T> sd, T 
         }
         if (threadIdx.x < PEND_SLOTS) pend_reset[threadIdx.x] = KEY_NONE;
     }
-    const TileCtx<T> ctx{g, sd, u, fixedmask, dep_start, deps, keys, tmax};
+    const TileCtx<T> ctx{g, sd, u, fixedmask, dep_start, deps, keys_out, tmax};
     unsigned long long my_sweeps = 0, my_visits = 0;
     unsigned wmin = KEY_NONE;
     for (int c0 = 0; c0 < owned; c0 += 256) {
         const int j = c0 + (int)threadIdx.x;
         const int t = (int)blockIdx.x + j * G;
-        unsigned k = c0 == 0 ? k0 : (j < owned ? keys[t] : KEY_NONE);
+        const unsigned k = c0 == 0 ? k0 : (j < owned ? keys_in[t] : KEY_NONE);
         const bool take = band_take(k, B);
-        if (take) (void)atomicExch(&keys[t], KEY_NONE);
         const unsigned left = (!take && k != KEY_NONE) ? k : KEY_NONE;
+        if (k != KEY_NONE) keys_in[t] = KEY_NONE;
+        if (left != KEY_NONE) atomicMin(&keys_out[t], left);
         const unsigned long long bm = __ballot(take);
         const unsigned lmin = wave_min_u32(left);
         wmin = lmin < wmin ? lmin : wmin;
@@ -679,17 +700,19 @@ hipError_t launch_exclusive_scan(const int *cnt, int *start, int n, hipStream_t 
 
 template <typename T>
 hipError_t launch_init(const Geom &g, T *u, unsigned long long *fixedmask, unsigned *keys, unsigned *tmax,
-                       long long np, const long long *seed_p, const double *seed_v, int nseeds, hipStream_t s) {
-    hipLaunchKernelGGL((k_init<T>), dim3(blocks_for(np, 256)), dim3(256), 0, s, g, u, fixedmask, keys, tmax, np);
+                       long long np, const long long *seed_p, const double *seed_v, int nseeds, hipStream_t s,
+                       int key_buffers) {
+    hipLaunchKernelGGL((k_init<T>), dim3(blocks_for(np, 256)), dim3(256), 0, s, g, u, fixedmask, keys, tmax, np,
+                       key_buffers);
     if (nseeds > 0)
         hipLaunchKernelGGL((k_set_seeds<T>), dim3(blocks_for(nseeds, 64)), dim3(64), 0, s, seed_p, seed_v, nseeds, u,
                            fixedmask);
     return hipGetLastError();
 }
 template hipError_t launch_init<float>(const Geom &, float *, unsigned long long *, unsigned *, unsigned *,
-                                       long long, const long long *, const double *, int, hipStream_t);
+                                       long long, const long long *, const double *, int, hipStream_t, int);
 template hipError_t launch_init<double>(const Geom &, double *, unsigned long long *, unsigned *, unsigned *,
-                                        long long, const long long *, const double *, int, hipStream_t);
+                                        long long, const long long *, const double *, int, hipStream_t, int);
 
 hipError_t launch_seed_activate(const long long *seed_p, const double *seed_v, int nseeds, const int *dep_start,
                                 const int *deps, unsigned *keys, const unsigned *tmax, unsigned *pend, hipStream_t s) {
@@ -711,11 +734,12 @@ static void fim_sweeps(const FimArgs<T> &a, hipStream_t s, hipEvent_t ev0, hipEv
     if (a.fused) {
         if (ev0)
             hipExtLaunchKernelGGL((k_fim_sched<M, T, MS>), dim3(a.grid), dim3(256), 0, s, ev0, ev1, 0, a.g, a.sd, a.u,
-                                  a.fixedmask, a.dep_start, a.deps, a.keys, a.tmax, a.sr, a.count_reset,
-                                  a.pend_reset, a.stats);
+                                  a.fixedmask, a.dep_start, a.deps, a.keys, a.keys_out, a.tmax, a.sr,
+                                  a.count_reset, a.pend_reset, a.stats);
         else
             hipLaunchKernelGGL((k_fim_sched<M, T, MS>), dim3(a.grid), dim3(256), 0, s, a.g, a.sd, a.u, a.fixedmask,
-                               a.dep_start, a.deps, a.keys, a.tmax, a.sr, a.count_reset, a.pend_reset, a.stats);
+                               a.dep_start, a.deps, a.keys, a.keys_out, a.tmax, a.sr, a.count_reset, a.pend_reset,
+                               a.stats);
         return;
     }
     if (ev0)
@@ -730,7 +754,8 @@ static void fim_sweeps(const FimArgs<T> &a, hipStream_t s, hipEvent_t ev0, hipEv
 
 template <int M, typename T>
 static void fim_one(const FimArgs<T> &a, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
-    if (a.max_sweeps <= 8) fim_sweeps<M, T, 8>(a, s, ev0, ev1);
+    if (a.max_sweeps <= 4) fim_sweeps<M, T, 4>(a, s, ev0, ev1);
+    else if (a.max_sweeps <= 8) fim_sweeps<M, T, 8>(a, s, ev0, ev1);
     else if (a.max_sweeps <= 16) fim_sweeps<M, T, 16>(a, s, ev0, ev1);
     else fim_sweeps<M, T, 32>(a, s, ev0, ev1);
 }

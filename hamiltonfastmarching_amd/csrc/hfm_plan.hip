@@ -309,7 +309,7 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
     HIPCHECK(hipEventRecord(e0, s));
     const int nseeds = (int)p->last_seeds.size();
     HIPCHECK(launch_init<T>(p->g, static_cast<T *>(p->u), p->fixedmask, p->keys, p->tmax, p->np, p->seed_p,
-                            p->seed_v, nseeds, s));
+                            p->seed_v, nseeds, s, 2));
     HIPCHECK(hipMemsetAsync(p->counts, 0, 4 * sizeof(int), s));
     HIPCHECK(hipMemsetAsync(p->stats, 0, 2 * PEND_SLOTS * sizeof(unsigned long long), s));
     HIPCHECK(hipMemsetAsync(p->nextmin, 0xff, 4 * PEND_SLOTS * sizeof(unsigned), s));
@@ -346,11 +346,17 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
     // 384 blocks, 3.4 -> 3.0 ms at 256 x 256 x 64.
     const int grid_default = p->model == MODEL_RSF2 ? 384 : 640;
     a.grid = (unsigned)std::min<long long>((p->g.ntiles + 3) / 4, p->fim_grid > 0 ? p->fim_grid : grid_default);
-    a.max_sweeps = p->max_sweeps;
+    // At most 4 sweeps per visit on Riemann3 (128^3: 2.73 against 2.79 ms at 8; a tile still
+    // changing after 4 sweeps goes back to the schedule); 8 elsewhere (Riemann2 256^2 FP32: 0.90
+    // ms at 8 against 1.07 at 4, the self-listing rounds needing more rounds at 4).
+    a.max_sweeps = p->max_sweeps > 0 ? p->max_sweeps : p->model == MODEL_RIEMANN3 ? 4 : 8;
     // Rounds that list their own tiles save the listing kernel and a kernel boundary per round,
-    // but each block then visits only the tiles it owns: Riemann2 256^2 0.96 -> 0.87 ms; 3-D rounds
-    // of thousands of tiles lose more to the uneven visits per wave (Riemann3 128^3 2.79 -> 3.67 ms).
-    a.fused = p->fused_schedule > 0 || (p->fused_schedule < 0 && p->g.dim == 2) ? 1 : 0;
+    // but each block then visits only the tiles it owns, and a mark is taken one round later than
+    // with the listing kernel (the keys are double-buffered).  That pays on small 2-D grids
+    // (Riemann2 256^2 FP32 0.98 -> 0.90 ms, FP64 1.63 -> 1.52 ms) and not on larger ones, whose
+    // rounds list a thousand tiles or more and leave the blocks' shares uneven (Riemann2 1024^2
+    // 5.40 -> 5.70 ms, Riemann3 128^3 2.79 -> 3.67 ms).
+    a.fused = p->fused_schedule > 0 || (p->fused_schedule < 0 && p->g.dim == 2 && p->g.ntiles <= 4096) ? 1 : 0;
     std::vector<hipEvent_t> tev;
     long long k = 0;
     const int batch = 8;  // even: every batch starts on an even round, so all batches are alike
@@ -361,7 +367,10 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
         for (long long kk = k0; kk < k0 + batch; ++kk) {
             if (a.fused) {
                 // round kk lists and visits its own tiles: pend[kk % 4] in, pend[(kk + 1) % 4] and
-                // counts[kk % 4] out, pend[(kk + 2) % 4] and counts[(kk + 1) % 4] cleared
+                // counts[kk % 4] out, pend[(kk + 2) % 4] and counts[(kk + 1) % 4] cleared; it takes
+                // from key buffer kk % 2 and marks into buffer (kk + 1) % 2
+                a.keys = p->keys + (kk & 1) * (long long)p->g.ntiles;
+                a.keys_out = p->keys + ((kk + 1) & 1) * (long long)p->g.ntiles;
                 a.sr = SchedRound{&p->nextmin[(kk % 4) * PEND_SLOTS],
                                   &p->nextmin[((kk + 1) % 4) * PEND_SLOTS],
                                   &p->bounds[(kk + 1) & 1],
@@ -456,8 +465,8 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
         }
         ahead = 0;
         HIPCHECK(hipStreamSynchronize(s));
-        // converged: an empty round (fused schedule: two empty rounds in a row)
-        if (p->h_count[0] == 0 && (!a.fused || p->h_count[1] == 0)) break;
+        // converged: the batch's last round listed nothing
+        if (p->h_count[a.fused ? 1 : 0] == 0) break;
         if (k > max_rounds) throw HipFail{"block fast-iterative method did not converge"};
     }
     p->prev_rounds = k;
@@ -615,7 +624,7 @@ int hfm_plan_create(hfm_plan **out, const char *model, const int64_t *dims, cons
         p->dep_src = dalloc<int>((size_t)nt * 64);
         p->dep_nsrc = dalloc<int>(nt);
         p->scan_work = dalloc<int>(scan_work_ints(nt));
-        p->keys = dalloc<unsigned>(nt);
+        p->keys = dalloc<unsigned>(2 * (size_t)nt);  // two buffers: rounds that list their own tiles alternate them
         p->tmax = dalloc<unsigned>(nt);
         p->nextmin = dalloc<unsigned>(4 * PEND_SLOTS);
         p->bounds = dalloc<float>(5);  // 2 band bounds, 2 band widths, initial band width
@@ -1164,7 +1173,12 @@ int hfm_plan_set_max_sweeps(hfm_plan *p, int sweeps) {
         set_error("hfm_plan_set_max_sweeps: null argument");
         return 1;
     }
-    p->max_sweeps = sweeps <= 8 ? 8 : sweeps <= 16 ? 16 : 32;
+    if (sweeps != 0 && sweeps != 4 && sweeps != 8 && sweeps != 16 && sweeps != 32) {
+        set_error("hfm_plan_set_max_sweeps: sweeps must be 4, 8, 16 or 32, or 0 for the default (got " +
+                  std::to_string(sweeps) + ")");
+        return 1;
+    }
+    p->max_sweeps = sweeps;
     return 0;
 }
 

@@ -232,8 +232,9 @@ int hfm_plan_set_round_target(hfm_plan *plan, int tiles);
 int hfm_plan_set_grid(hfm_plan *plan, int blocks);
 
 /* Jacobi sweeps a tile visit may run before an unconverged tile is handed back to the
- * schedule (8, 16 or 32; other values round up; default 8).  A tuning knob: it changes the
- * work per visit and the number of visits, never the fixed point. */
+ * schedule: 4, 8, 16 or 32, or 0 for the default (4 for Riemann3, 8 for the other models); any
+ * other value is an error.  A tuning knob: it changes the work per visit and the number of visits, never the
+ * fixed point. */
 int hfm_plan_set_max_sweeps(hfm_plan *plan, int sweeps);
 
 /* Per-launch timing of the update kernel (start/stop events taken from each dispatch's own
@@ -249,8 +250,9 @@ int hfm_plan_set_graph(hfm_plan *plan, int enable);
 /* Let every round of the update loop list its own tiles (enable = 1: one kernel a round, each
  * block taking the pending tiles it owns below the round's band bound, then visiting them), or
  * schedule each round with a separate listing kernel (0); -1 (the default): the first on 2-D
- * grids, the second on 3-D grids, whose large rounds balance better over a global list.  Both
- * compute the same fixed point.  No counterpart in the reference. */
+ * grids of at most 4096 tiles, the second elsewhere (rounds of a thousand tiles and more
+ * balance better over a global list).  Both compute the same fixed point.  No counterpart in
+ * the reference. */
 int hfm_plan_set_fused_schedule(hfm_plan *plan, int enable);
 
 /* ----------------------------------------------------------------------------------------- */

@@ -406,6 +406,30 @@ def stencil_cases():
     print("stencils ok")
 
 
+_TRIPLE = re.compile(r"\{(-?\d+),(-?\d+),(-?\d+)\}")
+
+
+def riemann3_offsets_all(p, chunks=4):
+    """Selling offsets of every grid point of a Riemann3 problem, as the reference prints them for
+    getStencils (HFMInterface.hxx:360-375): (N, 6, 3) int8 in row-major point order and slot order.
+    Each entry of the string is {{i,j,k},{"forward" -> {{}},"symmetric" -> {{{w,{a,b,c}},...}},}}:
+    seven integer triples, the index then the six offsets (a weight never sits in an integer triple)."""
+    dims = [int(n) for n in p["dims"]]
+    N = int(np.prod(dims))
+    pts = cell_centres(dims, p["gridScale"])
+    off = np.zeros((N, 6, 3), np.int8)
+    for ci, sl in enumerate(np.array_split(np.arange(N), chunks)):
+        r = oracle.run_reference(dict(p, getStencils=pts[sl]), timeout=3600)
+        assert r["retcode"] == 0, r["log"]
+        t = np.array(_TRIPLE.findall(r["stencils"]), dtype=np.int64).reshape(-1, 7, 3)
+        assert t.shape[0] == sl.size, (t.shape, sl.size)
+        lin = np.ravel_multi_index(tuple(t[:, 0, q] for q in range(3)), dims)
+        assert np.array_equal(lin, sl)
+        off[sl] = t[:, 1:, :].astype(np.int8)
+        print(f"  offsets chunk {ci + 1}/{chunks}", flush=True)
+    return off
+
+
 def big_case():
     p = syn.riemann3(128, seed=0)
     r = oracle.run_reference(p)
@@ -413,10 +437,12 @@ def big_case():
     v = r["values"]
     rng = np.random.Generator(np.random.PCG64(128))
     idx = rng.integers(0, v.size, size=4096)
+    off = riemann3_offsets_all(p)
     np.savez_compressed(os.path.join(HERE, "riemann3_128.npz"), values_sha256=sha(v), sample_index=idx,
                         sample_values=v.ravel()[idx], metric_sha256=sha(p["metric"]),
                         n_finite=int(np.isfinite(v).sum()), vmax=float(np.nanmax(np.where(np.isfinite(v), v, np.nan))),
-                        StencilCPUTime=r["StencilCPUTime"], FMCPUTime=r["FMCPUTime"])
+                        StencilCPUTime=r["StencilCPUTime"], FMCPUTime=r["FMCPUTime"],
+                        offsets_sha256=sha(off), sample_offsets=off[idx], MaxStencilWidth=r["MaxStencilWidth"])
     print("riemann3_128 ok")
 
 

@@ -207,17 +207,61 @@ def test_curvature_vs_golden(name, make, precision):
         assert serr <= (1e-9 if precision == 64 else 2e-3)
 
 
-@pytest.mark.parametrize("precision", [32, 64])
-def test_riemann3_128_vs_reference_samples(precision):
-    """Benchmark size: 4096 seeded samples of the reference's FP64 values."""
-    import torch
-    print(f"riemann3 128 precision {precision}", flush=True)
+@pytest.fixture(scope="module")
+def riemann3_128():
+    """Benchmark field (Riemann3 128^3) and the oracle's offsets, weights and FP64 values, each
+    checked first against the reference's digests."""
+    import hashlib
+
+    def sha(a):
+        return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
     g = _gold("riemann3_128")
     p = syn.riemann3(128, seed=0)
+    assert sha(p["metric"]) == str(g["metric_sha256"])
+    roff, rw = oracle.riemann_stencils((128,) * 3, p["gridScale"], p["metric"])
+    assert sha(roff) == str(g["offsets_sha256"])
+    ref = oracle.solve(p)["values"]
+    assert sha(ref) == str(g["values_sha256"])
+    return p, g, roff, rw, ref
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_riemann3_128_offsets_bitexact(riemann3_128, precision):
+    """Benchmark size: all 2,097,152 x 6 Selling offsets bit-exact, slot by slot, against the
+    oracle (whose digest equals the reference's getStencils output), and the weights equal to
+    the oracle's FP64 weights (rounded to FP32 in an FP32 plan)."""
+    import torch
+    print(f"riemann3 128 offsets precision {precision}", flush=True)
+    p, g, roff, rw, _ = riemann3_128
+    plan = hfm.Plan("Riemann3", (128,) * 3, p["gridScale"], precision=precision)
+    plan.set_metric(torch.from_numpy(p["metric"]).cuda())
+    off, w = plan.stencils()
+    nbad = int(np.sum(np.any(off != roff, axis=(1, 2))))
+    print(f"riemann3 128^3 fp{precision}: {nbad} points with differing offsets", flush=True)
+    assert nbad == 0
+    assert np.array_equal(off[g["sample_index"]], g["sample_offsets"])
+    if precision == 64:
+        assert np.array_equal(w, rw)
+    else:
+        assert np.array_equal(w, rw.astype(np.float32).astype(np.float64))
+    assert plan.stats()["max_stencil_width"] == int(g["MaxStencilWidth"])
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_riemann3_128_values(riemann3_128, precision):
+    """Benchmark size: every value against the oracle's FP64 values (equal to the reference's by
+    digest) and the 4096 seeded samples of the reference fixture."""
+    import torch
+    print(f"riemann3 128 precision {precision}", flush=True)
+    p, g, _, _, full = riemann3_128
     plan = hfm.Plan("Riemann3", (128,) * 3, p["gridScale"], precision=precision)
     plan.set_metric(torch.from_numpy(p["metric"]))
     plan.solve(p["seeds"])
-    u = plan.values().cpu().numpy().ravel()
+    uf = plan.values().cpu().numpy()
+    assert np.array_equal(np.isfinite(uf), np.isfinite(full))
+    ferr = _floor_rel(uf, full)
+    u = uf.ravel()
     idx, ref = g["sample_index"], g["sample_values"]
     us = u[idx]
     fin = np.isfinite(ref)
@@ -226,5 +270,7 @@ def test_riemann3_128_vs_reference_samples(precision):
     floor = ref[fin & (ref > 0)].min()
     m = fin & (ref > 0)
     err = float(np.max(np.abs(us[m] - ref[m]) / ref[m]))
-    print(f"riemann3 128^3 fp{precision}: sample rel err {err:.3e} (floor {floor:.3e})", flush=True)
+    print(f"riemann3 128^3 fp{precision}: all points rel err {ferr:.3e}, sample rel err {err:.3e} "
+          f"(floor {floor:.3e})", flush=True)
     assert err <= (1e-12 if precision == 64 else 1e-5)
+    assert ferr <= (1e-12 if precision == 64 else 1e-5)

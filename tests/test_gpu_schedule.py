@@ -26,6 +26,8 @@ SETTINGS = [
     (8.0, 0, 1, 16, True),      # one block of four waves (owning every tile)
     (8.0, 0, 1, 16, False),
     (8.0, 0, 7, 8, True),       # an odd grid
+    (8.0, 0, 0, 4, True),       # at most 4 sweeps per visit
+    (8.0, 0, 0, 4, False),
 ]
 CASES = [
     ("riemann2_96", lambda: syn.riemann2(96, seed=21)),
@@ -74,3 +76,41 @@ def test_fixed_point_independent_of_schedule(name, make, precision):
     spread = max(_floor_rel(u, outs[0]) for u in outs[1:])
     print(f"  largest difference between settings {spread:.2e}", flush=True)
     assert spread <= TOL[precision]
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_self_listing_rounds_many_chunks(precision):
+    """Rounds that list their own tiles, with several blocks each owning several 256-tile chunks
+    (Riemann2 256^2 = 1024 tiles on 2 to 3 blocks): a block visits chunk after chunk while the other
+    blocks, possibly on other XCDs, mark tiles it owns.  Values agree with the oracle and pass
+    the FP64 residual check."""
+    import torch
+    p = syn.riemann2(256, seed=24)
+    ref = oracle.solve(p)["values"]
+    plan = hfm.Plan("Riemann2", (256, 256), float(p["gridScale"]), precision=precision)
+    plan.set_metric(torch.from_numpy(p["metric"]).cuda())
+    for grid in (2, 3, 0):
+        print(f"self-listing rounds Riemann2 256^2 precision {precision} grid {grid}", flush=True)
+        plan.set_fused_schedule(True).set_grid(grid)
+        plan.solve(p["seeds"])
+        u = plan.values().cpu().numpy()
+        assert np.array_equal(np.isfinite(u), np.isfinite(ref))
+        err = _floor_rel(u, ref)
+        dev, bad = plan.residual()
+        st = plan.stats()
+        print(f"  {st['iterations']} rounds, {st['tile_visits']} visits, max rel err {err:.2e}, "
+              f"residual {dev:.2e}", flush=True)
+        assert err <= TOL[precision]
+        assert bad == 0 and dev <= (1e-12 if precision == 64 else 2e-6)
+
+
+def test_sweep_cap_values():
+    """The sweep cap takes 4, 8, 16, 32 or 0 (the model's default); anything else is an error,
+    not a silent rounding."""
+    p = syn.riemann2(16, seed=25)
+    plan = hfm.Plan("Riemann2", (16, 16), float(p["gridScale"]), precision=32)
+    for ok in (0, 4, 8, 16, 32):
+        plan.set_max_sweeps(ok)
+    for bad in (1, 5, 12, 64, -1):
+        with pytest.raises(hfm.PyExp, match="sweeps must be 4, 8, 16 or 32"):
+            plan.set_max_sweeps(bad)

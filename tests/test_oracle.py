@@ -128,6 +128,27 @@ def test_riemann3_128_digest():
     assert sha(v) == str(g["values_sha256"])
 
 
+def test_riemann3_128_offsets_digest():
+    """Benchmark-size Selling offsets: all 2,097,152 x 6 offsets against the digest of the
+    reference's getStencils output (row-major points, slot order), 4096 seeded samples, and
+    MaxStencilWidth.  A transposed component or a swapped slot changes the digest."""
+    g = load("riemann3_128")
+    p = syn.riemann3(128, seed=0)
+    off, w = oracle.riemann_stencils((128,) * 3, p["gridScale"], p["metric"])
+    assert np.array_equal(off[g["sample_index"]], g["sample_offsets"])
+    assert sha(off) == str(g["offsets_sha256"])
+    # MaxStencilWidth: largest L1 norm of a nonzero-weight side x +- e whose neighbour is in the grid
+    c = np.stack(np.unravel_index(np.arange(128 ** 3), (128,) * 3), axis=1)[:, None, :]
+    o = off.astype(np.int32)
+    inside = np.all((c + o >= 0) & (c + o < 128), axis=2) | np.all((c - o >= 0) & (c - o < 128), axis=2)
+    l1 = np.abs(o).sum(axis=2)
+    assert int(np.max(np.where((w != 0) & inside, l1, 0))) == int(g["MaxStencilWidth"])
+    assert int(np.max(np.where(w != 0, l1, 0))) > int(g["MaxStencilWidth"])  # the grid test matters here
+    # planted mistakes: components reversed, or two slots swapped, change the digest
+    assert sha(np.ascontiguousarray(off[..., ::-1])) != str(g["offsets_sha256"])
+    assert sha(np.ascontiguousarray(off[:, [1, 0, 2, 3, 4, 5]])) != str(g["offsets_sha256"])
+
+
 # ------------------------------------------------------------------------------------------
 # Closed forms and invariants
 # ------------------------------------------------------------------------------------------
