@@ -17,18 +17,24 @@ Printed JSON fields beyond the driver's contract:
                 achieved_per_visit counts each tile visit once instead (its data is read from
                 HBM once and swept in LDS/registers), and solve_effective_gbs divides the
                 stencil + value bytes of the whole grid by the solve time.  traffic = HBM bytes
-                per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by a 1 GiB copy
-                (profiles/fim_riemann3_128_traffic.json).
+                per launch from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) run by this
+                script over a short run of itself, each scaled by what it reads for a plain
+                1 GiB device copy (the committed profiles/fim_riemann3_128_traffic.json only
+                when rocprofv3 is absent); copy_ceiling_gbs = a timed 1 GiB device copy,
+                (read + write bytes) / time, the practical ceiling beside the 8 TB/s peak.
   serving       the continuous-batching server on a stream of mixed Riemann2 / Riemann3
                 requests (70 % 2-D, side 64-1024 log-uniform; 30 % 3-D, side 32-160; 1-16
                 seeds; anisotropy 1-50): sustained solves/s with 128 requests kept in flight
                 (closed loop, window excluding the first and last 128 completions), the rate
                 of a burst with every request submitted at once (makespan), p50 / p99
                 latency at an offered Poisson load of 5x the one-at-a-time rate, and the
-                one-at-a-time plain GPU solves/s of the same requests.
+                one-at-a-time plain GPU solves/s of the same requests; served_sample_check:
+                served results of the largest 2-D request and three smaller ones against the
+                oracle's FP64 solve, every point.
   cpu_baseline  the reference's FileHFM_Riemann3, compiled from its sources into oracle/_ref,
-                timed on this host (one full solve, 1 core; StencilCPUTime + FMCPUTime), or the
-                oracle's C++ restatement when the reference binary is absent.
+                timed on this host (median of 3 full solves, 1 core; StencilCPUTime + FMCPUTime),
+                or the oracle's C++ restatement when the reference binary is absent; with the
+                host's CPU model.
   accuracy      max relative error of the FP32 values vs the reference FP64 values (floored at
                 the smallest nonzero reference value), from the CPU-baseline run.
   configs       the other workloads on the GPU (ms per solve from HBM-resident inputs; geodesics
@@ -76,37 +82,96 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 1 GiB device-to-device copy (reference kernel for the "
                          "FETCH_SIZE / WRITE_SIZE counter scale)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="do not measure the update kernel's HBM traffic with rocprofv3 (use the committed "
+                         "profile's figure)")
     return ap.parse_args()
 
 
-def cpu_baseline(params):
-    """Time one full CPU solve: the reference binary if built, else the oracle restatement."""
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), as lscpu reports it."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(params, runs=3):
+    """Time full CPU solves: the reference binary if built, else the oracle restatement; the
+    median of `runs` runs (StencilCPUTime + FMCPUTime, clock() CPU time, 1 core)."""
     import oracle
     cores = 1
-    if oracle.ref_available(params["model"]):
+    use_ref = oracle.ref_available(params["model"])
+    times, fms, walls = [], [], []
+    values = None
+    for _ in range(runs):
         t = time.time()
-        out = oracle.run_reference(params)
-        wall = time.time() - t
-        if out.get("retcode", 1) != 0:
+        out = oracle.run_reference(params) if use_ref else oracle.solve(params)
+        walls.append(time.time() - t)
+        if use_ref and out.get("retcode", 1) != 0:
             raise RuntimeError("reference run failed: " + out.get("log", "")[-500:])
-        cpu_s = out["StencilCPUTime"] + out["FMCPUTime"]
-        kind = "reference"
+        times.append(out["StencilCPUTime"] + out["FMCPUTime"])
+        fms.append(out["FMCPUTime"])
         values = out["values"]
-        fm_s = out["FMCPUTime"]
-    else:
-        t = time.time()
-        out = oracle.solve(params)
-        wall = time.time() - t
-        cpu_s = out["StencilCPUTime"] + out["FMCPUTime"]
-        kind = "port"
-        values = out["values"]
-        fm_s = out["FMCPUTime"]
+    i = int(np.argsort(times)[len(times) // 2])
     return {
-        "value": round(cpu_s * 1e3, 1), "unit": "ms/solve", "cores": cores, "kind": kind,
-        "sample": f"1 full Riemann3 {params['dims'][0]:.0f}^3 solve (StencilCPUTime + FMCPUTime, "
-                  f"clock() CPU time), wall {wall:.1f} s",
-        "fm_ms": round(fm_s * 1e3, 1), "nproc": os.cpu_count(),
+        "value": round(times[i] * 1e3, 1), "unit": "ms/solve", "cores": cores,
+        "kind": "reference" if use_ref else "port",
+        "sample": f"median of {runs} full Riemann3 {params['dims'][0]:.0f}^3 solves (StencilCPUTime + FMCPUTime, "
+                  f"clock() CPU time, 1 core), wall {sum(walls):.1f} s in all",
+        "runs_ms": [round(t * 1e3, 1) for t in times],
+        "fm_ms": round(fms[i] * 1e3, 1), "nproc": os.cpu_count(), "cpu_model": cpu_model(),
     }, values
+
+
+def measure_traffic(timeout_s=180):
+    """HBM bytes per launch of the update kernel k_fim, measured now: two rocprofv3 --pmc passes
+    (FETCH_SIZE, then WRITE_SIZE) over a short run of this benchmark (2 timed solves and a plain
+    1 GiB device copy), each counter scaled by what it reads for the copy.  Returns (result, note)."""
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    from hamiltonfastmarching_amd import profiling
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None, "rocprofv3 not found"
+    child = [sys.executable, "-u", os.path.abspath(__file__), "--steps", "2", "--warmup", "1", "--no-cpu",
+             "--no-serving", "--no-configs", "--copy-calibration", "--no-traffic"]
+    with tempfile.TemporaryDirectory(prefix="hfm_pmc_") as td:
+        dbs = []
+        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(td, ctr)
+            r = subprocess.run(["timeout", "-s", "KILL", str(timeout_s), exe, "--pmc", ctr, "-d", d, "-o", "run", "--",
+                                *child], capture_output=True, text=True)
+            found = glob.glob(os.path.join(d, "**", "run_results.db"), recursive=True)
+            if not found:
+                return None, f"rocprofv3 --pmc {ctr}: rc {r.returncode}, no results ({r.stderr[-300:]})"
+            dbs.append(found[0])
+        res = profiling.counters(dbs[0], dbs[1], "k_fim", float(1 << 30))
+    return res, "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in this run, scaled by a 1 GiB device copy"
+
+
+def copy_ceiling_gbs(nbytes=1 << 30, reps=10):
+    """Practical HBM ceiling: a plain device-to-device copy of nbytes, (read + write bytes) / time."""
+    import torch
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return 2.0 * nbytes / (ms * 1e-3) / 1e9
 
 
 def serving_leg(count, with_cpu=False):
@@ -119,6 +184,7 @@ def serving_leg(count, with_cpu=False):
     srv = serving.Server(hbm_budget_bytes=64e9, max_resident=256)
     try:
         serving.run_load(srv, reqs[:16], qps=0)
+        check = served_sample_check(srv, reqs)
         burst = serving.run_load(srv, reqs, qps=0)
         closed = serving.run_closed_loop(srv, reqs, concurrency=128, rounds=3)
         offered = 5.0 * seq["solves_per_s"]
@@ -145,9 +211,36 @@ def serving_leg(count, with_cpu=False):
                                   "p50_ms": round(load["p50_ms"], 3), "p99_ms": round(load["p99_ms"], 3)},
         "sequential_latency": {"p50_ms": round(seq["p50_ms"], 3), "p99_ms": round(seq["p99_ms"], 3)},
         "max_resident_seen": stats["max_resident_seen"],
+        "served_sample_check": check,
         "cpu_aggregate": cpu,
         "speedup_vs_cpu_aggregate": round(sustained / cpu["solves_per_s"], 1) if cpu and cpu["solves_per_s"] else None,
     }
+
+
+def served_sample_check(srv, reqs, count=4, max_points=300_000):
+    """Results of the server checked against the oracle's FP64 Fast Marching (all points of each
+    request): the largest 2-D request of the stream and `count` - 1 others drawn at random among
+    those of at most max_points points (the oracle is a single-core CPU solve)."""
+    import oracle
+    from hamiltonfastmarching_amd import serving
+    rng = np.random.Generator(np.random.PCG64(5))
+    two = [i for i, r in enumerate(reqs) if r["model"] == "Riemann2"]
+    pick = [max(two, key=lambda i: int(np.prod(reqs[i]["dims"])))] if two else []
+    small = [i for i, r in enumerate(reqs) if int(np.prod(r["dims"])) <= max_points and i not in pick]
+    pick += [int(i) for i in rng.choice(small, size=min(count - len(pick), len(small)), replace=False)]
+    sub = [reqs[i] for i in pick]
+    res = serving.run_load(srv, sub, qps=0, collect_values=True)
+    worst = 0.0
+    cases = []
+    for r, u in zip(sub, res["values"]):
+        ref = oracle.solve({"model": r["model"], "dims": list(r["dims"]), "gridScale": r["gridScale"],
+                            "metric": r["metric"].cpu().numpy(), "seeds": r["seeds"]})["values"]
+        same = bool(np.array_equal(np.isfinite(u), np.isfinite(ref)))
+        err = floor_rel(u, ref) if same else float("inf")
+        worst = max(worst, err)
+        cases.append({"model": r["model"], "dims": list(r["dims"]), "kappa": round(r["kappa"], 1),
+                      "seeds": len(r["seeds"]), "max_rel_err": err})
+    return {"max_rel_err_vs_oracle": worst, "tolerance": 1e-5, "cases": cases}
 
 
 def cpu_serving(reqs, budget_s=15.0, max_workers=16):
@@ -267,11 +360,17 @@ def configs_leg(with_cpu):
     cost = torch.from_numpy(full["cost"]).cuda()
     ms = _time_gpu(lambda: (plan.set_cost(cost), plan.solve(full["seeds"])), 3)
     ams = _time_gpu(lambda: plan.adjoint(targets), 3)
+    u = plan.values().cpu().numpy()
+    sens = plan.adjoint(targets)[0].cpu().numpy()
     c, ref = cpu(full)
     r = {"gpu_ms": round(ms, 3), "adjoint_gpu_ms": round(ams, 3), "cpu": c}
     if ref is not None:
         r["speedup_solve"] = round(c["ms"] / ms, 1)
         r["cpu_wall_ms_with_backward"] = c["wall_ms"]
+        r["max_rel_err_vs_reference"] = floor_rel(u, np.asarray(ref["values"]).reshape(dims))
+        sref = np.asarray(ref["costSensitivity_0"]).reshape(dims)
+        r["adjoint_max_err_vs_reference"] = float(np.abs(sens - sref).max() / np.abs(sref).max())
+        r["adjoint_err_definition"] = "max |costSensitivity - reference| / max |reference| (FP32 solve and adjoint)"
     res["elastica2_192x192x64"] = r
     del plan
     torch.cuda.synchronize()
@@ -338,18 +437,33 @@ def main():
 
     achieved = gbs(sti["sweeps"] * 64 * BYTES_PER_POINT_UPDATE)
     achieved_visit = gbs(sti["tile_visits"] * 64 * BYTES_PER_POINT_UPDATE)
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "fim_riemann3_128_traffic.json")
-    if os.path.exists(tpath) and n == 128:
-        with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    traffic, traffic_source = None, None
+    if n == 128 and not args.no_traffic:
+        tr, note = measure_traffic()
+        if tr is not None and tr.get("hbm_bytes_per_launch") == tr.get("hbm_bytes_per_launch"):
+            traffic, traffic_source = tr["hbm_bytes_per_launch"], note
+        else:
+            traffic_source = "not measured: " + note
+    if traffic is None and n == 128:
+        tpath = os.path.join(ROOT, "profiles", "fim_riemann3_128_traffic.json")
+        if os.path.exists(tpath):
+            with open(tpath) as f:
+                traffic = json.load(f).get("hbm_bytes_per_launch")
+            traffic_source = (traffic_source + "; " if traffic_source else "") + \
+                "profiles/fim_riemann3_128_traffic.json (committed profile)"
+    ceiling = copy_ceiling_gbs()
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_source,
+        "traffic_gbs": round(traffic / (mean_launch_ms * 1e-3) / 1e9, 1) if traffic and mean_launch_ms > 0 else None,
+        "frac_traffic": round(traffic / (mean_launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        if traffic and mean_launch_ms > 0 else None,
+        "copy_ceiling_gbs": round(ceiling, 1),
         "kernel": "k_fim<Riemann3,float>", "launches": sti["fim_launches"],
         "mean_launch_us": round(mean_launch_ms * 1e3, 2), "unit_of_work": "point-update",
         "bytes_per_point_update": BYTES_PER_POINT_UPDATE, "point_updates": sti["sweeps"] * 64,
         "tile_visits": sti["tile_visits"], "achieved_per_visit": round(achieved_visit, 1),
+        "frac_per_visit": round(achieved_visit / HBM_PEAK_GBS, 4),
         "solve_effective_gbs": round(n ** 3 * BYTES_PER_POINT_SOLVE / (ms * 1e-3) / 1e9, 1),
     }
     result = {

@@ -64,8 +64,7 @@ __device__ inline int active_set(const Geom &g, const StencilData<T> &sd, const 
         idx[k] = a_first ? ia : ib;
         if (!(v[k] < uu)) v[k] = inf;
     }
-    uint32_t used = 0;
-    (void)local_solve<NC, T>(v, st.w, used);
+    const uint32_t used = active_slots<NC, T>(v, st.w);
     const double mult = S::SHARED ? (double)sd.speed[p] : 1.;
     int n = 0;
     double ws = 0., acc = 0.;

@@ -299,14 +299,47 @@ template <> struct SortNet<6> {
     }
 };
 
+// Active differences of a local solve (the bitmask of accumulated slots), decided in FP64 --
+// the reference's arithmetic -- whatever the storage precision of the values and weights.  An
+// FP32 evaluation of local_solve decides "next value below the current root" against a root
+// computed relative to the smallest value, which loses most of its digits when a low-weight
+// candidate lies far below a heavy cluster just under the root.
+template <int NC, typename T> __device__ inline uint32_t active_slots(const T (&v)[NC], const T (&w)[NC]) {
+    double vd[NC], wd[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        vd[k] = (double)v[k];
+        wd[k] = (double)w[k];
+    }
+    uint32_t used = 0;
+    (void)local_solve<NC, double>(vd, wd, used);
+    return used;
+}
+
 // Local solve used by the update kernel.  FP64: exactly local_solve (the reference's order of
 // operations, so converged values reproduce the reference's FP64 values).  FP32: the same
-// candidates in the same order, but the test "next v below the current root" is evaluated
-// without the root as f(v) < 1, f(t) = sum_accepted w (t - v_i)^2 (f is increasing above the
-// accepted values and f(root) = 1), so the square root and division run once per solve.
+// candidates in the same increasing order and the same root, but written relative to the
+// largest accepted value p instead of the smallest, which removes every cancellation:
+//   A = sum w_i, nB = sum w_i (p - v_i) >= 0, C = sum w_i (p - v_i)^2 < 1 over the accepted v_i;
+//   the next value v (s = v - p >= 0) is below the current root iff
+//     f = sum w_i (v - v_i)^2 = A s^2 + 2 nB s + C < 1,
+//   and accepting it moves the pivot to v: C <- f, nB <- nB + A s, A <- A + w, p <- v;
+//   the root is p + d with A d^2 + 2 nB d - (1 - C) = 0, i.e.
+//     d = (1 - C) / (sqrt(nB^2 + A (1 - C)) + nB).
+// Every sum adds terms of one sign.  The form relative to the smallest value m (the reference's)
+// computes b^2 - a (c - 1) as the difference of two terms that are both about a^2 (p - m)^2: with
+// a candidate of weight 4.8 at 0.005 below five of total weight 11575 at 0.4618 (Elastica2
+// 192 x 192 x 64 beside the seed, across the periodic theta axis), FP32 loses 3e-5 of the value.
+// One hardware square root and reciprocal per solve (1 ulp each).  Explicit fused multiply-adds:
+// FP32 only, never on the FP64 path.
 template <int NC, typename T> __device__ inline T local_solve_update(const T (&v)[NC], const T (&w)[NC]) {
     uint32_t used;
     return local_solve<NC, T>(v, w, used);
+}
+__device__ inline float root_from_pivot(float p, float A, float nB, float C) {
+    const float e = 1.f - C;
+    const float d2 = __builtin_fmaf(A, e, nB * nB);
+    return p + e * __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(d2) + nB);
 }
 template <int NC> __device__ inline float local_solve_update(const float (&v0)[NC], const float (&w0)[NC]) {
     const float inf = __builtin_huge_valf();
@@ -329,30 +362,21 @@ template <int NC> __device__ inline float local_solve_update(const float (&v0)[N
             w[j] = sw ? b0 : b1;
         }
         if (!(v[0] < inf)) return inf;
-        // Accept sorted neighbours while the quadratic through the accepted ones stays below 1
-        // at the next value: f(t) = a t^2 - 2 b t + c < 1 (t relative to the smallest value m),
-        // the square-root-free form of "next value below the current root".  Explicit fused
-        // multiply-adds: FP32 only, never on the FP64 path.
-        const float m = v[0];
-        float a = w[0], b = 0.f, c = 0.f;
+        float p = v[0], A = w[0], nB = 0.f, C = 0.f;
         bool go = true;
 #pragma unroll
         for (int k = 1; k < NC; ++k) {
-            const float t = v[k] - m;
-            go = go && (__builtin_fmaf(t, __builtin_fmaf(a, t, -2.f * b), c) < 1.f);
-            const float wk = go ? w[k] : 0.f;
-            const float tk = go ? t : 0.f;
-            const float wt = wk * tk;
-            a += wk;
-            b += wt;
-            c = __builtin_fmaf(wt, tk, c);
+            const float s = v[k] - p;  // >= 0; +inf for a missing neighbour (then f = +inf)
+            const float f = __builtin_fmaf(A * s, s, __builtin_fmaf(2.f * nB, s, C));
+            go = go && f < 1.f;
+            C = go ? f : C;
+            nB = go ? __builtin_fmaf(A, s, nB) : nB;
+            A = go ? A + w[k] : A;
+            p = go ? v[k] : p;
         }
-        // hardware square root and reciprocal (1 ulp each): they perturb only the increment
-        // over the smallest neighbour value, far below the FP32 rounding of u itself
-        const float d2 = __builtin_fmaf(-a, c - 1.f, b * b);
-        return m + (b + __builtin_amdgcn_sqrtf(d2 > 0.f ? d2 : 0.f)) * __builtin_amdgcn_rcpf(a);
+        return A > 0.f ? root_from_pivot(p, A, nB, C) : inf;  // no weight accepted: +inf
     } else {
-        float m = 0.f, a = 0.f, b = 0.f, c = 0.f;
+        float p = 0.f, A = 0.f, nB = 0.f, C = 0.f;
         uint32_t sel = 0;
         bool any = false;
         for (int step = 0; step < NC; ++step) {
@@ -367,22 +391,21 @@ template <int NC> __device__ inline float local_solve_update(const float (&v0)[N
             }
             if (bk < 0) break;
             if (step == 0) {
-                m = bv;
+                p = bv;
+                A = bw;
             } else {
-                const float t = bv - m;
-                if (!(t * (a * t - 2.f * b) + c < 1.f)) break;
+                const float s = bv - p;
+                const float f = __builtin_fmaf(A * s, s, __builtin_fmaf(2.f * nB, s, C));
+                if (!(f < 1.f)) break;
+                C = f;
+                nB = __builtin_fmaf(A, s, nB);
+                A += bw;
+                p = bv;
             }
-            const float vr = bv - m;
-            const float wv = bw * vr;
-            a += bw;
-            b += wv;
-            c += wv * vr;
             sel |= 1u << bk;
             any = true;
         }
-        if (!any) return inf;
-        const float d2 = b * b - a * (c - 1.f);
-        return m + (b + sqrtf(d2 > 0.f ? d2 : 0.f)) / a;
+        return any && A > 0.f ? root_from_pivot(p, A, nB, C) : inf;
     }
 }
 

@@ -240,8 +240,7 @@ __device__ int discrete_flow(const Geom &g, const StencilData<T> &sd, const T *u
         vv[k] = v < uu ? v : dev_inf<T>();
         ww[k] = (T)L.s_w[k];
     }
-    uint32_t used = 0;
-    (void)local_solve<NC, T>(vv, ww, used);
+    const uint32_t used = active_slots<NC, T>(vv, ww);
     if (!used) return 0;
     double m = __builtin_huge_val();
     for (int k = 0; k < NC; ++k)
@@ -346,7 +345,7 @@ __device__ void fill_flow(const Geom &g, const StencilData<T> &sd, const T *u, c
             vv[k] = v < uu ? v : inf;
             ww[k] = w[k];
         }
-        (void)local_solve<NC, T>(vv, ww, used);
+        used = active_slots<NC, T>(vv, ww);
     }
     double value = 0;
     if (used) {

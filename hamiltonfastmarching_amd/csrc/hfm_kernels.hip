@@ -583,7 +583,7 @@ __global__ __launch_bounds__(256) void k_flow(Geom g, StencilData<T> sd, const T
         // differences with v < u, in the local solver's order
         T vv[NC];
         for (int k = 0; k < NC; ++k) vv[k] = v[k] < uu ? v[k] : inf;
-        (void)local_solve<NC, T>(vv, st.w, used);
+        used = active_slots<NC, T>(vv, st.w);
     }
     double f[3] = {0., 0., 0.};
     double bits = 0.;

@@ -346,10 +346,13 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
     // 384 blocks, 3.4 -> 3.0 ms at 256 x 256 x 64.
     const int grid_default = p->model == MODEL_RSF2 ? 384 : 640;
     a.grid = (unsigned)std::min<long long>((p->g.ntiles + 3) / 4, p->fim_grid > 0 ? p->fim_grid : grid_default);
-    // At most 4 sweeps per visit on Riemann3 (128^3: 2.73 against 2.79 ms at 8; a tile still
-    // changing after 4 sweeps goes back to the schedule); 8 elsewhere (Riemann2 256^2 FP32: 0.90
-    // ms at 8 against 1.07 at 4, the self-listing rounds needing more rounds at 4).
-    a.max_sweeps = p->max_sweeps > 0 ? p->max_sweeps : p->model == MODEL_RIEMANN3 ? 4 : 8;
+    // At most 4 sweeps per visit on Riemann3 and Elastica2 (128^3: 2.73 against 2.79 ms at 8;
+    // Elastica2 192 x 192 x 64: 23.9 against 28.1 ms; a tile still changing after 4 sweeps goes
+    // back to the schedule); 8 on Riemann2 (256^2 FP32: 0.90 ms at 8 against 1.07 at 4, the
+    // self-listing rounds needing more rounds at 4) and ReedsSheppForward2 (3.08 against 3.21 ms).
+    a.max_sweeps = p->max_sweeps > 0                                           ? p->max_sweeps
+                   : (p->model == MODEL_RIEMANN3 || p->model == MODEL_ELASTICA2) ? 4
+                                                                                : 8;
     // Rounds that list their own tiles save the listing kernel and a kernel boundary per round,
     // but each block then visits only the tiles it owns, and a mark is taken one round later than
     // with the listing kernel (the keys are double-buffered).  That pays on small 2-D grids

@@ -232,8 +232,8 @@ int hfm_plan_set_round_target(hfm_plan *plan, int tiles);
 int hfm_plan_set_grid(hfm_plan *plan, int blocks);
 
 /* Jacobi sweeps a tile visit may run before an unconverged tile is handed back to the
- * schedule: 4, 8, 16 or 32, or 0 for the default (4 for Riemann3, 8 for the other models); any
- * other value is an error.  A tuning knob: it changes the work per visit and the number of visits, never the
+ * schedule: 4, 8, 16 or 32, or 0 for the default (4 for Riemann3 and Elastica2, 8 for Riemann2
+ * and ReedsSheppForward2); any other value is an error.  A tuning knob: it changes the work per visit and the number of visits, never the
  * fixed point. */
 int hfm_plan_set_max_sweeps(hfm_plan *plan, int sweeps);
 

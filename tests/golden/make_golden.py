@@ -16,7 +16,8 @@ Fixtures (all row-major):
   elastica2_24x24x16.npz  values, activeOffsets, costSensitivity_0, seedSensitivity_0, shared stencils
   geodesics.npz      geodesicPoints / geodesicLengths from `tips` (discrete solver), four models
   forward_variation.npz  valueVariation from costVariation / seedValueVariation, four models
-  riemann3_128.npz   digests of values at the benchmark size + 4096 seeded sample points
+  orderings.npz      outputs under every arrayOrdering of non-square Riemann2 / Riemann3 / RSF2 problems
+  riemann3_128.npz   digests of values and Selling offsets at the benchmark size + 4096 seeded samples
 Inputs are regenerated by hamiltonfastmarching_amd.synthetic; the SHA-256 of each generated
 input is stored so that a test can tell a generator change from a numerical one.
 """
@@ -32,6 +33,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))  # tests/: pack_utils
 
 import oracle  # noqa: E402
 from hamiltonfastmarching_amd import synthetic as syn  # noqa: E402
@@ -430,6 +432,37 @@ def riemann3_offsets_all(p, chunks=4):
     return off
 
 
+ORDERING_OUTPUTS = ("values", "activeNeighs", "geodesicFlow", "activeOffsets", "geodesicPoints", "geodesicLengths",
+                    "costSensitivity_0")
+
+
+def ordering_cases():
+    """The reference's outputs of each problem under every arrayOrdering, its field input
+    (metric or cost) given in that ordering.  Un-permuted, every ordering's values equal the
+    RowMajor values bit for bit, which checks the permutation recipe (tests/pack_utils.py)."""
+    from pack_utils import ORDERINGS, from_ordering, ordering_params, to_ordering
+    out = {}
+    for name, base in ordering_params().items():
+        d = len(base["dims"])
+        fkey = "metric" if "metric" in base else "cost"
+        rm = None
+        for o in ORDERINGS:
+            q = dict(base, arrayOrdering=o)
+            q[fkey] = to_ordering(base[fkey], o, d)
+            r = oracle.run_reference(q)
+            assert r["retcode"] == 0, r["log"]
+            v = from_ordering(np.asarray(r["values"]), o, d)
+            if rm is None:
+                rm = v
+            assert np.array_equal(v, rm), (name, o)
+            for k in ORDERING_OUTPUTS:
+                if k in r:
+                    out[f"{name}_{o}_{k}"] = np.asarray(r[k])
+            print(name, o, {k: np.asarray(r[k]).shape for k in ORDERING_OUTPUTS if k in r})
+    np.savez_compressed(os.path.join(HERE, "orderings.npz"), **out)
+    print("orderings ok")
+
+
 def big_case():
     p = syn.riemann3(128, seed=0)
     r = oracle.run_reference(p)
@@ -453,7 +486,8 @@ def main():
     only = [a for a in sys.argv[1:] if not a.startswith("--")]
     if only:  # regenerate the named fixture groups only
         groups = {"geodesics": geodesic_cases, "forward": forward_cases, "walls": wall_cases, "seeds": seed_cases,
-                  "bypass": bypass_cases, "stencils": stencil_cases, "big": big_case}
+                  "bypass": bypass_cases, "stencils": stencil_cases, "big": big_case,
+                  "orderings": ordering_cases}
         for a in only:
             groups[a]()
         return
@@ -468,6 +502,7 @@ def main():
     seed_cases()
     bypass_cases()
     stencil_cases()
+    ordering_cases()
     if "--big" in sys.argv:
         big_case()
 
