@@ -70,6 +70,8 @@ SIGNATURES = [
     ("hfm_plan_set_cost", c_int, [c_void_p, c_void_p, c_int, c_void_p]),
     ("hfm_plan_solve", c_int, [c_void_p, c_int, P(c_double), P(c_double), c_void_p]),
     ("hfm_plan_values", c_int, [c_void_p, c_void_p, c_void_p]),
+    ("hfm_riemann_solve_noalloc", c_int, [ctypes.c_char_p, P(c_int64), P(c_double), P(c_double), c_void_p, c_int,
+                                          c_int, c_int, P(c_double), P(c_double), c_void_p, c_void_p]),
     ("hfm_plan_load_values", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     ("hfm_plan_flow", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     ("hfm_plan_active_offsets", c_int, [c_void_p, c_void_p, c_void_p]),

@@ -118,6 +118,10 @@ struct hfm_plan {
     int8_t *wdist[2] = {nullptr, nullptr};
     bool vis_dirty = true;
 
+    // Buffers carved out of the static workspace of hfm_riemann_solve_noalloc: no event, graph,
+    // pinned buffer or device allocation is made, and a buffer that would have to grow (tile
+    // graph, seeds) is an error instead.
+    bool workspace = false;
     bool stencil_ready = false;
     bool solved = false;
     bool timing = false;

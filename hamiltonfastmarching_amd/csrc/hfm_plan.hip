@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -145,6 +146,9 @@ template <typename T> void build_deps(hfm_plan *p, hipStream_t s) {
     HIPCHECK(hipStreamSynchronize(s));
     total = *p->h_count;
     if (total > p->deps_cap) {
+        if (p->workspace)
+            throw HipFail{"tile dependency graph of " + std::to_string(total) + " entries exceeds the " +
+                          std::to_string(p->deps_cap) + " of the allocation-free workspace"};
         dfree(p->deps);
         p->deps = nullptr;
         p->deps_cap = std::max<long long>(total, 1);
@@ -303,10 +307,13 @@ void build_visibility(hfm_plan *p, hipStream_t s) {
 }
 
 template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
-    hipEvent_t e0, e1;
-    HIPCHECK(hipEventCreate(&e0));
-    HIPCHECK(hipEventCreate(&e1));
-    HIPCHECK(hipEventRecord(e0, s));
+    // solve time from events, except in the allocation-free workspace (creates no event)
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!p->workspace) {
+        HIPCHECK(hipEventCreate(&e0));
+        HIPCHECK(hipEventCreate(&e1));
+        HIPCHECK(hipEventRecord(e0, s));
+    }
     const int nseeds = (int)p->last_seeds.size();
     HIPCHECK(launch_init<T>(p->g, static_cast<T *>(p->u), p->fixedmask, p->keys, p->tmax, p->np, p->seed_p,
                             p->seed_v, nseeds, s, 2));
@@ -434,7 +441,7 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
             HIPCHECK(hipMemcpyAsync(p->h_count, &p->counts[(k0 + batch - 1) & 1], sizeof(int), hipMemcpyDeviceToHost,
                                     st));
     };
-    const bool graph = p->use_graph && !p->timing;
+    const bool graph = p->use_graph && !p->timing && !p->workspace;
     if (graph) {
         // launch arguments of the batch: re-capture when any of them changed
         std::vector<char> key(sizeof(a) + sizeof(target));
@@ -473,12 +480,12 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
         if (k > max_rounds) throw HipFail{"block fast-iterative method did not converge"};
     }
     p->prev_rounds = k;
-    HIPCHECK(hipEventRecord(e1, s));
+    if (e1) HIPCHECK(hipEventRecord(e1, s));
     HIPCHECK(hipMemcpyAsync(p->h_stats, p->stats, 2 * PEND_SLOTS * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                             s));
     HIPCHECK(hipStreamSynchronize(s));
     float ms = 0;
-    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+    if (e0) HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
     p->st.solve_ms = ms;
     p->st.iterations = k;
     p->st.tile_visits = 0;
@@ -500,8 +507,8 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
     }
     p->st.fim_ms = p->timing ? fim : 0.;
     p->st.fim_launches = p->timing ? (int64_t)(tev.size() / 2) : 0;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
 }
 
 template <typename T> void set_metric_impl(hfm_plan *p, const double *d_metric, int is_field, int dual, hipStream_t s) {
@@ -510,7 +517,7 @@ template <typename T> void set_metric_impl(hfm_plan *p, const double *d_metric, 
     for (int i = 0; i < D; ++i)
         for (int j = 0; j <= i; ++j) ih.v[i * (i + 1) / 2 + j] = 1. / (p->h[i] * p->h[j]);
     p->stencil_ev_pending = false;
-    HIPCHECK(hipEventRecord(p->stencil_ev[0], s));
+    if (!p->workspace) HIPCHECK(hipEventRecord(p->stencil_ev[0], s));
     int init[2] = {0, 0x7fffffff};
     HIPCHECK(hipMemcpyAsync(p->status, init, sizeof(init), hipMemcpyHostToDevice, s));
     HIPCHECK(hipMemsetAsync(p->red, 0, RED_SLOTS * sizeof(RedSlot), s));
@@ -533,8 +540,63 @@ template <typename T> void set_metric_impl(hfm_plan *p, const double *d_metric, 
                       std::to_string(hs[1]) + ")"};
     p->max_stencil_width = tot.l1max;
     // the second tile-graph pass is still running: the phase time is read from the events later
-    HIPCHECK(hipEventRecord(p->stencil_ev[1], s));
-    p->stencil_ev_pending = true;
+    if (!p->workspace) {
+        HIPCHECK(hipEventRecord(p->stencil_ev[1], s));
+        p->stencil_ev_pending = true;
+    }
+}
+
+// Model, grid geometry, tiling, grid scales and origin of a plan (no device memory).
+void setup_geometry(hfm_plan *p, int m, int precision, const int64_t *dims, const double *gridScales,
+                    const double *origin, double xi, double eps) {
+    p->model = m;
+    p->precision = precision;
+    p->shared = (m == MODEL_RSF2 || m == MODEL_ELASTICA2);
+    p->ndiff = model_ndiff(m);
+    const int dim = (m == MODEL_RIEMANN2) ? 2 : 3;
+    Geom &g = p->g;
+    g.dim = dim;
+    g.n[2] = 1;
+    p->N = 1;
+    for (int i = 0; i < dim; ++i) {
+        if (dims[i] <= 0) throw HipFail{"BaseGrid error : domain size must be positive"};
+        if (dims[i] > (1 << 24)) throw HipFail{"grid extent too large"};
+        g.n[i] = (int)dims[i];
+        p->N *= dims[i];
+    }
+    if (dim == 2) {
+        g.sh[0] = 3; g.sh[1] = 3; g.sh[2] = 0;
+    } else {
+        g.sh[0] = 2; g.sh[1] = 2; g.sh[2] = 2;
+    }
+    long long nt = 1;
+    for (int i = 0; i < 3; ++i) {
+        g.nt[i] = (g.n[i] + (1 << g.sh[i]) - 1) >> g.sh[i];
+        nt *= g.nt[i];
+    }
+    if (nt > (1LL << 26)) throw HipFail{"grid too large"};
+    g.ntiles = (int)nt;
+    g.periodic = p->shared ? 4 : 0;
+    p->np = nt * 64;
+    if (p->shared) {
+        if (!(gridScales[0] > 0)) throw HipFail{"Parametrization error : gridscale should be positive"};
+        p->h[0] = p->h[1] = gridScales[0];
+        p->dependScale = 2 * kPi / g.n[2];
+        p->h[2] = p->dependScale;
+        p->origin[0] = origin ? origin[0] : 0.;
+        p->origin[1] = origin ? origin[1] : 0.;
+        p->origin[2] = -p->dependScale / 2.;
+        p->xi = xi;
+        p->eps = eps;
+    } else {
+        for (int i = 0; i < dim; ++i) {
+            if (!(gridScales[i] > 0)) throw HipFail{"Parametrization error : gridscales should be positive"};
+            p->h[i] = gridScales[i];
+            p->origin[i] = origin ? origin[i] : 0.;
+        }
+    }
+    p->st.n_points = p->N;
+    p->st.n_tiles = nt;
 }
 
 }  // namespace
@@ -561,52 +623,9 @@ int hfm_plan_create(hfm_plan **out, const char *model, const int64_t *dims, cons
     }
     hfm_plan *p = new hfm_plan;
     try {
-        p->model = m;
-        p->precision = precision;
-        p->shared = (m == MODEL_RSF2 || m == MODEL_ELASTICA2);
-        p->ndiff = model_ndiff(m);
-        const int dim = (m == MODEL_RIEMANN2) ? 2 : 3;
-        Geom &g = p->g;
-        g.dim = dim;
-        g.n[2] = 1;
-        p->N = 1;
-        for (int i = 0; i < dim; ++i) {
-            if (dims[i] <= 0) throw HipFail{"BaseGrid error : domain size must be positive"};
-            if (dims[i] > (1 << 24)) throw HipFail{"grid extent too large"};
-            g.n[i] = (int)dims[i];
-            p->N *= dims[i];
-        }
-        if (dim == 2) {
-            g.sh[0] = 3; g.sh[1] = 3; g.sh[2] = 0;
-        } else {
-            g.sh[0] = 2; g.sh[1] = 2; g.sh[2] = 2;
-        }
-        long long nt = 1;
-        for (int i = 0; i < 3; ++i) {
-            g.nt[i] = (g.n[i] + (1 << g.sh[i]) - 1) >> g.sh[i];
-            nt *= g.nt[i];
-        }
-        if (nt > (1LL << 26)) throw HipFail{"grid too large"};
-        g.ntiles = (int)nt;
-        g.periodic = p->shared ? 4 : 0;
-        p->np = nt * 64;
-        if (p->shared) {
-            if (!(gridScales[0] > 0)) throw HipFail{"Parametrization error : gridscale should be positive"};
-            p->h[0] = p->h[1] = gridScales[0];
-            p->dependScale = 2 * kPi / g.n[2];
-            p->h[2] = p->dependScale;
-            p->origin[0] = origin ? origin[0] : 0.;
-            p->origin[1] = origin ? origin[1] : 0.;
-            p->origin[2] = -p->dependScale / 2.;
-            p->xi = xi;
-            p->eps = eps;
-        } else {
-            for (int i = 0; i < dim; ++i) {
-                if (!(gridScales[i] > 0)) throw HipFail{"Parametrization error : gridscales should be positive"};
-                p->h[i] = gridScales[i];
-                p->origin[i] = origin ? origin[i] : 0.;
-            }
-        }
+        setup_geometry(p, m, precision, dims, gridScales, origin, xi, eps);
+        const Geom &g = p->g;
+        const long long nt = g.ntiles;
         const size_t ts = precision == 32 ? 4 : 8;
         p->u = dalloc<char>(p->np * ts);
         if (!p->shared) {
@@ -767,6 +786,9 @@ int hfm_plan_solve(hfm_plan *p, int nseeds, const double *seeds, const double *s
         place_seeds(p, nseeds, seeds, seedValues, tp, vals, &inputs);
         if (tp.empty()) throw HipFail{nseeds > 0 ? "Error : seeds incorrectly set" : "No seeds to run fast marching solver"};
         if ((int)tp.size() > p->seed_cap) {
+            if (p->workspace)
+                throw HipFail{std::to_string(tp.size()) + " seed points exceed the " + std::to_string(p->seed_cap) +
+                              " of the allocation-free workspace"};
             dfree(p->seed_p);
             dfree(p->seed_v);
             p->seed_p = nullptr;
@@ -1207,3 +1229,134 @@ int hfm_plan_forward_variation(hfm_plan *p, int nvar, const double *d_cost_varia
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// Allocation-free Riemann solve.  Every buffer of an FP32 Riemann plan is carved, per call, out
+// of one static device array (`scratch`, HFM_NOALLOC_WORKSPACE_BYTES per GPU, held from the
+// library's load to the end of the process).  In host code the name `scratch` is a host-side
+// stand-in for the symbol, not a device address: the address is taken with
+// hipGetSymbolAddress once per device, with that device current, and only that address is used.
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+__device__ __align__(256) char scratch[HFM_NOALLOC_WORKSPACE_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+constexpr int NOALLOC_MAX_SEEDS = 4096;
+std::mutex g_ws_mutex[NOALLOC_MAX_DEVICES];  // one call at a time per device
+void *g_ws_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of the workspace on the current device (called with g_ws_mutex[dev] held).
+char *workspace_address(int dev) {
+    if (!g_ws_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            throw HipFail{std::string("hipGetSymbolAddress of the allocation-free workspace failed: ") +
+                          hipGetErrorString(e)};
+        g_ws_ptr[dev] = ptr;
+    }
+    return static_cast<char *>(g_ws_ptr[dev]);
+}
+
+// Consecutive 256-byte-aligned pieces of the workspace.
+struct Carver {
+    char *base;
+    size_t off = 0;
+    template <typename T> T *take(size_t n) {
+        off = (off + 255) & ~size_t(255);
+        T *r = reinterpret_cast<T *>(base + off);
+        off += (n ? n : 1) * sizeof(T);
+        if (off > HFM_NOALLOC_WORKSPACE_BYTES) throw HipFail{"allocation-free workspace exhausted"};
+        return r;
+    }
+};
+
+}  // namespace
+
+extern "C" int hfm_riemann_solve_noalloc(const char *model, const int64_t *dims, const double *gridScales,
+                                         const double *origin, const double *d_metric, int metric_is_field, int dual,
+                                         int nseeds, const double *seeds, const double *seedValues,
+                                         double *d_values, void *stream) {
+    if (!model || !dims || !gridScales || !d_metric || !d_values || (nseeds > 0 && !seeds)) {
+        set_error("hfm_riemann_solve_noalloc: null argument");
+        return 1;
+    }
+    const int m = model_from_name(model);
+    if (m != MODEL_RIEMANN2 && m != MODEL_RIEMANN3) {
+        set_error(std::string("hfm_riemann_solve_noalloc: model must be Riemann2 or Riemann3 (got ") + model + ")");
+        return 1;
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+        set_error("hfm_riemann_solve_noalloc: no usable current device");
+        return 1;
+    }
+    std::lock_guard<std::mutex> lock(g_ws_mutex[dev]);
+    hfm_plan plan;  // host bookkeeping only; its device buffers point into the workspace
+    hfm_plan *p = &plan;
+    // host mirrors of the read-backs (pageable: no pinned allocation)
+    struct HostMirror {
+        int count[4];
+        RedSlot red[RED_SLOTS];
+        unsigned long long stats[2 * PEND_SLOTS];
+    };
+    static thread_local HostMirror hm;
+    try {
+        setup_geometry(p, m, 32, dims, gridScales, origin, 1.0, 0.1);
+        const long long nt = p->g.ntiles;
+        if (nt > HFM_NOALLOC_MAX_TILES)
+            throw HipFail{"hfm_riemann_solve_noalloc: the grid has " + std::to_string(nt) +
+                          " tiles of 64 points, more than the " + std::to_string(HFM_NOALLOC_MAX_TILES) +
+                          " the allocation-free workspace holds (use hfm_plan_create)"};
+        if (nseeds > NOALLOC_MAX_SEEDS)
+            throw HipFail{"hfm_riemann_solve_noalloc: at most " + std::to_string(NOALLOC_MAX_SEEDS) + " seeds"};
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        Carver cv{workspace_address(dev)};
+        p->workspace = true;
+        p->use_graph = false;
+        p->u = cv.take<float>(p->np);
+        p->offs = cv.take<uint32_t>(p->np * p->ndiff);
+        p->w = cv.take<float>(p->np * p->ndiff);
+        p->fixedmask = cv.take<unsigned long long>(nt);
+        p->dep_cnt = cv.take<int>(nt);
+        p->dep_start = cv.take<int>(nt + 1);
+        p->dep_fill = cv.take<int>(nt);
+        p->lists[0] = cv.take<int>(nt);
+        p->counts = cv.take<int>(4);
+        p->dep_src = cv.take<int>(nt * 64);
+        p->dep_nsrc = cv.take<int>(nt);
+        p->scan_work = cv.take<int>(scan_work_ints(nt));
+        p->keys = cv.take<unsigned>(2 * nt);
+        p->tmax = cv.take<unsigned>(nt);
+        p->nextmin = cv.take<unsigned>(4 * PEND_SLOTS);
+        p->bounds = cv.take<float>(5);
+        p->red = cv.take<RedSlot>(RED_SLOTS);
+        p->status = cv.take<int>(4);
+        p->stats = cv.take<unsigned long long>(2 * PEND_SLOTS);
+        p->seed_p = cv.take<long long>(NOALLOC_MAX_SEEDS);
+        p->seed_v = cv.take<double>(NOALLOC_MAX_SEEDS);
+        p->seed_cap = NOALLOC_MAX_SEEDS;
+        // the tile graph gets the rest
+        cv.off = (cv.off + 255) & ~size_t(255);
+        p->deps = reinterpret_cast<int *>(cv.base + cv.off);
+        p->deps_cap = (long long)((HFM_NOALLOC_WORKSPACE_BYTES - cv.off) / sizeof(int));
+        p->h_count = hm.count;
+        p->h_red = hm.red;
+        p->h_stats = hm.stats;
+        set_metric_impl<float>(p, d_metric, metric_is_field, dual, s);
+        p->stencil_ready = true;
+    } catch (const HipFail &e) {
+        set_error(e.msg);
+        return 1;
+    }
+    if (hfm_plan_solve(p, nseeds, seeds, seedValues, stream)) return 1;
+    if (hfm_plan_values(p, d_values, stream)) return 1;
+    // the next call on this device may reuse the workspace as soon as this one returns
+    const hipError_t e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) {
+        set_error(std::string("hfm_riemann_solve_noalloc: ") + hipGetErrorString(e));
+        return 1;
+    }
+    return 0;
+}

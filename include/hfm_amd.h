@@ -152,6 +152,25 @@ int hfm_plan_load_values(hfm_plan *plan, const double *d_values, const double *d
 /* Arrival times of the last solve, row-major FP64 (device) -- the 'values' output. */
 int hfm_plan_values(hfm_plan *plan, double *d_values, void *stream);
 
+/* Allocation-free Riemann solve.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory -- tight serving loops, code around HIP graph capture,
+ * harnesses that forbid hipMalloc.  It computes what hfm_plan_create (precision 32) +
+ * hfm_plan_set_metric + hfm_plan_solve + hfm_plan_values compute, for Riemann2 / Riemann3,
+ * from the device metric (row-major FP64, as hfm_plan_set_metric) into d_values (row-major FP64
+ * on the device), and calls no hipMalloc, hipFree, hipHostMalloc, event or graph creation:
+ * every working buffer (stencils, tile graph, values, keys, lists, counters) lives in one static
+ * device workspace of HFM_NOALLOC_WORKSPACE_BYTES, reserved on each GPU the library is loaded on
+ * until the process ends.  Limits (an error, never an allocation, beyond them): at most
+ * HFM_NOALLOC_MAX_TILES tiles of 64 points (1024 x 1024 in 2-D, 100^3 in 3-D), 4096 seeds, and a
+ * tile dependency graph that fits the rest of the workspace.  Calls on one GPU share its
+ * workspace: the library serializes them (one call at a time per device, each returning after
+ * its work on `stream` has completed), so they never overlap inside one process. */
+#define HFM_NOALLOC_MAX_TILES 16384
+#define HFM_NOALLOC_WORKSPACE_BYTES (72ull << 20)
+int hfm_riemann_solve_noalloc(const char *model, const int64_t *dims, const double *gridScales, const double *origin,
+                              const double *d_metric, int metric_is_field, int dual, int nseeds, const double *seeds,
+                              const double *seedValues, double *d_values, void *stream);
+
 /* geodesicFlow (N x dim, FP64, device, may be NULL) and activeNeighs (N, FP64 holding the
  * bitset like the reference's export, device, may be NULL) (HFMInterface.hxx:536-553). */
 int hfm_plan_flow(hfm_plan *plan, double *d_flow, double *d_active, void *stream);
