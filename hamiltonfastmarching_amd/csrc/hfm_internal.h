@@ -82,8 +82,8 @@ struct hfm_plan {
     bool stencil_ev_pending = false;
     double tau = 0.;               // mean one-step time of the current stencils
     double delta_factor = 8.;      // initial band width in units of tau (<= 0: unordered)
-    int fim_grid = 0;              // update-kernel blocks of 4 waves (0: 640)
-    int round_target = 0;          // tiles per round the band width is steered to (0: 1024 for 2-D and small grids, else 128)
+    int fim_grid = 0;              // update-kernel blocks of 4 waves (0: 960, or 1280 for listed 2-D grids)
+    int round_target = 0;          // tiles per round the band width is steered to (0: 64 for 3-D grids of >= 4096 tiles, else 1024)
     int max_sweeps = 0;            // sweeps per tile visit (4, 8, 16 or 32; 0: 4 for Riemann3 and Elastica2, else 8)
     int *status = nullptr;  // stencil status: worst Selling error, smallest failing row-major index
     unsigned long long *stats = nullptr;

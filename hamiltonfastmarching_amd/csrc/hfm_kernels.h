@@ -21,12 +21,19 @@ struct FlowScale {
 
 constexpr unsigned KEY_NONE = 0xffffffffu;
 constexpr int PEND_SLOTS = 32;  // spread of the per-round minimum-key and statistics counters
+// The single-solve kernels keep each slot on its own 128-byte line (PEND_STRIDE words for the
+// pending minima, STAT_STRIDE 64-bit words for the statistics): atomics to different words of
+// one line serialise like atomics to one word.
+constexpr int PEND_STRIDE = 32;
+constexpr int STAT_STRIDE = 16;
+constexpr int PEND_WORDS = PEND_SLOTS * PEND_STRIDE;  // words of one round's pending minima
+constexpr int STAT_WORDS = PEND_SLOTS * STAT_STRIDE;  // 64-bit words of the statistics
 
 // One scheduling round (see k_schedule): reads the pending minimum and the previous bound,
 // writes this round's bound, lists the tiles it takes, accumulates the keys it leaves.
 struct SchedRound {
-    const unsigned *pend_in;   // PEND_SLOTS words
-    unsigned *pend_out;        // PEND_SLOTS words
+    const unsigned *pend_in;   // PEND_SLOTS words, PEND_STRIDE apart
+    unsigned *pend_out;        // PEND_SLOTS words, PEND_STRIDE apart
     const float *bound_in;
     float *bound_out;
     int *list;
@@ -53,7 +60,7 @@ template <typename T> struct FimArgs {
     unsigned *tmax;
     unsigned *pend_out;
     unsigned *pend_reset;
-    unsigned long long *stats;  // PEND_SLOTS pairs: (tiles processed, sweeps summed)
+    unsigned long long *stats;  // PEND_SLOTS pairs (tiles processed, sweeps summed), STAT_STRIDE apart
     unsigned grid;
     int max_sweeps;  // sweeps per tile visit before the tile is handed back to the schedule: 4, 8, 16 or 32
     int fused;       // 1: k_fim_sched lists its own tiles with the round parameters in sr (no list)

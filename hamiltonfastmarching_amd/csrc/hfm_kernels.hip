@@ -358,7 +358,7 @@ __global__ __launch_bounds__(SCHED_BLOCK) void k_schedule(int ntiles, unsigned *
     const float d0 = *sr.delta0;
     float delta = d0;
     if (delta > 0.f) delta = adapt_delta(*sr.delta_in, *sr.prev_count, sr.target, 0.5f * d0, 64.f * d0);
-    const float B = band_bound(key_value(read_pend(sr.pend_in)), *sr.bound_in, delta);
+    const float B = band_bound(key_value(read_pend(sr.pend_in, PEND_STRIDE)), *sr.bound_in, delta);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *sr.bound_out = B;
         *sr.delta_out = delta;
@@ -366,7 +366,8 @@ __global__ __launch_bounds__(SCHED_BLOCK) void k_schedule(int ntiles, unsigned *
     const bool take = band_take(k, B);
     const unsigned left = (!take && k != KEY_NONE) ? k : KEY_NONE;
     if (take) keys[t] = KEY_NONE;
-    const int pos = block_take(take, left, sr.count, nullptr, &sr.pend_out[blockIdx.x & (PEND_SLOTS - 1)]);
+    const int pos =
+        block_take(take, left, sr.count, nullptr, &sr.pend_out[(blockIdx.x & (PEND_SLOTS - 1)) * PEND_STRIDE]);
     if (take) sr.list[pos] = t;
 }
 
@@ -391,7 +392,7 @@ __global__ __launch_bounds__(256) void k_fim(Geom g, StencilData<T> sd, T *__res
     const int n = *count;
     if (blockIdx.x == 0) {
         if (threadIdx.x == 0) *count_reset = 0;
-        if (threadIdx.x < PEND_SLOTS) pend_reset[threadIdx.x] = KEY_NONE;
+        if (threadIdx.x < PEND_SLOTS) pend_reset[threadIdx.x * PEND_STRIDE] = KEY_NONE;
     }
     const TileCtx<T> ctx{g, sd, u, fixedmask, dep_start, deps, keys, tmax};
     unsigned long long my_sweeps = 0, my_visits = 0;
@@ -404,12 +405,7 @@ __global__ __launch_bounds__(256) void k_fim(Geom g, StencilData<T> sd, T *__res
         my_sweeps += fim_tile<ModelT, T, MAXSWEEP>(ctx, tile, lane, su[wv], wmin);
         my_visits += 1;
     }
-    const int slot = ((blockIdx.x * 4 + wv) & (PEND_SLOTS - 1));
-    if (lane == 0 && wmin != KEY_NONE) atomicMin(&pend_out[slot], wmin);
-    if (lane == 0 && my_visits) {
-        atomicAdd(&stats[2 * slot], my_visits);
-        atomicAdd(&stats[2 * slot + 1], my_sweeps);
-    }
+    block_flush(wmin, my_visits, my_sweeps, pend_out, stats);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -454,14 +450,14 @@ __global__ __launch_bounds__(256) void k_fim_sched(Geom g, StencilData<T> sd, T 
     const float d0 = *sr.delta0;
     float delta = d0;
     if (delta > 0.f) delta = adapt_delta(*sr.delta_in, *sr.prev_count, sr.target, 0.5f * d0, 64.f * d0);
-    const float B = band_bound(key_value(read_pend(sr.pend_in)), *sr.bound_in, delta);
+    const float B = band_bound(key_value(read_pend(sr.pend_in, PEND_STRIDE)), *sr.bound_in, delta);
     if (blockIdx.x == 0) {
         if (threadIdx.x == 0) {
             *sr.bound_out = B;
             *sr.delta_out = delta;
             *count_reset = 0;
         }
-        if (threadIdx.x < PEND_SLOTS) pend_reset[threadIdx.x] = KEY_NONE;
+        if (threadIdx.x < PEND_SLOTS) pend_reset[threadIdx.x * PEND_STRIDE] = KEY_NONE;
     }
     const TileCtx<T> ctx{g, sd, u, fixedmask, dep_start, deps, keys_out, tmax};
     unsigned long long my_sweeps = 0, my_visits = 0;
@@ -496,12 +492,7 @@ __global__ __launch_bounds__(256) void k_fim_sched(Geom g, StencilData<T> sd, T 
         }
         __syncthreads();  // blist and wcount are reused by the next chunk
     }
-    const int slot = ((blockIdx.x * 4 + wv) & (PEND_SLOTS - 1));
-    if (lane == 0 && wmin != KEY_NONE) atomicMin(&sr.pend_out[slot], wmin);
-    if (lane == 0 && my_visits) {
-        atomicAdd(&stats[2 * slot], my_visits);
-        atomicAdd(&stats[2 * slot + 1], my_sweeps);
-    }
+    block_flush(wmin, my_visits, my_sweeps, sr.pend_out, stats);
 }
 
 // ---------------------------------------------------------------------------------------------

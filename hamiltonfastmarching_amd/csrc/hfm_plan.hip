@@ -318,19 +318,17 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
     HIPCHECK(launch_init<T>(p->g, static_cast<T *>(p->u), p->fixedmask, p->keys, p->tmax, p->np, p->seed_p,
                             p->seed_v, nseeds, s, 2));
     HIPCHECK(hipMemsetAsync(p->counts, 0, 4 * sizeof(int), s));
-    HIPCHECK(hipMemsetAsync(p->stats, 0, 2 * PEND_SLOTS * sizeof(unsigned long long), s));
-    HIPCHECK(hipMemsetAsync(p->nextmin, 0xff, 4 * PEND_SLOTS * sizeof(unsigned), s));
+    HIPCHECK(hipMemsetAsync(p->stats, 0, STAT_WORDS * sizeof(unsigned long long), s));
+    HIPCHECK(hipMemsetAsync(p->nextmin, 0xff, 4 * PEND_WORDS * sizeof(unsigned), s));
     const float delta0 = (float)(p->delta_factor > 0 ? p->delta_factor * p->tau : 0.);
     const float binit[5] = {0.f, -__builtin_huge_valf(), delta0, delta0, delta0};
     HIPCHECK(hipMemcpyAsync(p->bounds, binit, sizeof(binit), hipMemcpyHostToDevice, s));
-    // Default round target: 1024 tiles for 2-D grids and for grids of at most 4096 tiles, whose
-    // rounds stay within the 2560 waves of the update grid, so wider bands cost no latency and
-    // save rounds (Riemann2 256^2 1.33 -> 1.03 ms, 1024^2 6.4 -> 5.7 ms, Riemann3 64^3 1.28 ->
-    // 1.21 ms); 128 for larger 3-D grids, whose rounds hold several tiles per wave and where
-    // every redundant visit lengthens a wave's chain (Riemann3 96^3 1.9 against 2.5 ms at 1024).
-    const int target = p->round_target > 0 ? p->round_target
-                       : (p->g.dim == 2 || p->g.ntiles <= 4096) ? 1024
-                                                                : std::max(64, std::min(128, p->g.ntiles / 4));
+    // Default round target: 1024 tiles for 2-D grids and 3-D grids of fewer than 4096 tiles,
+    // whose rounds fit the update grid's waves, so wider bands cost no latency and save rounds;
+    // 64 for larger 3-D grids, whose rounds hold several tiles per wave and where every redundant
+    // visit lengthens a wave's chain (Riemann3 128^3: 2.26 ms at 64, 2.29 at 128; 64^3: 0.90 ms at
+    // 64, 0.99 at 1024).
+    const int target = p->round_target > 0 ? p->round_target : (p->g.dim == 3 && p->g.ntiles >= 4096) ? 64 : 1024;
     HIPCHECK(launch_seed_activate(p->seed_p, p->seed_v, nseeds, p->dep_start, p->deps, p->keys, p->tmax,
                                   &p->nextmin[0], s));
 
@@ -346,12 +344,13 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
     a.tmax = p->tmax;
     a.list = p->lists[0];
     a.stats = p->stats;
-    // 640 blocks (2560 waves, 2.5 per SIMD): more waves made the scattered neighbour loads of
-    // every visit slower than the extra concurrency gained (Riemann3 128^3: 3.26 ms per solve
-    // at 640 blocks, 3.39 ms at 1024)
-    // ReedsSheppForward2 (four differences a point, light visits) does better with fewer:
-    // 384 blocks, 3.4 -> 3.0 ms at 256 x 256 x 64.
-    const int grid_default = p->model == MODEL_RSF2 ? 384 : 640;
+    // 960 blocks of 4 waves (3.75 waves per SIMD; the kernel's 81 VGPRs allow 5): Riemann3 128^3
+    // 2.33 / 2.26 / 2.27 ms at 640 / 960 / 1280 blocks, ReedsSheppForward2 256 x 256 x 64 2.91 /
+    // 2.59 / 2.50 ms at 384 / 640 / 960; 1280 for 2-D grids scheduled by the listing kernel
+    // (Riemann2 1024^2: 4.54 / 4.25 / 4.14 ms at 640 / 960 / 1280).  More waves used to be slower
+    // while every wave ended with its own atomics on two shared cache lines.
+    const bool fused = p->fused_schedule > 0 || (p->fused_schedule < 0 && p->g.dim == 2 && p->g.ntiles <= 4096);
+    const int grid_default = (p->g.dim == 2 && !fused) ? 1280 : 960;
     a.grid = (unsigned)std::min<long long>((p->g.ntiles + 3) / 4, p->fim_grid > 0 ? p->fim_grid : grid_default);
     // At most 4 sweeps per visit on Riemann3 and Elastica2 (128^3: 2.73 against 2.79 ms at 8;
     // Elastica2 192 x 192 x 64: 23.9 against 28.1 ms; a tile still changing after 4 sweeps goes
@@ -366,7 +365,7 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
     // (Riemann2 256^2 FP32 0.98 -> 0.90 ms, FP64 1.63 -> 1.52 ms) and not on larger ones, whose
     // rounds list a thousand tiles or more and leave the blocks' shares uneven (Riemann2 1024^2
     // 5.40 -> 5.70 ms, Riemann3 128^3 2.79 -> 3.67 ms).
-    a.fused = p->fused_schedule > 0 || (p->fused_schedule < 0 && p->g.dim == 2 && p->g.ntiles <= 4096) ? 1 : 0;
+    a.fused = fused ? 1 : 0;
     std::vector<hipEvent_t> tev;
     long long k = 0;
     const int batch = 8;  // even: every batch starts on an even round, so all batches are alike
@@ -381,8 +380,8 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
                 // from key buffer kk % 2 and marks into buffer (kk + 1) % 2
                 a.keys = p->keys + (kk & 1) * (long long)p->g.ntiles;
                 a.keys_out = p->keys + ((kk + 1) & 1) * (long long)p->g.ntiles;
-                a.sr = SchedRound{&p->nextmin[(kk % 4) * PEND_SLOTS],
-                                  &p->nextmin[((kk + 1) % 4) * PEND_SLOTS],
+                a.sr = SchedRound{&p->nextmin[(kk % 4) * PEND_WORDS],
+                                  &p->nextmin[((kk + 1) % 4) * PEND_WORDS],
                                   &p->bounds[(kk + 1) & 1],
                                   &p->bounds[kk & 1],
                                   nullptr,
@@ -393,7 +392,7 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
                                   &p->counts[(kk + 3) % 4],
                                   target};
                 a.count_reset = &p->counts[(kk + 1) % 4];
-                a.pend_reset = &p->nextmin[((kk + 2) % 4) * PEND_SLOTS];
+                a.pend_reset = &p->nextmin[((kk + 2) % 4) * PEND_WORDS];
                 if (timed) {
                     hipEvent_t ea, eb;
                     HIPCHECK(hipEventCreate(&ea));
@@ -407,8 +406,8 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
                 continue;
             }
             // round kk: schedule lists tiles into lists[0] / counts[kk&1], update consumes them
-            SchedRound sr{&p->nextmin[(kk & 1) * PEND_SLOTS],
-                          &p->nextmin[((kk + 1) & 1) * PEND_SLOTS],
+            SchedRound sr{&p->nextmin[(kk & 1) * PEND_WORDS],
+                          &p->nextmin[((kk + 1) & 1) * PEND_WORDS],
                           &p->bounds[(kk + 1) & 1],
                           &p->bounds[kk & 1],
                           p->lists[0],
@@ -421,8 +420,8 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
             HIPCHECK(launch_schedule(p->g.ntiles, p->keys, sr, st));
             a.count = &p->counts[kk & 1];
             a.count_reset = &p->counts[(kk + 1) & 1];
-            a.pend_out = &p->nextmin[((kk + 1) & 1) * PEND_SLOTS];
-            a.pend_reset = &p->nextmin[(kk & 1) * PEND_SLOTS];
+            a.pend_out = &p->nextmin[((kk + 1) & 1) * PEND_WORDS];
+            a.pend_reset = &p->nextmin[(kk & 1) * PEND_WORDS];
             if (timed) {
                 hipEvent_t ea, eb;
                 HIPCHECK(hipEventCreate(&ea));
@@ -481,8 +480,7 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
     }
     p->prev_rounds = k;
     if (e1) HIPCHECK(hipEventRecord(e1, s));
-    HIPCHECK(hipMemcpyAsync(p->h_stats, p->stats, 2 * PEND_SLOTS * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                            s));
+    HIPCHECK(hipMemcpyAsync(p->h_stats, p->stats, STAT_WORDS * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
     float ms = 0;
     if (e0) HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
@@ -491,8 +489,8 @@ template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
     p->st.tile_visits = 0;
     p->st.sweeps = 0;
     for (int i = 0; i < PEND_SLOTS; ++i) {
-        p->st.tile_visits += (int64_t)p->h_stats[2 * i];
-        p->st.sweeps += (int64_t)p->h_stats[2 * i + 1];
+        p->st.tile_visits += (int64_t)p->h_stats[i * STAT_STRIDE];
+        p->st.sweeps += (int64_t)p->h_stats[i * STAT_STRIDE + 1];
     }
     p->st.tiles_carried = 0;
     p->st.delta = delta0;
@@ -648,17 +646,16 @@ int hfm_plan_create(hfm_plan **out, const char *model, const int64_t *dims, cons
         p->scan_work = dalloc<int>(scan_work_ints(nt));
         p->keys = dalloc<unsigned>(2 * (size_t)nt);  // two buffers: rounds that list their own tiles alternate them
         p->tmax = dalloc<unsigned>(nt);
-        p->nextmin = dalloc<unsigned>(4 * PEND_SLOTS);
+        p->nextmin = dalloc<unsigned>(4 * PEND_WORDS);
         p->bounds = dalloc<float>(5);  // 2 band bounds, 2 band widths, initial band width
         p->red = dalloc<RedSlot>(RED_SLOTS);
         HIPCHECK(hipHostMalloc((void **)&p->h_red, sizeof(RedSlot) * RED_SLOTS, hipHostMallocDefault));
         HIPCHECK(hipEventCreate(&p->stencil_ev[0]));
         HIPCHECK(hipEventCreate(&p->stencil_ev[1]));
         p->status = dalloc<int>(4);
-        p->stats = dalloc<unsigned long long>(2 * PEND_SLOTS);
+        p->stats = dalloc<unsigned long long>(STAT_WORDS);
         HIPCHECK(hipHostMalloc((void **)&p->h_count, sizeof(int) * 4, hipHostMallocDefault));
-        HIPCHECK(hipHostMalloc((void **)&p->h_stats, sizeof(unsigned long long) * 2 * PEND_SLOTS,
-                               hipHostMallocDefault));
+        HIPCHECK(hipHostMalloc((void **)&p->h_stats, sizeof(unsigned long long) * STAT_WORDS, hipHostMallocDefault));
         p->st.n_points = p->N;
         p->st.n_tiles = nt;
     } catch (const HipFail &e) {
@@ -1299,7 +1296,7 @@ extern "C" int hfm_riemann_solve_noalloc(const char *model, const int64_t *dims,
     struct HostMirror {
         int count[4];
         RedSlot red[RED_SLOTS];
-        unsigned long long stats[2 * PEND_SLOTS];
+        unsigned long long stats[STAT_WORDS];
     };
     static thread_local HostMirror hm;
     try {
@@ -1329,11 +1326,11 @@ extern "C" int hfm_riemann_solve_noalloc(const char *model, const int64_t *dims,
         p->scan_work = cv.take<int>(scan_work_ints(nt));
         p->keys = cv.take<unsigned>(2 * nt);
         p->tmax = cv.take<unsigned>(nt);
-        p->nextmin = cv.take<unsigned>(4 * PEND_SLOTS);
+        p->nextmin = cv.take<unsigned>(4 * PEND_WORDS);
         p->bounds = cv.take<float>(5);
         p->red = cv.take<RedSlot>(RED_SLOTS);
         p->status = cv.take<int>(4);
-        p->stats = cv.take<unsigned long long>(2 * PEND_SLOTS);
+        p->stats = cv.take<unsigned long long>(STAT_WORDS);
         p->seed_p = cv.take<long long>(NOALLOC_MAX_SEEDS);
         p->seed_v = cv.take<double>(NOALLOC_MAX_SEEDS);
         p->seed_cap = NOALLOC_MAX_SEEDS;

@@ -45,10 +45,10 @@ struct SolveDesc {
     int active;
     float delta;                  // initial band width
     int target;                   // tiles per round the band width is steered to
-    unsigned *pend;               // 2 * PEND_SLOTS smallest pending keys (alternating rounds)
+    unsigned *pend;               // 2 * PEND_WORDS: smallest pending keys of alternating rounds, PEND_STRIDE apart
     float *bound;                 // 2 band bounds + 2 band widths
     int *count;                   // 2 tiles listed per round
-    unsigned long long *stats;    // PEND_SLOTS pairs (visits, sweeps)
+    unsigned long long *visits;   // tile visits of the solve so far (tiles listed by its rounds)
 };
 
 // Jacobi sweeps per tile visit in the serving update kernel (an unconverged tile goes back to
@@ -68,7 +68,8 @@ __global__ __launch_bounds__(SCHED_BLOCK) void k_mschedule(const SolveDesc *__re
     if (delta > 0.f)
         delta = adapt_delta(d.bound[2 + (parity ^ 1)], d.count[parity ^ 1], d.target, d.delta * (1.f / 16.f),
                             64.f * d.delta);
-    const float B = band_bound(key_value(read_pend(d.pend + parity * PEND_SLOTS)), d.bound[parity ^ 1], delta);
+    const float B =
+        band_bound(key_value(read_pend(d.pend + parity * PEND_WORDS, PEND_STRIDE)), d.bound[parity ^ 1], delta);
     if (bm.y == 0 && threadIdx.x == 0) {
         d.bound[parity] = B;
         d.bound[2 + parity] = delta;
@@ -80,7 +81,8 @@ __global__ __launch_bounds__(SCHED_BLOCK) void k_mschedule(const SolveDesc *__re
     const unsigned left = (!take && k != KEY_NONE) ? k : KEY_NONE;
     if (take) d.ctx.keys[t] = KEY_NONE;
     const int pos = block_take(take, left, list_count, &d.count[parity],
-                               &d.pend[(parity ^ 1) * PEND_SLOTS + (blockIdx.x & (PEND_SLOTS - 1))]);
+                               &d.pend[(parity ^ 1) * PEND_WORDS + (blockIdx.x & (PEND_SLOTS - 1)) * PEND_STRIDE],
+                               d.visits);
     if (take) list[pos] = (s << SOLVE_SHIFT) | t;
 }
 
@@ -98,7 +100,7 @@ __global__ __launch_bounds__(256) void k_mfim(const SolveDesc *__restrict__ desc
             if (descs[i].active) descs[i].count[parity ^ 1] = 0;
         for (int i = threadIdx.x; i < ndesc * PEND_SLOTS; i += blockDim.x) {
             const SolveDesc &d = descs[i / PEND_SLOTS];
-            if (d.active) d.pend[parity * PEND_SLOTS + (i % PEND_SLOTS)] = KEY_NONE;
+            if (d.active) d.pend[parity * PEND_WORDS + (i % PEND_SLOTS) * PEND_STRIDE] = KEY_NONE;
         }
     }
     const int gw = blockIdx.x * 4 + wv;
@@ -113,15 +115,11 @@ __global__ __launch_bounds__(256) void k_mfim(const SolveDesc *__restrict__ desc
         const int tile = e & ((1 << SOLVE_SHIFT) - 1);
         const SolveDesc &d = descs[s];
         unsigned wm = KEY_NONE;
-        int sw;
         // at most SERVE_SWEEPS sweeps per visit; an unconverged tile goes back to its solve's schedule
-        if (d.model == MODEL_RIEMANN3) sw = fim_tile<MODEL_RIEMANN3, float, SERVE_SWEEPS>(d.ctx, tile, lane, su[wv], wm);
-        else sw = fim_tile<MODEL_RIEMANN2, float, SERVE_SWEEPS>(d.ctx, tile, lane, su[wv], wm);
-        if (lane == 0) {
-            if (wm != KEY_NONE) atomicMin(&d.pend[(parity ^ 1) * PEND_SLOTS + slot], wm);
-            atomicAdd(&d.stats[2 * slot], 1ull);
-            atomicAdd(&d.stats[2 * slot + 1], (unsigned long long)sw);
-        }
+        if (d.model == MODEL_RIEMANN3) (void)fim_tile<MODEL_RIEMANN3, float, SERVE_SWEEPS>(d.ctx, tile, lane, su[wv], wm);
+        else (void)fim_tile<MODEL_RIEMANN2, float, SERVE_SWEEPS>(d.ctx, tile, lane, su[wv], wm);
+        // each slot on its own 128-byte line: same-line atomics serialise
+        if (lane == 0 && wm != KEY_NONE) atomicMin(&d.pend[(parity ^ 1) * PEND_WORDS + slot * PEND_STRIDE], wm);
     }
 }
 
@@ -147,13 +145,13 @@ __global__ __launch_bounds__(256) void k_setup_clear(int *__restrict__ status, R
 // Scheduling state of a solve joining the running set at a round of the given parity: the seed
 // key (smallest key the seed marks left pending) becomes the pending minimum of that round.
 __global__ __launch_bounds__(64) void k_join_state(unsigned *__restrict__ pend, int *__restrict__ count,
-                                                   unsigned long long *__restrict__ stats, float *__restrict__ bound,
+                                                   unsigned long long *__restrict__ visits, float *__restrict__ bound,
                                                    const unsigned *__restrict__ seed_key, int parity, float delta) {
     const int i = threadIdx.x;
-    for (int j = i; j < 2 * PEND_SLOTS; j += 64) {
-        pend[j] = (j == parity * PEND_SLOTS) ? *seed_key : KEY_NONE;
-        stats[j] = 0ull;
-    }
+    for (int j = i; j < 2 * PEND_SLOTS; j += 64)
+        pend[(j / PEND_SLOTS) * PEND_WORDS + (j % PEND_SLOTS) * PEND_STRIDE] =
+            (j == parity * PEND_SLOTS) ? *seed_key : KEY_NONE;
+    if (i == 0) *visits = 0ull;
     if (i < 2) count[i] = 0;
     if (i < 4) bound[i] = i < 2 ? -__builtin_huge_valf() : delta;
 }
@@ -281,9 +279,9 @@ struct hfm_server {
     long long list_cap = 0;
     int *d_lc = nullptr;          // 2 alternating list counters
     int *d_count = nullptr;       // MAX_SLOTS * 2
-    unsigned *d_pend = nullptr;   // MAX_SLOTS * 2 * PEND_SLOTS
+    unsigned *d_pend = nullptr;   // MAX_SLOTS * 2 * PEND_WORDS
     float *d_bound = nullptr;     // MAX_SLOTS * 2
-    unsigned long long *d_stats = nullptr;  // MAX_SLOTS * 2 * PEND_SLOTS
+    unsigned long long *d_stats = nullptr;  // MAX_SLOTS tile-visit counters
     int *h_count[2] = {};                   // pinned MAX_SLOTS * 2, per batch parity
     unsigned long long *h_stats[2] = {};    // pinned, per batch parity
     hipEvent_t batch_done[2] = {};
@@ -479,10 +477,10 @@ void join_solve(hfm_server *S, Slot *sl) {
     sl->idx = idx;
     sl->first_batch = S->nlaunched;
     const int parity = (int)(S->round & 1);
-    unsigned *pend = S->d_pend + (size_t)idx * 2 * PEND_SLOTS;
+    unsigned *pend = S->d_pend + (size_t)idx * 2 * PEND_WORDS;
     float *bound = S->d_bound + (size_t)idx * 4;
     int *count = S->d_count + (size_t)idx * 2;
-    unsigned long long *st = S->d_stats + (size_t)idx * 2 * PEND_SLOTS;
+    unsigned long long *st = S->d_stats + idx;
     hipLaunchKernelGGL(k_join_state, dim3(1), dim3(64), 0, s, pend, count, st, bound, sl->seed_key, parity, sl->delta);
     HIPCHECK(hipGetLastError());
     SolveDesc &d = S->h_desc[idx];
@@ -495,16 +493,13 @@ void join_solve(hfm_server *S, Slot *sl) {
     d.pend = pend;
     d.bound = bound;
     d.count = count;
-    d.stats = st;
+    d.visits = st;
     S->running.push_back(sl);
 }
 
 void finish_export(Slot *sl, const unsigned long long *h_stats) {
     RequestRec &r = *sl->rec;
-    unsigned long long v = 0;
-    const unsigned long long *hs = h_stats + (size_t)sl->idx * 2 * PEND_SLOTS;
-    for (int i = 0; i < PEND_SLOTS; ++i) v += hs[2 * i];
-    r.res.tile_visits = (int64_t)v;
+    r.res.tile_visits = (int64_t)h_stats[sl->idx];
     r.res.rounds = sl->rounds;
 }
 
@@ -561,8 +556,8 @@ void launch_batch(hfm_server *S, long long q, int batch) {
     }
     for (Slot *sl : S->running) sl->rounds += batch;
     HIPCHECK(hipMemcpyAsync(S->h_count[par], S->d_count, sizeof(int) * 2 * S->ndesc, hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipMemcpyAsync(S->h_stats[par], S->d_stats, sizeof(unsigned long long) * 2 * PEND_SLOTS * S->ndesc,
-                            hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(S->h_stats[par], S->d_stats, sizeof(unsigned long long) * S->ndesc, hipMemcpyDeviceToHost,
+                            s));
     HIPCHECK(hipEventRecord(S->batch_done[par], s));
     std::lock_guard<std::mutex> lk(S->mu);
     S->stats.batches++;
@@ -722,12 +717,12 @@ int hfm_server_create(hfm_server **out, double hbm_budget_bytes, int max_residen
         S->d_lc = dalloc<int>(2);
         HIPCHECK(hipMemset(S->d_lc, 0, 2 * sizeof(int)));
         S->d_count = dalloc<int>(MAX_SLOTS * 2);
-        S->d_pend = dalloc<unsigned>(MAX_SLOTS * 2 * PEND_SLOTS);
+        S->d_pend = dalloc<unsigned>((size_t)MAX_SLOTS * 2 * PEND_WORDS);
         S->d_bound = dalloc<float>(MAX_SLOTS * 4);
-        S->d_stats = dalloc<unsigned long long>(MAX_SLOTS * 2 * PEND_SLOTS);
+        S->d_stats = dalloc<unsigned long long>(MAX_SLOTS);
         for (int i = 0; i < 2; ++i) {
             HIPCHECK(hipHostMalloc((void **)&S->h_count[i], sizeof(int) * MAX_SLOTS * 2, hipHostMallocDefault));
-            HIPCHECK(hipHostMalloc((void **)&S->h_stats[i], sizeof(unsigned long long) * MAX_SLOTS * 2 * PEND_SLOTS,
+            HIPCHECK(hipHostMalloc((void **)&S->h_stats[i], sizeof(unsigned long long) * MAX_SLOTS,
                                    hipHostMallocDefault));
         }
         for (int i = 0; i < hfm_server::NSETUP; ++i) S->th_setup[i] = std::thread(setup_loop, S, i);

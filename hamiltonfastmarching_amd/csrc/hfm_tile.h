@@ -60,15 +60,48 @@ __device__ inline unsigned mark_pending(int d, unsigned key, unsigned *__restric
     return wave_min_u32(lowered ? key : KEY_NONE);
 }
 
-// The smallest pending key is accumulated in PEND_SLOTS words (one per wave id modulo
-// PEND_SLOTS) so that no single address takes an atomic from every wave.
-__device__ inline unsigned read_pend(const unsigned *slots) {
+// The smallest pending key is accumulated in PEND_SLOTS words, `stride` words apart (one per
+// block or wave id modulo PEND_SLOTS), so that no single address takes an atomic from every wave.
+__device__ inline unsigned read_pend(const unsigned *slots, int stride = 1) {
     unsigned m = KEY_NONE;
+#pragma unroll
     for (int i = 0; i < PEND_SLOTS; ++i) {
-        const unsigned v = slots[i];
+        const unsigned v = slots[i * stride];
         m = v < m ? v : m;
     }
     return m;
+}
+
+// End of an update launch: the block's four waves fold their smallest marked key and their
+// visit / sweep counts in LDS, and one thread adds them to the block's slot (one atomic per
+// counter per block instead of per wave).  Every thread of the block calls it.
+__device__ inline void block_flush(unsigned wmin, unsigned long long visits, unsigned long long sweeps,
+                                   unsigned *__restrict__ pend_out, unsigned long long *__restrict__ stats) {
+    __shared__ unsigned s_min[4];
+    __shared__ unsigned long long s_vis[4], s_sw[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        s_min[wv] = wmin;
+        s_vis[wv] = visits;
+        s_sw[wv] = sweeps;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned m = KEY_NONE;
+        unsigned long long v = 0, sw = 0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            m = s_min[w] < m ? s_min[w] : m;
+            v += s_vis[w];
+            sw += s_sw[w];
+        }
+        const int slot = blockIdx.x & (PEND_SLOTS - 1);
+        if (m != KEY_NONE) atomicMin(&pend_out[slot * PEND_STRIDE], m);
+        if (v) {
+            atomicAdd(&stats[slot * STAT_STRIDE], v);
+            atomicAdd(&stats[slot * STAT_STRIDE + 1], sw);
+        }
+    }
 }
 
 // Band bound of a round: a window sliding with the smallest pending key, never moving back --
@@ -104,11 +137,12 @@ constexpr int SCHED_BLOCK = 1024;
 // Block-level aggregation of a scheduling pass (every thread of the block calls it once):
 // `take` = the thread's tile is listed this round, `left` = the key it leaves pending
 // (KEY_NONE if none).  One atomicAdd per block reserves the block's list entries (and adds them
-// to *solve_count if given), one atomicMin folds the block's smallest left key into *pend_slot.
-// Returns the list position of a taken tile.  Per-wave atomics on the shared list counter
-// would serialise thousands of waves on one address.
+// to *solve_count and *visits if given), one atomicMin folds the block's smallest left key into
+// *pend_slot.  Returns the list position of a taken tile.  Per-wave atomics on the shared list
+// counter would serialise thousands of waves on one address.
 __device__ inline int block_take(bool take, unsigned left, int *__restrict__ list_count,
-                                 int *__restrict__ solve_count, unsigned *__restrict__ pend_slot) {
+                                 int *__restrict__ solve_count, unsigned *__restrict__ pend_slot,
+                                 unsigned long long *__restrict__ visits = nullptr) {
     constexpr int NW = SCHED_BLOCK / 64;
     __shared__ int wcnt[NW];
     __shared__ unsigned wmn[NW];
@@ -130,6 +164,7 @@ __device__ inline int block_take(bool take, unsigned left, int *__restrict__ lis
         }
         base_s = tot ? atomicAdd(list_count, tot) : 0;
         if (tot && solve_count) atomicAdd(solve_count, tot);
+        if (tot && visits) atomicAdd(visits, (unsigned long long)tot);
         if (m != KEY_NONE) atomicMin(pend_slot, m);
     }
     __syncthreads();

@@ -242,12 +242,12 @@ int hfm_plan_get_stencils(hfm_plan *plan, int8_t *h_offsets, double *h_weights);
 int hfm_plan_set_delta(hfm_plan *plan, double factor);
 
 /* Tiles per round that the band width is steered to (x1.5 / /1.5 each round; 0: a default of
- * 1024 for 2-D grids and grids of at most 4096 tiles, else n_tiles / 4 clamped to [64, 128]).
- * Changes the work, never the fixed point. */
+ * 64 for 3-D grids of 4096 tiles or more, else 1024).  Changes the work, never the fixed point. */
 int hfm_plan_set_round_target(hfm_plan *plan, int tiles);
 
 /* Blocks (of 4 wavefronts, one tile per wavefront at a time) of each update-kernel launch
- * (0: 640; 384 for ReedsSheppForward2).  A tuning knob; results do not depend on it. */
+ * (0: 960; 1280 for 2-D grids scheduled by the listing kernel).  A tuning knob; results do not
+ * depend on it. */
 int hfm_plan_set_grid(hfm_plan *plan, int blocks);
 
 /* Jacobi sweeps a tile visit may run before an unconverged tile is handed back to the
