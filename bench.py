@@ -40,6 +40,8 @@ Printed JSON fields beyond the driver's contract:
   configs       the other workloads on the GPU (ms per solve from HBM-resident inputs; geodesics
                 and adjoint timed separately) beside the reference executable on the same
                 inputs (StencilCPUTime + FMCPUTime, 1 core) and its accuracy.
+  io_path       the drop-in IO path (HFMIO set + hfm_run + values read back) on the same
+                workload, host buffers in and out: never the headline value.
   residual      final FP64 residual check of the last timed solve (hfm_plan_residual): the
                 largest relative deviation between each reached non-seed value and the FP64
                 root of its local scheme, and the finite / infinite mismatches.
@@ -154,6 +156,33 @@ def measure_traffic(timeout_s=180):
             dbs.append(found[0])
         res = profiling.counters(dbs[0], dbs[1], "k_fim", float(1 << 30))
     return res, "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in this run, scaled by a 1 GiB device copy"
+
+
+def io_path_leg(params, reps=3):
+    """The drop-in IO path on the same Riemann3 workload (FP32 update): the input keys copied into
+    an HFMIO object (set), hfm_run (run: host buffers in, upload, stencils, solve, exports back to
+    the host), and the values read back (get); medians of `reps` calls.  Never the headline value:
+    it includes host-memory copies and PCIe transfers that the device-resident path does not."""
+    import hamiltonfastmarching_amd as hfm
+    q = dict(params, precision=32.0)
+    hfm.run(q)
+    rows = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        io = hfm.HFMIO(default_ordering="")
+        for k, v in q.items():
+            io[k] = v
+        t1 = time.perf_counter()
+        io.run()
+        t2 = time.perf_counter()
+        _ = io["values"]
+        t3 = time.perf_counter()
+        rows.append(((t3 - t0) * 1e3, (t1 - t0) * 1e3, (t2 - t1) * 1e3, (t3 - t2) * 1e3))
+    a = np.median(np.array(rows), axis=0)
+    return {"io_path_ms": round(float(a[0]), 2), "set_ms": round(float(a[1]), 2), "run_ms": round(float(a[2]), 2),
+            "get_ms": round(float(a[3]), 2),
+            "definition": "HFMIO set of the input keys (a copy into the IO store, as the reference's interface "
+                          "does) + hfm_run + values read back, median of 3, FP32 update"}
 
 
 def copy_ceiling_gbs(nbytes=1 << 30, reps=10):
@@ -481,6 +510,8 @@ def main():
                      "definition": "FP64 recompute of the local scheme at every reached non-seed point: "
                                    "max |Lambda(u) - u| / |u|"},
     }
+    if n == 128 and not args.no_configs:
+        result["io_path"] = io_path_leg(params)
     if not args.no_serving:
         result["serving"] = serving_leg(args.serving_requests, with_cpu=not args.no_cpu)
     if not args.no_configs:

@@ -164,6 +164,11 @@ int hfm_plan_geodesics_impl(hfm_plan *p, int ntips, const double *tips, bool tip
                             double threshold, double volumeBound, double *points, int64_t max_points,
                             int64_t *lengths, int64_t *npoints, int *nfailed, void *stream);
 
+namespace hfm {
+// Number of reached grid points (finite values) of the last solve, counted on the device.
+int plan_count_reached(hfm_plan *p, int64_t *n, void *stream);
+}  // namespace hfm
+
 int hfm_plan_forward_variation_impl(hfm_plan *p, int nvar, const double *d_cost_variation, int nseeds,
                                     const double *seed_value_variation, double *d_value_variation, void *stream);
 

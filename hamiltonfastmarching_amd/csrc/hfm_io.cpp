@@ -37,6 +37,11 @@
 // Host copy of a curvature plan's per-theta stencil tables (hfm_plan.hip).
 int hfm_plan_curvature_tables(const hfm_plan *p, std::vector<uint32_t> &offs, std::vector<double> &w);
 
+// Reached (finite-valued) grid points of the last solve, counted on the device (hfm_plan.hip).
+namespace hfm {
+int plan_count_reached(hfm_plan *p, int64_t *n, void *stream);
+}
+
 // Geodesics from tips already in grid units (hfm_plan.hip).
 int hfm_plan_geodesics_impl(hfm_plan *p, int ntips, const double *tips, bool tips_adim, double step,
                             double threshold, double volumeBound, double *points, int64_t max_points,
@@ -254,14 +259,51 @@ std::vector<double> grid_field(hfm_io *io, const std::string &key, const std::ve
     throw IOError{"Field " + key + " has incorrect depth.\n"};
 }
 
-void set_grid_array(hfm_io *io, const std::string &key, const std::vector<double> &internal,
+// Store an internal row-major grid array as a computed key, in the IO's array ordering (moved,
+// not copied, when no reordering is needed).
+void set_grid_array(hfm_io *io, const std::string &key, std::vector<double> internal,
                     const std::vector<int64_t> &dims, int64_t ncomp) {
     Field f;
     std::vector<int64_t> stored;
-    f.data = reorder(internal, dims, ncomp, io->ordering, false, stored);
+    if (io->ordering == ROW_MAJOR || dims.size() < 2) {
+        stored = dims;
+        f.data = std::move(internal);
+    } else {
+        f.data = reorder(internal, dims, ncomp, io->ordering, false, stored);
+    }
     f.dims = stored;
     if (ncomp > 1) f.dims.push_back(ncomp);
     io->set_computed(key, std::move(f));
+}
+
+// The input grid field read in place when it is already a full row-major grid of the expected
+// dimensions (no copy of e.g. the 96 MB metric of a 128^3 grid), else converted by grid_field.
+struct FieldView {
+    const double *p = nullptr;
+    size_t n = 0;
+    std::vector<double> own;
+    const double *data() const { return p; }
+    size_t size() const { return n; }
+};
+
+FieldView grid_field_view(hfm_io *io, const std::string &key, const std::vector<int64_t> &dims, int64_t ncomp,
+                          bool bundle, bool &is_field) {
+    FieldView v;
+    const Field &f = io->raw(key);
+    if (!f.is_string && io->ordering == ROW_MAJOR) {
+        std::vector<int64_t> want = dims;
+        if (ncomp > 1) want.push_back(ncomp);
+        if (f.dims == want) {
+            is_field = true;
+            v.p = f.data.data();
+            v.n = f.data.size();
+            return v;
+        }
+    }
+    v.own = grid_field(io, key, dims, ncomp, bundle, is_field);
+    v.p = v.own.data();
+    v.n = v.own.size();
+    return v;
 }
 
 // Seed spreading (StaticFactoring.hxx:284-345), Riemann models: every grid point within
@@ -274,7 +316,7 @@ void set_grid_array(hfm_io *io, const std::string &key, const std::vector<double
 // `values` are replaced; spreadedSeeds / spreadedSeedValues are exported.
 template <int D>
 void spread_seeds_d(hfm_io *io, const std::vector<int64_t> &dims, const double *h, const std::vector<double> &origin,
-                    const std::vector<double> &metric, bool is_field, bool dual, double radius,
+                    const double *metric, bool is_field, bool dual, double radius,
                     std::vector<double> &seeds, std::vector<double> &values) {
     using Sym = hfm::SymPacked<D>;
     constexpr int K = Sym::K;
@@ -483,16 +525,18 @@ void run_impl(hfm_io *io) {
     // Read and validate every input field on the host before any device work.
     const int64_t K = dim * (dim + 1) / 2;
     bool is_field = false, hasDual = false;
-    std::vector<double> field;
+    FieldView field;
     if (riemann) {
         hasDual = io->has("dualMetric");
         if (hasDual && io->has("metric")) throw IOError{"Error: both primal and dual metric provided"};
-        field = grid_field(io, hasDual ? "dualMetric" : "metric", dims, K, false, is_field);
+        field = grid_field_view(io, hasDual ? "dualMetric" : "metric", dims, K, false, is_field);
     } else if (io->has("speed")) {
-        field = grid_field(io, "speed", dims, 1, true, is_field);
-        for (auto &v : field) v = 1. / v;
+        field.own = grid_field(io, "speed", dims, 1, true, is_field);
+        for (auto &v : field.own) v = 1. / v;
+        field.p = field.own.data();
+        field.n = field.own.size();
     } else {
-        field = grid_field(io, "cost", dims, 1, true, is_field);
+        field = grid_field_view(io, "cost", dims, 1, true, is_field);
     }
     if (precision != 32 && precision != 64) throw IOError{"precision must be 32 or 64"};
     // Walls (WallObstructionTest.h:30-36): an integral field read as bool (nonzero = wall),
@@ -711,8 +755,10 @@ void run_impl(hfm_io *io) {
                 if (!riemann) throw IOError{"Equation factoring error : no guess"};
                 if (seedRadius < 0.)
                     throw IOError{"Error : seedRadius < 0 (spreading along the reversed stencils) is not supported"};
-                if (dim == 2) spread_seeds_d<2>(io, dims, h, origin, field, is_field, hasDual, seedRadius, seeds, sv);
-                else spread_seeds_d<3>(io, dims, h, origin, field, is_field, hasDual, seedRadius, seeds, sv);
+                if (dim == 2)
+                    spread_seeds_d<2>(io, dims, h, origin, field.data(), is_field, hasDual, seedRadius, seeds, sv);
+                else
+                    spread_seeds_d<3>(io, dims, h, origin, field.data(), is_field, hasDual, seedRadius, seeds, sv);
             }
         }
         if (bundle && io->has("seeds_Unoriented")) {
@@ -753,13 +799,8 @@ void run_impl(hfm_io *io) {
         // nAccepted (CommonStoppingCriteria.h:162-165): without stopping criteria, every point the
         // heap accepts is a reached one, i.e. a finite value
         {
-            DevBuf dv(N * sizeof(double));
-            check(hfm_plan_values(plan, dv.d(), nullptr));
-            std::vector<double> v(N);
-            if (hipMemcpy(v.data(), dv.p, N * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
-                throw IOError{"hipMemcpy failed"};
             int64_t reached = 0;
-            for (double x : v) reached += std::isfinite(x) ? 1 : 0;
+            check(hfm::plan_count_reached(plan, &reached, nullptr));
             io->set_scalar("nAccepted", (double)reached);
         }
     }
@@ -844,7 +885,7 @@ void run_impl(hfm_io *io) {
         std::vector<double> v(N);
         if (hipMemcpy(v.data(), dvv.p, N * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
             throw IOError{"hipMemcpy failed"};
-        set_grid_array(io, "values", v, dims, 1);
+        set_grid_array(io, "values", std::move(v), dims, 1);
     }
     if (exportActive || exportFlow) {
         DevBuf dfl(exportFlow ? N * dim * sizeof(double) : 8);
@@ -854,13 +895,13 @@ void run_impl(hfm_io *io) {
             std::vector<double> a(N);
             if (hipMemcpy(a.data(), dac.p, N * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
                 throw IOError{"hipMemcpy failed"};
-            set_grid_array(io, "activeNeighs", a, dims, 1);
+            set_grid_array(io, "activeNeighs", std::move(a), dims, 1);
         }
         if (exportFlow) {
             std::vector<double> f(N * dim);
             if (hipMemcpy(f.data(), dfl.p, N * dim * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
                 throw IOError{"hipMemcpy failed"};
-            set_grid_array(io, "geodesicFlow", f, dims, dim);
+            set_grid_array(io, "geodesicFlow", std::move(f), dims, dim);
         }
     }
     if (io->scalar("exportActiveOffsets", 0.) != 0.) {
@@ -870,7 +911,7 @@ void run_impl(hfm_io *io) {
         std::vector<double> a(N * ns);
         if (hipMemcpy(a.data(), dao.p, N * ns * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
             throw IOError{"hipMemcpy failed"};
-        set_grid_array(io, "activeOffsets", a, dims, ns);
+        set_grid_array(io, "activeOffsets", std::move(a), dims, ns);
     }
     // FirstVariation::Finally (backward part)
     if (io->has("inspectSensitivity")) {
@@ -907,7 +948,7 @@ void run_impl(hfm_io *io) {
             std::vector<double> s(N);
             if (hipMemcpy(s.data(), dsens.p, N * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
                 throw IOError{"hipMemcpy failed"};
-            set_grid_array(io, "costSensitivity_" + std::to_string(i), s, dims, 1);
+            set_grid_array(io, "costSensitivity_" + std::to_string(i), std::move(s), dims, 1);
             // seedSensitivity: (ReDim(PointFromIndex(index)), weight) pairs (FirstVariation.h:90-98)
             Field f;
             const int n = std::min(ns, maxs);
@@ -981,7 +1022,7 @@ void run_impl(hfm_io *io) {
         }
         std::vector<int64_t> vdims(dims.begin(), dims.end());
         vdims.push_back(nvar);
-        set_grid_array(io, "valueVariation", vv, vdims, 1);
+        set_grid_array(io, "valueVariation", std::move(vv), vdims, 1);
     }
 }
 

@@ -117,6 +117,9 @@ hipError_t launch_fim(int model, const FimArgs<T> &a, hipStream_t s, hipEvent_t 
                       hipEvent_t ev1 = nullptr);
 template <typename T>
 hipError_t launch_export_values(const Geom &g, const T *u, double *out, long long np, hipStream_t s);
+// Reached (finite-valued) grid points, counted into the reduction slots' `count` fields.
+template <typename T>
+hipError_t launch_count_reached(const Geom &g, const T *u, long long np, RedSlot *red, hipStream_t s);
 template <typename T>
 hipError_t launch_flow(int model, const Geom &g, const StencilData<T> &sd, const T *u, const unsigned long long *fm,
                        const FlowScale &fs, double *flow, double *act, double *aoffs, long long np, hipStream_t s);

@@ -498,6 +498,27 @@ __global__ __launch_bounds__(256) void k_fim_sched(Geom g, StencilData<T> sd, T 
 // ---------------------------------------------------------------------------------------------
 // Exports.
 // ---------------------------------------------------------------------------------------------
+// Number of reached grid points (finite values): nAccepted of a solve without stopping criteria.
+template <typename T>
+__global__ __launch_bounds__(256) void k_count_reached(Geom g, const T *__restrict__ u, long long np,
+                                                       RedSlot *__restrict__ red) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double c = 0.;
+    if (p < np) {
+        int c3[3];
+        tiled_coords(g, (int)(p >> 6), (int)(p & 63), c3);
+        c = (in_grid(g, c3) && u[p] < dev_inf<T>()) ? 1. : 0.;
+    }
+    wave_add_tau(0., c, red);
+}
+
+template <typename T> hipError_t launch_count_reached(const Geom &g, const T *u, long long np, RedSlot *red, hipStream_t s) {
+    hipLaunchKernelGGL((k_count_reached<T>), dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, g, u, np, red);
+    return hipGetLastError();
+}
+template hipError_t launch_count_reached<float>(const Geom &, const float *, long long, RedSlot *, hipStream_t);
+template hipError_t launch_count_reached<double>(const Geom &, const double *, long long, RedSlot *, hipStream_t);
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_export_values(Geom g, const T *__restrict__ u, double *__restrict__ out,
                                                        long long np) {

@@ -1227,6 +1227,28 @@ int hfm_plan_forward_variation(hfm_plan *p, int nvar, const double *d_cost_varia
 
 }  // extern "C"
 
+int hfm::plan_count_reached(hfm_plan *p, int64_t *n, void *stream) {
+    if (!p || !n || !p->solved) {
+        set_error("plan_count_reached: no solve has run");
+        return 1;
+    }
+    try {
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        HIPCHECK(hipMemsetAsync(p->red, 0, RED_SLOTS * sizeof(RedSlot), s));
+        if (p->precision == 32)
+            HIPCHECK(launch_count_reached<float>(p->g, static_cast<const float *>(p->u), p->np, p->red, s));
+        else
+            HIPCHECK(launch_count_reached<double>(p->g, static_cast<const double *>(p->u), p->np, p->red, s));
+        HIPCHECK(hipMemcpyAsync(p->h_red, p->red, RED_SLOTS * sizeof(RedSlot), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        *n = (int64_t)red_total(p->h_red).count;
+    } catch (const HipFail &e) {
+        set_error(e.msg);
+        return 1;
+    }
+    return 0;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Allocation-free Riemann solve.  Every buffer of an FP32 Riemann plan is carved, per call, out
 // of one static device array (`scratch`, HFM_NOALLOC_WORKSPACE_BYTES per GPU, held from the
