@@ -84,16 +84,27 @@ __device__ inline int active_set(const Geom &g, const StencilData<T> &sd, const 
     return n;
 }
 
+// Lanes given to one point in a round of the reverse sweep: one per active edge it may have
+// (the model's differences, rounded up to a power of two); a wave takes 64 / ADJ_SLOTS points.
+template <int ModelT> constexpr int adj_slots() {
+    constexpr int NC = ModelShape<ModelT>::NFWD + ModelShape<ModelT>::NSYM;
+    return NC <= 4 ? 4 : NC <= 8 ? 8 : NC <= 16 ? 16 : 32;
+}
+
 template <int ModelT, typename T>
 __global__ __launch_bounds__(256) void k_adj_prepare(Geom g, StencilData<T> sd, const T *__restrict__ u,
                                                      const unsigned long long *__restrict__ fixedmask,
                                                      int *__restrict__ indeg, double *__restrict__ lambda,
-                                                     double *__restrict__ sens, long long np) {
+                                                     double *__restrict__ sens, int *__restrict__ ecnt,
+                                                     unsigned *__restrict__ eidx, double *__restrict__ var_out,
+                                                     double *__restrict__ ew, long long np) {
     constexpr int NC = ModelShape<ModelT>::NFWD + ModelShape<ModelT>::NSYM;
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     lambda[p] = 0.;
     sens[p] = 0.;
+    ecnt[p] = 0;
+    var_out[p] = 0.;
     int c[3];
     tiled_coords(g, (int)(p >> 6), (int)(p & 63), c);
     if (!in_grid(g, c)) return;
@@ -101,7 +112,14 @@ __global__ __launch_bounds__(256) void k_adj_prepare(Geom g, StencilData<T> sd, 
     long long nb[NC];
     double wp[NC], ws, var;
     const int n = active_set<ModelT, T>(g, sd, u, p, c, fixed, nb, wp, &ws, &var);
-    for (int i = 0; i < n; ++i) atomicAdd(&indeg[nb[i]], 1);
+    ecnt[p] = n;
+    var_out[p] = var;
+    const double inv = ws > 0. ? 1. / ws : 0.;
+    for (int i = 0; i < n; ++i) {
+        atomicAdd(&indeg[nb[i]], 1);
+        eidx[(long long)i * np + p] = (unsigned)nb[i];  // tiled indices fit 32 bits (at most 2^26 tiles)
+        ew[(long long)i * np + p] = wp[i] * inv;
+    }
 }
 
 // Wave-aggregated append of tiled indices into a list.
@@ -155,67 +173,57 @@ __global__ void k_adj_targets(const long long *__restrict__ tp, const double *__
     if (i < n) lambda[tp[i]] = tw[i];
 }
 
+// One level of the reverse sweep: every listed point (all its predecessors processed, so its
+// lambda is final) writes its sensitivity, pushes lambda * w'/S along its active edges, and
+// decrements its neighbours' in-degrees; those reaching zero form the next level's list.  A
+// point's edges are spread over ADJ_SLOTS lanes, one edge each: every lane then waits for at
+// most one returning atomic per pass (with one lane per point, a point's 27 decrements on
+// Elastica2 were 27 round trips in sequence).
 template <int ModelT, typename T>
-__global__ __launch_bounds__(256) void k_adj_round(Geom g, StencilData<T> sd, const T *__restrict__ u,
-                                                   const unsigned long long *__restrict__ fixedmask,
-                                                   int *__restrict__ indeg, double *__restrict__ lambda,
-                                                   double *__restrict__ sens, const long long *__restrict__ list_in,
+__global__ __launch_bounds__(256) void k_adj_round(int *__restrict__ indeg, double *__restrict__ lambda,
+                                                   double *__restrict__ sens, const int *__restrict__ ecnt,
+                                                   const unsigned *__restrict__ eidx, const double *__restrict__ var_in,
+                                                   const double *__restrict__ ew, long long np,
+                                                   const long long *__restrict__ list_in,
                                                    const int *__restrict__ count_in, long long *__restrict__ list_out,
                                                    int *__restrict__ count_out, int *__restrict__ count_reset,
                                                    long long *__restrict__ seeds_idx, double *__restrict__ seeds_w,
                                                    int *__restrict__ nseeds, int max_seeds) {
-    constexpr int NC = ModelShape<ModelT>::NFWD + ModelShape<ModelT>::NSYM;
+    constexpr int SL = adj_slots<ModelT>();
+    constexpr int GP = 64 / SL;  // points per wave pass
     const int n = *count_in;
     if (blockIdx.x == 0 && threadIdx.x == 0) *count_reset = 0;
     const int lane = threadIdx.x & 63;
+    const int g = lane / SL, k = lane % SL;
     // Uniform trip count per wave so that the wave-level pushes see every lane.
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (int base = wave * 64; base < n; base += nwaves * 64) {
-        const int i = base + lane;
-        long long nb[NC];
-        double wp[NC], ws = 0., var = 0.;
-        int na = 0;
-        double lam = 0.;
+    for (int base = wave * GP; base < n; base += nwaves * GP) {
+        const int i = base + g;
+        long long t = -1;
         if (i < n) {
             const long long p = list_in[i];
-            int c[3];
-            tiled_coords(g, (int)(p >> 6), (int)(p & 63), c);
-            const bool fixed = (fixedmask[p >> 6] >> (p & 63)) & 1ull;
-            na = active_set<ModelT, T>(g, sd, u, p, c, fixed, nb, wp, &ws, &var);
-            lam = lambda[p];
-            if (lam != 0.) {
-                sens[p] = lam * var;
+            const int na = ecnt[p];
+            const double lam = lambda[p];
+            if (k < na) {
+                t = eidx[(long long)k * np + p];
+                if (lam != 0.) atomicAdd(&lambda[t], lam * ew[(long long)k * np + p]);
+            }
+            if (k == 0 && lam != 0.) {
+                sens[p] = lam * var_in[p];
                 if (na == 0) {
                     const int s = atomicAdd(nseeds, 1);
                     if (s < max_seeds) {
                         seeds_idx[s] = p;
                         seeds_w[s] = lam;
                     }
-                } else if (ws > 0.) {
-                    const double inv = 1. / ws;
-                    for (int k = 0; k < na; ++k) atomicAdd(&lambda[nb[k]], lam * (wp[k] * inv));
                 }
             }
         }
-        // In-degree decrements, eight per lane at a time (independent atomics in flight), then one
-        // aggregated append of the neighbours that became ready.  They are processed by the next
-        // launch, which sees this launch's lambda updates.
-#pragma unroll
-        for (int k0 = 0; k0 < NC; k0 += 8) {
-            if (__ballot(k0 < na) == 0ull) continue;
-            long long items[8];
-            unsigned mask = 0;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int k = k0 + q;
-                items[q] = (k < NC && k < na) ? nb[k < NC ? k : 0] : -1;
-            }
-#pragma unroll
-            for (int q = 0; q < 8; ++q)
-                if (items[q] >= 0 && atomicSub(&indeg[items[q]], 1) == 1) mask |= 1u << q;
-            wave_push8(items, mask, lane, list_out, count_out);
-        }
+        // the next launch processes the neighbours that became ready, and sees this launch's
+        // lambda updates
+        const bool ready = t >= 0 && atomicSub(&indeg[t], 1) == 1;
+        adj_push(ready ? t : -1, lane, list_out, count_out);
     }
 }
 
@@ -389,7 +397,8 @@ template <int M, typename T> void adjoint_run(hfm_plan *p, const std::vector<lon
     const T *u = static_cast<const T *>(p->u);
     HIPCHECK(hipMemsetAsync(p->adj_indeg, 0, sizeof(int) * np, s));
     hipLaunchKernelGGL((k_adj_prepare<M, T>), dim3(nb), dim3(256), 0, s, p->g, p->sd<T>(), u, p->fixedmask,
-                       p->adj_indeg, p->adj_lambda, p->adj_sens, np);
+                       p->adj_indeg, p->adj_lambda, p->adj_sens, p->adj_ecnt, p->adj_eidx, p->adj_var, p->adj_ew,
+                       np);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(p->adj_tp, tp.data(), tp.size() * sizeof(long long), hipMemcpyHostToDevice, s));
     HIPCHECK(hipMemcpyAsync(p->adj_tw, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice, s));
@@ -402,15 +411,18 @@ template <int M, typename T> void adjoint_run(hfm_plan *p, const std::vector<lon
     hipLaunchKernelGGL(k_adj_frontier, dim3(nb), dim3(256), 0, s, p->g, p->adj_indeg, np, p->adj_list[0],
                        &p->adj_counts[0]);
     HIPCHECK(hipGetLastError());
-    const unsigned grid = (unsigned)std::min<long long>((np + 255) / 256, 1024);
+    // a level rarely lists more than a few thousand points: 256 blocks (1024 waves) cover 65536
+    // entries a pass, and a small grid starts and drains quickly
+    const unsigned grid = (unsigned)std::min<long long>((np + 255) / 256, 256);
     long long k = 0;
     const long long max_rounds = np + 16;
     while (true) {
         for (int b = 0; b < 8; ++b, ++k) {
-            hipLaunchKernelGGL((k_adj_round<M, T>), dim3(grid), dim3(256), 0, s, p->g, p->sd<T>(), u, p->fixedmask,
-                               p->adj_indeg, p->adj_lambda, p->adj_sens, p->adj_list[k & 1], &p->adj_counts[k % 3],
-                               p->adj_list[(k + 1) & 1], &p->adj_counts[(k + 1) % 3], &p->adj_counts[(k + 2) % 3],
-                               p->adj_seed_idx, p->adj_seed_w, &p->adj_counts[3], max_seeds);
+            hipLaunchKernelGGL((k_adj_round<M, T>), dim3(grid), dim3(256), 0, s, p->adj_indeg, p->adj_lambda,
+                               p->adj_sens, p->adj_ecnt, p->adj_eidx, p->adj_var, p->adj_ew, np,
+                               p->adj_list[k & 1], &p->adj_counts[k % 3], p->adj_list[(k + 1) & 1],
+                               &p->adj_counts[(k + 1) % 3], &p->adj_counts[(k + 2) % 3], p->adj_seed_idx,
+                               p->adj_seed_w, &p->adj_counts[3], max_seeds);
             HIPCHECK(hipGetLastError());
         }
         HIPCHECK(hipMemcpyAsync(p->h_count, &p->adj_counts[k % 3], sizeof(int), hipMemcpyDeviceToHost, s));
@@ -557,6 +569,12 @@ int hfm_plan_adjoint_impl(hfm_plan *p, int ntargets, const double *targets, cons
             p->adj_lambda = dalloc<double>(p->np);
             p->adj_sens = dalloc<double>(p->np);
             p->adj_indeg = dalloc<int>(p->np);
+            p->adj_ecnt = dalloc<int>(p->np);
+            p->adj_var = dalloc<double>(p->np);
+            // every point's active edges, one slot per difference of the model (Elastica2: 27 x
+            // 12 bytes a point, 0.77 GB at 192 x 192 x 64)
+            p->adj_eidx = dalloc<unsigned>((size_t)p->ndiff * p->np);
+            p->adj_ew = dalloc<double>((size_t)p->ndiff * p->np);
         }
         ensure_lists(p);
         const int cap = std::max(max_seeds, 1);

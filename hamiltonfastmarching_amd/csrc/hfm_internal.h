@@ -96,6 +96,11 @@ struct hfm_plan {
 
     // adjoint scratch (allocated on the first hfm_plan_adjoint call)
     double *adj_lambda = nullptr, *adj_sens = nullptr;
+    // each point's active edges, computed once per adjoint: count, var, and up to ADJ_EDGES
+    // (neighbour tiled index, w'/S) pairs, slot-major (slot k of point p at k * np + p)
+    int *adj_ecnt = nullptr;
+    unsigned *adj_eidx = nullptr;
+    double *adj_var = nullptr, *adj_ew = nullptr;
     int *adj_indeg = nullptr, *adj_counts = nullptr;
     long long *adj_list[2] = {nullptr, nullptr};
     long long *adj_seed_idx = nullptr, *adj_tp = nullptr;

@@ -101,6 +101,10 @@ void plan_free(hfm_plan *p) {
     dfree(p->red);
     dfree(p->adj_lambda);
     dfree(p->adj_sens);
+    dfree(p->adj_ecnt);
+    dfree(p->adj_eidx);
+    dfree(p->adj_var);
+    dfree(p->adj_ew);
     dfree(p->adj_indeg);
     dfree(p->adj_counts);
     dfree(p->adj_list[0]);
