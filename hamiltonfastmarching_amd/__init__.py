@@ -69,9 +69,18 @@ class HFMIO:
             raise PyExp(self._err())
 
     def set_array(self, key: str, value) -> None:
-        a = np.ascontiguousarray(value, dtype=np.float64)
-        shape = (ctypes.c_int64 * max(a.ndim, 1))(*a.shape)
-        if self._L.hfm_io_set_array(self._io, key.encode(), _dptr(a), a.ndim, shape):
+        """Copy an array into the IO store (FP64, as the reference stores every numerical field);
+        float32 arrays go through the FP32 setter, without an FP64 copy on the Python side."""
+        if isinstance(value, np.ndarray) and value.dtype == np.float32:
+            a = np.ascontiguousarray(value)
+            shape = (ctypes.c_int64 * max(a.ndim, 1))(*a.shape)
+            rc = self._L.hfm_io_set_array_f32(self._io, key.encode(), a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                              a.ndim, shape)
+        else:
+            a = np.ascontiguousarray(value, dtype=np.float64)
+            shape = (ctypes.c_int64 * max(a.ndim, 1))(*a.shape)
+            rc = self._L.hfm_io_set_array(self._io, key.encode(), _dptr(a), a.ndim, shape)
+        if rc:
             raise PyExp(self._err())
 
     def __setitem__(self, key, value):

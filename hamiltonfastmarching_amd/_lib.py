@@ -50,6 +50,7 @@ SIGNATURES = [
     ("hfm_io_set_string", c_int, [c_void_p, ctypes.c_char_p, ctypes.c_char_p]),
     ("hfm_io_set_scalar", c_int, [c_void_p, ctypes.c_char_p, c_double]),
     ("hfm_io_set_array", c_int, [c_void_p, ctypes.c_char_p, P(c_double), c_int, P(c_int64)]),
+    ("hfm_io_set_array_f32", c_int, [c_void_p, ctypes.c_char_p, P(ctypes.c_float), c_int, P(c_int64)]),
     ("hfm_io_has_field", c_int, [c_void_p, ctypes.c_char_p]),
     ("hfm_io_erase_field", c_int, [c_void_p, ctypes.c_char_p]),
     ("hfm_io_field_kind", c_int, [c_void_p, ctypes.c_char_p]),

@@ -1088,6 +1088,24 @@ int hfm_io_set_array(hfm_io *io, const char *key, const double *data, int ndim, 
     return 0;
 }
 
+int hfm_io_set_array_f32(hfm_io *io, const char *key, const float *data, int ndim, const int64_t *shape) {
+    if (!io || !key || ndim < 0 || (ndim > 0 && !shape)) return 1;
+    Field f;
+    int64_t n = 1;
+    for (int i = 0; i < ndim; ++i) {
+        if (shape[i] < 0) {
+            io->err = "negative dimension";
+            return 1;
+        }
+        f.dims.push_back(shape[i]);
+        n *= shape[i];
+    }
+    if (n > 0 && !data) return 1;
+    f.data.assign(data, data + n);  // widened to FP64
+    io->fields[key] = std::move(f);
+    return 0;
+}
+
 int hfm_io_has_field(const hfm_io *io, const char *key) { return (io && key && io->has(key)) ? 1 : 0; }
 
 int hfm_io_erase_field(hfm_io *io, const char *key) {

@@ -51,6 +51,9 @@ int hfm_io_set_string(hfm_io *io, const char *key, const char *value);
 int hfm_io_set_scalar(hfm_io *io, const char *key, double value);
 /* set_array (IO.h:105 SetArray; main.cpp:41): row-major data of the given shape as stored */
 int hfm_io_set_array(hfm_io *io, const char *key, const double *data, int ndim, const int64_t *shape);
+/* set_array from FP32 data (the same key store: values are widened to FP64, the reference's
+ * only numerical type, RawElement in BaseIO.hxx:10-33) */
+int hfm_io_set_array_f32(hfm_io *io, const char *key, const float *data, int ndim, const int64_t *shape);
 /* has_field (IO.h:92 HasField; main.cpp:43): 1 if present, 0 otherwise */
 int hfm_io_has_field(const hfm_io *io, const char *key);
 /* erase_field (main.cpp:45) */

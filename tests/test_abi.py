@@ -15,7 +15,7 @@ def declared_symbols():
     with open(os.path.join(ROOT, "include", "hfm_amd.h")) as f:
         text = f.read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(hfm_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(hfm_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -63,6 +63,16 @@ def test_run_rejects_bad_inputs_before_device_work(params, msg):
     """Input validation happens on the host, with the reference's messages, before any HIP call."""
     with pytest.raises(hfm.PyExp, match=msg):
         hfm.run(params)
+
+
+def test_io_fp32_array_setter():
+    """float32 arrays are stored widened to FP64 (the reference's only numerical type)."""
+    io = hfm.HFMIO(default_ordering="")
+    a = (np.arange(24, dtype=np.float32) / 7).reshape(2, 3, 4)
+    io.set_array("metric", a)
+    got = io.get_array("metric")
+    assert got.dtype == np.float64 and got.shape == a.shape
+    assert np.array_equal(got, a.astype(np.float64))
 
 
 def test_last_error_empty_initially():
