@@ -20,6 +20,7 @@
 // Unused / defaulted keys are reported like BaseIO::UsageReport (BaseIO.hxx:64-104).
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <limits>
@@ -316,10 +317,63 @@ FieldView grid_field_view(hfm_io *io, const std::string &key, const std::vector<
 // `values` are replaced; spreadedSeeds / spreadedSeedValues are exported.
 template <int D>
 void spread_seeds_d(hfm_io *io, const std::vector<int64_t> &dims, const double *h, const std::vector<double> &origin,
-                    const double *metric, bool is_field, bool dual, double radius,
+                    const double *metric, bool is_field, bool dual, double radius, int step_width,
                     std::vector<double> &seeds, std::vector<double> &values) {
     using Sym = hfm::SymPacked<D>;
     constexpr int K = Sym::K;
+    // Reversed offsets of a grid point y (HFM_StencilRecompute.hxx:81-124, read by
+    // StaticFactoring.hxx:333-340): the offsets e, sorted and unique, such that the point y - e has
+    // a nonzero-weight difference +-e_k equal to e (so that y = (y - e) + e), both points in the
+    // grid.  The stencils are the FP64 Selling ones the device computes, evaluated here for the
+    // points within step_width (MaxStencilWidth, a bound on every |e_i|) of y.
+    std::map<int64_t, std::pair<std::array<int8_t, 3 * K>, std::array<double, K>>> stencil_cache;
+    double invhh[6];
+    for (int i = 0; i < D; ++i)
+        for (int j = 0; j <= i; ++j) invhh[i * (i + 1) / 2 + j] = 1. / (h[i] * h[j]);
+    auto stencil_of = [&](const int64_t *x) -> const std::pair<std::array<int8_t, 3 * K>, std::array<double, K>> & {
+        int64_t lin = 0;
+        for (int i = 0; i < D; ++i) lin = lin * dims[i] + x[i];
+        auto it = stencil_cache.find(lin);
+        if (it != stencil_cache.end()) return it->second;
+        std::pair<std::array<int8_t, 3 * K>, std::array<double, K>> st{};
+        int8_t o8[K][D];
+        int mx = 0;
+        hfm::riemann_point_stencil<D>(metric + (is_field ? lin * K : 0), dual, invhh, o8, st.second.data(), &mx);
+        for (int k = 0; k < K; ++k)
+            for (int c = 0; c < D; ++c) st.first[3 * k + c] = o8[k][c];
+        return stencil_cache.emplace(lin, st).first->second;
+    };
+    auto reversed_offsets = [&](const int64_t *y) {
+        std::set<std::array<int, D>> offs;
+        int64_t x[D];
+        const int64_t side = 2 * (int64_t)step_width + 1;
+        int64_t box = 1;
+        for (int i = 0; i < D; ++i) box *= side;
+        for (int64_t b = 0; b < box; ++b) {
+            int64_t r = b;
+            bool ok = true;
+            for (int i = D - 1; i >= 0; --i) {
+                x[i] = y[i] + (r % side) - step_width;
+                r /= side;
+                ok = ok && x[i] >= 0 && x[i] < dims[i];
+            }
+            if (!ok) continue;
+            const auto &st = stencil_of(x);
+            for (int k = 0; k < K; ++k) {
+                if (st.second[k] == 0.) continue;
+                for (int sg = 1; sg >= -1; sg -= 2) {
+                    std::array<int, D> e;
+                    bool hit = true;
+                    for (int i = 0; i < D; ++i) {
+                        e[i] = (int)(int8_t)(sg * st.first[3 * k + i]);
+                        hit = hit && x[i] + e[i] == y[i];
+                    }
+                    if (hit) offs.insert(e);
+                }
+            }
+        }
+        return offs;
+    };
     auto org = [&](int i) { return origin.empty() ? 0. : origin[i]; };
     auto in_domain = [&](const int64_t *idx) {
         for (int i = 0; i < D; ++i)
@@ -420,6 +474,20 @@ void spread_seeds_d(hfm_io *io, const std::vector<int64_t> &dims, const double *
                 r /= side;
             }
             insert(idx, radius);
+        }
+        if (step_width <= 0) continue;
+        // negative seedRadius: also every point one step of the reversed stencil away from a point
+        // inserted above, visited in sorted index order and sorted offset order.  (An inserted
+        // index outside the grid has no stencil: the reference would read its reversed offsets at
+        // an index computed from outside the grid; here it is skipped.)
+        const std::set<std::vector<int64_t>> seen2 = seen;
+        for (const auto &y : seen2) {
+            if (!in_domain(y.data())) continue;
+            for (const auto &e : reversed_offsets(y.data())) {
+                int64_t idx[D];
+                for (int i = 0; i < D; ++i) idx[i] = y[i] + e[i];
+                insert(idx, std::numeric_limits<double>::infinity());
+            }
         }
     }
     const size_t n = newVals.size();
@@ -753,12 +821,13 @@ void run_impl(hfm_io *io) {
             const double seedRadius = io->scalar("seedRadius", 0.);
             if (seedRadius != 0.) {
                 if (!riemann) throw IOError{"Equation factoring error : no guess"};
-                if (seedRadius < 0.)
-                    throw IOError{"Error : seedRadius < 0 (spreading along the reversed stencils) is not supported"};
+                // negative: also one step of the reversed stencil (StaticFactoring.hxx:285-287)
+                const int step = seedRadius < 0. ? std::max(1, (int)st.max_stencil_width) : 0;
+                const double r = std::abs(seedRadius);
                 if (dim == 2)
-                    spread_seeds_d<2>(io, dims, h, origin, field.data(), is_field, hasDual, seedRadius, seeds, sv);
+                    spread_seeds_d<2>(io, dims, h, origin, field.data(), is_field, hasDual, r, step, seeds, sv);
                 else
-                    spread_seeds_d<3>(io, dims, h, origin, field.data(), is_field, hasDual, seedRadius, seeds, sv);
+                    spread_seeds_d<3>(io, dims, h, origin, field.data(), is_field, hasDual, r, step, seeds, sv);
             }
         }
         if (bundle && io->has("seeds_Unoriented")) {

@@ -274,6 +274,13 @@ SEED_CASES = {
                              {"seeds": np.array([[0.43, 0.61]]), "seedRadius": 1.7, "dual": True}),
     "riemann3_radius": (lambda: syn.riemann3(16, seed=7),
                         {"seeds": np.array([[0.47, 0.52, 0.55]]), "seedRadius": 1.5}),
+    # negative seedRadius: also the points one step of the reversed stencil away
+    # (StaticFactoring.hxx:285-287,333-340)
+    "riemann2_radius_neg": (lambda: syn.riemann2(40, seed=6),
+                            {"seeds": np.array([[0.45, 0.5], [0.62, 0.41]]), "seedValues": np.array([0.0, 0.03]),
+                             "seedRadius": -1.5}),
+    "riemann3_radius_neg": (lambda: syn.riemann3(16, seed=7),
+                            {"seeds": np.array([[0.5, 0.47, 0.53]]), "seedRadius": -1.2}),
     # unoriented seeds and tips of the curvature models (StaticFactoring.hxx:351-380,
     # HFMInterface.hxx:498-522)
     "rsf2_unoriented": (lambda: syn.rsf2(24, 16, seed=8),
