@@ -13,6 +13,8 @@
 
 #include <hip/hip_ext.h>
 
+#include <algorithm>
+
 #include "hfm_device.h"
 #include "selling.h"
 #include "hfm_tile.h"
@@ -315,7 +317,8 @@ __global__ void k_set_seeds(const long long *__restrict__ seed_p, const double *
     atomicOr(&fixedmask[p >> 6], 1ull << (p & 63));
 }
 
-// Seed tiles and their dependents are the first pending tiles, keyed by the seed value.
+// Seed tiles and their dependents are the first pending tiles, keyed by the seed value: one
+// wave per seed (block i takes seeds i, i + grid, ...).
 __global__ __launch_bounds__(64) void k_seed_activate(const long long *__restrict__ seed_p,
                                                       const double *__restrict__ seed_v, int nseeds,
                                                       const int *__restrict__ dep_start, const int *__restrict__ deps,
@@ -323,7 +326,7 @@ __global__ __launch_bounds__(64) void k_seed_activate(const long long *__restric
                                                       unsigned *__restrict__ pend_slots) {
     const int lane = threadIdx.x;
     unsigned wmin = KEY_NONE;
-    for (int i = 0; i < nseeds; ++i) {
+    for (int i = blockIdx.x; i < nseeds; i += gridDim.x) {
         const int tile = (int)(seed_p[i] >> 6);
         const unsigned key = key_of((float)seed_v[i]);
         unsigned m = mark_pending(lane == 0 ? tile : -1, key, keys, tmax);
@@ -728,8 +731,8 @@ template hipError_t launch_init<double>(const Geom &, double *, unsigned long lo
 
 hipError_t launch_seed_activate(const long long *seed_p, const double *seed_v, int nseeds, const int *dep_start,
                                 const int *deps, unsigned *keys, const unsigned *tmax, unsigned *pend, hipStream_t s) {
-    hipLaunchKernelGGL(k_seed_activate, dim3(1), dim3(64), 0, s, seed_p, seed_v, nseeds, dep_start, deps, keys, tmax,
-                       pend);
+    hipLaunchKernelGGL(k_seed_activate, dim3((unsigned)std::min(std::max(nseeds, 1), 1024)), dim3(64), 0, s, seed_p,
+                       seed_v, nseeds, dep_start, deps, keys, tmax, pend);
     return hipGetLastError();
 }
 

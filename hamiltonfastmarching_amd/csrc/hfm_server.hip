@@ -369,8 +369,13 @@ void setup_group(hfm_server *S, std::vector<SetupItem> &items, int which) {
         HIPCHECK(launch_riemann_stencil<float>(r->g, metric, r->metric_is_field, r->dual, ih, sl->offs, sl->w, r->np,
                                                sl->status, red, s));
         StencilData<float> sd{sl->offs, sl->w, nullptr, nullptr, nullptr, r->np};
-        HIPCHECK(launch_tile_deps<float>(r->model, r->g, sd, 0, sl->dep_cnt, nullptr, nullptr, nullptr, nullptr, nullptr, s));
-        HIPCHECK(launch_exclusive_scan(sl->dep_cnt, sl->dep_start, r->g.ntiles, s));
+        // The tile graph's first pass keeps each tile's sources (<= 64: 64 ints a tile) in `u` and
+        // their counts in `tmax`, and the multi-block scan works in `fixedmask`: all three are
+        // written by k_init only after the second pass has read them.
+        HIPCHECK(launch_tile_deps<float>(r->model, r->g, sd, 0, sl->dep_cnt, nullptr, nullptr, nullptr,
+                                         reinterpret_cast<int *>(sl->u), reinterpret_cast<int *>(sl->tmax), s));
+        HIPCHECK(launch_exclusive_scan(sl->dep_cnt, sl->dep_start, r->g.ntiles, s,
+                                       reinterpret_cast<int *>(sl->fixedmask)));
         HIPCHECK(hipMemcpyAsync(S->h_status[which] + 2 * g, sl->status, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
         HIPCHECK(hipMemcpyAsync(S->h_red[which] + (size_t)g * RED_SLOTS, red, RED_SLOTS * sizeof(RedSlot),
                                 hipMemcpyDeviceToHost, s));
@@ -395,7 +400,7 @@ void setup_group(hfm_server *S, std::vector<SetupItem> &items, int which) {
         HIPCHECK(hipMallocAsync((void **)&sl->deps, sizeof(int) * std::max(total, 1), s));
         StencilData<float> sd{sl->offs, sl->w, nullptr, nullptr, nullptr, r->np};
         HIPCHECK(launch_tile_deps<float>(r->model, r->g, sd, 1, sl->dep_cnt, sl->dep_start, sl->dep_fill, sl->deps,
-                                         nullptr, nullptr, s));
+                                         reinterpret_cast<int *>(sl->u), reinterpret_cast<int *>(sl->tmax), s));
         const int nseeds = (int)r->seed_tp.size();
         HIPCHECK(launch_init<float>(r->g, sl->u, sl->fixedmask, sl->keys, sl->tmax, r->np, sl->seed_p, sl->seed_v,
                                     nseeds, s));
