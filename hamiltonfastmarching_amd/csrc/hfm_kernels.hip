@@ -18,6 +18,7 @@
 #include "hfm_device.h"
 #include "selling.h"
 #include "hfm_tile.h"
+#include "hfm_setup_dev.h"
 
 namespace hfm {
 
@@ -42,16 +43,12 @@ __global__ __launch_bounds__(256) void k_riemann_stencil(Geom g, const double *_
             w[k * np + p] = T(0);
         }
     } else {
-        const long long lin = ((long long)c[0] * g.n[1] + c[1]) * g.n[2] + c[2];
-        double mc[K];
-        const double *src = metric + (metric_is_field ? lin * K : 0);
-        for (int k = 0; k < K; ++k) mc[k] = src[k];
+        long long lin;
         int8_t o8[K][D];
         double wd[K];
-        int mx = 0;
-        const int st = riemann_point_stencil<D>(mc, dual != 0, ih.v, o8, wd, &mx);
+        const int st = riemann_point<D>(g, metric, metric_is_field, dual, ih, c, o8, wd, lin);
         for (int k = 0; k < K; ++k) {
-            offs[k * np + p] = pack_offset(o8[k][0], D > 1 ? o8[k][D > 1 ? 1 : 0] : 0, D > 2 ? o8[k][D > 2 ? 2 : 0] : 0);
+            offs[k * np + p] = packed_offset<D>(o8, k);
             w[k * np + p] = (T)wd[k];
         }
         if (st != SELLING_OK) {
@@ -88,17 +85,11 @@ __global__ __launch_bounds__(256) void k_riemann_stencil(Geom g, const double *_
 // points x of T (nonzero weights only).  deps(S) = tiles T != S having S as a source: the tiles
 // to re-run when S changes.  Pass 0 counts, pass 1 fills (CSR offsets from an exclusive scan).
 // ---------------------------------------------------------------------------------------------
-// A tile has at most 64 points x 12 sides = 768 distinct source tiles, so 1024 slots always leave
-// a free slot for the linear probing.
-constexpr int HASH_SLOTS = 1024;
-
 template <int ModelT, typename T>
 __global__ __launch_bounds__(64) void k_tile_deps(Geom g, StencilData<T> sd, int pass, int *__restrict__ dep_count,
                                                   const int *__restrict__ dep_start, int *__restrict__ dep_fill,
                                                   int *__restrict__ deps, int *__restrict__ srcbuf,
                                                   int *__restrict__ srcn) {
-    using S = ModelShape<ModelT>;
-    constexpr int NC = S::NFWD + S::NSYM;
     __shared__ int set[HASH_SLOTS];
     const int tile = blockIdx.x;
     const int lane = threadIdx.x;
@@ -115,58 +106,7 @@ __global__ __launch_bounds__(64) void k_tile_deps(Geom g, StencilData<T> sd, int
             return;
         }
     }
-    for (int i = lane; i < HASH_SLOTS; i += 64) set[i] = -1;
-    __syncthreads();
-    const long long p = (long long)tile * 64 + lane;
-    int c[3];
-    tiled_coords(g, tile, lane, c);
-    if (in_grid(g, c)) {
-        LaneStencil<ModelT, T> st;
-        st.load(g, sd, p, c);
-        for (int k = 0; k < NC; ++k) {
-            if (st.w[k] == T(0)) continue;
-            int o0, o1, o2;
-            unpack_offset(st.off[k], o0, o1, o2);
-            const int nsides = (k >= S::NFWD) ? 2 : 1;
-            for (int s = 0; s < nsides; ++s) {
-                const int sg = s == 0 ? 1 : -1;
-                int r[3];
-                if (!shift_coords_m<ModelT>(g, c, sg * o0, sg * o1, sg * o2, r)) continue;
-                const int t = tile_of(g, r);
-                if (t == tile) continue;
-                unsigned h = ((unsigned)t * 2654435761u) & (HASH_SLOTS - 1);
-                for (int probe = 0; probe < HASH_SLOTS; ++probe) {
-                    const int prev = atomicCAS(&set[h], -1, t);
-                    if (prev == -1 || prev == t) break;
-                    h = (h + 1) & (HASH_SLOTS - 1);
-                }
-            }
-        }
-    }
-    __syncthreads();
-    // Compact the set to the front of the table, so the global atomics below take
-    // ceil(sources / 64) passes with every lane busy instead of one pass per 64 slots.
-    int mine[HASH_SLOTS / 64];
-    int nmine = 0;
-#pragma unroll
-    for (int j = 0; j < HASH_SLOTS / 64; ++j) {
-        const int v = set[j * 64 + lane];
-        mine[j] = v;
-        nmine += v >= 0;
-    }
-    int incl = nmine;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int t = __shfl_up(incl, o);
-        if (lane >= o) incl += t;
-    }
-    const int nsrc = __shfl(incl, 63);
-    __syncthreads();
-    int pos = incl - nmine;
-#pragma unroll
-    for (int j = 0; j < HASH_SLOTS / 64; ++j)
-        if (mine[j] >= 0) set[pos++] = mine[j];
-    __syncthreads();
+    const int nsrc = tile_sources<ModelT, T>(g, sd, tile, lane, set);
     for (int i = lane; i < nsrc; i += 64) {
         const int src = set[i];
         if (pass == 0) {
@@ -180,48 +120,6 @@ __global__ __launch_bounds__(64) void k_tile_deps(Geom g, StencilData<T> sd, int
         if (lane == 0) srcn[tile] = nsrc;
         if (nsrc <= 64 && lane < nsrc) srcbuf[(long long)tile * 64 + lane] = set[lane];
     }
-}
-
-// Exclusive scans of int counts.  The unit of work is a chunk of 4096 consecutive counts taken
-// by a block of 1024 threads (four consecutive counts per thread, coalesced), scanned with wave
-// shuffles plus one LDS pass over the 16 wave totals.
-constexpr int SCAN_CHUNK = 4096;
-
-// Scans chunk [base, base + 4096) of cnt into start with the given carry; returns the chunk total
-// (on every thread).  Contains barriers: every thread of the block calls it.
-__device__ inline int scan_chunk(const int *__restrict__ cnt, int *__restrict__ start, int n, int base, int carry,
-                                 int *wsum) {
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int i0 = base + tid * 4;
-    int v[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = i0 + k < n ? cnt[i0 + k] : 0;
-    const int tsum = v[0] + v[1] + v[2] + v[3];
-    int incl = tsum;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int t = __shfl_up(incl, o);
-        if (lane >= o) incl += t;
-    }
-    if (lane == 63) wsum[wid] = incl;
-    __syncthreads();
-    int wpre = 0, total = 0;
-#pragma unroll
-    for (int w = 0; w < 16; ++w) {
-        const int x = wsum[w];
-        wpre += w < wid ? x : 0;
-        total += x;
-    }
-    if (start) {
-        int run = carry + wpre + incl - tsum;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (i0 + k < n) start[i0 + k] = run;
-            run += v[k];
-        }
-    }
-    __syncthreads();  // wsum is reused by the next chunk
-    return total;
 }
 
 // One block of 1024 sweeps the whole array chunk by chunk, carrying the running total; writes

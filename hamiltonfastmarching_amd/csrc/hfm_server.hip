@@ -27,6 +27,7 @@
 
 #include "hfm_internal.h"
 #include "hfm_kernels.h"
+#include "hfm_setup_dev.h"
 #include "hfm_tile.h"
 #include "selling.h"
 
@@ -49,6 +50,9 @@ struct SolveDesc {
     float *bound;                 // 2 band bounds + 2 band widths
     int *count;                   // 2 tiles listed per round
     unsigned long long *visits;   // tile visits of the solve so far (tiles listed by its rounds)
+    const unsigned *seed_key;     // smallest key the seed marks left pending
+    double *values_dev;           // row-major FP64 export of the values
+    int joining;                  // 1 in the batch the solve joins: k_mjoin sets its scheduling state
 };
 
 // Jacobi sweeps per tile visit in the serving update kernel (an unconverged tile goes back to
@@ -123,37 +127,338 @@ __global__ __launch_bounds__(256) void k_mfim(const SolveDesc *__restrict__ desc
     }
 }
 
-// Setup-time clears of one solve in one launch: the stencil status words, the reduction slots,
-// the tile-graph counters and the seed key word.
-__global__ __launch_bounds__(256) void k_setup_clear(int *__restrict__ status, RedSlot *__restrict__ red,
-                                                     int *__restrict__ dep_cnt, int *__restrict__ dep_fill,
-                                                     unsigned *__restrict__ seed_key, int ntiles) {
-    const int i0 = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
-    if (i0 == 0) {
-        status[0] = 0;
-        status[1] = 0x7fffffff;
-        *seed_key = KEY_NONE;
+// ---------------------------------------------------------------------------------------------
+// Setup of a group of admitted requests: each kernel below covers every request of the group in
+// one launch.  The per-tile kernels run one 64-thread block per tile of each request (request j
+// owns blocks [jobs[j].tile_base, jobs[j + 1].tile_base)), the seed kernel one block per seed
+// (seed_base); the clear and scan kernels take the request from blockIdx.y / blockIdx.x.
+// ---------------------------------------------------------------------------------------------
+struct SetupReport {
+    int status[2];   // Selling status (largest code) and smallest failing row-major index
+    int total;       // dependency-list entries
+    int npts;        // in-grid points with a nonzero stencil
+    double tau_sum;  // sum of their one-step times 1/sqrt(sum_k w_k)
+    double pad_;
+};
+
+// Tiles per block of the group scan (four per thread, 256 threads), and the sums the first scan
+// kernel leaves per chunk.
+constexpr int GSCAN_CHUNK = 1024;
+struct ChunkSum {
+    int total;   // dependency counts
+    int npts;    // points with a nonzero stencil
+    double tau;  // their one-step times
+};
+
+struct SetupJob {
+    Geom g;
+    int metric_is_field, dual, nseeds;
+    long long np;
+    InvHH ih;
+    const double *metric;
+    uint32_t *offs;
+    float *w, *u;
+    unsigned long long *fixedmask;
+    unsigned *keys, *tmax;
+    int *dep_cnt, *dep_start, *dep_fill, *deps;
+    const long long *seed_p;
+    const double *seed_v;
+    unsigned *seed_key;
+    SetupReport *rep;
+    ChunkSum *chunks;  // ceil(ntiles / GSCAN_CHUNK)
+    int tile_base, seed_base, chunk_base;
+};
+
+// Request of block b: the last one whose first block is <= b (requests without blocks in this
+// launch have the same base as the next one and are passed over).
+enum { BY_TILE, BY_SEED, BY_CHUNK };
+__device__ inline int job_of(const SetupJob *jobs, int njobs, int b, int by) {
+    auto base = [&](int j) { return by == BY_TILE ? jobs[j].tile_base : by == BY_SEED ? jobs[j].seed_base : jobs[j].chunk_base; };
+    int j = 0;
+    while (j + 1 < njobs && base(j + 1) <= b) ++j;
+    return j;
+}
+
+// Sums over a 256-thread block, in a fixed order (every thread calls them; result on every thread).
+__device__ inline int block_sum_i(int v, int *lds4) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) lds4[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return lds4[0] + lds4[1] + lds4[2] + lds4[3];
+}
+__device__ inline double block_sum_d(double v, double *lds4) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) lds4[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return (lds4[0] + lds4[1]) + (lds4[2] + lds4[3]);
+}
+
+// Report words, seed key and the tile-graph counters of every request (blockIdx.y).
+__global__ __launch_bounds__(256) void k_gsetup_clear(const SetupJob *__restrict__ jobs) {
+    const SetupJob &J = jobs[blockIdx.y];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        J.rep->status[0] = 0;
+        J.rep->status[1] = 0x7fffffff;
+        *J.seed_key = KEY_NONE;
     }
-    int *rw = reinterpret_cast<int *>(red);
-    for (int i = i0; i < RED_SLOTS * (int)(sizeof(RedSlot) / sizeof(int)); i += stride) rw[i] = 0;
-    for (int i = i0; i < ntiles; i += stride) {
-        dep_cnt[i] = 0;
-        dep_fill[i] = 0;
+    const int nt = J.g.ntiles;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nt; i += gridDim.x * blockDim.x) {
+        J.dep_cnt[i] = 0;
+        J.dep_fill[i] = 0;
     }
 }
 
-// Scheduling state of a solve joining the running set at a round of the given parity: the seed
-// key (smallest key the seed marks left pending) becomes the pending minimum of that round.
-__global__ __launch_bounds__(64) void k_join_state(unsigned *__restrict__ pend, int *__restrict__ count,
-                                                   unsigned long long *__restrict__ visits, float *__restrict__ bound,
-                                                   const unsigned *__restrict__ seed_key, int parity, float delta) {
+// Stencils of one tile's points (FP64 Selling, as k_riemann_stencil) and, from the same
+// registers, the tile's source tiles (as the first pass of k_tile_deps): one count per source,
+// and the list itself kept for the fill when it has at most 64 entries (64 ints a tile in u, its
+// length in tmax).  The tile's sum and count of one-step times go to fixedmask and keys for the
+// scan kernel.  k_gsetup_fill_init overwrites all four after they have been read.
+template <int D>
+__device__ inline void gsetup_stencil_tile(const SetupJob &J, int tile, int lane, int *set) {
+    constexpr int K = D * (D + 1) / 2;
+    constexpr int M = D == 2 ? MODEL_RIEMANN2 : MODEL_RIEMANN3;
+    const Geom &g = J.g;
+    const long long np = J.np, p = (long long)tile * 64 + lane;
+    int c[3];
+    tiled_coords(g, tile, lane, c);
+    double tau = 0.;
+    int cnt = 0;
+    src_set_clear(set, lane);
+    __syncthreads();
+    if (!in_grid(g, c)) {
+        for (int k = 0; k < K; ++k) {
+            J.offs[k * np + p] = 0;
+            J.w[k * np + p] = 0.f;
+        }
+    } else {
+        long long lin;
+        int8_t o8[K][D];
+        double wd[K];
+        const int st = riemann_point<D>(g, J.metric, J.metric_is_field, J.dual, J.ih, c, o8, wd, lin);
+        double sw = 0.;
+        for (int k = 0; k < K; ++k) {
+            const float wk = (float)wd[k];
+            J.offs[k * np + p] = packed_offset<D>(o8, k);
+            J.w[k * np + p] = wk;
+            sw += wd[k];
+            if (wk != 0.f) src_set_add_sides<M>(g, c, tile, o8[k][0], o8[k][1], D > 2 ? o8[k][D > 2 ? 2 : 0] : 0, 2, set);
+        }
+        if (st != SELLING_OK) {
+            atomicMax(&J.rep->status[0], st);
+            atomicMin(&J.rep->status[1], (int)lin);
+        }
+        if (sw > 0.) {
+            tau = 1. / sqrt(sw);
+            cnt = 1;
+        }
+    }
+    __syncthreads();
+    const int nsrc = src_set_compact(set, lane);
+    for (int i = lane; i < nsrc; i += 64) atomicAdd(&J.dep_cnt[set[i]], 1);
+    if (nsrc <= 64 && lane < nsrc) reinterpret_cast<int *>(J.u)[p] = set[lane];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        tau += __shfl_xor(tau, o);
+        cnt += __shfl_xor(cnt, o);
+    }
+    if (lane == 0) {
+        J.tmax[tile] = (unsigned)nsrc;
+        J.fixedmask[tile] = (unsigned long long)__double_as_longlong(tau);
+        J.keys[tile] = (unsigned)cnt;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_gsetup_stencil(const SetupJob *__restrict__ jobs, int njobs) {
+    __shared__ int set[HASH_SLOTS];
+    const SetupJob &J = jobs[job_of(jobs, njobs, blockIdx.x, BY_TILE)];
+    const int tile = blockIdx.x - J.tile_base;
+    if (J.g.dim == 3) gsetup_stencil_tile<3>(J, tile, threadIdx.x, set);
+    else gsetup_stencil_tile<2>(J, tile, threadIdx.x, set);
+}
+
+// Exclusive scan of every request's dependency counts into dep_start (dep_start[ntiles] =
+// total) in two launches over chunks of GSCAN_CHUNK tiles, one 256-thread block per chunk of
+// each request (chunk_base: the request's first block).  The counts are final when the first
+// launch starts, so no block waits for another: the first launch sums each chunk (and its
+// one-step times), the second takes its carry as the sum of the totals of the request's earlier
+// chunks and scans its own chunk.  The last chunk's block writes the request's report.
+__global__ __launch_bounds__(256) void k_gsetup_chunks(const SetupJob *__restrict__ jobs, int njobs) {
+    __shared__ int li[4];
+    __shared__ double ld[4];
+    const SetupJob &J = jobs[job_of(jobs, njobs, blockIdx.x, BY_CHUNK)];
+    const int c = blockIdx.x - J.chunk_base;
+    const int n = J.g.ntiles;
+    const int i0 = c * GSCAN_CHUNK + threadIdx.x * 4;
+    int sum = 0, npts = 0;
+    double tau = 0.;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (i0 + k < n) {
+            sum += J.dep_cnt[i0 + k];
+            npts += (int)J.keys[i0 + k];
+            tau += __longlong_as_double((long long)J.fixedmask[i0 + k]);
+        }
+    }
+    sum = block_sum_i(sum, li);
+    npts = block_sum_i(npts, li);
+    tau = block_sum_d(tau, ld);
+    if (threadIdx.x == 0) J.chunks[c] = ChunkSum{sum, npts, tau};
+}
+
+__global__ __launch_bounds__(256) void k_gsetup_scan(const SetupJob *__restrict__ jobs, int njobs) {
+    __shared__ int li[4];
+    __shared__ double ld[4];
+    const SetupJob &J = jobs[job_of(jobs, njobs, blockIdx.x, BY_CHUNK)];
+    const int c = blockIdx.x - J.chunk_base;
+    const int n = J.g.ntiles;
+    const int nchunks = (n + GSCAN_CHUNK - 1) / GSCAN_CHUNK;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    int carry = 0;
+    for (int k = tid; k < c; k += blockDim.x) carry += J.chunks[k].total;
+    carry = block_sum_i(carry, li);
+    const int i0 = c * GSCAN_CHUNK + tid * 4;
+    int v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = i0 + k < n ? J.dep_cnt[i0 + k] : 0;
+    const int tsum = v[0] + v[1] + v[2] + v[3];
+    int incl = tsum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    __syncthreads();
+    if (lane == 63) li[wid] = incl;
+    __syncthreads();
+    int run = carry + incl - tsum;
+    for (int w = 0; w < wid; ++w) run += li[w];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (i0 + k < n) J.dep_start[i0 + k] = run;
+        run += v[k];
+    }
+    if (c != nchunks - 1) return;  // uniform over the block
+    // last chunk: run on the last thread is the request's total
+    int npts = 0;
+    double tau = 0.;
+    for (int k = tid; k < nchunks; k += blockDim.x) {
+        npts += J.chunks[k].npts;
+        tau += J.chunks[k].tau;
+    }
+    npts = block_sum_i(npts, li);
+    tau = block_sum_d(tau, ld);
+    if (tid == blockDim.x - 1) {
+        J.dep_start[n] = run;
+        J.rep->total = run;
+        J.rep->npts = npts;
+        J.rep->tau_sum = tau;
+    }
+}
+
+// Dependency lists of one tile's sources (the second pass of k_tile_deps: from the kept list,
+// or from the stored stencil when the tile has more than 64 sources), then the tile's initial
+// state (as k_init): values +inf, padding fixed, no key.
+template <int M>
+__device__ inline void gsetup_fill_tile(const SetupJob &J, int tile, int lane, int *set) {
+    const int ns = (int)J.tmax[tile];
+    if (ns <= 64) {
+        if (lane < ns) {
+            const int src = reinterpret_cast<const int *>(J.u)[(long long)tile * 64 + lane];
+            const int q = atomicAdd(&J.dep_fill[src], 1);
+            J.deps[J.dep_start[src] + q] = tile;
+        }
+    } else {
+        const StencilData<float> sd{J.offs, J.w, nullptr, nullptr, nullptr, J.np};
+        const int nsrc = tile_sources<M, float>(J.g, sd, tile, lane, set);
+        for (int i = lane; i < nsrc; i += 64) {
+            const int src = set[i];
+            const int q = atomicAdd(&J.dep_fill[src], 1);
+            J.deps[J.dep_start[src] + q] = tile;
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void k_gsetup_fill_init(const SetupJob *__restrict__ jobs, int njobs) {
+    __shared__ int set[HASH_SLOTS];
+    const SetupJob &J = jobs[job_of(jobs, njobs, blockIdx.x, BY_TILE)];
+    const int tile = blockIdx.x - J.tile_base;
+    const int lane = threadIdx.x;
+    if (J.g.dim == 3) gsetup_fill_tile<MODEL_RIEMANN3>(J, tile, lane, set);
+    else gsetup_fill_tile<MODEL_RIEMANN2>(J, tile, lane, set);
+    __syncthreads();  // the tile's kept list (in u and tmax) is read before it is overwritten
+    const long long p = (long long)tile * 64 + lane;
+    J.u[p] = dev_inf<float>();
+    int c[3];
+    tiled_coords(J.g, tile, lane, c);
+    const unsigned long long pad = __ballot(!in_grid(J.g, c));
+    if (lane == 0) {
+        J.fixedmask[tile] = pad;
+        J.keys[tile] = KEY_NONE;
+        J.tmax[tile] = KEY_NONE;
+    }
+}
+
+// One block per seed: its value and fixed bit (as k_set_seeds), and its tile and the tile's
+// dependents marked pending with the seed's key (as k_seed_activate); the smallest key left
+// pending goes to the request's seed key.
+__global__ __launch_bounds__(64) void k_gsetup_seeds(const SetupJob *__restrict__ jobs, int njobs) {
+    const SetupJob &J = jobs[job_of(jobs, njobs, blockIdx.x, BY_SEED)];
+    const int i = blockIdx.x - J.seed_base;
+    const int lane = threadIdx.x;
+    const long long p = J.seed_p[i];
+    const double v = J.seed_v[i];
+    if (lane == 0) {
+        J.u[p] = (float)v;
+        atomicOr(&J.fixedmask[p >> 6], 1ull << (p & 63));
+    }
+    const int tile = (int)(p >> 6);
+    const unsigned key = key_of((float)v);
+    unsigned wmin = mark_pending(lane == 0 ? tile : -1, key, J.keys, J.tmax);
+    const int b = J.dep_start[tile], e = J.dep_start[tile + 1];
+    for (int j0 = b; j0 < e; j0 += 64) {
+        const int j = j0 + lane;
+        const unsigned m = mark_pending(j < e ? J.deps[j] : -1, key, J.keys, J.tmax);
+        wmin = m < wmin ? m : wmin;
+    }
+    if (lane == 0 && wmin != KEY_NONE) atomicMin(J.seed_key, wmin);
+}
+
+// Scheduling state of the solves joining the running set at a round of the given parity (one
+// block per descriptor; the others return): each seed key (smallest key the seed marks left
+// pending) becomes the pending minimum of that round.
+__global__ __launch_bounds__(64) void k_mjoin(const SolveDesc *__restrict__ descs, int parity) {
+    const SolveDesc &d = descs[blockIdx.x];
+    if (!d.joining) return;
     const int i = threadIdx.x;
     for (int j = i; j < 2 * PEND_SLOTS; j += 64)
-        pend[(j / PEND_SLOTS) * PEND_WORDS + (j % PEND_SLOTS) * PEND_STRIDE] =
-            (j == parity * PEND_SLOTS) ? *seed_key : KEY_NONE;
-    if (i == 0) *visits = 0ull;
-    if (i < 2) count[i] = 0;
-    if (i < 4) bound[i] = i < 2 ? -__builtin_huge_valf() : delta;
+        d.pend[(j / PEND_SLOTS) * PEND_WORDS + (j % PEND_SLOTS) * PEND_STRIDE] =
+            (j == parity * PEND_SLOTS) ? *d.seed_key : KEY_NONE;
+    if (i == 0) *d.visits = 0ull;
+    if (i < 2) d.count[i] = 0;
+    if (i < 4) d.bound[i] = i < 2 ? -__builtin_huge_valf() : d.delta;
+}
+
+// Descriptor indices of the converged solves exported by one launch.
+constexpr int EXPORT_MAX = 32;
+struct ExportList {
+    int idx[EXPORT_MAX];
+};
+
+// Tiled FP32 values -> row-major FP64 of every listed solve (blockIdx.y: list entry).
+__global__ __launch_bounds__(256) void k_mexport(const SolveDesc *__restrict__ descs, ExportList list) {
+    const SolveDesc &d = descs[list.idx[blockIdx.y]];
+    const Geom &g = d.ctx.g;
+    const long long np = (long long)g.ntiles * 64;
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (long long)gridDim.x * blockDim.x) {
+        int c[3];
+        tiled_coords(g, (int)(p >> 6), (int)(p & 63), c);
+        if (!in_grid(g, c)) continue;
+        d.values_dev[((long long)c[0] * g.n[1] + c[1]) * g.n[2] + c[2]] = (double)d.ctx.u[p];
+    }
 }
 
 }  // namespace hfm
@@ -204,7 +509,6 @@ struct Slot {
     long long *seed_p = nullptr;
     double *seed_v = nullptr;
     double *values_dev = nullptr;
-    int *status = nullptr;
     unsigned *seed_key = nullptr;  // smallest key the seed marks left pending
     float delta = 0;
     int64_t rounds = 0;
@@ -233,8 +537,8 @@ size_t arena_bytes(const RequestRec &r) {
     add(r.seed_tp.size() * 16);  // seed points and values, one block
 
     add(r.N * 8);           // values_dev
-    add(16);                // status
     add(4);                 // seed key
+    add(((size_t)(nt + GSCAN_CHUNK - 1) / GSCAN_CHUNK) * sizeof(ChunkSum));  // scan chunk sums
     return b;
 }
 
@@ -260,10 +564,10 @@ struct hfm_server {
     static constexpr int NSETUP = 2;
     static constexpr int SETUP_GROUP = 8;  // requests set up per pass (two synchronisations)
     hipStream_t s_setup[NSETUP] = {}, s_comp = nullptr;
-    RedSlot *d_red[NSETUP] = {};           // per thread: SETUP_GROUP x RED_SLOTS stencil reductions
-    RedSlot *h_red[NSETUP] = {};           // pinned copies
-    int *h_status[NSETUP] = {};            // pinned SETUP_GROUP x 2
-    int *h_total[NSETUP] = {};             // pinned SETUP_GROUP dependency totals
+    SetupJob *d_jobs[NSETUP] = {};         // per thread: the group's request table
+    SetupJob *h_jobs[NSETUP] = {};         // pinned source of its uploads
+    SetupReport *d_rep[NSETUP] = {};       // per thread: SETUP_GROUP reports
+    SetupReport *h_rep[NSETUP] = {};       // pinned copies
     std::thread th_setup[NSETUP], th_comp;
 
     // compute-thread state.  One batch of rounds stays in flight while the host reads the
@@ -313,11 +617,11 @@ void release_budget(hfm_server *S, size_t bytes) {
     S->cv_submit.notify_all();
 }
 
-// Setup of a group of admitted requests on one setup stream: stencils, tile graph, initial
-// values and seed marks.  Phase 1 launches every request's stencil, dependency count and scan
-// and copies back its status, reductions and dependency total; one synchronisation; phase 2
-// allocates and fills the dependency lists, initialises values and marks the seeds; one
-// synchronisation.  Failed requests are reported in `err` (empty: ready).
+// Setup of a group of admitted requests on one setup stream, one launch per kernel for the whole
+// group.  Phase 1: clears, stencils with the tile sources, scans; the reports (status, dependency
+// total, one-step times) are copied back; one synchronisation.  Phase 2 allocates the dependency
+// lists of the requests whose stencils succeeded, fills them and initialises values, then marks
+// the seeds; one synchronisation.  Failed requests are reported in `err` (empty: ready).
 struct SetupItem {
     std::shared_ptr<RequestRec> r;
     Slot *sl = nullptr;
@@ -327,6 +631,8 @@ struct SetupItem {
 void setup_group(hfm_server *S, std::vector<SetupItem> &items, int which) {
     hipStream_t s = S->s_setup[which];
     const int n = (int)items.size();
+    SetupJob *hj = S->h_jobs[which];
+    int tiles = 0, max_nt = 0, nchunk = 0;
     for (int g = 0; g < n; ++g) {
         const std::shared_ptr<RequestRec> &r = items[g].r;
         Slot *sl = new Slot;
@@ -347,8 +653,8 @@ void setup_group(hfm_server *S, std::vector<SetupItem> &items, int which) {
         sl->seed_p = carve<long long>(p, 2 * ns);
         sl->seed_v = reinterpret_cast<double *>(sl->seed_p + ns);
         sl->values_dev = carve<double>(p, r->N);
-        sl->status = carve<int>(p, 4);
         sl->seed_key = carve<unsigned>(p, 1);
+        ChunkSum *chunks = carve<ChunkSum>(p, (r->g.ntiles + GSCAN_CHUNK - 1) / GSCAN_CHUNK);
         const double *metric = r->metric_dev;
         if (!metric) {
             const size_t mcount = r->metric_is_field ? (size_t)r->N * r->K : (size_t)r->K;
@@ -358,61 +664,85 @@ void setup_group(hfm_server *S, std::vector<SetupItem> &items, int which) {
             metric = sl->metric_stage;
         }
         HIPCHECK(hipMemcpyAsync(sl->seed_p, r->seed_blob.data(), r->seed_blob.size() * 8, hipMemcpyHostToDevice, s));
-        RedSlot *red = S->d_red[which] + (size_t)g * RED_SLOTS;
-        hipLaunchKernelGGL(k_setup_clear, dim3((unsigned)std::min<long long>((r->g.ntiles + 255) / 256 + 16, 1024)),
-                           dim3(256), 0, s, sl->status, red, sl->dep_cnt, sl->dep_fill, sl->seed_key, r->g.ntiles);
-        HIPCHECK(hipGetLastError());
-        InvHH ih{};
+        SetupJob &J = hj[g];
+        J = SetupJob{};
+        J.g = r->g;
+        J.metric_is_field = r->metric_is_field;
+        J.dual = r->dual;
+        J.nseeds = (int)ns;
+        J.np = r->np;
         const int D = r->g.dim;
         for (int i = 0; i < D; ++i)
-            for (int j = 0; j <= i; ++j) ih.v[i * (i + 1) / 2 + j] = 1. / (r->h * r->h);
-        HIPCHECK(launch_riemann_stencil<float>(r->g, metric, r->metric_is_field, r->dual, ih, sl->offs, sl->w, r->np,
-                                               sl->status, red, s));
-        StencilData<float> sd{sl->offs, sl->w, nullptr, nullptr, nullptr, r->np};
-        // The tile graph's first pass keeps each tile's sources (<= 64: 64 ints a tile) in `u` and
-        // their counts in `tmax`, and the multi-block scan works in `fixedmask`: all three are
-        // written by k_init only after the second pass has read them.
-        HIPCHECK(launch_tile_deps<float>(r->model, r->g, sd, 0, sl->dep_cnt, nullptr, nullptr, nullptr,
-                                         reinterpret_cast<int *>(sl->u), reinterpret_cast<int *>(sl->tmax), s));
-        HIPCHECK(launch_exclusive_scan(sl->dep_cnt, sl->dep_start, r->g.ntiles, s,
-                                       reinterpret_cast<int *>(sl->fixedmask)));
-        HIPCHECK(hipMemcpyAsync(S->h_status[which] + 2 * g, sl->status, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
-        HIPCHECK(hipMemcpyAsync(S->h_red[which] + (size_t)g * RED_SLOTS, red, RED_SLOTS * sizeof(RedSlot),
-                                hipMemcpyDeviceToHost, s));
-        HIPCHECK(hipMemcpyAsync(S->h_total[which] + g, sl->dep_start + r->g.ntiles, sizeof(int),
-                                hipMemcpyDeviceToHost, s));
+            for (int j = 0; j <= i; ++j) J.ih.v[i * (i + 1) / 2 + j] = 1. / (r->h * r->h);
+        J.metric = metric;
+        J.offs = sl->offs;
+        J.w = sl->w;
+        J.u = sl->u;
+        J.fixedmask = sl->fixedmask;
+        J.keys = sl->keys;
+        J.tmax = sl->tmax;
+        J.dep_cnt = sl->dep_cnt;
+        J.dep_start = sl->dep_start;
+        J.dep_fill = sl->dep_fill;
+        J.seed_p = sl->seed_p;
+        J.seed_v = sl->seed_v;
+        J.seed_key = sl->seed_key;
+        J.rep = S->d_rep[which] + g;
+        J.chunks = chunks;
+        J.tile_base = tiles;
+        J.chunk_base = nchunk;
+        tiles += r->g.ntiles;
+        nchunk += (r->g.ntiles + GSCAN_CHUNK - 1) / GSCAN_CHUNK;
+        max_nt = std::max(max_nt, r->g.ntiles);
     }
+    HIPCHECK(hipMemcpyAsync(S->d_jobs[which], hj, sizeof(SetupJob) * n, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_gsetup_clear, dim3((unsigned)std::min((max_nt + 255) / 256, 64), (unsigned)n), dim3(256), 0, s,
+                       S->d_jobs[which]);
+    hipLaunchKernelGGL(k_gsetup_stencil, dim3((unsigned)tiles), dim3(64), 0, s, S->d_jobs[which], n);
+    hipLaunchKernelGGL(k_gsetup_chunks, dim3((unsigned)nchunk), dim3(256), 0, s, S->d_jobs[which], n);
+    hipLaunchKernelGGL(k_gsetup_scan, dim3((unsigned)nchunk), dim3(256), 0, s, S->d_jobs[which], n);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(S->h_rep[which], S->d_rep[which], sizeof(SetupReport) * n, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
+    // Phase 2 over the requests whose stencils succeeded, compacted to the front of the table.
+    int m = 0, seeds = 0;
+    tiles = 0;
     for (int g = 0; g < n; ++g) {
-        const std::shared_ptr<RequestRec> &r = items[g].r;
         Slot *sl = items[g].sl;
-        const int st = S->h_status[which][2 * g];
-        if (st != SELLING_OK) {
-            items[g].err = st == SELLING_NOT_SPD ? "ObtuseSuperbase error : input must be positive definite"
-                                                 : "ObtuseSuperbase error : did not terminate in 200 iterations";
+        const SetupReport &rep = S->h_rep[which][g];
+        if (rep.status[0] != SELLING_OK) {
+            items[g].err = rep.status[0] == SELLING_NOT_SPD
+                               ? "ObtuseSuperbase error : input must be positive definite"
+                               : "ObtuseSuperbase error : did not terminate in 200 iterations";
             if (sl->metric_stage) HIPCHECK(hipFreeAsync(sl->metric_stage, s));
             HIPCHECK(hipFreeAsync(sl->arena, s));
             delete sl;
             items[g].sl = nullptr;
             continue;
         }
-        const int total = S->h_total[which][g];
-        HIPCHECK(hipMallocAsync((void **)&sl->deps, sizeof(int) * std::max(total, 1), s));
-        StencilData<float> sd{sl->offs, sl->w, nullptr, nullptr, nullptr, r->np};
-        HIPCHECK(launch_tile_deps<float>(r->model, r->g, sd, 1, sl->dep_cnt, sl->dep_start, sl->dep_fill, sl->deps,
-                                         reinterpret_cast<int *>(sl->u), reinterpret_cast<int *>(sl->tmax), s));
-        const int nseeds = (int)r->seed_tp.size();
-        HIPCHECK(launch_init<float>(r->g, sl->u, sl->fixedmask, sl->keys, sl->tmax, r->np, sl->seed_p, sl->seed_v,
-                                    nseeds, s));
-        HIPCHECK(launch_seed_activate(sl->seed_p, sl->seed_v, nseeds, sl->dep_start, sl->deps, sl->keys, sl->tmax,
-                                      sl->seed_key, s));
-        if (sl->metric_stage) {
+        HIPCHECK(hipMallocAsync((void **)&sl->deps, sizeof(int) * std::max(rep.total, 1), s));
+        const double tau = rep.npts > 0 ? rep.tau_sum / rep.npts : 0.;
+        sl->delta = (float)(8. * tau);
+        if (m != g) hj[m] = hj[g];
+        SetupJob &J = hj[m++];
+        J.deps = sl->deps;
+        J.tile_base = tiles;
+        J.seed_base = seeds;
+        tiles += J.g.ntiles;
+        seeds += J.nseeds;
+    }
+    if (m > 0) {
+        HIPCHECK(hipMemcpyAsync(S->d_jobs[which], hj, sizeof(SetupJob) * m, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_gsetup_fill_init, dim3((unsigned)tiles), dim3(64), 0, s, S->d_jobs[which], m);
+        if (seeds > 0) hipLaunchKernelGGL(k_gsetup_seeds, dim3((unsigned)seeds), dim3(64), 0, s, S->d_jobs[which], m);
+        HIPCHECK(hipGetLastError());
+    }
+    for (int g = 0; g < n; ++g) {
+        Slot *sl = items[g].sl;
+        if (sl && sl->metric_stage) {
             HIPCHECK(hipFreeAsync(sl->metric_stage, s));
             sl->metric_stage = nullptr;
         }
-        const RedTotals tot = red_total(S->h_red[which] + (size_t)g * RED_SLOTS);
-        const double tau = tot.count > 0 ? tot.sum / tot.count : 0.;
-        sl->delta = (float)(8. * tau);
     }
     HIPCHECK(hipStreamSynchronize(s));
 }
@@ -471,7 +801,6 @@ void setup_loop(hfm_server *S, int which) {
 
 void join_solve(hfm_server *S, Slot *sl) {
     const RequestRec &r = *sl->rec;
-    hipStream_t s = S->s_comp;
     int idx;
     if (!S->free_idx.empty()) {
         idx = S->free_idx.back();
@@ -481,13 +810,10 @@ void join_solve(hfm_server *S, Slot *sl) {
     }
     sl->idx = idx;
     sl->first_batch = S->nlaunched;
-    const int parity = (int)(S->round & 1);
     unsigned *pend = S->d_pend + (size_t)idx * 2 * PEND_WORDS;
     float *bound = S->d_bound + (size_t)idx * 4;
     int *count = S->d_count + (size_t)idx * 2;
     unsigned long long *st = S->d_stats + idx;
-    hipLaunchKernelGGL(k_join_state, dim3(1), dim3(64), 0, s, pend, count, st, bound, sl->seed_key, parity, sl->delta);
-    HIPCHECK(hipGetLastError());
     SolveDesc &d = S->h_desc[idx];
     d.ctx = TileCtx<float>{r.g, StencilData<float>{sl->offs, sl->w, nullptr, nullptr, nullptr, r.np}, sl->u,
                            sl->fixedmask, sl->dep_start, sl->deps, sl->keys, sl->tmax};
@@ -499,6 +825,9 @@ void join_solve(hfm_server *S, Slot *sl) {
     d.bound = bound;
     d.count = count;
     d.visits = st;
+    d.seed_key = sl->seed_key;
+    d.values_dev = sl->values_dev;
+    d.joining = 1;  // k_mjoin at the start of the next batch sets the scheduling state
     S->running.push_back(sl);
 }
 
@@ -545,10 +874,19 @@ void launch_batch(hfm_server *S, long long q, int batch) {
     for (int i = 0; i < S->ndesc; ++i)
         S->h_desc[i].target = S->h_desc[i].model == MODEL_RIEMANN2 ? std::min(1024, 4 * tgt) : tgt;
     std::memcpy(S->h_desc_stage[par], S->h_desc, sizeof(SolveDesc) * S->ndesc);
+    bool joining = false;
+    for (int i = 0; i < S->ndesc; ++i) {
+        joining |= S->h_desc[i].joining != 0;
+        S->h_desc[i].joining = 0;
+    }
     std::memcpy(S->h_bmap_stage[par], bm.data(), bm.size() * sizeof(int2));
     HIPCHECK(hipMemcpyAsync(S->d_desc, S->h_desc_stage[par], sizeof(SolveDesc) * S->ndesc, hipMemcpyHostToDevice, s));
     HIPCHECK(hipMemcpyAsync(S->d_bmap, S->h_bmap_stage[par], bm.size() * sizeof(int2), hipMemcpyHostToDevice, s));
     const unsigned grid = (unsigned)std::min<long long>((tiles + 3) / 4, 2048);
+    if (joining) {
+        hipLaunchKernelGGL(k_mjoin, dim3((unsigned)S->ndesc), dim3(64), 0, s, S->d_desc, (int)(S->round & 1));
+        HIPCHECK(hipGetLastError());
+    }
     for (int b = 0; b < batch; ++b) {
         const int parity = (int)(S->round & 1);
         hipLaunchKernelGGL(k_mschedule, dim3((unsigned)bm.size()), dim3(SCHED_BLOCK), 0, s, S->d_desc, S->d_bmap,
@@ -594,10 +932,23 @@ void process_batch(hfm_server *S, long long q, int last_parity) {
     }
     if (done.empty()) return;
     S->running.swap(still);
+    // One export launch per EXPORT_MAX converged solves.  The descriptor table on the device is
+    // the one uploaded with the latest batch, which still lists them.
+    for (size_t b = 0; b < done.size(); b += EXPORT_MAX) {
+        ExportList el{};
+        long long np_max = 0;
+        const int ne = (int)std::min<size_t>(EXPORT_MAX, done.size() - b);
+        for (int i = 0; i < ne; ++i) {
+            el.idx[i] = done[b + i]->idx;
+            np_max = std::max(np_max, done[b + i]->rec->np);
+        }
+        hipLaunchKernelGGL(k_mexport, dim3((unsigned)std::min<long long>((np_max + 255) / 256, 1024), (unsigned)ne),
+                           dim3(256), 0, S->s_comp, S->d_desc, el);
+        HIPCHECK(hipGetLastError());
+    }
     for (Slot *sl : done) {
         S->h_desc[sl->idx].active = 0;
         finish_export(sl, S->h_stats[par]);
-        HIPCHECK(launch_export_values<float>(sl->rec->g, sl->u, sl->values_dev, sl->rec->np, S->s_comp));
         if (sl->rec->values_out)
             HIPCHECK(hipMemcpyAsync(sl->rec->values_out, sl->values_dev, sl->rec->N * sizeof(double),
                                     hipMemcpyDeviceToHost, S->s_comp));
@@ -703,12 +1054,11 @@ int hfm_server_create(hfm_server **out, double hbm_budget_bytes, int max_residen
         S->stats.hbm_budget = hbm_budget_bytes;
         for (int i = 0; i < hfm_server::NSETUP; ++i) {
             HIPCHECK(hipStreamCreateWithFlags(&S->s_setup[i], hipStreamNonBlocking));
-            S->d_red[i] = dalloc<RedSlot>((size_t)RED_SLOTS * hfm_server::SETUP_GROUP);
-            HIPCHECK(hipHostMalloc((void **)&S->h_red[i], sizeof(RedSlot) * RED_SLOTS * hfm_server::SETUP_GROUP,
+            S->d_jobs[i] = dalloc<SetupJob>(hfm_server::SETUP_GROUP);
+            HIPCHECK(hipHostMalloc((void **)&S->h_jobs[i], sizeof(SetupJob) * hfm_server::SETUP_GROUP,
                                    hipHostMallocDefault));
-            HIPCHECK(hipHostMalloc((void **)&S->h_status[i], sizeof(int) * 2 * hfm_server::SETUP_GROUP,
-                                   hipHostMallocDefault));
-            HIPCHECK(hipHostMalloc((void **)&S->h_total[i], sizeof(int) * hfm_server::SETUP_GROUP,
+            S->d_rep[i] = dalloc<SetupReport>(hfm_server::SETUP_GROUP);
+            HIPCHECK(hipHostMalloc((void **)&S->h_rep[i], sizeof(SetupReport) * hfm_server::SETUP_GROUP,
                                    hipHostMallocDefault));
         }
         HIPCHECK(hipStreamCreateWithFlags(&S->s_comp, hipStreamNonBlocking));
@@ -903,10 +1253,10 @@ void hfm_server_destroy(hfm_server *S) {
         if (S->s_setup[i]) (void)hipStreamSynchronize(S->s_setup[i]);
     if (S->d_list) (void)hipFree(S->d_list);
     for (int i = 0; i < hfm_server::NSETUP; ++i) {
-        dfree(S->d_red[i]);
-        if (S->h_red[i]) (void)hipHostFree(S->h_red[i]);
-        if (S->h_status[i]) (void)hipHostFree(S->h_status[i]);
-        if (S->h_total[i]) (void)hipHostFree(S->h_total[i]);
+        dfree(S->d_jobs[i]);
+        if (S->h_jobs[i]) (void)hipHostFree(S->h_jobs[i]);
+        dfree(S->d_rep[i]);
+        if (S->h_rep[i]) (void)hipHostFree(S->h_rep[i]);
     }
     dfree(S->d_desc);
     dfree(S->d_lc);
