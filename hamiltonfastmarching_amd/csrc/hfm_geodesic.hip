@@ -11,16 +11,20 @@
 // geodesicVolumeBound (bound x 1.3 from the 5th restart, at most 50 restarts).
 //
 // Layout.  The live set (at most GEO_CAP points: the reference's std::map keyed by
-// (value, index)) lives in the wave's registers, GEO_CAP / 64 slots per lane, with a per-lane
-// occupancy mask; taking the largest key is a per-lane pass plus two DPP maxima, finding a key
-// a compare and a ballot.  The flow at a point comes from a cache in LDS that holds the flows of
-// every point of the last tiles the mass visited, each tile computed one lane per point when it
-// is first needed: a step then waits on no memory access unless the mass enters a new tile.
-// All bookkeeping (weight sum, index sum, covariance, steps) is FP64 in the reference's order
-// of operations -- a vector divided by a scalar is a multiplication by the reciprocal
-// (JMM_CPPLibs LinearAlgebra/VectorType.h:37-42) -- and runs uniformly on every lane.  On an
-// FP64 solve the emitted points are those the reference's solver computes from the same values
-// and active sets.
+// (value, index)) lives in the wave's registers, GEO_CAP / 64 rows of one slot per lane.  A slot
+// holds the point's value as an order-preserving 64-bit key, its grid index packed into 64 bits
+// (lexicographic order = unsigned order), and its mass; an empty slot holds key 0 and index
+// ~0, and which slots are taken is a 64-bit lane mask per row in scalar registers.  An index
+// fixes the point and hence its value, so a point is found by its packed index alone (one
+// 64-bit compare per row and a ballot).  Taking the largest key is a per-lane pass over the rows
+// plus two DPP maxima; a slot is written under one scalar branch on its row.  The flow at a
+// point comes from a cache in LDS that holds the flows of every point of the last tiles the mass
+// visited, each tile computed one lane per point when it is first needed: a step then waits on
+// no memory access unless the mass enters a new tile.  All bookkeeping (weight sum, index sum,
+// covariance, steps) is FP64 in the reference's order of operations -- a vector divided by a
+// scalar is a multiplication by the reciprocal (JMM_CPPLibs LinearAlgebra/VectorType.h:37-42) --
+// and runs uniformly on every lane.  On an FP64 solve the emitted
+// points are those the reference's solver computes from the same values and active sets.
 #include "hfm_kernels.h"
 
 #include "hfm_device.h"
@@ -33,18 +37,39 @@ constexpr int GEO_STEPS = 256;  // steps accumulated between two emitted points
 
 namespace {
 
-struct Key {
-    double val;
-    int i0, i1, i2;
-    int slot;  // -1: none
-};
-
-// (value, index) order of std::map<std::pair<double, IndexType>, double> (std::array compares
-// lexicographically).
-// Branch-free: per-lane comparisons become mask operations rather than divergent branches.
-__device__ inline bool key_less(const Key &a, const Key &b) {
-    return (a.val < b.val) |
-           ((a.val == b.val) & ((a.i0 < b.i0) | ((a.i0 == b.i0) & ((a.i1 < b.i1) | ((a.i1 == b.i1) & (a.i2 < b.i2))))));
+// Grid indices packed 21 bits per axis with a bias of 2^20: the unsigned order of the packed
+// keys is the lexicographic order of the indices (std::array's, as the reference's map key).
+constexpr int GEO_IBITS = 21;
+constexpr int GEO_IBIAS = 1 << (GEO_IBITS - 1);
+constexpr unsigned long long GEO_IMASK = (1ull << GEO_IBITS) - 1;
+constexpr unsigned long long GEO_INONE = ~0ull;  // empty slot (above every packed index)
+__device__ inline bool pack_index(const int *ind, unsigned long long &key) {
+    unsigned long long k = 0;
+    bool ok = true;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int v = ind[q] + GEO_IBIAS;
+        ok &= (unsigned)v <= (unsigned)GEO_IMASK;
+        k = (k << GEO_IBITS) | ((unsigned long long)(unsigned)v & GEO_IMASK);
+    }
+    key = k;
+    return ok;
+}
+__device__ inline void unpack_index(unsigned long long key, int *ind) {
+    ind[2] = (int)(key & GEO_IMASK) - GEO_IBIAS;
+    ind[1] = (int)((key >> GEO_IBITS) & GEO_IMASK) - GEO_IBIAS;
+    ind[0] = (int)((key >> (2 * GEO_IBITS)) & GEO_IMASK) - GEO_IBIAS;
+}
+// The packed key of index + o is key + packed_delta(o) (two's complement): no field carries as
+// long as every index of the key lies within +-(2^20 - 128) and |o| <= 127, which
+// index_has_margin checks before a point's targets are formed.
+__device__ inline unsigned long long packed_delta(int o0, int o1, int o2) {
+    return ((unsigned long long)(long long)o0 << (2 * GEO_IBITS)) + ((unsigned long long)(long long)o1 << GEO_IBITS) +
+           (unsigned long long)(long long)o2;
+}
+__device__ inline bool index_has_margin(const int *ind) {
+    constexpr int lim = GEO_IBIAS - 128;
+    return (ind[0] > -lim) & (ind[0] < lim) & (ind[1] > -lim) & (ind[1] < lim) & (ind[2] > -lim) & (ind[2] < lim);
 }
 
 struct GeoLDS {
@@ -71,6 +96,9 @@ constexpr int GEO_UNCACHED = 0xff;
 template <int ModelT, typename T> struct FlowCacheShape {
     static constexpr int NC = ModelShape<ModelT>::NFWD + ModelShape<ModelT>::NSYM;
     static constexpr int ECAP = 64 * (NC < 6 ? NC : 6);  // entries per tile
+    // every point's entries at a fixed place (point * NC): its count and entries are read in
+    // one LDS round trip, and no point is ever left uncached
+    static constexpr bool FIXED = NC <= 6;
     static constexpr int SLOT_BYTES = ECAP * (4 + 8 + (int)sizeof(T)) + 64 * (8 + 4);
     static constexpr int BUDGET = 64 * 1024;  // with GeoLDS, two workgroups fit in a CU's 160 KB
     static constexpr int NSLOT = BUDGET / SLOT_BYTES < 12 ? BUDGET / SLOT_BYTES : 12;
@@ -83,13 +111,15 @@ template <int ModelT, typename T> struct FlowCache {
     T e_v[CS::NSLOT][CS::ECAP];         // value at the flow's target point
     uint32_t e_off[CS::NSLOT][CS::ECAP];  // signed offset to the target (pack_offset)
     uint32_t info[CS::NSLOT][64];         // first entry << 8 | number of entries (or GEO_UNCACHED)
+    // (with the fixed layout the first entry is point * NC)
 };
 
 // Wave-uniform copies of lane l's value (l uniform).
 __device__ inline int rl_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 // A condition or value every lane holds, marked as such: branches on it are scalar branches and
-// the values merged after them stay in scalar registers.
-__device__ inline bool uni(bool c) { return __builtin_amdgcn_readfirstlane((int)c) != 0; }
+// the values merged after them stay in scalar registers (a uniform condition is tested through
+// its own lane mask).
+__device__ inline bool uni(bool c) { return __ballot(c) != 0ull; }
 __device__ inline double uni_d(double v) {
     const unsigned long long b = (unsigned long long)__double_as_longlong(v);
     const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
@@ -109,7 +139,8 @@ __device__ inline unsigned long long order_key(double v) {
     return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
 }
 
-// FP64 running sums of GeodesicDiscrete's UpdateSums (GeodesicDiscreteSolver.hxx:80-85).
+// FP64 running sums of GeodesicDiscrete's UpdateSums (GeodesicDiscreteSolver.hxx:80-85), held
+// alike by every lane: ten independent accumulations that the wave issues back to back.
 struct Sums {
     double weightSum = 0., indexSum[3] = {0., 0., 0.}, cov[6] = {0., 0., 0., 0., 0., 0.};
     template <int D> __device__ void update(double weight, const int *ind) {
@@ -366,14 +397,18 @@ __device__ void fill_flow(const Geom &g, const StencilData<T> &sd, const T *u, c
         value = m + (b + sqrt(d2 > 0. ? d2 : 0.)) / a;
     }
     const int n = __builtin_popcount(used);
-    int incl = n;
+    int st = lane * NC;
+    bool fits = true;
+    if (!CS::FIXED) {
+        int incl = n;
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int t = __shfl_up(incl, d);
-        if (lane >= d) incl += t;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int t = __shfl_up(incl, d);
+            if (lane >= d) incl += t;
+        }
+        st = incl - n;
+        fits = incl <= CS::ECAP;
     }
-    const int st = incl - n;
-    const bool fits = incl <= CS::ECAP;
     double ws = 0;
     int e = st;
 #pragma unroll
@@ -402,7 +437,8 @@ struct GeoParams {
 };
 
 // Status codes per tip.
-enum GeoStatus { GEO_OK = 0, GEO_FAILED = 1, GEO_SET_OVERFLOW = 2, GEO_STEPS_OVERFLOW = 3, GEO_NO_END = 4 };
+enum GeoStatus { GEO_OK = 0, GEO_FAILED = 1, GEO_SET_OVERFLOW = 2, GEO_STEPS_OVERFLOW = 3, GEO_NO_END = 4,
+                 GEO_INDEX_RANGE = 5 };
 
 template <int ModelT, typename T>
 __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, const T *__restrict__ u,
@@ -412,20 +448,43 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
     constexpr int D = ModelShape<ModelT>::DIM;
     constexpr int ROWS = GEO_CAP / 64;
     using CS = FlowCacheShape<ModelT, T>;
+    constexpr int NC = CS::NC;
     __shared__ GeoLDS L;
     __shared__ FlowCache<ModelT, T> C;
     const int tip = blockIdx.x, lane = threadIdx.x;
-    // The live set: slot 64 j + l is row j of lane l's registers; bit j of lane l's my_live says
-    // whether it holds a point.
-    double lv_val[ROWS], lv_w[ROWS];
-    int lv_i[ROWS][3];
-    unsigned my_live = 0;
+    // The live set: row j of every lane's registers; live[j] (wave-uniform) marks the taken slots.
+    unsigned long long okey[ROWS], ikey[ROWS], live[ROWS];
+    double wt[ROWS];
+    int nlive = 0;
+    // Write a slot (row r and lane l uniform) under one scalar branch on its row.
+    auto slot_put = [&](int r, int l, unsigned long long ok, unsigned long long ik, double w) {
 #pragma unroll
-    for (int j = 0; j < ROWS; ++j) {
-        lv_val[j] = 0.;
-        lv_w[j] = 0.;
-        lv_i[j][0] = lv_i[j][1] = lv_i[j][2] = 0;
-    }
+        for (int j = 0; j < ROWS; ++j)
+            if (j == r) {
+                if (lane == l) {
+                    okey[j] = ok;
+                    ikey[j] = ik;
+                    wt[j] = w;
+                }
+                live[j] |= 1ull << l;
+            }
+    };
+    auto slot_clear = [&](int r, int l) {
+#pragma unroll
+        for (int j = 0; j < ROWS; ++j)
+            if (j == r) {
+                if (lane == l) {
+                    okey[j] = 0ull;
+                    ikey[j] = GEO_INONE;
+                }
+                live[j] &= ~(1ull << l);
+            }
+    };
+    auto slot_add = [&](int r, int l, double w) {
+#pragma unroll
+        for (int j = 0; j < ROWS; ++j)
+            if (j == r && lane == l) wt[j] += w;
+    };
     // Flow cache directory: lane s < NSLOT holds the tile in slot s (-1: empty) and its last use.
     int my_tag = -1;
     unsigned my_use = 0, clock = 0;
@@ -448,8 +507,14 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
     while (nRestarts < 50 && st == GEO_OK) {
         // ---- one pass of GeodesicDiscrete from `last` (GeodesicDiscreteSolver.hxx:56-231)
         bool restart = false;
-        my_live = 0;
-        int nlive = 0;
+#pragma unroll
+        for (int j = 0; j < ROWS; ++j) {
+            okey[j] = 0ull;
+            ikey[j] = GEO_INONE;
+            wt[j] = 0.;
+            live[j] = 0ull;
+        }
+        nlive = 0;
         Sums sums;
         double indexAvg[3] = {0., 0., 0.}, indexBase[3] = {0., 0., 0.};
         double stepNorm = 0.;
@@ -475,16 +540,16 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
                 }
                 int c[3];
                 if (!uni(periodize<ModelT>(g, index, c))) continue;
-                const double v0 = value_at(g, u, c);
-                if (lane == nlive) {  // slot nlive < 8: row 0
-                    lv_val[0] = v0;
-                    lv_w[0] = weight;
-                    for (int q = 0; q < 3; ++q) lv_i[0][q] = q < D ? index[q] : 0;
-                    my_live |= 1u;
+                unsigned long long ik;
+                if (!uni(pack_index(index, ik))) {
+                    st = GEO_INDEX_RANGE;
+                    break;
                 }
+                slot_put(0, nlive, order_key(value_at(g, u, c)), ik, weight);  // nlive < 8: row 0
                 ++nlive;
                 sums.update<D>(weight, index);
             }
+            if (st != GEO_OK) break;
             if (uni(sums.weightSum == 0)) break;
             sums.mean<D>(indexAvg);
             for (int q = 0; q < 3; ++q) indexBase[q] = indexAvg[q];
@@ -495,54 +560,54 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
                 st = GEO_NO_END;
                 break;
             }
-            // Take the live point of largest (value, index): each lane's largest row, then the
-            // largest value over the wave (two 32-bit DPP maxima of its order-preserving 64-bit
-            // key); equal values in several lanes are ordered by index.
-            int bj = -1, b0 = 0, b1 = 0, b2 = 0;
-            double bv = 0.;
+            // Take the live point of largest (value, index): each lane's largest row (empty slots
+            // hold key 0 and index ~0, and lose to any taken slot), then the largest key over the
+            // wave (two 32-bit DPP maxima); equal keys in several lanes are ordered by index.
+            unsigned long long bo = okey[0], bi = ikey[0];
+            double bw = wt[0];
+            int br = 0;
 #pragma unroll
-            for (int j = 0; j < ROWS; ++j) {
-                const Key k{lv_val[j], lv_i[j][0], lv_i[j][1], lv_i[j][2], j};
-                const bool take = (my_live >> j & 1u) & ((bj < 0) | key_less(Key{bv, b0, b1, b2, bj}, k));
-                bj = take ? j : bj;
-                bv = take ? k.val : bv;
-                b0 = take ? k.i0 : b0;
-                b1 = take ? k.i1 : b1;
-                b2 = take ? k.i2 : b2;
+            for (int j = 1; j < ROWS; ++j) {
+                const bool gt = (okey[j] > bo) | ((okey[j] == bo) & (ikey[j] > bi));
+                bo = gt ? okey[j] : bo;
+                bi = gt ? ikey[j] : bi;
+                bw = gt ? wt[j] : bw;
+                br = gt ? j : br;
             }
-            const unsigned long long km = bj >= 0 ? order_key(bv) : 0ull;
-            const unsigned khi = (unsigned)(km >> 32), klo = (unsigned)km;
+            const unsigned khi = (unsigned)(bo >> 32), klo = (unsigned)bo;
             const unsigned mhi = wave_max_u32(khi);
             const unsigned mlo = wave_max_u32(khi == mhi ? klo : 0u);
-            const unsigned long long tie = __ballot(bj >= 0 && khi == mhi && klo == mlo);
-            if (!tie) {  // the live count and the live bits disagree: never expected
+            const unsigned long long tie = __ballot((khi == mhi) & (klo == mlo));
+            if ((mhi | mlo) == 0u) {  // the live count and the live slots disagree: never expected
                 st = GEO_NO_END;
                 break;
             }
             int wl = __ffsll((long long)tie) - 1;
-            if (tie & (tie - 1)) {
-                int w0 = rl_i(b0, wl), w1 = rl_i(b1, wl), w2 = rl_i(b2, wl);
-                for (unsigned long long r = tie & (tie - 1); r; r &= r - 1) {
-                    const int l = __ffsll((long long)r) - 1;
-                    const int c0 = rl_i(b0, l), c1 = rl_i(b1, l), c2 = rl_i(b2, l);
-                    if (c0 != w0 ? w0 < c0 : c1 != w1 ? w1 < c1 : w2 < c2) {
-                        wl = l;
-                        w0 = c0;
-                        w1 = c1;
-                        w2 = c2;
-                    }
+            unsigned long long wik = (unsigned long long)(unsigned)rl_i((int)(unsigned)(bi >> 32), wl) << 32 |
+                                     (unsigned)rl_i((int)(unsigned)bi, wl);
+            for (unsigned long long r = tie & (tie - 1); r; r &= r - 1) {
+                const int l = __ffsll((long long)r) - 1;
+                const unsigned long long cik = (unsigned long long)(unsigned)rl_i((int)(unsigned)(bi >> 32), l) << 32 |
+                                               (unsigned)rl_i((int)(unsigned)bi, l);
+                if (cik > wik) {
+                    wl = l;
+                    wik = cik;
                 }
             }
-            const int wj = rl_i(bj, wl);
-            const double value = rl_d(bv, wl);
-            const int index[3] = {rl_i(b0, wl), rl_i(b1, wl), rl_i(b2, wl)};
-            double myw = 0.;
-#pragma unroll
-            for (int j = 0; j < ROWS; ++j) myw = j == bj ? lv_w[j] : myw;
-            const double weight = rl_d(myw, wl);
-            if (lane == wl) my_live &= ~(1u << wj);
+            const int wr = rl_i(br, wl);
+            const unsigned long long wkey = (unsigned long long)mhi << 32 | mlo;
+            const double value =
+                __longlong_as_double((long long)((wkey >> 63) ? (wkey & 0x7fffffffffffffffull) : ~wkey));
+            const double weight = rl_d(bw, wl);
+            int index[3];
+            unpack_index(wik, index);
+            slot_clear(wr, wl);
             --nlive;
             if (uni(weight == 0)) continue;
+            if (!uni(index_has_margin(index))) {
+                st = GEO_INDEX_RANGE;
+                break;
+            }
             sums.update<D>(-weight, index);
             int c[3];
             if (!uni(periodize<ModelT>(g, index, c))) continue;  // live points are in the domain
@@ -562,76 +627,96 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
             }
             ++clock;
             if (lane == slot) my_use = clock;
-            const unsigned info = (unsigned)__builtin_amdgcn_readfirstlane((int)C.info[slot][loc]);
-            double wSum = uni_d(C.wsum[slot][loc]);
             // the point's flow entries, entry k in lane k
             int nf;
-            double my_w = 0., my_v = 0.;
+            double my_w = 0., my_v = 0., wSum;
             uint32_t my_off = 0;
-            if (info != (unsigned)GEO_UNCACHED) {
-                nf = (int)(info & 0xffu);
-                const int e = (int)(info >> 8) + lane;
+            if (CS::FIXED) {
+                // count, weight sum and entries in one round trip (entries past the count unused)
+                const int e = loc * NC + (lane < NC ? lane : 0);
+                const unsigned info = C.info[slot][loc];
+                const double ws = C.wsum[slot][loc];
+                const double ew = C.e_w[slot][e];
+                const T ev = C.e_v[slot][e];
+                const uint32_t eo = C.e_off[slot][e];
+                nf = (int)(__builtin_amdgcn_readfirstlane((int)info) & 0xff);
+                wSum = uni_d(ws);
                 if (lane < nf) {
-                    my_w = C.e_w[slot][e];
-                    my_v = (double)C.e_v[slot][e];
-                    my_off = C.e_off[slot][e];
+                    my_w = ew;
+                    my_v = (double)ev;
+                    my_off = eo;
                 }
             } else {
-                nf = discrete_flow<ModelT, T>(g, sd, u, fixedmask, c, L, lane);
-                wSum = 0;
-                for (int k = 0; k < nf; ++k) wSum += L.f_w[k];
-                wSum = uni_d(wSum);
-                if (lane < nf) {
-                    my_w = L.f_w[lane];
-                    my_v = L.f_val[lane];
-                    my_off = pack_offset(L.f_off[lane][0], L.f_off[lane][1], L.f_off[lane][2]);
+                const unsigned info = (unsigned)__builtin_amdgcn_readfirstlane((int)C.info[slot][loc]);
+                wSum = uni_d(C.wsum[slot][loc]);
+                if (info != (unsigned)GEO_UNCACHED) {
+                    nf = (int)(info & 0xffu);
+                    const int e = (int)(info >> 8) + lane;
+                    if (lane < nf) {
+                        my_w = C.e_w[slot][e];
+                        my_v = (double)C.e_v[slot][e];
+                        my_off = C.e_off[slot][e];
+                    }
+                } else {
+                    nf = discrete_flow<ModelT, T>(g, sd, u, fixedmask, c, L, lane);
+                    wSum = 0;
+                    for (int k = 0; k < nf; ++k) wSum += L.f_w[k];
+                    wSum = uni_d(wSum);
+                    if (lane < nf) {
+                        my_w = L.f_w[lane];
+                        my_v = L.f_val[lane];
+                        my_off = pack_offset(L.f_off[lane][0], L.f_off[lane][1], L.f_off[lane][2]);
+                    }
                 }
             }
             if (uni(wSum == 0)) continue;
             const double my_share = weight * my_w / wSum;  // entry k's mass, in lane k
+            // entry k's target in lane k: its index and packed key offset
+            int my_ind[3] = {index[0], index[1], index[2]};
+            unsigned long long my_delta;
+            {
+                int o[3];
+                unpack_offset(my_off, o[0], o[1], o[2]);
+                for (int q = 0; q < D; ++q) my_ind[q] += o[q];
+                my_delta = packed_delta(o[0], o[1], D > 2 ? o[2] : 0);
+            }
             // spread the mass over the flow's targets; a flow entry carries its target's value
             for (int k = 0; k < nf; ++k) {
-                const double w = rl_d(my_share, k), val = rl_d(my_v, k);
-                int o[3];
-                unpack_offset((uint32_t)rl_i((int)my_off, k), o[0], o[1], o[2]);
+                const double w = rl_d(my_share, k);
                 int ind[3] = {0, 0, 0};
-                for (int q = 0; q < D; ++q) ind[q] = index[q] + o[q];
-                // branch-free over the rows: selects rather than per-row branches
-                int fj = -1;
+                for (int q = 0; q < D; ++q) ind[q] = rl_i(my_ind[q], k);
+                const unsigned long long tk =
+                    wik + ((unsigned long long)(unsigned)rl_i((int)(unsigned)(my_delta >> 32), k) << 32 |
+                           (unsigned)rl_i((int)(unsigned)my_delta, k));
+                // the target's slot, if live: the first row holding its index
+                int fr = -1;
+                unsigned long long fm = 0ull;
 #pragma unroll
                 for (int j = 0; j < ROWS; ++j) {
-                    const bool eq = (my_live >> j & 1u) & (lv_val[j] == val) & (lv_i[j][0] == ind[0]) &
-                                    (lv_i[j][1] == ind[1]) & (lv_i[j][2] == ind[2]);
-                    fj = eq ? j : fj;
-                }
-                const unsigned long long bf = __ballot(fj >= 0);
-                if (bf) {
-                    const int fl = __ffsll((long long)bf) - 1, fr = rl_i(fj, fl);
-#pragma unroll
-                    for (int j = 0; j < ROWS; ++j) {
-                        const double add = lv_w[j] + w;
-                        lv_w[j] = ((j == fr) & (lane == fl)) ? add : lv_w[j];
+                    const unsigned long long m = __ballot(ikey[j] == tk);
+                    if (fr < 0 && m) {
+                        fr = j;
+                        fm = m;
                     }
+                }
+                if (fm) {
+                    slot_add(fr, __ffsll((long long)fm) - 1, w);
                 } else {
                     if (uni(w < sums.weightSum * gp.threshold)) continue;
-                    // a free slot: the first free row of the first lane that has one
-                    const int myfree = (my_live & ((1u << ROWS) - 1)) == (1u << ROWS) - 1 ? -1 : __builtin_ctz(~my_live);
-                    const unsigned long long bfree = __ballot(myfree >= 0);
-                    if (!bfree) {
+                    // a free slot: the first free lane of the first row that has one
+                    int r = -1;
+                    unsigned long long fmask = 0ull;
+#pragma unroll
+                    for (int j = 0; j < ROWS; ++j)
+                        if (r < 0 && ~live[j]) {
+                            r = j;
+                            fmask = ~live[j];
+                        }
+                    if (r < 0) {
                         st = GEO_SET_OVERFLOW;
                         break;
                     }
-                    const int fl = __ffsll((long long)bfree) - 1, fr = rl_i(myfree, fl);
-#pragma unroll
-                    for (int j = 0; j < ROWS; ++j) {
-                        const bool mine = (j == fr) & (lane == fl);
-                        lv_val[j] = mine ? val : lv_val[j];
-                        lv_w[j] = mine ? w : lv_w[j];
-                        lv_i[j][0] = mine ? ind[0] : lv_i[j][0];
-                        lv_i[j][1] = mine ? ind[1] : lv_i[j][1];
-                        lv_i[j][2] = mine ? ind[2] : lv_i[j][2];
-                    }
-                    my_live |= lane == fl ? 1u << fr : 0u;
+                    slot_put(r, __ffsll((long long)fmask) - 1, order_key(rl_d(my_v, k)), tk, w);
                     ++nlive;
                 }
                 sums.update<D>(w, ind);
@@ -686,8 +771,8 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
             nsteps = 0;
             // restart when the mass is too spread: det(cov / W - m m^T + I) > bound^2
             double var[6], mm[3];
-            const double bw = 1. / sums.weightSum;
-            for (int k = 0; k < D * (D + 1) / 2; ++k) var[k] = sums.cov[k] * bw;
+            const double bw2 = 1. / sums.weightSum;
+            for (int k = 0; k < D * (D + 1) / 2; ++k) var[k] = sums.cov[k] * bw2;
             sums.mean<D>(mm);
             for (int i = 0; i < D; ++i)
                 for (int j = 0; j <= i; ++j) {

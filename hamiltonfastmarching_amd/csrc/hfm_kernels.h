@@ -76,7 +76,7 @@ struct GeoParamsHost {
     long long max_iter = 0;    // per pass
 };
 enum GeoStatusHost { GEO_STATUS_OK = 0, GEO_STATUS_FAILED = 1, GEO_STATUS_SET_OVERFLOW = 2,
-                     GEO_STATUS_STEPS_OVERFLOW = 3, GEO_STATUS_NO_END = 4 };
+                     GEO_STATUS_STEPS_OVERFLOW = 3, GEO_STATUS_NO_END = 4, GEO_STATUS_INDEX_RANGE = 5 };
 template <typename T>
 hipError_t launch_geodesics(int model, const Geom &g, const StencilData<T> &sd, const T *u,
                             const unsigned long long *fm, const GeoParamsHost &h, int ntips, const double *tips,

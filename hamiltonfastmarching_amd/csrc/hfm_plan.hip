@@ -1036,6 +1036,8 @@ int hfm_plan_geodesics_impl(hfm_plan *p, int ntips, const double *tips, bool tip
                 err = "geodesic extraction: more than 256 steps between two geodesic points";
             else if (hst[t] == GEO_STATUS_NO_END)
                 err = "geodesic extraction: mass transport did not end";
+            else if (hst[t] == GEO_STATUS_INDEX_RANGE)
+                err = "geodesic extraction: the transported mass left the range of grid indices (|index| < 2^20)";
             else if (hlen[t] > cap)
                 err = "geodesic extraction: geodesic longer than the point buffer";
         }
