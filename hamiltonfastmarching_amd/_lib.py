@@ -98,6 +98,7 @@ SIGNATURES = [
     ("hfm_server_poll", c_int, [c_void_p, c_int64, P(c_int)]),
     ("hfm_server_wait", c_int, [c_void_p, c_int64, c_double, P(HfmRequestResult)]),
     ("hfm_server_get_stats", c_int, [c_void_p, P(HfmServerStats)]),
+    ("hfm_server_set_persistent", c_int, [c_void_p, c_int]),
     ("hfm_server_destroy", None, [c_void_p]),
     ("hfm_last_error", ctypes.c_char_p, []),
 ]

@@ -59,12 +59,10 @@ struct SolveDesc {
 // its solve's schedule).
 constexpr int SERVE_SWEEPS = 8;
 
-// Scheduling round over every running solve.  Block b covers tiles [bmap[b].y, bmap[b].y +
-// SCHED_BLOCK) of solve bmap[b].x: the grid has one block per SCHED_BLOCK tiles of each running solve.
-__global__ __launch_bounds__(SCHED_BLOCK) void k_mschedule(const SolveDesc *__restrict__ descs,
-                                                           const int2 *__restrict__ bmap, int parity,
-                                                           int *__restrict__ list, int *__restrict__ list_count) {
-    const int2 bm = bmap[blockIdx.x];
+// Scheduling of one block-table entry: tiles [bm.y, bm.y + SCHED_BLOCK) of solve bm.x, by a
+// SCHED_BLOCK-thread block; `b` (the entry's index) spreads the pending-minimum atomics.
+__device__ inline void msched_entry(const SolveDesc *__restrict__ descs, int2 bm, int b, int parity,
+                                    int *__restrict__ list, int *__restrict__ list_count) {
     const int s = bm.x;
     const SolveDesc &d = descs[s];
     const int nt = d.ctx.g.ntiles;
@@ -85,20 +83,27 @@ __global__ __launch_bounds__(SCHED_BLOCK) void k_mschedule(const SolveDesc *__re
     const unsigned left = (!take && k != KEY_NONE) ? k : KEY_NONE;
     if (take) d.ctx.keys[t] = KEY_NONE;
     const int pos = block_take(take, left, list_count, &d.count[parity],
-                               &d.pend[(parity ^ 1) * PEND_WORDS + (blockIdx.x & (PEND_SLOTS - 1)) * PEND_STRIDE],
-                               d.visits);
+                               &d.pend[(parity ^ 1) * PEND_WORDS + (b & (PEND_SLOTS - 1)) * PEND_STRIDE], d.visits);
     if (take) list[pos] = (s << SOLVE_SHIFT) | t;
 }
 
-// Update round over the packed list of (solve, tile) entries of every resident solve.
-__global__ __launch_bounds__(256) void k_mfim(const SolveDesc *__restrict__ descs, int ndesc, int parity,
-                                              const int *__restrict__ list, long long list_cap,
-                                              const int *__restrict__ list_count,
-                                              int *__restrict__ list_count_reset) {
-    __shared__ float su[4][SU_ROW];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+// Scheduling round over every running solve.  Block b covers tiles [bmap[b].y, bmap[b].y +
+// SCHED_BLOCK) of solve bmap[b].x: the grid has one block per SCHED_BLOCK tiles of each running solve.
+__global__ __launch_bounds__(SCHED_BLOCK) void k_mschedule(const SolveDesc *__restrict__ descs,
+                                                           const int2 *__restrict__ bmap, int parity,
+                                                           int *__restrict__ list, int *__restrict__ list_count) {
+    msched_entry(descs, bmap[blockIdx.x], blockIdx.x, parity, list, list_count);
+}
+
+// Update round over the packed list of (solve, tile) entries of every resident solve, by blocks
+// `blk` of `nblk` (each of blockDim.x / 64 waves, wave w with LDS row su[w]).  Block 0 also
+// clears the next round's list counter and the counters of the round after.
+__device__ inline void mfim_round(const SolveDesc *__restrict__ descs, int ndesc, int parity,
+                                  const int *__restrict__ list, long long list_cap, const int *__restrict__ list_count,
+                                  int *__restrict__ list_count_reset, int blk, int nblk, float (*su)[SU_ROW]) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int n = *list_count;
-    if (blockIdx.x == 0) {
+    if (blk == 0) {
         if (threadIdx.x == 0) *list_count_reset = 0;
         for (int i = threadIdx.x; i < ndesc; i += blockDim.x)
             if (descs[i].active) descs[i].count[parity ^ 1] = 0;
@@ -107,9 +112,9 @@ __global__ __launch_bounds__(256) void k_mfim(const SolveDesc *__restrict__ desc
             if (d.active) d.pend[parity * PEND_WORDS + (i % PEND_SLOTS) * PEND_STRIDE] = KEY_NONE;
         }
     }
-    const int gw = blockIdx.x * 4 + wv;
+    const int gw = blk * nw + wv;
     const int slot = gw & (PEND_SLOTS - 1);
-    const int stride = gridDim.x * 4;
+    const int stride = nblk * nw;
     // read this wave's first entry together with the count (entries past it are never used)
     int next = gw < list_cap ? list[gw] : 0;
     for (int i = gw; i < n; i += stride) {
@@ -124,6 +129,72 @@ __global__ __launch_bounds__(256) void k_mfim(const SolveDesc *__restrict__ desc
         else (void)fim_tile<MODEL_RIEMANN2, float, SERVE_SWEEPS>(d.ctx, tile, lane, su[wv], wm);
         // each slot on its own 128-byte line: same-line atomics serialise
         if (lane == 0 && wm != KEY_NONE) atomicMin(&d.pend[(parity ^ 1) * PEND_WORDS + slot * PEND_STRIDE], wm);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mfim(const SolveDesc *__restrict__ descs, int ndesc, int parity,
+                                              const int *__restrict__ list, long long list_cap,
+                                              const int *__restrict__ list_count,
+                                              int *__restrict__ list_count_reset) {
+    __shared__ float su[4][SU_ROW];
+    mfim_round(descs, ndesc, parity, list, list_cap, list_count, list_count_reset, blockIdx.x, gridDim.x, su);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Persistent variant of a batch of rounds, kept for comparison with the per-round launches
+// (off by default): one launch runs `nrounds` rounds, each a scheduling phase and an update
+// phase over every running solve, separated by grid-wide barriers.  One SCHED_BLOCK-thread
+// block per CU (the grid is the number of blocks that fit on the device at once, so every
+// block is resident).  A barrier publishes the block's writes to the device (agent-scope
+// release: the XCD's L2 written back), counts the block in, waits for the others, and
+// acquires (the L2's stale lines invalidated).  The wait is bounded: past GRID_SPIN_LIMIT polls a
+// block raises `abort`, and every block leaves at its next barrier, so the launch always ends.
+// ---------------------------------------------------------------------------------------------
+constexpr unsigned GRID_SPIN_LIMIT = 1u << 22;
+
+__device__ inline bool grid_barrier(unsigned long long *bar, unsigned long long target, int *abort) {
+    __shared__ int ok_s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(bar, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int ok = 1;
+        unsigned spins = 0;
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                ok = 0;
+                break;
+            }
+            if (++spins > GRID_SPIN_LIMIT) {
+                __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        ok_s = ok;
+    }
+    __syncthreads();
+    return ok_s != 0;
+}
+
+__global__ __launch_bounds__(SCHED_BLOCK) void k_mbatch(const SolveDesc *__restrict__ descs,
+                                                        const int2 *__restrict__ bmap, int nb, int ndesc,
+                                                        int parity0, int nrounds, int *__restrict__ list,
+                                                        long long list_cap, int *__restrict__ lc,
+                                                        unsigned long long *bar, unsigned long long bar_base,
+                                                        int *abort) {
+    __shared__ float su[SCHED_BLOCK / 64][SU_ROW];
+    unsigned long long target = bar_base;
+    for (int r = 0; r < nrounds; ++r) {
+        const int parity = (parity0 + r) & 1;
+        for (int b = blockIdx.x; b < nb; b += gridDim.x) msched_entry(descs, bmap[b], b, parity, list, &lc[parity]);
+        target += gridDim.x;
+        if (!grid_barrier(bar, target, abort)) return;
+        mfim_round(descs, ndesc, parity, list, list_cap, &lc[parity], &lc[parity ^ 1], blockIdx.x, gridDim.x, su);
+        target += gridDim.x;
+        if (!grid_barrier(bar, target, abort)) return;
     }
 }
 
@@ -595,6 +666,15 @@ struct hfm_server {
     int ndesc = 0;
     long long round = 0;
     long long nlaunched = 0;  // batches launched
+    // persistent batches (k_mbatch): on/off, grid (blocks resident at once), barrier counter
+    // (monotonic: each launch waits for bar_count + 2 * rounds * grid arrivals), abort flag and
+    // its per-batch host copies
+    std::atomic<int> persistent{0};
+    int grid_persist = 0;
+    unsigned long long *d_bar = nullptr;
+    unsigned long long bar_count = 0;
+    int *d_abort = nullptr;
+    int *h_abort[2] = {};
 };
 
 namespace {
@@ -887,7 +967,17 @@ void launch_batch(hfm_server *S, long long q, int batch) {
         hipLaunchKernelGGL(k_mjoin, dim3((unsigned)S->ndesc), dim3(64), 0, s, S->d_desc, (int)(S->round & 1));
         HIPCHECK(hipGetLastError());
     }
-    for (int b = 0; b < batch; ++b) {
+    const bool persistent = S->persistent.load() && S->grid_persist > 0;
+    if (persistent) {
+        const int G = S->grid_persist;
+        hipLaunchKernelGGL(k_mbatch, dim3((unsigned)G), dim3(SCHED_BLOCK), 0, s, S->d_desc, S->d_bmap,
+                           (int)bm.size(), S->ndesc, (int)(S->round & 1), batch, S->d_list, S->list_cap, S->d_lc,
+                           S->d_bar, S->bar_count, S->d_abort);
+        HIPCHECK(hipGetLastError());
+        S->bar_count += 2ull * (unsigned long long)batch * (unsigned long long)G;
+        S->round += batch;
+    }
+    for (int b = 0; b < batch && !persistent; ++b) {
         const int parity = (int)(S->round & 1);
         hipLaunchKernelGGL(k_mschedule, dim3((unsigned)bm.size()), dim3(SCHED_BLOCK), 0, s, S->d_desc, S->d_bmap,
                            parity, S->d_list, &S->d_lc[parity]);
@@ -901,6 +991,7 @@ void launch_batch(hfm_server *S, long long q, int batch) {
     HIPCHECK(hipMemcpyAsync(S->h_count[par], S->d_count, sizeof(int) * 2 * S->ndesc, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipMemcpyAsync(S->h_stats[par], S->d_stats, sizeof(unsigned long long) * S->ndesc, hipMemcpyDeviceToHost,
                             s));
+    HIPCHECK(hipMemcpyAsync(S->h_abort[par], S->d_abort, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipEventRecord(S->batch_done[par], s));
     std::lock_guard<std::mutex> lk(S->mu);
     S->stats.batches++;
@@ -914,6 +1005,8 @@ void launch_batch(hfm_server *S, long long q, int batch) {
 void process_batch(hfm_server *S, long long q, int last_parity) {
     const int par = (int)(q & 1);
     HIPCHECK(hipEventSynchronize(S->batch_done[par]));
+    if (*S->h_abort[par])
+        throw HipFail{"persistent round kernel: a grid-wide barrier did not complete (blocks not resident)"};
     std::vector<Slot *> done, still;
     for (Slot *sl : S->running) {
         const bool member = sl->first_batch <= q;
@@ -1075,8 +1168,23 @@ int hfm_server_create(hfm_server **out, double hbm_budget_bytes, int max_residen
         S->d_pend = dalloc<unsigned>((size_t)MAX_SLOTS * 2 * PEND_WORDS);
         S->d_bound = dalloc<float>(MAX_SLOTS * 4);
         S->d_stats = dalloc<unsigned long long>(MAX_SLOTS);
+        S->d_bar = dalloc<unsigned long long>(1);
+        S->d_abort = dalloc<int>(1);
+        HIPCHECK(hipMemset(S->d_bar, 0, sizeof(unsigned long long)));
+        HIPCHECK(hipMemset(S->d_abort, 0, sizeof(int)));
+        {
+            // persistent grid: blocks of k_mbatch resident at once on the whole device
+            int dev = 0, per_cu = 0;
+            hipDeviceProp_t prop;
+            HIPCHECK(hipGetDevice(&dev));
+            HIPCHECK(hipGetDeviceProperties(&prop, dev));
+            HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mbatch, SCHED_BLOCK, 0));
+            S->grid_persist = per_cu * prop.multiProcessorCount;
+        }
         for (int i = 0; i < 2; ++i) {
             HIPCHECK(hipHostMalloc((void **)&S->h_count[i], sizeof(int) * MAX_SLOTS * 2, hipHostMallocDefault));
+            HIPCHECK(hipHostMalloc((void **)&S->h_abort[i], sizeof(int), hipHostMallocDefault));
+            *S->h_abort[i] = 0;
             HIPCHECK(hipHostMalloc((void **)&S->h_stats[i], sizeof(unsigned long long) * MAX_SLOTS,
                                    hipHostMallocDefault));
         }
@@ -1226,6 +1334,19 @@ int hfm_server_wait(hfm_server *S, int64_t ticket, double timeout_ms, hfm_reques
     return 0;
 }
 
+int hfm_server_set_persistent(hfm_server *S, int on) {
+    if (!S) {
+        set_error("hfm_server_set_persistent: invalid argument");
+        return 1;
+    }
+    if (on && S->grid_persist <= 0) {
+        set_error("hfm_server_set_persistent: the device cannot hold the persistent round kernel");
+        return 1;
+    }
+    S->persistent.store(on ? 1 : 0);
+    return 0;
+}
+
 int hfm_server_get_stats(hfm_server *S, hfm_server_stats *st) {
     if (!S || !st) return 1;
     std::lock_guard<std::mutex> lk(S->mu);
@@ -1264,12 +1385,15 @@ void hfm_server_destroy(hfm_server *S) {
     dfree(S->d_pend);
     dfree(S->d_bound);
     dfree(S->d_stats);
+    dfree(S->d_bar);
+    dfree(S->d_abort);
     delete[] S->h_desc;
     if (S->d_bmap) (void)hipFree(S->d_bmap);
     for (int i = 0; i < 2; ++i) {
         if (S->h_desc_stage[i]) (void)hipHostFree(S->h_desc_stage[i]);
         if (S->h_bmap_stage[i]) (void)hipHostFree(S->h_bmap_stage[i]);
         if (S->h_count[i]) (void)hipHostFree(S->h_count[i]);
+        if (S->h_abort[i]) (void)hipHostFree(S->h_abort[i]);
         if (S->h_stats[i]) (void)hipHostFree(S->h_stats[i]);
         if (S->batch_done[i]) (void)hipEventDestroy(S->batch_done[i]);
     }

@@ -21,13 +21,20 @@ __all__ = ["Server", "make_requests", "run_load", "run_closed_loop", "run_sequen
 
 
 class Server:
-    def __init__(self, hbm_budget_bytes: float = 64e9, max_resident: int = 256):
+    def __init__(self, hbm_budget_bytes: float = 64e9, max_resident: int = 256, persistent: bool = False):
+        """persistent: run each batch of rounds in one persistent launch with grid-wide barriers
+        instead of a scheduling and an update launch per round (hfm_server_set_persistent)."""
         self._L = _lib.load()
         h = ctypes.c_void_p()
         if self._L.hfm_server_create(ctypes.byref(h), float(hbm_budget_bytes), int(max_resident)):
             raise PyExp(_lib.last_error())
         self._h = h
         self._keep = {}
+        if persistent and self._L.hfm_server_set_persistent(self._h, 1):
+            err = _lib.last_error()
+            self._L.hfm_server_destroy(self._h)
+            self._h = None
+            raise PyExp(err)
 
     def submit(self, model: str, dims, grid_scale: float, metric, seeds, seed_values=None, values_out=None,
                dual: bool = False) -> int:
@@ -294,7 +301,8 @@ def run_sequential(requests, collect_values: bool = False):
 
 def main(argv=None):
     """python -m hamiltonfastmarching_amd.serving [--requests N] [--concurrency C] [--rounds R]
-    [--burst] [--seed S]: the closed-loop sustained rate (or a burst) on the request mix, as JSON."""
+    [--burst] [--seed S] [--persistent]: the closed-loop sustained rate (or a burst) on the request
+    mix, as JSON."""
     import argparse
     import json
     ap = argparse.ArgumentParser()
@@ -303,9 +311,10 @@ def main(argv=None):
     ap.add_argument("--rounds", type=int, default=3)
     ap.add_argument("--burst", action="store_true")
     ap.add_argument("--seed", type=int, default=11)
+    ap.add_argument("--persistent", action="store_true", help="persistent round kernel (grid-wide barriers)")
     a = ap.parse_args(argv)
     reqs = make_requests(a.requests, seed=a.seed)
-    srv = Server(hbm_budget_bytes=64e9, max_resident=256)
+    srv = Server(hbm_budget_bytes=64e9, max_resident=256, persistent=a.persistent)
     try:
         run_load(srv, reqs[:16], qps=0)
         out = run_load(srv, reqs, qps=0) if a.burst else run_closed_loop(srv, reqs, a.concurrency, a.rounds)

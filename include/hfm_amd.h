@@ -329,6 +329,12 @@ int hfm_server_poll(hfm_server *srv, int64_t ticket, int *done);
  * timeout or unknown ticket. */
 int hfm_server_wait(hfm_server *srv, int64_t ticket, double timeout_ms, hfm_request_result *res);
 int hfm_server_get_stats(hfm_server *srv, hfm_server_stats *st);
+/* Round execution: on = 0 (default) launches a scheduling kernel and an update kernel per
+ * round; on = 1 runs each batch of 8 rounds in one persistent launch with grid-wide barriers
+ * (one block per CU).  Takes effect from the next batch; returns nonzero if the device cannot
+ * hold the persistent grid.  No counterpart in the reference (its solver is a serial heap loop,
+ * HamiltonFastMarching.hxx:59-112): this selects between two executions of the same rounds. */
+int hfm_server_set_persistent(hfm_server *srv, int on);
 /* Stop the worker threads (waits for in-flight solves) and free every device buffer. */
 void hfm_server_destroy(hfm_server *srv);
 

@@ -148,3 +148,20 @@ def test_top_of_range_requests_alone_mix_oracle():
         err = _rel(u, ref)
         print(f"  {r['model']} {r['dims']} kappa {r['kappa']:.1f}: rel err vs oracle {err:.2e}", flush=True)
         assert err <= 1e-5
+
+
+def test_persistent_rounds_match_plan(reqs):
+    """Rounds run by the persistent kernel (a batch of rounds per launch, grid-wide barriers between
+    the scheduling and update phases) give each solve's fixed point: within 1e-6 of a plain plan
+    solve, as the per-round launches are."""
+    from hamiltonfastmarching_amd import serving
+    print("serving persistent rounds", flush=True)
+    srv = serving.Server(hbm_budget_bytes=8e9, max_resident=64, persistent=True)
+    mix = serving.run_load(srv, reqs, qps=0, collect_values=True)
+    st = srv.stats()
+    srv.close()
+    assert mix["completed"] == len(reqs)
+    seq = serving.run_sequential(reqs, collect_values=True)
+    worst = max(_rel(m, s) for m, s in zip(mix["values"], seq["values"]))
+    print(f"persistent: {st['batches']} batches, {st['rounds']} rounds, vs plan {worst:.2e}", flush=True)
+    assert worst <= 1e-6
