@@ -3,7 +3,8 @@
 
 * seedRadius spreading (StaticFactoring.hxx:284-345): spreadedSeeds / spreadedSeedValues exactly,
   values 1e-12 relative, valueVariation of the given seeds 1e-11 (primal and dual metrics,
-  2-D and 3-D);
+  2-D and 3-D; a negative radius also seeds one step of the reversed stencil from every point of
+  the box);
 * seeds_Unoriented / seedValues_Unoriented (StaticFactoring.hxx:351-380) and tips_Unoriented
   (HFMInterface.hxx:498-522) of the curvature models: values 1e-12, geodesicLengths_Unoriented
   equal, geodesicPoints_Unoriented within 1e-6 of a grid step;
