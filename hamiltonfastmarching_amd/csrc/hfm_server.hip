@@ -680,6 +680,12 @@ struct hfm_server {
     unsigned long long bar_count = 0;
     int *d_abort = nullptr;
     int *h_abort[2] = {};
+    // Device memory of the solves (arenas, dependency lists, staged metrics, the round list):
+    // the server's own stream-ordered pool, which keeps freed memory for reuse (release
+    // threshold: unlimited) until the server is destroyed.  With the default pool's threshold
+    // of 0 a free returned memory to the system, and hipFreeAsync / hipMallocAsync calls of
+    // 1 ms or more held up the compute and setup threads.
+    hipMemPool_t pool = nullptr;
 };
 
 namespace {
@@ -723,7 +729,7 @@ void setup_group(hfm_server *S, std::vector<SetupItem> &items, int which) {
         Slot *sl = new Slot;
         items[g].sl = sl;
         sl->rec = r;
-        HIPCHECK(hipMallocAsync((void **)&sl->arena, arena_bytes(*r), s));
+        HIPCHECK(hipMallocFromPoolAsync((void **)&sl->arena, arena_bytes(*r), S->pool, s));
         char *p = sl->arena;
         sl->u = carve<float>(p, r->np);
         sl->offs = carve<uint32_t>(p, r->np * r->K);
@@ -743,7 +749,7 @@ void setup_group(hfm_server *S, std::vector<SetupItem> &items, int which) {
         const double *metric = r->metric_dev;
         if (!metric) {
             const size_t mcount = r->metric_is_field ? (size_t)r->N * r->K : (size_t)r->K;
-            HIPCHECK(hipMallocAsync((void **)&sl->metric_stage, mcount * sizeof(double), s));
+            HIPCHECK(hipMallocFromPoolAsync((void **)&sl->metric_stage, mcount * sizeof(double), S->pool, s));
             HIPCHECK(hipMemcpyAsync(sl->metric_stage, r->metric_host.data(), mcount * sizeof(double),
                                     hipMemcpyHostToDevice, s));
             metric = sl->metric_stage;
@@ -805,7 +811,7 @@ void setup_group(hfm_server *S, std::vector<SetupItem> &items, int which) {
             items[g].sl = nullptr;
             continue;
         }
-        HIPCHECK(hipMallocAsync((void **)&sl->deps, sizeof(int) * std::max(rep.total, 1), s));
+        HIPCHECK(hipMallocFromPoolAsync((void **)&sl->deps, sizeof(int) * std::max(rep.total, 1), S->pool, s));
         const double tau = rep.npts > 0 ? rep.tau_sum / rep.npts : 0.;
         sl->delta = (float)(8. * tau);
         if (m != g) hj[m] = hj[g];
@@ -937,7 +943,7 @@ void launch_batch(hfm_server *S, long long q, int batch) {
     if (tiles > S->list_cap) {
         if (S->d_list) HIPCHECK(hipFreeAsync(S->d_list, s));
         S->list_cap = std::max<long long>(tiles, 2 * S->list_cap);
-        HIPCHECK(hipMallocAsync((void **)&S->d_list, S->list_cap * sizeof(int), s));
+        HIPCHECK(hipMallocFromPoolAsync((void **)&S->d_list, S->list_cap * sizeof(int), S->pool, s));
     }
     if ((long long)bm.size() > S->bmap_cap) {
         // The staging buffer of the other parity may still feed an in-flight copy: wait for it.
@@ -1147,6 +1153,18 @@ int hfm_server_create(hfm_server **out, double hbm_budget_bytes, int max_residen
     }
     hfm_server *S = new hfm_server;
     try {
+        {
+            int dev = 0;
+            HIPCHECK(hipGetDevice(&dev));
+            hipMemPoolProps props = {};
+            props.allocType = hipMemAllocationTypePinned;
+            props.handleTypes = hipMemHandleTypeNone;
+            props.location.type = hipMemLocationTypeDevice;
+            props.location.id = dev;
+            HIPCHECK(hipMemPoolCreate(&S->pool, &props));
+            uint64_t keep = UINT64_MAX;
+            HIPCHECK(hipMemPoolSetAttribute(S->pool, hipMemPoolAttrReleaseThreshold, &keep));
+        }
         S->budget = hbm_budget_bytes;
         S->max_resident = std::min(max_resident, MAX_SLOTS);
         S->stats.hbm_budget = hbm_budget_bytes;
@@ -1197,7 +1215,7 @@ int hfm_server_create(hfm_server **out, double hbm_budget_bytes, int max_residen
         S->th_comp = std::thread(compute_loop, S);
     } catch (const HipFail &e) {
         set_error(e.msg);
-        delete S;
+        hfm_server_destroy(S);  // no worker thread started: frees what was allocated
         return 1;
     }
     *out = S;
@@ -1377,7 +1395,10 @@ void hfm_server_destroy(hfm_server *S) {
     if (S->s_comp) (void)hipStreamSynchronize(S->s_comp);
     for (int i = 0; i < hfm_server::NSETUP; ++i)
         if (S->s_setup[i]) (void)hipStreamSynchronize(S->s_setup[i]);
-    if (S->d_list) (void)hipFree(S->d_list);
+    if (S->d_list) {
+        (void)hipFreeAsync(S->d_list, S->s_comp);
+        (void)hipStreamSynchronize(S->s_comp);
+    }
     for (int i = 0; i < hfm_server::NSETUP; ++i) {
         dfree(S->d_jobs[i]);
         if (S->h_jobs[i]) (void)hipHostFree(S->h_jobs[i]);
@@ -1405,6 +1426,8 @@ void hfm_server_destroy(hfm_server *S) {
     if (S->s_comp) (void)hipStreamDestroy(S->s_comp);
     for (int i = 0; i < hfm_server::NSETUP; ++i)
         if (S->s_setup[i]) (void)hipStreamDestroy(S->s_setup[i]);
+    // every allocation from the pool has been freed on a stream synchronised above
+    if (S->pool) (void)hipMemPoolDestroy(S->pool);
     delete S;
 }
 
