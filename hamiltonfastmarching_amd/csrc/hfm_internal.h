@@ -115,6 +115,15 @@ struct hfm_plan {
     long long *fwd_sp = nullptr;
     int fwd_seed_cap = 0;
 
+    // geodesic scratch (grown on demand, kept for the next call): tips, lengths and status on the
+    // device, the points buffer (doubles), and one pinned host buffer for the read-back
+    double *geo_tips = nullptr, *geo_pts = nullptr;
+    long long *geo_len = nullptr;
+    int *geo_st = nullptr;
+    long long geo_tip_cap = 0, geo_pts_cap = 0;
+    char *geo_host = nullptr;
+    size_t geo_host_cap = 0;
+
     // walls (hfm_plan_set_walls): row-major bytes, the visibility mask derived from them and the
     // current stencils (rebuilt when either changes), scratch for the wall distances
     uint8_t *walls = nullptr;

@@ -101,6 +101,11 @@ void plan_free(hfm_plan *p) {
     dfree(p->red);
     dfree(p->adj_lambda);
     dfree(p->adj_sens);
+    dfree(p->geo_tips);
+    dfree(p->geo_pts);
+    dfree(p->geo_len);
+    dfree(p->geo_st);
+    if (p->geo_host) (void)hipHostFree(p->geo_host);
     dfree(p->adj_ecnt);
     dfree(p->adj_eidx);
     dfree(p->adj_var);
@@ -998,32 +1003,63 @@ int hfm_plan_geodesics_impl(hfm_plan *p, int ntips, const double *tips, bool tip
         h.max_iter = 4 * p->np + 4096;
         long long cap = 1024;
         for (int i = 0; i < dim; ++i) cap += 8LL * p->g.n[i];
-        double *d_tips = nullptr, *d_pts = nullptr;
-        long long *d_len = nullptr;
-        int *d_st = nullptr;
+        if (ntips > p->geo_tip_cap) {
+            dfree(p->geo_tips);
+            dfree(p->geo_len);
+            dfree(p->geo_st);
+            p->geo_tips = nullptr;
+            p->geo_len = nullptr;
+            p->geo_st = nullptr;
+            p->geo_tip_cap = 0;
+            p->geo_tips = dalloc<double>((size_t)ntips * dim);
+            p->geo_len = dalloc<long long>(ntips);
+            p->geo_st = dalloc<int>(ntips);
+            p->geo_tip_cap = ntips;
+        }
+        // pinned read-back layout: lengths, status, then the points of every tip (cap per tip)
+        auto ensure_host = [&](size_t bytes) {
+            if (bytes <= p->geo_host_cap) return;
+            if (p->geo_host) HIPCHECK(hipHostFree(p->geo_host));
+            p->geo_host = nullptr;
+            p->geo_host_cap = 0;
+            HIPCHECK(hipHostMalloc((void **)&p->geo_host, bytes, hipHostMallocDefault));
+            p->geo_host_cap = bytes;
+        };
+        const size_t head = (((size_t)ntips * (sizeof(long long) + sizeof(int)) + 255) / 256) * 256;
+        ensure_host(std::max(head, tad.size() * sizeof(double)));
+        std::memcpy(p->geo_host, tad.data(), tad.size() * sizeof(double));
+        HIPCHECK(hipMemcpyAsync(p->geo_tips, p->geo_host, tad.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHECK(hipStreamSynchronize(s));  // the pinned buffer is reused for the read-back below
         std::vector<long long> hlen(ntips);
         std::vector<int> hst(ntips);
-        HIPCHECK(hipMallocAsync((void **)&d_tips, tad.size() * sizeof(double), s));
-        HIPCHECK(hipMallocAsync((void **)&d_len, ntips * sizeof(long long), s));
-        HIPCHECK(hipMallocAsync((void **)&d_st, ntips * sizeof(int), s));
-        HIPCHECK(hipMemcpyAsync(d_tips, tad.data(), tad.size() * sizeof(double), hipMemcpyHostToDevice, s));
         for (int attempt = 0;; ++attempt) {
             h.max_points = cap;
-            HIPCHECK(hipMallocAsync((void **)&d_pts, (size_t)ntips * cap * dim * sizeof(double), s));
+            const long long need = (long long)ntips * cap * dim;
+            if (need > p->geo_pts_cap) {
+                dfree(p->geo_pts);
+                p->geo_pts = nullptr;
+                p->geo_pts_cap = 0;
+                p->geo_pts = dalloc<double>((size_t)need);
+                p->geo_pts_cap = need;
+            }
             const hipError_t e =
                 p->precision == 32
                     ? launch_geodesics<float>(p->model, p->g, p->sd<float>(), static_cast<const float *>(p->u),
-                                              p->fixedmask, h, ntips, d_tips, d_pts, d_len, d_st, s)
+                                              p->fixedmask, h, ntips, p->geo_tips, p->geo_pts, p->geo_len, p->geo_st, s)
                     : launch_geodesics<double>(p->model, p->g, p->sd<double>(), static_cast<const double *>(p->u),
-                                               p->fixedmask, h, ntips, d_tips, d_pts, d_len, d_st, s);
+                                               p->fixedmask, h, ntips, p->geo_tips, p->geo_pts, p->geo_len,
+                                               p->geo_st, s);
             HIPCHECK(e);
-            HIPCHECK(hipMemcpyAsync(hlen.data(), d_len, ntips * sizeof(long long), hipMemcpyDeviceToHost, s));
-            HIPCHECK(hipMemcpyAsync(hst.data(), d_st, ntips * sizeof(int), hipMemcpyDeviceToHost, s));
+            long long *hl = reinterpret_cast<long long *>(p->geo_host);
+            int *hs = reinterpret_cast<int *>(p->geo_host + (size_t)ntips * sizeof(long long));
+            HIPCHECK(hipMemcpyAsync(hl, p->geo_len, ntips * sizeof(long long), hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipMemcpyAsync(hs, p->geo_st, ntips * sizeof(int), hipMemcpyDeviceToHost, s));
             HIPCHECK(hipStreamSynchronize(s));
+            std::memcpy(hlen.data(), hl, ntips * sizeof(long long));
+            std::memcpy(hst.data(), hs, ntips * sizeof(int));
             long long mx = 0;
             for (long long l : hlen) mx = std::max(mx, l);
             if (mx <= cap || attempt >= 2) break;
-            HIPCHECK(hipFreeAsync(d_pts, s));
             cap = mx;
         }
         int failed = 0;
@@ -1043,22 +1079,25 @@ int hfm_plan_geodesics_impl(hfm_plan *p, int ntips, const double *tips, bool tip
         }
         long long total = 0;
         if (err.empty()) {
+            if (points && max_points > 0) {
+                // the whole point buffer in one copy into pinned memory, then each tip's points
+                const size_t bytes = (size_t)ntips * cap * dim * sizeof(double);
+                ensure_host(bytes);
+                HIPCHECK(hipMemcpyAsync(p->geo_host, p->geo_pts, bytes, hipMemcpyDeviceToHost, s));
+                HIPCHECK(hipStreamSynchronize(s));
+            }
+            const double *hp = reinterpret_cast<const double *>(p->geo_host);
             for (int t = 0; t < ntips; ++t) {
                 lengths[t] = hlen[t];
                 const long long n = hlen[t];
                 if (points && total < max_points) {
                     const long long take = std::min<long long>(n, max_points - total);
-                    HIPCHECK(hipMemcpyAsync(points + (size_t)total * dim, d_pts + (size_t)t * cap * dim,
-                                            (size_t)take * dim * sizeof(double), hipMemcpyDeviceToHost, s));
+                    std::memcpy(points + (size_t)total * dim, hp + (size_t)t * cap * dim,
+                                (size_t)take * dim * sizeof(double));
                 }
                 total += n;
             }
         }
-        HIPCHECK(hipFreeAsync(d_pts, s));
-        HIPCHECK(hipFreeAsync(d_tips, s));
-        HIPCHECK(hipFreeAsync(d_len, s));
-        HIPCHECK(hipFreeAsync(d_st, s));
-        HIPCHECK(hipStreamSynchronize(s));
         if (!err.empty()) throw HipFail{err};
         *npoints = total;
         if (nfailed) *nfailed = failed;
