@@ -396,6 +396,13 @@ class Plan:
         self._check(self._L.hfm_plan_set_fused_schedule(self._h, -1 if enable is None else int(bool(enable))))
         return self
 
+    def set_order(self, order: int):
+        """Scheme order (the `order` key): 1, or 2 / 3 for the recompute at acceptance with second /
+        third order differences, solved by the reference's serial acceptance loop on the device
+        (FP64 plans without walls)."""
+        self._check(self._L.hfm_plan_set_order(self._h, int(order)))
+        return self
+
 
 class EikonalSolve:
     """``EikonalSolve.apply(cost, plan, seeds, targets, weights)`` -> sum_t weights[t] u(targets[t]).

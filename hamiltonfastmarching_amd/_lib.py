@@ -87,6 +87,7 @@ SIGNATURES = [
     ("hfm_plan_set_timing", c_int, [c_void_p, c_int]),
     ("hfm_plan_set_graph", c_int, [c_void_p, c_int]),
     ("hfm_plan_set_fused_schedule", c_int, [c_void_p, c_int]),
+    ("hfm_plan_set_order", c_int, [c_void_p, c_int]),
     ("hfm_plan_set_delta", c_int, [c_void_p, c_double]),
     ("hfm_plan_set_round_target", c_int, [c_void_p, c_int]),
     ("hfm_plan_set_grid", c_int, [c_void_p, c_int]),

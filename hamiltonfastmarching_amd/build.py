@@ -22,7 +22,8 @@ OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libhfm_amd.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
-SOURCES = ["hfm_kernels.hip", "hfm_plan.hip", "hfm_adjoint.hip", "hfm_server.hip", "hfm_geodesic.hip", "hfm_walls.hip", "hfm_io.cpp"]
+SOURCES = ["hfm_kernels.hip", "hfm_plan.hip", "hfm_adjoint.hip", "hfm_server.hip", "hfm_geodesic.hip", "hfm_walls.hip",
+           "hfm_exact.hip", "hfm_io.cpp"]
 HEADERS = ["hfm_device.h", "hfm_kernels.h", "hfm_internal.h", "selling.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

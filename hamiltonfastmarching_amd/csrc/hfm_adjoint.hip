@@ -533,6 +533,11 @@ int hfm_plan_adjoint_impl(hfm_plan *p, int ntargets, const double *targets, cons
         set_error("hfm_plan_adjoint: no solve has run");
         return 1;
     }
+    if (p->exact_solved) {
+        set_error("hfm_plan_adjoint: differentiation of an order " + std::to_string(p->order) +
+                  " solve is not supported by this library (order 1 only)");
+        return 1;
+    }
     try {
         hipStream_t s = static_cast<hipStream_t>(stream);
         // Targets: ADim + floor; out of range is an error; a repeated index keeps its first
@@ -638,6 +643,11 @@ int hfm_plan_forward_variation_impl(hfm_plan *p, int nvar, const double *d_cost_
     }
     if (!p->solved) {
         set_error("hfm_plan_forward_variation: no solve has run");
+        return 1;
+    }
+    if (p->exact_solved) {
+        set_error("hfm_plan_forward_variation: differentiation of an order " + std::to_string(p->order) +
+                  " solve is not supported by this library (order 1 only)");
         return 1;
     }
     if (nvar <= 0) {

@@ -1,0 +1,464 @@
+// hfm_exact.hip -- the reference's serial acceptance loop on the device, for the options whose
+// result is fixed by the order in which the heap accepts points: the second / third order
+// recompute at acceptance (`order` = 2 or 3, Headers/Base/Implementation/HamiltonFastMarching.hxx
+// :59-123 with Recompute :187-272 and EulerianStencil.hxx:155-233).
+//
+// With order >= 2 a point's value is recomputed when it leaves the heap, from its active
+// differences and, where the next points along an offset are already accepted and no larger,
+// second (third) order combinations of their values.  The recomputed value replaces the one every
+// later update reads, and the corrected values are not monotone in the neighbours: there is no
+// order-independent fixed point for a parallel iteration to converge to, and pops whose key lies
+// below the previous pop's (0.2 to 1.2 % of them on the test fields) change which neighbours
+// count as accepted.  The loop therefore runs as the reference runs it: one thread pops a binary
+// heap kept in device memory -- the push / pop steps of libstdc++'s std::priority_queue, so that
+// equal keys leave in the same order -- updates the popped point's reversed neighbours with the
+// incremental quadratic of HopfLaxUpdate, and recomputes each point at acceptance.  Every value
+// is FP64, in the reference's order of operations.  It is correct rather than fast: one GPU thread
+// walks the heap with dependent device-memory accesses.
+#include "hfm_kernels.h"
+
+#include "hfm_device.h"
+
+namespace hfm {
+
+namespace {
+
+struct HeapEntry {
+    double value;
+    long long p;  // tiled point index
+};
+
+// The queue's order (QueueElement::operator<, HamiltonFastMarching.hxx:51-56): a larger value
+// has the lower priority.
+__device__ inline bool heap_less(const HeapEntry &a, const HeapEntry &b) { return a.value > b.value; }
+
+// libstdc++'s __push_heap: move the hole up while its parent has lower priority than v.
+__device__ inline void heap_sift_up(HeapEntry *h, long long hole, long long top, const HeapEntry v) {
+    long long parent = (hole - 1) / 2;
+    while (hole > top && heap_less(h[parent], v)) {
+        h[hole] = h[parent];
+        hole = parent;
+        parent = (hole - 1) / 2;
+    }
+    h[hole] = v;
+}
+
+// std::priority_queue::push: push_back then push_heap.
+__device__ inline void heap_push(HeapEntry *h, long long &n, const HeapEntry v) {
+    ++n;
+    heap_sift_up(h, n - 1, 0, v);
+}
+
+// std::priority_queue::pop after top(): pop_heap (the last element re-enters through
+// __adjust_heap from the root: the hole goes down to a leaf along the higher-priority child, the
+// right one on equality, then the element moves up), then pop_back.
+__device__ inline HeapEntry heap_pop(HeapEntry *h, long long &n) {
+    const HeapEntry top = h[0];
+    if (n > 1) {
+        const HeapEntry v = h[n - 1];
+        const long long len = n - 1;
+        long long hole = 0, second = 0;
+        while (second < (len - 1) / 2) {
+            second = 2 * (second + 1);
+            if (heap_less(h[second], h[second - 1])) --second;
+            h[hole] = h[second];
+            hole = second;
+        }
+        if ((len & 1) == 0 && second == (len - 2) / 2) {
+            second = 2 * (second + 1);
+            h[hole] = h[second - 1];
+            hole = second - 1;
+        }
+        heap_sift_up(h, hole, 0, v);
+    }
+    --n;
+    return top;
+}
+
+// Tiled index of the point c + s * o, or -1 off a closed axis.
+template <int ModelT> __device__ inline long long shifted(const Geom &g, const int *c, int o0, int o1, int o2, int s) {
+    int r[3];
+    if (!shift_coords_m<ModelT>(g, c, s * o0, s * o1, s * o2, r)) return -1;
+    return (long long)tile_of(g, r) * 64 + local_of(g, r);
+}
+
+// QuadraticMax<ScalarType,1> (EulerianStencil.hxx:30-72): a, b, c relative to minVal, c from -1.
+struct Quad {
+    double minVal, a, b, c;
+    __device__ void add(double value, double w) {
+        const double v = value - minVal;
+        const double wv = w * v, wv2 = (w * v) * v;
+        a += w;
+        b += wv;
+        c += wv2;
+    }
+    __device__ double solve() const {
+        if (a == 0.) return minVal + __builtin_huge_val();
+        const double delta2 = b * b - a * c;
+        const double delta = sqrt(delta2 > 0. ? delta2 : 0.);
+        return minVal + (b + delta) / a;
+    }
+};
+
+// One active difference of x as it enters the recompute: its value (the accepted neighbour's,
+// or the second / third order combination), the order used, and the signed offset.
+struct RecomputeEntry {
+    double value, w;
+    int ord;
+    int o[3];
+};
+
+// GetValueCorr of Recompute (HamiltonFastMarching.hxx:199-262) for the difference x -> x + o
+// (o signed), without factoring: the neighbour's value, or (order >= 2) (2/3)(2 v1 - v2 / 2) when
+// x + 2 o is accepted and v2 <= v1, or (order >= 3) (6/11)(3 v1 - 3 v2 / 2 + v3 / 3) when x + 3 o
+// also is and v3 <= v2.  The small-correction tests apply to strictly causal schemes only
+// (HamiltonFastMarching.h:46), which these are not.
+template <int ModelT>
+__device__ inline double value_corr(const Geom &g, const double *u, const unsigned char *acc, const int *c, const int *o,
+                                    int order, int &ord) {
+    int c1[3], c2[3], c3[3];
+    shift_coords_m<ModelT>(g, c, o[0], o[1], o[2], c1);  // active: in the domain and accepted
+    const double v1 = u[(long long)tile_of(g, c1) * 64 + local_of(g, c1)];
+    ord = 1;
+    double val = v1 + 0.;
+    if (order >= 2 && shift_coords_m<ModelT>(g, c1, o[0], o[1], o[2], c2)) {
+        const long long q2 = (long long)tile_of(g, c2) * 64 + local_of(g, c2);
+        const double v2 = u[q2];
+        if (acc[q2] && !(v2 > v1)) {
+            ord = 2;
+            val = (2. / 3.) * ((2. * v1 - 0.5 * v2) + 0.);
+            if (order >= 3 && shift_coords_m<ModelT>(g, c2, o[0], o[1], o[2], c3)) {
+                const long long q3 = (long long)tile_of(g, c3) * 64 + local_of(g, c3);
+                const double v3 = u[q3];
+                if (acc[q3] && !(v3 > v2)) {
+                    ord = 3;
+                    val = (6. / 11.) * ((3. * v1 - 1.5 * v2 + (1. / 3.) * v3) + 0.);
+                }
+            }
+        }
+    }
+    return val;
+}
+
+// Recompute (HamiltonFastMarching.hxx:187-272, EulerianStencil.hxx:155-233) of the point at c
+// from its active differences `act` in stencil order: the entries into e[], their number
+// returned, and the recomputed value in *value.  The values enter one quadratic relative to their
+// minimum with the weights scaled by 1, 9/4, 121/36 by order.
+template <int ModelT>
+__device__ inline int recompute(const Geom &g, const StencilData<double> &sd, const double *u, const unsigned char *acc,
+                                long long p, const int *c, unsigned long long act, int order, RecomputeEntry *e,
+                                double *value) {
+    using S = ModelShape<ModelT>;
+    constexpr int NF = S::NFWD, NC = NF + S::NSYM;
+    LaneStencil<ModelT, double> st;
+    st.load(g, sd, p, c);
+    int n = 0;
+    for (int k = 0; k < NC; ++k) {
+        int o0, o1, o2;
+        unpack_offset(st.off[k], o0, o1, o2);
+        for (int s = 0; s < (k < NF ? 1 : 2); ++s) {
+            if (!(act >> side_bit<NF>(k, s) & 1ull)) continue;
+            const int sg = s == 0 ? 1 : -1;
+            RecomputeEntry &r = e[n++];
+            r.o[0] = sg * o0;
+            r.o[1] = sg * o1;
+            r.o[2] = sg * o2;
+            r.w = st.w[k];
+            r.value = value_corr<ModelT>(g, u, acc, c, r.o, order, r.ord);
+        }
+    }
+    Quad q{__builtin_huge_val(), 0., 0., -1.};
+    for (int i = 0; i < n; ++i) q.minVal = e[i].value < q.minVal ? e[i].value : q.minVal;
+    constexpr double mult2[4] = {0., 1., 9. / 4., 121. / 36.};
+    for (int i = 0; i < n; ++i) q.add(e[i].value, mult2[e[i].ord] * e[i].w);
+    *value = q.solve();
+    return n;
+}
+
+// The serial loop (HamiltonFastMarching.hxx:59-169), run by thread 0 of a one-wave block.
+template <int ModelT>
+__global__ __launch_bounds__(64) void k_fm_exact(ExactArgs a) {
+    if (threadIdx.x != 0) return;
+    using S = ModelShape<ModelT>;
+    constexpr int NF = S::NFWD, NC = NF + S::NSYM;
+    const Geom &g = a.g;
+    HeapEntry *heap = static_cast<HeapEntry *>(a.heap);
+    long long n = 0, accepted = 0, maxn = 0;
+    // RunInit: the seeds in index order, values set and pushed
+    for (int i = 0; i < a.nseeds; ++i) {
+        a.u[a.seed_p[i]] = a.seed_v[i];
+        heap_push(heap, n, HeapEntry{a.seed_v[i], a.seed_p[i]});
+    }
+    RecomputeEntry re[2 * NC];
+    while (n > 0) {
+        maxn = n > maxn ? n : maxn;
+        HeapEntry top = heap_pop(heap, n);
+        const long long y = top.p;
+        if (a.accepted[y]) continue;
+        int cy[3];
+        tiled_coords(g, (int)(y >> 6), (int)(y & 63), cy);
+        // PostProcess with recompute: a point without active differences keeps its value
+        if (a.order > 1 && a.active[y]) {
+            double v;
+            (void)recompute<ModelT>(g, a.sd, a.u, a.accepted, y, cy, a.active[y], a.order, re, &v);
+            top.value = v;
+            a.u[y] = v;
+        }
+        a.accepted[y] = 1;
+        ++accepted;
+        const long long key = S::SHARED ? (long long)cy[2] : y;
+        const long long r0 = a.rstart[key], r1 = r0 + a.rcount[key];
+        for (long long r = r0; r < r1; ++r) {
+            int e0, e1, e2;
+            unpack_offset(a.roffs[r], e0, e1, e2);
+            // ConditionalUpdate (HamiltonFastMarching.hxx:136-154)
+            const long long x = shifted<ModelT>(g, cy, e0, e1, e2, -1);
+            if (x < 0 || a.accepted[x]) continue;
+            unsigned long long act = a.active[x];
+            if (act == 0ull && a.u[x] != __builtin_huge_val()) continue;  // seeds are never updated
+            if (a.u[x] <= top.value) continue;
+            // HopfLaxUpdate (EulerianStencil.hxx:104-153): activate the differences with this
+            // offset (a symmetric one only if its other side is not active), add them to the
+            // point's quadratic, solve
+            int cx[3];
+            tiled_coords(g, (int)(x >> 6), (int)(x & 63), cx);
+            LaneStencil<ModelT, double> st;
+            st.load(g, a.sd, x, cx);
+            double *qs = a.quad + 4 * x;
+            Quad q{qs[0], qs[1], qs[2], qs[3]};
+            if (q.minVal == -__builtin_huge_val()) q.minVal = top.value;
+            const uint32_t eo = pack_offset(e0, e1, e2), meo = pack_offset(-e0, -e1, -e2);
+            for (int k = 0; k < NC; ++k) {
+                if (k < NF) {
+                    if (st.off[k] == eo) {
+                        act |= 1ull << side_bit<NF>(k, 0);
+                        q.add(top.value, st.w[k]);
+                    }
+                    continue;
+                }
+                const int b0 = side_bit<NF>(k, 0), b1 = side_bit<NF>(k, 1);
+                if (st.off[k] == eo && !(act >> b1 & 1ull)) {
+                    act |= 1ull << b0;
+                    q.add(top.value, st.w[k]);
+                }
+                if (st.off[k] == meo && !(act >> b0 & 1ull)) {
+                    act |= 1ull << b1;
+                    q.add(top.value, st.w[k]);
+                }
+            }
+            const double value = q.solve();
+            qs[0] = q.minVal;
+            qs[1] = q.a;
+            qs[2] = q.b;
+            qs[3] = q.c;
+            a.active[x] = act;
+            a.u[x] = value;
+            if (n >= a.heap_cap) {  // cannot happen: the capacity is the number of pushes possible
+                a.out[2] = 1;
+                a.out[0] = accepted;
+                return;
+            }
+            heap_push(heap, n, HeapEntry{value, x});
+        }
+    }
+    a.out[0] = accepted;
+    a.out[1] = maxn;
+    a.out[2] = 0;
+}
+
+// Reversed offsets of the per-point stencils (HFM_StencilRecompute.hxx:81-124): every point y
+// lists the offsets e for which some x = y - e has a nonzero-weight difference reaching y
+// (x + e for a forward one, x +- e for a symmetric one), sorted and without repetitions.  Pass
+// 0 counts, pass 1 fills (positions from an exclusive scan), then each list is sorted.
+template <int ModelT>
+__global__ __launch_bounds__(256) void k_rev_build(Geom g, StencilData<double> sd, int pass, int *__restrict__ cnt,
+                                                  const int *__restrict__ start, int *__restrict__ fill,
+                                                  uint32_t *__restrict__ roffs, long long np) {
+    using S = ModelShape<ModelT>;
+    constexpr int NF = S::NFWD, NC = NF + S::NSYM;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    int c[3];
+    tiled_coords(g, (int)(p >> 6), (int)(p & 63), c);
+    if (!in_grid(g, c)) return;
+    LaneStencil<ModelT, double> st;
+    st.load(g, sd, p, c);
+    for (int k = 0; k < NC; ++k) {
+        if (st.w[k] == 0.) continue;
+        int o0, o1, o2;
+        unpack_offset(st.off[k], o0, o1, o2);
+        for (int s = 0; s < (k < NF ? 1 : 2); ++s) {
+            const int sg = s == 0 ? 1 : -1;
+            const long long y = shifted<ModelT>(g, c, o0, o1, o2, sg);
+            if (y < 0) continue;
+            if (pass == 0) {
+                atomicAdd(&cnt[y], 1);
+            } else {
+                const int pos = atomicAdd(&fill[y], 1);
+                roffs[start[y] + pos] = pack_offset(sg * o0, sg * o1, sg * o2);
+            }
+        }
+    }
+}
+
+__device__ inline bool offset_less(uint32_t a, uint32_t b) {
+    int a0, a1, a2, b0, b1, b2;
+    unpack_offset(a, a0, a1, a2);
+    unpack_offset(b, b0, b1, b2);
+    if (a0 != b0) return a0 < b0;
+    if (a1 != b1) return a1 < b1;
+    return a2 < b2;
+}
+
+__global__ __launch_bounds__(256) void k_rev_sort(const int *__restrict__ cnt, const int *__restrict__ start,
+                                                 uint32_t *__restrict__ roffs, int *__restrict__ rcount, long long np) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int n = cnt[p];
+    uint32_t *l = roffs + start[p];
+    for (int i = 1; i < n; ++i) {  // insertion sort of a few dozen offsets at most
+        const uint32_t v = l[i];
+        int j = i - 1;
+        while (j >= 0 && offset_less(v, l[j])) {
+            l[j + 1] = l[j];
+            --j;
+        }
+        l[j + 1] = v;
+    }
+    int m = 0;
+    for (int i = 0; i < n; ++i)
+        if (m == 0 || l[i] != l[m - 1]) l[m++] = l[i];
+    rcount[p] = m;
+}
+
+// Exports of an exact solve (HFMInterface.hxx:526-577): activeNeighs are the flags the loop set;
+// geodesicFlow is -ReDim(sum_i w_i max(0, r - v_i) m_i e_i) from the recompute with the final
+// accepted set (m = 1, 3/2, 11/6 by order, EulerianStencil.hxx:218-231); activeOffsets are the
+// recompute's offsets packed as by the direct path.
+template <int ModelT>
+__global__ __launch_bounds__(256) void k_flow_exact(Geom g, StencilData<double> sd, const double *__restrict__ u,
+                                                   const unsigned char *__restrict__ acc,
+                                                   const unsigned long long *__restrict__ active, int order,
+                                                   FlowScale fs, double *__restrict__ flow,
+                                                   double *__restrict__ active_out, double *__restrict__ aoffs_out,
+                                                   long long np) {
+    using S = ModelShape<ModelT>;
+    constexpr int NC = S::NFWD + S::NSYM;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    int c[3];
+    tiled_coords(g, (int)(p >> 6), (int)(p & 63), c);
+    if (!in_grid(g, c)) return;
+    const long long lin = ((long long)c[0] * g.n[1] + c[1]) * g.n[2] + c[2];
+    const unsigned long long act = active[p];
+    RecomputeEntry re[2 * NC];
+    int n = 0;
+    double value = 0.;
+    if (act) n = recompute<ModelT>(g, sd, u, acc, p, c, act, order, re, &value);
+    constexpr double mult[4] = {0., 1., 3. / 2., 11. / 6.};
+    double f[3] = {0., 0., 0.};
+    for (int i = 0; i < n; ++i) {
+        const double d = value - re[i].value;
+        const double wpd = (re[i].w * (d > 0. ? d : 0.)) * mult[re[i].ord];
+        for (int q = 0; q < 3; ++q) f[q] += wpd * (double)re[i].o[q];
+    }
+    if (flow)
+        for (int q = 0; q < g.dim; ++q) flow[lin * g.dim + q] = -(f[q] * fs.h[q]);
+    if (active_out) active_out[lin] = (double)act;
+    if (aoffs_out) {
+        int j = 0;
+        for (; j < n; ++j) {
+            unsigned long long packed = 0;
+            for (int q = 0; q < S::DIM; ++q)
+                packed |= (unsigned long long)(uint8_t)(int8_t)re[j].o[q] << (8 * (S::DIM - 1 - q));
+            aoffs_out[lin * NC + j] = (double)packed;
+        }
+        for (; j < NC; ++j) aoffs_out[lin * NC + j] = 0.;
+    }
+}
+
+// Solver state before the loop: values +inf, nothing accepted or active, every point's quadratic
+// empty (minVal = -inf until the first update, a = b = 0, c = -1).
+__global__ __launch_bounds__(256) void k_exact_init(double *__restrict__ u, unsigned char *__restrict__ acc,
+                                                   unsigned long long *__restrict__ active, double *__restrict__ quad,
+                                                   long long np) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    u[p] = __builtin_huge_val();
+    acc[p] = 0;
+    active[p] = 0ull;
+    quad[4 * p] = -__builtin_huge_val();
+    quad[4 * p + 1] = 0.;
+    quad[4 * p + 2] = 0.;
+    quad[4 * p + 3] = -1.;
+}
+
+}  // namespace
+
+static inline unsigned exact_blocks(long long n) { return (unsigned)((n + 255) / 256); }
+
+hipError_t launch_rev_build(int model, const Geom &g, const StencilData<double> &sd, int pass, int *cnt,
+                            const int *start, int *fill, uint32_t *roffs, long long np, hipStream_t s) {
+    switch (model) {
+        case MODEL_RIEMANN2:
+            hipLaunchKernelGGL((k_rev_build<MODEL_RIEMANN2>), dim3(exact_blocks(np)), dim3(256), 0, s, g, sd, pass, cnt,
+                               start, fill, roffs, np);
+            break;
+        case MODEL_RIEMANN3:
+            hipLaunchKernelGGL((k_rev_build<MODEL_RIEMANN3>), dim3(exact_blocks(np)), dim3(256), 0, s, g, sd, pass, cnt,
+                               start, fill, roffs, np);
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_exact_init(double *u, unsigned char *acc, unsigned long long *active, double *quad, long long np,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k_exact_init, dim3(exact_blocks(np)), dim3(256), 0, s, u, acc, active, quad, np);
+    return hipGetLastError();
+}
+
+hipError_t launch_rev_sort(const int *cnt, const int *start, uint32_t *roffs, int *rcount, long long np, hipStream_t s) {
+    hipLaunchKernelGGL(k_rev_sort, dim3(exact_blocks(np)), dim3(256), 0, s, cnt, start, roffs, rcount, np);
+    return hipGetLastError();
+}
+
+hipError_t launch_fm_exact(int model, const ExactArgs &a, hipStream_t s) {
+    switch (model) {
+        case MODEL_RIEMANN2: hipLaunchKernelGGL((k_fm_exact<MODEL_RIEMANN2>), dim3(1), dim3(64), 0, s, a); break;
+        case MODEL_RIEMANN3: hipLaunchKernelGGL((k_fm_exact<MODEL_RIEMANN3>), dim3(1), dim3(64), 0, s, a); break;
+        case MODEL_RSF2: hipLaunchKernelGGL((k_fm_exact<MODEL_RSF2>), dim3(1), dim3(64), 0, s, a); break;
+        case MODEL_ELASTICA2: hipLaunchKernelGGL((k_fm_exact<MODEL_ELASTICA2>), dim3(1), dim3(64), 0, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_flow_exact(int model, const Geom &g, const StencilData<double> &sd, const double *u,
+                             const unsigned char *acc, const unsigned long long *active, int order, const FlowScale &fs,
+                             double *flow, double *act_out, double *aoffs, long long np, hipStream_t s) {
+    const dim3 gr(exact_blocks(np)), bl(256);
+    switch (model) {
+        case MODEL_RIEMANN2:
+            hipLaunchKernelGGL((k_flow_exact<MODEL_RIEMANN2>), gr, bl, 0, s, g, sd, u, acc, active, order, fs, flow,
+                               act_out, aoffs, np);
+            break;
+        case MODEL_RIEMANN3:
+            hipLaunchKernelGGL((k_flow_exact<MODEL_RIEMANN3>), gr, bl, 0, s, g, sd, u, acc, active, order, fs, flow,
+                               act_out, aoffs, np);
+            break;
+        case MODEL_RSF2:
+            hipLaunchKernelGGL((k_flow_exact<MODEL_RSF2>), gr, bl, 0, s, g, sd, u, acc, active, order, fs, flow,
+                               act_out, aoffs, np);
+            break;
+        case MODEL_ELASTICA2:
+            hipLaunchKernelGGL((k_flow_exact<MODEL_ELASTICA2>), gr, bl, 0, s, g, sd, u, acc, active, order, fs, flow,
+                               act_out, aoffs, np);
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace hfm

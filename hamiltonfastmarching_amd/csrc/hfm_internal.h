@@ -144,6 +144,22 @@ struct hfm_plan {
     // graph is captured on graph_stream and re-captured when its launch arguments change.
     bool use_graph = true;
     int fused_schedule = -1;       // rounds list their own tiles (k_fim_sched) instead of k_schedule + k_fim: 1 on, 0 off, -1 on for 2-D grids
+    // Serial acceptance loop (hfm_exact.hip), run instead of the block fast-iterative method when
+    // order > 1: reversed offsets per key (tiled point, or theta index for shared stencils), the
+    // accepted flags, activeNeighs bits and incremental quadratics of the last such solve.
+    int order = 1;
+    bool exact_solved = false;
+    bool exact_rev_dirty = true;
+    int *ex_cnt = nullptr, *ex_start = nullptr, *ex_fill = nullptr, *ex_rcount = nullptr, *ex_scan_work = nullptr;
+    uint32_t *ex_roffs = nullptr;
+    long long ex_roffs_cap = 0;
+    long long ex_pushes = 0;  // reversed offsets summed over every point: the heap's capacity less the seeds
+    unsigned char *ex_acc = nullptr;
+    unsigned long long *ex_active = nullptr;
+    double *ex_quad = nullptr;
+    void *ex_heap = nullptr;
+    long long ex_heap_cap = 0;
+    long long *ex_out = nullptr;
     hipStream_t graph_stream = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     std::vector<char> graph_key;

@@ -788,9 +788,18 @@ void run_impl(hfm_io *io) {
         }
         io->set_computed("pointFromIndex", std::move(f));
     }
-    if (io->has("order") && io->scalar("order") != 1.)
-        throw IOError{"Error : order " + std::to_string(io->scalar("order")) +
-                      " is not supported by this library (order 1 only)"};
+    // order (HFMInterface.hxx:358, read as an int): 2 / 3 recompute every point at its acceptance
+    // with second / third order differences, solved by the reference's serial acceptance loop
+    // on the device (hfm_plan_set_order): FP64 plans without walls.
+    const int order = (int)io->scalar("order", 1.);
+    if (order > 1) {
+        if (precision != 64)
+            throw IOError{"Error : order " + std::to_string(order) + " needs precision 64 (the serial acceptance "
+                          "loop reproduces the reference's FP64 values)"};
+        if (!walls.empty())
+            throw IOError{"Error : order " + std::to_string(order) + " with walls is not supported by this library"};
+        check(hfm_plan_set_order(plan, order));
+    }
     // Static factoring and the early-stopping criteria change the values through the serial
     // heap's acceptance order (StaticFactoring.hxx, CommonStoppingCriteria.h); refused rather
     // than ignored.

@@ -67,6 +67,37 @@ template <typename T> struct FimArgs {
     SchedRound sr;
 };
 
+// Serial acceptance loop of the options fixed by the heap's acceptance order (hfm_exact.hip):
+// order 2 / 3 recompute.  Arrays are tiled; the reversed offsets are listed per key (the tiled
+// point, or the theta index for shared stencils): rstart / rcount / roffs (packed int8 x 3).
+struct ExactArgs {
+    Geom g;
+    StencilData<double> sd;
+    double *u;
+    unsigned char *accepted;
+    unsigned long long *active;  // activeNeighs bits (side_bit layout)
+    double *quad;                // per point: minVal, a, b, c of the incremental quadratic
+    const int *rstart;
+    const int *rcount;
+    const uint32_t *roffs;
+    void *heap;                  // heap_cap entries of 16 bytes
+    long long heap_cap;
+    const long long *seed_p;     // tiled seed points in row-major index order
+    const double *seed_v;
+    int nseeds;
+    int order;
+    long long *out;              // accepted count, largest heap size, status (0 ok, 1 heap full)
+};
+hipError_t launch_exact_init(double *u, unsigned char *acc, unsigned long long *active, double *quad, long long np,
+                             hipStream_t s);
+hipError_t launch_rev_build(int model, const Geom &g, const StencilData<double> &sd, int pass, int *cnt,
+                            const int *start, int *fill, uint32_t *roffs, long long np, hipStream_t s);
+hipError_t launch_rev_sort(const int *cnt, const int *start, uint32_t *roffs, int *rcount, long long np, hipStream_t s);
+hipError_t launch_fm_exact(int model, const ExactArgs &a, hipStream_t s);
+hipError_t launch_flow_exact(int model, const Geom &g, const StencilData<double> &sd, const double *u,
+                             const unsigned char *acc, const unsigned long long *active, int order, const FlowScale &fs,
+                             double *flow, double *act_out, double *aoffs, long long np, hipStream_t s);
+
 // Geodesic backtracking (hfm_geodesic.hip).  Tips are ADim'd (grid-index units); points are
 // written ReDim'd, max_points per tip, tip t at points[t * max_points * dim ...].
 struct GeoParamsHost {

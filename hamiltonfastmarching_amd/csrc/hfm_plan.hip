@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -92,6 +93,17 @@ void plan_free(hfm_plan *p) {
 
     dfree(p->counts);
     dfree(p->dep_src);
+    dfree(p->ex_cnt);
+    dfree(p->ex_start);
+    dfree(p->ex_fill);
+    dfree(p->ex_rcount);
+    dfree(p->ex_scan_work);
+    dfree(p->ex_roffs);
+    dfree(p->ex_acc);
+    dfree(p->ex_active);
+    dfree(p->ex_quad);
+    dfree(p->ex_heap);
+    dfree(p->ex_out);
     dfree(p->dep_nsrc);
     dfree(p->scan_work);
     dfree(p->keys);
@@ -146,6 +158,7 @@ void plan_free(hfm_plan *p) {
 // Build the tile dependency graph from the stencils now on the device.
 template <typename T> void build_deps(hfm_plan *p, hipStream_t s) {
     const int nt = p->g.ntiles;
+    p->exact_rev_dirty = true;
     HIPCHECK(hipMemsetAsync(p->dep_cnt, 0, sizeof(int) * nt, s));
     HIPCHECK(launch_tile_deps<T>(p->model, p->g, p->sd<T>(), 0, p->dep_cnt, nullptr, nullptr, nullptr, p->dep_src,
                                  p->dep_nsrc, s));
@@ -313,6 +326,161 @@ void build_visibility(hfm_plan *p, hipStream_t s) {
         HIPCHECK(launch_wall_visibility<double>(p->model, p->g, p->sd<double>(), p->walls, maxDist, p->walls_refine,
                                                 p->wdist[0], p->wdist[1], p->vis, p->np, s));
     p->vis_dirty = false;
+}
+
+// Reversed offsets of the shared (per-theta) stencils, on the host like the stencils themselves
+// (HFM_StencilShare.hxx:132-181): for each theta the stencil of the mid-domain point x at that
+// theta is walked, and every nonzero-weight side x + s e lists s e under the theta of x + s e;
+// each list is sorted (signed lexicographic order) without repetitions.
+void shared_reversed_offsets(hfm_plan *p, std::vector<int> &start, std::vector<int> &count,
+                             std::vector<uint32_t> &offs) {
+    std::vector<uint32_t> toff;
+    std::vector<double> tw;
+    int maxw = 0;
+    curvature_table(p, toff, tw, maxw);
+    const int nth = p->g.n[2], nd = p->ndiff, nf = p->model == MODEL_RSF2 ? 3 : nd;
+    std::vector<std::vector<std::array<int, 3>>> lists(nth);
+    for (int t = 0; t < nth; ++t) {
+        const int c[3] = {p->g.n[0] / 2, p->g.n[1] / 2, t};
+        for (int k = 0; k < nd; ++k) {
+            if (tw[(size_t)t * nd + k] == 0.) continue;
+            int o[3];
+            unpack_offset(toff[(size_t)t * nd + k], o[0], o[1], o[2]);
+            for (int sg = 1; sg >= (k < nf ? 1 : -1); sg -= 2) {
+                int r[3];
+                if (!shift_coords(p->g, c, sg * o[0], sg * o[1], sg * o[2], r)) continue;
+                lists[r[2]].push_back({sg * o[0], sg * o[1], sg * o[2]});
+            }
+        }
+    }
+    start.assign(nth + 1, 0);
+    count.assign(nth, 0);
+    offs.clear();
+    for (int t = 0; t < nth; ++t) {
+        auto &l = lists[t];
+        std::sort(l.begin(), l.end());
+        l.erase(std::unique(l.begin(), l.end()), l.end());
+        start[t] = (int)offs.size();
+        count[t] = (int)l.size();
+        for (const auto &e : l) offs.push_back(pack_offset(e[0], e[1], e[2]));
+    }
+    start[nth] = (int)offs.size();
+}
+
+// The serial acceptance loop (hfm_exact.hip) for order 2 / 3: reversed offsets (rebuilt after a
+// change of stencils), the solver state, then one launch of k_fm_exact.
+void run_exact(hfm_plan *p, hipStream_t s) {
+    if (p->precision != 64)
+        throw HipFail{"order " + std::to_string(p->order) + " needs an FP64 plan (precision 64): the serial "
+                      "acceptance loop reproduces the reference's FP64 values"};
+    if (p->walls) throw HipFail{"order " + std::to_string(p->order) + " with walls is not supported by this library"};
+    if (p->workspace) throw HipFail{"order > 1 is not available in the allocation-free workspace"};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    HIPCHECK(hipEventRecord(e0, s));
+    const long long np = p->np;
+    const long long nkeys = p->shared ? p->g.n[2] : np;
+    if (!p->ex_acc) {
+        p->ex_acc = dalloc<unsigned char>(np);
+        p->ex_active = dalloc<unsigned long long>(np);
+        p->ex_quad = dalloc<double>(4 * np);
+        p->ex_out = dalloc<long long>(4);
+        p->ex_rcount = dalloc<int>(nkeys);
+        p->ex_start = dalloc<int>(nkeys + 1);
+        if (!p->shared) {
+            p->ex_cnt = dalloc<int>(np);
+            p->ex_fill = dalloc<int>(np);
+            p->ex_scan_work = dalloc<int>(scan_work_ints(np));
+        }
+    }
+    if (p->exact_rev_dirty) {
+        long long total = 0;
+        if (!p->shared) {
+            if (np > (1LL << 30)) throw HipFail{"grid too large for the serial acceptance loop"};
+            HIPCHECK(hipMemsetAsync(p->ex_cnt, 0, np * sizeof(int), s));
+            HIPCHECK(launch_rev_build(p->model, p->g, p->sd<double>(), 0, p->ex_cnt, nullptr, nullptr, nullptr, np, s));
+            HIPCHECK(launch_exclusive_scan(p->ex_cnt, p->ex_start, (int)np, s, p->ex_scan_work));
+            HIPCHECK(hipMemcpyAsync(p->h_count, p->ex_start + np, sizeof(int), hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            total = p->h_count[0];
+            if (total > p->ex_roffs_cap) {
+                dfree(p->ex_roffs);
+                p->ex_roffs = nullptr;
+                p->ex_roffs_cap = std::max<long long>(total, 1);
+                p->ex_roffs = dalloc<uint32_t>(p->ex_roffs_cap);
+            }
+            HIPCHECK(hipMemsetAsync(p->ex_fill, 0, np * sizeof(int), s));
+            HIPCHECK(launch_rev_build(p->model, p->g, p->sd<double>(), 1, p->ex_cnt, p->ex_start, p->ex_fill,
+                                      p->ex_roffs, np, s));
+            HIPCHECK(launch_rev_sort(p->ex_cnt, p->ex_start, p->ex_roffs, p->ex_rcount, np, s));
+            p->ex_pushes = total;
+        } else {
+            std::vector<int> st, cnt;
+            std::vector<uint32_t> offs;
+            shared_reversed_offsets(p, st, cnt, offs);
+            total = (long long)offs.size();
+            if (total > p->ex_roffs_cap) {
+                dfree(p->ex_roffs);
+                p->ex_roffs = nullptr;
+                p->ex_roffs_cap = std::max<long long>(total, 1);
+                p->ex_roffs = dalloc<uint32_t>(p->ex_roffs_cap);
+            }
+            if (total > 0)
+                HIPCHECK(hipMemcpyAsync(p->ex_roffs, offs.data(), total * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+            HIPCHECK(hipMemcpyAsync(p->ex_start, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice, s));
+            HIPCHECK(hipMemcpyAsync(p->ex_rcount, cnt.data(), cnt.size() * sizeof(int), hipMemcpyHostToDevice, s));
+            HIPCHECK(hipStreamSynchronize(s));  // the host vectors go out of scope
+            long long per_layer = 0;
+            for (int c : cnt) per_layer += c;
+            p->ex_pushes = per_layer * p->g.n[0] * p->g.n[1];
+        }
+        p->exact_rev_dirty = false;
+    }
+    // every update pushes once at most: the seeds plus every point's reversed offsets bound the heap
+    const int nseeds = (int)p->last_seeds.size();
+    const long long cap = p->ex_pushes + nseeds + 1;
+    if (cap > p->ex_heap_cap) {
+        dfree(p->ex_heap);
+        p->ex_heap = nullptr;
+        p->ex_heap_cap = cap;
+        p->ex_heap = dalloc<char>((size_t)cap * 16);
+    }
+    double *u = static_cast<double *>(p->u);
+    HIPCHECK(launch_exact_init(u, p->ex_acc, p->ex_active, p->ex_quad, np, s));
+    ExactArgs a;
+    a.g = p->g;
+    a.sd = p->sd<double>();
+    a.u = u;
+    a.accepted = p->ex_acc;
+    a.active = p->ex_active;
+    a.quad = p->ex_quad;
+    a.rstart = p->ex_start;
+    a.rcount = p->ex_rcount;
+    a.roffs = p->ex_roffs;
+    a.heap = p->ex_heap;
+    a.heap_cap = p->ex_heap_cap;
+    a.seed_p = p->seed_p;
+    a.seed_v = p->seed_v;
+    a.nseeds = nseeds;
+    a.order = p->order;
+    a.out = p->ex_out;
+    HIPCHECK(launch_fm_exact(p->model, a, s));
+    HIPCHECK(hipEventRecord(e1, s));
+    long long out[3] = {0, 0, 0};
+    HIPCHECK(hipMemcpyAsync(out, p->ex_out, sizeof(out), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (out[2] != 0) throw HipFail{"serial acceptance loop: heap capacity exceeded"};
+    p->st.solve_ms = ms;
+    p->st.iterations = out[0];  // points accepted
+    p->st.tile_visits = 0;
+    p->st.sweeps = 0;
+    p->st.fim_ms = 0.;
+    p->st.fim_launches = 0;
 }
 
 template <typename T> void run_fim(hfm_plan *p, hipStream_t s) {
@@ -808,8 +976,15 @@ int hfm_plan_solve(hfm_plan *p, int nseeds, const double *seeds, const double *s
         p->last_seeds = tp;
         p->seed_inputs = inputs;
         if (p->walls && p->vis_dirty) build_visibility(p, s);
-        if (p->precision == 32) run_fim<float>(p, s);
-        else run_fim<double>(p, s);
+        p->exact_solved = false;
+        if (p->order > 1) {
+            run_exact(p, s);
+            p->exact_solved = true;
+        } else if (p->precision == 32) {
+            run_fim<float>(p, s);
+        } else {
+            run_fim<double>(p, s);
+        }
         p->solved = true;
     } catch (const HipFail &e) {
         set_error(e.msg);
@@ -837,6 +1012,7 @@ int hfm_plan_load_values(hfm_plan *p, const double *d_values, const double *d_ac
             HIPCHECK(launch_load_state<double>(p->g, d_values, d_active, static_cast<double *>(p->u), p->fixedmask,
                                                p->np, s));
         HIPCHECK(hipStreamSynchronize(s));
+        p->exact_solved = false;
         p->last_seeds.clear();
         p->seed_inputs.clear();
         p->solved = true;
@@ -878,6 +1054,16 @@ static int plan_flow(const char *who, hfm_plan *p, double *d_flow, double *d_act
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
     FlowScale fs{{p->h[0], p->h[1], p->h[2]}};
+    if (p->exact_solved) {  // the flags of the serial loop and its order-aware recompute
+        const hipError_t e = launch_flow_exact(p->model, p->g, p->sd<double>(), static_cast<const double *>(p->u),
+                                               p->ex_acc, p->ex_active, p->order, fs, d_flow, d_active, d_aoffs, p->np,
+                                               s);
+        if (e != hipSuccess) {
+            set_error(std::string(who) + ": " + hipGetErrorString(e));
+            return 1;
+        }
+        return 0;
+    }
     hipError_t e = p->precision == 32
                        ? launch_flow<float>(p->model, p->g, p->sd<float>(), static_cast<const float *>(p->u),
                                             p->fixedmask, fs, d_flow, d_active, d_aoffs, p->np, s)
@@ -911,6 +1097,11 @@ int hfm_plan_residual(hfm_plan *p, double *max_rel, int64_t *n_mismatch, void *s
     }
     if (!p->solved) {
         set_error("hfm_plan_residual: no solve has run");
+        return 1;
+    }
+    if (p->exact_solved) {
+        set_error("hfm_plan_residual: the residual check is the first-order scheme's; an order " +
+                  std::to_string(p->order) + " solve does not satisfy it");
         return 1;
     }
     try {
@@ -977,6 +1168,11 @@ int hfm_plan_geodesics_impl(hfm_plan *p, int ntips, const double *tips, bool tip
     }
     if (!p->solved) {
         set_error("hfm_plan_geodesics: no solve has run");
+        return 1;
+    }
+    if (p->exact_solved) {
+        set_error("hfm_plan_geodesics: geodesics of an order " + std::to_string(p->order) +
+                  " solve are not supported by this library (order 1 only)");
         return 1;
     }
     *npoints = 0;
@@ -1207,6 +1403,15 @@ int hfm_plan_set_fused_schedule(hfm_plan *p, int enable) {
         return 1;
     }
     p->fused_schedule = enable < 0 ? -1 : enable != 0;
+    return 0;
+}
+
+int hfm_plan_set_order(hfm_plan *p, int order) {
+    if (!p) {
+        set_error("hfm_plan_set_order: null argument");
+        return 1;
+    }
+    p->order = order > 1 ? (order > 3 ? 3 : order) : 1;
     return 0;
 }
 

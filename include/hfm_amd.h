@@ -277,6 +277,16 @@ int hfm_plan_set_graph(hfm_plan *plan, int enable);
  * the reference. */
 int hfm_plan_set_fused_schedule(hfm_plan *plan, int enable);
 
+/* Order of the scheme: 1 (default) or 2 / 3, the recompute of every point at its acceptance with
+ * second / third order differences where the next points along an offset are accepted and no
+ * larger (replaces the `order` key, HFMInterface.hxx:358, HamiltonFastMarching.hxx:86-123,
+ * 187-272).  Values above 3 act as 3, below 2 as 1.  Order 2 / 3 solves run the reference's serial
+ * acceptance loop on the device (one thread popping a binary heap in device memory, so that
+ * equal keys leave in the reference's order): FP64 plans without walls; their exports are values,
+ * activeNeighs, geodesicFlow and activeOffsets (the order-aware recompute); geodesics,
+ * differentiation and the residual check refuse them. */
+int hfm_plan_set_order(hfm_plan *plan, int order);
+
 /* ----------------------------------------------------------------------------------------- */
 /* Continuous-batching solve server (no reference counterpart: the reference solves one      */
 /* problem per Run(IO&) call; this serves many Riemann2 / Riemann3 solves concurrently)      */

@@ -62,6 +62,9 @@ def lib():
                                                P(d), i32, P(d), P(d), P(d), P(d)]
         L.oracle_set_walls.argtypes = [ctypes.c_long, P(ctypes.c_ubyte), i32]
         L.oracle_set_walls.restype = None
+        L.oracle_set_order.argtypes = [i32]
+        L.oracle_set_order.restype = None
+        L.oracle_last_dips.restype = ctypes.c_long
         L.oracle_geodesics_given.argtypes = [i32, P(i32), P(d), P(d), P(d), i32, i32, d, d, P(d), i32, P(d),
                                              P(ctypes.c_uint64), i32, P(d), d, d, d, P(d), ctypes.c_long, P(i64),
                                              P(ctypes.c_long)]
@@ -152,7 +155,13 @@ def _common_args(params):
     xi = float(params.get("xi", 1.0))
     eps = float(params.get("eps", 0.1))
     _set_walls(params, dims, model)
+    lib().oracle_set_order(int(float(params.get("order", 1))))
     return model, dims, dim, N, h, origin, metric, metric_is_field, dual, cost, cost_is_field, seeds, sv, xi, eps
+
+
+def last_dips() -> int:
+    """Pops of the last solve whose key was below the previous pop's key."""
+    return int(lib().oracle_last_dips())
 
 
 def grid_walls(params, dims, model):

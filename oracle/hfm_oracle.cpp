@@ -493,6 +493,7 @@ struct FMState {
     std::vector<double> values;
     std::vector<uint64_t> active;
     std::vector<char> accepted;
+    std::vector<int8_t> wdist;  // distance to the walls (empty without walls)
 };
 
 // Reversed offsets (HFM_StencilRecompute.hxx:81-124; HFM_StencilShare.hxx:132-181): for each
@@ -538,13 +539,88 @@ struct QueueElement {
     bool operator<(const QueueElement &o) const { return value > o.value; }
 };
 
+// High-order recompute settings of the problems solved next on this thread (order 1: none).
+int g_order = 1;
+// Pops whose key lies below the previous pop's key, counted by the last solve (diagnostics).
+long g_dips = 0;
+
+// VisibleOffset (HamiltonFastMarching.hxx:160-170): base + v periodized, -1 when it leaves a
+// closed axis or a wall hides it.
+long visible_shift(const Problem &P, const std::vector<int8_t> &wdist, long base, const int8_t *v) {
+    const long x = P.shift(base, v, 1);
+    if (x < 0 || wdist.empty()) return x;
+    int bi[3] = {0, 0, 0}, vi[3] = {v[0], v[1], v[2]};
+    P.unravel(base, bi);
+    return wall_visible(P, wdist, bi, vi, x) ? x : -1;
+}
+
+// Recompute at acceptance (HamiltonFastMarching.hxx:187-272 with EulerianStencil.hxx:155-233):
+// every active difference of x takes its accepted neighbour's value, or, at order >= 2, the
+// second (third) order combination when the next point(s) along the offset are accepted and no
+// larger (the causality tests; the small-correction tests apply to strictly causal schemes
+// only, which these are not, HamiltonFastMarching.h:46), with the weight scaled by 9/4
+// (121/36).  The values enter one quadratic relative to their minimum, in stencil order.
+struct RecEntry {
+    double value, w;
+    int ord;
+    int8_t off[3];  // signed offset
+};
+int recompute_entries(const Problem &P, const FMState &S, long x, int order, RecEntry *out, double *value) {
+    const Stencil &st = P.stencil(x);
+    const double mlt = P.mult(x);
+    const uint64_t act = S.active[x];
+    int n = 0;
+    auto push = [&](const int8_t *off, int s, double w) {
+        RecEntry &r = out[n++];
+        for (int c = 0; c < 3; ++c) r.off[c] = (int8_t)(s * off[c]);
+        const long a1 = P.shift(x, r.off, 1);  // active: accepted and in the domain
+        const double v1 = S.values[a1];
+        double val = v1 + 0.;
+        int used = 1;
+        if (std::min(order, 3) >= 2) {
+            const long a2 = visible_shift(P, S.wdist, a1, r.off);
+            if (a2 >= 0 && S.accepted[a2] && !(S.values[a2] > v1)) {
+                const double v2 = S.values[a2];
+                used = 2;
+                val = (2. / 3.) * ((2. * v1 - 0.5 * v2) + 0.);
+                if (order >= 3) {
+                    const long a3 = visible_shift(P, S.wdist, a2, r.off);
+                    if (a3 >= 0 && S.accepted[a3] && !(S.values[a3] > v2)) {
+                        const double v3 = S.values[a3];
+                        used = 3;
+                        val = (6. / 11.) * ((3. * v1 - 1.5 * v2 + (1. / 3.) * v3) + 0.);
+                    }
+                }
+            }
+        }
+        r.value = val;
+        r.w = w;
+        r.ord = used;
+    };
+    int iNeigh = 0;
+    for (int k = 0; k < st.nfwd; ++k, ++iNeigh)
+        if (act >> iNeigh & 1) push(st.d[k].off, 1, P.weight(st.d[k], mlt));
+    for (int k = st.nfwd; k < st.nfwd + st.nsym; ++k, iNeigh += 2) {
+        if (act >> iNeigh & 1) push(st.d[k].off, 1, P.weight(st.d[k], mlt));
+        if (act >> (iNeigh + 1) & 1) push(st.d[k].off, -1, P.weight(st.d[k], mlt));
+    }
+    static const double mult2[4] = {0., 1., 9. / 4., 121. / 36.};
+    Quad q;
+    q.minVal = kInf;
+    for (int i = 0; i < n; ++i) q.minVal = std::min(q.minVal, out[i].value);
+    for (int i = 0; i < n; ++i) q.add(out[i].value, mult2[out[i].ord] * out[i].w);
+    *value = q.solve();
+    return n;
+}
+
 void fast_marching(const Problem &P, const std::vector<std::pair<long, double>> &seeds, FMState &S,
                    double *tStencil, double *tFM) {
     const clock_t t0 = clock();
     std::vector<long> start;
     std::vector<Off3> offs;
     reversed_offsets(P, start, offs);
-    std::vector<int8_t> wdist;
+    std::vector<int8_t> &wdist = S.wdist;
+    wdist.clear();
     if (!P.walls.empty()) {
         int maxw = 0;  // MaxStencilWidth (HamiltonFastMarching.hxx:37-45)
         for (const Off3 &e : offs) maxw = std::max(maxw, std::abs(e.o[0]) + std::abs(e.o[1]) + std::abs(e.o[2]));
@@ -560,11 +636,22 @@ void fast_marching(const Problem &P, const std::vector<std::pair<long, double>> 
         S.values[sv.first] = sv.second;
         queue.push({sv.first, sv.second});
     }
+    g_dips = 0;
+    double lastKey = -kInf;
     while (!queue.empty()) {
-        const QueueElement top = queue.top();
+        QueueElement top = queue.top();
         queue.pop();
         if (S.accepted[top.linearIndex]) continue;
         const long y = top.linearIndex;
+        if (top.value < lastKey) ++g_dips;
+        lastKey = top.value;
+        // PostProcess / Recompute (HamiltonFastMarching.hxx:86-99,115-123): the accepted value
+        // is recomputed; a point without active differences (a seed) keeps its value.
+        if (g_order > 1 && S.active[y]) {
+            RecEntry re[2 * kMaxDiff];
+            (void)recompute_entries(P, S, y, g_order, re, &top.value);
+            S.values[y] = top.value;
+        }
         S.accepted[y] = 1;
         const long key = P.shared ? y % P.dims[2] : y;
         for (long r = start[key]; r < start[key + 1]; ++r) {
@@ -632,20 +719,15 @@ int active_diffs(const Problem &P, const FMState &S, long x, ActiveDiff *out) {
 // GeodesicFlow (HamiltonFastMarching.hxx:363-378) via Recompute / HopfLaxRecompute at order 1
 // (EulerianStencil.hxx:155-233), exported as -ReDim(flow) (HFMInterface.hxx:544-553).
 void geodesic_flow(const Problem &P, const FMState &S, double *flow) {
-    ActiveDiff ad[2 * kMaxDiff];
+    RecEntry re[2 * kMaxDiff];
+    static const double mult[4] = {0., 1., 3. / 2., 11. / 6.};  // EulerianStencil.hxx:226-228
     for (long x = 0; x < P.N; ++x) {
         double f[3] = {0, 0, 0};
-        const int n = S.active[x] ? active_diffs(P, S, x, ad) : 0;
-        if (n > 0) {
-            double vals[2 * kMaxDiff];
-            Quad q; q.minVal = kInf;
-            for (int i = 0; i < n; ++i) { vals[i] = S.values[ad[i].neigh]; q.minVal = std::min(q.minVal, vals[i]); }
-            for (int i = 0; i < n; ++i) q.add(vals[i], 1. * ad[i].w);
-            const double value = q.solve();
-            for (int i = 0; i < n; ++i) {
-                const double wpd = ad[i].w * std::max(0., value - vals[i]);
-                for (int c = 0; c < P.dim; ++c) f[c] += wpd * double(ad[i].s * ad[i].off[c]);
-            }
+        double value = 0.;
+        const int n = S.active[x] ? recompute_entries(P, S, x, g_order, re, &value) : 0;
+        for (int i = 0; i < n; ++i) {
+            const double wpd = (re[i].w * std::max(0., value - re[i].value)) * mult[re[i].ord];
+            for (int c = 0; c < P.dim; ++c) f[c] += wpd * double(re[i].off[c]);
         }
         for (int c = 0; c < P.dim; ++c) flow[x * P.dim + c] = -P.redim_vec(f[c], c);
     }
@@ -942,6 +1024,9 @@ std::vector<std::pair<long, double>> make_seeds(const Problem &P, int nseeds, co
 extern "C" {
 
 const char *oracle_last_error(void) { return g_err.c_str(); }
+
+void oracle_set_order(int order) { g_order = order < 1 ? 1 : (order > 3 ? 3 : order); }
+long oracle_last_dips(void) { return g_dips; }
 
 // Walls of the problems set up next on this thread: n bytes (nonzero = wall), n = 0 for none.
 void oracle_set_walls(long n, const unsigned char *walls, int refine) {

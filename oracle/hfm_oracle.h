@@ -37,6 +37,12 @@ enum { ORACLE_RIEMANN2 = 0, ORACLE_RIEMANN3 = 1, ORACLE_RSF2 = 2, ORACLE_ELASTIC
 /* Last error message (empty if none). */
 const char *oracle_last_error(void);
 
+/* Order (1, 2 or 3) of the acceptance-time recompute of the solves run next on this thread
+ * (the reference's `order` key, HFMInterface.hxx:358; HamiltonFastMarching.hxx:86-99,187-272). */
+void oracle_set_order(int order);
+/* Pops of the last solve whose key was below the previous pop's key (diagnostics). */
+long oracle_last_dips(void);
+
 /* Walls (ExtraAlgorithms/WallObstructionTest.h) of the problems set up next on this thread:
  * n bytes in row-major point order, nonzero = wall; n = 0 for none.  refine bit 0 =
  * refineStencilAtWallBoundary; bit 1 plants a mistake for the tests (the line walk of the

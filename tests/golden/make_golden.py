@@ -486,6 +486,43 @@ def big_case():
     print("riemann3_128 ok")
 
 
+def isotropic(dim, n):
+    """Identity metric with a seed on a grid point: equal keys everywhere, so the heap's order
+    among equal keys decides the result."""
+    h = 1.0 / n
+    return {"model": f"Riemann{dim}", "dims": np.array([n] * dim, float), "gridScale": h,
+            "origin": np.zeros(dim), "seeds": np.full((1, dim), (n // 2 + 0.5) * h), "exportValues": 1.0,
+            "metric": np.array([1.0, 0.0, 1.0] if dim == 2 else [1.0, 0.0, 1.0, 0.0, 0.0, 1.0]),
+            "arrayOrdering": "RowMajor"}
+
+
+HIGH_ORDER_CASES = {
+    "riemann2_iso": lambda: isotropic(2, 33),
+    "riemann3_iso": lambda: isotropic(3, 13),
+    "riemann2": lambda: syn.riemann2(40, seed=11),
+    "riemann3": lambda: syn.riemann3(14, seed=12),
+    "rsf2": lambda: syn.rsf2(20, 12, seed=13),
+    "elastica2": lambda: syn.elastica2(16, 12, seed=14),
+}
+
+
+def high_order_cases():
+    """values / activeNeighs / geodesicFlow / activeOffsets at order 2 and 3 (the recompute at
+    acceptance, HamiltonFastMarching.hxx:86-123,187-272), and nAccepted."""
+    out = {}
+    for name, make in HIGH_ORDER_CASES.items():
+        p = {k: v for k, v in make().items()
+             if k not in ("tips", "inspectSensitivity", "inspectSensitivityWeights", "inspectSensitivityLengths")}
+        for order in (2, 3):
+            q = dict(p, order=float(order), exportActiveNeighs=1.0, exportGeodesicFlow=1.0, exportActiveOffsets=1.0)
+            r = oracle.run_reference(q)
+            assert r["retcode"] == 0, r["log"]
+            for k in ("values", "activeNeighs", "geodesicFlow", "activeOffsets", "nAccepted"):
+                out[f"{name}_order{order}_{k}"] = np.asarray(r[k])
+    np.savez_compressed(os.path.join(HERE, "high_order.npz"), **out)
+    print("high_order ok")
+
+
 def main():
     for m in ("Riemann2", "Riemann3", "ReedsSheppForward2", "Elastica2"):
         if not oracle.ref_available(m):
@@ -494,7 +531,7 @@ def main():
     if only:  # regenerate the named fixture groups only
         groups = {"geodesics": geodesic_cases, "forward": forward_cases, "walls": wall_cases, "seeds": seed_cases,
                   "bypass": bypass_cases, "stencils": stencil_cases, "big": big_case,
-                  "orderings": ordering_cases}
+                  "orderings": ordering_cases, "high_order": high_order_cases}
         for a in only:
             groups[a]()
         return
@@ -510,6 +547,7 @@ def main():
     bypass_cases()
     stencil_cases()
     ordering_cases()
+    high_order_cases()
     if "--big" in sys.argv:
         big_case()
 

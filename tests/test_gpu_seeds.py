@@ -68,7 +68,7 @@ def test_seed_radius_on_curvature_model_is_the_reference_error():
         hfm.run(dict(p, seeds=np.array([[0.5, 0.5, 1.0]]), seedRadius=2.0))
 
 
-@pytest.mark.parametrize("key,value", [("order", 2.0), ("factoringRadius", 10.0), ("stopAtDistance", 0.5),
+@pytest.mark.parametrize("key,value", [("factoringRadius", 10.0), ("stopAtDistance", 0.5),
                                        ("nMaxAccepted", 100.0)])
 def test_unsupported_keys_are_refused(key, value):
     """Keys whose effect depends on the serial heap's acceptance order are refused, not ignored."""
