@@ -143,6 +143,10 @@ class HFMIO:
         if self._L.hfm_run(self._io):
             raise PyExp(self._err())
 
+    def messages(self) -> str:
+        """Warnings and notes of the last run (what the reference writes to its log stream)."""
+        return (self._L.hfm_io_messages(self._io) or b"").decode()
+
 
 def run(params: dict) -> dict:
     """Solve an HFM input dictionary on the GPU; returns the computed keys (RowMajor default)."""
@@ -401,6 +405,21 @@ class Plan:
         third order differences, solved by the reference's serial acceptance loop on the device
         (FP64 plans without walls)."""
         self._check(self._L.hfm_plan_set_order(self._h, int(order)))
+        return self
+
+    def set_stopping(self, max_accepted=-1, stop_distance=float("inf"), any_idx=(), all_idx=(),
+                     any_unoriented=(), all_unoriented=()):
+        """Stopping criteria (nMaxAccepted, stopAtDistance, stopWhenAnyAccepted / AllAccepted and
+        their _Unoriented forms), as row-major grid indices ((x, y) indices x * n1 + y for the
+        _unoriented lists of the curvature models).  No argument removes them.  A solve with a
+        criterion runs the serial acceptance loop; stats()["n_accepted"] is the reference's
+        nAccepted."""
+        lists = [np.ascontiguousarray(np.asarray(a, np.int64).ravel())
+                 for a in (any_idx, all_idx, any_unoriented, all_unoriented)]
+        args = []
+        for a in lists:
+            args += [len(a), a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))]
+        self._check(self._L.hfm_plan_set_stopping(self._h, int(max_accepted), float(stop_distance), *args))
         return self
 
 

@@ -22,7 +22,7 @@ class HfmPlanStats(ctypes.Structure):
                 ("n_deps", c_int64), ("stencil_ms", c_double), ("solve_ms", c_double),
                 ("fim_ms", c_double), ("fim_launches", c_int64), ("tiles_carried", c_int64),
                 ("delta", c_double), ("adjoint_rounds", c_int64),
-                ("forward_rounds", c_int64)]
+                ("forward_rounds", c_int64), ("n_accepted", c_int64)]
 
 
 class HfmRequest(ctypes.Structure):
@@ -88,6 +88,8 @@ SIGNATURES = [
     ("hfm_plan_set_graph", c_int, [c_void_p, c_int]),
     ("hfm_plan_set_fused_schedule", c_int, [c_void_p, c_int]),
     ("hfm_plan_set_order", c_int, [c_void_p, c_int]),
+    ("hfm_plan_set_stopping", c_int, [c_void_p, c_int64, c_double, c_int, P(c_int64), c_int, P(c_int64), c_int,
+                                      P(c_int64), c_int, P(c_int64)]),
     ("hfm_plan_set_delta", c_int, [c_void_p, c_double]),
     ("hfm_plan_set_round_target", c_int, [c_void_p, c_int]),
     ("hfm_plan_set_grid", c_int, [c_void_p, c_int]),

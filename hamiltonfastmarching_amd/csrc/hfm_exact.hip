@@ -190,6 +190,7 @@ __global__ __launch_bounds__(64) void k_fm_exact(ExactArgs a) {
         heap_push(heap, n, HeapEntry{a.seed_v[i], a.seed_p[i]});
     }
     RecomputeEntry re[2 * NC];
+    long long npost = 0, all_left = a.all_left, all_left_u = a.all_left_u;
     while (n > 0) {
         maxn = n > maxn ? n : maxn;
         HeapEntry top = heap_pop(heap, n);
@@ -197,13 +198,36 @@ __global__ __launch_bounds__(64) void k_fm_exact(ExactArgs a) {
         if (a.accepted[y]) continue;
         int cy[3];
         tiled_coords(g, (int)(y >> 6), (int)(y & 63), cy);
-        // PostProcess with recompute: a point without active differences keeps its value
+        // PostProcess (HamiltonFastMarching.hxx:86-99): the stopping criteria see the point with
+        // its heap value, before the recompute (ExtraAlgorithms/CommonStoppingCriteria.h:57-92)
+        ++npost;
+        bool stop = npost >= a.max_accepted;
+        if (a.stop_flags) {
+            const unsigned char f = a.stop_flags[y];
+            if (f & 1) stop = true;
+            if (f & 2) {
+                a.stop_flags[y] = f & (unsigned char)~2;
+                if (--all_left == 0) stop = true;
+            }
+        }
+        if (a.stop_flags_u) {
+            const long long xy = (long long)cy[0] * g.n[1] + cy[1];
+            const unsigned char f = a.stop_flags_u[xy];
+            if (f & 1) stop = true;
+            if (f & 2) {
+                a.stop_flags_u[xy] = f & (unsigned char)~2;
+                if (--all_left_u == 0) stop = true;
+            }
+        }
+        if (a.u[y] > a.stop_distance) stop = true;
+        // the recompute: a point without active differences keeps its value
         if (a.order > 1 && a.active[y]) {
             double v;
             (void)recompute<ModelT>(g, a.sd, a.u, a.accepted, y, cy, a.active[y], a.order, re, &v);
             top.value = v;
             a.u[y] = v;
         }
+        if (stop) break;  // the point that triggered the stop is not accepted
         a.accepted[y] = 1;
         ++accepted;
         const long long key = S::SHARED ? (long long)cy[2] : y;
@@ -256,6 +280,7 @@ __global__ __launch_bounds__(64) void k_fm_exact(ExactArgs a) {
             if (n >= a.heap_cap) {  // cannot happen: the capacity is the number of pushes possible
                 a.out[2] = 1;
                 a.out[0] = accepted;
+                a.out[3] = npost;
                 return;
             }
             heap_push(heap, n, HeapEntry{value, x});
@@ -264,6 +289,7 @@ __global__ __launch_bounds__(64) void k_fm_exact(ExactArgs a) {
     a.out[0] = accepted;
     a.out[1] = maxn;
     a.out[2] = 0;
+    a.out[3] = npost;
 }
 
 // Reversed offsets of the per-point stencils (HFM_StencilRecompute.hxx:81-124): every point y

@@ -160,6 +160,12 @@ struct hfm_plan {
     void *ex_heap = nullptr;
     long long ex_heap_cap = 0;
     long long *ex_out = nullptr;
+    // stopping criteria (hfm_plan_set_stopping), row-major grid indices; flags uploaded per solve
+    bool has_stopping = false;
+    long long stop_max = -1;
+    double stop_dist = 0.;
+    std::vector<long long> stop_any, stop_all, stop_any_u, stop_all_u;
+    unsigned char *ex_flags = nullptr, *ex_flags_u = nullptr;
     hipStream_t graph_stream = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     std::vector<char> graph_key;

@@ -801,13 +801,68 @@ void run_impl(hfm_io *io) {
         check(hfm_plan_set_order(plan, order));
     }
     // Static factoring and the early-stopping criteria change the values through the serial
-    // heap's acceptance order (StaticFactoring.hxx, CommonStoppingCriteria.h); refused rather
+    // heap's acceptance order (StaticFactoring.hxx, ExtraAlgorithms/CommonStoppingCriteria.h); refused rather
     // than ignored.
     if ((io->has("factoringRadius") && io->scalar("factoringRadius") != 0.) || io->has("factoringValues"))
         throw IOError{"Error : static factoring is not supported by this library"};
-    for (const char *k : {"stopWhenAnyAccepted", "stopWhenAllAccepted", "stopWhenAnyAccepted_Unoriented",
-                          "stopWhenAllAccepted_Unoriented", "stopAtDistance", "nMaxAccepted"})
-        if (io->has(k)) throw IOError{std::string("Error : stopping criterion ") + k + " is not supported by this library"};
+    // Stopping criteria (ExtraAlgorithms/CommonStoppingCriteria.h:115-165): the targets' indices are
+    // IndexFromPoint(ADim(p)), periodized, an out-of-range one dropped with a warning; an
+    // unoriented target is its (x, y) at any theta.  They run the serial acceptance loop.
+    bool stopping = false;
+    {
+        const bool hasAny = io->has("stopWhenAnyAccepted"), hasAll = io->has("stopWhenAllAccepted");
+        const bool hasAnyU = bundle && io->has("stopWhenAnyAccepted_Unoriented");
+        const bool hasAllU = bundle && io->has("stopWhenAllAccepted_Unoriented");
+        const bool hasMax = io->has("nMaxAccepted"), hasDist = io->has("stopAtDistance");
+        if (hasAny || hasAll || hasAnyU || hasAllU || hasMax || hasDist) {
+            if (precision != 64)
+                throw IOError{"Error : stopping criteria need precision 64 (the serial acceptance loop reproduces "
+                              "the reference's FP64 values)"};
+            if (!walls.empty()) throw IOError{"Error : stopping criteria with walls are not supported by this library"};
+            auto targets = [&](const char *key, bool unoriented) {
+                std::vector<int64_t> out;
+                const int pd = unoriented ? 2 : dim;
+                const std::vector<double> pts = io->vec(key);
+                if (pts.size() % pd) throw IOError{std::string("IO error : ") + key + " has the wrong size"};
+                for (size_t q = 0; q < pts.size(); q += pd) {
+                    int64_t idx[3] = {0, 0, 0};
+                    bool ok = true;
+                    for (int i = 0; i < pd; ++i) {
+                        const double f = std::floor((pts[q + i] - axis_origin(i)) / (double)axis_scale(i));
+                        if (!(f > -2e9 && f < 2e9)) { ok = false; break; }
+                        int64_t v = (int64_t)f;
+                        if (bundle && i == 2) v = ((v % dims[2]) + dims[2]) % dims[2];
+                        else if (v < 0 || v >= dims[i]) ok = false;
+                        idx[i] = v;
+                    }
+                    if (!ok) {
+                        // ExtraAlgorithms/CommonStoppingCriteria.h:110,130: the point as the reference prints it, no newline
+                        std::ostringstream os;
+                        os << (unoriented ? "Warning : unoriented target \"p\" -> {" : "Warning : target {");
+                        for (int i = 0; i < pd; ++i) os << (i ? "," : "") << pts[q + i];
+                        os << (unoriented ? "}, is out of range." : "} is out of range.");
+                        io->log += os.str();
+                        continue;
+                    }
+                    out.push_back(unoriented ? idx[0] * dims[1] + idx[1]
+                                             : (dim == 2 ? idx[0] * dims[1] + idx[1]
+                                                         : (idx[0] * dims[1] + idx[1]) * dims[2] + idx[2]));
+                }
+                return out;
+            };
+            const std::vector<int64_t> any = hasAny ? targets("stopWhenAnyAccepted", false) : std::vector<int64_t>();
+            const std::vector<int64_t> all = hasAll ? targets("stopWhenAllAccepted", false) : std::vector<int64_t>();
+            const std::vector<int64_t> anyU =
+                hasAnyU ? targets("stopWhenAnyAccepted_Unoriented", true) : std::vector<int64_t>();
+            const std::vector<int64_t> allU =
+                hasAllU ? targets("stopWhenAllAccepted_Unoriented", true) : std::vector<int64_t>();
+            const int64_t nmax = hasMax ? (int64_t)(size_t)io->scalar("nMaxAccepted") : -1;
+            const double dist = hasDist ? io->scalar("stopAtDistance") : std::numeric_limits<double>::infinity();
+            check(hfm_plan_set_stopping(plan, nmax, dist, (int)any.size(), any.data(), (int)all.size(), all.data(),
+                                        (int)anyU.size(), anyU.data(), (int)allU.size(), allU.data()));
+            stopping = true;
+        }
+    }
     // Seeds (StaticFactoring::SetSeeds, StaticFactoring.hxx:242-393): `seeds` / `seedValues`,
     // optionally spread over the grid points within `seedRadius` (Riemann models), then for the
     // curvature models `seeds_Unoriented` / `seedValues_Unoriented` at every theta.
@@ -894,9 +949,12 @@ void run_impl(hfm_io *io) {
             os << "Fast marching solver completed in " << (t3 - t2) << " s.\n";
             io->log += os.str();
         }
-        // nAccepted (CommonStoppingCriteria.h:162-165): without stopping criteria, every point the
-        // heap accepts is a reached one, i.e. a finite value
-        {
+        // nAccepted (ExtraAlgorithms/CommonStoppingCriteria.h:162-165): the serial loop's count when it ran (the
+        // pop that triggered a stop included); otherwise every point the heap accepts is a reached
+        // one, i.e. a finite value
+        if (order > 1 || stopping) {
+            io->set_scalar("nAccepted", (double)st.n_accepted);
+        } else {
             int64_t reached = 0;
             check(hfm::plan_count_reached(plan, &reached, nullptr));
             io->set_scalar("nAccepted", (double)reached);

@@ -86,7 +86,17 @@ struct ExactArgs {
     const double *seed_v;
     int nseeds;
     int order;
-    long long *out;              // accepted count, largest heap size, status (0 ok, 1 heap full)
+    long long *out;              // accepted, largest heap size, status (0 ok, 1 heap full), nAccepted
+    // Stopping criteria (ExtraAlgorithms/CommonStoppingCriteria.h:57-92), checked when a point leaves the heap:
+    // at most max_accepted pops; stop above stop_distance; stop_flags (per tiled point; null for
+    // none): bit 0 stopWhenAnyAccepted, bit 1 stopWhenAllAccepted (cleared as it is accepted,
+    // all_left of them to go); stop_flags_u (per (x, y) of the curvature models): the same for the
+    // _Unoriented sets.
+    long long max_accepted;
+    double stop_distance;
+    unsigned char *stop_flags;
+    unsigned char *stop_flags_u;
+    long long all_left, all_left_u;
 };
 hipError_t launch_exact_init(double *u, unsigned char *acc, unsigned long long *active, double *quad, long long np,
                              hipStream_t s);

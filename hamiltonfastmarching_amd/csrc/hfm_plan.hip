@@ -104,6 +104,8 @@ void plan_free(hfm_plan *p) {
     dfree(p->ex_quad);
     dfree(p->ex_heap);
     dfree(p->ex_out);
+    dfree(p->ex_flags);
+    dfree(p->ex_flags_u);
     dfree(p->dep_nsrc);
     dfree(p->scan_work);
     dfree(p->keys);
@@ -370,11 +372,12 @@ void shared_reversed_offsets(hfm_plan *p, std::vector<int> &start, std::vector<i
 // The serial acceptance loop (hfm_exact.hip) for order 2 / 3: reversed offsets (rebuilt after a
 // change of stencils), the solver state, then one launch of k_fm_exact.
 void run_exact(hfm_plan *p, hipStream_t s) {
+    const std::string what = p->order > 1 ? "order " + std::to_string(p->order) : std::string("a stopping criterion");
     if (p->precision != 64)
-        throw HipFail{"order " + std::to_string(p->order) + " needs an FP64 plan (precision 64): the serial "
-                      "acceptance loop reproduces the reference's FP64 values"};
-    if (p->walls) throw HipFail{"order " + std::to_string(p->order) + " with walls is not supported by this library"};
-    if (p->workspace) throw HipFail{"order > 1 is not available in the allocation-free workspace"};
+        throw HipFail{what + " needs an FP64 plan (precision 64): the serial acceptance loop reproduces the "
+                      "reference's FP64 values"};
+    if (p->walls) throw HipFail{what + " with walls is not supported by this library"};
+    if (p->workspace) throw HipFail{what + " is not available in the allocation-free workspace"};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIPCHECK(hipEventCreate(&e0));
     HIPCHECK(hipEventCreate(&e1));
@@ -465,9 +468,47 @@ void run_exact(hfm_plan *p, hipStream_t s) {
     a.nseeds = nseeds;
     a.order = p->order;
     a.out = p->ex_out;
+    // stopping criteria: target flags per tiled point / per (x, y), uploaded for this solve
+    a.max_accepted = p->stop_max >= 0 ? p->stop_max : (long long)(~0ull >> 1);
+    a.stop_distance = p->has_stopping ? p->stop_dist : __builtin_huge_val();
+    a.stop_flags = nullptr;
+    a.stop_flags_u = nullptr;
+    a.all_left = 0;
+    a.all_left_u = 0;
+    std::vector<unsigned char> fl, flu;
+    if (!p->stop_any.empty() || !p->stop_all.empty()) {
+        fl.assign(np, 0);
+        for (int pass = 0; pass < 2; ++pass)
+            for (long long lin : pass == 0 ? p->stop_any : p->stop_all) {
+                int c[3] = {0, 0, 0};
+                long long r = lin;
+                for (int i = p->g.dim - 1; i >= 0; --i) {
+                    c[i] = (int)(r % p->g.n[i]);
+                    r /= p->g.n[i];
+                }
+                unsigned char &f = fl[(long long)tile_of(p->g, c) * 64 + local_of(p->g, c)];
+                if (pass == 1 && !(f & 2)) ++a.all_left;  // a set: each index once
+                f |= pass == 0 ? 1 : 2;
+            }
+        if (!p->ex_flags) p->ex_flags = dalloc<unsigned char>(np);
+        HIPCHECK(hipMemcpyAsync(p->ex_flags, fl.data(), np, hipMemcpyHostToDevice, s));
+        a.stop_flags = p->ex_flags;
+    }
+    if (!p->stop_any_u.empty() || !p->stop_all_u.empty()) {
+        const long long nxy = (long long)p->g.n[0] * p->g.n[1];
+        flu.assign(nxy, 0);
+        for (long long xy : p->stop_any_u) flu[xy] |= 1;
+        for (long long xy : p->stop_all_u) {
+            if (!(flu[xy] & 2)) ++a.all_left_u;
+            flu[xy] |= 2;
+        }
+        if (!p->ex_flags_u) p->ex_flags_u = dalloc<unsigned char>(nxy);
+        HIPCHECK(hipMemcpyAsync(p->ex_flags_u, flu.data(), nxy, hipMemcpyHostToDevice, s));
+        a.stop_flags_u = p->ex_flags_u;
+    }
     HIPCHECK(launch_fm_exact(p->model, a, s));
     HIPCHECK(hipEventRecord(e1, s));
-    long long out[3] = {0, 0, 0};
+    long long out[4] = {0, 0, 0, 0};
     HIPCHECK(hipMemcpyAsync(out, p->ex_out, sizeof(out), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
     float ms = 0;
@@ -477,6 +518,7 @@ void run_exact(hfm_plan *p, hipStream_t s) {
     if (out[2] != 0) throw HipFail{"serial acceptance loop: heap capacity exceeded"};
     p->st.solve_ms = ms;
     p->st.iterations = out[0];  // points accepted
+    p->st.n_accepted = out[3];  // pops seen by the stopping criteria (the reference's nAccepted)
     p->st.tile_visits = 0;
     p->st.sweeps = 0;
     p->st.fim_ms = 0.;
@@ -977,7 +1019,8 @@ int hfm_plan_solve(hfm_plan *p, int nseeds, const double *seeds, const double *s
         p->seed_inputs = inputs;
         if (p->walls && p->vis_dirty) build_visibility(p, s);
         p->exact_solved = false;
-        if (p->order > 1) {
+        p->st.n_accepted = -1;
+        if (p->order > 1 || p->has_stopping) {
             run_exact(p, s);
             p->exact_solved = true;
         } else if (p->precision == 32) {
@@ -1412,6 +1455,39 @@ int hfm_plan_set_order(hfm_plan *p, int order) {
         return 1;
     }
     p->order = order > 1 ? (order > 3 ? 3 : order) : 1;
+    return 0;
+}
+
+int hfm_plan_set_stopping(hfm_plan *p, int64_t max_accepted, double stop_distance, int n_any, const int64_t *any_idx,
+                          int n_all, const int64_t *all_idx, int n_any_u, const int64_t *any_u_idx, int n_all_u,
+                          const int64_t *all_u_idx) {
+    if (!p || (n_any > 0 && !any_idx) || (n_all > 0 && !all_idx) || (n_any_u > 0 && !any_u_idx) ||
+        (n_all_u > 0 && !all_u_idx) || n_any < 0 || n_all < 0 || n_any_u < 0 || n_all_u < 0) {
+        set_error("hfm_plan_set_stopping: null argument");
+        return 1;
+    }
+    if ((n_any_u > 0 || n_all_u > 0) && !p->shared) {
+        set_error("hfm_plan_set_stopping: unoriented targets apply to the curvature models only");
+        return 1;
+    }
+    const long long npts = p->N, nxy = (long long)p->g.n[0] * p->g.n[1];
+    auto take = [&](int n, const int64_t *idx, long long lim, std::vector<long long> &dst) {
+        dst.clear();
+        for (int i = 0; i < n; ++i) {
+            if (idx[i] < 0 || idx[i] >= lim) return false;
+            dst.push_back(idx[i]);
+        }
+        return true;
+    };
+    if (!take(n_any, any_idx, npts, p->stop_any) || !take(n_all, all_idx, npts, p->stop_all) ||
+        !take(n_any_u, any_u_idx, nxy, p->stop_any_u) || !take(n_all_u, all_u_idx, nxy, p->stop_all_u)) {
+        set_error("hfm_plan_set_stopping: target index out of the grid");
+        return 1;
+    }
+    p->stop_max = max_accepted;
+    p->stop_dist = stop_distance;
+    p->has_stopping = max_accepted >= 0 || stop_distance < __builtin_huge_val() || n_any > 0 || n_all > 0 ||
+                      n_any_u > 0 || n_all_u > 0;
     return 0;
 }
 

@@ -109,6 +109,9 @@ typedef struct hfm_plan_stats {
     double delta;           /* band width used by the last solve (0: unordered) */
     int64_t adjoint_rounds; /* rounds of the last backward sweep */
     int64_t forward_rounds; /* rounds of the last forward-variation sweep */
+    int64_t n_accepted;     /* serial acceptance loop (order 2 / 3, stopping criteria): nAccepted as
+                               the reference counts it (the pop that triggered a stop included);
+                               -1 after a block fast-iterative solve (every reached point) */
 } hfm_plan_stats;
 
 /* model: "Riemann2" | "Riemann3" | "ReedsSheppForward2" | "Elastica2".
@@ -286,6 +289,19 @@ int hfm_plan_set_fused_schedule(hfm_plan *plan, int enable);
  * activeNeighs, geodesicFlow and activeOffsets (the order-aware recompute); geodesics,
  * differentiation and the residual check refuse them. */
 int hfm_plan_set_order(hfm_plan *plan, int order);
+
+/* Stopping criteria (replace nMaxAccepted, stopAtDistance, stopWhenAnyAccepted,
+ * stopWhenAllAccepted and their _Unoriented forms, ExtraAlgorithms/CommonStoppingCriteria.h:58-165), checked as
+ * each point leaves the heap, before its recompute: stop at the max_accepted-th pop (< 0: no
+ * limit), at a point whose value exceeds stop_distance (+inf: none), at any point of any_idx, or
+ * once every point of all_idx has left the heap; the _u lists hold (x, y) indices x * n1 + y of
+ * the curvature models, matched at any theta.  Indices are row-major grid indices.  A solve with
+ * any criterion runs the serial acceptance loop (see hfm_plan_set_order), whose partial result --
+ * accepted values, tentative values beyond them, their activeNeighs -- is the reference's; FP64
+ * plans without walls.  All lists empty, max_accepted < 0 and stop_distance = +inf remove them. */
+int hfm_plan_set_stopping(hfm_plan *plan, int64_t max_accepted, double stop_distance, int n_any,
+                          const int64_t *any_idx, int n_all, const int64_t *all_idx, int n_any_u,
+                          const int64_t *any_u_idx, int n_all_u, const int64_t *all_u_idx);
 
 /* ----------------------------------------------------------------------------------------- */
 /* Continuous-batching solve server (no reference counterpart: the reference solves one      */

@@ -65,6 +65,10 @@ def lib():
         L.oracle_set_order.argtypes = [i32]
         L.oracle_set_order.restype = None
         L.oracle_last_dips.restype = ctypes.c_long
+        L.oracle_set_stopping.argtypes = [ctypes.c_long, d, i32, P(ctypes.c_long), i32, P(ctypes.c_long), i32,
+                                          P(ctypes.c_long), i32, P(ctypes.c_long)]
+        L.oracle_set_stopping.restype = None
+        L.oracle_last_naccepted.restype = ctypes.c_long
         L.oracle_geodesics_given.argtypes = [i32, P(i32), P(d), P(d), P(d), i32, i32, d, d, P(d), i32, P(d),
                                              P(ctypes.c_uint64), i32, P(d), d, d, d, P(d), ctypes.c_long, P(i64),
                                              P(ctypes.c_long)]
@@ -156,7 +160,53 @@ def _common_args(params):
     eps = float(params.get("eps", 0.1))
     _set_walls(params, dims, model)
     lib().oracle_set_order(int(float(params.get("order", 1))))
+    _set_stopping(params, dims, model, h, origin)
     return model, dims, dim, N, h, origin, metric, metric_is_field, dual, cost, cost_is_field, seeds, sv, xi, eps
+
+
+def _stop_indices(params, key, dims, model, h, origin, unoriented):
+    """IndexFromPoint(ADim(p)) of the targets, periodized, out-of-range ones dropped
+    (ExtraAlgorithms/CommonStoppingCriteria.h:115-151); row-major indices, (x, y) ones when unoriented."""
+    if key not in params:
+        return np.zeros(0, np.int64)
+    pd = 2 if unoriented else len(dims)
+    pts = np.asarray(params[key], np.float64).reshape(-1, pd)
+    bundle = model in ("ReedsSheppForward2", "Elastica2")
+    out = []
+    for p in pts:
+        idx = []
+        ok = True
+        for i in range(pd):
+            if bundle and i == 2:
+                ds = 2 * np.pi / dims[2]
+                v = int(np.floor((p[i] + ds / 2) / ds)) % dims[2]
+            else:
+                v = int(np.floor((p[i] - (0. if origin is None else origin[i])) / h[i]))
+                ok = ok and 0 <= v < dims[i]
+            idx.append(v)
+        if ok:
+            out.append(np.ravel_multi_index(tuple(idx), tuple(dims[:pd])))
+    return np.asarray(out, np.int64)
+
+
+def _set_stopping(params, dims, model, h, origin):
+    bundle = model in ("ReedsSheppForward2", "Elastica2")
+    lists = [_stop_indices(params, k, dims, model, h, origin, False)
+             for k in ("stopWhenAnyAccepted", "stopWhenAllAccepted")]
+    lists += [_stop_indices(params, k, dims, model, h, origin, True) if bundle else np.zeros(0, np.int64)
+              for k in ("stopWhenAnyAccepted_Unoriented", "stopWhenAllAccepted_Unoriented")]
+    lists = [np.ascontiguousarray(a, dtype=np.int64) for a in lists]
+    nmax = int(float(params["nMaxAccepted"])) if "nMaxAccepted" in params else -1
+    dist = float(params["stopAtDistance"]) if "stopAtDistance" in params else float("inf")
+    args = []
+    for a in lists:
+        args += [len(a), a.ctypes.data_as(ctypes.POINTER(ctypes.c_long))]
+    lib().oracle_set_stopping(nmax, dist, *args)
+
+
+def last_naccepted() -> int:
+    """nAccepted of the last solve (ExtraAlgorithms/CommonStoppingCriteria.h:162-165)."""
+    return int(lib().oracle_last_naccepted())
 
 
 def last_dips() -> int:
@@ -213,7 +263,8 @@ def solve(params, flow=False, active=False):
                            _ptr(act, ctypes.c_uint64), _ptr(fl, ctypes.c_double), _ptr(tm, ctypes.c_double))
     if r:
         raise RuntimeError(_err())
-    out = {"values": values.reshape(dims), "StencilCPUTime": tm[0], "FMCPUTime": tm[1]}
+    out = {"values": values.reshape(dims), "StencilCPUTime": tm[0], "FMCPUTime": tm[1],
+           "nAccepted": last_naccepted()}
     if active:
         out["activeNeighs"] = act.reshape(dims)
     if flow:

@@ -16,6 +16,7 @@
 #include <map>
 #include <queue>
 #include <string>
+#include <set>
 #include <vector>
 
 namespace {
@@ -543,6 +544,12 @@ struct QueueElement {
 int g_order = 1;
 // Pops whose key lies below the previous pop's key, counted by the last solve (diagnostics).
 long g_dips = 0;
+// Stopping criteria of the solves run next (ExtraAlgorithms/CommonStoppingCriteria.h:57-92): row-major indices,
+// (x, y) indices for the unoriented sets; g_naccepted = nAccepted of the last solve.
+long g_stop_max = -1;
+double g_stop_dist = std::numeric_limits<double>::infinity();
+std::set<long> g_stop_any, g_stop_all, g_stop_any_u, g_stop_all_u;
+long g_naccepted = 0;
 
 // VisibleOffset (HamiltonFastMarching.hxx:160-170): base + v periodized, -1 when it leaves a
 // closed axis or a wall hides it.
@@ -637,7 +644,9 @@ void fast_marching(const Problem &P, const std::vector<std::pair<long, double>> 
         queue.push({sv.first, sv.second});
     }
     g_dips = 0;
+    g_naccepted = 0;
     double lastKey = -kInf;
+    std::set<long> all = g_stop_all, allU = g_stop_all_u;
     while (!queue.empty()) {
         QueueElement top = queue.top();
         queue.pop();
@@ -645,13 +654,28 @@ void fast_marching(const Problem &P, const std::vector<std::pair<long, double>> 
         const long y = top.linearIndex;
         if (top.value < lastKey) ++g_dips;
         lastKey = top.value;
-        // PostProcess / Recompute (HamiltonFastMarching.hxx:86-99,115-123): the accepted value
-        // is recomputed; a point without active differences (a seed) keeps its value.
+        // PostProcess (HamiltonFastMarching.hxx:86-99): the stopping criteria, then the recompute
+        // (a point without active differences -- a seed -- keeps its value), then the stop.
+        ++g_naccepted;
+        bool stop = g_stop_max >= 0 && g_naccepted >= g_stop_max;
+        if (!stop && g_stop_any.count(y)) stop = true;
+        if (!stop && !all.empty()) {
+            all.erase(y);
+            if (all.empty()) stop = true;
+        }
+        const long xy = P.shared ? y / P.dims[2] : y;
+        if (!stop && P.shared && g_stop_any_u.count(xy)) stop = true;
+        if (!stop && P.shared && !allU.empty()) {
+            allU.erase(xy);
+            if (allU.empty()) stop = true;
+        }
+        if (!stop && g_stop_dist < kInf && S.values[y] > g_stop_dist) stop = true;
         if (g_order > 1 && S.active[y]) {
             RecEntry re[2 * kMaxDiff];
             (void)recompute_entries(P, S, y, g_order, re, &top.value);
             S.values[y] = top.value;
         }
+        if (stop) break;
         S.accepted[y] = 1;
         const long key = P.shared ? y % P.dims[2] : y;
         for (long r = start[key]; r < start[key + 1]; ++r) {
@@ -1027,6 +1051,16 @@ const char *oracle_last_error(void) { return g_err.c_str(); }
 
 void oracle_set_order(int order) { g_order = order < 1 ? 1 : (order > 3 ? 3 : order); }
 long oracle_last_dips(void) { return g_dips; }
+void oracle_set_stopping(long max_accepted, double stop_distance, int nany, const long *any, int nall,
+                         const long *all, int nanyu, const long *anyu, int nallu, const long *allu) {
+    g_stop_max = max_accepted;
+    g_stop_dist = stop_distance;
+    g_stop_any = std::set<long>(any, any + nany);
+    g_stop_all = std::set<long>(all, all + nall);
+    g_stop_any_u = std::set<long>(anyu, anyu + nanyu);
+    g_stop_all_u = std::set<long>(allu, allu + nallu);
+}
+long oracle_last_naccepted(void) { return g_naccepted; }
 
 // Walls of the problems set up next on this thread: n bytes (nonzero = wall), n = 0 for none.
 void oracle_set_walls(long n, const unsigned char *walls, int refine) {

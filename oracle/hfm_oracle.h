@@ -42,6 +42,13 @@ const char *oracle_last_error(void);
 void oracle_set_order(int order);
 /* Pops of the last solve whose key was below the previous pop's key (diagnostics). */
 long oracle_last_dips(void);
+/* Stopping criteria of the solves run next (nMaxAccepted, stopAtDistance, stopWhenAnyAccepted,
+ * stopWhenAllAccepted, the _Unoriented sets; ExtraAlgorithms/CommonStoppingCriteria.h:57-165): max_accepted < 0
+ * for none, row-major indices, (x, y) indices x * n1 + y for the unoriented sets.  nAccepted of
+ * the last solve (the pop that triggered a stop included). */
+void oracle_set_stopping(long max_accepted, double stop_distance, int nany, const long *any, int nall,
+                         const long *all, int nanyu, const long *anyu, int nallu, const long *allu);
+long oracle_last_naccepted(void);
 
 /* Walls (ExtraAlgorithms/WallObstructionTest.h) of the problems set up next on this thread:
  * n bytes in row-major point order, nonzero = wall; n = 0 for none.  refine bit 0 =

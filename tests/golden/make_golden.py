@@ -17,6 +17,8 @@ Fixtures (all row-major):
   geodesics.npz      geodesicPoints / geodesicLengths from `tips` (discrete solver), four models
   forward_variation.npz  valueVariation from costVariation / seedValueVariation, four models
   orderings.npz      outputs under every arrayOrdering of non-square Riemann2 / Riemann3 / RSF2 problems
+  high_order.npz     values, activeNeighs, geodesicFlow, activeOffsets, nAccepted at order 2 and 3
+  stopping.npz       values, activeNeighs, nAccepted under each stopping criterion
   riemann3_128.npz   digests of values and Selling offsets at the benchmark size + 4096 seeded samples
 Inputs are regenerated by hamiltonfastmarching_amd.synthetic; the SHA-256 of each generated
 input is stored so that a test can tell a generator change from a numerical one.
@@ -523,6 +525,54 @@ def high_order_cases():
     print("high_order ok")
 
 
+def _centre(idx, h):
+    return [(k + 0.5) * h for k in idx]
+
+
+def stopping_params():
+    """Problems with the stopping criteria of ExtraAlgorithms/CommonStoppingCriteria.h:96-165 (targets at cell
+    centres; one target out of range, which the reference drops with a warning)."""
+    r2 = {k: v for k, v in syn.riemann2(40, seed=21).items()}
+    h2 = 1.0 / 40
+    r3 = syn.riemann3(14, seed=22)
+    h3 = 1.0 / 14
+    rs = {k: v for k, v in syn.rsf2(20, 12, seed=23).items() if k != "tips"}
+    el = {k: v for k, v in syn.elastica2(16, 12, seed=24).items()
+          if k not in ("inspectSensitivity", "inspectSensitivityWeights", "inspectSensitivityLengths")}
+    ds = 2 * np.pi / 12
+    return {
+        "riemann2_dist_o1": dict(r2, stopAtDistance=0.6),
+        "riemann2_max_o1": dict(r2, nMaxAccepted=300.0),
+        "riemann2_any_o2": dict(r2, order=2.0, stopWhenAnyAccepted=np.array([_centre((4, 32), h2)])),
+        "riemann2_all_o1": dict(r2, stopWhenAllAccepted=np.array(
+            [_centre((4, 4), h2), _centre((35, 12), h2), [2.0, 0.5]])),
+        "riemann2_distmax_o3": dict(r2, order=3.0, stopAtDistance=0.7, nMaxAccepted=500.0),
+        "riemann3_all_o1": dict(r3, stopWhenAllAccepted=np.array([_centre((2, 3, 4), h3), _centre((11, 9, 1), h3)])),
+        "riemann3_dist_o2": dict(r3, order=2.0, stopAtDistance=1.6),
+        "rsf2_anyU_o1": dict(rs, stopWhenAnyAccepted_Unoriented=np.array([_centre((2, 15), 0.05)])),
+        "rsf2_allU_o2": dict(rs, order=2.0, stopWhenAllAccepted_Unoriented=np.array(
+            [_centre((2, 15), 0.05), _centre((17, 4), 0.05)])),
+        "rsf2_any_o1": dict(rs, stopWhenAnyAccepted=np.array([_centre((6, 13), 0.05) + [5 * ds]])),
+        "elastica2_allU_max_o1": dict(el, nMaxAccepted=1500.0, stopWhenAllAccepted_Unoriented=np.array(
+            [_centre((3, 3), 1 / 16), _centre((12, 10), 1 / 16)])),
+        "elastica2_dist_o1": dict(el, stopAtDistance=1.2),
+    }
+
+
+def stopping_cases():
+    """values / activeNeighs / nAccepted under each stopping criterion."""
+    out = {}
+    for name, p in stopping_params().items():
+        q = dict(p, exportActiveNeighs=1.0)
+        r = oracle.run_reference(q)
+        assert r["retcode"] == 0, r["log"]
+        for k in ("values", "activeNeighs", "nAccepted"):
+            out[f"{name}_{k}"] = np.asarray(r[k])
+        print(name, int(np.asarray(r["nAccepted"])), int(np.isfinite(np.asarray(r["values"])).sum()))
+    np.savez_compressed(os.path.join(HERE, "stopping.npz"), **out)
+    print("stopping ok")
+
+
 def main():
     for m in ("Riemann2", "Riemann3", "ReedsSheppForward2", "Elastica2"):
         if not oracle.ref_available(m):
@@ -531,7 +581,8 @@ def main():
     if only:  # regenerate the named fixture groups only
         groups = {"geodesics": geodesic_cases, "forward": forward_cases, "walls": wall_cases, "seeds": seed_cases,
                   "bypass": bypass_cases, "stencils": stencil_cases, "big": big_case,
-                  "orderings": ordering_cases, "high_order": high_order_cases}
+                  "orderings": ordering_cases, "high_order": high_order_cases,
+                  "stopping": stopping_cases}
         for a in only:
             groups[a]()
         return
@@ -548,6 +599,7 @@ def main():
     stencil_cases()
     ordering_cases()
     high_order_cases()
+    stopping_cases()
     if "--big" in sys.argv:
         big_case()
 

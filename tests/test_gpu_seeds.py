@@ -8,7 +8,7 @@
 * seeds_Unoriented / seedValues_Unoriented (StaticFactoring.hxx:351-380) and tips_Unoriented
   (HFMInterface.hxx:498-522) of the curvature models: values 1e-12, geodesicLengths_Unoriented
   equal, geodesicPoints_Unoriented within 1e-6 of a grid step;
-* nAccepted (CommonStoppingCriteria.h:162-165) equal.
+* nAccepted (ExtraAlgorithms/CommonStoppingCriteria.h:162-165) equal.
 FP64 solves."""
 import os
 import sys
@@ -68,8 +68,7 @@ def test_seed_radius_on_curvature_model_is_the_reference_error():
         hfm.run(dict(p, seeds=np.array([[0.5, 0.5, 1.0]]), seedRadius=2.0))
 
 
-@pytest.mark.parametrize("key,value", [("factoringRadius", 10.0), ("stopAtDistance", 0.5),
-                                       ("nMaxAccepted", 100.0)])
+@pytest.mark.parametrize("key,value", [("factoringRadius", 10.0)])
 def test_unsupported_keys_are_refused(key, value):
     """Keys whose effect depends on the serial heap's acceptance order are refused, not ignored."""
     from hamiltonfastmarching_amd import synthetic as syn
