@@ -407,6 +407,31 @@ class Plan:
         self._check(self._L.hfm_plan_set_order(self._h, int(order)))
         return self
 
+    def set_factoring(self, values=None, gradients=None, mask=None, shift=None):
+        """Static factor (factoringValues / factoringGradients / factoringMask /
+        factoringIndexShift): values over a box (its shape), gradients (box + (dim,)) in grid-index
+        units, mask (nonzero = on; None: everywhere), shift = the box's first grid index (None: the
+        box is the whole grid).  values None removes the factor.  A factored solve runs the serial
+        acceptance loop (FP64 plans without walls)."""
+        if values is None:
+            self._check(self._L.hfm_plan_set_factoring(self._h, None, None, None, None, None))
+            return self
+        v = np.ascontiguousarray(values, np.float64)
+        g = np.ascontiguousarray(gradients, np.float64)
+        if v.ndim != self.dim or g.shape != v.shape + (self.dim,):
+            raise PyExp("set_factoring: values must be a box of the grid and gradients (box..., dim)")
+        bd = np.array(v.shape, np.int64)
+        sh = None if shift is None else np.ascontiguousarray(np.asarray(shift, np.int64).ravel())
+        m = None if mask is None else np.ascontiguousarray(np.asarray(mask) != 0, np.uint8)
+        if m is not None and m.shape != v.shape:
+            raise PyExp("set_factoring: mask must have the shape of values")
+        P = ctypes.POINTER
+        self._check(self._L.hfm_plan_set_factoring(
+            self._h, bd.ctypes.data_as(P(ctypes.c_int64)),
+            None if sh is None else sh.ctypes.data_as(P(ctypes.c_int64)), _dptr(v), _dptr(g),
+            None if m is None else m.ctypes.data_as(P(ctypes.c_uint8))))
+        return self
+
     def set_stopping(self, max_accepted=-1, stop_distance=float("inf"), any_idx=(), all_idx=(),
                      any_unoriented=(), all_unoriented=()):
         """Stopping criteria (nMaxAccepted, stopAtDistance, stopWhenAnyAccepted / AllAccepted and

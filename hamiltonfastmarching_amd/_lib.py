@@ -90,6 +90,8 @@ SIGNATURES = [
     ("hfm_plan_set_order", c_int, [c_void_p, c_int]),
     ("hfm_plan_set_stopping", c_int, [c_void_p, c_int64, c_double, c_int, P(c_int64), c_int, P(c_int64), c_int,
                                       P(c_int64), c_int, P(c_int64)]),
+    ("hfm_plan_set_factoring", c_int, [c_void_p, P(c_int64), P(c_int64), P(c_double), P(c_double),
+                                       P(ctypes.c_uint8)]),
     ("hfm_plan_set_delta", c_int, [c_void_p, c_double]),
     ("hfm_plan_set_round_target", c_int, [c_void_p, c_int]),
     ("hfm_plan_set_grid", c_int, [c_void_p, c_int]),

@@ -534,8 +534,8 @@ int hfm_plan_adjoint_impl(hfm_plan *p, int ntargets, const double *targets, cons
         return 1;
     }
     if (p->exact_solved) {
-        set_error("hfm_plan_adjoint: differentiation of an order " + std::to_string(p->order) +
-                  " solve is not supported by this library (order 1 only)");
+        set_error("hfm_plan_adjoint: differentiation of " + p->exact_what +
+                  " (the serial acceptance loop) is not supported by this library");
         return 1;
     }
     try {
@@ -646,8 +646,8 @@ int hfm_plan_forward_variation_impl(hfm_plan *p, int nvar, const double *d_cost_
         return 1;
     }
     if (p->exact_solved) {
-        set_error("hfm_plan_forward_variation: differentiation of an order " + std::to_string(p->order) +
-                  " solve is not supported by this library (order 1 only)");
+        set_error("hfm_plan_forward_variation: differentiation of " + p->exact_what +
+                  " (the serial acceptance loop) is not supported by this library");
         return 1;
     }
     if (nvar <= 0) {

@@ -409,6 +409,17 @@ template <int NC> __device__ inline float local_solve_update(const float (&v0)[N
     }
 }
 
+// Static factor (StaticFactoring.h:20-63): values T, gradients of T in grid units (dim per point)
+// and mask over a box of bd points whose first grid index is `shift` (sub), or over the whole grid
+// (sub = 0, bd = the grid's extents), row-major over the box.  on = 0: no factoring.
+struct FactorDev {
+    const double *vals = nullptr;
+    const double *grads = nullptr;
+    const unsigned char *mask = nullptr;
+    int bd[3] = {1, 1, 1}, shift[3] = {0, 0, 0};
+    int sub = 0, on = 0;
+};
+
 // Launch-wide reductions (mean one-step time, widest stencil) go to RED_SLOTS accumulators, each
 // on its own 256-byte line, chosen by the global wave index.  A single accumulator would
 // serialise every wave of the launch on one address (tens of thousands of same-address atomics

@@ -1,7 +1,9 @@
 // hfm_exact.hip -- the reference's serial acceptance loop on the device, for the options whose
 // result is fixed by the order in which the heap accepts points: the second / third order
 // recompute at acceptance (`order` = 2 or 3, Headers/Base/Implementation/HamiltonFastMarching.hxx
-// :59-123 with Recompute :187-272 and EulerianStencil.hxx:155-233).
+// :59-123 with Recompute :187-272 and EulerianStencil.hxx:155-233), static factoring (the same
+// recompute with corrected differences, Headers/ExtraAlgorithms/Implementation/
+// StaticFactoring.hxx:11-55) and the stopping criteria (ExtraAlgorithms/CommonStoppingCriteria.h).
 //
 // With order >= 2 a point's value is recomputed when it leaves the heap, from its active
 // differences and, where the next points along an offset are already accepted and no larger,
@@ -108,36 +110,83 @@ struct RecomputeEntry {
     int o[3];
 };
 
+// SetIndex (StaticFactoring.hxx:21-30): the box index of grid point c into cur, and whether the
+// factor applies there (in the box and in the mask).  NeedsRecompute (:11-19) is the same test:
+// these schemes do not factor their first pass (HamiltonFastMarching.h:47-50).
+__device__ inline bool factor_at(const Geom &g, const FactorDev &F, const int *c, int *cur) {
+    if (!F.on) return false;
+    for (int i = 0; i < 3; ++i) cur[i] = i < g.dim ? c[i] - F.shift[i] : 0;
+    if (F.sub)
+        for (int i = 0; i < g.dim; ++i)
+            if (cur[i] < 0 || cur[i] >= F.bd[i]) return false;
+    return F.mask[((long long)cur[0] * F.bd[1] + cur[1]) * F.bd[2] + cur[2]] != 0;
+}
+
+// Correction (StaticFactoring.hxx:33-55) of the difference along the signed offset o at box index
+// cur for the order ord the difference uses: the factor's directional derivative grad T . o plus
+// the factor's own one-sided difference of that order with the opposite sign.  On a box, a
+// difference that leaves it gets none; on the whole grid the angle axis of the curvature models
+// wraps.
+template <int ModelT>
+__device__ inline double factor_corr(const Geom &g, const FactorDev &F, const int *cur, const int *o, int ord) {
+    int last[3], i1[3], i2[3];
+    for (int i = 0; i < 3; ++i) {
+        last[i] = cur[i] + ord * o[i];
+        i1[i] = cur[i] + o[i];
+        i2[i] = cur[i] + 2 * o[i];
+    }
+    if (F.sub) {
+        for (int i = 0; i < g.dim; ++i)
+            if (last[i] < 0 || last[i] >= F.bd[i]) return 0.;
+    } else if (ModelShape<ModelT>::SHARED) {
+        const int n = F.bd[2];
+        last[2] = ((last[2] % n) + n) % n;
+        i1[2] = ((i1[2] % n) + n) % n;
+        i2[2] = ((i2[2] % n) + n) % n;
+    }
+    auto lin = [&](const int *i) { return ((long long)i[0] * F.bd[1] + i[1]) * F.bd[2] + i[2]; };
+    const long long c = lin(cur);
+    double deriv = 0.;
+    for (int i = 0; i < g.dim; ++i) deriv += F.grads[c * g.dim + i] * (double)o[i];
+    const double v0 = F.vals[c], vl = F.vals[lin(last)];
+    if (ord == 1) return deriv + (v0 - vl);
+    const double v1 = F.vals[lin(i1)];
+    if (ord == 2) return deriv + (1.5 * v0 - 2. * v1 + 0.5 * vl);
+    const double v2 = F.vals[lin(i2)];
+    return deriv + ((11. / 6.) * v0 - 3. * v1 + 1.5 * v2 - (1. / 3.) * vl);
+}
+
 // GetValueCorr of Recompute (HamiltonFastMarching.hxx:199-262) for the difference x -> x + o
-// (o signed), without factoring: the neighbour's value, or (order >= 2) (2/3)(2 v1 - v2 / 2) when
-// x + 2 o is accepted and v2 <= v1, or (order >= 3) (6/11)(3 v1 - 3 v2 / 2 + v3 / 3) when x + 3 o
-// also is and v3 <= v2.  The small-correction tests apply to strictly causal schemes only
-// (HamiltonFastMarching.h:46), which these are not.
+// (o signed): the neighbour's value, or (order >= 2) (2/3)(2 v1 - v2 / 2) when x + 2 o is accepted
+// and v2 <= v1, or (order >= 3) (6/11)(3 v1 - 3 v2 / 2 + v3 / 3) when x + 3 o also is and v3 <= v2,
+// each plus the factor's correction of that order inside the bracket when the factor applies at x
+// (fcur: x's box index, null otherwise).  The small-correction tests apply to strictly causal
+// schemes only (HamiltonFastMarching.h:46), which these are not.
 template <int ModelT>
 __device__ inline double value_corr(const Geom &g, const double *u, const unsigned char *acc, const int *c, const int *o,
-                                    int order, int &ord) {
+                                    int order, const FactorDev &F, const int *fcur, int &ord) {
+    auto corr = [&](int k) { return fcur ? factor_corr<ModelT>(g, F, fcur, o, k) : 0.; };
     int c1[3], c2[3], c3[3];
     shift_coords_m<ModelT>(g, c, o[0], o[1], o[2], c1);  // active: in the domain and accepted
     const double v1 = u[(long long)tile_of(g, c1) * 64 + local_of(g, c1)];
     ord = 1;
-    double val = v1 + 0.;
     if (order >= 2 && shift_coords_m<ModelT>(g, c1, o[0], o[1], o[2], c2)) {
         const long long q2 = (long long)tile_of(g, c2) * 64 + local_of(g, c2);
         const double v2 = u[q2];
         if (acc[q2] && !(v2 > v1)) {
-            ord = 2;
-            val = (2. / 3.) * ((2. * v1 - 0.5 * v2) + 0.);
             if (order >= 3 && shift_coords_m<ModelT>(g, c2, o[0], o[1], o[2], c3)) {
                 const long long q3 = (long long)tile_of(g, c3) * 64 + local_of(g, c3);
                 const double v3 = u[q3];
                 if (acc[q3] && !(v3 > v2)) {
                     ord = 3;
-                    val = (6. / 11.) * ((3. * v1 - 1.5 * v2 + (1. / 3.) * v3) + 0.);
+                    return (6. / 11.) * ((3. * v1 - 1.5 * v2 + (1. / 3.) * v3) + corr(3));
                 }
             }
+            ord = 2;
+            return (2. / 3.) * ((2. * v1 - 0.5 * v2) + corr(2));
         }
     }
-    return val;
+    return v1 + corr(1);
 }
 
 // Recompute (HamiltonFastMarching.hxx:187-272, EulerianStencil.hxx:155-233) of the point at c
@@ -146,12 +195,14 @@ __device__ inline double value_corr(const Geom &g, const double *u, const unsign
 // minimum with the weights scaled by 1, 9/4, 121/36 by order.
 template <int ModelT>
 __device__ inline int recompute(const Geom &g, const StencilData<double> &sd, const double *u, const unsigned char *acc,
-                                long long p, const int *c, unsigned long long act, int order, RecomputeEntry *e,
-                                double *value) {
+                                long long p, const int *c, unsigned long long act, int order, const FactorDev &F,
+                                RecomputeEntry *e, double *value) {
     using S = ModelShape<ModelT>;
     constexpr int NF = S::NFWD, NC = NF + S::NSYM;
     LaneStencil<ModelT, double> st;
     st.load(g, sd, p, c);
+    int fcur[3];
+    const bool fact = factor_at(g, F, c, fcur);
     int n = 0;
     for (int k = 0; k < NC; ++k) {
         int o0, o1, o2;
@@ -164,7 +215,7 @@ __device__ inline int recompute(const Geom &g, const StencilData<double> &sd, co
             r.o[1] = sg * o1;
             r.o[2] = sg * o2;
             r.w = st.w[k];
-            r.value = value_corr<ModelT>(g, u, acc, c, r.o, order, r.ord);
+            r.value = value_corr<ModelT>(g, u, acc, c, r.o, order, F, fact ? fcur : nullptr, r.ord);
         }
     }
     Quad q{__builtin_huge_val(), 0., 0., -1.};
@@ -220,10 +271,12 @@ __global__ __launch_bounds__(64) void k_fm_exact(ExactArgs a) {
             }
         }
         if (a.u[y] > a.stop_distance) stop = true;
-        // the recompute: a point without active differences keeps its value
-        if (a.order > 1 && a.active[y]) {
+        // the recompute (order >= 2, or the factor applies): a point without active differences
+        // keeps its value
+        int fcur[3];
+        if ((a.order > 1 || factor_at(g, a.fac, cy, fcur)) && a.active[y]) {
             double v;
-            (void)recompute<ModelT>(g, a.sd, a.u, a.accepted, y, cy, a.active[y], a.order, re, &v);
+            (void)recompute<ModelT>(g, a.sd, a.u, a.accepted, y, cy, a.active[y], a.order, a.fac, re, &v);
             top.value = v;
             a.u[y] = v;
         }
@@ -365,7 +418,7 @@ template <int ModelT>
 __global__ __launch_bounds__(256) void k_flow_exact(Geom g, StencilData<double> sd, const double *__restrict__ u,
                                                    const unsigned char *__restrict__ acc,
                                                    const unsigned long long *__restrict__ active, int order,
-                                                   FlowScale fs, double *__restrict__ flow,
+                                                   FactorDev fac, FlowScale fs, double *__restrict__ flow,
                                                    double *__restrict__ active_out, double *__restrict__ aoffs_out,
                                                    long long np) {
     using S = ModelShape<ModelT>;
@@ -380,7 +433,7 @@ __global__ __launch_bounds__(256) void k_flow_exact(Geom g, StencilData<double> 
     RecomputeEntry re[2 * NC];
     int n = 0;
     double value = 0.;
-    if (act) n = recompute<ModelT>(g, sd, u, acc, p, c, act, order, re, &value);
+    if (act) n = recompute<ModelT>(g, sd, u, acc, p, c, act, order, fac, re, &value);
     constexpr double mult[4] = {0., 1., 3. / 2., 11. / 6.};
     double f[3] = {0., 0., 0.};
     for (int i = 0; i < n; ++i) {
@@ -462,24 +515,25 @@ hipError_t launch_fm_exact(int model, const ExactArgs &a, hipStream_t s) {
 }
 
 hipError_t launch_flow_exact(int model, const Geom &g, const StencilData<double> &sd, const double *u,
-                             const unsigned char *acc, const unsigned long long *active, int order, const FlowScale &fs,
-                             double *flow, double *act_out, double *aoffs, long long np, hipStream_t s) {
+                             const unsigned char *acc, const unsigned long long *active, int order,
+                             const FactorDev &fac, const FlowScale &fs, double *flow, double *act_out, double *aoffs,
+                             long long np, hipStream_t s) {
     const dim3 gr(exact_blocks(np)), bl(256);
     switch (model) {
         case MODEL_RIEMANN2:
-            hipLaunchKernelGGL((k_flow_exact<MODEL_RIEMANN2>), gr, bl, 0, s, g, sd, u, acc, active, order, fs, flow,
+            hipLaunchKernelGGL((k_flow_exact<MODEL_RIEMANN2>), gr, bl, 0, s, g, sd, u, acc, active, order, fac, fs, flow,
                                act_out, aoffs, np);
             break;
         case MODEL_RIEMANN3:
-            hipLaunchKernelGGL((k_flow_exact<MODEL_RIEMANN3>), gr, bl, 0, s, g, sd, u, acc, active, order, fs, flow,
+            hipLaunchKernelGGL((k_flow_exact<MODEL_RIEMANN3>), gr, bl, 0, s, g, sd, u, acc, active, order, fac, fs, flow,
                                act_out, aoffs, np);
             break;
         case MODEL_RSF2:
-            hipLaunchKernelGGL((k_flow_exact<MODEL_RSF2>), gr, bl, 0, s, g, sd, u, acc, active, order, fs, flow,
+            hipLaunchKernelGGL((k_flow_exact<MODEL_RSF2>), gr, bl, 0, s, g, sd, u, acc, active, order, fac, fs, flow,
                                act_out, aoffs, np);
             break;
         case MODEL_ELASTICA2:
-            hipLaunchKernelGGL((k_flow_exact<MODEL_ELASTICA2>), gr, bl, 0, s, g, sd, u, acc, active, order, fs, flow,
+            hipLaunchKernelGGL((k_flow_exact<MODEL_ELASTICA2>), gr, bl, 0, s, g, sd, u, acc, active, order, fac, fs, flow,
                                act_out, aoffs, np);
             break;
         default: return hipErrorInvalidValue;

@@ -149,6 +149,7 @@ struct hfm_plan {
     // accepted flags, activeNeighs bits and incremental quadratics of the last such solve.
     int order = 1;
     bool exact_solved = false;
+    std::string exact_what;  // what made the last solve run the serial loop, for refusals
     bool exact_rev_dirty = true;
     int *ex_cnt = nullptr, *ex_start = nullptr, *ex_fill = nullptr, *ex_rcount = nullptr, *ex_scan_work = nullptr;
     uint32_t *ex_roffs = nullptr;
@@ -166,6 +167,11 @@ struct hfm_plan {
     double stop_dist = 0.;
     std::vector<long long> stop_any, stop_all, stop_any_u, stop_all_u;
     unsigned char *ex_flags = nullptr, *ex_flags_u = nullptr;
+    // static factor (hfm_plan_set_factoring): device copies of values, gradients (grid units) and
+    // mask over the box; fac.on = 0 for none
+    hfm::FactorDev fac;
+    double *fac_vals = nullptr, *fac_grads = nullptr;
+    unsigned char *fac_mask = nullptr;
     hipStream_t graph_stream = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     std::vector<char> graph_key;

@@ -155,6 +155,13 @@ struct hfm_io {
 
 namespace {
 
+// The angle step of the curvature models' bundle axis, 2 pi / n_theta in FP64 as the reference
+// forms it (mathPi a double, CommonTraits.h:30; Curvature2.h:108).
+double theta_step(int64_t ntheta) {
+    constexpr double pi = 3.141592653589793238462643383279502884;
+    return 2 * pi / (double)ntheta;
+}
+
 // Index map between a stored array (as the user wrote it) and the internal row-major grid
 // array: internal multi-index n = P(s) of stored multi-index s (only the first g axes move).
 std::vector<int64_t> permute_dims(const std::vector<int64_t> &d, Ordering o, bool inverse = false) {
@@ -327,6 +334,96 @@ FieldView grid_field_view(hfm_io *io, const std::string &key, const std::vector<
     return v;
 }
 
+// Distance guesses of the Riemannian stencil (Riemannian.h:62-71): the metric as given (primal,
+// or dual and inverted) scaled by h_i h_j, at a grid index, or at a point (grid-index units)
+// interpolated over the 2^d cells around it (BaseGrid.h:80-104): out-of-domain corners weighted 0
+// without renormalisation (PeriodicGrid.hxx:28-40), the first corner always evaluated, the
+// others only with a nonzero weight (MapWeightedSum.h:21-36).
+template <int D> struct MetricGuess {
+    using Sym = hfm::SymPacked<D>;
+    static constexpr int K = Sym::K;
+    const std::vector<int64_t> &dims;
+    const double *h;
+    const double *metric;
+    bool is_field, dual;
+    bool in_domain(const int64_t *idx) const {
+        for (int i = 0; i < D; ++i)
+            if (idx[i] < 0 || idx[i] >= dims[i]) return false;
+        return true;
+    }
+    Sym raw(const int64_t *idx) const {
+        Sym m;
+        int64_t lin = 0;
+        for (int i = 0; i < D; ++i) lin = lin * dims[i] + idx[i];
+        for (int k = 0; k < K; ++k) m.m[k] = metric[(is_field ? lin : 0) * K + k];
+        return m;
+    }
+    // Rescale<false>: primal metric times the rank-one tensor h h^T, component by component
+    Sym rescale(Sym m) const {
+        if (dual) m = hfm::sym_inverse<D>(m);
+        for (int i = 0; i < D; ++i)
+            for (int j = 0; j <= i; ++j) m.m[Sym::lin(i, j)] *= h[i] * h[j];
+        return m;
+    }
+    Sym at_index(const int64_t *idx) const { return rescale(raw(idx)); }
+    Sym at_point(const double *p) const {
+        int64_t nearest[D];
+        int dir[D];
+        double wgt[D];
+        for (int i = 0; i < D; ++i) {
+            nearest[i] = (int64_t)std::floor(p[i]);
+            const double diff = p[i] - (double)nearest[i] - 0.5;
+            dir[i] = diff > 0 ? 1 : -1;
+            wgt[i] = std::abs(diff);
+        }
+        Sym mp{};
+        for (int c = 0; c < (1 << D); ++c) {
+            int64_t q[D];
+            double w = 1.;
+            for (int j = 0; j < D; ++j) {
+                q[j] = nearest[j];
+                if (c >> j & 1) {
+                    w *= wgt[j];
+                    q[j] += dir[j];
+                } else {
+                    w *= 1. - wgt[j];
+                }
+            }
+            if (!in_domain(q)) w = 0.;
+            if (c == 0) {
+                // the reference evaluates the first corner unconditionally; outside the domain there is
+                // no metric, and no corner at all is valid (PeriodicGrid.hxx:37-38)
+                if (!in_domain(q)) throw IOError{"Error : point has no valid neighbors in the grid domain"};
+                const Sym m = raw(q);
+                for (int k = 0; k < K; ++k) mp.m[k] = w * m.m[k];
+            } else if (w != 0.) {
+                const Sym m = raw(q);
+                for (int k = 0; k < K; ++k) mp.m[k] += w * m.m[k];
+            }
+        }
+        return rescale(mp);
+    }
+    // sqrt(<v, M v>) and M v / Norm(v) (a reciprocal multiply) in the reference's order
+    // (SymmetricMatrixType.h:42-54, SymmetricMatrixType.hxx:18-44, VectorType.h:37-42)
+    static double norm(const Sym &m, const double *v) {
+        double sum = 0.;
+        for (int i = 0; i < D; ++i) {
+            double sumi = 0.;
+            for (int j = 0; j < D; ++j) sumi += m(i, j) * v[j];
+            sum += v[i] * sumi;
+        }
+        return std::sqrt(sum);
+    }
+    static void gradient(const Sym &m, const double *v, double *g) {
+        for (int i = 0; i < D; ++i) {
+            g[i] = 0.;
+            for (int j = 0; j < D; ++j) g[i] += m(i, j) * v[j];
+        }
+        const double b = 1. / norm(m, v);
+        for (int i = 0; i < D; ++i) g[i] *= b;
+    }
+};
+
 // Seed spreading (StaticFactoring.hxx:284-345), Riemann models: every grid point within
 // `radius` (grid units) of a seed p, and p's own cell, becomes a seed with value
 // v + (|p - q|_{M(p)} + |p - q|_{M(q)}) / 2, where M(p) is the metric interpolated at p from the
@@ -395,34 +492,8 @@ void spread_seeds_d(hfm_io *io, const std::vector<int64_t> &dims, const double *
         return offs;
     };
     auto org = [&](int i) { return origin.empty() ? 0. : origin[i]; };
-    auto in_domain = [&](const int64_t *idx) {
-        for (int i = 0; i < D; ++i)
-            if (idx[i] < 0 || idx[i] >= dims[i]) return false;
-        return true;
-    };
-    auto metric_at = [&](const int64_t *idx) {
-        Sym m;
-        int64_t lin = 0;
-        for (int i = 0; i < D; ++i) lin = lin * dims[i] + idx[i];
-        for (int k = 0; k < K; ++k) m.m[k] = metric[(is_field ? lin : 0) * K + k];
-        return m;
-    };
-    // Rescale<false>: primal metric times the rank-one tensor h h^T, component by component
-    auto rescale = [&](Sym m) {
-        if (dual) m = hfm::sym_inverse<D>(m);
-        for (int i = 0; i < D; ++i)
-            for (int j = 0; j <= i; ++j) m.m[Sym::lin(i, j)] *= h[i] * h[j];
-        return m;
-    };
-    auto norm = [&](const Sym &m, const double *v) {  // sqrt(<v, M v>) in the reference's order
-        double sum = 0.;
-        for (int i = 0; i < D; ++i) {
-            double sumi = 0.;
-            for (int j = 0; j < D; ++j) sumi += m(i, j) * v[j];
-            sum += v[i] * sumi;
-        }
-        return std::sqrt(sum);
-    };
+    const MetricGuess<D> guess{dims, h, metric, is_field, dual};
+    auto in_domain = [&](const int64_t *idx) { return guess.in_domain(idx); };
     const int n_in = (int)(seeds.size() / D);
     std::vector<double> newPts, newVals;
     const int64_t bd = 1 + (int64_t)std::ceil(radius);
@@ -433,42 +504,9 @@ void spread_seeds_d(hfm_io *io, const std::vector<int64_t> &dims, const double *
         double p[D];
         for (int i = 0; i < D; ++i) p[i] = (seeds[(size_t)s * D + i] - org(i)) / h[i];
         const double value = values[s];
-        // GetGuess(p): interpolated metric
         int64_t nearest[D];
-        int dir[D];
-        double wgt[D];
-        for (int i = 0; i < D; ++i) {
-            nearest[i] = (int64_t)std::floor(p[i]);
-            const double diff = p[i] - (double)nearest[i] - 0.5;
-            dir[i] = diff > 0 ? 1 : -1;
-            wgt[i] = std::abs(diff);
-        }
-        Sym mp{};
-        for (int c = 0; c < (1 << D); ++c) {
-            int64_t q[D];
-            double w = 1.;
-            for (int j = 0; j < D; ++j) {
-                q[j] = nearest[j];
-                if (c >> j & 1) {
-                    w *= wgt[j];
-                    q[j] += dir[j];
-                } else {
-                    w *= 1. - wgt[j];
-                }
-            }
-            if (!in_domain(q)) w = 0.;
-            if (c == 0) {
-                // the reference evaluates the first corner unconditionally; outside the domain there is
-                // no metric, and no corner at all is valid (PeriodicGrid.hxx:37-38)
-                if (!in_domain(q)) throw IOError{"Error : point has no valid neighbors in the grid domain"};
-                const Sym m = metric_at(q);
-                for (int k = 0; k < K; ++k) mp.m[k] = w * m.m[k];
-            } else if (w != 0.) {
-                const Sym m = metric_at(q);
-                for (int k = 0; k < K; ++k) mp.m[k] += w * m.m[k];
-            }
-        }
-        const Sym distp = rescale(mp);
+        for (int i = 0; i < D; ++i) nearest[i] = (int64_t)std::floor(p[i]);
+        const Sym distp = guess.at_point(p);
         std::set<std::vector<int64_t>> seen;
         auto insert = [&](const int64_t *index, double rad) {
             double v[D], q[D];
@@ -481,9 +519,9 @@ void spread_seeds_d(hfm_io *io, const std::vector<int64_t> &dims, const double *
             if (std::sqrt(e2) >= rad) return;
             if (!seen.insert(std::vector<int64_t>(index, index + D)).second) return;
             if (!in_domain(index)) return;
-            const Sym distq = rescale(metric_at(index));
+            const Sym distq = guess.at_index(index);
             for (int i = 0; i < D; ++i) newPts.push_back(q[i]);
-            newVals.push_back(value + 0.5 * (norm(distp, v) + norm(distq, v)));
+            newVals.push_back(value + 0.5 * (MetricGuess<D>::norm(distp, v) + MetricGuess<D>::norm(distq, v)));
         };
         insert(nearest, std::numeric_limits<double>::infinity());
         for (int64_t b = 0; b < boxn; ++b) {
@@ -525,6 +563,176 @@ void spread_seeds_d(hfm_io *io, const std::vector<int64_t> &dims, const double *
     fv.data = newVals;
     io->set_computed("spreadedSeeds", std::move(fp));
     io->set_computed("spreadedSeedValues", std::move(fv));
+}
+
+// A static factor on the host, as hfm_plan_set_factoring takes it: extents and first grid index
+// of its box, values, gradients (grid-index units, dim per point) and mask (empty: everywhere),
+// row-major over the box.
+struct FactorHost {
+    bool on = false, sub = false;
+    std::vector<int64_t> bd, shift;
+    std::vector<double> vals, grads;
+    std::vector<uint8_t> mask;
+};
+
+// ComputeFactor (StaticFactoring.hxx:116-239), Riemann models, from the seed p (grid-index
+// units): over the box of grid points from max(0, floor(p - R)) to min(n - 1, ceil(p + R))
+// (upper end excluded, as the reference sizes it), or the whole grid for R < 0.
+//   "Seed" / "Key": T(x) = grad . v, grad = M v / |v|_M with v = x - p and M the metric at p
+//     (so T = |v|_M); both 0 at v = 0.
+//   "Both": T = g . v with g the mean of that gradient and the one of the metric at x -- the
+//     metric read at the point's index within the box, as the reference reads it -- and the
+//     gradient of T completed by centred differences of |v|_{M(x)} / 2 along each axis (one-sided
+//     at the box's faces), accumulated point by point in the reference's order.
+template <int D>
+FactorHost compute_factor_d(const std::vector<int64_t> &dims, const double *h, const double *metric, bool is_field,
+                            bool dual, const double *seed, int64_t R, bool both) {
+    const MetricGuess<D> guess{dims, h, metric, is_field, dual};
+    using Sym = typename MetricGuess<D>::Sym;
+    FactorHost F;
+    F.sub = R >= 0;
+    F.bd.assign(D, 0);
+    F.shift.assign(D, 0);
+    int64_t size = 1;
+    for (int i = 0; i < D; ++i) {
+        if (R < 0) {
+            F.bd[i] = dims[i];
+        } else {
+            const int64_t bottom = std::max<int64_t>(0, (int64_t)std::floor(seed[i] - (double)R));
+            const int64_t top = std::min<int64_t>(dims[i] - 1, (int64_t)std::ceil(seed[i] + (double)R));
+            F.shift[i] = bottom;
+            F.bd[i] = top - bottom;
+        }
+        if (F.bd[i] <= 0) return FactorHost();  // an empty box: the factor is disabled
+        size *= F.bd[i];
+    }
+    F.on = true;
+    F.vals.assign((size_t)size, std::numeric_limits<double>::quiet_NaN());
+    F.grads.assign((size_t)size * D, 0.);
+    const Sym seedDist = guess.at_point(seed);
+    auto unravel = [&](int64_t i, int64_t *idx) {
+        for (int k = D - 1; k >= 0; --k) {
+            idx[k] = i % F.bd[k];
+            i /= F.bd[k];
+        }
+    };
+    auto in_box = [&](const int64_t *idx) {
+        for (int k = 0; k < D; ++k)
+            if (idx[k] < 0 || idx[k] >= F.bd[k]) return false;
+        return true;
+    };
+    auto conv = [&](const int64_t *idx) {
+        int64_t r = idx[0];
+        for (int k = 1; k < D; ++k) r = r * F.bd[k] + idx[k];
+        return r;
+    };
+    // -Gradient(-v) of a guess (the sign convention of asymmetric norms), 0 at v = 0
+    auto grad_of = [&](const Sym &m, const double *v, bool null, double *g) {
+        if (null) {
+            for (int k = 0; k < D; ++k) g[k] = 0.;
+            return;
+        }
+        double mv[D], gg[D];
+        for (int k = 0; k < D; ++k) mv[k] = -v[k];
+        MetricGuess<D>::gradient(m, mv, gg);
+        for (int k = 0; k < D; ++k) g[k] = -gg[k];
+    };
+    for (int64_t pi = 0; pi < size; ++pi) {
+        int64_t loc[D];
+        unravel(pi, loc);
+        double v[D];
+        bool null = true;
+        for (int k = 0; k < D; ++k) {
+            v[k] = ((double)(loc[k] + F.shift[k]) + 0.5) - seed[k];
+            null = null && v[k] == 0.;
+        }
+        double g[D];
+        if (!both) {
+            grad_of(seedDist, v, null, g);
+            double sp = 0.;
+            for (int k = 0; k < D; ++k) {
+                F.grads[(size_t)pi * D + k] = g[k];
+                sp += g[k] * v[k];
+            }
+            F.vals[(size_t)pi] = sp;
+            continue;
+        }
+        const Sym pDist = guess.at_index(loc);
+        double seedG[D], pG[D];
+        grad_of(seedDist, v, null, seedG);
+        grad_of(pDist, v, null, pG);
+        double sp = 0.;
+        for (int k = 0; k < D; ++k) g[k] = 0.5 * (pG[k] + seedG[k]);
+        for (int k = 0; k < D; ++k) sp += g[k] * v[k];
+        F.vals[(size_t)pi] = sp;
+        for (int k = 0; k < D; ++k) F.grads[(size_t)pi * D + k] += g[k];
+        for (int k = 0; k < D; ++k)
+            for (int eps = -1; eps <= 1; eps += 2) {
+                int64_t q[D];
+                for (int j = 0; j < D; ++j) q[j] = loc[j];
+                q[k] += eps;
+                if (in_box(q)) {
+                    double w[D];
+                    for (int j = 0; j < D; ++j) w[j] = -((((double)(q[j] + F.shift[j])) + 0.5) - seed[j]);
+                    double qVal = MetricGuess<D>::norm(pDist, w) / 2.;
+                    const int64_t qi = conv(q);
+                    q[k] += eps;
+                    if (in_box(q)) qVal /= 2;
+                    F.grads[(size_t)qi * D + k] -= eps * qVal;
+                } else {
+                    double pgv = 0.;
+                    for (int j = 0; j < D; ++j) pgv += pG[j] * v[j];
+                    F.grads[(size_t)pi * D + k] += eps * pgv / 2.;
+                }
+            }
+    }
+    return F;
+}
+
+// ImportFactor (StaticFactoring.hxx:77-113): factoringValues, factoringGradients (physical,
+// re-dimensioned into grid-index units by the scale of each axis), factoringMask (nonzero = on;
+// default everywhere) and factoringIndexShift (the box's first grid index, truncated to integers;
+// without it the arrays cover the whole grid), in the IO's array ordering.
+FactorHost import_factor(hfm_io *io, const std::vector<int64_t> &dims, const double *scales, int dim) {
+    auto box_array = [&](const std::string &key, int64_t ncomp, std::vector<int64_t> &bdims) {
+        const Field &f = io->raw(key);
+        if (f.is_string) throw IOError{"IO error : key " + key + " is a string, not a numerical field."};
+        std::vector<int64_t> gd = f.dims;
+        if (ncomp > 1) {
+            if (gd.empty() || gd.back() != ncomp)
+                throw IOError{"BaseIO input error first dimension " + std::to_string(gd.empty() ? 0 : gd.back()) +
+                              " of field " + key + " does not match expected value " + std::to_string(ncomp) + "."};
+            gd.pop_back();
+        }
+        if ((int)gd.size() != dim) throw IOError{"Field " + key + " has incorrect depth.\n"};
+        return reorder(f.data, gd, ncomp, io->ordering, true, bdims);
+    };
+    FactorHost F;
+    std::vector<int64_t> vd, gdims, md;
+    F.vals = box_array("factoringValues", 1, vd);
+    if (!io->has("factoringGradients")) throw IOError{"StaticFactoring error : factoringGradients are not provided"};
+    F.grads = box_array("factoringGradients", dim, gdims);
+    if (io->has("factoringMask")) {
+        const std::vector<double> m = box_array("factoringMask", 1, md);
+        F.mask.resize(m.size());
+        for (size_t i = 0; i < m.size(); ++i) F.mask[i] = m[i] != 0. ? 1 : 0;
+    } else {
+        md = vd;
+    }
+    F.sub = io->has("factoringIndexShift");
+    F.shift.assign(dim, 0);
+    if (F.sub) {
+        const std::vector<double> sh = io->vec("factoringIndexShift");
+        if ((int)sh.size() != dim) throw IOError{"IO error : field factoringIndexShift has the wrong size"};
+        for (int i = 0; i < dim; ++i) F.shift[i] = (int64_t)sh[i];
+    }
+    const std::vector<int64_t> &want = F.sub ? vd : dims;
+    if (vd != want || gdims != want || md != want)
+        throw IOError{"Static factoring error : inconsistent data dimension."};
+    for (size_t k = 0; k < F.grads.size(); ++k) F.grads[k] *= scales[k % dim];
+    F.bd = vd;
+    F.on = !F.vals.empty();
+    return F;
 }
 
 // seedValueVariation is given for the `seeds` as passed; the solver may run on more seeds
@@ -687,7 +895,7 @@ void run_impl(hfm_io *io) {
             bool ok = true;
             for (int i = 0; i < dim; ++i) {
                 const bool th = bundle && i == 2;
-                const double step = th ? 2 * 3.141592653589793238462643383279502884L / dims[2] : h[i];
+                const double step = th ? theta_step(dims[2]) : h[i];
                 const double o = th ? -step / 2. : (origin.empty() ? 0. : origin[i]);
                 const double a = std::floor((pts[q + i] - o) / step);
                 if (!(a > -2e9 && a < 2e9)) { ok = false; break; }
@@ -761,7 +969,7 @@ void run_impl(hfm_io *io) {
     }
     // pointToIndex -> indexFromPoint (IndexFromPoint(ADim(p)), unperiodized) and indexToPoint ->
     // pointFromIndex (ReDim(index + 1/2), the index cast by truncation) (HFMInterface.hxx:377-396)
-    auto axis_scale = [&](int i) { return bundle && i == 2 ? 2 * 3.141592653589793238462643383279502884L / dims[2] : h[i]; };
+    auto axis_scale = [&](int i) { return bundle && i == 2 ? theta_step(dims[2]) : h[i]; };
     auto axis_origin = [&](int i) {
         return bundle && i == 2 ? -(double)axis_scale(2) / 2. : (origin.empty() ? 0. : origin[i]);
     };
@@ -800,11 +1008,20 @@ void run_impl(hfm_io *io) {
             throw IOError{"Error : order " + std::to_string(order) + " with walls is not supported by this library"};
         check(hfm_plan_set_order(plan, order));
     }
-    // Static factoring and the early-stopping criteria change the values through the serial
-    // heap's acceptance order (StaticFactoring.hxx, ExtraAlgorithms/CommonStoppingCriteria.h); refused rather
-    // than ignored.
-    if ((io->has("factoringRadius") && io->scalar("factoringRadius") != 0.) || io->has("factoringValues"))
-        throw IOError{"Error : static factoring is not supported by this library"};
+    // Static factoring (StaticFactoring.hxx:58-113): an imported factor now, a computed one
+    // (factoringRadius != 0) once the seeds are known; both at once is an error.  A factored solve
+    // runs the serial acceptance loop.
+    FactorHost factor;
+    const bool facRadiusKey = io->has("factoringRadius");
+    const bool facCompute = facRadiusKey && io->scalar("factoringRadius", 10.) != 0.;
+    if (io->has("factoringValues") && facCompute)
+        throw IOError{"StaticFactoring error : factoringValues (import) and factoringRadius (compute) both "
+                      "specified."};
+    if (io->has("factoringValues")) {
+        double scales[3] = {h[0], h[1], h[2]};
+        if (bundle) scales[2] = theta_step(dims[2]);
+        factor = import_factor(io, dims, scales, dim);
+    }
     // Stopping criteria (ExtraAlgorithms/CommonStoppingCriteria.h:115-165): the targets' indices are
     // IndexFromPoint(ADim(p)), periodized, an out-of-range one dropped with a warning; an
     // unoriented target is its (x, y) at any theta.  They run the serial acceptance loop.
@@ -866,7 +1083,7 @@ void run_impl(hfm_io *io) {
     // Seeds (StaticFactoring::SetSeeds, StaticFactoring.hxx:242-393): `seeds` / `seedValues`,
     // optionally spread over the grid points within `seedRadius` (Riemann models), then for the
     // curvature models `seeds_Unoriented` / `seedValues_Unoriented` at every theta.
-    const double thetaStep = bundle ? 2 * 3.141592653589793238462643383279502884L / dims[2] : 0.;
+    const double thetaStep = bundle ? theta_step(dims[2]) : 0.;
     auto redim = [&](double q, int i) {
         const double o = bundle && i == 2 ? -thetaStep / 2. : (origin.empty() ? 0. : origin[i]);
         return q * (bundle && i == 2 ? thetaStep : h[i]) + o;
@@ -878,6 +1095,8 @@ void run_impl(hfm_io *io) {
     // Run_RunSolver (HFMInterface.hxx:422-450): given both values and activeNeighs, the solver is
     // bypassed and everything downstream (geodesics, exports, differentiation) uses them.
     const bool bypass = io->has("values") && io->has("activeNeighs");
+    if (bypass && (factor.on || facCompute))
+        throw IOError{"Error : static factoring with cached values and activeNeighs is not supported by this library"};
     if (bypass) {
         bool vf = false, af = false;
         const std::vector<double> vals = grid_field(io, "values", dims, 1, false, vf);
@@ -902,7 +1121,13 @@ void run_impl(hfm_io *io) {
             } else {
                 sv.assign(nseeds_in, 0.);
             }
-            const double seedRadius = io->scalar("seedRadius", 0.);
+            // SetSeeds (StaticFactoring.hxx:247-262,285): a computed factor needs exactly one seed;
+            // seeds spread over seedRadius = 2 by default once factoring is on
+            if (facRadiusKey && nseeds_in != 1)
+                throw IOError{"Exactly one seed point must be specified for computing the factor, found " +
+                              std::to_string(nseeds_in) + "."};
+            const bool facDisabled = !factor.on && !facRadiusKey;
+            const double seedRadius = io->scalar("seedRadius", facDisabled ? 0. : 2.);
             if (seedRadius != 0.) {
                 if (!riemann) throw IOError{"Equation factoring error : no guess"};
                 // negative: also one step of the reversed stencil (StaticFactoring.hxx:285-287)
@@ -918,6 +1143,8 @@ void run_impl(hfm_io *io) {
             const std::vector<double> up = io->vec("seeds_Unoriented");
             if (up.size() % 2) throw IOError{"IO error : seeds_Unoriented has the wrong size"};
             const size_t nu = up.size() / 2;
+            if (facRadiusKey && nu > 0)
+                throw IOError{"Factor computation incompatible with unoriented seed points"};
             std::vector<double> uv(nu, 0.);
             if (io->has("seedValues_Unoriented")) uv = io->vec("seedValues_Unoriented");
             if (uv.size() != nu) throw IOError{"Error : Inconsistent size of seedValues_Unoriented."};
@@ -935,6 +1162,42 @@ void run_impl(hfm_io *io) {
             return;
         }
         nseeds = (int)(seeds.size() / dim);
+        // ComputeFactor (StaticFactoring.hxx:116-239) from the seed as given, then the factor on the plan
+        if (facCompute) {
+            if (!riemann) throw IOError{"Equation factoring error : no guess"};
+            const int64_t R = (int64_t)io->scalar("factoringRadius", 10.);
+            const std::string choice = io->str("factoringPointChoice", order < 3 ? "Seed" : "Both");
+            const bool both = choice == "Both";
+            if (!both && choice != "Seed" && choice != "Key")
+                throw IOError{"StaticFactoring error : Unsupported factoringPointChoice " + choice + "."};
+            if (both)
+                io->log += "***** Warning ! *****\nfactoringPointChoice = 'Both' yields a factor that is best computed "
+                           "in Python, via dictIn.SetFactor, which uses higher order interpolation methods."
+                           "********************\n";
+            double p0[3];
+            for (int i = 0; i < dim; ++i) p0[i] = (seeds_given[i] - (origin.empty() ? 0. : origin[i])) / h[i];
+            factor = dim == 2 ? compute_factor_d<2>(dims, h, field.p, is_field, hasDual, p0, R, both)
+                              : compute_factor_d<3>(dims, h, field.p, is_field, hasDual, p0, R, both);
+            if (io->scalar("exportFactoring", 0.) != 0. && factor.on) {
+                set_grid_array(io, "factoringValues", factor.vals, factor.bd, 1);
+                set_grid_array(io, "factoringGradients", factor.grads, factor.bd, dim);
+                if (factor.sub) {
+                    Field f;
+                    f.dims = {(int64_t)dim};
+                    for (int i = 0; i < dim; ++i) f.data.push_back((double)factor.shift[i]);
+                    io->set_computed("factoringIndexShift", std::move(f));
+                }
+            }
+        }
+        if (factor.on) {
+            if (precision != 64)
+                throw IOError{"Error : static factoring needs precision 64 (the serial acceptance loop reproduces "
+                              "the reference's FP64 values)"};
+            if (!walls.empty()) throw IOError{"Error : static factoring with walls is not supported by this library"};
+            check(hfm_plan_set_factoring(plan, factor.bd.data(), factor.sub ? factor.shift.data() : nullptr,
+                                         factor.vals.data(), factor.grads.data(),
+                                         factor.mask.empty() ? nullptr : factor.mask.data()));
+        }
         // Run_RunSolver
         const double t2 = wall();
         check(hfm_plan_solve(plan, nseeds, seeds.data(), sv.data(), nullptr));
@@ -952,7 +1215,7 @@ void run_impl(hfm_io *io) {
         // nAccepted (ExtraAlgorithms/CommonStoppingCriteria.h:162-165): the serial loop's count when it ran (the
         // pop that triggered a stop included); otherwise every point the heap accepts is a reached
         // one, i.e. a finite value
-        if (order > 1 || stopping) {
+        if (order > 1 || stopping || factor.on) {
             io->set_scalar("nAccepted", (double)st.n_accepted);
         } else {
             int64_t reached = 0;
@@ -1119,8 +1382,8 @@ void run_impl(hfm_io *io) {
                 for (int q = 0; q < dim; ++q) {
                     const double pt = idx[q] + 0.5;
                     double o;
-                    if (bundle && q == 2) o = pt * (2 * 3.141592653589793238462643383279502884L / dims[2]) +
-                                              (-(2 * 3.141592653589793238462643383279502884L / dims[2]) / 2.);
+                    if (bundle && q == 2) o = pt * (theta_step(dims[2])) +
+                                              (-(theta_step(dims[2])) / 2.);
                     else o = pt * h[q] + (origin.empty() ? 0. : origin[q]);
                     f.data.push_back(o);
                 }

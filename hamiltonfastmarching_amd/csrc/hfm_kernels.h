@@ -97,6 +97,9 @@ struct ExactArgs {
     unsigned char *stop_flags;
     unsigned char *stop_flags_u;
     long long all_left, all_left_u;
+    // Static factoring (StaticFactoring.hxx:11-55): a point in the factor's mask is recomputed at
+    // acceptance at every order, its differences corrected.
+    FactorDev fac;
 };
 hipError_t launch_exact_init(double *u, unsigned char *acc, unsigned long long *active, double *quad, long long np,
                              hipStream_t s);
@@ -105,8 +108,9 @@ hipError_t launch_rev_build(int model, const Geom &g, const StencilData<double> 
 hipError_t launch_rev_sort(const int *cnt, const int *start, uint32_t *roffs, int *rcount, long long np, hipStream_t s);
 hipError_t launch_fm_exact(int model, const ExactArgs &a, hipStream_t s);
 hipError_t launch_flow_exact(int model, const Geom &g, const StencilData<double> &sd, const double *u,
-                             const unsigned char *acc, const unsigned long long *active, int order, const FlowScale &fs,
-                             double *flow, double *act_out, double *aoffs, long long np, hipStream_t s);
+                             const unsigned char *acc, const unsigned long long *active, int order,
+                             const FactorDev &fac, const FlowScale &fs, double *flow, double *act_out, double *aoffs,
+                             long long np, hipStream_t s);
 
 // Geodesic backtracking (hfm_geodesic.hip).  Tips are ADim'd (grid-index units); points are
 // written ReDim'd, max_points per tip, tip t at points[t * max_points * dim ...].

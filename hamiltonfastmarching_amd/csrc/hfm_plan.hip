@@ -106,6 +106,9 @@ void plan_free(hfm_plan *p) {
     dfree(p->ex_out);
     dfree(p->ex_flags);
     dfree(p->ex_flags_u);
+    dfree(p->fac_vals);
+    dfree(p->fac_grads);
+    dfree(p->fac_mask);
     dfree(p->dep_nsrc);
     dfree(p->scan_work);
     dfree(p->keys);
@@ -372,7 +375,12 @@ void shared_reversed_offsets(hfm_plan *p, std::vector<int> &start, std::vector<i
 // The serial acceptance loop (hfm_exact.hip) for order 2 / 3: reversed offsets (rebuilt after a
 // change of stencils), the solver state, then one launch of k_fm_exact.
 void run_exact(hfm_plan *p, hipStream_t s) {
-    const std::string what = p->order > 1 ? "order " + std::to_string(p->order) : std::string("a stopping criterion");
+    const std::string what = p->order > 1  ? "order " + std::to_string(p->order)
+                             : p->fac.on ? std::string("static factoring")
+                                         : std::string("a stopping criterion");
+    p->exact_what = p->order > 1 ? "an order " + std::to_string(p->order) + " solve"
+                    : p->fac.on  ? std::string("a factored solve")
+                                 : std::string("a solve with stopping criteria");
     if (p->precision != 64)
         throw HipFail{what + " needs an FP64 plan (precision 64): the serial acceptance loop reproduces the "
                       "reference's FP64 values"};
@@ -475,6 +483,7 @@ void run_exact(hfm_plan *p, hipStream_t s) {
     a.stop_flags_u = nullptr;
     a.all_left = 0;
     a.all_left_u = 0;
+    a.fac = p->fac;
     std::vector<unsigned char> fl, flu;
     if (!p->stop_any.empty() || !p->stop_all.empty()) {
         fl.assign(np, 0);
@@ -1020,7 +1029,7 @@ int hfm_plan_solve(hfm_plan *p, int nseeds, const double *seeds, const double *s
         if (p->walls && p->vis_dirty) build_visibility(p, s);
         p->exact_solved = false;
         p->st.n_accepted = -1;
-        if (p->order > 1 || p->has_stopping) {
+        if (p->order > 1 || p->has_stopping || p->fac.on) {
             run_exact(p, s);
             p->exact_solved = true;
         } else if (p->precision == 32) {
@@ -1099,8 +1108,8 @@ static int plan_flow(const char *who, hfm_plan *p, double *d_flow, double *d_act
     FlowScale fs{{p->h[0], p->h[1], p->h[2]}};
     if (p->exact_solved) {  // the flags of the serial loop and its order-aware recompute
         const hipError_t e = launch_flow_exact(p->model, p->g, p->sd<double>(), static_cast<const double *>(p->u),
-                                               p->ex_acc, p->ex_active, p->order, fs, d_flow, d_active, d_aoffs, p->np,
-                                               s);
+                                               p->ex_acc, p->ex_active, p->order, p->fac, fs, d_flow, d_active, d_aoffs,
+                                               p->np, s);
         if (e != hipSuccess) {
             set_error(std::string(who) + ": " + hipGetErrorString(e));
             return 1;
@@ -1143,8 +1152,8 @@ int hfm_plan_residual(hfm_plan *p, double *max_rel, int64_t *n_mismatch, void *s
         return 1;
     }
     if (p->exact_solved) {
-        set_error("hfm_plan_residual: the residual check is the first-order scheme's; an order " +
-                  std::to_string(p->order) + " solve does not satisfy it");
+        set_error("hfm_plan_residual: the residual check is the first-order scheme's; " + p->exact_what +
+                  " does not satisfy it");
         return 1;
     }
     try {
@@ -1214,8 +1223,8 @@ int hfm_plan_geodesics_impl(hfm_plan *p, int ntips, const double *tips, bool tip
         return 1;
     }
     if (p->exact_solved) {
-        set_error("hfm_plan_geodesics: geodesics of an order " + std::to_string(p->order) +
-                  " solve are not supported by this library (order 1 only)");
+        set_error("hfm_plan_geodesics: geodesics of " + p->exact_what +
+                  " (the serial acceptance loop) are not supported by this library");
         return 1;
     }
     *npoints = 0;
@@ -1488,6 +1497,75 @@ int hfm_plan_set_stopping(hfm_plan *p, int64_t max_accepted, double stop_distanc
     p->stop_dist = stop_distance;
     p->has_stopping = max_accepted >= 0 || stop_distance < __builtin_huge_val() || n_any > 0 || n_all > 0 ||
                       n_any_u > 0 || n_all_u > 0;
+    return 0;
+}
+
+int hfm_plan_set_factoring(hfm_plan *p, const int64_t *box_dims, const int64_t *shift, const double *values,
+                           const double *gradients, const uint8_t *mask) {
+    if (!p) {
+        set_error("hfm_plan_set_factoring: null argument");
+        return 1;
+    }
+    try {
+        if (!values) {  // remove the factor
+            dfree(p->fac_vals);
+            dfree(p->fac_grads);
+            dfree(p->fac_mask);
+            p->fac_vals = p->fac_grads = nullptr;
+            p->fac_mask = nullptr;
+            p->fac = FactorDev();
+            return 0;
+        }
+        if (!box_dims || !gradients) {
+            set_error("hfm_plan_set_factoring: box_dims and gradients are required with values");
+            return 1;
+        }
+        const int dim = p->g.dim;
+        FactorDev f;
+        long long n = 1;
+        for (int i = 0; i < dim; ++i) {
+            if (box_dims[i] <= 0 || box_dims[i] > (1 << 24)) {
+                set_error("hfm_plan_set_factoring: box extents must be positive");
+                return 1;
+            }
+            if (!shift && box_dims[i] != p->g.n[i]) {
+                set_error("hfm_plan_set_factoring: without a shift the factor covers the whole grid "
+                          "(Static factoring error : inconsistent data dimension.)");
+                return 1;
+            }
+            if (shift && (shift[i] < -(1LL << 30) || shift[i] > (1LL << 30))) {
+                set_error("hfm_plan_set_factoring: shift out of range");
+                return 1;
+            }
+            f.bd[i] = (int)box_dims[i];
+            f.shift[i] = shift ? (int)shift[i] : 0;
+            n *= box_dims[i];
+        }
+        f.sub = shift ? 1 : 0;
+        f.on = 1;
+        std::vector<unsigned char> m(n, 1);
+        if (mask)
+            for (long long i = 0; i < n; ++i) m[i] = mask[i] != 0;
+        dfree(p->fac_vals);
+        dfree(p->fac_grads);
+        dfree(p->fac_mask);
+        p->fac_vals = p->fac_grads = nullptr;
+        p->fac_mask = nullptr;
+        p->fac = FactorDev();
+        p->fac_vals = dalloc<double>(n);
+        p->fac_grads = dalloc<double>(n * dim);
+        p->fac_mask = dalloc<unsigned char>(n);
+        HIPCHECK(hipMemcpy(p->fac_vals, values, n * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(p->fac_grads, gradients, n * dim * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(p->fac_mask, m.data(), n, hipMemcpyHostToDevice));
+        f.vals = p->fac_vals;
+        f.grads = p->fac_grads;
+        f.mask = p->fac_mask;
+        p->fac = f;
+    } catch (const HipFail &e) {
+        set_error(e.msg);
+        return 1;
+    }
     return 0;
 }
 

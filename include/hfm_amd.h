@@ -303,6 +303,20 @@ int hfm_plan_set_stopping(hfm_plan *plan, int64_t max_accepted, double stop_dist
                           const int64_t *any_idx, int n_all, const int64_t *all_idx, int n_any_u,
                           const int64_t *any_u_idx, int n_all_u, const int64_t *all_u_idx);
 
+/* Static factoring (replaces the imported factor factoringValues / factoringGradients /
+ * factoringMask / factoringIndexShift, StaticFactoring.hxx:77-113, and holds the computed one of
+ * factoringRadius, :116-239, which hfm_run builds on the host): the factor T over a box of
+ * box_dims grid points whose first grid index is shift[] (shift NULL: the whole grid, box_dims
+ * = the grid's extents).  Host arrays, row-major over the box: values (T), gradients (grad T in
+ * grid-index units, dim per point: a physical gradient times the grid scale of each axis) and
+ * mask (NULL: everywhere; nonzero = the factor applies).  They are copied.  A point where the
+ * factor applies is recomputed at acceptance, each difference corrected by
+ * grad T . e + (one-sided difference of T along e of the order used, with the opposite sign);
+ * the solve runs the serial acceptance loop (see hfm_plan_set_order): FP64 plans without walls.
+ * values NULL removes the factor. */
+int hfm_plan_set_factoring(hfm_plan *plan, const int64_t *box_dims, const int64_t *shift, const double *values,
+                           const double *gradients, const uint8_t *mask);
+
 /* ----------------------------------------------------------------------------------------- */
 /* Continuous-batching solve server (no reference counterpart: the reference solves one      */
 /* problem per Run(IO&) call; this serves many Riemann2 / Riemann3 solves concurrently)      */

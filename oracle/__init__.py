@@ -69,6 +69,8 @@ def lib():
                                           P(ctypes.c_long), i32, P(ctypes.c_long)]
         L.oracle_set_stopping.restype = None
         L.oracle_last_naccepted.restype = ctypes.c_long
+        L.oracle_set_factoring.argtypes = [i32, i32, P(i32), P(i32), P(d), P(d), P(ctypes.c_ubyte)]
+        L.oracle_set_factoring.restype = None
         L.oracle_geodesics_given.argtypes = [i32, P(i32), P(d), P(d), P(d), i32, i32, d, d, P(d), i32, P(d),
                                              P(ctypes.c_uint64), i32, P(d), d, d, d, P(d), ctypes.c_long, P(i64),
                                              P(ctypes.c_long)]
@@ -161,6 +163,7 @@ def _common_args(params):
     _set_walls(params, dims, model)
     lib().oracle_set_order(int(float(params.get("order", 1))))
     _set_stopping(params, dims, model, h, origin)
+    seeds, sv = _set_factoring(params, model, dims, h, origin, dual, seeds, sv)
     return model, dims, dim, N, h, origin, metric, metric_is_field, dual, cost, cost_is_field, seeds, sv, xi, eps
 
 
@@ -202,6 +205,37 @@ def _set_stopping(params, dims, model, h, origin):
     for a in lists:
         args += [len(a), a.ctypes.data_as(ctypes.POINTER(ctypes.c_long))]
     lib().oracle_set_stopping(nmax, dist, *args)
+
+
+_last_exports: dict = {}
+
+
+def _set_factoring(params, model, dims, h, origin, dual, seeds, sv):
+    """Seed spreading and the static factor (oracle/factoring.py) for the next solve: returns the
+    seeds and values the solver starts from."""
+    from . import factoring as fac
+    global _last_exports
+    _last_exports = {}
+    keys = ("factoringRadius", "factoringValues", "seedRadius")
+    f = None
+    if any(k in params for k in keys):
+        raw = params.get("dualMetric", params.get("metric")) if model in ("Riemann2", "Riemann3") else None
+        seeds, sv, f, _last_exports = fac.setup(params, model, dims, h, origin, raw, dual, seeds, sv)
+        seeds = np.ascontiguousarray(seeds, np.float64)
+        sv = np.ascontiguousarray(sv, np.float64)
+    if f is None:
+        lib().oracle_set_factoring(len(dims), 0, None, None, None, None, None)
+    else:
+        sub, bd, shift, vals, grads, mask = f
+        bda = np.array(bd, np.int32)
+        sha = np.array(shift, np.int32)
+        vals = np.ascontiguousarray(vals, np.float64)
+        grads = np.ascontiguousarray(grads, np.float64)
+        mask = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+        lib().oracle_set_factoring(len(dims), int(sub), _ptr(bda, ctypes.c_int), _ptr(sha, ctypes.c_int),
+                                   _ptr(vals, ctypes.c_double), _ptr(grads, ctypes.c_double),
+                                   _ptr(mask, ctypes.c_ubyte))
+    return seeds, sv
 
 
 def last_naccepted() -> int:
@@ -264,7 +298,7 @@ def solve(params, flow=False, active=False):
     if r:
         raise RuntimeError(_err())
     out = {"values": values.reshape(dims), "StencilCPUTime": tm[0], "FMCPUTime": tm[1],
-           "nAccepted": last_naccepted()}
+           "nAccepted": last_naccepted(), **_last_exports}
     if active:
         out["activeNeighs"] = act.reshape(dims)
     if flow:

@@ -551,6 +551,69 @@ double g_stop_dist = std::numeric_limits<double>::infinity();
 std::set<long> g_stop_any, g_stop_all, g_stop_any_u, g_stop_all_u;
 long g_naccepted = 0;
 
+// Static factoring of the problems solved next on this thread (StaticFactoring.h:20-63,
+// StaticFactoring.hxx:11-55): factor values, gradients (index units, D per point) and mask over a
+// box of the grid whose first grid index is `shift` (sub), or over the whole grid.
+struct Factoring {
+    bool on = false, sub = false;
+    int bd[3] = {1, 1, 1}, shift[3] = {0, 0, 0};
+    std::vector<double> vals, grads;
+    std::vector<unsigned char> mask;
+    long lin(const int *i, int dim) const {
+        long r = i[0];
+        for (int k = 1; k < dim; ++k) r = r * bd[k] + i[k];
+        return r;
+    }
+    bool in_box(const int *i, int dim) const {
+        for (int k = 0; k < dim; ++k)
+            if (i[k] < 0 || i[k] >= bd[k]) return false;
+        return true;
+    }
+};
+Factoring g_fact;
+
+// SetIndex / NeedsRecompute (StaticFactoring.hxx:11-30): x's box index into cur, and whether the
+// factor applies at x (the recompute at acceptance is then needed at every order: these schemes
+// do not factor their first pass, HamiltonFastMarching.h:47-50).
+bool factor_at(const Problem &P, long x, int *cur) {
+    if (!g_fact.on) return false;
+    P.unravel(x, cur);
+    for (int i = 0; i < P.dim; ++i) cur[i] -= g_fact.shift[i];
+    if (g_fact.sub && !g_fact.in_box(cur, P.dim)) return false;
+    return g_fact.mask[g_fact.lin(cur, P.dim)] != 0;
+}
+
+// Correction (StaticFactoring.hxx:33-55) of the difference along the signed offset `off` at the
+// box index cur, for the order `ord` the difference uses: the factor's directional derivative
+// plus the factor's own finite difference with the opposite sign.  Box indices are periodized on
+// the whole grid only; on a box, a difference leaving it gets no correction.
+double factor_correction(const Problem &P, const int *cur, const int8_t *off, int ord) {
+    const Factoring &F = g_fact;
+    const int D = P.dim;
+    int last[3] = {0, 0, 0}, i1[3] = {0, 0, 0}, i2[3] = {0, 0, 0};
+    for (int i = 0; i < D; ++i) {
+        last[i] = cur[i] + ord * off[i];
+        i1[i] = cur[i] + off[i];
+        i2[i] = cur[i] + 2 * off[i];
+    }
+    if (F.sub) {
+        if (!F.in_box(last, D)) return 0.;
+    } else {
+        P.periodize(last);
+        P.periodize(i1);
+        P.periodize(i2);
+    }
+    const long c = F.lin(cur, D);
+    double deriv = 0.;
+    for (int i = 0; i < D; ++i) deriv += F.grads[c * D + i] * double(off[i]);
+    const double v0 = F.vals[c], vl = F.vals[F.lin(last, D)];
+    if (ord == 1) return deriv + (v0 - vl);
+    const double v1 = F.vals[F.lin(i1, D)];
+    if (ord == 2) return deriv + (1.5 * v0 - 2 * v1 + 0.5 * vl);
+    const double v2 = F.vals[F.lin(i2, D)];
+    return deriv + ((11. / 6.) * v0 - 3. * v1 + 1.5 * v2 - (1. / 3.) * vl);
+}
+
 // VisibleOffset (HamiltonFastMarching.hxx:160-170): base + v periodized, -1 when it leaves a
 // closed axis or a wall hides it.
 long visible_shift(const Problem &P, const std::vector<int8_t> &wdist, long base, const int8_t *v) {
@@ -576,26 +639,29 @@ int recompute_entries(const Problem &P, const FMState &S, long x, int order, Rec
     const Stencil &st = P.stencil(x);
     const double mlt = P.mult(x);
     const uint64_t act = S.active[x];
+    int cur[3] = {0, 0, 0};
+    const bool fact = factor_at(P, x, cur);
+    auto corr = [&](const int8_t *off, int ord) { return fact ? factor_correction(P, cur, off, ord) : 0.; };
     int n = 0;
     auto push = [&](const int8_t *off, int s, double w) {
         RecEntry &r = out[n++];
         for (int c = 0; c < 3; ++c) r.off[c] = (int8_t)(s * off[c]);
         const long a1 = P.shift(x, r.off, 1);  // active: accepted and in the domain
         const double v1 = S.values[a1];
-        double val = v1 + 0.;
+        double val = v1 + corr(r.off, 1);
         int used = 1;
         if (std::min(order, 3) >= 2) {
             const long a2 = visible_shift(P, S.wdist, a1, r.off);
             if (a2 >= 0 && S.accepted[a2] && !(S.values[a2] > v1)) {
                 const double v2 = S.values[a2];
                 used = 2;
-                val = (2. / 3.) * ((2. * v1 - 0.5 * v2) + 0.);
+                val = (2. / 3.) * ((2. * v1 - 0.5 * v2) + corr(r.off, 2));
                 if (order >= 3) {
                     const long a3 = visible_shift(P, S.wdist, a2, r.off);
                     if (a3 >= 0 && S.accepted[a3] && !(S.values[a3] > v2)) {
                         const double v3 = S.values[a3];
                         used = 3;
-                        val = (6. / 11.) * ((3. * v1 - 1.5 * v2 + (1. / 3.) * v3) + 0.);
+                        val = (6. / 11.) * ((3. * v1 - 1.5 * v2 + (1. / 3.) * v3) + corr(r.off, 3));
                     }
                 }
             }
@@ -670,7 +736,8 @@ void fast_marching(const Problem &P, const std::vector<std::pair<long, double>> 
             if (allU.empty()) stop = true;
         }
         if (!stop && g_stop_dist < kInf && S.values[y] > g_stop_dist) stop = true;
-        if (g_order > 1 && S.active[y]) {
+        int cur[3];
+        if ((g_order > 1 || factor_at(P, y, cur)) && S.active[y]) {
             RecEntry re[2 * kMaxDiff];
             (void)recompute_entries(P, S, y, g_order, re, &top.value);
             S.values[y] = top.value;
@@ -1061,6 +1128,27 @@ void oracle_set_stopping(long max_accepted, double stop_distance, int nany, cons
     g_stop_all_u = std::set<long>(allu, allu + nallu);
 }
 long oracle_last_naccepted(void) { return g_naccepted; }
+
+// Static factor of the problems solved next on this thread (vals == NULL: none): box extents bd
+// (the grid's when sub = 0), the box's first grid index shift, values and gradients (index units)
+// row-major over the box, mask (NULL: everywhere).
+void oracle_set_factoring(int dim, int sub, const int *bd, const int *shift, const double *vals,
+                          const double *grads, const unsigned char *mask) {
+    g_fact = Factoring();
+    if (!vals) return;
+    g_fact.on = true;
+    g_fact.sub = sub != 0;
+    long n = 1;
+    for (int i = 0; i < dim; ++i) {
+        g_fact.bd[i] = bd[i];
+        g_fact.shift[i] = sub ? shift[i] : 0;
+        n *= bd[i];
+    }
+    g_fact.vals.assign(vals, vals + n);
+    g_fact.grads.assign(grads, grads + n * dim);
+    if (mask) g_fact.mask.assign(mask, mask + n);
+    else g_fact.mask.assign(n, 1);
+}
 
 // Walls of the problems set up next on this thread: n bytes (nonzero = wall), n = 0 for none.
 void oracle_set_walls(long n, const unsigned char *walls, int refine) {

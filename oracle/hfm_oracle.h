@@ -49,6 +49,11 @@ long oracle_last_dips(void);
 void oracle_set_stopping(long max_accepted, double stop_distance, int nany, const long *any, int nall,
                          const long *all, int nanyu, const long *anyu, int nallu, const long *allu);
 long oracle_last_naccepted(void);
+/* Static factor (StaticFactoring.hxx:11-55) of the problems solved next on this thread, vals =
+ * NULL for none: box extents bd (the grid's when sub = 0), the box's first grid index shift,
+ * values and gradients (index units) row-major over the box, mask (NULL: everywhere). */
+void oracle_set_factoring(int dim, int sub, const int *bd, const int *shift, const double *vals,
+                          const double *grads, const unsigned char *mask);
 
 /* Walls (ExtraAlgorithms/WallObstructionTest.h) of the problems set up next on this thread:
  * n bytes in row-major point order, nonzero = wall; n = 0 for none.  refine bit 0 =

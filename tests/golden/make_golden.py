@@ -19,6 +19,8 @@ Fixtures (all row-major):
   orderings.npz      outputs under every arrayOrdering of non-square Riemann2 / Riemann3 / RSF2 problems
   high_order.npz     values, activeNeighs, geodesicFlow, activeOffsets, nAccepted at order 2 and 3
   stopping.npz       values, activeNeighs, nAccepted under each stopping criterion
+  factoring.npz      values, activeNeighs, geodesicFlow, nAccepted and the exported factor with static
+                     factoring (computed or imported)
   riemann3_128.npz   digests of values and Selling offsets at the benchmark size + 4096 seeded samples
 Inputs are regenerated by hamiltonfastmarching_amd.synthetic; the SHA-256 of each generated
 input is stored so that a test can tell a generator change from a numerical one.
@@ -573,6 +575,78 @@ def stopping_cases():
     print("stopping ok")
 
 
+def _factor_import_r2():
+    """An imported factor for the 40x40 Riemann2 problem: the first-order distance guess from
+    (0.43, 0.61) over an 11x13 box from index (12, 18), physical gradients, a mask with holes."""
+    p = syn.riemann2(40, seed=31)
+    h = 1.0 / 40
+    x0 = np.array([0.43, 0.61])
+    m0 = p["metric"][17, 24]
+    M = np.array([[m0[0], m0[1]], [m0[1], m0[2]]])
+    bi, bj = np.meshgrid(np.arange(11) + 12, np.arange(13) + 18, indexing="ij")
+    X = np.stack([(bi + 0.5) * h, (bj + 0.5) * h], axis=-1) - x0
+    MX = X @ M
+    nrm = np.sqrt(np.einsum("...i,...i->...", X, MX))
+    grad = MX / np.maximum(nrm, 1e-300)[..., None]
+    mask = np.ones((11, 13))
+    mask[3, 4] = mask[7, 2] = mask[0, 12] = 0.0
+    return dict(p, seeds=np.array([x0]), factoringValues=nrm, factoringGradients=grad, factoringMask=mask,
+                factoringIndexShift=np.array([12.0, 18.0]))
+
+
+def _factor_import_rsf2():
+    """An imported factor over the whole 20x20x12 RSF2 grid (no index shift, so differences wrap
+    around the angle axis), seeds not spread."""
+    p = {k: v for k, v in syn.rsf2(20, 12, seed=33).items() if k != "tips"}
+    i, j, k = np.meshgrid(np.arange(20), np.arange(20), np.arange(12), indexing="ij")
+    th = (k * 2 * np.pi / 12)
+    vals = 0.05 * np.sin(2 * np.pi * (i + 0.5) / 20) * np.cos(th) + 0.02 * np.cos(2 * np.pi * (j + 0.5) / 20)
+    grads = np.stack([0.1 * np.pi * np.cos(2 * np.pi * (i + 0.5) / 20) * np.cos(th),
+                      -0.04 * np.pi * np.sin(2 * np.pi * (j + 0.5) / 20),
+                      -0.05 * np.sin(2 * np.pi * (i + 0.5) / 20) * np.sin(th)], axis=-1)
+    return dict(p, seeds=p["seeds"][:1], factoringValues=vals, factoringGradients=grads, seedRadius=0.0)
+
+
+def factoring_params():
+    """Static factoring (StaticFactoring.hxx:58-239): computed over a box or the whole grid, the
+    "Seed" and "Both" choices, orders 1-3, primal and dual metrics; imported factors with a mask,
+    an index shift, and on a periodic angle axis."""
+    r2 = syn.riemann2(40, seed=31)
+    r2dual = dict(r2)
+    r2dual["dualMetric"] = _pack_sym(np.linalg.inv(_unpack_sym(r2dual.pop("metric"))))
+    r3 = syn.riemann3(14, seed=32)
+    off = np.array([[0.43, 0.61]])
+    return {
+        "r2_rad10_o1": dict(r2, factoringRadius=10.0),
+        "r2_rad8_o2_off": dict(r2, order=2.0, factoringRadius=8.0, seeds=off, seedRadius=0.0),
+        "r2_rad6_o3_both": dict(r2, order=3.0, factoringRadius=6.0, seeds=off),
+        "r2_all_o1_dual": dict(r2dual, factoringRadius=-1.0, seeds=off, seedRadius=1.5),
+        "r2_rad7_o3_seed": dict(r2, order=3.0, factoringRadius=7.0, factoringPointChoice="Seed"),
+        "r2_all_o2_both": dict(r2, order=2.0, factoringRadius=-1.0, factoringPointChoice="Both", seeds=off),
+        "r3_rad5_o1": dict(r3, factoringRadius=5.0),
+        "r3_all_o2": dict(r3, order=2.0, factoringRadius=-1.0, seeds=np.array([[0.47, 0.52, 0.55]])),
+        "r2_import_o1": _factor_import_r2(),
+        "r2_import_o2": dict(_factor_import_r2(), order=2.0),
+        "rsf2_import_o1": _factor_import_rsf2(),
+    }
+
+
+def factoring_cases():
+    """values / activeNeighs / geodesicFlow / nAccepted, and the exported factor where computed."""
+    out = {}
+    for name, p in factoring_params().items():
+        q = dict(p, exportActiveNeighs=1.0, exportGeodesicFlow=1.0, exportFactoring=1.0)
+        r = oracle.run_reference(q)
+        assert r["retcode"] == 0, r["log"]
+        for k in ("values", "activeNeighs", "geodesicFlow", "nAccepted", "factoringValues", "factoringGradients",
+                  "factoringIndexShift", "spreadedSeeds", "spreadedSeedValues"):
+            if k in r and not (k.startswith("factoring") and "factoringValues" in p):
+                out[f"{name}_{k}"] = np.asarray(r[k])
+        print(name, sorted(k for k in r if k not in ("log", "retcode")))
+    np.savez_compressed(os.path.join(HERE, "factoring.npz"), **out)
+    print("factoring ok")
+
+
 def main():
     for m in ("Riemann2", "Riemann3", "ReedsSheppForward2", "Elastica2"):
         if not oracle.ref_available(m):
@@ -582,7 +656,7 @@ def main():
         groups = {"geodesics": geodesic_cases, "forward": forward_cases, "walls": wall_cases, "seeds": seed_cases,
                   "bypass": bypass_cases, "stencils": stencil_cases, "big": big_case,
                   "orderings": ordering_cases, "high_order": high_order_cases,
-                  "stopping": stopping_cases}
+                  "stopping": stopping_cases, "factoring": factoring_cases}
         for a in only:
             groups[a]()
         return
@@ -600,6 +674,7 @@ def main():
     ordering_cases()
     high_order_cases()
     stopping_cases()
+    factoring_cases()
     if "--big" in sys.argv:
         big_case()
 

@@ -68,15 +68,6 @@ def test_seed_radius_on_curvature_model_is_the_reference_error():
         hfm.run(dict(p, seeds=np.array([[0.5, 0.5, 1.0]]), seedRadius=2.0))
 
 
-@pytest.mark.parametrize("key,value", [("factoringRadius", 10.0)])
-def test_unsupported_keys_are_refused(key, value):
-    """Keys whose effect depends on the serial heap's acceptance order are refused, not ignored."""
-    from hamiltonfastmarching_amd import synthetic as syn
-    p = syn.riemann2(16, seed=3)
-    with pytest.raises(hfm.PyExp, match="not supported"):
-        hfm.run(dict(p, **{key: value}))
-
-
 @pytest.mark.parametrize("name", list(mg.BYPASS_CASES))
 def test_bypass_with_cached_values_vs_golden(name):
     """values + activeNeighs given: the solver is bypassed (HFMInterface.hxx:423-426) and geodesics,
