@@ -34,49 +34,6 @@ struct HeapEntry {
 // has the lower priority.
 __device__ inline bool heap_less(const HeapEntry &a, const HeapEntry &b) { return a.value > b.value; }
 
-// libstdc++'s __push_heap: move the hole up while its parent has lower priority than v.
-__device__ inline void heap_sift_up(HeapEntry *h, long long hole, long long top, const HeapEntry v) {
-    long long parent = (hole - 1) / 2;
-    while (hole > top && heap_less(h[parent], v)) {
-        h[hole] = h[parent];
-        hole = parent;
-        parent = (hole - 1) / 2;
-    }
-    h[hole] = v;
-}
-
-// std::priority_queue::push: push_back then push_heap.
-__device__ inline void heap_push(HeapEntry *h, long long &n, const HeapEntry v) {
-    ++n;
-    heap_sift_up(h, n - 1, 0, v);
-}
-
-// std::priority_queue::pop after top(): pop_heap (the last element re-enters through
-// __adjust_heap from the root: the hole goes down to a leaf along the higher-priority child, the
-// right one on equality, then the element moves up), then pop_back.
-__device__ inline HeapEntry heap_pop(HeapEntry *h, long long &n) {
-    const HeapEntry top = h[0];
-    if (n > 1) {
-        const HeapEntry v = h[n - 1];
-        const long long len = n - 1;
-        long long hole = 0, second = 0;
-        while (second < (len - 1) / 2) {
-            second = 2 * (second + 1);
-            if (heap_less(h[second], h[second - 1])) --second;
-            h[hole] = h[second];
-            hole = second;
-        }
-        if ((len & 1) == 0 && second == (len - 2) / 2) {
-            second = 2 * (second + 1);
-            h[hole] = h[second - 1];
-            hole = second - 1;
-        }
-        heap_sift_up(h, hole, 0, v);
-    }
-    --n;
-    return top;
-}
-
 // Tiled index of the point c + s * o, or -1 off a closed axis.
 template <int ModelT> __device__ inline long long shifted(const Geom &g, const int *c, int o0, int o1, int o2, int s) {
     int r[3];
@@ -115,10 +72,14 @@ struct RecomputeEntry {
 // these schemes do not factor their first pass (HamiltonFastMarching.h:47-50).
 __device__ inline bool factor_at(const Geom &g, const FactorDev &F, const int *c, int *cur) {
     if (!F.on) return false;
-    for (int i = 0; i < 3; ++i) cur[i] = i < g.dim ? c[i] - F.shift[i] : 0;
-    if (F.sub)
-        for (int i = 0; i < g.dim; ++i)
-            if (cur[i] < 0 || cur[i] >= F.bd[i]) return false;
+    // loops over the three axes with a constant bound, so that cur[] stays in registers
+    bool in = true;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        cur[i] = i < g.dim ? c[i] - F.shift[i] : 0;
+        in = in && (!F.sub || i >= g.dim || (cur[i] >= 0 && cur[i] < F.bd[i]));
+    }
+    if (!in) return false;
     return F.mask[((long long)cur[0] * F.bd[1] + cur[1]) * F.bd[2] + cur[2]] != 0;
 }
 
@@ -130,14 +91,16 @@ __device__ inline bool factor_at(const Geom &g, const FactorDev &F, const int *c
 template <int ModelT>
 __device__ inline double factor_corr(const Geom &g, const FactorDev &F, const int *cur, const int *o, int ord) {
     int last[3], i1[3], i2[3];
+    bool in = true;
+#pragma unroll
     for (int i = 0; i < 3; ++i) {
         last[i] = cur[i] + ord * o[i];
         i1[i] = cur[i] + o[i];
         i2[i] = cur[i] + 2 * o[i];
+        in = in && (i >= g.dim || (last[i] >= 0 && last[i] < F.bd[i]));
     }
     if (F.sub) {
-        for (int i = 0; i < g.dim; ++i)
-            if (last[i] < 0 || last[i] >= F.bd[i]) return 0.;
+        if (!in) return 0.;
     } else if (ModelShape<ModelT>::SHARED) {
         const int n = F.bd[2];
         last[2] = ((last[2] % n) + n) % n;
@@ -147,7 +110,9 @@ __device__ inline double factor_corr(const Geom &g, const FactorDev &F, const in
     auto lin = [&](const int *i) { return ((long long)i[0] * F.bd[1] + i[1]) * F.bd[2] + i[2]; };
     const long long c = lin(cur);
     double deriv = 0.;
-    for (int i = 0; i < g.dim; ++i) deriv += F.grads[c * g.dim + i] * (double)o[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        if (i < g.dim) deriv += F.grads[c * g.dim + i] * (double)o[i];
     const double v0 = F.vals[c], vl = F.vals[lin(last)];
     if (ord == 1) return deriv + (v0 - vl);
     const double v1 = F.vals[lin(i1)];
@@ -160,31 +125,30 @@ __device__ inline double factor_corr(const Geom &g, const FactorDev &F, const in
 // (o signed): the neighbour's value, or (order >= 2) (2/3)(2 v1 - v2 / 2) when x + 2 o is accepted
 // and v2 <= v1, or (order >= 3) (6/11)(3 v1 - 3 v2 / 2 + v3 / 3) when x + 3 o also is and v3 <= v2,
 // each plus the factor's correction of that order inside the bracket when the factor applies at x
-// (fcur: x's box index, null otherwise).  The small-correction tests apply to strictly causal
+// (fact, with fcur x's box index).  The small-correction tests apply to strictly causal
 // schemes only (HamiltonFastMarching.h:46), which these are not.
 template <int ModelT>
 __device__ inline double value_corr(const Geom &g, const double *u, const unsigned char *acc, const int *c, const int *o,
-                                    int order, const FactorDev &F, const int *fcur, int &ord) {
-    auto corr = [&](int k) { return fcur ? factor_corr<ModelT>(g, F, fcur, o, k) : 0.; };
+                                    int order, const FactorDev &F, bool fact, const int *fcur, int &ord) {
+    auto corr = [&](int k) { return fact ? factor_corr<ModelT>(g, F, fcur, o, k) : 0.; };
     int c1[3], c2[3], c3[3];
     shift_coords_m<ModelT>(g, c, o[0], o[1], o[2], c1);  // active: in the domain and accepted
-    const double v1 = u[(long long)tile_of(g, c1) * 64 + local_of(g, c1)];
+    const long long q1 = (long long)tile_of(g, c1) * 64 + local_of(g, c1);
+    const bool h2 = order >= 2 && shift_coords_m<ModelT>(g, c1, o[0], o[1], o[2], c2);
+    const bool h3 = h2 && order >= 3 && shift_coords_m<ModelT>(g, c2, o[0], o[1], o[2], c3);
+    const long long q2 = h2 ? (long long)tile_of(g, c2) * 64 + local_of(g, c2) : q1;
+    const long long q3 = h3 ? (long long)tile_of(g, c3) * 64 + local_of(g, c3) : q1;
+    // every value the tests may read, loaded at once (the indices are in the grid)
+    const double v1 = u[q1], v2 = u[q2], v3 = u[q3];
+    const bool a2 = acc[q2] != 0, a3 = acc[q3] != 0;
     ord = 1;
-    if (order >= 2 && shift_coords_m<ModelT>(g, c1, o[0], o[1], o[2], c2)) {
-        const long long q2 = (long long)tile_of(g, c2) * 64 + local_of(g, c2);
-        const double v2 = u[q2];
-        if (acc[q2] && !(v2 > v1)) {
-            if (order >= 3 && shift_coords_m<ModelT>(g, c2, o[0], o[1], o[2], c3)) {
-                const long long q3 = (long long)tile_of(g, c3) * 64 + local_of(g, c3);
-                const double v3 = u[q3];
-                if (acc[q3] && !(v3 > v2)) {
-                    ord = 3;
-                    return (6. / 11.) * ((3. * v1 - 1.5 * v2 + (1. / 3.) * v3) + corr(3));
-                }
-            }
-            ord = 2;
-            return (2. / 3.) * ((2. * v1 - 0.5 * v2) + corr(2));
+    if (h2 && a2 && !(v2 > v1)) {
+        if (h3 && a3 && !(v3 > v2)) {
+            ord = 3;
+            return (6. / 11.) * ((3. * v1 - 1.5 * v2 + (1. / 3.) * v3) + corr(3));
         }
+        ord = 2;
+        return (2. / 3.) * ((2. * v1 - 0.5 * v2) + corr(2));
     }
     return v1 + corr(1);
 }
@@ -215,7 +179,7 @@ __device__ inline int recompute(const Geom &g, const StencilData<double> &sd, co
             r.o[1] = sg * o1;
             r.o[2] = sg * o2;
             r.w = st.w[k];
-            r.value = value_corr<ModelT>(g, u, acc, c, r.o, order, F, fact ? fcur : nullptr, r.ord);
+            r.value = value_corr<ModelT>(g, u, acc, c, r.o, order, F, fact, fcur, r.ord);
         }
     }
     Quad q{__builtin_huge_val(), 0., 0., -1.};
@@ -226,123 +190,308 @@ __device__ inline int recompute(const Geom &g, const StencilData<double> &sd, co
     return n;
 }
 
-// The serial loop (HamiltonFastMarching.hxx:59-169), run by thread 0 of a one-wave block.
+// Heap entries 0 .. HEAP_LDS - 1 (the top 13 levels) live in LDS, the rest in device memory:
+// a pop walks from the root, so most of its dependent loads stay in LDS.  The LDS part is a
+// file-scope LDS array, so that its accesses compile to LDS instructions (a pointer that may
+// point to either memory compiles to flat accesses, several times slower).
+constexpr int HEAP_LDS = 8192;
+__shared__ HeapEntry s_heap[HEAP_LDS];
+using GlobalHeapEntry = __attribute__((address_space(1))) HeapEntry;
+struct HeapView {
+    // the device-memory side through a global-address-space pointer: the compiler then cannot
+    // fold the two sides into one flat access through a selected address
+    GlobalHeapEntry *glob;
+    __device__ HeapEntry get(long long i) const {
+        if (i < HEAP_LDS) return s_heap[i];
+        const GlobalHeapEntry &e = glob[i];
+        return HeapEntry{e.value, e.p};
+    }
+    __device__ void set(long long i, const HeapEntry &e) const {
+        if (i < HEAP_LDS) {
+            s_heap[i] = e;
+        } else {
+            GlobalHeapEntry &d = glob[i];
+            d.value = e.value;
+            d.p = e.p;
+        }
+    }
+};
+
+// libstdc++'s __push_heap: move the hole up while its parent has lower priority than v.
+__device__ inline void heap_sift_up(const HeapView &h, long long hole, long long top, const HeapEntry v) {
+    long long parent = (hole - 1) / 2;
+    while (hole > top) {
+        const HeapEntry pe = h.get(parent);
+        if (!heap_less(pe, v)) break;
+        h.set(hole, pe);
+        hole = parent;
+        parent = (hole - 1) / 2;
+    }
+    h.set(hole, v);
+}
+
+// std::priority_queue::push: push_back then push_heap.
+__device__ inline void heap_push(const HeapView &h, long long &n, const HeapEntry v) {
+    ++n;
+    heap_sift_up(h, n - 1, 0, v);
+}
+
+// std::priority_queue::pop after top(): pop_heap (the last element re-enters through
+// __adjust_heap from the root: the hole goes down to a leaf along the higher-priority child, the
+// right one on equality, then the element moves up), then pop_back.
+__device__ inline HeapEntry heap_pop(const HeapView &h, long long &n) {
+    const HeapEntry top = h.get(0);
+    if (n > 1) {
+        const HeapEntry v = h.get(n - 1);
+        const long long len = n - 1;
+        long long hole = 0, second = 0;
+        while (second < (len - 1) / 2) {
+            second = 2 * (second + 1);
+            const HeapEntry r = h.get(second), l = h.get(second - 1);
+            if (heap_less(r, l)) {
+                --second;
+                h.set(hole, l);
+            } else {
+                h.set(hole, r);
+            }
+            hole = second;
+        }
+        if ((len & 1) == 0 && second == (len - 2) / 2) {
+            second = 2 * (second + 1);
+            h.set(hole, h.get(second - 1));
+            hole = second - 1;
+        }
+        heap_sift_up(h, hole, 0, v);
+    }
+    --n;
+    return top;
+}
+
+// Slot k of the stencil of point p (coordinates c), as LaneStencil::load reads it.
+template <int ModelT>
+__device__ inline void stencil_slot(const Geom &g, const StencilData<double> &sd, long long p, const int *c, int k,
+                                    uint32_t &off, double &w) {
+    using S = ModelShape<ModelT>;
+    constexpr int NC = S::NFWD + S::NSYM;
+    if (!S::SHARED) {
+        off = sd.offs[k * sd.np + p];
+        w = sd.w[k * sd.np + p];
+    } else {
+        const int th = c[2] < g.n[2] ? c[2] : g.n[2] - 1;
+        const double sp = sd.speed[p];
+        const double s2 = sp * sp;
+        off = sd.tab_offs[th * NC + k];
+        w = (double)sd.tab_w[th * NC + k] * s2;
+    }
+}
+
+// The serial loop (HamiltonFastMarching.hxx:59-169) on one wave.  Lane 0 owns the heap and the
+// sequence of pops and pushes; the work of one pop that does not depend on that sequence is
+// spread over the lanes: the recompute's differences (one stencil slot per lane, written to LDS
+// at their place in stencil order, then summed by lane 0 in that order) and the conditional
+// updates of the popped point's reversed neighbours (one neighbour per lane: the neighbours are
+// distinct points, each update touches its own point only; lane 0 then pushes them in the
+// reversed offsets' order).  Every value is computed with the same operations, in the same
+// order, as the single-threaded loop.
 template <int ModelT>
 __global__ __launch_bounds__(64) void k_fm_exact(ExactArgs a) {
-    if (threadIdx.x != 0) return;
     using S = ModelShape<ModelT>;
     constexpr int NF = S::NFWD, NC = NF + S::NSYM;
+    __shared__ double s_rv[2 * NC], s_rw[2 * NC];
+    __shared__ int s_ro[2 * NC];
+    __shared__ long long s_px[64];
+    __shared__ double s_pv[64];
+    const int lane = threadIdx.x;
     const Geom &g = a.g;
-    HeapEntry *heap = static_cast<HeapEntry *>(a.heap);
-    long long n = 0, accepted = 0, maxn = 0;
+    const HeapView heap{(GlobalHeapEntry *)a.heap};
+    long long n = 0, accepted = 0, maxn = 0, npost = 0;  // lane 0's
+    long long all_left = a.all_left, all_left_u = a.all_left_u;
+    int status = 0;
     // RunInit: the seeds in index order, values set and pushed
-    for (int i = 0; i < a.nseeds; ++i) {
-        a.u[a.seed_p[i]] = a.seed_v[i];
-        heap_push(heap, n, HeapEntry{a.seed_v[i], a.seed_p[i]});
-    }
-    RecomputeEntry re[2 * NC];
-    long long npost = 0, all_left = a.all_left, all_left_u = a.all_left_u;
-    while (n > 0) {
-        maxn = n > maxn ? n : maxn;
-        HeapEntry top = heap_pop(heap, n);
-        const long long y = top.p;
-        if (a.accepted[y]) continue;
+    if (lane == 0)
+        for (int i = 0; i < a.nseeds; ++i) {
+            a.u[a.seed_p[i]] = a.seed_v[i];
+            heap_push(heap, n, HeapEntry{a.seed_v[i], a.seed_p[i]});
+        }
+    while (true) {
+        // the next entry; an already accepted point's is skipped (lazy deletion)
+        long long y = -1;
+        double key = 0.;
+        if (lane == 0 && n > 0) {
+            maxn = n > maxn ? n : maxn;
+            const HeapEntry t = heap_pop(heap, n);
+            y = t.p;
+            key = t.value;
+        }
+        y = __shfl(y, 0);
+        if (y < 0) break;
         int cy[3];
         tiled_coords(g, (int)(y >> 6), (int)(y & 63), cy);
+        // what the pop needs from device memory, loaded at once before the accepted test
+        const unsigned char acc_y = a.accepted[y];
+        const unsigned long long act = a.active[y];
+        const double u_y = a.u[y];
+        const long long kk = S::SHARED ? (long long)cy[2] : y;
+        const long long r0 = a.rstart[kk], r1 = r0 + a.rcount[kk];
+        uint32_t slot_off = 0;
+        double slot_w = 0.;
+        if (lane < NC) stencil_slot<ModelT>(g, a.sd, y, cy, lane, slot_off, slot_w);
+        asm volatile("" ::: "memory");
+        if (acc_y) continue;
         // PostProcess (HamiltonFastMarching.hxx:86-99): the stopping criteria see the point with
         // its heap value, before the recompute (ExtraAlgorithms/CommonStoppingCriteria.h:57-92)
-        ++npost;
-        bool stop = npost >= a.max_accepted;
-        if (a.stop_flags) {
-            const unsigned char f = a.stop_flags[y];
-            if (f & 1) stop = true;
-            if (f & 2) {
-                a.stop_flags[y] = f & (unsigned char)~2;
-                if (--all_left == 0) stop = true;
+        int stop = 0;
+        if (lane == 0) {
+            ++npost;
+            stop = npost >= a.max_accepted;
+            if (a.stop_flags) {
+                const unsigned char f = a.stop_flags[y];
+                if (f & 1) stop = 1;
+                if (f & 2) {
+                    a.stop_flags[y] = f & (unsigned char)~2;
+                    if (--all_left == 0) stop = 1;
+                }
             }
-        }
-        if (a.stop_flags_u) {
-            const long long xy = (long long)cy[0] * g.n[1] + cy[1];
-            const unsigned char f = a.stop_flags_u[xy];
-            if (f & 1) stop = true;
-            if (f & 2) {
-                a.stop_flags_u[xy] = f & (unsigned char)~2;
-                if (--all_left_u == 0) stop = true;
+            if (a.stop_flags_u) {
+                const long long xy = (long long)cy[0] * g.n[1] + cy[1];
+                const unsigned char f = a.stop_flags_u[xy];
+                if (f & 1) stop = 1;
+                if (f & 2) {
+                    a.stop_flags_u[xy] = f & (unsigned char)~2;
+                    if (--all_left_u == 0) stop = 1;
+                }
             }
+            if (u_y > a.stop_distance) stop = 1;
         }
-        if (a.u[y] > a.stop_distance) stop = true;
+        stop = __shfl(stop, 0);
         // the recompute (order >= 2, or the factor applies): a point without active differences
         // keeps its value
         int fcur[3];
-        if ((a.order > 1 || factor_at(g, a.fac, cy, fcur)) && a.active[y]) {
-            double v;
-            (void)recompute<ModelT>(g, a.sd, a.u, a.accepted, y, cy, a.active[y], a.order, a.fac, re, &v);
-            top.value = v;
-            a.u[y] = v;
+        const bool fact = factor_at(g, a.fac, cy, fcur);
+        if ((a.order > 1 || fact) && act) {
+            if (lane < NC) {
+                const double w = slot_w;
+                int o0, o1, o2;
+                unpack_offset(slot_off, o0, o1, o2);
+                for (int sd = 0; sd < (lane < NF ? 1 : 2); ++sd) {
+                    const int bit = side_bit<NF>(lane, sd);
+                    if (!(act >> bit & 1ull)) continue;
+                    const int sg = sd == 0 ? 1 : -1;
+                    const int o[3] = {sg * o0, sg * o1, sg * o2};
+                    const int pos = __popcll(act & ((1ull << bit) - 1ull));
+                    int ord;
+                    s_rv[pos] = value_corr<ModelT>(g, a.u, a.accepted, cy, o, a.order, a.fac, fact, fcur, ord);
+                    s_rw[pos] = w;
+                    s_ro[pos] = ord;
+                }
+            }
+            __syncthreads();
+            if (lane == 0) {
+                const int ne = __popcll(act);
+                Quad q{__builtin_huge_val(), 0., 0., -1.};
+                for (int i = 0; i < ne; ++i) q.minVal = s_rv[i] < q.minVal ? s_rv[i] : q.minVal;
+                constexpr double mult2[4] = {0., 1., 9. / 4., 121. / 36.};
+                for (int i = 0; i < ne; ++i) q.add(s_rv[i], mult2[s_ro[i]] * s_rw[i]);
+                key = q.solve();
+                a.u[y] = key;
+            }
+            key = __shfl(key, 0);
+            __syncthreads();
+        } else {
+            key = __shfl(key, 0);
         }
         if (stop) break;  // the point that triggered the stop is not accepted
-        a.accepted[y] = 1;
-        ++accepted;
-        const long long key = S::SHARED ? (long long)cy[2] : y;
-        const long long r0 = a.rstart[key], r1 = r0 + a.rcount[key];
-        for (long long r = r0; r < r1; ++r) {
-            int e0, e1, e2;
-            unpack_offset(a.roffs[r], e0, e1, e2);
-            // ConditionalUpdate (HamiltonFastMarching.hxx:136-154)
-            const long long x = shifted<ModelT>(g, cy, e0, e1, e2, -1);
-            if (x < 0 || a.accepted[x]) continue;
-            unsigned long long act = a.active[x];
-            if (act == 0ull && a.u[x] != __builtin_huge_val()) continue;  // seeds are never updated
-            if (a.u[x] <= top.value) continue;
-            // HopfLaxUpdate (EulerianStencil.hxx:104-153): activate the differences with this
-            // offset (a symmetric one only if its other side is not active), add them to the
-            // point's quadratic, solve
-            int cx[3];
-            tiled_coords(g, (int)(x >> 6), (int)(x & 63), cx);
-            LaneStencil<ModelT, double> st;
-            st.load(g, a.sd, x, cx);
-            double *qs = a.quad + 4 * x;
-            Quad q{qs[0], qs[1], qs[2], qs[3]};
-            if (q.minVal == -__builtin_huge_val()) q.minVal = top.value;
-            const uint32_t eo = pack_offset(e0, e1, e2), meo = pack_offset(-e0, -e1, -e2);
-            for (int k = 0; k < NC; ++k) {
-                if (k < NF) {
-                    if (st.off[k] == eo) {
-                        act |= 1ull << side_bit<NF>(k, 0);
-                        q.add(top.value, st.w[k]);
-                    }
-                    continue;
-                }
-                const int b0 = side_bit<NF>(k, 0), b1 = side_bit<NF>(k, 1);
-                if (st.off[k] == eo && !(act >> b1 & 1ull)) {
-                    act |= 1ull << b0;
-                    q.add(top.value, st.w[k]);
-                }
-                if (st.off[k] == meo && !(act >> b0 & 1ull)) {
-                    act |= 1ull << b1;
-                    q.add(top.value, st.w[k]);
-                }
-            }
-            const double value = q.solve();
-            qs[0] = q.minVal;
-            qs[1] = q.a;
-            qs[2] = q.b;
-            qs[3] = q.c;
-            a.active[x] = act;
-            a.u[x] = value;
-            if (n >= a.heap_cap) {  // cannot happen: the capacity is the number of pushes possible
-                a.out[2] = 1;
-                a.out[0] = accepted;
-                a.out[3] = npost;
-                return;
-            }
-            heap_push(heap, n, HeapEntry{value, x});
+        if (lane == 0) {
+            a.accepted[y] = 1;
+            ++accepted;
         }
+        // every store of an iteration completes at a barrier before the next iteration reads
+        // (the update chunks end with one; a point without reversed neighbours takes its own)
+        if (r0 == r1) __syncthreads();
+        for (long long base = r0; base < r1; base += 64) {
+            const long long r = base + lane;
+            long long px = -1;
+            double pv = 0.;
+            if (r < r1) {
+                int e0, e1, e2;
+                unpack_offset(a.roffs[r], e0, e1, e2);
+                // ConditionalUpdate (HamiltonFastMarching.hxx:136-154)
+                const long long x = shifted<ModelT>(g, cy, e0, e1, e2, -1);
+                if (x >= 0) {
+                    // the neighbour's state and stencil, loaded at once before the tests
+                    const unsigned char accx = a.accepted[x];
+                    unsigned long long ax = a.active[x];
+                    const double ux = a.u[x];
+                    int cx[3];
+                    tiled_coords(g, (int)(x >> 6), (int)(x & 63), cx);
+                    LaneStencil<ModelT, double> st;
+                    st.load(g, a.sd, x, cx);
+                    double *qs = a.quad + 4 * x;
+                    Quad q{qs[0], qs[1], qs[2], qs[3]};
+                    asm volatile("" ::: "memory");
+                    // seeds are never updated; no update unless the value would decrease
+                    if (!accx && !(ax == 0ull && ux != __builtin_huge_val()) && !(ux <= key)) {
+                        // HopfLaxUpdate (EulerianStencil.hxx:104-153): activate the differences
+                        // with this offset (a symmetric one only if its other side is not
+                        // active), add them to the point's quadratic, solve
+                        if (q.minVal == -__builtin_huge_val()) q.minVal = key;
+                        const uint32_t eo = pack_offset(e0, e1, e2), meo = pack_offset(-e0, -e1, -e2);
+                        for (int k = 0; k < NC; ++k) {
+                            if (k < NF) {
+                                if (st.off[k] == eo) {
+                                    ax |= 1ull << side_bit<NF>(k, 0);
+                                    q.add(key, st.w[k]);
+                                }
+                                continue;
+                            }
+                            const int b0 = side_bit<NF>(k, 0), b1 = side_bit<NF>(k, 1);
+                            if (st.off[k] == eo && !(ax >> b1 & 1ull)) {
+                                ax |= 1ull << b0;
+                                q.add(key, st.w[k]);
+                            }
+                            if (st.off[k] == meo && !(ax >> b0 & 1ull)) {
+                                ax |= 1ull << b1;
+                                q.add(key, st.w[k]);
+                            }
+                        }
+                        const double value = q.solve();
+                        qs[0] = q.minVal;
+                        qs[1] = q.a;
+                        qs[2] = q.b;
+                        qs[3] = q.c;
+                        a.active[x] = ax;
+                        a.u[x] = value;
+                        px = x;
+                        pv = value;
+                    }
+                }
+            }
+            s_px[lane] = px;
+            s_pv[lane] = pv;
+            __syncthreads();
+            if (lane == 0) {
+                const int cnt = r1 - base < 64 ? (int)(r1 - base) : 64;
+                for (int j = 0; j < cnt; ++j) {
+                    if (s_px[j] < 0) continue;
+                    if (n >= a.heap_cap) {  // cannot happen: the capacity is the number of pushes possible
+                        status = 1;
+                        break;
+                    }
+                    heap_push(heap, n, HeapEntry{s_pv[j], s_px[j]});
+                }
+            }
+            status = __shfl(status, 0);
+            __syncthreads();
+            if (status) break;
+        }
+        if (status) break;
     }
-    a.out[0] = accepted;
-    a.out[1] = maxn;
-    a.out[2] = 0;
-    a.out[3] = npost;
+    if (lane == 0) {
+        a.out[0] = accepted;
+        a.out[1] = maxn;
+        a.out[2] = status;
+        a.out[3] = npost;
+    }
 }
 
 // Reversed offsets of the per-point stencils (HFM_StencilRecompute.hxx:81-124): every point y
