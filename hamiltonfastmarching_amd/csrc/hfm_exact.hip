@@ -217,51 +217,86 @@ struct HeapView {
     }
 };
 
-// libstdc++'s __push_heap: move the hole up while its parent has lower priority than v.
-__device__ inline void heap_sift_up(const HeapView &h, long long hole, long long top, const HeapEntry v) {
-    long long parent = (hole - 1) / 2;
-    while (hole > top) {
-        const HeapEntry pe = h.get(parent);
-        if (!heap_less(pe, v)) break;
-        h.set(hole, pe);
-        hole = parent;
-        parent = (hole - 1) / 2;
+// The heap's steps, run by the whole wave with wave-uniform arguments; they leave the heap
+// exactly as libstdc++'s sequential steps do, because the same comparisons decide the same moves.
+
+// __push_heap(first, hole, 0, v): the hole moves up while its parent has lower priority than v.
+// Lane j compares the ancestor j + 1 levels up with v; the sequential loop stops at the first
+// ancestor that does not have lower priority, so the moves are the ancestors below it (the
+// trailing ones of the ballot), each written one level down by its lane, and v takes the last
+// place freed.  Every read precedes every write.
+__device__ inline void wave_sift_up(const HeapView &h, long long hole, const HeapEntry &v, int lane) {
+    const int depth = 63 - __clzll((unsigned long long)(hole + 1));  // number of ancestors
+    double ev = 0.;
+    long long ep = 0;
+    bool lower = false;
+    if (lane < depth) {
+        const HeapEntry e = h.get(((hole + 1) >> (lane + 1)) - 1);
+        ev = e.value;
+        ep = e.p;
+        lower = ev > v.value;  // heap_less(e, v)
     }
-    h.set(hole, v);
+    const int t = __builtin_ctzll(~__ballot(lower));
+    if (lane < t) h.set(((hole + 1) >> lane) - 1, HeapEntry{ev, ep});
+    if (lane == t) h.set(((hole + 1) >> t) - 1, v);
+    if (hole >= HEAP_LDS) __syncthreads();  // device-memory writes complete before the next reads
 }
 
 // std::priority_queue::push: push_back then push_heap.
-__device__ inline void heap_push(const HeapView &h, long long &n, const HeapEntry v) {
+__device__ inline void wave_push(const HeapView &h, long long &n, const HeapEntry &v, int lane) {
     ++n;
-    heap_sift_up(h, n - 1, 0, v);
+    wave_sift_up(h, n - 1, v, lane);
 }
 
-// std::priority_queue::pop after top(): pop_heap (the last element re-enters through
-// __adjust_heap from the root: the hole goes down to a leaf along the higher-priority child, the
-// right one on equality, then the element moves up), then pop_back.
-__device__ inline HeapEntry heap_pop(const HeapView &h, long long &n) {
+// std::priority_queue::pop after top(): pop_heap, then pop_back.  The last element re-enters
+// through __adjust_heap from the root: while the hole has two children it takes the one of
+// higher priority (the right one on equality), then the element moves up.  The descent goes five
+// levels at a time: lane j < 31 takes the j-th node of the next five levels below the hole (the
+// nodes that can become holes) and, if that node would descend, reads its two children and
+// picks one; the path then follows the picks from the top (scalar bit steps), and the lanes on
+// it move their picked child up.  Every read of a chunk precedes its writes, and a chunk's reads
+// lie below the nodes earlier chunks wrote.
+__device__ inline HeapEntry wave_pop(const HeapView &h, long long &n, int lane) {
     const HeapEntry top = h.get(0);
     if (n > 1) {
-        const HeapEntry v = h.get(n - 1);
         const long long len = n - 1;
-        long long hole = 0, second = 0;
-        while (second < (len - 1) / 2) {
-            second = 2 * (second + 1);
-            const HeapEntry r = h.get(second), l = h.get(second - 1);
-            if (heap_less(r, l)) {
-                --second;
-                h.set(hole, l);
-            } else {
-                h.set(hole, r);
+        const HeapEntry v = h.get(len);
+        const long long lim = (len - 1) / 2;  // a hole below lim has both children
+        long long hole = 0;
+        while (hole < lim) {
+            long long node = 0;
+            double cv = 0.;
+            long long cp = 0;
+            bool desc = false, left = false;
+            if (lane < 31) {
+                const int l = 31 - __clz(lane + 1);  // level within the chunk
+                node = ((hole + 1) << l) - 1 + (lane + 1 - (1 << l));
+                desc = node < lim;
+                if (desc) {
+                    const HeapEntry r = h.get(2 * node + 2), lf = h.get(2 * node + 1);
+                    left = heap_less(r, lf);
+                    cv = left ? lf.value : r.value;
+                    cp = left ? lf.p : r.p;
+                }
             }
-            hole = second;
+            const unsigned long long dm = __ballot(desc), lm = __ballot(left);
+            unsigned long long path = 0;
+            int m = 0;
+            for (int step = 0; step < 5 && (dm >> m & 1ull); ++step) {
+                path |= 1ull << m;
+                m = 2 * m + ((lm >> m & 1ull) ? 1 : 2);
+            }
+            if (path >> lane & 1ull) h.set(node, HeapEntry{cv, cp});
+            const int l = 31 - __clz(m + 1);
+            hole = ((hole + 1) << l) - 1 + (m + 1 - (1 << l));
         }
-        if ((len & 1) == 0 && second == (len - 2) / 2) {
-            second = 2 * (second + 1);
-            h.set(hole, h.get(second - 1));
-            hole = second - 1;
+        if ((len & 1) == 0 && hole == (len - 2) / 2) {  // a last hole with a left child only
+            const HeapEntry lf = h.get(2 * hole + 1);
+            if (lane == 0) h.set(hole, lf);
+            hole = 2 * hole + 1;
         }
-        heap_sift_up(h, hole, 0, v);
+        if (hole >= HEAP_LDS) __syncthreads();  // device-memory writes complete before the next reads
+        wave_sift_up(h, hole, v, lane);
     }
     --n;
     return top;
@@ -285,14 +320,14 @@ __device__ inline void stencil_slot(const Geom &g, const StencilData<double> &sd
     }
 }
 
-// The serial loop (HamiltonFastMarching.hxx:59-169) on one wave.  Lane 0 owns the heap and the
-// sequence of pops and pushes; the work of one pop that does not depend on that sequence is
-// spread over the lanes: the recompute's differences (one stencil slot per lane, written to LDS
-// at their place in stencil order, then summed by lane 0 in that order) and the conditional
-// updates of the popped point's reversed neighbours (one neighbour per lane: the neighbours are
-// distinct points, each update touches its own point only; lane 0 then pushes them in the
-// reversed offsets' order).  Every value is computed with the same operations, in the same
-// order, as the single-threaded loop.
+// The serial loop (HamiltonFastMarching.hxx:59-169) on one wave.  The pops and pushes run in
+// the reference's sequence, each spread over the wave (wave_pop, wave_push); so is the work of
+// one pop: the recompute's differences (one stencil slot per lane, written to LDS at their place
+// in stencil order, then summed by lane 0 in that order) and the conditional updates of the
+// popped point's reversed neighbours (one neighbour per lane: the neighbours are distinct points,
+// each update touches its own point only; they are then pushed in the reversed offsets' order).
+// Every value is computed with the same operations, in the same order, as the single-threaded
+// loop.
 template <int ModelT>
 __global__ __launch_bounds__(64) void k_fm_exact(ExactArgs a) {
     using S = ModelShape<ModelT>;
@@ -304,27 +339,22 @@ __global__ __launch_bounds__(64) void k_fm_exact(ExactArgs a) {
     const int lane = threadIdx.x;
     const Geom &g = a.g;
     const HeapView heap{(GlobalHeapEntry *)a.heap};
-    long long n = 0, accepted = 0, maxn = 0, npost = 0;  // lane 0's
+    long long n = 0, maxn = 0;                   // heap size, the same in every lane
+    long long accepted = 0, npost = 0;           // lane 0's
     long long all_left = a.all_left, all_left_u = a.all_left_u;
     int status = 0;
     // RunInit: the seeds in index order, values set and pushed
-    if (lane == 0)
-        for (int i = 0; i < a.nseeds; ++i) {
-            a.u[a.seed_p[i]] = a.seed_v[i];
-            heap_push(heap, n, HeapEntry{a.seed_v[i], a.seed_p[i]});
-        }
+    for (int i = 0; i < a.nseeds; ++i) {
+        if (lane == 0) a.u[a.seed_p[i]] = a.seed_v[i];
+        wave_push(heap, n, HeapEntry{a.seed_v[i], a.seed_p[i]}, lane);
+    }
     while (true) {
         // the next entry; an already accepted point's is skipped (lazy deletion)
-        long long y = -1;
-        double key = 0.;
-        if (lane == 0 && n > 0) {
-            maxn = n > maxn ? n : maxn;
-            const HeapEntry t = heap_pop(heap, n);
-            y = t.p;
-            key = t.value;
-        }
-        y = __shfl(y, 0);
-        if (y < 0) break;
+        if (n == 0) break;
+        maxn = n > maxn ? n : maxn;
+        const HeapEntry t = wave_pop(heap, n, lane);
+        const long long y = t.p;
+        double key = t.value;
         int cy[3];
         tiled_coords(g, (int)(y >> 6), (int)(y & 63), cy);
         // what the pop needs from device memory, loaded at once before the accepted test
@@ -469,18 +499,16 @@ __global__ __launch_bounds__(64) void k_fm_exact(ExactArgs a) {
             s_px[lane] = px;
             s_pv[lane] = pv;
             __syncthreads();
-            if (lane == 0) {
-                const int cnt = r1 - base < 64 ? (int)(r1 - base) : 64;
-                for (int j = 0; j < cnt; ++j) {
-                    if (s_px[j] < 0) continue;
-                    if (n >= a.heap_cap) {  // cannot happen: the capacity is the number of pushes possible
-                        status = 1;
-                        break;
-                    }
-                    heap_push(heap, n, HeapEntry{s_pv[j], s_px[j]});
+            const int cnt = r1 - base < 64 ? (int)(r1 - base) : 64;
+            for (int j = 0; j < cnt; ++j) {
+                const long long px_j = s_px[j];
+                if (px_j < 0) continue;
+                if (n >= a.heap_cap) {  // cannot happen: the capacity is the number of pushes possible
+                    status = 1;
+                    break;
                 }
+                wave_push(heap, n, HeapEntry{s_pv[j], px_j}, lane);
             }
-            status = __shfl(status, 0);
             __syncthreads();
             if (status) break;
         }

@@ -1,7 +1,8 @@
 """The workloads at their full sizes (HFM inputs from the seeded generator):
 
 * Riemann2 256^2, h = 1/256, FP64 update: values against the oracle's FP64 Fast Marching to
-  1e-12 relative, Selling offsets bit-exact.
+  1e-12 relative, Selling offsets bit-exact; through the serial acceptance loop (order 2/3,
+  static factoring, a stopping criterion) bit-equal to the oracle's loop.
 * ReedsSheppForward2 256 x 256 x 64, xi = 0.1, 4 seeds, FP32 update: values against the oracle to
   1e-5 relative; geodesics from the 32 tips, FP64 solve: bit-exact against the oracle's discrete
   solver on the GPU's own values and active sets, and within one cell of the oracle's curves.
@@ -60,6 +61,20 @@ def test_riemann2_256_fp64():
     off, _ = plan.stencils()
     roff, _ = oracle.riemann_stencils((256, 256), p["gridScale"], p["metric"])
     assert np.array_equal(off, roff)
+
+
+@pytest.mark.parametrize("extra", [{"order": 2.0}, {"order": 3.0, "factoringRadius": 10.0},
+                                   {"factoringRadius": -1.0}, {"stopAtDistance": 0.3, "order": 2.0}])
+def test_riemann2_256_serial_loop(extra):
+    """Riemann2 256^2 through the serial acceptance loop (order 2/3, static factoring, a stopping
+    criterion): values, activeNeighs and nAccepted bit-equal to the oracle's loop."""
+    print(f"Riemann2 256^2 serial loop {extra}", flush=True)
+    p = dict(syn.riemann2(256, seed=0), **extra)
+    ref = oracle.solve(p, active=True)
+    out = hfm.run(dict(p, exportActiveNeighs=1.0))
+    assert np.array_equal(out["values"], ref["values"])
+    assert np.array_equal(out["activeNeighs"], ref["activeNeighs"].astype(np.float64))
+    assert out["nAccepted"] == ref["nAccepted"]
 
 
 def test_rsf2_full_values_fp32():
