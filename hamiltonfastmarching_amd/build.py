@@ -24,7 +24,7 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 SOURCES = ["hfm_kernels.hip", "hfm_plan.hip", "hfm_adjoint.hip", "hfm_server.hip", "hfm_geodesic.hip", "hfm_walls.hip",
            "hfm_exact.hip", "hfm_io.cpp"]
-HEADERS = ["hfm_device.h", "hfm_kernels.h", "hfm_internal.h", "selling.h"]
+HEADERS = ["hfm_device.h", "hfm_kernels.h", "hfm_internal.h", "hfm_recompute.h", "selling.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

@@ -24,10 +24,15 @@
 // covariance, steps) is FP64 in the reference's order of operations -- a vector divided by a
 // scalar is a multiplication by the reciprocal (JMM_CPPLibs LinearAlgebra/VectorType.h:37-42) --
 // and runs uniformly on every lane.  On an FP64 solve the emitted
-// points are those the reference's solver computes from the same values and active sets.
+// points are those the reference's solver computes from the same values and active sets.  After
+// a serial solve (order 2/3, static factoring, stopping criteria) the flow at a point is that
+// loop's recompute over its accepted set instead (ExactFlow), as the reference takes it.
 #include "hfm_kernels.h"
 
+#include <type_traits>
+
 #include "hfm_device.h"
+#include "hfm_recompute.h"
 #include "hfm_tile.h"
 
 namespace hfm {
@@ -215,9 +220,32 @@ __device__ inline void load_slot(const StencilData<T> &sd, long long p, int th, 
 // offset, w_i max(0, r - v_i)) in stencil order into L.f_off / L.f_w; returns their number.
 template <int ModelT, typename T>
 __device__ int discrete_flow(const Geom &g, const StencilData<T> &sd, const T *u,
-                             const unsigned long long *fixedmask, const int *c, GeoLDS &L, int lane) {
+                             const unsigned long long *fixedmask, const ExactFlow &xf, const int *c, GeoLDS &L,
+                             int lane) {
     using S = ModelShape<ModelT>;
     constexpr int NF = S::NFWD, NC = NF + S::NSYM;
+    if constexpr (std::is_same<T, double>::value) {
+        if (xf.on) {  // after a serial solve: its recompute, evaluated by lane 0
+            int n = 0;
+            if (lane == 0) {
+                const long long p = (long long)tile_of(g, c) * 64 + local_of(g, c);
+                const unsigned long long act = xf.active[p];
+                RecomputeEntry re[2 * NC];
+                double value = 0.;
+                if (act) n = recompute<ModelT>(g, sd, u, xf.acc, p, c, act, xf.order, xf.fac, re, &value);
+                for (int i = 0; i < n; ++i) {
+                    int r[3];
+                    shift_coords_m<ModelT>(g, c, re[i].o[0], re[i].o[1], re[i].o[2], r);
+                    for (int q = 0; q < 3; ++q) L.f_off[i][q] = re[i].o[q];
+                    L.f_w[i] = flow_weight(re[i], value);
+                    L.f_val[i] = u[(long long)tile_of(g, r) * 64 + local_of(g, r)];
+                }
+            }
+            n = __shfl(n, 0);
+            __syncthreads();
+            return n;
+        }
+    }
     const int tile = tile_of(g, c), loc = local_of(g, c);
     const long long p = (long long)tile * 64 + loc;
     // Everything that depends on the point alone is loaded at once (its value, fixed bit and
@@ -307,10 +335,52 @@ __device__ int discrete_flow(const Geom &g, const StencilData<T> &sd, const T *u
 // stencil loads, then all its neighbour-value loads, before it uses any of them.
 template <int ModelT, typename T>
 __device__ void fill_flow(const Geom &g, const StencilData<T> &sd, const T *u, const unsigned long long *fixedmask,
-                          int tile, int slot, FlowCache<ModelT, T> &C, int lane) {
+                          const ExactFlow &xf, int tile, int slot, FlowCache<ModelT, T> &C, int lane) {
     using S = ModelShape<ModelT>;
     constexpr int NF = S::NFWD, NC = NF + S::NSYM;
     using CS = FlowCacheShape<ModelT, T>;
+    if constexpr (std::is_same<T, double>::value) {
+        if (xf.on) {  // after a serial solve: its recompute, one lane per point
+            int c[3];
+            tiled_coords(g, tile, lane, c);
+            const long long p = (long long)tile * 64 + lane;
+            RecomputeEntry re[2 * NC];
+            double value = 0.;
+            int n = 0;
+            if (c[0] < g.n[0] && c[1] < g.n[1] && c[2] < g.n[2]) {
+                const unsigned long long act = xf.active[p];
+                if (act) n = recompute<ModelT>(g, sd, u, xf.acc, p, c, act, xf.order, xf.fac, re, &value);
+            }
+            // the loop activates at most one side of a symmetric difference: at most NC entries
+            int st = lane * NC;
+            bool fits = n <= NC;
+            if (!CS::FIXED) {
+                int incl = n;
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    const int t = __shfl_up(incl, d);
+                    if (lane >= d) incl += t;
+                }
+                st = incl - n;
+                fits = incl <= CS::ECAP;
+            }
+            double ws = 0;
+            for (int i = 0; i < n; ++i) {
+                const double fw = flow_weight(re[i], value);
+                ws += fw;
+                if (fits) {
+                    int r[3];
+                    shift_coords_m<ModelT>(g, c, re[i].o[0], re[i].o[1], re[i].o[2], r);
+                    C.e_off[slot][st + i] = pack_offset(re[i].o[0], re[i].o[1], re[i].o[2]);
+                    C.e_w[slot][st + i] = fw;
+                    C.e_v[slot][st + i] = u[(long long)tile_of(g, r) * 64 + local_of(g, r)];
+                }
+            }
+            C.wsum[slot][lane] = ws;
+            C.info[slot][lane] = fits ? (uint32_t)st << 8 | (uint32_t)n : (uint32_t)GEO_UNCACHED;
+            return;
+        }
+    }
     const T inf = dev_inf<T>();
     int c[3];
     tiled_coords(g, tile, lane, c);
@@ -442,7 +512,8 @@ enum GeoStatus { GEO_OK = 0, GEO_FAILED = 1, GEO_SET_OVERFLOW = 2, GEO_STEPS_OVE
 
 template <int ModelT, typename T>
 __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, const T *__restrict__ u,
-                                                 const unsigned long long *__restrict__ fixedmask, GeoParams gp,
+                                                 const unsigned long long *__restrict__ fixedmask, ExactFlow xf,
+                                                 GeoParams gp,
                                                  const double *__restrict__ tips, double *__restrict__ points,
                                                  long long *__restrict__ lengths, int *__restrict__ status) {
     constexpr int D = ModelShape<ModelT>::DIM;
@@ -621,7 +692,7 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
             } else {
                 const unsigned oldest = wave_min_u32(lane < CS::NSLOT ? my_use : 0xffffffffu);
                 slot = __ffsll((long long)__ballot(lane < CS::NSLOT && my_use == oldest)) - 1;
-                fill_flow<ModelT, T>(g, sd, u, fixedmask, tile, slot, C, lane);
+                fill_flow<ModelT, T>(g, sd, u, fixedmask, xf, tile, slot, C, lane);
                 if (lane == slot) my_tag = tile;
                 __syncthreads();
             }
@@ -658,7 +729,7 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
                         my_off = C.e_off[slot][e];
                     }
                 } else {
-                    nf = discrete_flow<ModelT, T>(g, sd, u, fixedmask, c, L, lane);
+                    nf = discrete_flow<ModelT, T>(g, sd, u, fixedmask, xf, c, L, lane);
                     wSum = 0;
                     for (int k = 0; k < nf; ++k) wSum += L.f_w[k];
                     wSum = uni_d(wSum);
@@ -802,8 +873,8 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
 
 template <int M, typename T>
 static void geo_one(const Geom &g, const StencilData<T> &sd, const T *u, const unsigned long long *fm,
-                    const GeoParamsHost &h, int ntips, const double *tips, double *points, long long *lengths,
-                    int *status, hipStream_t s) {
+                    const ExactFlow &xf, const GeoParamsHost &h, int ntips, const double *tips, double *points,
+                    long long *lengths, int *status, hipStream_t s) {
     GeoParams gp;
     gp.step = h.step;
     gp.threshold = h.threshold;
@@ -814,29 +885,29 @@ static void geo_one(const Geom &g, const StencilData<T> &sd, const T *u, const u
     }
     gp.max_points = h.max_points;
     gp.max_iter = h.max_iter;
-    hipLaunchKernelGGL((k_geodesic<M, T>), dim3(ntips), dim3(64), 0, s, g, sd, u, fm, gp, tips, points, lengths,
-                       status);
+    hipLaunchKernelGGL((k_geodesic<M, T>), dim3(ntips), dim3(64), 0, s, g, sd, u, fm, xf, gp, tips, points,
+                       lengths, status);
 }
 
 template <typename T>
 hipError_t launch_geodesics(int model, const Geom &g, const StencilData<T> &sd, const T *u,
-                            const unsigned long long *fm, const GeoParamsHost &h, int ntips, const double *tips,
-                            double *points, long long *lengths, int *status, hipStream_t s) {
+                            const unsigned long long *fm, const ExactFlow &xf, const GeoParamsHost &h, int ntips,
+                            const double *tips, double *points, long long *lengths, int *status, hipStream_t s) {
     if (ntips <= 0) return hipSuccess;
     switch (model) {
-        case MODEL_RIEMANN2: geo_one<MODEL_RIEMANN2, T>(g, sd, u, fm, h, ntips, tips, points, lengths, status, s); break;
-        case MODEL_RIEMANN3: geo_one<MODEL_RIEMANN3, T>(g, sd, u, fm, h, ntips, tips, points, lengths, status, s); break;
-        case MODEL_RSF2: geo_one<MODEL_RSF2, T>(g, sd, u, fm, h, ntips, tips, points, lengths, status, s); break;
-        case MODEL_ELASTICA2: geo_one<MODEL_ELASTICA2, T>(g, sd, u, fm, h, ntips, tips, points, lengths, status, s); break;
+        case MODEL_RIEMANN2: geo_one<MODEL_RIEMANN2, T>(g, sd, u, fm, xf, h, ntips, tips, points, lengths, status, s); break;
+        case MODEL_RIEMANN3: geo_one<MODEL_RIEMANN3, T>(g, sd, u, fm, xf, h, ntips, tips, points, lengths, status, s); break;
+        case MODEL_RSF2: geo_one<MODEL_RSF2, T>(g, sd, u, fm, xf, h, ntips, tips, points, lengths, status, s); break;
+        case MODEL_ELASTICA2: geo_one<MODEL_ELASTICA2, T>(g, sd, u, fm, xf, h, ntips, tips, points, lengths, status, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 template hipError_t launch_geodesics<float>(int, const Geom &, const StencilData<float> &, const float *,
-                                            const unsigned long long *, const GeoParamsHost &, int, const double *,
-                                            double *, long long *, int *, hipStream_t);
+                                            const unsigned long long *, const ExactFlow &, const GeoParamsHost &, int,
+                                            const double *, double *, long long *, int *, hipStream_t);
 template hipError_t launch_geodesics<double>(int, const Geom &, const StencilData<double> &, const double *,
-                                             const unsigned long long *, const GeoParamsHost &, int, const double *,
-                                             double *, long long *, int *, hipStream_t);
+                                             const unsigned long long *, const ExactFlow &, const GeoParamsHost &,
+                                             int, const double *, double *, long long *, int *, hipStream_t);
 
 }  // namespace hfm

@@ -170,6 +170,10 @@ struct hfm_plan {
     // static factor (hfm_plan_set_factoring): device copies of values, gradients (grid units) and
     // mask over the box; fac.on = 0 for none
     hfm::FactorDev fac;
+    // the order and factor the last serial solve ran with (its flows and geodesics use them); a
+    // new factor invalidates that solve, whose factor arrays it frees
+    int ex_order = 1;
+    hfm::FactorDev ex_fac;
     double *fac_vals = nullptr, *fac_grads = nullptr;
     unsigned char *fac_mask = nullptr;
     hipStream_t graph_stream = nullptr;

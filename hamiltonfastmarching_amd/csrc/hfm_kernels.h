@@ -122,10 +122,21 @@ struct GeoParamsHost {
 };
 enum GeoStatusHost { GEO_STATUS_OK = 0, GEO_STATUS_FAILED = 1, GEO_STATUS_SET_OVERFLOW = 2,
                      GEO_STATUS_STEPS_OVERFLOW = 3, GEO_STATUS_NO_END = 4, GEO_STATUS_INDEX_RANGE = 5 };
+// The flows of a serial solve for geodesic backtracking: the recompute over that loop's accepted
+// set (order 2/3 combinations, the factor's corrections, HamiltonFastMarching.hxx:187-272), as
+// the reference's solver takes them (GeodesicDiscreteSolver.hxx:131-132).  on = 0: a block solve,
+// whose flows are recomputed from the values alone.
+struct ExactFlow {
+    const unsigned char *acc = nullptr;
+    const unsigned long long *active = nullptr;
+    int order = 1;
+    FactorDev fac;
+    int on = 0;
+};
 template <typename T>
 hipError_t launch_geodesics(int model, const Geom &g, const StencilData<T> &sd, const T *u,
-                            const unsigned long long *fm, const GeoParamsHost &h, int ntips, const double *tips,
-                            double *points, long long *lengths, int *status, hipStream_t s);
+                            const unsigned long long *fm, const ExactFlow &xf, const GeoParamsHost &h, int ntips,
+                            const double *tips, double *points, long long *lengths, int *status, hipStream_t s);
 
 template <typename T>
 hipError_t launch_riemann_stencil(const Geom &g, const double *metric, int is_field, int dual, const InvHH &ih,

@@ -484,6 +484,8 @@ void run_exact(hfm_plan *p, hipStream_t s) {
     a.all_left = 0;
     a.all_left_u = 0;
     a.fac = p->fac;
+    p->ex_order = p->order;
+    p->ex_fac = p->fac;
     std::vector<unsigned char> fl, flu;
     if (!p->stop_any.empty() || !p->stop_all.empty()) {
         fl.assign(np, 0);
@@ -1108,8 +1110,8 @@ static int plan_flow(const char *who, hfm_plan *p, double *d_flow, double *d_act
     FlowScale fs{{p->h[0], p->h[1], p->h[2]}};
     if (p->exact_solved) {  // the flags of the serial loop and its order-aware recompute
         const hipError_t e = launch_flow_exact(p->model, p->g, p->sd<double>(), static_cast<const double *>(p->u),
-                                               p->ex_acc, p->ex_active, p->order, p->fac, fs, d_flow, d_active, d_aoffs,
-                                               p->np, s);
+                                               p->ex_acc, p->ex_active, p->ex_order, p->ex_fac, fs, d_flow, d_active,
+                                               d_aoffs, p->np, s);
         if (e != hipSuccess) {
             set_error(std::string(who) + ": " + hipGetErrorString(e));
             return 1;
@@ -1222,11 +1224,6 @@ int hfm_plan_geodesics_impl(hfm_plan *p, int ntips, const double *tips, bool tip
         set_error("hfm_plan_geodesics: no solve has run");
         return 1;
     }
-    if (p->exact_solved) {
-        set_error("hfm_plan_geodesics: geodesics of " + p->exact_what +
-                  " (the serial acceptance loop) are not supported by this library");
-        return 1;
-    }
     *npoints = 0;
     if (nfailed) *nfailed = 0;
     if (ntips <= 0) return 0;
@@ -1290,12 +1287,22 @@ int hfm_plan_geodesics_impl(hfm_plan *p, int ntips, const double *tips, bool tip
                 p->geo_pts = dalloc<double>((size_t)need);
                 p->geo_pts_cap = need;
             }
+            // after a serial solve the flows are its recompute over its accepted set (FP64 plans only)
+            ExactFlow xf;
+            if (p->exact_solved) {
+                xf.acc = p->ex_acc;
+                xf.active = p->ex_active;
+                xf.order = p->ex_order;
+                xf.fac = p->ex_fac;
+                xf.on = 1;
+            }
             const hipError_t e =
                 p->precision == 32
                     ? launch_geodesics<float>(p->model, p->g, p->sd<float>(), static_cast<const float *>(p->u),
-                                              p->fixedmask, h, ntips, p->geo_tips, p->geo_pts, p->geo_len, p->geo_st, s)
+                                              p->fixedmask, xf, h, ntips, p->geo_tips, p->geo_pts, p->geo_len,
+                                              p->geo_st, s)
                     : launch_geodesics<double>(p->model, p->g, p->sd<double>(), static_cast<const double *>(p->u),
-                                               p->fixedmask, h, ntips, p->geo_tips, p->geo_pts, p->geo_len,
+                                               p->fixedmask, xf, h, ntips, p->geo_tips, p->geo_pts, p->geo_len,
                                                p->geo_st, s);
             HIPCHECK(e);
             long long *hl = reinterpret_cast<long long *>(p->geo_host);
@@ -1507,6 +1514,10 @@ int hfm_plan_set_factoring(hfm_plan *p, const int64_t *box_dims, const int64_t *
         return 1;
     }
     try {
+        if (p->exact_solved) {  // that solve's flows would read the arrays freed here
+            p->exact_solved = false;
+            p->solved = false;
+        }
         if (!values) {  // remove the factor
             dfree(p->fac_vals);
             dfree(p->fac_grads);

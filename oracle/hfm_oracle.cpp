@@ -903,20 +903,19 @@ struct GeoParams {
 };
 using Idx3 = std::array<int, 3>;
 
-// Discrete flow of HopfLaxRecompute (EulerianStencil.hxx:155-233) at order 1: the active
-// differences in stencil order, weight w_i max(0, r - v_i) with r the recomputed value.
+// Discrete flow of Recompute (HamiltonFastMarching.hxx:187-272 with HopfLaxRecompute,
+// EulerianStencil.hxx:155-233), as GeodesicDiscreteSolver.hxx:131-132 takes it: the active
+// differences in stencil order, weight (w_i max(0, r - v_i)) times 1, 3/2, 11/6 by the order the
+// difference uses, r the recomputed value; at order 1 without a factor, v_i is the neighbour's
+// value.
 int discrete_flow(const Problem &P, const FMState &S, long x, int (*off)[3], double *w) {
-    ActiveDiff ad[2 * kMaxDiff];
-    const int n = S.active[x] ? active_diffs(P, S, x, ad) : 0;
-    if (n == 0) return 0;
-    double vals[2 * kMaxDiff];
-    Quad q; q.minVal = kInf;
-    for (int i = 0; i < n; ++i) { vals[i] = S.values[ad[i].neigh]; q.minVal = std::min(q.minVal, vals[i]); }
-    for (int i = 0; i < n; ++i) q.add(vals[i], 1. * ad[i].w);
-    const double value = q.solve();
+    RecEntry re[2 * kMaxDiff];
+    double value = 0.;
+    const int n = S.active[x] ? recompute_entries(P, S, x, g_order, re, &value) : 0;
+    static const double mult[4] = {0., 1., 3. / 2., 11. / 6.};
     for (int i = 0; i < n; ++i) {
-        w[i] = ad[i].w * std::max(0., value - vals[i]);
-        for (int c = 0; c < 3; ++c) off[i][c] = c < P.dim ? ad[i].s * ad[i].off[c] : 0;
+        w[i] = (re[i].w * std::max(0., value - re[i].value)) * mult[re[i].ord];
+        for (int c = 0; c < 3; ++c) off[i][c] = c < P.dim ? re[i].off[c] : 0;
     }
     return n;
 }

@@ -21,6 +21,7 @@ Fixtures (all row-major):
   stopping.npz       values, activeNeighs, nAccepted under each stopping criterion
   factoring.npz      values, activeNeighs, geodesicFlow, nAccepted and the exported factor with static
                      factoring (computed or imported)
+  serial_geodesics.npz  geodesicPoints / geodesicLengths after order 2/3, factored and stopped solves
   riemann3_128.npz   digests of values and Selling offsets at the benchmark size + 4096 seeded samples
 Inputs are regenerated by hamiltonfastmarching_amd.synthetic; the SHA-256 of each generated
 input is stored so that a test can tell a generator change from a numerical one.
@@ -647,6 +648,45 @@ def factoring_cases():
     print("factoring ok")
 
 
+def serial_geodesic_params():
+    """Geodesics after a serial solve (order 2/3, static factoring, a stopping criterion): the
+    reference's solver takes each point's flow from Recompute (GeodesicDiscreteSolver.hxx:131-132)."""
+    def strip(p):
+        return {k: v for k, v in p.items()
+                if k not in ("tips", "inspectSensitivity", "inspectSensitivityWeights", "inspectSensitivityLengths")}
+    r2 = strip(syn.riemann2(40, seed=41))
+    t2 = np.array([[0.2125, 0.8125], [0.9125, 0.1125], [0.5625, 0.3125]])  # cell centres
+    r3 = strip(syn.riemann3(14, seed=42))
+    t3 = np.array([[0.1, 0.85, 0.2], [0.8, 0.3, 0.9]])
+    rs = strip(syn.rsf2(20, 12, seed=43))
+    ts = np.array([[0.2, 0.8, 1.0], [0.85, 0.2, 4.0], [0.6, 0.55, 2.5]])
+    el = strip(syn.elastica2(16, 12, seed=44))
+    te = np.array([[0.2, 0.75, 0.5], [0.8, 0.3, 3.0]])
+    return {
+        "r2_o2": dict(r2, order=2.0, tips=t2),
+        "r2_o3_fact": dict(r2, order=3.0, factoringRadius=10.0, tips=t2),
+        "r2_fact_all": dict(r2, factoringRadius=-1.0, tips=t2),
+        "r2_stop_tips": dict(r2, stopWhenAllAccepted=t2, tips=t2),
+        "r3_o2": dict(r3, order=2.0, tips=t3),
+        "rsf2_o2": dict(rs, order=2.0, tips=ts),
+        "rsf2_stop_o3": dict(rs, order=3.0, stopWhenAnyAccepted=ts[:1], tips=ts[:1]),
+        "elastica2_o2": dict(el, order=2.0, tips=te),
+    }
+
+
+def serial_geodesic_cases():
+    out = {}
+    for name, p in serial_geodesic_params().items():
+        r = oracle.run_reference(p)
+        assert r["retcode"] == 0, r["log"]
+        out[name + "_values"] = r["values"]
+        out[name + "_points"] = r["geodesicPoints"]
+        out[name + "_lengths"] = r["geodesicLengths"]
+        print(name, np.asarray(r["geodesicLengths"]).astype(int).tolist())
+    np.savez_compressed(os.path.join(HERE, "serial_geodesics.npz"), **out)
+    print("serial_geodesics ok")
+
+
 def main():
     for m in ("Riemann2", "Riemann3", "ReedsSheppForward2", "Elastica2"):
         if not oracle.ref_available(m):
@@ -656,7 +696,8 @@ def main():
         groups = {"geodesics": geodesic_cases, "forward": forward_cases, "walls": wall_cases, "seeds": seed_cases,
                   "bypass": bypass_cases, "stencils": stencil_cases, "big": big_case,
                   "orderings": ordering_cases, "high_order": high_order_cases,
-                  "stopping": stopping_cases, "factoring": factoring_cases}
+                  "stopping": stopping_cases, "factoring": factoring_cases,
+                  "serial_geodesics": serial_geodesic_cases}
         for a in only:
             groups[a]()
         return
@@ -675,6 +716,7 @@ def main():
     high_order_cases()
     stopping_cases()
     factoring_cases()
+    serial_geodesic_cases()
     if "--big" in sys.argv:
         big_case()
 

@@ -71,7 +71,6 @@ def test_plan_api_factoring():
     ({"precision": 32.0}, "precision 64"),
     ({"seeds": np.array([[0.3, 0.3], [0.6, 0.6]])}, "Exactly one seed"),
     ({"factoringPointChoice": "Bad"}, "Unsupported factoringPointChoice Bad"),
-    ({"tips": np.array([[0.8, 0.8]])}, "geodesics of a factored solve"),
 ])
 def test_refusals(extra, msg):
     with pytest.raises(hfm.PyExp, match=msg):

@@ -77,7 +77,6 @@ def test_plan_api_order():
 @pytest.mark.parametrize("extra,msg", [
     ({"precision": 32.0}, "precision 64"),
     ({"walls": "box"}, "walls"),
-    ({"tips": np.array([[0.8, 0.8]])}, "geodesics"),
     ({"inspectSensitivity": np.array([[0.8, 0.8]]), "inspectSensitivityWeights": np.array([1.0]),
       "inspectSensitivityLengths": np.array([1.0])}, "differentiation"),
 ])
