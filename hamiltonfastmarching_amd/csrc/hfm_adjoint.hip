@@ -27,16 +27,50 @@
 namespace hfm {
 
 // Active differences of tiled point p recomputed from converged values.  Returns the number of
-// active entries; nb[] their tiled indices, wp[] the w'_k, and *wsum, *var.
+// active entries; nb[] their tiled indices, wp[] the w'_k, and *wsum, *var.  After a serial solve
+// (xact: that loop's activeNeighs) the active differences are the loop's, in stencil order, as
+// ValueVariation reads them (FirstVariation.h:198-251: the stored flags and values, first-order
+// weights w max(0, u - v), also after order 2/3 and factoring).
 template <int ModelT, typename T>
 __device__ inline int active_set(const Geom &g, const StencilData<T> &sd, const T *__restrict__ u, long long p,
-                                 const int *c, bool fixed, long long *nb, double *wp, double *wsum, double *var) {
+                                 const int *c, bool fixed, const unsigned long long *xact, long long *nb, double *wp,
+                                 double *wsum, double *var) {
     using S = ModelShape<ModelT>;
     constexpr int NF = S::NFWD, NC = NF + S::NSYM;
     const T inf = dev_inf<T>();
     const T uu = u[p];
     *wsum = 0.;
     *var = 0.;
+    if (xact) {
+        const unsigned long long act = xact[p];
+        if (!act || !(uu < inf)) return 0;  // seeds and unreached points have no active difference
+        LaneStencil<ModelT, T> st;
+        st.load(g, sd, p, c);
+        const double mult = S::SHARED ? (double)sd.speed[p] : 1.;
+        int n = 0;
+        double ws = 0., acc = 0.;
+        for (int k = 0; k < NC; ++k) {
+            int o0, o1, o2;
+            unpack_offset(st.off[k], o0, o1, o2);
+            for (int sd2 = 0; sd2 < (k < NF ? 1 : 2); ++sd2) {
+                if (!(act >> side_bit<NF>(k, sd2) & 1ull)) continue;
+                const int sg = sd2 == 0 ? 1 : -1;
+                int r[3];
+                shift_coords_m<ModelT>(g, c, sg * o0, sg * o1, sg * o2, r);  // active: in the grid
+                const long long q = (long long)tile_of(g, r) * 64 + local_of(g, r);
+                const double d = fmax(0., (double)uu - (double)u[q]);
+                const double w = (double)st.w[k] * d;
+                ws += w;
+                acc += w * d;
+                nb[n] = q;
+                wp[n] = w;
+                ++n;  // the loop activates one side of a symmetric difference: n <= NC
+            }
+        }
+        *wsum = ws;
+        *var = ws > 0. ? (1. / ws) * (mult * acc) : 0.;
+        return n;
+    }
     if (fixed || !(uu < inf)) return 0;
     LaneStencil<ModelT, T> st;
     st.load(g, sd, p, c);
@@ -94,6 +128,7 @@ template <int ModelT> constexpr int adj_slots() {
 template <int ModelT, typename T>
 __global__ __launch_bounds__(256) void k_adj_prepare(Geom g, StencilData<T> sd, const T *__restrict__ u,
                                                      const unsigned long long *__restrict__ fixedmask,
+                                                     const unsigned long long *__restrict__ xact,
                                                      int *__restrict__ indeg, double *__restrict__ lambda,
                                                      double *__restrict__ sens, int *__restrict__ ecnt,
                                                      unsigned *__restrict__ eidx, double *__restrict__ var_out,
@@ -111,7 +146,7 @@ __global__ __launch_bounds__(256) void k_adj_prepare(Geom g, StencilData<T> sd, 
     const bool fixed = (fixedmask[p >> 6] >> (p & 63)) & 1ull;
     long long nb[NC];
     double wp[NC], ws, var;
-    const int n = active_set<ModelT, T>(g, sd, u, p, c, fixed, nb, wp, &ws, &var);
+    const int n = active_set<ModelT, T>(g, sd, u, p, c, fixed, xact, nb, wp, &ws, &var);
     ecnt[p] = n;
     var_out[p] = var;
     const double inv = ws > 0. ? 1. / ws : 0.;
@@ -254,6 +289,7 @@ __global__ void k_export_sens(Geom g, const double *__restrict__ s, double *__re
 template <int ModelT, typename T>
 __global__ __launch_bounds__(256) void k_fwd_prepare(Geom g, StencilData<T> sd, const T *__restrict__ u,
                                                      const unsigned long long *__restrict__ fixedmask,
+                                                     const unsigned long long *__restrict__ xact,
                                                      int *__restrict__ pending, int *__restrict__ rcount,
                                                      long long np) {
     constexpr int NC = ModelShape<ModelT>::NFWD + ModelShape<ModelT>::NSYM;
@@ -268,7 +304,7 @@ __global__ __launch_bounds__(256) void k_fwd_prepare(Geom g, StencilData<T> sd, 
     const bool fixed = (fixedmask[p >> 6] >> (p & 63)) & 1ull;
     long long nb[NC];
     double wp[NC], ws, var;
-    const int n = active_set<ModelT, T>(g, sd, u, p, c, fixed, nb, wp, &ws, &var);
+    const int n = active_set<ModelT, T>(g, sd, u, p, c, fixed, xact, nb, wp, &ws, &var);
     pending[p] = n;
     for (int i = 0; i < n; ++i) atomicAdd(&rcount[nb[i]], 1);
 }
@@ -278,6 +314,7 @@ __global__ __launch_bounds__(256) void k_fwd_prepare(Geom g, StencilData<T> sd, 
 template <int ModelT, typename T>
 __global__ __launch_bounds__(256) void k_fwd_fill(Geom g, StencilData<T> sd, const T *__restrict__ u,
                                                   const unsigned long long *__restrict__ fixedmask,
+                                                  const unsigned long long *__restrict__ xact,
                                                   const int *__restrict__ rstart, int *__restrict__ rcount,
                                                   int *__restrict__ rev, long long np) {
     constexpr int NC = ModelShape<ModelT>::NFWD + ModelShape<ModelT>::NSYM;
@@ -289,7 +326,7 @@ __global__ __launch_bounds__(256) void k_fwd_fill(Geom g, StencilData<T> sd, con
     const bool fixed = (fixedmask[p >> 6] >> (p & 63)) & 1ull;
     long long nb[NC];
     double wp[NC], ws, var;
-    const int n = active_set<ModelT, T>(g, sd, u, p, c, fixed, nb, wp, &ws, &var);
+    const int n = active_set<ModelT, T>(g, sd, u, p, c, fixed, xact, nb, wp, &ws, &var);
     for (int i = 0; i < n; ++i) {
         const int slot = atomicSub(&rcount[nb[i]], 1) - 1;
         rev[rstart[nb[i]] + slot] = (int)p;
@@ -310,6 +347,7 @@ __global__ void k_fwd_seeds(const long long *__restrict__ sp, const double *__re
 template <int ModelT, typename T>
 __global__ __launch_bounds__(256) void k_fwd_round(Geom g, StencilData<T> sd, const T *__restrict__ u,
                                                    const unsigned long long *__restrict__ fixedmask,
+                                                   const unsigned long long *__restrict__ xact,
                                                    int *__restrict__ pending, const int *__restrict__ rstart,
                                                    const int *__restrict__ rev, const double *__restrict__ cv,
                                                    int nvar, double *__restrict__ res,
@@ -332,7 +370,7 @@ __global__ __launch_bounds__(256) void k_fwd_round(Geom g, StencilData<T> sd, co
             const bool fixed = (fixedmask[p >> 6] >> (p & 63)) & 1ull;
             long long nb[NC];
             double wp[NC], ws = 0., var = 0.;
-            const int na = active_set<ModelT, T>(g, sd, u, p, c, fixed, nb, wp, &ws, &var);
+            const int na = active_set<ModelT, T>(g, sd, u, p, c, fixed, xact, nb, wp, &ws, &var);
             // neighbour weights w_i / S as the reference normalises them (weight *= 1 / S)
             const double inv = ws > 0. ? 1. / ws : 0.;
             const long long lin = ((long long)c[0] * g.n[1] + c[1]) * g.n[2] + c[2];
@@ -395,8 +433,9 @@ template <int M, typename T> void adjoint_run(hfm_plan *p, const std::vector<lon
     const long long np = p->np;
     const unsigned nb = (unsigned)((np + 255) / 256);
     const T *u = static_cast<const T *>(p->u);
+    const unsigned long long *xact = p->exact_solved ? p->ex_active : nullptr;  // a serial solve's flags
     HIPCHECK(hipMemsetAsync(p->adj_indeg, 0, sizeof(int) * np, s));
-    hipLaunchKernelGGL((k_adj_prepare<M, T>), dim3(nb), dim3(256), 0, s, p->g, p->sd<T>(), u, p->fixedmask,
+    hipLaunchKernelGGL((k_adj_prepare<M, T>), dim3(nb), dim3(256), 0, s, p->g, p->sd<T>(), u, p->fixedmask, xact,
                        p->adj_indeg, p->adj_lambda, p->adj_sens, p->adj_ecnt, p->adj_eidx, p->adj_var, p->adj_ew,
                        np);
     HIPCHECK(hipGetLastError());
@@ -469,8 +508,9 @@ template <int M, typename T> void forward_run(hfm_plan *p, int nvar, const doubl
     const long long np = p->np;
     const unsigned nb = (unsigned)((np + 255) / 256);
     const T *u = static_cast<const T *>(p->u);
+    const unsigned long long *xact = p->exact_solved ? p->ex_active : nullptr;  // a serial solve's flags
     HIPCHECK(hipMemsetAsync(p->fwd_rcount, 0, sizeof(int) * np, s));
-    hipLaunchKernelGGL((k_fwd_prepare<M, T>), dim3(nb), dim3(256), 0, s, p->g, p->sd<T>(), u, p->fixedmask,
+    hipLaunchKernelGGL((k_fwd_prepare<M, T>), dim3(nb), dim3(256), 0, s, p->g, p->sd<T>(), u, p->fixedmask, xact,
                        p->fwd_pending, p->fwd_rcount, np);
     HIPCHECK(hipGetLastError());
     HIPCHECK(launch_exclusive_scan(p->fwd_rcount, p->fwd_rstart, (int)np, s, p->fwd_scan_work));
@@ -483,7 +523,7 @@ template <int M, typename T> void forward_run(hfm_plan *p, int nvar, const doubl
         p->fwd_rev_cap = std::max<long long>(nedges, np);
         p->fwd_rev = dalloc<int>(p->fwd_rev_cap);
     }
-    hipLaunchKernelGGL((k_fwd_fill<M, T>), dim3(nb), dim3(256), 0, s, p->g, p->sd<T>(), u, p->fixedmask,
+    hipLaunchKernelGGL((k_fwd_fill<M, T>), dim3(nb), dim3(256), 0, s, p->g, p->sd<T>(), u, p->fixedmask, xact,
                        p->fwd_rstart, p->fwd_rcount, p->fwd_rev, np);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemsetAsync(p->fwd_res, 0, sizeof(double) * np * nvar, s));
@@ -504,7 +544,7 @@ template <int M, typename T> void forward_run(hfm_plan *p, int nvar, const doubl
     const long long max_rounds = np + 16;
     while (true) {
         for (int b = 0; b < 8; ++b, ++k) {
-            hipLaunchKernelGGL((k_fwd_round<M, T>), dim3(grid), dim3(256), 0, s, p->g, p->sd<T>(), u, p->fixedmask,
+            hipLaunchKernelGGL((k_fwd_round<M, T>), dim3(grid), dim3(256), 0, s, p->g, p->sd<T>(), u, p->fixedmask, xact,
                                p->fwd_pending, p->fwd_rstart, p->fwd_rev, cv, nvar, p->fwd_res, p->adj_list[k & 1],
                                &p->adj_counts[k % 3], p->adj_list[(k + 1) & 1], &p->adj_counts[(k + 1) % 3],
                                &p->adj_counts[(k + 2) % 3]);
@@ -531,11 +571,6 @@ int hfm_plan_adjoint_impl(hfm_plan *p, int ntargets, const double *targets, cons
     }
     if (!p->solved) {
         set_error("hfm_plan_adjoint: no solve has run");
-        return 1;
-    }
-    if (p->exact_solved) {
-        set_error("hfm_plan_adjoint: differentiation of " + p->exact_what +
-                  " (the serial acceptance loop) is not supported by this library");
         return 1;
     }
     try {
@@ -643,11 +678,6 @@ int hfm_plan_forward_variation_impl(hfm_plan *p, int nvar, const double *d_cost_
     }
     if (!p->solved) {
         set_error("hfm_plan_forward_variation: no solve has run");
-        return 1;
-    }
-    if (p->exact_solved) {
-        set_error("hfm_plan_forward_variation: differentiation of " + p->exact_what +
-                  " (the serial acceptance loop) is not supported by this library");
         return 1;
     }
     if (nvar <= 0) {

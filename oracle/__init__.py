@@ -365,7 +365,23 @@ def forward_variation(params, cost_variation=None, seed_value_variation=None):
         cvar = np.ascontiguousarray(np.asarray(cost_variation, np.float64).reshape(N, -1))
         nvar = cvar.shape[1]
     if seed_value_variation is not None:
-        svar = np.ascontiguousarray(np.asarray(seed_value_variation, np.float64).reshape(-1, len(seeds)))
+        given = np.asarray(params["seeds"], np.float64).reshape(-1, dim)
+        svar = np.asarray(seed_value_variation, np.float64).reshape(-1, len(given))
+        if len(given) != len(seeds) or not np.array_equal(given, seeds):
+            # the solver's seeds were spread: the variations sit at the given seeds' grid points
+            # (the last one written wins), every other solver seed varies by 0 (FirstVariation.h:106-119)
+            def cell(pt):  # IndexFromPoint(ADim(p)) on a Riemann grid (spreading is Riemann only)
+                return tuple(int(np.floor((pt[i] - (0. if origin is None else origin[i])) / h[i])) for i in range(dim))
+            at = {}
+            for i, pt in enumerate(given):
+                at[cell(pt)] = i
+            sv2 = np.zeros((svar.shape[0], len(seeds)))
+            for j, pt in enumerate(seeds):
+                i = at.get(cell(pt))
+                if i is not None:
+                    sv2[:, j] = svar[:, i]
+            svar = sv2
+        svar = np.ascontiguousarray(svar)
         nvar = max(nvar, svar.shape[0])
     values = np.zeros(N)
     out = np.zeros((N, nvar))

@@ -22,6 +22,7 @@ Fixtures (all row-major):
   factoring.npz      values, activeNeighs, geodesicFlow, nAccepted and the exported factor with static
                      factoring (computed or imported)
   serial_geodesics.npz  geodesicPoints / geodesicLengths after order 2/3, factored and stopped solves
+  serial_variations.npz costSensitivity_0 / seedSensitivity_0 / valueVariation after such solves
   riemann3_128.npz   digests of values and Selling offsets at the benchmark size + 4096 seeded samples
 Inputs are regenerated by hamiltonfastmarching_amd.synthetic; the SHA-256 of each generated
 input is stored so that a test can tell a generator change from a numerical one.
@@ -687,6 +688,55 @@ def serial_geodesic_cases():
     print("serial_geodesics ok")
 
 
+def serial_variation_params():
+    """Differentiation after a serial solve (FirstVariation.h:198-251,341-385,389-429: the stored
+    activeNeighs and values, first-order weights): costSensitivity / seedSensitivity of target
+    arrival times and valueVariation, after order 2/3, static factoring and stopping criteria."""
+    rng = np.random.Generator(np.random.PCG64(55))
+
+    def sens(p, targets, weights):
+        return dict(p, inspectSensitivity=targets, inspectSensitivityWeights=weights,
+                    inspectSensitivityLengths=np.array([float(len(targets))]))
+
+    def fwd(p):
+        dims = tuple(int(x) for x in p["dims"])
+        cv = rng.normal(size=dims + (2,))
+        sv = rng.normal(size=(2, len(np.asarray(p["seeds"]).reshape(-1, len(dims)))))
+        return dict(p, costVariation=cv, seedValueVariation=sv)
+
+    el = {k: v for k, v in syn.elastica2(16, 12, seed=51).items() if not k.startswith("inspect")}
+    rs = {k: v for k, v in syn.rsf2(20, 12, seed=53).items() if k != "tips"}
+    tg = np.array([[0.3, 0.7, 1.0], [0.8, 0.2, 2.0], [0.55, 0.45, 4.5]])
+    tw = np.array([1.0, 2.0, 0.5])
+    fi = {k: v for k, v in _factor_import_rsf2().items()}
+    return {
+        "el_o2_sens": sens(dict(el, order=2.0), tg, tw),
+        "el_stop_sens": sens(dict(el, nMaxAccepted=1500.0), tg[:1], tw[:1]),
+        "rsf2_o3_sens": sens(dict(rs, order=3.0), tg, tw),
+        "rsf2_fact_sens": sens(fi, tg, tw),
+        "rsf2_o2_fwd": fwd(dict(rs, order=2.0)),
+        "r2_o2_fwd": fwd(dict(syn.riemann2(24, seed=56), order=2.0)),
+        "r3_fact_fwd": fwd(dict(syn.riemann3(12, seed=57), factoringRadius=4.0)),
+        "el_stop_fwd": fwd(dict(el, stopAtDistance=1.0)),
+    }
+
+
+def serial_variation_cases():
+    out = {}
+    for name, p in serial_variation_params().items():
+        r = oracle.run_reference(p)
+        assert r["retcode"] == 0, r["log"]
+        for k in ("values", "costSensitivity_0", "seedSensitivity_0", "valueVariation", "costVariation",
+                  "seedValueVariation"):
+            if k in r:
+                out[f"{name}_{k}"] = np.asarray(r[k])
+            elif k in p:
+                out[f"{name}_{k}"] = np.asarray(p[k])
+        print(name, sorted(k for k in r if k in ("costSensitivity_0", "seedSensitivity_0", "valueVariation")))
+    np.savez_compressed(os.path.join(HERE, "serial_variations.npz"), **out)
+    print("serial_variations ok")
+
+
 def main():
     for m in ("Riemann2", "Riemann3", "ReedsSheppForward2", "Elastica2"):
         if not oracle.ref_available(m):
@@ -697,7 +747,7 @@ def main():
                   "bypass": bypass_cases, "stencils": stencil_cases, "big": big_case,
                   "orderings": ordering_cases, "high_order": high_order_cases,
                   "stopping": stopping_cases, "factoring": factoring_cases,
-                  "serial_geodesics": serial_geodesic_cases}
+                  "serial_geodesics": serial_geodesic_cases, "serial_variations": serial_variation_cases}
         for a in only:
             groups[a]()
         return
@@ -717,6 +767,7 @@ def main():
     stopping_cases()
     factoring_cases()
     serial_geodesic_cases()
+    serial_variation_cases()
     if "--big" in sys.argv:
         big_case()
 

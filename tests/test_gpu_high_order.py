@@ -1,5 +1,6 @@
 """GPU: order 2 / 3 solves (the serial acceptance loop) against the reference's outputs and the
-oracle, bit for bit in FP64, and the refusals of what an order 2 / 3 solve does not support."""
+oracle, bit for bit in FP64, and the refusals of what an order 2 / 3 solve does not support
+(FP32 plans, walls)."""
 import os
 import sys
 
@@ -77,8 +78,6 @@ def test_plan_api_order():
 @pytest.mark.parametrize("extra,msg", [
     ({"precision": 32.0}, "precision 64"),
     ({"walls": "box"}, "walls"),
-    ({"inspectSensitivity": np.array([[0.8, 0.8]]), "inspectSensitivityWeights": np.array([1.0]),
-      "inspectSensitivityLengths": np.array([1.0])}, "differentiation"),
 ])
 def test_refusals(extra, msg):
     p = syn.riemann2(24, seed=23)
