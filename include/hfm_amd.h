@@ -284,10 +284,11 @@ int hfm_plan_set_fused_schedule(hfm_plan *plan, int enable);
  * second / third order differences where the next points along an offset are accepted and no
  * larger (replaces the `order` key, HFMInterface.hxx:358, HamiltonFastMarching.hxx:86-123,
  * 187-272).  Values above 3 act as 3, below 2 as 1.  Order 2 / 3 solves run the reference's serial
- * acceptance loop on the device (one thread popping a binary heap in device memory, so that
- * equal keys leave in the reference's order): FP64 plans without walls; their exports are values,
- * activeNeighs, geodesicFlow and activeOffsets (the order-aware recompute); geodesics,
- * differentiation and the residual check refuse them. */
+ * acceptance loop on the device (one wave popping a binary heap with libstdc++'s steps, so that
+ * equal keys leave in the reference's order): FP64 plans without walls.  After such a solve,
+ * geodesicFlow, activeOffsets and geodesics come from the loop's recompute over its accepted set
+ * (order-aware, with the factor's corrections), differentiation reads its activeNeighs and values
+ * as the reference's FirstVariation does, and the residual check (first order) refuses it. */
 int hfm_plan_set_order(hfm_plan *plan, int order);
 
 /* Stopping criteria (replace nMaxAccepted, stopAtDistance, stopWhenAnyAccepted,
