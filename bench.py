@@ -331,7 +331,9 @@ def configs_leg(with_cpu):
       riemann2_256_fp64    Riemann2 256^2, h = 1/256, FP64 update, centre seed
       rsf2_256x256x64      ReedsSheppForward2, xi = 0.1, FP32, 4 seeds; then 32 geodesics
       elastica2_192x192x64 Elastica2, xi = 0.2, FP32, 1 seed; then the adjoint of the sum of the
-                           arrival times at 32 targets (costSensitivity)"""
+                           arrival times at 32 targets (costSensitivity)
+      riemann2_256_order2_serial  Riemann2 256^2 at order 2 (the serial acceptance loop), FP64,
+                           beside the reference's FMCPUTime"""
     import torch
     import oracle
     import hamiltonfastmarching_amd as hfm
@@ -401,6 +403,22 @@ def configs_leg(with_cpu):
         r["adjoint_max_err_vs_reference"] = float(np.abs(sens - sref).max() / np.abs(sref).max())
         r["adjoint_err_definition"] = "max |costSensitivity - reference| / max |reference| (FP32 solve and adjoint)"
     res["elastica2_192x192x64"] = r
+    del plan
+    # Riemann2 256^2 at order 2: the serial acceptance loop, bit-equal to the reference
+    p = dict(syn.riemann2(256, seed=0), order=2.0)
+    plan = hfm.Plan("Riemann2", (256, 256), p["gridScale"], precision=64)
+    m = torch.from_numpy(p["metric"]).cuda()
+    plan.set_metric(m)
+    plan.set_order(2)
+    out = torch.empty((256, 256), dtype=torch.float64, device="cuda")
+    ms = _time_gpu(lambda: (plan.solve(p["seeds"]), plan.values(out)), 2)
+    c, ref = cpu(p, keys=("FMCPUTime",))
+    r = {"gpu_solve_ms": round(ms, 3), "cpu": c, "note": "one wave runs the reference's heap loop"}
+    if ref is not None:
+        r["bit_equal_to_reference"] = bool(np.array_equal(out.cpu().numpy(),
+                                                          np.asarray(ref["values"]).reshape(256, 256)))
+        r["gpu_over_cpu_time"] = round(ms / c["ms"], 1)
+    res["riemann2_256_order2_serial"] = r
     del plan
     torch.cuda.synchronize()
     return res
