@@ -309,6 +309,26 @@ class Plan:
                 return pts[:npts.value].copy(), lengths.copy()
             cap = npts.value
 
+    def geodesics_ode(self, tips, step=0.25, causality_tolerance=4.0):
+        """Geodesics from `tips` by the ODE solver (GeodesicODESolver.hxx:156-217, midpoint steps
+        along the interpolated flow): (geodesicPoints (M x dim), geodesicLengths (ntips),
+        failed (ntips bools: the tips the reference reports as failed))."""
+        torch = _torch()
+        t = np.ascontiguousarray(np.asarray(tips, np.float64).reshape(-1, self.dim))
+        lengths = np.zeros(len(t), np.int64)
+        failed = np.zeros(len(t), np.int32)
+        npts = ctypes.c_int64(0)
+        cap = 1 << 14
+        while True:
+            pts = np.zeros((cap, self.dim))
+            self._check(self._L.hfm_plan_geodesics_ode(
+                self._h, len(t), _dptr(t), float(step), float(causality_tolerance), _dptr(pts), cap,
+                lengths.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), ctypes.byref(npts),
+                failed.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _stream_ptr(torch)))
+            if npts.value <= cap:
+                return pts[:npts.value].copy(), lengths.copy(), failed.astype(bool)
+            cap = npts.value
+
     def adjoint(self, targets, weights=None, max_seeds=4096):
         torch = _torch()
         t = np.ascontiguousarray(np.asarray(targets, np.float64).reshape(-1, self.dim))

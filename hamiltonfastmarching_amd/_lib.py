@@ -78,6 +78,8 @@ SIGNATURES = [
     ("hfm_plan_active_offsets", c_int, [c_void_p, c_void_p, c_void_p]),
     ("hfm_plan_geodesics", c_int, [c_void_p, c_int, P(c_double), c_double, c_double, c_double, P(c_double),
                                    c_int64, P(c_int64), P(c_int64), P(c_int), c_void_p]),
+    ("hfm_plan_geodesics_ode", c_int, [c_void_p, c_int, P(c_double), c_double, c_double, P(c_double), c_int64,
+                                       P(c_int64), P(c_int64), P(c_int), c_void_p]),
     ("hfm_plan_n_active_slots", c_int, [c_void_p]),
     ("hfm_plan_residual", c_int, [c_void_p, P(c_double), P(c_int64), c_void_p]),
     ("hfm_plan_adjoint", c_int, [c_void_p, c_int, P(c_double), P(c_double), c_void_p, P(c_int64), P(c_double),

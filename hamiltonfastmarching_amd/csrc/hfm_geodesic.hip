@@ -330,6 +330,97 @@ __device__ int discrete_flow(const Geom &g, const StencilData<T> &sd, const T *u
     return n;
 }
 
+// The discrete flow of a block solve at one point (tiled index p, coordinates c): the
+// differences the local solver accepts among those below u (the flow export's active set), each
+// with its stencil offset and weight, the value on its chosen side sv and that side, and the
+// recomputed value (HopfLaxRecompute at order 1, EulerianStencil.hxx:155-233).  Returns the
+// used differences as a bit mask (0: a seed, an unreached point or a point past the grid).
+template <int ModelT, typename T>
+__device__ inline uint32_t block_point_flow(const Geom &g, const StencilData<T> &sd, const T *u,
+                                            const unsigned long long *fixedmask, long long p, const int *c,
+                                            bool inside, uint32_t *off, T *w, double *sv, int *side, double &value) {
+    using S = ModelShape<ModelT>;
+    constexpr int NF = S::NFWD, NC = NF + S::NSYM;
+    const T inf = dev_inf<T>();
+    const T uu = u[p];
+    const bool fixed = (fixedmask[p >> 6] >> (p & 63)) & 1ull;
+    const uint32_t vm = sd.vis ? sd.vis[p] : 0xffffffffu;
+    const int th = inside ? c[2] : 0;  // theta row of a shared stencil (points past the grid: any row)
+#pragma unroll
+    for (int k = 0; k < NC; ++k) load_slot<ModelT, T>(sd, p, th, k, off[k], w[k]);
+    // zero-weight differences are active only through a nonzero-weight one with their offset
+    // (shared stencils); rare, so the table is read again rather than kept in registers
+    auto shares = [&](int k, int a0, int a1, int a2) {
+        for (int j = 0; j < NC; ++j) {
+            if (j == k) continue;
+            uint32_t oj;
+            T wj;
+            load_slot<ModelT, T>(sd, p, th, j, oj, wj);
+            if ((double)wj == 0.) continue;
+            int p0, p1, p2;
+            unpack_offset(oj, p0, p1, p2);
+            if (p0 == a0 && p1 == a1 && p2 == a2) return true;
+            if (j >= NF && p0 == -a0 && p1 == -a1 && p2 == -a2) return true;
+        }
+        return false;
+    };
+    bool pa[NC], pb[NC];
+    T va[NC], vb[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        int o0, o1, o2, r[3];
+        unpack_offset(off[k], o0, o1, o2);
+        const bool nz = (double)w[k] != 0.;
+        pa[k] = inside && (vm >> side_bit<NF>(k, 0) & 1u) && (nz || (S::SHARED && shares(k, o0, o1, o2)));
+        pb[k] = inside && k >= NF && (vm >> side_bit<NF>(k, 1) & 1u) &&
+                (nz || (S::SHARED && shares(k, -o0, -o1, -o2)));
+        pa[k] = pa[k] && shift_coords_m<ModelT>(g, c, o0, o1, o2, r);
+        va[k] = u[pa[k] ? (long long)tile_of(g, r) * 64 + local_of(g, r) : p];
+        if (k >= NF) {
+            pb[k] = pb[k] && shift_coords_m<ModelT>(g, c, -o0, -o1, -o2, r);
+            vb[k] = u[pb[k] ? (long long)tile_of(g, r) * 64 + local_of(g, r) : p];
+        } else {
+            vb[k] = inf;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        const T a = pa[k] ? va[k] : inf, b = pb[k] ? vb[k] : inf;
+        sv[k] = (double)(a <= b ? a : b);
+        side[k] = a <= b ? 1 : -1;
+    }
+    uint32_t used = 0;
+    if (inside && !fixed && uu < inf) {
+        T vv[NC], ww[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            const T v = (T)sv[k];
+            vv[k] = v < uu ? v : inf;
+            ww[k] = w[k];
+        }
+        used = active_slots<NC, T>(vv, ww);
+    }
+    value = 0;
+    if (used) {
+        double m = __builtin_huge_val();
+#pragma unroll
+        for (int k = 0; k < NC; ++k)
+            if (used >> k & 1u) m = sv[k] < m ? sv[k] : m;
+        double a = 0, b = 0, cc = -1;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            if (!(used >> k & 1u)) continue;
+            const double wk = (double)w[k], vr = sv[k] - m;
+            a += wk;
+            b += wk * vr;
+            cc += (wk * vr) * vr;
+        }
+        const double d2 = b * b - a * cc;
+        value = m + (b + sqrt(d2 > 0. ? d2 : 0.)) / a;
+    }
+    return used;
+}
+
 // The discrete flow of the 64 points of a tile, one lane per point, into cache slot `slot`: the
 // same active set and weights as discrete_flow, in the same operations.  Every lane issues its
 // stencil loads, then all its neighbour-value loads, before it uses any of them.
@@ -381,91 +472,16 @@ __device__ void fill_flow(const Geom &g, const StencilData<T> &sd, const T *u, c
             return;
         }
     }
-    const T inf = dev_inf<T>();
     int c[3];
     tiled_coords(g, tile, lane, c);
     const bool inside = c[0] < g.n[0] && c[1] < g.n[1] && c[2] < g.n[2];
     const long long p = (long long)tile * 64 + lane;
-    const T uu = u[p];
-    const bool fixed = (fixedmask[tile] >> lane) & 1ull;
-    const uint32_t vm = sd.vis ? sd.vis[p] : 0xffffffffu;
     uint32_t off[NC];
     T w[NC];
-    const int th = inside ? c[2] : 0;  // theta row of a shared stencil (points past the grid: any row)
-#pragma unroll
-    for (int k = 0; k < NC; ++k) load_slot<ModelT, T>(sd, p, th, k, off[k], w[k]);
-    // zero-weight differences are active only through a nonzero-weight one with their offset
-    // (shared stencils); rare, so the table is read again rather than kept in registers
-    auto shares = [&](int k, int a0, int a1, int a2) {
-        for (int j = 0; j < NC; ++j) {
-            if (j == k) continue;
-            uint32_t oj;
-            T wj;
-            load_slot<ModelT, T>(sd, p, th, j, oj, wj);
-            if ((double)wj == 0.) continue;
-            int p0, p1, p2;
-            unpack_offset(oj, p0, p1, p2);
-            if (p0 == a0 && p1 == a1 && p2 == a2) return true;
-            if (j >= NF && p0 == -a0 && p1 == -a1 && p2 == -a2) return true;
-        }
-        return false;
-    };
-    bool pa[NC], pb[NC];
-    T va[NC], vb[NC];
-#pragma unroll
-    for (int k = 0; k < NC; ++k) {
-        int o0, o1, o2, r[3];
-        unpack_offset(off[k], o0, o1, o2);
-        const bool nz = (double)w[k] != 0.;
-        pa[k] = inside && (vm >> side_bit<NF>(k, 0) & 1u) && (nz || (S::SHARED && shares(k, o0, o1, o2)));
-        pb[k] = inside && k >= NF && (vm >> side_bit<NF>(k, 1) & 1u) &&
-                (nz || (S::SHARED && shares(k, -o0, -o1, -o2)));
-        pa[k] = pa[k] && shift_coords_m<ModelT>(g, c, o0, o1, o2, r);
-        va[k] = u[pa[k] ? (long long)tile_of(g, r) * 64 + local_of(g, r) : p];
-        if (k >= NF) {
-            pb[k] = pb[k] && shift_coords_m<ModelT>(g, c, -o0, -o1, -o2, r);
-            vb[k] = u[pb[k] ? (long long)tile_of(g, r) * 64 + local_of(g, r) : p];
-        } else {
-            vb[k] = inf;
-        }
-    }
     double sv[NC];
     int side[NC];
-#pragma unroll
-    for (int k = 0; k < NC; ++k) {
-        const T a = pa[k] ? va[k] : inf, b = pb[k] ? vb[k] : inf;
-        sv[k] = (double)(a <= b ? a : b);
-        side[k] = a <= b ? 1 : -1;
-    }
-    uint32_t used = 0;
-    if (inside && !fixed && uu < inf) {
-        T vv[NC], ww[NC];
-#pragma unroll
-        for (int k = 0; k < NC; ++k) {
-            const T v = (T)sv[k];
-            vv[k] = v < uu ? v : inf;
-            ww[k] = w[k];
-        }
-        used = active_slots<NC, T>(vv, ww);
-    }
     double value = 0;
-    if (used) {
-        double m = __builtin_huge_val();
-#pragma unroll
-        for (int k = 0; k < NC; ++k)
-            if (used >> k & 1u) m = sv[k] < m ? sv[k] : m;
-        double a = 0, b = 0, cc = -1;
-#pragma unroll
-        for (int k = 0; k < NC; ++k) {
-            if (!(used >> k & 1u)) continue;
-            const double wk = (double)w[k], vr = sv[k] - m;
-            a += wk;
-            b += wk * vr;
-            cc += (wk * vr) * vr;
-        }
-        const double d2 = b * b - a * cc;
-        value = m + (b + sqrt(d2 > 0. ? d2 : 0.)) / a;
-    }
+    const uint32_t used = block_point_flow<ModelT, T>(g, sd, u, fixedmask, p, c, inside, off, w, sv, side, value);
     const int n = __builtin_popcount(used);
     int st = lane * NC;
     bool fits = true;
@@ -869,6 +885,296 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// The ODE geodesic solver (geodesicSolver = "ODE"; Headers/Base/GeodesicODESolver.h:17-40,
+// Implementation/GeodesicODESolver.hxx:10-261), one wavefront per tip.  From the tip it takes
+// midpoint steps of geodesicStep along the normalised flow.  The flow at a point averages the
+// GeodesicFlow of the 2^d surrounding grid points (HamiltonFastMarching.hxx:363-378: the
+// recompute's value, its width and sum_i weight_i offset_i) with the interpolation weights, over
+// the corners whose value lies within causalityTolerance widths of the smallest; lane j handles
+// corner j and lane order is the corners' order in every sum.  The walk stops at a seed (zero
+// flow), when the L-infinity grid distance to the seeds (capped at 6) has grown against
+// sqrt(d) / step steps before, or when a corner's flow has been read more than 6 sqrt(d) / step
+// times (a stall: the tip fails).  The read counts live in a per-tip hash table in device memory.
+// ---------------------------------------------------------------------------------------------
+constexpr int ODE_HIST = 1024;        // distance history: sqrt(d) / geodesicStep entries at most
+constexpr int ODE_DIST_MAX = 32767;   // the reference's unreached distance (largest short)
+enum { GEO_HASH_FULL = 6, GEO_HISTORY = 7 };
+
+// One level of LInfDistance (GeodesicODESolver.hxx:221-258; upper bound 6, the value the
+// solver's constructor uses): a point not yet reached becomes level + 1 when one of its 3^d - 1
+// neighbours is at level.  Points set in this pass are at level + 1, so updating in place adds
+// no second step.
+template <int ModelT>
+__global__ __launch_bounds__(256) void k_ode_dist(Geom g, int *__restrict__ dist, int level, long long np) {
+    constexpr int D = ModelShape<ModelT>::DIM;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    int c[3];
+    tiled_coords(g, (int)(p >> 6), (int)(p & 63), c);
+    if (!in_grid(g, c) || dist[p] != ODE_DIST_MAX) return;
+    constexpr int NOFF = D == 2 ? 9 : 27;
+    for (int k = 0; k < NOFF; ++k) {
+        const int o0 = k % 3 - 1, o1 = (k / 3) % 3 - 1, o2 = D == 3 ? k / 9 - 1 : 0;
+        if (o0 == 0 && o1 == 0 && o2 == 0) continue;
+        int r[3];
+        if (!shift_coords_m<ModelT>(g, c, o0, o1, o2, r)) continue;
+        if (dist[(long long)tile_of(g, r) * 64 + local_of(g, r)] == level) {
+            dist[p] = level + 1;
+            return;
+        }
+    }
+}
+
+__global__ void k_ode_dist_init(int *__restrict__ dist, long long np) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < np) dist[p] = ODE_DIST_MAX;
+}
+
+__global__ void k_ode_dist_seeds(int *__restrict__ dist, const long long *__restrict__ seeds, int nseeds) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nseeds) dist[seeds[i]] = 0;
+}
+
+// GeodesicFlow at grid point q (coordinates c): value, width and flow vector.
+template <int ModelT, typename T>
+__device__ inline void ode_point_flow(const Geom &g, const StencilData<T> &sd, const T *u,
+                                      const unsigned long long *fixedmask, const ExactFlow &xf, long long q,
+                                      const int *c, double &value, double &width, double *f) {
+    using S = ModelShape<ModelT>;
+    constexpr int NF = S::NFWD, NC = NF + S::NSYM;
+    f[0] = f[1] = f[2] = 0.;
+    value = (double)u[q];
+    width = 0.;
+    if constexpr (std::is_same<T, double>::value) {
+        if (xf.on) {  // after a serial solve: its recompute over its accepted set
+            const unsigned long long act = xf.active[q];
+            if (!act) return;
+            RecomputeEntry re[2 * NC];
+            double r = 0.;
+            const int n = recompute<ModelT>(g, sd, u, xf.acc, q, c, act, xf.order, xf.fac, re, &r);
+            constexpr double mult[4] = {0., 1., 3. / 2., 11. / 6.};
+            double wid = 0., ws = 0.;
+            for (int i = 0; i < n; ++i) {
+                const double d = r - re[i].value;
+                const double wpd = re[i].w * (d > 0. ? d : 0.);
+                wid += wpd;
+                const double fw = wpd * mult[re[i].ord];
+                for (int k = 0; k < 3; ++k) f[k] += fw * (double)re[i].o[k];
+            }
+            for (int i = 0; i < n; ++i) ws += re[i].w;
+            if (ws > 0) wid /= ws;
+            value = r;
+            width = wid;
+            return;
+        }
+    }
+    uint32_t off[NC];
+    T w[NC];
+    double sv[NC];
+    int side[NC];
+    double r = 0.;
+    const uint32_t used = block_point_flow<ModelT, T>(g, sd, u, fixedmask, q, c, true, off, w, sv, side, r);
+    if (!used) return;
+    double wid = 0., ws = 0.;
+    for (int k = 0; k < NC; ++k) {
+        if (!(used >> k & 1u)) continue;
+        const double d = r - sv[k];
+        const double wpd = (double)w[k] * (d > 0. ? d : 0.);
+        wid += wpd;
+        int o[3];
+        unpack_offset(off[k], o[0], o[1], o[2]);
+        for (int j = 0; j < 3; ++j) f[j] += wpd * (double)(side[k] * o[j]);
+    }
+    for (int k = 0; k < NC; ++k)
+        if (used >> k & 1u) ws += (double)w[k];
+    if (ws > 0) wid /= ws;
+    value = r;
+    width = wid;
+}
+
+template <int ModelT, typename T>
+__global__ __launch_bounds__(64) void k_geodesic_ode(Geom g, StencilData<T> sd, const T *__restrict__ u,
+                                                     const unsigned long long *__restrict__ fixedmask, ExactFlow xf,
+                                                     GeoParams gp, double causality, const int *__restrict__ dist,
+                                                     unsigned long long *__restrict__ hkeys, int *__restrict__ hcnt,
+                                                     int hlog2, const double *__restrict__ tips,
+                                                     double *__restrict__ points, long long *__restrict__ lengths,
+                                                     int *__restrict__ status) {
+    constexpr int D = ModelShape<ModelT>::DIM;
+    constexpr int NCORNER = 1 << D;
+    __shared__ long long hist[ODE_HIST];
+    const int tip = blockIdx.x, lane = threadIdx.x;
+    const long long hcap = 1LL << hlog2;
+    unsigned long long *keys = hkeys + (long long)tip * hcap;
+    int *cnts = hcnt + (long long)tip * hcap;
+    double p[3] = {0., 0., 0.};
+    for (int c = 0; c < D; ++c) p[c] = tips[tip * D + c];
+    long long npts = 0;
+    int st = GEO_OK;
+    auto emit = [&](const double *q) {
+        if (lane == 0 && npts < gp.max_points)
+            for (int c = 0; c < D; ++c) points[((long long)tip * gp.max_points + npts) * D + c] = q[c] * gp.scale[c] + gp.origin[c];
+        ++npts;
+    };
+    emit(p);
+    const long long nStatMax = (long long)((6. * sqrt((double)D)) / gp.step);
+    int K = 0;
+    while (K <= ODE_HIST && K < sqrt((double)D) / gp.step) ++K;
+    if (K > ODE_HIST) {
+        if (lane == 0) {
+            lengths[tip] = npts;
+            status[tip] = GEO_HISTORY;
+        }
+        return;
+    }
+    for (int i = lane; i < K; i += 64) hist[i] = 2147483647LL;
+    __syncthreads();
+    int head = 0;
+    // the flow at x, in every lane; false if the count table is full
+    auto flow_at = [&](const double *x, double *fo, long long &nStat, long long &tol) {
+        long long pIndex[3] = {0, 0, 0};
+        double w[3] = {0., 0., 0.}, sgn[3] = {0., 0., 0.};
+        for (int i = 0; i < D; ++i) {
+            pIndex[i] = (long long)floor(x[i]);
+            w[i] = x[i] - (double)pIndex[i] - 0.5;
+            sgn[i] = w[i] > 0 ? 1 : -1;
+            w[i] = fabs(w[i]);
+        }
+        double cw = 0., cval = __builtin_huge_val(), cwid = 0., cf[3] = {0., 0., 0.};
+        long long ctol = 2147483647LL, cstat = 0;
+        bool full = false;
+        if (lane < NCORNER) {
+            int base[3] = {0, 0, 0}, off[3] = {0, 0, 0};
+            bool inrange = true;
+            for (int j = 0; j < D; ++j) {
+                off[j] = (lane >> j & 1) ? (int)sgn[j] : 0;
+                inrange = inrange && pIndex[j] > -(1LL << 30) && pIndex[j] < (1LL << 30);
+                base[j] = inrange ? (int)pIndex[j] : 0;
+            }
+            int r[3];
+            if (inrange && shift_coords_m<ModelT>(g, base, off[0], off[1], off[2], r)) {
+                const long long q = (long long)tile_of(g, r) * 64 + local_of(g, r);
+                ctol = dist[q];
+                // the corner's read count: linear probing on key q + 1 (0 marks an empty slot)
+                const unsigned long long key = (unsigned long long)q + 1ull;
+                long long h = (long long)((key * 0x9E3779B97F4A7C15ull) >> (64 - hlog2));
+                long long slot = -1;
+                for (long long probe = 0; probe < hcap; ++probe) {
+                    const unsigned long long k = keys[h];
+                    if (k == key) {
+                        slot = h;
+                        break;
+                    }
+                    if (k == 0ull) {
+                        const unsigned long long prev = atomicCAS(&keys[h], 0ull, key);
+                        if (prev == 0ull || prev == key) {
+                            slot = h;
+                            break;
+                        }
+                    }
+                    h = (h + 1) & (hcap - 1);
+                }
+                if (slot < 0) {
+                    full = true;
+                } else {
+                    cstat = atomicAdd(&cnts[slot], 1);
+                    cw = 1.;
+                    for (int j = 0; j < D; ++j) cw *= off[j] ? w[j] : (1. - w[j]);
+                    ode_point_flow<ModelT, T>(g, sd, u, fixedmask, xf, q, r, cval, cwid, cf);
+                }
+            }
+        }
+        // the corners in order: smallest value (first on ties), threshold, weighted mean
+        double minVal = __builtin_huge_val(), thrW = 0.;
+        long long mstat = 0, mtol = 2147483647LL;
+        bool anyfull = false;
+        double wv[NCORNER], vv[NCORNER], fv[NCORNER][3];
+        for (int i = 0; i < NCORNER; ++i) {
+            wv[i] = __shfl(cw, i);
+            vv[i] = __shfl(cval, i);
+            const double wd = __shfl(cwid, i);
+            for (int k = 0; k < 3; ++k) fv[i][k] = __shfl(cf[k], i);
+            const long long si = __shfl(cstat, i), ti = __shfl(ctol, i);
+            mstat = si > mstat ? si : mstat;
+            mtol = ti < mtol ? ti : mtol;
+            anyfull = anyfull || __shfl((int)full, i);
+            if (wv[i] > 0 && vv[i] < minVal) {
+                minVal = vv[i];
+                thrW = wd;
+            }
+        }
+        nStat = mstat;
+        tol = mtol;
+        fo[0] = fo[1] = fo[2] = 0.;
+        if (minVal == __builtin_huge_val()) return !anyfull;
+        const double thr = minVal + causality * thrW;
+        double wsum = 0.;
+        for (int i = 0; i < NCORNER; ++i) {
+            if (wv[i] <= 0) continue;
+            if (vv[i] <= thr) {
+                wsum += wv[i];
+                for (int k = 0; k < D; ++k) fo[k] += wv[i] * fv[i][k];
+            }
+        }
+        if (wsum > 0) {
+            const double b = 1. / wsum;
+            for (int k = 0; k < D; ++k) fo[k] *= b;
+        }
+        return !anyfull;
+    };
+    auto norm = [&](const double *f) {
+        double s2 = 0.;
+        for (int k = 0; k < D; ++k) s2 += f[k] * f[k];
+        return sqrt(s2);
+    };
+    while (true) {
+        double f[3];
+        long long nStat, tol;
+        if (!flow_at(p, f, nStat, tol)) {
+            st = GEO_HASH_FULL;
+            break;
+        }
+        const long long old = hist[head];
+        __syncthreads();
+        if (lane == 0) hist[head] = tol;
+        __syncthreads();
+        head = head + 1 == K ? 0 : head + 1;
+        if (old < tol) break;  // the walk moves away from the seeds
+        if (nStat > nStatMax) {
+            st = GEO_FAILED;  // stall
+            break;
+        }
+        double fn = norm(f);
+        if (fn == 0) break;  // at a seed
+        double q[3] = {p[0], p[1], p[2]};
+        {
+            const double a = 0.5 * gp.step, b = 1. / fn;
+            for (int k = 0; k < D; ++k) q[k] = p[k] + (f[k] * a) * b;
+        }
+        long long ns2, tol2;  // the midpoint's counts and distance are not used
+        if (!flow_at(q, f, ns2, tol2)) {
+            st = GEO_HASH_FULL;
+            break;
+        }
+        fn = norm(f);
+        if (fn == 0) break;
+        {
+            const double b = 1. / fn;
+            for (int k = 0; k < D; ++k) p[k] = p[k] + (f[k] * gp.step) * b;
+        }
+        emit(p);
+        if (npts > 64 * gp.max_points) {  // far beyond the host's buffer: give up
+            st = GEO_STEPS_OVERFLOW;
+            break;
+        }
+    }
+    if (lane == 0) {
+        lengths[tip] = npts;
+        status[tip] = st;
+    }
+}
+
 }  // namespace
 
 template <int M, typename T>
@@ -903,6 +1209,76 @@ hipError_t launch_geodesics(int model, const Geom &g, const StencilData<T> &sd, 
     }
     return hipGetLastError();
 }
+template <int M, typename T>
+static void geo_ode_one(const Geom &g, const StencilData<T> &sd, const T *u, const unsigned long long *fm,
+                        const ExactFlow &xf, const GeoParamsHost &h, double causality, const int *dist,
+                        unsigned long long *hkeys, int *hcnt, int hlog2, int ntips, const double *tips, double *points,
+                        long long *lengths, int *status, hipStream_t s) {
+    GeoParams gp;
+    gp.step = h.step;
+    gp.threshold = h.threshold;
+    gp.volumeBound = h.volumeBound;
+    for (int i = 0; i < 3; ++i) {
+        gp.scale[i] = h.scale[i];
+        gp.origin[i] = h.origin[i];
+    }
+    gp.max_points = h.max_points;
+    gp.max_iter = h.max_iter;
+    hipLaunchKernelGGL((k_geodesic_ode<M, T>), dim3(ntips), dim3(64), 0, s, g, sd, u, fm, xf, gp, causality, dist,
+                       hkeys, hcnt, hlog2, tips, points, lengths, status);
+}
+
+template <typename T>
+hipError_t launch_geodesics_ode(int model, const Geom &g, const StencilData<T> &sd, const T *u,
+                                const unsigned long long *fm, const ExactFlow &xf, const GeoParamsHost &h,
+                                double causality, const int *dist, unsigned long long *hkeys, int *hcnt, int hlog2,
+                                int ntips, const double *tips, double *points, long long *lengths, int *status,
+                                hipStream_t s) {
+    if (ntips <= 0) return hipSuccess;
+#define HFM_ODE_CASE(M)                                                                                        \
+    case M:                                                                                                    \
+        geo_ode_one<M, T>(g, sd, u, fm, xf, h, causality, dist, hkeys, hcnt, hlog2, ntips, tips, points,     \
+                          lengths, status, s);                                                                 \
+        break;
+    switch (model) {
+        HFM_ODE_CASE(MODEL_RIEMANN2)
+        HFM_ODE_CASE(MODEL_RIEMANN3)
+        HFM_ODE_CASE(MODEL_RSF2)
+        HFM_ODE_CASE(MODEL_ELASTICA2)
+        default: return hipErrorInvalidValue;
+    }
+#undef HFM_ODE_CASE
+    return hipGetLastError();
+}
+
+hipError_t launch_ode_distance(int model, const Geom &g, int *dist, const long long *seeds, int nseeds, long long np,
+                               hipStream_t s) {
+    const unsigned nb = (unsigned)((np + 255) / 256);
+    hipLaunchKernelGGL(k_ode_dist_init, dim3(nb), dim3(256), 0, s, dist, np);
+    if (nseeds > 0)
+        hipLaunchKernelGGL(k_ode_dist_seeds, dim3((unsigned)((nseeds + 255) / 256)), dim3(256), 0, s, dist, seeds,
+                           nseeds);
+    for (int level = 0; level < 6; ++level) {
+        switch (model) {
+            case MODEL_RIEMANN2: hipLaunchKernelGGL((k_ode_dist<MODEL_RIEMANN2>), dim3(nb), dim3(256), 0, s, g, dist, level, np); break;
+            case MODEL_RIEMANN3: hipLaunchKernelGGL((k_ode_dist<MODEL_RIEMANN3>), dim3(nb), dim3(256), 0, s, g, dist, level, np); break;
+            case MODEL_RSF2: hipLaunchKernelGGL((k_ode_dist<MODEL_RSF2>), dim3(nb), dim3(256), 0, s, g, dist, level, np); break;
+            case MODEL_ELASTICA2: hipLaunchKernelGGL((k_ode_dist<MODEL_ELASTICA2>), dim3(nb), dim3(256), 0, s, g, dist, level, np); break;
+            default: return hipErrorInvalidValue;
+        }
+    }
+    return hipGetLastError();
+}
+
+template hipError_t launch_geodesics_ode<float>(int, const Geom &, const StencilData<float> &, const float *,
+                                                const unsigned long long *, const ExactFlow &, const GeoParamsHost &,
+                                                double, const int *, unsigned long long *, int *, int, int,
+                                                const double *, double *, long long *, int *, hipStream_t);
+template hipError_t launch_geodesics_ode<double>(int, const Geom &, const StencilData<double> &, const double *,
+                                                 const unsigned long long *, const ExactFlow &, const GeoParamsHost &,
+                                                 double, const int *, unsigned long long *, int *, int, int,
+                                                 const double *, double *, long long *, int *, hipStream_t);
+
 template hipError_t launch_geodesics<float>(int, const Geom &, const StencilData<float> &, const float *,
                                             const unsigned long long *, const ExactFlow &, const GeoParamsHost &, int,
                                             const double *, double *, long long *, int *, hipStream_t);

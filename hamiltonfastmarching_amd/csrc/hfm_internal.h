@@ -120,6 +120,12 @@ struct hfm_plan {
     double *geo_tips = nullptr, *geo_pts = nullptr;
     long long *geo_len = nullptr;
     int *geo_st = nullptr;
+    // the ODE geodesic solver: the seeds' L-infinity distance and the per-tip read-count tables
+    int *ode_dist = nullptr;
+    unsigned long long *ode_hkeys = nullptr;
+    int *ode_hcnt = nullptr;
+    long long ode_hcap = 0;
+    int ode_hlog2 = 15;
     long long geo_tip_cap = 0, geo_pts_cap = 0;
     char *geo_host = nullptr;
     size_t geo_host_cap = 0;
@@ -208,7 +214,8 @@ int hfm_plan_curvature_tables(const hfm_plan *p, std::vector<uint32_t> &offs, st
 // hfm_plan_geodesics with tips already in grid units (ADim'd) when tips_adim.
 int hfm_plan_geodesics_impl(hfm_plan *p, int ntips, const double *tips, bool tips_adim, double step,
                             double threshold, double volumeBound, double *points, int64_t max_points,
-                            int64_t *lengths, int64_t *npoints, int *nfailed, void *stream);
+                            int64_t *lengths, int64_t *npoints, int *nfailed, void *stream, int solver = 0,
+                            double causality = 4., int *tip_failed = nullptr);
 
 namespace hfm {
 // Number of reached grid points (finite values) of the last solve, counted on the device.

@@ -48,7 +48,8 @@ int plan_count_reached(hfm_plan *p, int64_t *n, void *stream);
 // Geodesics from tips already in grid units (hfm_plan.hip).
 int hfm_plan_geodesics_impl(hfm_plan *p, int ntips, const double *tips, bool tips_adim, double step,
                             double threshold, double volumeBound, double *points, int64_t max_points,
-                            int64_t *lengths, int64_t *npoints, int *nfailed, void *stream);
+                            int64_t *lengths, int64_t *npoints, int *nfailed, void *stream, int solver = 0,
+                            double causality = 4., int *tip_failed = nullptr);
 
 namespace {
 
@@ -1228,26 +1229,53 @@ void run_impl(hfm_io *io) {
     const double tg = wall();
     bool anyGeodesic = false;
     auto export_geodesics = [&](const std::string &suffix, const std::vector<double> &tips, bool tips_adim) {
+        // HFMInterface.hxx:459-464: 'Discrete' (the default) or 'ODE'
         const std::string solver = io->str("geodesicSolver", "Discrete");
-        if (solver != "Discrete")
-            throw IOError{"Error : geodesic solver " + solver + " is not available (only 'Discrete')."};
+        if (solver != "Discrete" && solver != "ODE")
+            throw IOError{"Error : Unrecognized geodesic solver " + solver + ".\n"};
+        const bool ode = solver == "ODE";
         const int ntips = (int)(tips.size() / dim);
         const double step = io->scalar("geodesicStep", 0.25);
-        const double thr = io->scalar("geodesicWeightThreshold", 0.001);
-        const double vb = io->scalar("geodesicVolumeBound", 5. * std::pow(1.3, dim));
+        double thr = 0., vb = 0., causality = 4.;
+        if (ode) {
+            // GeodesicODESolver.hxx:17-23; the target tolerance is read but the solver keeps the
+            // bound 6 its constructor gave the seeds' distance
+            causality = io->scalar("geodesicCausalityTolerance", 4.);
+            (void)io->scalar("geodesicTargetTolerance", 6.);
+        } else {
+            thr = io->scalar("geodesicWeightThreshold", 0.001);
+            vb = io->scalar("geodesicVolumeBound", 5. * std::pow(1.3, dim));
+        }
         std::vector<int64_t> lens(ntips);
         std::vector<double> pts((size_t)4096 * dim);
+        std::vector<int> tipFailed(ntips, 0);
         int64_t npts = 0;
         int nfail = 0;
         check(hfm_plan_geodesics_impl(plan, ntips, tips.data(), tips_adim, step, thr, vb, pts.data(), 4096,
-                                      lens.data(), &npts, &nfail, nullptr));
+                                      lens.data(), &npts, &nfail, nullptr, ode ? 1 : 0, causality,
+                                      tipFailed.data()));
         if (npts > 4096) {
             pts.resize((size_t)npts * dim);
             check(hfm_plan_geodesics_impl(plan, ntips, tips.data(), tips_adim, step, thr, vb, pts.data(), npts,
-                                          lens.data(), &npts, &nfail, nullptr));
+                                          lens.data(), &npts, &nfail, nullptr, ode ? 1 : 0, causality,
+                                          tipFailed.data()));
+        }
+        if (ode) {
+            // GeodesicODESolver.hxx:32-36: one message per failed tip, the tip in physical units
+            int64_t first = 0;
+            for (int t = 0; t < ntips; ++t) {
+                if (tipFailed[t]) {
+                    std::ostringstream os;
+                    os << "Geodesic ODE solver seems to have failed for tip {";
+                    for (int c = 0; c < dim; ++c) os << (c ? "," : "") << pts[(size_t)first * dim + c];
+                    os << "}.\n";
+                    io->log += os.str();
+                }
+                first += lens[t];
+            }
         }
         pts.resize((size_t)npts * dim);
-        if (nfail && io->verbosity >= 1)
+        if (nfail && !ode && io->verbosity >= 1)
             io->log += "Geodesic extraction failed for " + std::to_string(nfail) + " tip(s)\n";
         const std::string sep = suffix.empty() ? "" : "_";
         Field fp;

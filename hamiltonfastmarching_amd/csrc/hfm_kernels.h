@@ -137,6 +137,18 @@ template <typename T>
 hipError_t launch_geodesics(int model, const Geom &g, const StencilData<T> &sd, const T *u,
                             const unsigned long long *fm, const ExactFlow &xf, const GeoParamsHost &h, int ntips,
                             const double *tips, double *points, long long *lengths, int *status, hipStream_t s);
+// The ODE geodesic solver (hfm_geodesic.hip): dist, the L-infinity grid distance to the seeds
+// capped at 6 (launch_ode_distance), per-tip read-count tables of 2^hlog2 slots (hkeys / hcnt,
+// zeroed by the caller).  Status GEO_STATUS_FAILED marks a stall; 6 a full count table; 7 a step
+// too small for the distance history.
+template <typename T>
+hipError_t launch_geodesics_ode(int model, const Geom &g, const StencilData<T> &sd, const T *u,
+                                const unsigned long long *fm, const ExactFlow &xf, const GeoParamsHost &h,
+                                double causality, const int *dist, unsigned long long *hkeys, int *hcnt, int hlog2,
+                                int ntips, const double *tips, double *points, long long *lengths, int *status,
+                                hipStream_t s);
+hipError_t launch_ode_distance(int model, const Geom &g, int *dist, const long long *seeds, int nseeds, long long np,
+                               hipStream_t s);
 
 template <typename T>
 hipError_t launch_riemann_stencil(const Geom &g, const double *metric, int is_field, int dual, const InvHH &ih,

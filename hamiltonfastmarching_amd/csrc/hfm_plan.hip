@@ -122,6 +122,9 @@ void plan_free(hfm_plan *p) {
     dfree(p->geo_pts);
     dfree(p->geo_len);
     dfree(p->geo_st);
+    dfree(p->ode_dist);
+    dfree(p->ode_hkeys);
+    dfree(p->ode_hcnt);
     if (p->geo_host) (void)hipHostFree(p->geo_host);
     dfree(p->adj_ecnt);
     dfree(p->adj_eidx);
@@ -1195,6 +1198,14 @@ int hfm_plan_geodesics(hfm_plan *p, int ntips, const double *tips, double step, 
                                    npoints, nfailed, stream);
 }
 
+int hfm_plan_geodesics_ode(hfm_plan *p, int ntips, const double *tips, double step, double causality_tolerance,
+                           double *points, int64_t max_points, int64_t *lengths, int64_t *npoints, int *tip_failed,
+                           void *stream) {
+    int nf = 0;
+    return hfm_plan_geodesics_impl(p, ntips, tips, false, step, 0., 0., points, max_points, lengths, npoints, &nf,
+                                   stream, 1, causality_tolerance, tip_failed);
+}
+
 }  // extern "C"
 
 int hfm_plan_curvature_tables(const hfm_plan *p, std::vector<uint32_t> &offs, std::vector<double> &w) {
@@ -1215,13 +1226,18 @@ int hfm_plan_curvature_tables(const hfm_plan *p, std::vector<uint32_t> &offs, st
 
 int hfm_plan_geodesics_impl(hfm_plan *p, int ntips, const double *tips, bool tips_adim, double step,
                             double threshold, double volumeBound, double *points, int64_t max_points,
-                            int64_t *lengths, int64_t *npoints, int *nfailed, void *stream) {
+                            int64_t *lengths, int64_t *npoints, int *nfailed, void *stream, int solver,
+                            double causality, int *tip_failed) {
     if (!p || (ntips > 0 && (!tips || !lengths)) || !npoints) {
         set_error("hfm_plan_geodesics: null argument");
         return 1;
     }
     if (!p->solved) {
         set_error("hfm_plan_geodesics: no solve has run");
+        return 1;
+    }
+    if (solver == 1 && p->walls) {
+        set_error("geodesic ODE solver: not available with walls");
         return 1;
     }
     *npoints = 0;
@@ -1296,14 +1312,46 @@ int hfm_plan_geodesics_impl(hfm_plan *p, int ntips, const double *tips, bool tip
                 xf.fac = p->ex_fac;
                 xf.on = 1;
             }
-            const hipError_t e =
-                p->precision == 32
-                    ? launch_geodesics<float>(p->model, p->g, p->sd<float>(), static_cast<const float *>(p->u),
-                                              p->fixedmask, xf, h, ntips, p->geo_tips, p->geo_pts, p->geo_len,
-                                              p->geo_st, s)
-                    : launch_geodesics<double>(p->model, p->g, p->sd<double>(), static_cast<const double *>(p->u),
-                                               p->fixedmask, xf, h, ntips, p->geo_tips, p->geo_pts, p->geo_len,
-                                               p->geo_st, s);
+            hipError_t e;
+            if (solver == 1) {
+                // the ODE solver: the seeds' L-infinity distance, then per-tip read-count tables
+                if (!p->ode_dist) p->ode_dist = dalloc<int>(p->np);
+                HIPCHECK(launch_ode_distance(p->model, p->g, p->ode_dist, p->seed_p, (int)p->last_seeds.size(),
+                                             p->np, s));
+                int hlog2 = 15;
+                while (hlog2 > 10 && (double)ntips * (double)(1LL << hlog2) * 12. > 512. * 1024. * 1024.) --hlog2;
+                const long long slots = (long long)ntips << hlog2;
+                if (slots > p->ode_hcap) {
+                    dfree(p->ode_hkeys);
+                    dfree(p->ode_hcnt);
+                    p->ode_hkeys = nullptr;
+                    p->ode_hcnt = nullptr;
+                    p->ode_hcap = 0;
+                    p->ode_hkeys = dalloc<unsigned long long>(slots);
+                    p->ode_hcnt = dalloc<int>(slots);
+                    p->ode_hcap = slots;
+                }
+                HIPCHECK(hipMemsetAsync(p->ode_hkeys, 0, slots * sizeof(unsigned long long), s));
+                HIPCHECK(hipMemsetAsync(p->ode_hcnt, 0, slots * sizeof(int), s));
+                p->ode_hlog2 = hlog2;
+                e = p->precision == 32
+                        ? launch_geodesics_ode<float>(p->model, p->g, p->sd<float>(), static_cast<const float *>(p->u),
+                                                      p->fixedmask, xf, h, causality, p->ode_dist, p->ode_hkeys,
+                                                      p->ode_hcnt, hlog2, ntips, p->geo_tips, p->geo_pts, p->geo_len,
+                                                      p->geo_st, s)
+                        : launch_geodesics_ode<double>(p->model, p->g, p->sd<double>(),
+                                                       static_cast<const double *>(p->u), p->fixedmask, xf, h,
+                                                       causality, p->ode_dist, p->ode_hkeys, p->ode_hcnt, hlog2,
+                                                       ntips, p->geo_tips, p->geo_pts, p->geo_len, p->geo_st, s);
+            } else {
+                e = p->precision == 32
+                        ? launch_geodesics<float>(p->model, p->g, p->sd<float>(), static_cast<const float *>(p->u),
+                                                  p->fixedmask, xf, h, ntips, p->geo_tips, p->geo_pts, p->geo_len,
+                                                  p->geo_st, s)
+                        : launch_geodesics<double>(p->model, p->g, p->sd<double>(),
+                                                   static_cast<const double *>(p->u), p->fixedmask, xf, h, ntips,
+                                                   p->geo_tips, p->geo_pts, p->geo_len, p->geo_st, s);
+            }
             HIPCHECK(e);
             long long *hl = reinterpret_cast<long long *>(p->geo_host);
             int *hs = reinterpret_cast<int *>(p->geo_host + (size_t)ntips * sizeof(long long));
@@ -1320,7 +1368,13 @@ int hfm_plan_geodesics_impl(hfm_plan *p, int ntips, const double *tips, bool tip
         int failed = 0;
         std::string err;
         for (int t = 0; t < ntips; ++t) {
+            if (tip_failed) tip_failed[t] = hst[t] == GEO_STATUS_FAILED;
             if (hst[t] == GEO_STATUS_FAILED) ++failed;
+            else if (hst[t] == 6)
+                err = "geodesic ODE solver: more than " + std::to_string(1LL << p->ode_hlog2) +
+                      " distinct grid points read for one tip";
+            else if (hst[t] == 7)
+                err = "geodesic ODE solver: geodesicStep too small (more than 1024 steps of distance history)";
             else if (hst[t] == GEO_STATUS_SET_OVERFLOW)
                 err = "geodesic extraction: the transported mass spread over more than 256 grid points";
             else if (hst[t] == GEO_STATUS_STEPS_OVERFLOW)

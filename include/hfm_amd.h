@@ -212,6 +212,21 @@ int hfm_plan_geodesics(hfm_plan *plan, int ntips, const double *tips, double ste
                        double volumeBound, double *points, int64_t max_points, int64_t *lengths, int64_t *npoints,
                        int *nfailed, void *stream);
 
+/* The ODE geodesic solver (GeodesicODESolver::Run, Headers/Base/Implementation/GeodesicODESolver.hxx
+ * :141-217, flow GeodesicODESolver.hxx:42-140; HFMInterface.hxx:464-475 selects it for
+ * geodesicSolver = "ODE"): midpoint (RK2) steps of length step (grid units) along the flow
+ * interpolated from the 2^d cell corners around the point, each corner's flow taken if its
+ * value is within causality_tolerance widths of the smallest.  Stops at a seed of the last
+ * solve, when the L-infinity grid distance to the seeds (capped at 6) has not decreased over
+ * sqrt(d) / step steps, or (a failure) when a corner's flow has been read more than
+ * 6 sqrt(d) / step times.  tips, points, lengths, npoints as hfm_plan_geodesics;
+ * tip_failed (may be NULL, ntips entries) = 1 for each tip that failed.  Not available with
+ * walls.  Errors: "geodesic ODE solver: ..." when a tip reads more than 32768 distinct grid
+ * points or geodesicStep leaves more than 1024 steps of distance history. */
+int hfm_plan_geodesics_ode(hfm_plan *plan, int ntips, const double *tips, double step, double causality_tolerance,
+                           double *points, int64_t max_points, int64_t *lengths, int64_t *npoints, int *tip_failed,
+                           void *stream);
+
 /* Backward differentiation (FirstVariation::BackwardVariation, FirstVariation.h:341-385):
  * gradient of sum_t weights[t] * u(targets[t]) w.r.t. the cost field (row-major FP64 device
  * array costSensitivity; zero outside the targets' dependency cone).  targets are host

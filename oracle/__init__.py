@@ -71,9 +71,12 @@ def lib():
         L.oracle_last_naccepted.restype = ctypes.c_long
         L.oracle_set_factoring.argtypes = [i32, i32, P(i32), P(i32), P(d), P(d), P(ctypes.c_ubyte)]
         L.oracle_set_factoring.restype = None
+        L.oracle_set_geodesic_solver.argtypes = [i32, d]
+        L.oracle_set_geodesic_solver.restype = None
+        L.oracle_last_geodesic_failures.restype = ctypes.c_long
         L.oracle_geodesics_given.argtypes = [i32, P(i32), P(d), P(d), P(d), i32, i32, d, d, P(d), i32, P(d),
                                              P(ctypes.c_uint64), i32, P(d), d, d, d, P(d), ctypes.c_long, P(i64),
-                                             P(ctypes.c_long)]
+                                             P(ctypes.c_long), i32, P(d)]
         _lib = L
     return _lib
 
@@ -332,8 +335,14 @@ def solve_backward(params, targets, weights=None):
 
 
 def geodesics(params, tips, step=0.25, threshold=0.001, volume_bound=-1.0):
-    """FM + the discrete geodesic solver from each tip: (points (M x dim), lengths (ntips))."""
+    """FM + the geodesic solver from each tip: (points (M x dim), lengths (ntips)).  The discrete
+    solver, or the ODE solver when params['geodesicSolver'] is "ODE" (its step and causality
+    tolerance from geodesicStep / geodesicCausalityTolerance, defaults 0.25 and 4)."""
     (model, dims, dim, N, h, origin, metric, mif, dual, cost, cif, seeds, sv, xi, eps) = _common_args(params)
+    ode = params.get("geodesicSolver", "Discrete") == "ODE"
+    lib().oracle_set_geodesic_solver(int(ode), float(params.get("geodesicCausalityTolerance", 4.0)))
+    if ode:
+        step = float(params.get("geodesicStep", step))
     tips = np.ascontiguousarray(np.asarray(tips, np.float64).reshape(-1, dim))
     lengths = np.zeros(len(tips), np.int64)
     npts = ctypes.c_long(0)
@@ -351,8 +360,14 @@ def geodesics(params, tips, step=0.25, threshold=0.001, volume_bound=-1.0):
         if r:
             raise RuntimeError(_err())
         if npts.value <= cap:
+            lib().oracle_set_geodesic_solver(0, 4.0)
             return pts[:npts.value].copy(), lengths.copy()
         cap = npts.value
+
+
+def last_geodesic_failures() -> int:
+    """Tips of the last ODE run that stalled (the reference reports them as failed)."""
+    return int(lib().oracle_last_geodesic_failures())
 
 
 def forward_variation(params, cost_variation=None, seed_value_variation=None):
@@ -398,8 +413,14 @@ def forward_variation(params, cost_variation=None, seed_value_variation=None):
 
 
 def geodesics_given(params, values, active, tips, step=0.25, threshold=0.001, volume_bound=-1.0):
-    """The discrete geodesic solver on given values / activeNeighs (e.g. another solver's)."""
+    """The geodesic solver on given values / activeNeighs (e.g. another solver's): the discrete
+    one, or the ODE one when params['geodesicSolver'] is "ODE" (as in geodesics; the seeds'
+    distance from params['seeds'])."""
     (model, dims, dim, N, h, origin, metric, mif, dual, cost, cif, seeds, sv, xi, eps) = _common_args(params)
+    ode = params.get("geodesicSolver", "Discrete") == "ODE"
+    lib().oracle_set_geodesic_solver(int(ode), float(params.get("geodesicCausalityTolerance", 4.0)))
+    if ode:
+        step = float(params.get("geodesicStep", step))
     tips = np.ascontiguousarray(np.asarray(tips, np.float64).reshape(-1, dim))
     vals = np.ascontiguousarray(np.asarray(values, np.float64).ravel())
     act = np.ascontiguousarray(np.asarray(active).astype(np.uint64).ravel())
@@ -416,10 +437,12 @@ def geodesics_given(params, values, active, tips, step=0.25, threshold=0.001, vo
                                          _ptr(act, ctypes.c_uint64), len(tips), _ptr(tips, ctypes.c_double),
                                          float(step), float(threshold), float(volume_bound),
                                          _ptr(pts, ctypes.c_double), cap, _ptr(lengths, ctypes.c_int64),
-                                         ctypes.byref(npts))
+                                         ctypes.byref(npts), len(seeds), _ptr(seeds, ctypes.c_double))
         if r:
+            lib().oracle_set_geodesic_solver(0, 4.0)
             raise RuntimeError(_err())
         if npts.value <= cap:
+            lib().oracle_set_geodesic_solver(0, 4.0)
             return pts[:npts.value].copy(), lengths.copy()
         cap = npts.value
 

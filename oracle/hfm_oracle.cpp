@@ -16,6 +16,7 @@
 #include <map>
 #include <queue>
 #include <string>
+#include <deque>
 #include <set>
 #include <vector>
 
@@ -1092,6 +1093,182 @@ bool geodesic_run(const Problem &P, const FMState &S, std::vector<std::array<dou
     return nRestarts == 50;
 }
 
+// ---------------------------------------------------------------------------------------------
+// The ODE geodesic solver (geodesicSolver = "ODE"; Headers/Base/GeodesicODESolver.h:17-40,
+// Implementation/GeodesicODESolver.hxx:10-261): a midpoint (second order Runge-Kutta) walk of step
+// geodesicStep along the normalised flow, the flow at a point being the 2^d corners' GeodesicFlow
+// (HamiltonFastMarching.hxx:363-378) averaged with the interpolation weights over the corners
+// whose value is within causalityTolerance widths of the smallest.  It stops at a seed (zero
+// flow), when the L-infinity grid distance to the seeds (capped at 6) grows again over the last
+// sqrt(d) / step steps, or when some corner's flow has been read more than 6 sqrt(d) / step
+// times (stall, a failure).  Walls are not restated here.
+// ---------------------------------------------------------------------------------------------
+int g_geo_ode = 0;
+double g_geo_causality = 4.;
+long g_geo_failed = 0;  // stalled tips of the last ODE run
+
+struct OdeFlowData {
+    double value, width;
+    double flow[3];
+};
+
+// GeodesicFlow at a grid point: Recompute's value and width (sum of w_i max(0, r - v_i) over the
+// weights' sum) and the flow sum_i weight_i offset_i with the discrete flow's weights; a point
+// without active differences keeps its value, width 0, no flow.
+OdeFlowData ode_point_flow(const Problem &P, const FMState &S, long x) {
+    OdeFlowData f{S.values[x], 0., {0., 0., 0.}};
+    if (!S.active[x]) return f;
+    RecEntry re[2 * kMaxDiff];
+    double value = 0.;
+    const int n = recompute_entries(P, S, x, g_order, re, &value);
+    static const double mult[4] = {0., 1., 3. / 2., 11. / 6.};
+    double width = 0., wsum = 0.;
+    for (int i = 0; i < n; ++i) {
+        const double wpd = re[i].w * std::max(0., value - re[i].value);
+        width += wpd;
+        const double fw = wpd * mult[re[i].ord];
+        for (int c = 0; c < P.dim; ++c) f.flow[c] += fw * double(re[i].off[c]);
+    }
+    for (int i = 0; i < n; ++i) wsum += re[i].w;
+    if (wsum > 0) width /= wsum;
+    f.value = value;
+    f.width = width;
+    return f;
+}
+
+// LInfDistance (GeodesicODESolver.hxx:221-258) with the upper bound 6 the solver's constructor
+// uses: breadth-first over the 3^d - 1 neighbours, points at the bound not expanded, unreached
+// points at the largest short.
+std::vector<int> ode_linf_distance(const Problem &P, const std::vector<long> &seeds) {
+    const int upper = 6, kMax = std::numeric_limits<short>::max();
+    std::vector<int> d(P.N, kMax);
+    std::vector<long> front, next;
+    for (long s : seeds) {
+        d[s] = 0;
+        front.push_back(s);
+    }
+    const int nOff = P.dim == 2 ? 9 : 27;
+    for (int level = 0; level < upper && !front.empty(); ++level) {
+        next.clear();
+        for (long a : front) {
+            int ai[3] = {0, 0, 0};
+            P.unravel(a, ai);
+            for (int k = 0; k < nOff; ++k) {
+                int idx[3] = {ai[0] + k % 3 - 1, ai[1] + (k / 3) % 3 - 1, P.dim == 3 ? ai[2] + k / 9 - 1 : 0};
+                if (k == (nOff - 1) / 2) continue;  // the null offset
+                if (!P.periodize(idx)) continue;
+                const long y = P.ravel(idx);
+                if (d[y] <= level + 1) continue;
+                d[y] = level + 1;
+                next.push_back(y);
+            }
+        }
+        front.swap(next);
+    }
+    return d;
+}
+
+// GeodesicODESolver::Run for one tip: returns true on failure (a stall).
+bool geodesic_ode(const Problem &P, const FMState &S, const std::vector<int> &target,
+                  std::vector<std::array<double, 3>> &geodesic, double step) {
+    const int D = P.dim;
+    std::map<long, std::pair<OdeFlowData, int>> cache;
+    const long nStatMax = long((6 * std::sqrt(double(D))) / step);
+    std::deque<long> history;
+    for (int i = 0; i < std::sqrt(double(D)) / step; ++i) history.push_back(std::numeric_limits<int>::max());
+    struct Avg {
+        double flow[3];
+        long nStat, tol;
+    };
+    auto flowAt = [&](const double *p) {
+        long pIndex[3] = {0, 0, 0};
+        double w[3] = {0, 0, 0}, sign[3] = {0, 0, 0};
+        for (int i = 0; i < D; ++i) {
+            pIndex[i] = (long)std::floor(p[i]);
+            w[i] = p[i] - pIndex[i] - 0.5;
+            sign[i] = w[i] > 0 ? 1 : -1;
+            w[i] = std::abs(w[i]);
+        }
+        long minTol = std::numeric_limits<int>::max(), maxStat = 0;
+        const OdeFlowData *fl[8];
+        double wt[8];
+        for (int i = 0; i < (1 << D); ++i) {
+            int idx[3] = {0, 0, 0};
+            int off[3] = {0, 0, 0};
+            for (int j = 0; j < D; ++j) {
+                off[j] = (i & (1 << j)) ? (int)sign[j] : 0;
+                idx[j] = (int)(pIndex[j] + off[j]);
+            }
+            fl[i] = nullptr;
+            wt[i] = 0.;
+            if (!P.periodize(idx)) continue;
+            const long q = P.ravel(idx);
+            minTol = std::min(minTol, (long)target[q]);
+            auto it = cache.find(q);
+            if (it == cache.end()) it = cache.insert({q, {ode_point_flow(P, S, q), 0}}).first;
+            double weight = 1;
+            for (int j = 0; j < D; ++j) weight *= off[j] ? w[j] : (1 - w[j]);
+            fl[i] = &it->second.first;
+            wt[i] = weight;
+            maxStat = std::max(maxStat, (long)it->second.second);
+            ++it->second.second;
+        }
+        Avg r{{0., 0., 0.}, maxStat, minTol};
+        double minVal = kInf;
+        int iMin = -1;
+        for (int i = 0; i < (1 << D); ++i)
+            if (wt[i] > 0 && fl[i]->value < minVal) {
+                minVal = fl[i]->value;
+                iMin = i;
+            }
+        if (minVal == kInf) return r;
+        const double thr = minVal + g_geo_causality * fl[iMin]->width;
+        double wsum = 0;
+        for (int i = 0; i < (1 << D); ++i) {
+            if (wt[i] <= 0) continue;
+            if (fl[i]->value <= thr) {
+                wsum += wt[i];
+                for (int c = 0; c < D; ++c) r.flow[c] += wt[i] * fl[i]->flow[c];
+            }
+        }
+        if (wsum > 0) {
+            const double b = 1. / wsum;
+            for (int c = 0; c < D; ++c) r.flow[c] *= b;
+        }
+        return r;
+    };
+    auto norm = [&](const double *f) {
+        double s2 = 0;
+        for (int c = 0; c < D; ++c) s2 += f[c] * f[c];
+        return std::sqrt(s2);
+    };
+    while (true) {
+        const std::array<double, 3> p = geodesic.back();
+        Avg fd = flowAt(p.data());
+        const long old = history.front();
+        history.pop_front();
+        history.push_back(fd.tol);
+        if (old < fd.tol) return false;  // moving away from the seeds
+        if (fd.nStat > nStatMax) return true;  // stall
+        double fn = norm(fd.flow);
+        if (fn == 0) return false;  // at a seed
+        std::array<double, 3> q = p;
+        {
+            const double a = 0.5 * step, b = 1. / fn;
+            for (int c = 0; c < D; ++c) q[c] = p[c] + (fd.flow[c] * a) * b;
+        }
+        fd = flowAt(q.data());
+        fn = norm(fd.flow);
+        if (fn == 0) return false;
+        std::array<double, 3> r = p;
+        {
+            const double b = 1. / fn;
+            for (int c = 0; c < D; ++c) r[c] = p[c] + (fd.flow[c] * step) * b;
+        }
+        geodesic.push_back(r);
+    }
+}
+
 // Seeds: ADim then floor, out-of-range seeds dropped, first value kept for duplicates
 // (StaticFactoring.hxx:242-393, BaseGrid.h:61-67).
 std::vector<std::pair<long, double>> make_seeds(const Problem &P, int nseeds, const double *seeds,
@@ -1127,6 +1304,14 @@ void oracle_set_stopping(long max_accepted, double stop_distance, int nany, cons
     g_stop_all_u = std::set<long>(allu, allu + nallu);
 }
 long oracle_last_naccepted(void) { return g_naccepted; }
+
+// Geodesic solver of the geodesics computed next on this thread: ode = 0 the discrete solver,
+// 1 the ODE solver with the given causality tolerance (geodesicCausalityTolerance, default 4).
+void oracle_set_geodesic_solver(int ode, double causality_tolerance) {
+    g_geo_ode = ode;
+    g_geo_causality = causality_tolerance;
+}
+long oracle_last_geodesic_failures(void) { return g_geo_failed; }
 
 // Static factor of the problems solved next on this thread (vals == NULL: none): box extents bd
 // (the grid's when sub = 0), the box's first grid index shift, values and gradients (index units)
@@ -1271,18 +1456,26 @@ int oracle_solve_backward(int model, const int *dims, const double *gridScales, 
 }
 
 static int geodesics_impl(Problem &P, FMState &S, int ntips, const double *tips, double step, double threshold,
-                          double volumeBound, double *points, long max_points, int64_t *lengths, long *npoints) {
+                          double volumeBound, double *points, long max_points, int64_t *lengths, long *npoints,
+                          const std::vector<long> &seeds = std::vector<long>()) {
     GeoParams gp;
     gp.step = step;
     gp.threshold = threshold;
     gp.volumeBound = volumeBound;
     long np = 0;
+    std::vector<int> target;
+    if (g_geo_ode) target = ode_linf_distance(P, seeds);
+    g_geo_failed = 0;
     for (int t = 0; t < ntips; ++t) {
         std::vector<std::array<double, 3>> geodesic;
         std::array<double, 3> tip{0, 0, 0};
         P.adim(tips + t * P.dim, tip.data());
         geodesic.push_back(tip);
-        geodesic_run(P, S, geodesic, gp);
+        if (g_geo_ode) {
+            if (geodesic_ode(P, S, target, geodesic, step)) ++g_geo_failed;
+        } else {
+            geodesic_run(P, S, geodesic, gp);
+        }
         lengths[t] = (int64_t)geodesic.size();
         for (const auto &q : geodesic) {
             if (np < max_points)
@@ -1311,8 +1504,10 @@ int oracle_geodesics(int model, const int *dims, const double *gridScales, const
         const auto sd = make_seeds(P, nseeds, seeds, seedValues);
         FMState S;
         fast_marching(P, sd, S, nullptr, nullptr);
+        std::vector<long> seedIdx;
+        for (const auto &sv : sd) seedIdx.push_back(sv.first);
         return geodesics_impl(P, S, ntips, tips, step, threshold, volumeBound, points, max_points, lengths,
-                              npoints);
+                              npoints, seedIdx);
     } catch (const OracleError &e) {
         g_err = e.msg;
         return -1;
@@ -1324,7 +1519,7 @@ int oracle_geodesics_given(int model, const int *dims, const double *gridScales,
                            const double *cost, int cost_is_field, const double *values,
                            const uint64_t *activeNeighs, int ntips, const double *tips, double step,
                            double threshold, double volumeBound, double *points, long max_points,
-                           int64_t *lengths, long *npoints) {
+                           int64_t *lengths, long *npoints, int nseeds, const double *seeds) {
     try {
         g_err.clear();
         Problem P;
@@ -1333,8 +1528,11 @@ int oracle_geodesics_given(int model, const int *dims, const double *gridScales,
         FMState S;
         S.values.assign(values, values + P.N);
         S.active.assign(activeNeighs, activeNeighs + P.N);
+        std::vector<long> seedIdx;
+        if (nseeds > 0)
+            for (const auto &sv : make_seeds(P, nseeds, seeds, nullptr)) seedIdx.push_back(sv.first);
         return geodesics_impl(P, S, ntips, tips, step, threshold, volumeBound, points, max_points, lengths,
-                              npoints);
+                              npoints, seedIdx);
     } catch (const OracleError &e) {
         g_err = e.msg;
         return -1;

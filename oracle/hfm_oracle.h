@@ -52,6 +52,10 @@ long oracle_last_naccepted(void);
 /* Static factor (StaticFactoring.hxx:11-55) of the problems solved next on this thread, vals =
  * NULL for none: box extents bd (the grid's when sub = 0), the box's first grid index shift,
  * values and gradients (index units) row-major over the box, mask (NULL: everywhere). */
+/* Geodesic solver of the geodesics computed next on this thread: ode = 0 the discrete solver, 1
+ * the ODE solver (GeodesicODESolver.hxx) with the given causality tolerance. */
+void oracle_set_geodesic_solver(int ode, double causality_tolerance);
+long oracle_last_geodesic_failures(void);
 void oracle_set_factoring(int dim, int sub, const int *bd, const int *shift, const double *vals,
                           const double *grads, const unsigned char *mask);
 
@@ -124,7 +128,7 @@ int oracle_geodesics_given(int model, const int *dims, const double *gridScales,
                            const double *cost, int cost_is_field, const double *values,
                            const uint64_t *activeNeighs, int ntips, const double *tips, double step,
                            double threshold, double volumeBound, double *points, long max_points,
-                           int64_t *lengths, long *npoints);
+                           int64_t *lengths, long *npoints, int nseeds, const double *seeds);
 
 /* Same as oracle_solve, then FirstVariation::ForwardVariation: valueVariation (N x nVar,
  * row-major) from costVariation (N x nVar or NULL) and seedValueVariation (nVar x nseeds or

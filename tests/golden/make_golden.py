@@ -23,6 +23,7 @@ Fixtures (all row-major):
                      factoring (computed or imported)
   serial_geodesics.npz  geodesicPoints / geodesicLengths after order 2/3, factored and stopped solves
   serial_variations.npz costSensitivity_0 / seedSensitivity_0 / valueVariation after such solves
+  ode_geodesics.npz  geodesicPoints / geodesicLengths of the ODE geodesic solver, four models
   riemann3_128.npz   digests of values and Selling offsets at the benchmark size + 4096 seeded samples
 Inputs are regenerated by hamiltonfastmarching_amd.synthetic; the SHA-256 of each generated
 input is stored so that a test can tell a generator change from a numerical one.
@@ -737,6 +738,47 @@ def serial_variation_cases():
     print("serial_variations ok")
 
 
+def ode_geodesic_params():
+    """The ODE geodesic solver (geodesicSolver = "ODE", GeodesicODESolver.hxx:10-261), per model,
+    with its default step and causality tolerance and with others."""
+    def strip(p):
+        return {k: v for k, v in p.items()
+                if k not in ("tips", "inspectSensitivity", "inspectSensitivityWeights", "inspectSensitivityLengths")}
+    r2 = strip(syn.riemann2(40, seed=61))
+    t2 = np.array([[0.2, 0.8], [0.9, 0.1], [0.55, 0.3], [0.05, 0.5]])
+    r3 = strip(syn.riemann3(16, seed=62))
+    t3 = np.array([[0.1, 0.85, 0.2], [0.8, 0.3, 0.9]])
+    rs = strip(syn.rsf2(24, 16, seed=63))
+    ts = np.array([[0.2, 0.8, 1.0], [0.85, 0.2, 4.0], [0.6, 0.55, 2.5]])
+    el = strip(syn.elastica2(20, 16, seed=64))
+    te = np.array([[0.2, 0.75, 0.5], [0.8, 0.3, 3.0]])
+    return {
+        "r2": dict(r2, geodesicSolver="ODE", tips=t2),
+        "r2_step": dict(r2, geodesicSolver="ODE", geodesicStep=0.4, geodesicCausalityTolerance=2.0, tips=t2),
+        "r2_o2": dict(r2, geodesicSolver="ODE", order=2.0, tips=t2),
+        "r3": dict(r3, geodesicSolver="ODE", tips=t3),
+        "rsf2": dict(rs, geodesicSolver="ODE", tips=ts),
+        "elastica2": dict(el, geodesicSolver="ODE", tips=te),
+        # causality tolerance 0: one of these 40 tips stalls, which the reference reports
+        "r2_stall": dict(r2, geodesicSolver="ODE", geodesicCausalityTolerance=0.0,
+                         tips=np.random.default_rng(5).uniform(0.02, 0.98, (40, 2))),
+    }
+
+
+def ode_geodesic_cases():
+    out = {}
+    for name, p in ode_geodesic_params().items():
+        r = oracle.run_reference(p)
+        assert r["retcode"] == 0, r["log"]
+        out[name + "_points"] = r["geodesicPoints"]
+        out[name + "_lengths"] = r["geodesicLengths"]
+        out[name + "_failed"] = np.array([r["log"].count("seems to have failed")])
+        out[name + "_messages"] = np.array([ln for ln in r["log"].splitlines() if "seems to have failed" in ln] or [""])
+        print(name, np.asarray(r["geodesicLengths"]).astype(int).tolist(), r["log"].count("seems to have failed"))
+    np.savez_compressed(os.path.join(HERE, "ode_geodesics.npz"), **out)
+    print("ode_geodesics ok")
+
+
 def main():
     for m in ("Riemann2", "Riemann3", "ReedsSheppForward2", "Elastica2"):
         if not oracle.ref_available(m):
@@ -747,7 +789,8 @@ def main():
                   "bypass": bypass_cases, "stencils": stencil_cases, "big": big_case,
                   "orderings": ordering_cases, "high_order": high_order_cases,
                   "stopping": stopping_cases, "factoring": factoring_cases,
-                  "serial_geodesics": serial_geodesic_cases, "serial_variations": serial_variation_cases}
+                  "serial_geodesics": serial_geodesic_cases, "serial_variations": serial_variation_cases,
+                  "ode_geodesics": ode_geodesic_cases}
         for a in only:
             groups[a]()
         return
@@ -768,6 +811,7 @@ def main():
     factoring_cases()
     serial_geodesic_cases()
     serial_variation_cases()
+    ode_geodesic_cases()
     if "--big" in sys.argv:
         big_case()
 

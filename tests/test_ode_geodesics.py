@@ -1,0 +1,131 @@
+"""The ODE geodesic solver (geodesicSolver = "ODE": Headers/Base/Implementation/
+GeodesicODESolver.hxx:17-217, selected at HFMInterface.hxx:459-464): midpoint steps along the
+flow interpolated from the cell corners, stopped at the seeds, by the seeds' L-infinity
+distance, or by a stall that the reference reports as a failed tip.  The oracle against the
+reference's outputs (CPU), and the GPU against them, bit for bit in FP64 (marked gpu)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+import make_golden as mg  # noqa: E402
+
+FIX = os.path.join(HERE, "golden", "ode_geodesics.npz")
+CASES = mg.ode_geodesic_params()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_oracle_matches_reference(name):
+    d = np.load(FIX)
+    p = {k: v for k, v in CASES[name].items() if k != "tips"}
+    pts, lens = oracle.geodesics(p, CASES[name]["tips"])
+    assert np.array_equal(np.asarray(lens), d[name + "_lengths"].astype(np.int64))
+    assert np.array_equal(np.asarray(pts), d[name + "_points"])
+    assert oracle.last_geodesic_failures() == int(d[name + "_failed"][0])
+
+
+def test_oracle_geodesics_descend():
+    """Along an ODE geodesic the arrival time decreases overall (first point above the last),
+    and the walk ends within a few cells of the seed."""
+    p = {k: v for k, v in CASES["r2"].items() if k != "tips"}
+    pts, lens = oracle.geodesics(p, CASES["r2"]["tips"])
+    vals = oracle.solve(p)["values"]
+    n = vals.shape[0]
+    seed = np.asarray(p["seeds"], np.float64).reshape(-1, 2)[0]
+    start = 0
+    for ln in np.asarray(lens, np.int64):
+        g = pts[start:start + ln]
+        start += ln
+        i0 = np.clip((g[0] * n).astype(int), 0, n - 1)
+        i1 = np.clip((g[-1] * n).astype(int), 0, n - 1)
+        assert vals[tuple(i1)] < vals[tuple(i0)]
+        assert np.abs(g[-1] - seed).max() < 6.0 / n
+
+
+def test_oracle_discrete_after_ode_resets():
+    """The solver choice does not leak into the next call: the discrete solver's output is the
+    same before and after an ODE run."""
+    p = {k: v for k, v in CASES["r2"].items() if k not in ("tips", "geodesicSolver")}
+    a = oracle.geodesics(p, CASES["r2"]["tips"])
+    oracle.geodesics(dict(p, geodesicSolver="ODE"), CASES["r2"]["tips"])
+    b = oracle.geodesics(p, CASES["r2"]["tips"])
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_oracle_given_matches_oracle():
+    """The ODE solver on given values / activeNeighs (the form the GPU tests use) is the ODE
+    solver of a full oracle run when those are the oracle's own."""
+    for name in ("r2", "elastica2"):
+        c = CASES[name]
+        p = {k: v for k, v in c.items() if k != "tips"}
+        o = oracle.solve(p, active=True)
+        a = oracle.geodesics_given(p, o["values"], o["activeNeighs"], c["tips"])
+        b = oracle.geodesics(p, c["tips"])
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", list(CASES))
+def test_gpu_matches_reference(name):
+    """Bit-exact against the oracle's ODE solver on the GPU's own FP64 values and active sets;
+    against the reference's fixture bit-exact where those values are the reference's (the
+    serial loop at order 2, and the block solves whose fixed point lands on the same bits), else
+    the same number of points per tip within 1e-9 (the block solver's FP64 values are within
+    1e-12 of the reference's)."""
+    import hamiltonfastmarching_amd as hfm
+    print(f"ode geodesics {name}", flush=True)
+    d = np.load(FIX)
+    io = hfm.HFMIO(default_ordering="RowMajor")
+    for k, v in CASES[name].items():
+        io[k] = v
+    io["exportValues"] = 1.0
+    io["exportActiveNeighs"] = 1.0
+    io.run()
+    pts, lens = np.asarray(io["geodesicPoints"]), np.asarray(io["geodesicLengths"])
+    p = {k: v for k, v in CASES[name].items() if k != "tips"}
+    assert np.array_equal(lens, d[name + "_lengths"])
+    if p.get("order", 1.0) > 1:  # the serial loop: the reference's values, flows and points
+        assert np.array_equal(pts, d[name + "_points"])
+        return
+    ref_pts, ref_lens = oracle.geodesics_given(p, io["values"], io["activeNeighs"], CASES[name]["tips"])
+    assert np.array_equal(lens.astype(np.int64), ref_lens) and np.array_equal(pts, ref_pts)
+    same_values = np.array_equal(io["values"], oracle.solve(p)["values"])
+    print(f"  values bit-equal to the reference's: {same_values}", flush=True)
+    if same_values:
+        assert np.array_equal(pts, d[name + "_points"])
+    else:
+        assert np.abs(pts - d[name + "_points"]).max() < 1e-9
+    msgs = [ln for ln in io.messages().splitlines() if "seems to have failed" in ln]
+    assert msgs == [m for m in d[name + "_messages"].tolist() if m]
+
+
+@pytest.mark.gpu
+def test_gpu_plan_api_matches_io():
+    """hfm_plan_geodesics_ode on a Plan gives the same points, lengths and failed tips as the
+    input-dictionary path (tips in physical units, the stall case)."""
+    import torch
+
+    import hamiltonfastmarching_amd as hfm
+    print("ode geodesics plan api", flush=True)
+    d = np.load(FIX)
+    c = CASES["r2_stall"]
+    plan = hfm.Plan("Riemann2", c["metric"].shape[:2], c["gridScale"], origin=c["origin"], precision=64)
+    plan.set_metric(torch.from_numpy(c["metric"]).cuda())
+    plan.solve(np.asarray(c["seeds"], np.float64).reshape(-1, 2))
+    pts, lens, failed = plan.geodesics_ode(c["tips"], step=0.25, causality_tolerance=0.0)
+    assert np.array_equal(lens, d["r2_stall_lengths"].astype(np.int64))
+    assert np.array_equal(pts, d["r2_stall_points"])
+    assert int(failed.sum()) == int(d["r2_stall_failed"][0])
+
+
+@pytest.mark.gpu
+def test_gpu_unrecognized_solver():
+    import hamiltonfastmarching_amd as hfm
+    print("ode geodesics unrecognized solver", flush=True)
+    with pytest.raises(RuntimeError, match="Unrecognized geodesic solver Foo"):
+        hfm.run(dict(CASES["r2"], geodesicSolver="Foo"))
