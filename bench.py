@@ -329,7 +329,8 @@ def configs_leg(with_cpu):
     """The other workloads, each timed on the GPU from HBM-resident inputs and, with with_cpu,
     beside the reference executable on the same inputs (StencilCPUTime + FMCPUTime, 1 core):
       riemann2_256_fp64    Riemann2 256^2, h = 1/256, FP64 update, centre seed
-      rsf2_256x256x64      ReedsSheppForward2, xi = 0.1, FP32, 4 seeds; then 32 geodesics
+      rsf2_256x256x64      ReedsSheppForward2, xi = 0.1, FP32, 4 seeds; then 32 geodesics by the
+                           discrete solver and by the ODE solver
       elastica2_192x192x64 Elastica2, xi = 0.2, FP32, 1 seed; then the adjoint of the sum of the
                            arrival times at 32 targets (costSensitivity)
       riemann2_256_order2_serial  Riemann2 256^2 at order 2 (the serial acceptance loop), FP64,
@@ -380,6 +381,15 @@ def configs_leg(with_cpu):
         r["geodesics_cpu_ms"] = round(float(ref["GeodesicCPUTime"]) * 1e3, 1)
         r["max_rel_err_vs_reference"] = floor_rel(out.cpu().numpy(), np.asarray(ref["values"]).reshape(dims))
         r["speedup"] = round(c["ms"] / ms, 1)
+    # the same 32 tips through the ODE geodesic solver (geodesicSolver = "ODE")
+    oms = _time_gpu(lambda: plan.geodesics_ode(tips), 2)
+    r["geodesics_ode_gpu_ms"] = round(oms, 3)
+    if c is not None:
+        _, oref = cpu(dict(full, tips=tips, geodesicSolver="ODE"))
+        if oref is not None:
+            r["geodesics_ode_cpu_ms"] = round(float(oref["GeodesicCPUTime"]) * 1e3, 1)
+            _, olens, _ = plan.geodesics_ode(tips)
+            r["geodesics_ode_lengths_equal"] = float(np.mean(olens == np.asarray(oref["geodesicLengths"])))
     res["rsf2_256x256x64"] = r
     del plan
     # Elastica2 192 x 192 x 64, FP32, adjoint for 32 targets

@@ -899,31 +899,34 @@ __global__ __launch_bounds__(64) void k_geodesic(Geom g, StencilData<T> sd, cons
 // ---------------------------------------------------------------------------------------------
 constexpr int ODE_HIST = 1024;        // distance history: sqrt(d) / geodesicStep entries at most
 constexpr int ODE_DIST_MAX = 32767;   // the reference's unreached distance (largest short)
+constexpr int ODE_DIST_BOUND = 6;     // the distance's upper bound
+constexpr int ODE_LDS_SLOTS = 2048;   // read counts and flows kept in LDS before the device-memory table
+constexpr int ODE_LDS_PROBES = 64;
+static_assert(ODE_LDS_SLOTS == 1 << 11, "the LDS table's hash takes 11 bits");
 enum { GEO_HASH_FULL = 6, GEO_HISTORY = 7 };
 
-// One level of LInfDistance (GeodesicODESolver.hxx:221-258; upper bound 6, the value the
-// solver's constructor uses): a point not yet reached becomes level + 1 when one of its 3^d - 1
-// neighbours is at level.  Points set in this pass are at level + 1, so updating in place adds
-// no second step.
+// LInfDistance (GeodesicODESolver.hxx:221-258; upper bound 6, the value the solver's
+// constructor uses) is a breadth-first search over the 3^d - 1 neighbours that stops at the
+// bound.  Without walls its level is the Chebyshev distance (periodic axes wrapped) to the
+// nearest seed when that is at most 6, and unreached beyond: each thread here takes one offset
+// of the (13)^d box around one seed and lowers the point's distance to max |offset_i|.
 template <int ModelT>
-__global__ __launch_bounds__(256) void k_ode_dist(Geom g, int *__restrict__ dist, int level, long long np) {
+__global__ __launch_bounds__(256) void k_ode_dist_box(Geom g, int *__restrict__ dist,
+                                                      const long long *__restrict__ seeds, int nseeds) {
     constexpr int D = ModelShape<ModelT>::DIM;
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= np) return;
-    int c[3];
-    tiled_coords(g, (int)(p >> 6), (int)(p & 63), c);
-    if (!in_grid(g, c) || dist[p] != ODE_DIST_MAX) return;
-    constexpr int NOFF = D == 2 ? 9 : 27;
-    for (int k = 0; k < NOFF; ++k) {
-        const int o0 = k % 3 - 1, o1 = (k / 3) % 3 - 1, o2 = D == 3 ? k / 9 - 1 : 0;
-        if (o0 == 0 && o1 == 0 && o2 == 0) continue;
-        int r[3];
-        if (!shift_coords_m<ModelT>(g, c, o0, o1, o2, r)) continue;
-        if (dist[(long long)tile_of(g, r) * 64 + local_of(g, r)] == level) {
-            dist[p] = level + 1;
-            return;
-        }
-    }
+    constexpr int W = 2 * ODE_DIST_BOUND + 1, BOX = D == 2 ? W * W : W * W * W;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nseeds * BOX) return;
+    const int s = (int)(t / BOX), k = (int)(t % BOX);
+    const int o0 = k % W - ODE_DIST_BOUND, o1 = (k / W) % W - ODE_DIST_BOUND,
+              o2 = D == 3 ? k / (W * W) - ODE_DIST_BOUND : 0;
+    const long long sp = seeds[s];
+    int c[3], r[3];
+    tiled_coords(g, (int)(sp >> 6), (int)(sp & 63), c);
+    if (!shift_coords_m<ModelT>(g, c, o0, o1, o2, r)) return;
+    const int a0 = o0 < 0 ? -o0 : o0, a1 = o1 < 0 ? -o1 : o1, a2 = o2 < 0 ? -o2 : o2;
+    const int d = a0 > a1 ? (a0 > a2 ? a0 : a2) : (a1 > a2 ? a1 : a2);
+    atomicMin(&dist[(long long)tile_of(g, r) * 64 + local_of(g, r)], d);
 }
 
 __global__ void k_ode_dist_init(int *__restrict__ dist, long long np) {
@@ -931,10 +934,6 @@ __global__ void k_ode_dist_init(int *__restrict__ dist, long long np) {
     if (p < np) dist[p] = ODE_DIST_MAX;
 }
 
-__global__ void k_ode_dist_seeds(int *__restrict__ dist, const long long *__restrict__ seeds, int nseeds) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nseeds) dist[seeds[i]] = 0;
-}
 
 // GeodesicFlow at grid point q (coordinates c): value, width and flow vector.
 template <int ModelT, typename T>
@@ -1004,6 +1003,9 @@ __global__ __launch_bounds__(64) void k_geodesic_ode(Geom g, StencilData<T> sd, 
     constexpr int D = ModelShape<ModelT>::DIM;
     constexpr int NCORNER = 1 << D;
     __shared__ long long hist[ODE_HIST];
+    __shared__ unsigned long long s_keys[ODE_LDS_SLOTS];
+    __shared__ int s_cnt[ODE_LDS_SLOTS];
+    __shared__ double s_flow[ODE_LDS_SLOTS][5];  // value, width, flow: computed at the first read
     const int tip = blockIdx.x, lane = threadIdx.x;
     const long long hcap = 1LL << hlog2;
     unsigned long long *keys = hkeys + (long long)tip * hcap;
@@ -1029,10 +1031,15 @@ __global__ __launch_bounds__(64) void k_geodesic_ode(Geom g, StencilData<T> sd, 
         return;
     }
     for (int i = lane; i < K; i += 64) hist[i] = 2147483647LL;
+    for (int i = lane; i < ODE_LDS_SLOTS; i += 64) {
+        s_keys[i] = 0ull;
+        s_cnt[i] = 0;
+    }
     __syncthreads();
     int head = 0;
     // the flow at x, in every lane; false if the count table is full
     auto flow_at = [&](const double *x, double *fo, long long &nStat, long long &tol) {
+        __syncthreads();  // the previous evaluation's LDS writes before this one's reads
         long long pIndex[3] = {0, 0, 0};
         double w[3] = {0., 0., 0.}, sgn[3] = {0., 0., 0.};
         for (int i = 0; i < D; ++i) {
@@ -1056,11 +1063,34 @@ __global__ __launch_bounds__(64) void k_geodesic_ode(Geom g, StencilData<T> sd, 
             if (inrange && shift_coords_m<ModelT>(g, base, off[0], off[1], off[2], r)) {
                 const long long q = (long long)tile_of(g, r) * 64 + local_of(g, r);
                 ctol = dist[q];
-                // the corner's read count: linear probing on key q + 1 (0 marks an empty slot)
+                // the corner's read count: linear probing on key q + 1 (0 marks an empty slot),
+                // first over at most ODE_LDS_PROBES slots of the LDS table, then, when those are
+                // all taken by other keys, in the device-memory table.  Slots are never freed,
+                // so a key that went to device memory finds the same full LDS chain again.
                 const unsigned long long key = (unsigned long long)q + 1ull;
-                long long h = (long long)((key * 0x9E3779B97F4A7C15ull) >> (64 - hlog2));
+                const unsigned long long mix = key * 0x9E3779B97F4A7C15ull;
+                int ls = -1;
+                {
+                    int h = (int)(mix >> 53);  // 11 bits: ODE_LDS_SLOTS = 2048
+                    for (int probe = 0; probe < ODE_LDS_PROBES; ++probe) {
+                        const unsigned long long k = s_keys[h];
+                        if (k == key) {
+                            ls = h;
+                            break;
+                        }
+                        if (k == 0ull) {
+                            const unsigned long long prev = atomicCAS(&s_keys[h], 0ull, key);
+                            if (prev == 0ull || prev == key) {
+                                ls = h;
+                                break;
+                            }
+                        }
+                        h = (h + 1) & (ODE_LDS_SLOTS - 1);
+                    }
+                }
+                long long h = (long long)(mix >> (64 - hlog2));
                 long long slot = -1;
-                for (long long probe = 0; probe < hcap; ++probe) {
+                for (long long probe = 0; ls < 0 && probe < hcap; ++probe) {
                     const unsigned long long k = keys[h];
                     if (k == key) {
                         slot = h;
@@ -1075,13 +1105,27 @@ __global__ __launch_bounds__(64) void k_geodesic_ode(Geom g, StencilData<T> sd, 
                     }
                     h = (h + 1) & (hcap - 1);
                 }
-                if (slot < 0) {
+                if (ls < 0 && slot < 0) {
                     full = true;
                 } else {
-                    cstat = atomicAdd(&cnts[slot], 1);
+                    cstat = ls >= 0 ? atomicAdd(&s_cnt[ls], 1) : atomicAdd(&cnts[slot], 1);
                     cw = 1.;
                     for (int j = 0; j < D; ++j) cw *= off[j] ? w[j] : (1. - w[j]);
-                    ode_point_flow<ModelT, T>(g, sd, u, fixedmask, xf, q, r, cval, cwid, cf);
+                    // a point's flow is a function of the solve alone: computed at its first read
+                    // and kept with its count (the corners of one evaluation are distinct points,
+                    // so no other lane reads a slot while it is written)
+                    if (ls >= 0 && cstat > 0) {
+                        cval = s_flow[ls][0];
+                        cwid = s_flow[ls][1];
+                        for (int k = 0; k < 3; ++k) cf[k] = s_flow[ls][2 + k];
+                    } else {
+                        ode_point_flow<ModelT, T>(g, sd, u, fixedmask, xf, q, r, cval, cwid, cf);
+                        if (ls >= 0) {
+                            s_flow[ls][0] = cval;
+                            s_flow[ls][1] = cwid;
+                            for (int k = 0; k < 3; ++k) s_flow[ls][2 + k] = cf[k];
+                        }
+                    }
                 }
             }
         }
@@ -1255,17 +1299,16 @@ hipError_t launch_ode_distance(int model, const Geom &g, int *dist, const long l
                                hipStream_t s) {
     const unsigned nb = (unsigned)((np + 255) / 256);
     hipLaunchKernelGGL(k_ode_dist_init, dim3(nb), dim3(256), 0, s, dist, np);
-    if (nseeds > 0)
-        hipLaunchKernelGGL(k_ode_dist_seeds, dim3((unsigned)((nseeds + 255) / 256)), dim3(256), 0, s, dist, seeds,
-                           nseeds);
-    for (int level = 0; level < 6; ++level) {
-        switch (model) {
-            case MODEL_RIEMANN2: hipLaunchKernelGGL((k_ode_dist<MODEL_RIEMANN2>), dim3(nb), dim3(256), 0, s, g, dist, level, np); break;
-            case MODEL_RIEMANN3: hipLaunchKernelGGL((k_ode_dist<MODEL_RIEMANN3>), dim3(nb), dim3(256), 0, s, g, dist, level, np); break;
-            case MODEL_RSF2: hipLaunchKernelGGL((k_ode_dist<MODEL_RSF2>), dim3(nb), dim3(256), 0, s, g, dist, level, np); break;
-            case MODEL_ELASTICA2: hipLaunchKernelGGL((k_ode_dist<MODEL_ELASTICA2>), dim3(nb), dim3(256), 0, s, g, dist, level, np); break;
-            default: return hipErrorInvalidValue;
-        }
+    if (nseeds <= 0) return hipGetLastError();
+    constexpr long long W = 2 * ODE_DIST_BOUND + 1;
+    const long long box = model == MODEL_RIEMANN2 ? W * W : W * W * W;
+    const unsigned bb = (unsigned)(((long long)nseeds * box + 255) / 256);
+    switch (model) {
+        case MODEL_RIEMANN2: hipLaunchKernelGGL((k_ode_dist_box<MODEL_RIEMANN2>), dim3(bb), dim3(256), 0, s, g, dist, seeds, nseeds); break;
+        case MODEL_RIEMANN3: hipLaunchKernelGGL((k_ode_dist_box<MODEL_RIEMANN3>), dim3(bb), dim3(256), 0, s, g, dist, seeds, nseeds); break;
+        case MODEL_RSF2: hipLaunchKernelGGL((k_ode_dist_box<MODEL_RSF2>), dim3(bb), dim3(256), 0, s, g, dist, seeds, nseeds); break;
+        case MODEL_ELASTICA2: hipLaunchKernelGGL((k_ode_dist_box<MODEL_ELASTICA2>), dim3(bb), dim3(256), 0, s, g, dist, seeds, nseeds); break;
+        default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

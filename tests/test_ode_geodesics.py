@@ -129,3 +129,27 @@ def test_gpu_unrecognized_solver():
     print("ode geodesics unrecognized solver", flush=True)
     with pytest.raises(RuntimeError, match="Unrecognized geodesic solver Foo"):
         hfm.run(dict(CASES["r2"], geodesicSolver="Foo"))
+
+
+@pytest.mark.gpu
+def test_gpu_long_walk_spills_read_counts():
+    """A 4000-cell walk along a 4096 x 8 strip reads more distinct grid points than the LDS
+    read-count table holds, so part of the counts go to the device-memory table: still
+    bit-exact against the oracle's ODE solver on the GPU's values."""
+    import hamiltonfastmarching_amd as hfm
+    print("ode geodesics long walk", flush=True)
+    n0, n1 = 4096, 8
+    x = np.linspace(0.0, 1.0, n0)[:, None]
+    metric = np.zeros((n0, n1, 3))
+    metric[..., 0] = 1.0 + 0.5 * np.sin(7.0 * x)
+    metric[..., 2] = 1.0 + 0.3 * np.cos(5.0 * x)
+    metric[..., 1] = 0.1 * np.sin(3.0 * x)
+    p = {"model": "Riemann2", "dims": np.array([n0, n1], np.float64), "gridScale": 1.0,
+         "origin": np.zeros(2), "arrayOrdering": "RowMajor", "seeds": np.array([[0.5, 4.5]]),
+         "metric": metric, "geodesicSolver": "ODE"}
+    tips = np.array([[4090.3, 2.2], [4000.7, 6.1]])
+    out = hfm.run(dict(p, tips=tips, exportValues=1.0, exportActiveNeighs=1.0))
+    pts, lens = np.asarray(out["geodesicPoints"]), np.asarray(out["geodesicLengths"]).astype(np.int64)
+    assert lens.min() > 4 * 3900  # four points per cell along the strip
+    ref_pts, ref_lens = oracle.geodesics_given(p, out["values"], out["activeNeighs"], tips)
+    assert np.array_equal(lens, ref_lens) and np.array_equal(pts, ref_pts)
