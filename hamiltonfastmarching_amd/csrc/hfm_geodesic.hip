@@ -1006,6 +1006,7 @@ __global__ __launch_bounds__(64) void k_geodesic_ode(Geom g, StencilData<T> sd, 
     __shared__ unsigned long long s_keys[ODE_LDS_SLOTS];
     __shared__ int s_cnt[ODE_LDS_SLOTS];
     __shared__ double s_flow[ODE_LDS_SLOTS][5];  // value, width, flow: computed at the first read
+    __shared__ int s_dist[ODE_LDS_SLOTS];         // the point's distance to the seeds
     const int tip = blockIdx.x, lane = threadIdx.x;
     const long long hcap = 1LL << hlog2;
     unsigned long long *keys = hkeys + (long long)tip * hcap;
@@ -1062,7 +1063,6 @@ __global__ __launch_bounds__(64) void k_geodesic_ode(Geom g, StencilData<T> sd, 
             int r[3];
             if (inrange && shift_coords_m<ModelT>(g, base, off[0], off[1], off[2], r)) {
                 const long long q = (long long)tile_of(g, r) * 64 + local_of(g, r);
-                ctol = dist[q];
                 // the corner's read count: linear probing on key q + 1 (0 marks an empty slot),
                 // first over at most ODE_LDS_PROBES slots of the LDS table, then, when those are
                 // all taken by other keys, in the device-memory table.  Slots are never freed,
@@ -1107,6 +1107,7 @@ __global__ __launch_bounds__(64) void k_geodesic_ode(Geom g, StencilData<T> sd, 
                 }
                 if (ls < 0 && slot < 0) {
                     full = true;
+                    ctol = dist[q];
                 } else {
                     cstat = ls >= 0 ? atomicAdd(&s_cnt[ls], 1) : atomicAdd(&cnts[slot], 1);
                     cw = 1.;
@@ -1118,12 +1119,15 @@ __global__ __launch_bounds__(64) void k_geodesic_ode(Geom g, StencilData<T> sd, 
                         cval = s_flow[ls][0];
                         cwid = s_flow[ls][1];
                         for (int k = 0; k < 3; ++k) cf[k] = s_flow[ls][2 + k];
+                        ctol = s_dist[ls];
                     } else {
+                        ctol = dist[q];
                         ode_point_flow<ModelT, T>(g, sd, u, fixedmask, xf, q, r, cval, cwid, cf);
                         if (ls >= 0) {
                             s_flow[ls][0] = cval;
                             s_flow[ls][1] = cwid;
                             for (int k = 0; k < 3; ++k) s_flow[ls][2 + k] = cf[k];
+                            s_dist[ls] = (int)ctol;
                         }
                     }
                 }
@@ -1138,7 +1142,7 @@ __global__ __launch_bounds__(64) void k_geodesic_ode(Geom g, StencilData<T> sd, 
             wv[i] = __shfl(cw, i);
             vv[i] = __shfl(cval, i);
             const double wd = __shfl(cwid, i);
-            for (int k = 0; k < 3; ++k) fv[i][k] = __shfl(cf[k], i);
+            for (int k = 0; k < D; ++k) fv[i][k] = __shfl(cf[k], i);
             const long long si = __shfl(cstat, i), ti = __shfl(ctol, i);
             mstat = si > mstat ? si : mstat;
             mtol = ti < mtol ? ti : mtol;
