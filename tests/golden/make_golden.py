@@ -759,6 +759,9 @@ def ode_geodesic_params():
         "r3": dict(r3, geodesicSolver="ODE", tips=t3),
         "rsf2": dict(rs, geodesicSolver="ODE", tips=ts),
         "elastica2": dict(el, geodesicSolver="ODE", tips=te),
+        # after a serial solve with static factoring, and at order 2 on a curvature model
+        "r2_o3_fact": dict(r2, geodesicSolver="ODE", order=3.0, factoringRadius=10.0, tips=t2),
+        "rsf2_o2": dict(rs, geodesicSolver="ODE", order=2.0, tips=ts),
         # causality tolerance 0: one of these 40 tips stalls, which the reference reports
         "r2_stall": dict(r2, geodesicSolver="ODE", geodesicCausalityTolerance=0.0,
                          tips=np.random.default_rng(5).uniform(0.02, 0.98, (40, 2))),
