@@ -1007,6 +1007,8 @@ __global__ __launch_bounds__(64) void k_geodesic_ode(Geom g, StencilData<T> sd, 
     __shared__ int s_cnt[ODE_LDS_SLOTS];
     __shared__ double s_flow[ODE_LDS_SLOTS][5];  // value, width, flow: computed at the first read
     __shared__ int s_dist[ODE_LDS_SLOTS];         // the point's distance to the seeds
+    __shared__ double s_corner[NCORNER][6];       // one evaluation's corners: weight, value, width, flow
+    __shared__ long long s_cint[NCORNER][3];      // read count, distance, table full
     const int tip = blockIdx.x, lane = threadIdx.x;
     const long long hcap = 1LL << hlog2;
     unsigned long long *keys = hkeys + (long long)tip * hcap;
@@ -1138,15 +1140,27 @@ __global__ __launch_bounds__(64) void k_geodesic_ode(Geom g, StencilData<T> sd, 
         long long mstat = 0, mtol = 2147483647LL;
         bool anyfull = false;
         double wv[NCORNER], vv[NCORNER], fv[NCORNER][3];
+        // every lane reads the corners back from LDS (broadcast reads); the next evaluation
+        // starts with a barrier, so these slots are not rewritten before they are read
+        if (lane < NCORNER) {
+            s_corner[lane][0] = cw;
+            s_corner[lane][1] = cval;
+            s_corner[lane][2] = cwid;
+            for (int k = 0; k < 3; ++k) s_corner[lane][3 + k] = cf[k];
+            s_cint[lane][0] = cstat;
+            s_cint[lane][1] = ctol;
+            s_cint[lane][2] = full ? 1 : 0;
+        }
+        __syncthreads();
         for (int i = 0; i < NCORNER; ++i) {
-            wv[i] = __shfl(cw, i);
-            vv[i] = __shfl(cval, i);
-            const double wd = __shfl(cwid, i);
-            for (int k = 0; k < D; ++k) fv[i][k] = __shfl(cf[k], i);
-            const long long si = __shfl(cstat, i), ti = __shfl(ctol, i);
+            wv[i] = s_corner[i][0];
+            vv[i] = s_corner[i][1];
+            const double wd = s_corner[i][2];
+            for (int k = 0; k < D; ++k) fv[i][k] = s_corner[i][3 + k];
+            const long long si = s_cint[i][0], ti = s_cint[i][1];
             mstat = si > mstat ? si : mstat;
             mtol = ti < mtol ? ti : mtol;
-            anyfull = anyfull || __shfl((int)full, i);
+            anyfull = anyfull || s_cint[i][2] != 0;
             if (wv[i] > 0 && vv[i] < minVal) {
                 minVal = vv[i];
                 thrW = wd;
