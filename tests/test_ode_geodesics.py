@@ -153,3 +153,28 @@ def test_gpu_long_walk_spills_read_counts():
     assert lens.min() > 4 * 3900  # four points per cell along the strip
     ref_pts, ref_lens = oracle.geodesics_given(p, out["values"], out["activeNeighs"], tips)
     assert np.array_equal(lens, ref_lens) and np.array_equal(pts, ref_pts)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["r2", "r3", "rsf2", "elastica2"])
+def test_gpu_fp32_solve_close_to_reference(name):
+    """The ODE solver after an FP32 solve (the configuration the benchmark times): every tip
+    within one point of the reference's count, and the tips of equal count within 1e-4 of its
+    points (FP32 values are within about 1e-7 of the reference's)."""
+    import hamiltonfastmarching_amd as hfm
+    print(f"ode geodesics fp32 {name}", flush=True)
+    d = np.load(FIX)
+    out = hfm.run(dict(CASES[name], precision=32.0))
+    lens = np.asarray(out["geodesicLengths"]).astype(np.int64)
+    ref_lens = d[name + "_lengths"].astype(np.int64)
+    assert np.abs(lens - ref_lens).max() <= 1
+    pts, ref_pts = np.asarray(out["geodesicPoints"]), d[name + "_points"]
+    a = b = 0
+    worst = 0.0
+    for la, lb in zip(lens, ref_lens):
+        if la == lb:
+            worst = max(worst, float(np.abs(pts[a:a + la] - ref_pts[b:b + lb]).max()))
+        a += la
+        b += lb
+    print(f"  lengths equal {np.mean(lens == ref_lens):.2f}, max point diff {worst:.2e}", flush=True)
+    assert worst < 1e-4
